@@ -1,0 +1,293 @@
+/*
+ * obme.h — C ABI of libobme_hip.so, the MI355X (gfx950) biogeochemistry step.
+ *
+ * The library is a drop-in for the biogeochemistry step that OceanBioME.jl
+ * (v0.17.6) plugs into Oceananigans: per-cell NPZD / LOBSTER(+CarbonateSystem)
+ * tendencies, the two-band PAR column scan, the sinking (drift-velocity) flux
+ * divergence, the CarbonChemistry pH solve and the GasExchange CO2 flux, and
+ * the RK3 stage sequence that strings them together.
+ *
+ * Conventions
+ *  - Every entry point returns an obme_status (0 = OK). obme_last_error()
+ *    returns a thread-local message for the last failure on this thread.
+ *  - Array arguments are DEVICE pointers unless stated otherwise.  Host
+ *    arrays inside configuration structs are copied at obme_create().
+ *  - 3-D fields use the Oceananigans Field layout: parent(field) data of
+ *    size (Nx+2Hx, Ny+2Hy, Nz+2Hz), i fastest, k = 1 at the bottom; interior
+ *    cell (i,j,k) (1-based) lives at
+ *      (i-1+Hx) + (Nx+2Hx)*((j-1+Hy) + (Ny+2Hy)*(k-1+Hz)).
+ *  - Element type of tracer / tendency / PAR arrays: double for OBME_FP64,
+ *    float for OBME_FP32_MIXED.  Carbonate-chemistry arrays are always double.
+ *  - All calls are asynchronous on the given stream (NULL = default stream),
+ *    one handle per host thread; a handle is not re-entrant.
+ *  - Numerical failures are not errors: NaN propagates (min/max propagate NaN
+ *    as Julia's do; a non-integer power of a negative value gives NaN).
+ */
+#ifndef OBME_H
+#define OBME_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t obme_status;
+enum {
+    OBME_OK = 0,
+    OBME_ERR_INVALID = 1,     /* bad argument or configuration            */
+    OBME_ERR_HIP = 2,         /* HIP runtime error                        */
+    OBME_ERR_UNSUPPORTED = 3, /* valid request this build does not handle */
+    OBME_ERR_NOMEM = 4
+};
+
+/* Model compositions (NutrientsPlanktonDetritus/constructors.jl). */
+enum {
+    OBME_NPZD = 1,    /* Nutrient + PhytoZoo + Detritus; tracers (N, P, Z, T, D)      */
+    OBME_LOBSTER = 2  /* NitrateAmmonia + PhytoZoo + TwoParticleAndDissolved;
+                         tracers (NO3, NH4, P, Z, sPOM, bPOM, DOM [, DIC, Alk])    */
+};
+
+enum { OBME_FP64 = 0, OBME_FP32_MIXED = 1 };
+
+enum { OBME_LIGHT_MONDO = 0, OBME_LIGHT_ANALYTICAL = 1 };   /* plankton.jl:216-220 */
+enum { OBME_LINEAR = 0, OBME_QUADRATIC = 1 };               /* plankton.jl:83-90   */
+enum { OBME_UPWIND1 = 0, OBME_CENTERED2 = 1 };              /* sinking face value  */
+
+enum {
+    OBME_SURFACE_CONSTANT = 0,  /* PAR0 = surface_value                          */
+    OBME_SURFACE_DIURNAL = 1,   /* PAR0 = 100 max(0, cos(t pi / 12 h)), Light.jl:219 */
+    OBME_SURFACE_ARRAY = 2      /* PAR0[i,j] from a device array (Nx*Ny, i fastest) */
+};
+
+enum { OBME_FIELD_CONSTANT = 0, OBME_FIELD_PROFILE = 1, OBME_FIELD_3D = 2 };
+
+#define OBME_MAX_TRACERS 32
+
+/* NutrientsPlanktonDetritus parameters, one flat POD struct.
+ * PhytoZoo:                plankton.jl:19-58
+ * NitrateAmmonia:          nutrients.jl:17-19
+ * TwoParticleAndDissolved: detritus.jl:116-128
+ * Detritus:                detritus.jl:355-361
+ * Defaults for NPZD / LOBSTER are set by the host package (constructors.jl). */
+typedef struct obme_npd_params {
+    double nitrate_half_saturation;
+    double ammonia_half_saturation;
+    double nitrate_ammonia_inhibition;
+    double light_half_saturation;
+    double phytoplankton_maximum_growth_rate;
+    double phytoplankton_exudation_fraction;
+    double ammonia_fraction_of_exudate;
+    double temperature_coefficient;          /* used when use_temperature_coefficient */
+    double phytoplankton_mortality_rate;
+    double zooplankton_mortality_rate;
+    double zooplankton_excretion_rate;
+    double phytoplankton_solid_waste_fraction;
+    double excretion_inorganic_fraction;
+    double preference_for_phytoplankton;
+    double maximum_grazing_rate;
+    double grazing_half_saturation;
+    double zooplankton_assimilation_fraction;
+    double redfield_ratio;                   /* PhytoZoo redfield_ratio      */
+    double carbon_calcite_ratio;
+    double zooplankton_gut_calcite_dissolution;
+    double phytoplankton_chlorophyll_ratio;
+    double nitrification_rate;               /* NitrateAmmonia               */
+    double remineralisation_inorganic_fraction; /* TwoParticleAndDissolved   */
+    double small_remineralisation_rate;
+    double large_remineralisation_rate;
+    double dissolved_remineralisation_rate;
+    double small_solid_waste_fraction;
+    double detritus_redfield_ratio;          /* detritus redfield_ratio      */
+    double remineralisation_rate;            /* Detritus (single class)      */
+    double small_particle_fraction;          /* Detritus (single class)      */
+    int32_t light_limitation;                /* OBME_LIGHT_*                 */
+    int32_t use_temperature_coefficient;     /* 0 = `nothing` (no Q10)       */
+    int32_t phytoplankton_mortality_formulation; /* OBME_LINEAR/QUADRATIC   */
+    int32_t grazing_concentration_formulation;   /* OBME_LINEAR/QUADRATIC   */
+} obme_npd_params;
+
+/* TwoBandPhotosyntheticallyActiveRadiation (Light/2band.jl:35-70, defaults :106-117). */
+typedef struct obme_two_band_par {
+    double water_red_attenuation;
+    double water_blue_attenuation;
+    double chlorophyll_red_attenuation;
+    double chlorophyll_blue_attenuation;
+    double chlorophyll_red_exponent;
+    double chlorophyll_blue_exponent;
+    double pigment_ratio;
+    double phytoplankton_chlorophyll_ratio;
+    int32_t surface_mode;          /* OBME_SURFACE_*                        */
+    int32_t pad0;
+    double surface_value;          /* PAR0 for OBME_SURFACE_CONSTANT (W/m2) */
+    const void* surface_array;     /* device, Nx*Ny of the tracer element type */
+} obme_two_band_par;
+
+/* Drift velocities (Utils/sinking_velocity_fields.jl:10-35): per tracer, in
+ * tracer order, a sinking speed >= 0 in m/s; 0 means no drift velocity. */
+typedef struct obme_sinking {
+    double speed[OBME_MAX_TRACERS];
+    int32_t open_bottom;           /* 1: w = -speed on faces 1..Nz; 0: damped to 0 at the bottom */
+    int32_t scheme;                /* OBME_UPWIND1 or OBME_CENTERED2 */
+    double smoothing_distance;     /* default 2 */
+} obme_sinking;
+
+/* DampedNewtonRaphsonSolver (Utils/solvers.jl:81-88) with the CarbonChemistry
+ * lower bound 0 (CarbonChemistry/carbon_chemistry.jl:81). */
+typedef struct obme_carbon_solver {
+    int32_t max_iters;             /* 100   */
+    int32_t pad0;
+    double atol;                   /* 1e-20 */
+    double damping;                /* 0.5   */
+    double armijo_constant;        /* 0.5   */
+    double min_damping;            /* 0.5^10 */
+    double initial_pH_guess;       /* 8     */
+} obme_carbon_solver;
+
+/* CarbonDioxideGasExchangeBoundaryCondition (GasExchange/GasExchange.jl:104-124). */
+typedef struct obme_co2_exchange {
+    int32_t enabled;               /* adds the top DIC flux when the carbonate system is on */
+    int32_t pad0;
+    double air_concentration;      /* ppmv, 413                                      */
+    double wind_speed;             /* m/s, 2                                         */
+    double air_pressure;           /* atm, 1 (carbon_dioxide_concentration.jl:222)   */
+    double schmidt[5];             /* Sc(T) = a + bT + cT^2 + dT^3 + eT^4 (gas_solubility.jl:211) */
+    double transfer_velocity[3];   /* k660(u) = c0 + c1 u + c2 u^2, Ho06 (gas_transfer_velocity.jl:61) */
+} obme_co2_exchange;
+
+/* Grid: interior sizes, halos and vertical coordinates (host arrays, copied). */
+typedef struct obme_grid {
+    int32_t Nx, Ny, Nz;
+    int32_t Hx, Hy, Hz;
+    const double* zc;              /* [Nz]   cell centres, bottom to top      */
+    const double* zf;              /* [Nz+1] faces, bottom to top             */
+    const double* dz;              /* [Nz]   cell thickness (Delta z^ccc)     */
+} obme_grid;
+
+/* Read-only physical field (temperature or salinity). */
+typedef struct obme_field_source {
+    int32_t mode;                  /* OBME_FIELD_*                            */
+    int32_t pad0;
+    double value;                  /* OBME_FIELD_CONSTANT                     */
+    const double* profile;         /* host [Nz], OBME_FIELD_PROFILE (copied)  */
+} obme_field_source;
+
+typedef struct obme_config {
+    int32_t model;                 /* OBME_NPZD | OBME_LOBSTER                */
+    int32_t precision;             /* OBME_FP64 | OBME_FP32_MIXED             */
+    int32_t carbonate_system;      /* LOBSTER: 1 adds DIC and Alk             */
+    int32_t pad0;
+    obme_grid grid;
+    obme_npd_params npd;
+    obme_two_band_par par;
+    obme_sinking sinking;
+    obme_carbon_solver solver;
+    obme_co2_exchange co2;
+    obme_field_source temperature; /* LOBSTER gas exchange; NPZD reads T as a tracer */
+    obme_field_source salinity;
+} obme_config;
+
+typedef struct obme_handle obme_handle;
+
+/* ---- lifecycle ---------------------------------------------------------- */
+
+/* Replaces the construction of NPZD(grid) / LOBSTER(grid; carbonate_system)
+ * (NutrientsPlanktonDetritus/constructors.jl:65-96, :177-227) for the device. */
+obme_status obme_create(const obme_config* cfg, obme_handle** out);
+obme_status obme_destroy(obme_handle* h);
+const char* obme_last_error(void);
+/* Library version string, e.g. "obme 0.1.0 gfx950". */
+const char* obme_version(void);
+
+/* required_biogeochemical_tracers(bgc) (src/OceanBioME.jl:122;
+ * NutrientsPlanktonDetritus.jl:69-74): writes up to `cap` tracer names,
+ * separated by ';', into `buf`; returns the tracer count or <0 on error. */
+int32_t obme_required_tracers(const obme_handle* h, char* buf, int32_t cap);
+
+/* ---- the Oceananigans.Biogeochemistry hooks ----------------------------- */
+
+/* update_biogeochemical_state!(bgc, model) (src/OceanBioME.jl:161-167):
+ * the light step, i.e. the TwoBand PAR column scan (Light/2band.jl:1-33,
+ * :148-155). Reads tracers[P], writes the interior of PAR at time t. */
+obme_status obme_update_state(obme_handle* h, const void* const* tracers,
+                              void* PAR, double t, void* stream);
+
+/* update_tendencies!(bgc, model) (src/OceanBioME.jl:148-155), in the form
+ * of the sediment coupling hook (src/Sediments/tracer_coupling.jl:3-38):
+ * ADDS into G[n] the biogeochemical transition of every tracer
+ * ((bgc)(i,j,k,grid,Val(name),clock,fields), src/OceanBioME.jl:157-159),
+ * minus the drift-velocity flux divergence (biogeochemical_drift_velocity,
+ * src/OceanBioME.jl:126) and, with the carbonate system and gas exchange
+ * enabled, minus F/dz of the top-cell CO2 flux (GasExchange/gas_exchange.jl:26-38).
+ * T and S: optional 3-D fields (NULL = the configured source). */
+obme_status obme_add_tendencies(obme_handle* h, const void* const* tracers,
+                                void* const* G, const void* PAR,
+                                const void* T, const void* S,
+                                double t, void* stream);
+
+/* Standalone fused stepper with frozen physics: nsteps RK3 steps
+ * (Oceananigans RungeKutta3: gamma 8/15, 5/12, 3/4; zeta -17/60, -5/12; the
+ * substep form of src/BoxModel/timesteppers.jl:79-92) of update-state,
+ * tendencies and substep, in place on `tracers`. */
+obme_status obme_step(obme_handle* h, void* const* tracers,
+                      const void* T, const void* S,
+                      int32_t nsteps, double dt, double t0, void* stream);
+
+/* ---- carbonate chemistry / gas exchange on point sets (SoA, n points) --- */
+
+/* CarbonChemistry()(; DIC, Alk, T, S, output = Val(:pHᶠ)) and the CO2
+ * GasExchange flux F = k(u10,T,S) (pCO2_w - pCO2_air)
+ * (CarbonChemistry/carbon_chemistry.jl:111-165; GasExchange/gas_exchange.jl:26-38).
+ * mode 0: the reference damped Newton from pH 8 in FP64;
+ * mode 1: an FP32 initial guess refined by FP64 Newton.
+ * All arrays are double; u10 may be NULL (use cfg wind_speed). out_pH and
+ * out_flux may be NULL.  Also counts residual evaluations into
+ * *out_evals (device int64, may be NULL). */
+obme_status obme_co2_flux(obme_handle* h, const double* DIC, const double* Alk,
+                          const double* T, const double* S, const double* u10,
+                          double pCO2_air, double* out_pH, double* out_flux,
+                          int64_t n, int32_t mode, void* stream);
+
+/* ---- batched ensemble of independent NPZD water columns ----------------- */
+
+/* Per-column NPZD parameter set, SoA of n_columns entries each (device). */
+typedef struct obme_batch_params {
+    const double* fields[40];      /* see OBME_BP_* indices */
+} obme_batch_params;
+
+enum {
+    OBME_BP_NITRATE_HALF_SATURATION = 0,
+    OBME_BP_LIGHT_HALF_SATURATION,
+    OBME_BP_MAX_GROWTH_RATE,
+    OBME_BP_EXUDATION_FRACTION,
+    OBME_BP_AMMONIA_FRACTION_OF_EXUDATE,
+    OBME_BP_TEMPERATURE_COEFFICIENT,
+    OBME_BP_PHYTO_MORTALITY_RATE,
+    OBME_BP_ZOO_MORTALITY_RATE,
+    OBME_BP_ZOO_EXCRETION_RATE,
+    OBME_BP_PHYTO_SOLID_WASTE_FRACTION,
+    OBME_BP_EXCRETION_INORGANIC_FRACTION,
+    OBME_BP_PREFERENCE_FOR_PHYTO,
+    OBME_BP_MAX_GRAZING_RATE,
+    OBME_BP_GRAZING_HALF_SATURATION,
+    OBME_BP_ZOO_ASSIMILATION_FRACTION,
+    OBME_BP_REMINERALISATION_RATE,
+    OBME_BP_SMALL_PARTICLE_FRACTION,
+    OBME_BP_P_SINKING_SPEED,
+    OBME_BP_D_SINKING_SPEED,
+    OBME_BP_SURFACE_PAR,
+    OBME_BP_COUNT
+};
+
+/* Size in bytes of ABI struct `which` as compiled into the library
+ * (0 obme_config, 1 obme_grid, 2 obme_npd_params, 3 obme_two_band_par,
+ * 4 obme_sinking, 5 obme_carbon_solver, 6 obme_co2_exchange,
+ * 7 obme_field_source); -1 for an unknown index.  No reference
+ * counterpart: it lets a binding check its struct layouts. */
+int64_t obme_abi_sizeof(int32_t which);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OBME_H */

@@ -1,0 +1,553 @@
+// obme_device.h — device-side building blocks of the gfx950 biogeochemistry
+// step: Julia scalar semantics, precision-dispatched math, the
+// NutrientsPlanktonDetritus tendencies (NPZD, LOBSTER + CarbonateSystem) and
+// the FP64 carbonate chemistry (constants, alkalinity residual, damped
+// Newton, pCO2, CO2 gas-exchange flux).
+//
+// FT = double is the plain FP64 path (the reference's arithmetic, operand
+// order kept; contraction into FMA is the only change).  FT = float is the
+// FP32-mixed path: native v_exp/v_log/v_rcp based transcendentals and
+// divisions on the vector ALUs; the carbonate chemistry is always FP64.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/obme.h"
+
+namespace obme {
+
+// ---------------------------------------------------------------- scalars
+
+// Julia Base min/max for IEEE floats (NaN propagates; ±0 ordered).
+template <typename FT>
+__device__ __forceinline__ FT jmin(FT x, FT y)
+{
+    FT d = x - y;
+    FT a = __builtin_signbit(d) ? x : y;
+    return (__builtin_isnan(x) || __builtin_isnan(y)) ? d : a;
+}
+template <typename FT>
+__device__ __forceinline__ FT jmax(FT x, FT y)
+{
+    FT d = x - y;
+    FT a = __builtin_signbit(d) ? y : x;
+    return (__builtin_isnan(x) || __builtin_isnan(y)) ? d : a;
+}
+
+// Julia eps(x): spacing of floats at |x|; eps(0) is the smallest subnormal.
+__device__ __forceinline__ double jeps(double x)
+{
+    uint64_t b = __double_as_longlong(x);
+    int e = (int)((b >> 52) & 0x7ff);
+    uint64_t r = (e > 52) ? ((uint64_t)(e - 52) << 52) : ((uint64_t)1 << ((e > 1 ? e : 1) - 1));
+    double out = __longlong_as_double(r);
+    return (e == 0x7ff) ? __longlong_as_double(0x7ff8000000000000ull) : out;
+}
+__device__ __forceinline__ float jeps(float x)
+{
+    uint32_t b = __float_as_uint(x);
+    int e = (int)((b >> 23) & 0xff);
+    uint32_t r = (e > 23) ? ((uint32_t)(e - 23) << 23) : ((uint32_t)1 << ((e > 1 ? e : 1) - 1));
+    float out = __uint_as_float(r);
+    return (e == 0xff) ? __uint_as_float(0x7fc00000u) : out;
+}
+
+template <typename FT> struct Tiny;
+template <> struct Tiny<double> { static constexpr double eps0 = 4.9406564584124654e-324; };
+template <> struct Tiny<float> { static constexpr float eps0 = 1.40129846e-45f; };
+
+// Precision-dispatched math.  FP64: the accurate device library.  FP32:
+// the hardware instructions v_exp_f32 (2^x), v_log_f32 (log2 x),
+// v_rcp_f32 and v_sqrt_f32 (about 1 ulp each).
+__device__ __forceinline__ double m_div(double a, double b) { return a / b; }
+__device__ __forceinline__ float m_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double m_exp2(double x) { return exp2(x); }
+__device__ __forceinline__ float m_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ double m_log2(double x) { return log2(x); }
+__device__ __forceinline__ float m_log2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ __forceinline__ double m_exp(double x) { return exp(x); }
+__device__ __forceinline__ float m_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+__device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float m_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ double m_pow(double x, double y) { return pow(x, y); }
+// x^y for x >= 0 as 2^(y log2 x); a negative x gives NaN, as on the GPU
+// reference path.
+__device__ __forceinline__ float m_pow(float x, float y) { return m_exp2(y * m_log2(x)); }
+
+// ---------------------------------------------------------------- parameters
+
+// obme_npd_params converted to the arithmetic type.
+template <typename FT>
+struct NPDParams {
+    FT kNO3, kNH4, psi, kPAR, mu0, gamma, alpha_x, Q10;
+    FT mP, mZ, muZ, aP_solid, aZ_exc, pref, g, kG, aZ_ass, R, rho_c, R_chl;
+    FT nitrif, a_rem, s_rem, b_rem, d_rem, small_solid, Rd, rem, small_frac;
+    int light_form, use_Q10, mort_form, graz_form;
+};
+
+template <typename FT>
+inline NPDParams<FT> convert_params(const obme_npd_params& p)
+{
+    NPDParams<FT> q;
+    q.kNO3 = (FT)p.nitrate_half_saturation;
+    q.kNH4 = (FT)p.ammonia_half_saturation;
+    q.psi = (FT)p.nitrate_ammonia_inhibition;
+    q.kPAR = (FT)p.light_half_saturation;
+    q.mu0 = (FT)p.phytoplankton_maximum_growth_rate;
+    q.gamma = (FT)p.phytoplankton_exudation_fraction;
+    q.alpha_x = (FT)p.ammonia_fraction_of_exudate;
+    q.Q10 = (FT)p.temperature_coefficient;
+    q.mP = (FT)p.phytoplankton_mortality_rate;
+    q.mZ = (FT)p.zooplankton_mortality_rate;
+    q.muZ = (FT)p.zooplankton_excretion_rate;
+    q.aP_solid = (FT)p.phytoplankton_solid_waste_fraction;
+    q.aZ_exc = (FT)p.excretion_inorganic_fraction;
+    q.pref = (FT)p.preference_for_phytoplankton;
+    q.g = (FT)p.maximum_grazing_rate;
+    q.kG = (FT)p.grazing_half_saturation;
+    q.aZ_ass = (FT)p.zooplankton_assimilation_fraction;
+    q.R = (FT)p.redfield_ratio;
+    q.rho_c = (FT)p.carbon_calcite_ratio;
+    q.R_chl = (FT)p.phytoplankton_chlorophyll_ratio;
+    q.nitrif = (FT)p.nitrification_rate;
+    q.a_rem = (FT)p.remineralisation_inorganic_fraction;
+    q.s_rem = (FT)p.small_remineralisation_rate;
+    q.b_rem = (FT)p.large_remineralisation_rate;
+    q.d_rem = (FT)p.dissolved_remineralisation_rate;
+    q.small_solid = (FT)p.small_solid_waste_fraction;
+    q.Rd = (FT)p.detritus_redfield_ratio;
+    q.rem = (FT)p.remineralisation_rate;
+    q.small_frac = (FT)p.small_particle_fraction;
+    q.light_form = p.light_limitation;
+    q.use_Q10 = p.use_temperature_coefficient;
+    q.mort_form = p.phytoplankton_mortality_formulation;
+    q.graz_form = p.grazing_concentration_formulation;
+    return q;
+}
+
+// ---------------------------------------------------------------- tendencies
+
+// Shared plankton terms (NutrientsPlanktonDetritus/plankton.jl).
+template <typename FT>
+struct Grazing {
+    FT total, onP, onSmall;
+};
+
+template <typename FT>
+__device__ __forceinline__ FT light_limitation(const NPDParams<FT>& q, FT PAR)
+{   // plankton.jl:219-220
+    if (q.light_form == OBME_LIGHT_MONDO) return m_div(PAR, q.kPAR + PAR);
+    return m_div(PAR, m_sqrt(PAR * PAR + q.kPAR * q.kPAR));
+}
+
+template <typename FT>
+__device__ __forceinline__ Grazing<FT> grazing(const NPDParams<FT>& q, FT P, FT Z, FT sPOM)
+{   // total_grazing :118-135, grazing(Val{:P}) :352-369, grazing(Val{:sPOM}/Val{:D}) :371-389,
+    // weighted_phytoplankton_preference :393-397, concentration_limit :89-90
+    FT p = m_div(q.pref * P, q.pref * P + (FT(1) - q.pref) * sPOM + Tiny<FT>::eps0);
+    FT food = p * P + (FT(1) - p) * sPOM;
+    FT L = (q.graz_form == OBME_LINEAR) ? m_div(food, food + q.kG)
+                                        : m_div(food * food, food * food + q.kG * q.kG);
+    FT den = food + jeps(food);
+    Grazing<FT> G;
+    G.total = q.g * L * Z;
+    G.onP = m_div(q.g * p * L * P, den) * Z;
+    G.onSmall = m_div(q.g * (FT(1) - p) * L * sPOM, den) * Z;
+    return G;
+}
+
+template <typename FT>
+__device__ __forceinline__ FT mortality(const NPDParams<FT>& q, FT P)
+{   // plankton.jl:86-87
+    return q.mort_form == OBME_LINEAR ? q.mP * P : q.mP * (P * P);
+}
+
+// NPZD transitions (N, P, Z, D); T has zero tendency.
+template <typename FT>
+__device__ __forceinline__ void npzd_tendencies(const NPDParams<FT>& q, FT N, FT P, FT Z, FT T, FT D, FT PAR,
+                                                FT Lt, FT& dN, FT& dP, FT& dZ, FT& dD)
+{
+    // phytoplankton_growth (plankton.jl:191-206), Nutrient limitation (:180-188)
+    FT Ln = m_div(N, N + q.kNO3);
+    FT Ll = light_limitation(q, PAR);
+    FT muP = q.mu0 * Ll * Ln * Lt * P;
+    FT sPOM = D * q.small_frac;  // detritus.jl:386-387
+    Grazing<FT> G = grazing(q, P, Z, sPOM);
+    FT nuP = mortality(q, P);
+    dP = (FT(1) - q.gamma) * muP - G.onP - nuP;                     // plankton.jl:92-105
+    dZ = q.aZ_ass * G.total - q.mZ * (Z * Z) - q.muZ * Z;          // :107-116
+    FT pinw = q.aZ_exc * q.muZ * Z + (FT(1) - q.aP_solid) * nuP;  // :292-306
+    FT dinw = D * q.rem;                                           // detritus.jl:389-390
+    FT uptake = muP * (FT(1) - q.alpha_x * q.gamma);               // plankton.jl:271-278
+    dN = pinw + dinw - uptake;                                     // nutrients.jl:86-90
+    FT ponw = (FT(1) - q.alpha_x) * q.gamma * muP + (FT(1) - q.aZ_exc) * q.muZ * Z;  // :312-325
+    FT solid = (FT(1) - q.aZ_ass) * G.total + q.aP_solid * nuP + q.mZ * (Z * Z);    // :330-346
+    dD = ponw + solid - G.onSmall - q.rem * D;                     // detritus.jl:373-379
+}
+
+// LOBSTER transitions; v/out in tracer order NO3, NH4, P, Z, sPOM, bPOM, DOM[, DIC, Alk].
+template <typename FT, bool CARB>
+__device__ __forceinline__ void lobster_tendencies(const NPDParams<FT>& q, const FT* v, FT PAR, FT* out)
+{
+    FT NO3 = v[0], NH4 = v[1], P = v[2], Z = v[3], sPOM = v[4], bPOM = v[5], DOM = v[6];
+    // nitrogen_limitation (plankton.jl:150-166)
+    FT nl = m_div(NO3 * m_exp(-q.psi * NH4), NO3 + q.kNO3);
+    FT al = jmax(FT(0), m_div(NH4, q.kNH4 + NH4));
+    FT Ln = (nl + al) / FT(2);
+    FT Ll = light_limitation(q, PAR);
+    FT muP = q.mu0 * Ll * Ln * FT(1) * P;
+    Grazing<FT> G = grazing(q, P, Z, sPOM);
+    FT nuP = mortality(q, P);
+    FT nitrification = q.nitrif * NH4;                       // nutrients.jl:55
+    FT share = nl + al + Tiny<FT>::eps0;
+    FT up_NO3 = m_div(muP * nl, share);                     // plankton.jl:223-239
+    FT waste = q.alpha_x * q.gamma * muP;
+    FT up_NH4 = m_div(muP * al, share) - waste;             // :241-261
+    FT pinw = q.aZ_exc * q.muZ * Z + (FT(1) - q.aP_solid) * nuP;
+    FT dinw = (q.a_rem * (q.s_rem * sPOM + q.b_rem * bPOM) + q.d_rem * DOM);     // detritus.jl:252-263
+    FT ponw = (FT(1) - q.alpha_x) * q.gamma * muP + (FT(1) - q.aZ_exc) * q.muZ * Z;
+    FT donw = (FT(1) - q.a_rem) * (q.s_rem * sPOM + q.b_rem * bPOM);             // :265-274
+    FT solid = (FT(1) - q.aZ_ass) * G.total + q.aP_solid * nuP + q.mZ * (Z * Z);
+    FT dNO3 = nitrification - up_NO3;                        // nutrients.jl:57-60
+    FT dNH4 = pinw + dinw - nitrification - up_NH4;          // :62-67
+    out[0] = dNO3;
+    out[1] = dNH4;
+    out[2] = (FT(1) - q.gamma) * muP - G.onP - nuP;
+    out[3] = q.aZ_ass * G.total - q.mZ * (Z * Z) - q.muZ * Z;
+    out[4] = q.small_solid * solid - G.onSmall - q.s_rem * sPOM;       // detritus.jl:240-244
+    out[5] = (FT(1) - q.small_solid) * solid - q.b_rem * bPOM;        // :246-249
+    out[6] = ponw + donw - q.d_rem * DOM;                              // :234-238
+    if constexpr (CARB) {
+        FT pp = (FT(1) + q.rho_c * (FT(1) - q.gamma) - q.alpha_x * q.gamma) * muP * q.R;  // plankton.jl:280-290
+        FT picw = q.R * pinw;                                                           // :308-309
+        FT dicw = q.a_rem * (q.s_rem * (sPOM * q.Rd) + q.b_rem * (bPOM * q.Rd)) + q.d_rem * (DOM * q.Rd);
+        FT cdis = (G.onP + nuP) * q.rho_c * q.R;                                        // :429-441
+        FT cup = FT(2) * q.rho_c * muP * q.R;                                           // :443-450
+        out[7] = -pp + picw + dicw + cdis;                                              // carbonate_system.jl:50-55
+        out[8] = dNH4 * FT(0.9375) - dNO3 * FT(1.0625) - FT(2) * cup + FT(2) * cdis;    // :57-62
+    }
+}
+
+// ---------------------------------------------------------------- carbonate chemistry (FP64)
+
+// teos10_polynomial_approximation (Models/seawater_density.jl:33-39) over
+// SeawaterPolynomials' TEOS10 _ρ (Roquet et al. 2015 polynomial).
+__device__ __forceinline__ double teos10_density(double T, double S, double Pbar)
+{
+    const double Sau = 40.0 * 35.16504 / 35.0;
+    double t = T / 40.0;
+    double s = sqrt((S + 32.0) / Sau);
+    double z = -(10.0 * Pbar) / 1e4;
+    double r0 = (((((-1.7243708991e-03 * z + 1.5616995503e-02) * z + 6.4326772569e-02) * z
+                   + 2.2601900708e-01) * z - 5.2099962525e+00) * z + 4.6494977072e+01) * z;
+    double n3 = 3.7969820455e-01 * t + -1.8507636718e-02 * s + -2.3342758797e-02;
+    double n2 = (-1.2419983026e+00 * t + -2.1311365518e-01 * s + 2.0564311499e+00) * t
+                + (2.5019633244e+00 * s + -4.9527603989e+00) * s + 2.0660924175e+00;
+    double n1 = (((5.5927935970e-01 * t + -5.5077101279e-01 * s + -2.4649669534e+00) * t
+                  + (-1.8795372996e+00 * s + 3.5063081279e+00) * s + 6.7080479603e+00) * t
+                 + ((-6.5399043664e-01 * s + 5.0042598061e+00) * s + -4.4870114575e+00) * s + -1.3336301113e+01) * t
+                + (((6.6051753097e+00 * s + -3.0938076334e+01) * s + 5.0774768218e+01) * s + -4.2549998214e+01) * s
+                + 1.9681925209e+01;
+    double n0 = (((((-1.9083568888e-01 * t + 4.8169980163e-01 * s + 5.4048723791e-01) * t
+                    + (-5.3563304045e+00 * s + 1.1311538584e+01) * s + -8.3627885467e+00) * t
+                   + ((-3.1742946532e+00 * s + 1.9717078466e+01) * s + -3.3449108469e+01) * s + 2.1661789529e+01) * t
+                  + (((-5.4723692739e+00 * s + 2.9130021253e+01) * s + -6.0362551501e+01) * s + 6.1548258127e+01) * s
+                  + -3.7074170417e+01) * t
+                 + ((((-1.9193502195e+00 * s + 1.7681814114e+01) * s + -5.6888046321e+01) * s + 8.1770425108e+01) * s
+                    + -6.5281885265e+01) * s + 2.6010145068e+01) * t
+                + (((((-6.0579916612e+01 * s + 4.3227585684e+02) * s + -1.2849161071e+03) * s + 2.0375295546e+03) * s
+                    + -1.7864682637e+03) * s + 8.6672408165e+02) * s + 8.0189615746e+02;
+    return r0 + (((n3 * z + n2) * z + n1) * z + n0);
+}
+
+struct CarbonParams {
+    double DIC, Alk, boron, sulfate, fluoride, silicate, phosphate;
+    double K1, K2, KB, KW, KS, KF, KP1, KP2, KP3, KSi, K0, Tk;
+};
+
+// Equilibrium constants at the surface (P = nothing): CarbonChemistry/
+// equilibrium_constants.jl K0 :65-80, K1 :103-126, K2 :149-172, KB :202-250,
+// KW :279-313, IonicStrength :335-341, KS :366-419, KF :451-488,
+// KP1-3 :511-651, KSi :674-713.  log(T), √S, √Is are computed once.
+__device__ __forceinline__ double k0_constant(double T, double S, double lT)
+{
+    return exp(-60.2409 + (93.4517 * 100) / T + 23.3585 * (lT - log(100.0)) + 0.0 * (T * T)
+               + (0.023517 + (-0.023656 / 100) * T + (0.0047036 / 10000.0) * (T * T)) * S);
+}
+
+__device__ __forceinline__ void carbon_constants(double Tk, double S, CarbonParams& p)
+{
+    double lT = log(Tk);
+    double sS = sqrt(S);
+    double Is = 19.924 * S / (1000.0 + -1.005 * S);
+    double sIs = sqrt(Is);
+    p.K0 = k0_constant(Tk, S, lT);
+    p.K1 = pow(10.0, 61.2172 + -3633.86 / Tk + -9.67770 * lT + 0.011555 * S + -0.0001152 * (S * S));
+    p.K2 = pow(10.0, -25.9290 + -471.78 / Tk + 0.01781 * S + -0.0001122 * (S * S) + 3.16967 * lT);
+    p.KB = exp(148.0248 + (-8966.90 + -2890.53 * sS + -77.942 * S + 1.728 * pow(S, 1.5) + -0.0996 * (S * S)) / Tk
+               + 137.1942 * sS + 1.62142 * S + (-24.4344 + -25.085 * sS + -0.2474 * S) * lT + 0.053105 * sS * Tk);
+    p.KW = exp(148.9652 + -13847.26 / Tk + -23.6521 * lT + (-5.977 + 118.67 / Tk + 1.0495 * lT) * sS + -0.01615 * S);
+    double l1S = log(1 + -0.001005 * S);
+    p.KS = exp(141.328 + -4276.1 / Tk + -23.093 * lT + (324.57 + -13856.0 / Tk + -47.986 * lT) * sIs
+               + (-771.54 + 35474.0 / Tk + 114.723 * lT) * Is + -2698.0 * pow(Is, 1.5) / Tk
+               + 1776.0 * (Is * Is) / Tk + l1S);
+    p.KF = exp(-9.68 + 874.0 / Tk + 0.111 * sS + log(1 + 0.0 * S) + log(1 + 0.0 * S / p.KS));
+    p.KP1 = exp(115.525 + -4576.752 / Tk + -18.453 * lT + (0.69171 + -106.736 / Tk) * sS + (-0.01844 + -0.65643 / Tk) * S);
+    p.KP2 = exp(172.0883 + -8814.715 / Tk + -27.927 * lT + (1.3566 + -160.340 / Tk) * sS + (-0.05778 + 0.37335 / Tk) * S);
+    p.KP3 = exp(-18.141 + -3070.75 / Tk + 0.0 * lT + (2.81197 + 17.27039 / Tk) * sS + (-0.09984 + -44.99486 / Tk) * S);
+    p.KSi = exp(117.385 + -8904.2 / Tk + -19.334 * lT + (3.5913 + -458.79 / Tk) * sIs + (-1.5998 + 188.74 / Tk) * Is
+                + (0.07871 + -12.1652 / Tk) * (Is * Is) + l1S);
+}
+
+// Conversion of the functor's inputs (carbon_chemistry.jl:116-135): ρ at
+// Pbar = 1 (:123), kelvin, mmol/m³ → mol/kg.
+__device__ __forceinline__ void carbon_setup(double DIC, double Alk, double T, double S, CarbonParams& p)
+{
+    p.boron = 0.000232 / 10.811 * S / 1.80655;
+    p.sulfate = 0.14 / 96.06 * S / 1.80655;
+    p.fluoride = 0.000067 / 18.9984 * S / 1.80655;
+    double rho = teos10_density(T, S, 1.0);
+    p.Tk = T + 273.15;
+    p.Alk = Alk * (1e-3 / rho);
+    p.DIC = DIC * (1e-3 / rho);
+    p.phosphate = 0.0;
+    p.silicate = 0.0;
+    carbon_constants(p.Tk, S, p);
+}
+
+// alkalinity_residual / ∂ₕ (CarbonChemistry/alkalinity_residual.jl:18-75)
+// with silicate = phosphate = 0 (the defaults).  The phosphate and silicate
+// terms are then ±0 and adding them to the nonzero partial sums is exact, so
+// they are left out.
+__device__ __forceinline__ double alk_residual(double H, const CarbonParams& p)
+{
+    double cd = H * H + p.K1 * H + p.K1 * p.K2;
+    double sd = 1 + p.sulfate / p.KS;
+    double bicarbonate = p.K1 * H * p.DIC / cd;
+    double carbonate = 2 * p.DIC * p.K1 * p.K2 / cd;
+    double borate = p.boron / (1 + H / p.KB);
+    double hydroxide = p.KW / H;
+    double free_hydrogen = -H / sd;
+    double hydrogen_sulfate = -p.sulfate / (1 + p.KS / H * sd);
+    double hydrogen_fluoride = -p.fluoride / (1 + p.KF / H);
+    return bicarbonate + carbonate + borate + hydroxide + free_hydrogen + hydrogen_sulfate + hydrogen_fluoride
+           - p.Alk;
+}
+
+__device__ __forceinline__ double alk_residual_derivative(double H, const CarbonParams& p)
+{
+    double cd = H * H + p.K1 * H + p.K1 * p.K2;
+    double sd = 1 + p.sulfate / p.KS;
+    double dcd = 2 * H + p.K1;
+    double d_bicarbonate = p.K1 * p.DIC * (p.K1 * p.K2 - H * H) / (cd * cd);
+    double d_carbonate = -dcd * 2 * p.DIC * p.K1 * p.K2 / (cd * cd);
+    double bd = 1 + H / p.KB;
+    double d_borate = -p.boron / (bd * bd) / p.KB;
+    double d_hydroxide = -p.KW / (H * H);
+    double d_free_hydrogen = -1 / sd;
+    double hsd = 1 + p.KS / H * sd;
+    double d_hydrogen_sulfate = p.sulfate / (hsd * hsd) * (p.KS / H * 0.0 - p.KS / (H * H) * sd);
+    double fd = 1 + p.KF / H;
+    double d_hydrogen_fluoride = -p.fluoride / (fd * fd) * p.KF / (H * H);
+    return d_bicarbonate + d_carbonate + d_borate + d_hydroxide + d_free_hydrogen + d_hydrogen_sulfate
+           + d_hydrogen_fluoride;
+}
+
+__device__ __forceinline__ double jsign(double x)
+{
+    return (x > 0) ? 1.0 : ((x < 0) ? -1.0 : x);
+}
+
+// DampedNewtonRaphsonSolver (Utils/solvers.jl:90-121) with bounded_λ for
+// the lower bound 0 (:126-131).  An iteration that leaves x unchanged is an
+// exact fixed point of the loop (same x, same f(x), same δ, same λ
+// sequence), so the remaining iterations are skipped with the identical
+// result.  evals counts residual evaluations.
+__device__ __forceinline__ double solve_for_H(const CarbonParams& p, const obme_carbon_solver& s, double x0,
+                                              int& evals)
+{
+    double x = x0;
+    int N = 0;
+    double fx = alk_residual(x, p);
+    int ne = 1;
+    while ((fabs(fx) > s.atol) && (N < s.max_iters)) {
+        double delta = fx / alk_residual_derivative(x, p);
+        double lb = (x - 0.0) / delta;
+        lb = (lb < 0) ? 1.0 : lb;
+        double lambda = jmin(1.0, lb);
+        double fnew = alk_residual(x - lambda * delta, p);
+        ne++;
+        while ((fabs(fnew) >= fabs(fx) - s.armijo_constant * lambda * fx * jsign(fx)) && (lambda > s.min_damping)) {
+            lambda *= s.damping;
+            fnew = alk_residual(x - lambda * delta, p);
+            ne++;
+        }
+        double xn = x - lambda * delta;
+        N += 1;
+        bool stuck = (xn == x);
+        x = xn;
+        fx = fnew;
+        if (stuck) N = s.max_iters;
+    }
+    evals += ne;
+    return x;
+}
+
+// FP32 first guess for [H+]: Newton on the residual in FP32 from pH 8, each
+// step limited to half the distance to the bound H = 0, at most 16 steps.
+__device__ __forceinline__ float guess_H_fp32(const CarbonParams& p)
+{
+    float K1 = (float)p.K1, K2 = (float)p.K2, KB = (float)p.KB, KW = (float)p.KW, KS = (float)p.KS,
+          KF = (float)p.KF, DIC = (float)p.DIC, Alk = (float)p.Alk, boron = (float)p.boron,
+          sulfate = (float)p.sulfate, fluoride = (float)p.fluoride;
+    float sd = 1.0f + sulfate / KS;
+    float rsd = 1.0f / sd;
+    float K12 = K1 * K2;
+    float x = 1e-8f;
+#pragma unroll 1
+    for (int it = 0; it < 16; it++) {
+        float rx = 1.0f / x;
+        float cd = x * x + K1 * x + K12;
+        float rcd = 1.0f / cd;
+        float rbd = 1.0f / (1.0f + x / KB);
+        float rhsd = 1.0f / (1.0f + KS * rx * sd);
+        float rfd = 1.0f / (1.0f + KF * rx);
+        float f = (K1 * x + 2.0f * K12) * DIC * rcd + boron * rbd + KW * rx - x * rsd - sulfate * rhsd
+                  - fluoride * rfd - Alk;
+        float df = K1 * DIC * (K12 - x * x) * rcd * rcd - (2.0f * x + K1) * 2.0f * DIC * K12 * rcd * rcd
+                   - boron * rbd * rbd / KB - KW * rx * rx - rsd - sulfate * rhsd * rhsd * KS * rx * rx * sd
+                   - fluoride * rfd * rfd * KF * rx * rx;
+        float d = f / df;
+        float lam = (d > 0.0f) ? fminf(1.0f, 0.5f * x / d) : 1.0f;
+        x = x - lam * d;
+        if (fabsf(d) <= 2e-7f * x) break;
+    }
+    return x;
+}
+
+// Refinement of a close first guess by plain FP64 Newton on the same
+// residual; returns false when it has not converged after 6 steps.
+__device__ __forceinline__ bool refine_H(const CarbonParams& p, double& x, int& evals)
+{
+#pragma unroll 1
+    for (int it = 0; it < 6; it++) {
+        double f = alk_residual(x, p);
+        double df = alk_residual_derivative(x, p);
+        evals++;
+        double d = f / df;
+        double xn = x - d;
+        if (!(xn > 0.0)) return false;
+        x = xn;
+        if (fabs(d) <= 1e-14 * x) return true;
+    }
+    return false;
+}
+
+// pCO2 from H (carbon_chemistry.jl:160-162 and selected_output(::Val{:pCO₂}) :170-193).
+__device__ __forceinline__ double pco2_from_H(const CarbonParams& p, double H)
+{
+    double CO2 = p.DIC * (H * H) / (H * H + p.K1 * H + p.K1 * p.K2);
+    double fCO2 = (CO2 / p.K0) * 1e6;
+    double P = 101325.0;
+    double Tk = p.Tk;
+    double B = (-1636.75 + 12.0408 * Tk + (-3.27957 * 1e-2) * (Tk * Tk) + (3.16528 * 1e-5) * (Tk * Tk * Tk)) * 1e-6;
+    double dl = (57.7 + -0.118 * Tk) * 1e-6;
+    fCO2 *= 0.09807;
+    double phi = 1.0;
+    double x = fCO2 / (phi * P);
+    for (int n = 0; n < 3; n++) {
+        double om = 1.0 - x;
+        phi = exp((B + 2.0 * (om * om) * dl) * P / (8.31446261815324 * Tk));
+        x = fCO2 / (phi * P);
+    }
+    double pCO2 = fCO2 / phi;
+    return pCO2 / 0.09807;
+}
+
+// Julia's literal x^4 (compensated power by squaring).
+__device__ __forceinline__ double pow4_compensated(double x)
+{
+    double hi2 = x * x;
+    double lo2 = fma(x, x, -hi2);
+    double err = hi2 * 2.0 * lo2;
+    double hi4 = hi2 * hi2;
+    double lo4 = fma(hi2, hi2, -hi4);
+    double lo = lo4 + err;
+    return (isfinite(hi4) && isfinite(lo)) ? fma(hi4, 1.0, lo) : hi4;
+}
+
+// Gas transfer velocity (gas_transfer_velocity.jl:33-34, :61; schmidt
+// number gas_solubility.jl:211-212; solubility gas_solubility.jl:204).
+__device__ __forceinline__ double transfer_velocity(const obme_co2_exchange& g, double u10, double T, double S)
+{
+    double k660 = g.transfer_velocity[0] + g.transfer_velocity[1] * u10 + g.transfer_velocity[2] * (u10 * u10);
+    double sc = g.schmidt[0] + g.schmidt[1] * T + g.schmidt[2] * (T * T) + g.schmidt[3] * (T * T * T)
+                + g.schmidt[4] * pow4_compensated(T);
+    double Tk = T + 273.15;
+    double K0 = k0_constant(Tk, S, log(Tk));
+    double sol = K0 * teos10_density(T, S, 0.0) / 1000.0;
+    return k660 / sqrt(sc / 660.0) * sol;
+}
+
+// Everything of the CO2 flux that depends on T and S only: the carbonate
+// parameters without DIC/Alk, the mmol/m³ → mol/kg factor 1e-3/ρ and the
+// transfer velocity k (a function of u10, T, S).
+struct SurfaceCarbon {
+    CarbonParams p;   // DIC and Alk unset
+    double conv;      // 1e-3 / ρ(T, S, Pbar = 1)
+    double k;         // transfer velocity
+};
+
+__device__ __forceinline__ void surface_carbon_setup(const obme_co2_exchange& g, double T, double S, double u10,
+                                                     SurfaceCarbon& sc)
+{
+    CarbonParams& p = sc.p;
+    p.boron = 0.000232 / 10.811 * S / 1.80655;
+    p.sulfate = 0.14 / 96.06 * S / 1.80655;
+    p.fluoride = 0.000067 / 18.9984 * S / 1.80655;
+    double rho = teos10_density(T, S, 1.0);
+    p.Tk = T + 273.15;
+    sc.conv = 1e-3 / rho;
+    p.phosphate = 0.0;
+    p.silicate = 0.0;
+    carbon_constants(p.Tk, S, p);
+    sc.k = transfer_velocity(g, u10, T, S);
+}
+
+// Solve for H and return the flux F = k (pCO2_w - pCO2_air)
+// (GasExchange/gas_exchange.jl:26-38).  mode 0: the reference damped Newton
+// from 10^-initial_pH_guess; mode 1: FP32 first guess refined by FP64
+// Newton; with FALLBACK, a refinement that has not converged is redone
+// with the mode-0 solve (the fused step kernels omit the fallback to keep
+// their register budget; a non-converged refinement keeps its last iterate).
+template <bool FALLBACK = true>
+__device__ __forceinline__ double surface_flux(const SurfaceCarbon& sc, const obme_carbon_solver& s, double DIC,
+                                               double Alk, double pCO2_air, int mode, double& pH, int& evals)
+{
+    CarbonParams p = sc.p;
+    p.Alk = Alk * sc.conv;
+    p.DIC = DIC * sc.conv;
+    double H = 0.0;
+    if (mode == 1) {
+        H = (double)guess_H_fp32(p);
+        bool done = refine_H(p, H, evals);
+        if (FALLBACK && !done) H = solve_for_H(p, s, pow(10.0, -s.initial_pH_guess), evals);
+    } else {
+        H = solve_for_H(p, s, pow(10.0, -s.initial_pH_guess), evals);
+    }
+    pH = -log10(H);
+    double pCO2 = pco2_from_H(p, H);
+    return sc.k * (pCO2 - pCO2_air);
+}
+
+__device__ __forceinline__ double co2_flux(const obme_co2_exchange& g, const obme_carbon_solver& s, double DIC,
+                                           double Alk, double T, double S, double u10, double pCO2_air, int mode,
+                                           double& pH, int& evals)
+{
+    SurfaceCarbon sc;
+    surface_carbon_setup(g, T, S, u10, sc);
+    return surface_flux(sc, s, DIC, Alk, pCO2_air, mode, pH, evals);
+}
+
+}  // namespace obme

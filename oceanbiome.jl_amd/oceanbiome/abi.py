@@ -1,0 +1,158 @@
+"""ctypes mirror of include/obme.h (the C ABI of libobme_hip.so).
+
+The structs here must match the header field for field; the library exports
+obme_abi_sizeof() so the test-suite can check every size against the compiled
+layout.
+"""
+import ctypes as C
+
+OBME_OK, OBME_ERR_INVALID, OBME_ERR_HIP, OBME_ERR_UNSUPPORTED, OBME_ERR_NOMEM = 0, 1, 2, 3, 4
+OBME_NPZD, OBME_LOBSTER = 1, 2
+OBME_FP64, OBME_FP32_MIXED = 0, 1
+OBME_LIGHT_MONDO, OBME_LIGHT_ANALYTICAL = 0, 1
+OBME_LINEAR, OBME_QUADRATIC = 0, 1
+OBME_UPWIND1, OBME_CENTERED2 = 0, 1
+OBME_SURFACE_CONSTANT, OBME_SURFACE_DIURNAL, OBME_SURFACE_ARRAY = 0, 1, 2
+OBME_FIELD_CONSTANT, OBME_FIELD_PROFILE, OBME_FIELD_3D = 0, 1, 2
+OBME_MAX_TRACERS = 32
+
+NPD_PARAM_NAMES = (
+    "nitrate_half_saturation",
+    "ammonia_half_saturation",
+    "nitrate_ammonia_inhibition",
+    "light_half_saturation",
+    "phytoplankton_maximum_growth_rate",
+    "phytoplankton_exudation_fraction",
+    "ammonia_fraction_of_exudate",
+    "temperature_coefficient",
+    "phytoplankton_mortality_rate",
+    "zooplankton_mortality_rate",
+    "zooplankton_excretion_rate",
+    "phytoplankton_solid_waste_fraction",
+    "excretion_inorganic_fraction",
+    "preference_for_phytoplankton",
+    "maximum_grazing_rate",
+    "grazing_half_saturation",
+    "zooplankton_assimilation_fraction",
+    "redfield_ratio",
+    "carbon_calcite_ratio",
+    "zooplankton_gut_calcite_dissolution",
+    "phytoplankton_chlorophyll_ratio",
+    "nitrification_rate",
+    "remineralisation_inorganic_fraction",
+    "small_remineralisation_rate",
+    "large_remineralisation_rate",
+    "dissolved_remineralisation_rate",
+    "small_solid_waste_fraction",
+    "detritus_redfield_ratio",
+    "remineralisation_rate",
+    "small_particle_fraction",
+)
+
+
+class NPDParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in NPD_PARAM_NAMES] + [
+        ("light_limitation", C.c_int32),
+        ("use_temperature_coefficient", C.c_int32),
+        ("phytoplankton_mortality_formulation", C.c_int32),
+        ("grazing_concentration_formulation", C.c_int32),
+    ]
+
+
+class TwoBandPAR(C.Structure):
+    _fields_ = [
+        ("water_red_attenuation", C.c_double),
+        ("water_blue_attenuation", C.c_double),
+        ("chlorophyll_red_attenuation", C.c_double),
+        ("chlorophyll_blue_attenuation", C.c_double),
+        ("chlorophyll_red_exponent", C.c_double),
+        ("chlorophyll_blue_exponent", C.c_double),
+        ("pigment_ratio", C.c_double),
+        ("phytoplankton_chlorophyll_ratio", C.c_double),
+        ("surface_mode", C.c_int32),
+        ("pad0", C.c_int32),
+        ("surface_value", C.c_double),
+        ("surface_array", C.c_void_p),
+    ]
+
+
+class Sinking(C.Structure):
+    _fields_ = [
+        ("speed", C.c_double * OBME_MAX_TRACERS),
+        ("open_bottom", C.c_int32),
+        ("scheme", C.c_int32),
+        ("smoothing_distance", C.c_double),
+    ]
+
+
+class CarbonSolver(C.Structure):
+    _fields_ = [
+        ("max_iters", C.c_int32),
+        ("pad0", C.c_int32),
+        ("atol", C.c_double),
+        ("damping", C.c_double),
+        ("armijo_constant", C.c_double),
+        ("min_damping", C.c_double),
+        ("initial_pH_guess", C.c_double),
+    ]
+
+
+class CO2Exchange(C.Structure):
+    _fields_ = [
+        ("enabled", C.c_int32),
+        ("pad0", C.c_int32),
+        ("air_concentration", C.c_double),
+        ("wind_speed", C.c_double),
+        ("air_pressure", C.c_double),
+        ("schmidt", C.c_double * 5),
+        ("transfer_velocity", C.c_double * 3),
+    ]
+
+
+class Grid(C.Structure):
+    _fields_ = [
+        ("Nx", C.c_int32), ("Ny", C.c_int32), ("Nz", C.c_int32),
+        ("Hx", C.c_int32), ("Hy", C.c_int32), ("Hz", C.c_int32),
+        ("zc", C.POINTER(C.c_double)),
+        ("zf", C.POINTER(C.c_double)),
+        ("dz", C.POINTER(C.c_double)),
+    ]
+
+
+class FieldSource(C.Structure):
+    _fields_ = [
+        ("mode", C.c_int32),
+        ("pad0", C.c_int32),
+        ("value", C.c_double),
+        ("profile", C.POINTER(C.c_double)),
+    ]
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32),
+        ("precision", C.c_int32),
+        ("carbonate_system", C.c_int32),
+        ("pad0", C.c_int32),
+        ("grid", Grid),
+        ("npd", NPDParams),
+        ("par", TwoBandPAR),
+        ("sinking", Sinking),
+        ("solver", CarbonSolver),
+        ("co2", CO2Exchange),
+        ("temperature", FieldSource),
+        ("salinity", FieldSource),
+    ]
+
+
+# obme_abi_sizeof(which) indices (see the header)
+ABI_STRUCTS = (
+    ("obme_config", Config),
+    ("obme_grid", Grid),
+    ("obme_npd_params", NPDParams),
+    ("obme_two_band_par", TwoBandPAR),
+    ("obme_sinking", Sinking),
+    ("obme_carbon_solver", CarbonSolver),
+    ("obme_co2_exchange", CO2Exchange),
+    ("obme_field_source", FieldSource),
+)

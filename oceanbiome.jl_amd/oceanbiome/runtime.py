@@ -1,0 +1,172 @@
+"""ctypes binding of libobme_hip.so and the HIP-backed biogeochemistry.
+
+`HIPBiogeochemistry(bgc, precision)` is the device counterpart of a
+`Biogeochemistry` (NPZD / LOBSTER): it owns one library handle and exposes
+the Oceananigans.Biogeochemistry hooks the library implements
+(update_biogeochemical_state, update_tendencies) plus the fused stepper.
+Tracer, tendency and PAR fields are torch tensors on the GPU holding
+Oceananigans parent arrays (shape (Nz+2Hz, Ny+2Hy, Nx+2Hx)).
+
+The library must be built (oceanbiome.jl_amd/csrc/build.sh); there is no
+CPU fallback: a missing library or a missing GPU raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libobme_hip.so")
+
+_lib = None
+
+
+class OBMEError(RuntimeError):
+    pass
+
+
+def library():
+    """Load libobme_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OBMEError(f"{LIB_PATH} not found: build it with oceanbiome.jl_amd/csrc/build.sh")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    sig = {
+        "obme_create": (C.c_int32, [C.POINTER(abi.Config), C.POINTER(vp)]),
+        "obme_destroy": (C.c_int32, [vp]),
+        "obme_last_error": (C.c_char_p, []),
+        "obme_version": (C.c_char_p, []),
+        "obme_required_tracers": (C.c_int32, [vp, C.c_char_p, C.c_int32]),
+        "obme_update_state": (C.c_int32, [vp, C.POINTER(vp), vp, C.c_double, vp]),
+        "obme_add_tendencies": (C.c_int32, [vp, C.POINTER(vp), C.POINTER(vp), vp, vp, vp, C.c_double, vp]),
+        "obme_step": (C.c_int32, [vp, C.POINTER(vp), vp, vp, C.c_int32, C.c_double, C.c_double, vp]),
+        "obme_co2_flux": (C.c_int32, [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp, C.c_int64, C.c_int32, vp]),
+        "obme_abi_sizeof": (C.c_int64, [C.c_int32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+EXPORTED_SYMBOLS = (
+    "obme_create", "obme_destroy", "obme_last_error", "obme_version", "obme_required_tracers",
+    "obme_update_state", "obme_add_tendencies", "obme_step", "obme_co2_flux", "obme_abi_sizeof",
+)
+
+
+def _check(status):
+    if status != abi.OBME_OK:
+        msg = library().obme_last_error().decode()
+        raise OBMEError(f"obme status {status}: {msg}")
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    return C.c_void_p(t.data_ptr())
+
+
+def _ptr_array(ts):
+    return (C.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class HIPBiogeochemistry:
+    """Device biogeochemistry for a Biogeochemistry model description."""
+
+    def __init__(self, bgc, precision="fp64", surface_array=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise OBMEError("HIPBiogeochemistry needs a GPU")
+        self.bgc = bgc
+        self.precision = abi.OBME_FP64 if precision == "fp64" else abi.OBME_FP32_MIXED
+        self.dtype = torch.float64 if self.precision == abi.OBME_FP64 else torch.float32
+        self.cfg = bgc.config(self.precision)
+        self._surface = None
+        if surface_array is not None:
+            self._surface = surface_array.to(device="cuda", dtype=self.dtype).contiguous()
+            self.cfg.par.surface_mode = abi.OBME_SURFACE_ARRAY
+            self.cfg.par.surface_array = self._surface.data_ptr()
+        h = C.c_void_p()
+        _check(library().obme_create(C.byref(self.cfg), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            library().obme_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- Oceananigans.Biogeochemistry API ---------------------------------------
+    def required_biogeochemical_tracers(self):
+        buf = C.create_string_buffer(512)
+        n = library().obme_required_tracers(self.handle, buf, 512)
+        if n < 0:
+            raise OBMEError("obme_required_tracers failed")
+        return tuple(buf.value.decode().split(";"))
+
+    def _check_fields(self, fields):
+        g = self.bgc.grid
+        for f in fields:
+            if f.dtype != self.dtype or tuple(f.shape) != g.parent_shape or not f.is_cuda or not f.is_contiguous():
+                raise OBMEError(f"field must be a contiguous {self.dtype} CUDA tensor of shape {g.parent_shape}")
+
+    def update_biogeochemical_state(self, tracers, PAR, t=0.0):
+        """The light step (Light/2band.jl:148-155): PAR interior from P."""
+        self._check_fields(list(tracers) + [PAR])
+        _check(library().obme_update_state(self.handle, _ptr_array(tracers), _ptr(PAR), t, _stream()))
+
+    def update_tendencies(self, tracers, G, PAR=None, T=None, S=None, t=0.0):
+        """Add BGC transitions, drift divergence and the top CO2 flux into G."""
+        self._check_fields(list(tracers) + list(G) + ([PAR] if PAR is not None else []))
+        _check(library().obme_add_tendencies(self.handle, _ptr_array(tracers), _ptr_array(G), _ptr(PAR),
+                                             _ptr(T), _ptr(S), t, _stream()))
+
+    def step(self, tracers, nsteps, dt, t0=0.0, T=None, S=None):
+        """nsteps fused RK3 steps in place (physics frozen)."""
+        self._check_fields(tracers)
+        _check(library().obme_step(self.handle, _ptr_array(tracers), _ptr(T), _ptr(S), int(nsteps), float(dt),
+                                   float(t0), _stream()))
+
+    def co2_flux(self, DIC, Alk, T, S, u10=None, pCO2_air=None, mode=0, pH=None, flux=None):
+        """pH (free scale) and CO2 flux on point sets of float64 CUDA tensors."""
+        import torch
+        n = DIC.numel()
+        for t in (DIC, Alk, T, S) + ((u10,) if u10 is not None else ()):
+            if t.dtype != torch.float64 or not t.is_cuda or t.numel() != n or not t.is_contiguous():
+                raise OBMEError("co2_flux inputs must be contiguous float64 CUDA tensors of equal size")
+        if pH is None:
+            pH = torch.empty_like(DIC)
+        if flux is None:
+            flux = torch.empty_like(DIC)
+        air = self.cfg.co2.air_concentration if pCO2_air is None else pCO2_air
+        _check(library().obme_co2_flux(self.handle, _ptr(DIC), _ptr(Alk), _ptr(T), _ptr(S), _ptr(u10), float(air),
+                                       _ptr(pH), _ptr(flux), n, int(mode), _stream()))
+        return pH, flux
+
+
+def abi_layout_mismatches():
+    """Names of the ABI structs whose ctypes size differs from the library's."""
+    L = library()
+    bad = []
+    for idx, (name, st) in enumerate(abi.ABI_STRUCTS):
+        if L.obme_abi_sizeof(idx) != C.sizeof(st):
+            bad.append((name, L.obme_abi_sizeof(idx), C.sizeof(st)))
+    return bad

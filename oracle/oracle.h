@@ -1,0 +1,143 @@
+/*
+ * oracle.h — FP64 CPU restatement of OceanBioME.jl v0.17.6's biogeochemistry
+ * step.  TEST INFRASTRUCTURE ONLY: this code is the checker for the HIP
+ * library.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it; nothing in the product links or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative
+ * to the OceanBioME.jl source tree).  Evaluation order follows the Julia
+ * expressions left to right; the build uses -ffp-contract=off so no FMA is
+ * formed where Julia would not form one.
+ */
+#ifndef OBME_ORACLE_H
+#define OBME_ORACLE_H
+
+#include "../include/obme.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Julia scalar semantics ---- */
+double orc_julia_min(double x, double y);
+double orc_julia_max(double x, double y);
+double orc_julia_eps(double x);
+double orc_pow_int(double x, int n);  /* Julia Float64^Integer */
+double orc_pow_int4(double x);
+
+/* ---- seawater density and carbonate chemistry ---- */
+double orc_teos10_density(double T, double Sp, double Pbar);
+typedef struct orc_constants {
+    double K0, K1, K2, KB, KW, KS, KF, KP1, KP2, KP3, KSi, Is;
+} orc_constants;
+void orc_equilibrium_constants(double Tk, double S, int has_P, double P, orc_constants* out);
+double orc_pressure_correction(int which, double Tk, double P);
+double orc_ksp_calcite(double Tk, double S, int has_P, double P);
+double orc_ksp_aragonite(double Tk, double S, int has_P, double P);
+double orc_virial_B(double Tk);
+double orc_virial_delta(double Tk);
+
+typedef struct orc_carbon_params {
+    double DIC, Alk, boron, sulfate, fluoride, silicate, phosphate;
+    double K1, K2, KB, KW, KS, KF, KP1, KP2, KP3, KSi;
+} orc_carbon_params;
+double orc_alkalinity_residual(double H, const orc_carbon_params* p);
+double orc_dalkalinity_residual(double H, const orc_carbon_params* p);
+
+/* Damped Newton on the alkalinity residual; *evals counts residual calls.
+ * fast_exit=1 stops at an exact fixed point (bit-identical result). */
+double orc_solve_for_H(const orc_carbon_params* p, const obme_carbon_solver* s,
+                       double x0, int fast_exit, int* iters, int* evals);
+
+/* Damped Newton on one of the analytic test functions of test/test_solvers.jl
+ * (which = 1..5); lower bound used iff has_lower. */
+double orc_damped_newton_test(int which, double x0, double damping, int has_lower,
+                              double lower, int fast_exit, int* iters);
+
+/* CarbonChemistry functor outputs (carbon_chemistry.jl:111-211).
+ * output: 0 fCO2, 1 pCO2, 2 pH free, 3 pH total, 4 pH seawater.
+ * pH < -100 means "no pH given, solve for it". */
+double orc_carbon_chemistry(double DIC, double Alk, double T, double S, double pH,
+                            int output, const obme_carbon_solver* s, int fast_exit,
+                            double silicate, double phosphate, int* evals);
+double orc_carbonate_concentration(double DIC, double Alk, double T, double S,
+                                   int has_P, double P, const obme_carbon_solver* s);
+double orc_calcite_saturation(double DIC, double Alk, double T, double S,
+                              int has_P, double P, const obme_carbon_solver* s);
+
+double orc_schmidt_co2(const obme_co2_exchange* g, double T);
+double orc_transfer_velocity(const obme_co2_exchange* g, double u10, double T, double S);
+/* F = k (pCO2_w - pCO2_air); also returns the free-scale pH of the solve. */
+double orc_co2_flux(const obme_co2_exchange* g, const obme_carbon_solver* s,
+                    double DIC, double Alk, double T, double S, double u10,
+                    double pCO2_air, int fast_exit, double* pH_out, int* evals);
+void orc_co2_flux_batch(const obme_co2_exchange* g, const obme_carbon_solver* s,
+                        const double* DIC, const double* Alk, const double* T,
+                        const double* S, const double* u10, double pCO2_air,
+                        double* out_pH, double* out_flux, long n, int nthreads);
+
+/* ---- NutrientsPlanktonDetritus tendencies ---- */
+/* vals: tracer values in tracer order (NPZD: N,P,Z,T,D; LOBSTER: NO3,NH4,P,Z,
+ * sPOM,bPOM,DOM[,DIC,Alk]); out: the BGC transition of every tracer. */
+void orc_npd_tendencies(int model, int carbonate, const obme_npd_params* p,
+                        const double* vals, double PAR, double* out);
+int orc_npd_ntracers(int model, int carbonate);
+
+/* ---- light ---- */
+/* TwoBand PAR for one column (Light/2band.jl:1-33); P and PAR of length Nz,
+ * bottom to top. */
+void orc_two_band_column(const obme_two_band_par* par, int Nz, const double* zc,
+                         const double* zf, const double* P, double PAR0, double* PAR);
+double orc_default_surface_PAR(double t);
+
+/* ---- one column, nsteps of RK3 ---- */
+typedef struct orc_column_env {
+    const obme_config* cfg;
+    int Nz;
+    const double* zc;   /* [Nz]   */
+    const double* zf;   /* [Nz+1] */
+    const double* dz;   /* [Nz]   */
+    double PAR0_const;  /* used when surface_mode != DIURNAL */
+    double T_surface;   /* gas exchange inputs (LOBSTER) */
+    double S_surface;
+    const double* Tcol; /* NPZD: T tracer column (read-only) or NULL */
+    int fast_exit;
+} orc_column_env;
+
+/* U[n*Nz + k]: tracer n at level k (0-based, bottom to top).  Advances nsteps
+ * in place; if PAR_out != NULL writes the PAR of the last stage evaluated. */
+void orc_column_run(const orc_column_env* env, double* U, int nsteps, double dt,
+                    double t0, double* PAR_out);
+/* Tendencies of one column at time t (PAR scan + BGC + sinking + CO2 flux). */
+void orc_column_tendencies(const orc_column_env* env, const double* U, double t,
+                           double* G, double* PAR);
+
+/* ---- 3-D driver on Oceananigans-layout arrays ---- */
+/* tracers[n]: parent arrays (Nx+2Hx)(Ny+2Hy)(Nz+2Hz).  T3/S3 may be NULL.
+ * Runs nthreads OpenMP threads over columns. */
+/* PAR0: host Nx*Ny surface PAR (OBME_SURFACE_ARRAY) or NULL. */
+int orc_grid_run(const obme_config* cfg, double* const* tracers, const double* T3,
+                 const double* S3, const double* PAR0, int nsteps, double dt, double t0,
+                 int nthreads, int fast_exit);
+int orc_grid_run_columns(const obme_config* cfg, double* const* tracers, const double* T3,
+                         const double* S3, const double* PAR0, int nsteps, double dt,
+                         double t0, const int* col_i, const int* col_j, int ncols,
+                         int nthreads, int fast_exit);
+int orc_grid_update_state(const obme_config* cfg, const double* const* tracers,
+                          const double* PAR0, double* PAR, double t);
+int orc_grid_tendencies(const obme_config* cfg, const double* const* tracers,
+                        double* const* G, const double* T3, const double* S3,
+                        const double* PAR0, double t, double* PAR);
+
+/* ---- batched ensemble of independent NPZD columns ---- */
+/* offsets[c]..offsets[c+1]: levels of column c, k fastest, bottom to top.
+ * params: OBME_BP_COUNT arrays of n_columns doubles. U: 4 tracers (N,P,Z,D)
+ * each of total length offsets[n]; T is uniform T_const. */
+int orc_batch_run(const obme_config* cfg, int n_columns, const int* offsets,
+                  const double* const* params, double dz, double T_const,
+                  double* const* U, int nsteps, double dt, double t0, int nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

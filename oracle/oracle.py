@@ -1,0 +1,199 @@
+"""ctypes wrapper of the FP64 oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
+"""
+import ctypes as C
+import os
+import subprocess
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "oceanbiome.jl_amd"))
+
+from oceanbiome import abi  # noqa: E402  (ABI structs shared with the library)
+
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+D = C.c_double
+I = C.c_int
+DP = C.POINTER(C.c_double)
+
+
+def build(force=False):
+    """Compile liboracle.so with the committed Makefile."""
+    if force and os.path.exists(LIB_PATH):
+        os.remove(LIB_PATH)
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "orc_julia_min": (D, [D, D]),
+        "orc_julia_max": (D, [D, D]),
+        "orc_julia_eps": (D, [D]),
+        "orc_pow_int": (D, [D, I]),
+        "orc_teos10_density": (D, [D, D, D]),
+        "orc_pressure_correction": (D, [I, D, D]),
+        "orc_ksp_calcite": (D, [D, D, I, D]),
+        "orc_ksp_aragonite": (D, [D, D, I, D]),
+        "orc_virial_B": (D, [D]),
+        "orc_virial_delta": (D, [D]),
+        "orc_damped_newton_test": (D, [I, D, D, I, D, I, C.POINTER(I)]),
+        "orc_carbon_chemistry": (D, [D, D, D, D, D, I, C.POINTER(abi.CarbonSolver), I, D, D, C.POINTER(I)]),
+        "orc_carbonate_concentration": (D, [D, D, D, D, I, D, C.POINTER(abi.CarbonSolver)]),
+        "orc_calcite_saturation": (D, [D, D, D, D, I, D, C.POINTER(abi.CarbonSolver)]),
+        "orc_schmidt_co2": (D, [C.POINTER(abi.CO2Exchange), D]),
+        "orc_transfer_velocity": (D, [C.POINTER(abi.CO2Exchange), D, D, D]),
+        "orc_co2_flux": (D, [C.POINTER(abi.CO2Exchange), C.POINTER(abi.CarbonSolver), D, D, D, D, D, D, I,
+                             DP, C.POINTER(I)]),
+        "orc_co2_flux_batch": (None, [C.POINTER(abi.CO2Exchange), C.POINTER(abi.CarbonSolver), DP, DP, DP, DP,
+                                      DP, D, DP, DP, C.c_long, I]),
+        "orc_npd_tendencies": (None, [I, I, C.POINTER(abi.NPDParams), DP, D, DP]),
+        "orc_npd_ntracers": (I, [I, I]),
+        "orc_two_band_column": (None, [C.POINTER(abi.TwoBandPAR), I, DP, DP, DP, D, DP]),
+        "orc_default_surface_PAR": (D, [D]),
+        "orc_grid_run": (I, [C.POINTER(abi.Config), C.POINTER(DP), DP, DP, DP, I, D, D, I, I]),
+        "orc_grid_run_columns": (I, [C.POINTER(abi.Config), C.POINTER(DP), DP, DP, DP, I, D, D,
+                                     C.POINTER(I), C.POINTER(I), I, I, I]),
+        "orc_grid_update_state": (I, [C.POINTER(abi.Config), C.POINTER(DP), DP, DP, D]),
+        "orc_grid_tendencies": (I, [C.POINTER(abi.Config), C.POINTER(DP), C.POINTER(DP), DP, DP, DP, D, DP]),
+        "orc_batch_run": (I, [C.POINTER(abi.Config), I, C.POINTER(I), C.POINTER(DP), D, D, C.POINTER(DP), I, D,
+                              D, I]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(DP)
+
+
+def _ptrs(arrs):
+    return (DP * len(arrs))(*[_dp(a) for a in arrs])
+
+
+# ---- scalar helpers -------------------------------------------------------
+
+def julia_min(x, y):
+    return lib().orc_julia_min(x, y)
+
+
+def julia_max(x, y):
+    return lib().orc_julia_max(x, y)
+
+
+def julia_eps(x):
+    return lib().orc_julia_eps(x)
+
+
+def pow_int(x, n):
+    return lib().orc_pow_int(x, n)
+
+
+def density(T, S, Pbar):
+    return lib().orc_teos10_density(T, S, Pbar)
+
+
+def pressure_correction(which, Tk, P):
+    return lib().orc_pressure_correction(which, Tk, P)
+
+
+def damped_newton_test(which, x0, damping=0.5, lower=None, fast_exit=0):
+    it = I(0)
+    x = lib().orc_damped_newton_test(which, x0, damping, 0 if lower is None else 1,
+                                     0.0 if lower is None else lower, fast_exit, C.byref(it))
+    return x, it.value
+
+
+def carbon_chemistry(solver, DIC, Alk, T, S, pH=None, output="fCO2", fast_exit=1, silicate=0.0,
+                     phosphate=0.0):
+    outs = {"fCO2": 0, "pCO2": 1, "pHf": 2, "pHt": 3, "pHs": 4}
+    ev = I(0)
+    v = lib().orc_carbon_chemistry(DIC, Alk, T, S, -1000.0 if pH is None else pH, outs[output],
+                                   C.byref(solver), fast_exit, silicate, phosphate, C.byref(ev))
+    return v, ev.value
+
+
+def co2_flux(gas, solver, DIC, Alk, T, S, u10, pCO2_air, fast_exit=1):
+    pH = D(0)
+    ev = I(0)
+    F = lib().orc_co2_flux(C.byref(gas), C.byref(solver), DIC, Alk, T, S, u10, pCO2_air, fast_exit,
+                           C.byref(pH), C.byref(ev))
+    return F, pH.value, ev.value
+
+
+def co2_flux_batch(gas, solver, DIC, Alk, T, S, u10, pCO2_air, nthreads=8):
+    n = DIC.size
+    pH = np.empty(n)
+    F = np.empty(n)
+    lib().orc_co2_flux_batch(C.byref(gas), C.byref(solver), _dp(DIC), _dp(Alk), _dp(T), _dp(S), _dp(u10),
+                             pCO2_air, _dp(pH), _dp(F), n, nthreads)
+    return pH, F
+
+
+def npd_tendencies(model, carbonate, params, vals, PAR):
+    n = lib().orc_npd_ntracers(model, carbonate)
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    out = np.zeros(n)
+    lib().orc_npd_tendencies(model, carbonate, C.byref(params), _dp(v), PAR, _dp(out))
+    return out
+
+
+def two_band_column(par, zc, zf, P, PAR0):
+    Nz = len(zc)
+    out = np.zeros(Nz)
+    lib().orc_two_band_column(C.byref(par), Nz, _dp(zc), _dp(zf), _dp(np.ascontiguousarray(P)), PAR0,
+                              _dp(out))
+    return out
+
+
+# ---- grid drivers ------------------------------------------------------------
+
+def grid_run(cfg, tracers, nsteps, dt, t0=0.0, T3=None, S3=None, PAR0=None, nthreads=8, fast_exit=1):
+    """Advance Oceananigans-layout float64 arrays in place."""
+    lib().orc_grid_run(C.byref(cfg), _ptrs(tracers), _dp(T3), _dp(S3), _dp(PAR0), nsteps, dt, t0,
+                       nthreads, fast_exit)
+
+
+def grid_run_columns(cfg, tracers, cols_i, cols_j, nsteps, dt, t0=0.0, T3=None, S3=None, PAR0=None,
+                     nthreads=8, fast_exit=1):
+    ci = np.ascontiguousarray(cols_i, dtype=np.int32)
+    cj = np.ascontiguousarray(cols_j, dtype=np.int32)
+    lib().orc_grid_run_columns(C.byref(cfg), _ptrs(tracers), _dp(T3), _dp(S3), _dp(PAR0), nsteps, dt, t0,
+                               ci.ctypes.data_as(C.POINTER(I)), cj.ctypes.data_as(C.POINTER(I)), len(ci),
+                               nthreads, fast_exit)
+
+
+def grid_update_state(cfg, tracers, PAR, t=0.0, PAR0=None):
+    lib().orc_grid_update_state(C.byref(cfg), _ptrs(tracers), _dp(PAR0), _dp(PAR), t)
+
+
+def grid_tendencies(cfg, tracers, G, t=0.0, T3=None, S3=None, PAR0=None, PAR=None):
+    lib().orc_grid_tendencies(C.byref(cfg), _ptrs(tracers), _ptrs(G), _dp(T3), _dp(S3), _dp(PAR0), t,
+                              _dp(PAR))
+
+
+def batch_run(cfg, offsets, params, dz, T_const, U, nsteps, dt, t0=0.0, nthreads=8):
+    off = np.ascontiguousarray(offsets, dtype=np.int32)
+    lib().orc_batch_run(C.byref(cfg), len(off) - 1, off.ctypes.data_as(C.POINTER(I)), _ptrs(params), dz,
+                        T_const, _ptrs(U), nsteps, dt, t0, nthreads)

@@ -1,0 +1,569 @@
+/*
+ * oracle_npd.c — FP64 restatement of OceanBioME's NutrientsPlanktonDetritus
+ * tendencies (NPZD and LOBSTER + CarbonateSystem), the TwoBand PAR column
+ * recurrence, the sinking flux divergence and the RK3 stage sequence.
+ * TEST INFRASTRUCTURE ONLY (the checker; see oracle.h).
+ *
+ * Reference files (relative to OceanBioME.jl/src):
+ *   Models/AdvectedPopulations/NutrientsPlanktonDetritus/{plankton,nutrients,
+ *   detritus,carbonate_system}.jl, Light/2band.jl, Light/Light.jl,
+ *   Utils/sinking_velocity_fields.jl, BoxModel/timesteppers.jl, OceanBioME.jl.
+ *
+ * Each reference helper recomputes shared sub-expressions (growth, grazing);
+ * in FP64 a recomputation yields the identical value, so each is computed
+ * once here and reused, with the reference's operand order kept.
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define EPS0 4.9406564584124654e-324 /* eps(0.0) */
+
+int orc_npd_ntracers(int model, int carbonate)
+{
+    if (model == OBME_NPZD) return 5;
+    if (model == OBME_LOBSTER) return carbonate ? 9 : 7;
+    return 0;
+}
+
+/* mortality / concentration_limit (plankton.jl:86-90) */
+static double mortality(int form, double X, double m) { return form == OBME_LINEAR ? m * X : m * (X * X); }
+static double concentration_limit(int form, double X, double k)
+{
+    return form == OBME_LINEAR ? X / (X + k) : (X * X) / ((X * X) + (k * k));
+}
+
+/* light_limitation (plankton.jl:219-220) */
+static double light_limitation(int form, double PAR, double kPAR)
+{
+    if (form == OBME_LIGHT_MONDO) return PAR / (kPAR + PAR);
+    return PAR / sqrt((PAR * PAR) + (kPAR * kPAR));
+}
+
+/* weighted_phytoplankton_preference (plankton.jl:393-397) */
+static double preference(const obme_npd_params* q, double P, double sPOM)
+{
+    double pt = q->preference_for_phytoplankton;
+    return pt * P / (pt * P + (1 - pt) * sPOM + EPS0);
+}
+
+typedef struct grazing_terms {
+    double total;   /* total_grazing (plankton.jl:118-135)      */
+    double onP;     /* grazing(Val{:P}) (:352-369)              */
+    double onSmall; /* grazing(Val{:sPOM}/Val{:D}) (:371-389)   */
+} grazing_terms;
+
+static grazing_terms grazing(const obme_npd_params* q, double P, double Z, double sPOM)
+{
+    double kG = q->grazing_half_saturation;
+    double g = q->maximum_grazing_rate;
+    double p = preference(q, P, sPOM);
+    double food = p * P + (1 - p) * sPOM;
+    double L = concentration_limit(q->grazing_concentration_formulation, food, kG);
+    grazing_terms out;
+    out.total = g * L * Z;
+    out.onP = g * p * L * P / (food + orc_julia_eps(food)) * Z;
+    out.onSmall = g * (1 - p) * L * sPOM / (food + orc_julia_eps(food)) * Z;
+    return out;
+}
+
+/* NPZD: Nutrient + PhytoZoo + Detritus; tracers (N, P, Z, T, D). */
+static void npzd_tendencies(const obme_npd_params* q, const double* v, double PAR, double* out)
+{
+    double N = v[0], P = v[1], Z = v[2], T = v[3], D = v[4];
+
+    /* phytoplankton_growth (plankton.jl:191-206) with nutrient_limitation
+     * (Nutrient, :180-188) and temperature_limitation (:208-214) */
+    double Ln = N / (N + q->nitrate_half_saturation);
+    double Ll = light_limitation(q->light_limitation, PAR, q->light_half_saturation);
+    double Lt = q->use_temperature_coefficient ? pow(q->temperature_coefficient, T / 10) : 1.0;
+    double muP = q->phytoplankton_maximum_growth_rate * Ll * Ln * Lt * P;
+
+    /* small_particulate_concentration(::Detritus) (detritus.jl:386-387) */
+    double sPOM = D * q->small_particle_fraction;
+    grazing_terms G = grazing(q, P, Z, sPOM);
+    double nuP = mortality(q->phytoplankton_mortality_formulation, P, q->phytoplankton_mortality_rate);
+
+    /* P (plankton.jl:92-105) */
+    out[1] = (1 - q->phytoplankton_exudation_fraction) * muP - G.onP - nuP;
+    /* Z (plankton.jl:107-116) */
+    out[2] = q->zooplankton_assimilation_fraction * G.total - q->zooplankton_mortality_rate * (Z * Z)
+             - q->zooplankton_excretion_rate * Z;
+    /* N (nutrients.jl:86-90): plankton_inorganic_nitrogen_waste (plankton.jl:292-306)
+     * + detritus_inorganic_nitrogen_waste (detritus.jl:389-390)
+     * - nutrient_uptake(Val{:N}) (plankton.jl:271-278) */
+    double pinw = q->excretion_inorganic_fraction * q->zooplankton_excretion_rate * Z
+                  + (1 - q->phytoplankton_solid_waste_fraction) * nuP;
+    double dinw = D * q->remineralisation_rate;
+    double uptake = muP * (1 - q->ammonia_fraction_of_exudate * q->phytoplankton_exudation_fraction);
+    out[0] = pinw + dinw - uptake;
+    /* T: no method -> zero(grid) (NutrientsPlanktonDetritus.jl:88) */
+    out[3] = 0.0;
+    /* D (detritus.jl:373-379): plankton_organic_nitrogen_waste (plankton.jl:312-325)
+     * + solid_waste (:330-346) - grazing(Val{:D}) - remineralisation */
+    double ponw = (1 - q->ammonia_fraction_of_exudate) * q->phytoplankton_exudation_fraction * muP
+                  + (1 - q->excretion_inorganic_fraction) * q->zooplankton_excretion_rate * Z;
+    double solid = (1 - q->zooplankton_assimilation_fraction) * G.total
+                   + q->phytoplankton_solid_waste_fraction * nuP + q->zooplankton_mortality_rate * (Z * Z);
+    out[4] = ponw + solid - G.onSmall - q->remineralisation_rate * D;
+}
+
+/* LOBSTER: NitrateAmmonia + PhytoZoo + TwoParticleAndDissolved [+ CarbonateSystem];
+ * tracers (NO3, NH4, P, Z, sPOM, bPOM, DOM [, DIC, Alk]). */
+static void lobster_tendencies(const obme_npd_params* q, int carbonate, const double* v, double PAR,
+                               double* out)
+{
+    double NO3 = v[0], NH4 = v[1], P = v[2], Z = v[3], sPOM = v[4], bPOM = v[5], DOM = v[6];
+
+    /* nitrogen_limitation (plankton.jl:150-166) */
+    double nl = NO3 * exp(-q->nitrate_ammonia_inhibition * NH4) / (NO3 + q->nitrate_half_saturation);
+    double al = orc_julia_max(0.0, NH4 / (q->ammonia_half_saturation + NH4));
+    double Ln = (nl + al) / 2;
+    double Ll = light_limitation(q->light_limitation, PAR, q->light_half_saturation);
+    /* temperature_coefficient = nothing -> one(eltype) (plankton.jl:208) */
+    double Lt = 1.0;
+    double muP = q->phytoplankton_maximum_growth_rate * Ll * Ln * Lt * P;
+
+    grazing_terms G = grazing(q, P, Z, sPOM);
+    double nuP = mortality(q->phytoplankton_mortality_formulation, P, q->phytoplankton_mortality_rate);
+
+    double gam = q->phytoplankton_exudation_fraction;
+    double alpha_x = q->ammonia_fraction_of_exudate;
+    double nitrification = q->nitrification_rate * NH4; /* nutrients.jl:55 */
+
+    /* nutrient_uptake NO3 / NH4 (plankton.jl:223-261) */
+    double up_NO3 = muP * nl / (nl + al + EPS0);
+    double waste = alpha_x * gam * muP;
+    double up_NH4 = muP * al / (nl + al + EPS0) - waste;
+
+    /* plankton_inorganic_nitrogen_waste (plankton.jl:292-306) */
+    double pinw = q->excretion_inorganic_fraction * q->zooplankton_excretion_rate * Z
+                  + (1 - q->phytoplankton_solid_waste_fraction) * nuP;
+    /* detritus_inorganic_nitrogen_waste (detritus.jl:252-263) */
+    double a = q->remineralisation_inorganic_fraction, smu = q->small_remineralisation_rate,
+           bmu = q->large_remineralisation_rate, dmu = q->dissolved_remineralisation_rate;
+    double dinw = (a * (smu * sPOM + bmu * bPOM) + dmu * DOM);
+    /* plankton_organic_nitrogen_waste (plankton.jl:312-325) */
+    double ponw = (1 - alpha_x) * gam * muP
+                  + (1 - q->excretion_inorganic_fraction) * q->zooplankton_excretion_rate * Z;
+    /* detritus_organic_nitrogen_waste (detritus.jl:265-274) */
+    double donw = (1 - a) * (smu * sPOM + bmu * bPOM);
+    /* solid_waste (plankton.jl:330-346) */
+    double solid = (1 - q->zooplankton_assimilation_fraction) * G.total
+                   + q->phytoplankton_solid_waste_fraction * nuP + q->zooplankton_mortality_rate * (Z * Z);
+
+    /* NO3, NH4 (nutrients.jl:57-67) */
+    double dNO3 = nitrification - up_NO3;
+    double dNH4 = pinw + dinw - nitrification - up_NH4;
+    out[0] = dNO3;
+    out[1] = dNH4;
+    /* P, Z (plankton.jl:92-116) */
+    out[2] = (1 - gam) * muP - G.onP - nuP;
+    out[3] = q->zooplankton_assimilation_fraction * G.total - q->zooplankton_mortality_rate * (Z * Z)
+             - q->zooplankton_excretion_rate * Z;
+    /* sPOM, bPOM, DOM (detritus.jl:234-249) */
+    out[4] = q->small_solid_waste_fraction * solid - G.onSmall - smu * sPOM;
+    out[5] = (1 - q->small_solid_waste_fraction) * solid - bmu * bPOM;
+    out[6] = ponw + donw - dmu * DOM;
+
+    if (carbonate) {
+        double R = q->redfield_ratio, rho = q->carbon_calcite_ratio;
+        double Rd = q->detritus_redfield_ratio;
+        /* phytoplankton_primary_production (plankton.jl:280-290) */
+        double pp = (1 + rho * (1 - gam) - alpha_x * gam) * muP * R;
+        /* plankton_inorganic_carbon_waste (plankton.jl:308-309) */
+        double picw = R * pinw;
+        /* detritus_inorganic_carbon_waste (detritus.jl:276-287) */
+        double sPOC = sPOM * Rd, bPOC = bPOM * Rd, DOC = DOM * Rd;
+        double dicw = a * (smu * sPOC + bmu * bPOC) + dmu * DOC;
+        /* calcite_dissolution, fixed-Redfield override (plankton.jl:429-441) */
+        double cdis = (G.onP + nuP) * rho * R;
+        /* calcite_uptake (plankton.jl:443-450) */
+        double cup = 2 * rho * muP * R;
+        /* DIC, Alk (carbonate_system.jl:50-62) */
+        out[7] = -pp + picw + dicw + cdis;
+        out[8] = dNH4 * 0.9375 - dNO3 * 1.0625 - 2.0 * cup + 2.0 * cdis;
+    }
+}
+
+void orc_npd_tendencies(int model, int carbonate, const obme_npd_params* p, const double* vals,
+                        double PAR, double* out)
+{
+    if (model == OBME_NPZD) npzd_tendencies(p, vals, PAR, out);
+    else lobster_tendencies(p, carbonate, vals, PAR, out);
+}
+
+/* ---------------- light ---------------- */
+
+/* default_surface_PAR (Light/Light.jl:217-219); hour = 3600 s. */
+double orc_default_surface_PAR(double t)
+{
+    return 100 * orc_julia_max(0.0, cos(t * M_PI / (12 * 3600.0)));
+}
+
+/* update_TwoBandPhotosyntheticallyActiveRadiation! (Light/2band.jl:1-33). */
+void orc_two_band_column(const obme_two_band_par* par, int Nz, const double* zc, const double* zf,
+                         const double* P, double PAR0, double* PAR)
+{
+    double kr = par->water_red_attenuation, kb = par->water_blue_attenuation;
+    double xr = par->chlorophyll_red_attenuation, xb = par->chlorophyll_blue_attenuation;
+    double er = par->chlorophyll_red_exponent, eb = par->chlorophyll_blue_exponent;
+    double r = par->pigment_ratio, Rcp = par->phytoplankton_chlorophyll_ratio;
+    int t = Nz - 1; /* 0-based top cell */
+    double chlr = (zf[Nz] - zc[t]) * pow(P[t] * Rcp / r, er);
+    double chlb = (zf[Nz] - zc[t]) * pow(P[t] * Rcp / r, eb);
+    PAR[t] = PAR0 * (exp(kr * zc[t] - xr * chlr) + exp(kb * zc[t] - xb * chlb)) / 2;
+    for (int k = Nz - 2; k >= 0; k--) {
+        chlr += (zc[k + 1] - zf[k + 1]) * pow(P[k + 1] * Rcp / r, er) + (zf[k + 1] - zc[k]) * pow(P[k] * Rcp / r, er);
+        chlb += (zc[k + 1] - zf[k + 1]) * pow(P[k + 1] * Rcp / r, eb) + (zf[k + 1] - zc[k]) * pow(P[k] * Rcp / r, eb);
+        PAR[k] = PAR0 * (exp(kr * zc[k] - xr * chlr) + exp(kb * zc[k] - xb * chlb)) / 2;
+    }
+}
+
+/* ---------------- sinking ---------------- */
+
+/* setup_velocity_fields (Utils/sinking_velocity_fields.jl:10-35): w on faces
+ * 1..Nz (0-based 0..Nz-1) = -speed * (open ? 1 : 1 - exp((1-k)/d)); top face 0. */
+static void sinking_profile(const obme_sinking* s, double speed, int Nz, double* w)
+{
+    double d = s->smoothing_distance > 0 ? s->smoothing_distance : 2.0;
+    for (int k = 1; k <= Nz; k++)
+        w[k - 1] = -speed * (s->open_bottom ? 1.0 : (1 - exp((1 - k) / d)));
+    w[Nz] = 0.0;
+}
+
+/* -∂z(w c) in flux form for one tracer column (Oceananigans div_Uc with the
+ * drift velocity; the in-tree flux use is Sediments/tracked_fields.jl:59-60).
+ * Face k (0-based, between cells k-1 and k): w <= 0, the upwind value is
+ * cell k; at the bottom face the zero-gradient halo gives cell 0 for every
+ * scheme; the top face carries w = 0. */
+static void sinking_divergence(int scheme, int Nz, const double* w, const double* c, const double* dz,
+                               double* div)
+{
+    for (int k = 0; k < Nz; k++) {
+        double cf_hi, cf_lo;
+        if (scheme == OBME_CENTERED2) {
+            cf_hi = (k + 1 < Nz) ? (c[k] + c[k + 1]) / 2 : c[k];
+            cf_lo = (k > 0) ? (c[k - 1] + c[k]) / 2 : c[k];
+        } else {
+            cf_hi = (k + 1 < Nz) ? c[k + 1] : c[k];
+            cf_lo = c[k];
+        }
+        double Fhi = w[k + 1] * cf_hi;
+        double Flo = w[k] * cf_lo;
+        div[k] = (Fhi - Flo) / dz[k];
+    }
+}
+
+/* ---------------- column stepper ---------------- */
+
+static double surface_PAR(const orc_column_env* env, double t)
+{
+    if (env->cfg->par.surface_mode == OBME_SURFACE_DIURNAL) return orc_default_surface_PAR(t);
+    return env->PAR0_const;
+}
+
+void orc_column_tendencies(const orc_column_env* env, const double* U, double t, double* G, double* PAR)
+{
+    const obme_config* cfg = env->cfg;
+    int Nz = env->Nz;
+    int nt = orc_npd_ntracers(cfg->model, cfg->carbonate_system);
+    int Pidx = (cfg->model == OBME_NPZD) ? 1 : 2;
+
+    /* update_biogeochemical_state!: light (src/OceanBioME.jl:161-167) */
+    orc_two_band_column(&cfg->par, Nz, env->zc, env->zf, U + Pidx * Nz, surface_PAR(env, t), PAR);
+
+    /* biogeochemical transitions cell by cell */
+    double vals[OBME_MAX_TRACERS], out[OBME_MAX_TRACERS];
+    for (int k = 0; k < Nz; k++) {
+        for (int n = 0; n < nt; n++) vals[n] = U[n * Nz + k];
+        if (cfg->model == OBME_NPZD && env->Tcol) vals[3] = env->Tcol[k];
+        orc_npd_tendencies(cfg->model, cfg->carbonate_system, &cfg->npd, vals, PAR[k], out);
+        for (int n = 0; n < nt; n++) G[n * Nz + k] = out[n];
+    }
+
+    /* minus the drift-velocity flux divergence */
+    double* w = (double*)malloc(sizeof(double) * (Nz + 1));
+    double* div = (double*)malloc(sizeof(double) * Nz);
+    for (int n = 0; n < nt; n++) {
+        double sp = cfg->sinking.speed[n];
+        if (sp == 0.0) continue;
+        sinking_profile(&cfg->sinking, sp, Nz, w);
+        sinking_divergence(cfg->sinking.scheme, Nz, w, U + n * Nz, env->dz, div);
+        for (int k = 0; k < Nz; k++) G[n * Nz + k] = G[n * Nz + k] - div[k];
+    }
+    free(w);
+    free(div);
+
+    /* top DIC flux boundary condition: G[Nz] -= F/Δz (GasExchange) */
+    if (cfg->model == OBME_LOBSTER && cfg->carbonate_system && cfg->co2.enabled) {
+        int top = Nz - 1;
+        double DIC = U[7 * Nz + top], Alk = U[8 * Nz + top];
+        double F = orc_co2_flux(&cfg->co2, &cfg->solver, DIC, Alk, env->T_surface, env->S_surface,
+                                cfg->co2.wind_speed, cfg->co2.air_concentration, env->fast_exit,
+                                NULL, NULL);
+        G[7 * Nz + top] = G[7 * Nz + top] - F / env->dz[top];
+    }
+}
+
+/* RungeKutta3 (Oceananigans): γ = 8/15, 5/12, 3/4; ζ = -17/60, -5/12; the
+ * substep form of BoxModel/timesteppers.jl:79-92; stage clocks t, t+8/15Δt,
+ * t+2/3Δt. */
+void orc_column_run(const orc_column_env* env, double* U, int nsteps, double dt, double t0,
+                    double* PAR_out)
+{
+    int Nz = env->Nz;
+    int nt = orc_npd_ntracers(env->cfg->model, env->cfg->carbonate_system);
+    int ncell = nt * Nz;
+    double* G = (double*)malloc(sizeof(double) * ncell);
+    double* Gm = (double*)malloc(sizeof(double) * ncell);
+    double* PAR = (double*)malloc(sizeof(double) * Nz);
+    const double g1 = 8.0 / 15.0, g2 = 5.0 / 12.0, g3 = 3.0 / 4.0;
+    const double z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
+    int readonly = (env->cfg->model == OBME_NPZD) ? 3 : -1;
+    for (int s = 0; s < nsteps; s++) {
+        double t = t0 + s * dt;
+        orc_column_tendencies(env, U, t, G, PAR);
+        for (int i = 0; i < ncell; i++) {
+            if (i / Nz == readonly) continue;
+            U[i] += dt * g1 * G[i];
+        }
+        memcpy(Gm, G, sizeof(double) * ncell);
+        orc_column_tendencies(env, U, t + g1 * dt, G, PAR);
+        for (int i = 0; i < ncell; i++) {
+            if (i / Nz == readonly) continue;
+            U[i] += dt * (g2 * G[i] + z2 * Gm[i]);
+        }
+        memcpy(Gm, G, sizeof(double) * ncell);
+        orc_column_tendencies(env, U, t + (g1 + g2 + z2) * dt, G, PAR);
+        for (int i = 0; i < ncell; i++) {
+            if (i / Nz == readonly) continue;
+            U[i] += dt * (g3 * G[i] + z3 * Gm[i]);
+        }
+    }
+    if (PAR_out) memcpy(PAR_out, PAR, sizeof(double) * Nz);
+    free(G);
+    free(Gm);
+    free(PAR);
+}
+
+/* ---------------- 3-D driver ---------------- */
+
+static long lidx(const obme_grid* g, int i, int j, int k)
+{
+    long sx = g->Nx + 2 * g->Hx, sy = g->Ny + 2 * g->Hy;
+    return (long)(i + g->Hx) + sx * ((long)(j + g->Hy) + sy * (long)(k + g->Hz));
+}
+
+static double field_value(const obme_field_source* f, const double* arr3, const obme_grid* g, int i,
+                          int j, int k)
+{
+    if (arr3) return arr3[lidx(g, i, j, k)];
+    if (f->mode == OBME_FIELD_PROFILE && f->profile) return f->profile[k];
+    return f->value;
+}
+
+static void env_for_column(const obme_config* cfg, const double* T3, const double* S3, int i, int j,
+                           double* Tcol, orc_column_env* env, int fast_exit)
+{
+    const obme_grid* g = &cfg->grid;
+    env->cfg = cfg;
+    env->Nz = g->Nz;
+    env->zc = g->zc;
+    env->zf = g->zf;
+    env->dz = g->dz;
+    env->PAR0_const = cfg->par.surface_value;
+    env->fast_exit = fast_exit;
+    env->T_surface = field_value(&cfg->temperature, T3, g, i, j, g->Nz - 1);
+    env->S_surface = field_value(&cfg->salinity, S3, g, i, j, g->Nz - 1);
+    env->Tcol = NULL;
+    (void)Tcol;
+}
+
+static int run_one_column(const obme_config* cfg, double* const* tracers, const double* T3,
+                          const double* S3, const double* PAR0, int i, int j, int nsteps, double dt,
+                          double t0, int fast_exit)
+{
+    const obme_grid* g = &cfg->grid;
+    int Nz = g->Nz;
+    int nt = orc_npd_ntracers(cfg->model, cfg->carbonate_system);
+    double* U = (double*)malloc(sizeof(double) * nt * Nz);
+    double* Tcol = (double*)malloc(sizeof(double) * Nz);
+    orc_column_env env;
+    env_for_column(cfg, T3, S3, i, j, Tcol, &env, fast_exit);
+    if (PAR0) env.PAR0_const = PAR0[i + g->Nx * j];
+    for (int n = 0; n < nt; n++)
+        for (int k = 0; k < Nz; k++) U[n * Nz + k] = tracers[n][lidx(g, i, j, k)];
+    if (cfg->model == OBME_NPZD) {
+        for (int k = 0; k < Nz; k++) Tcol[k] = tracers[3][lidx(g, i, j, k)];
+        env.Tcol = Tcol;
+    }
+    orc_column_run(&env, U, nsteps, dt, t0, NULL);
+    for (int n = 0; n < nt; n++) {
+        if (cfg->model == OBME_NPZD && n == 3) continue;
+        for (int k = 0; k < Nz; k++) tracers[n][lidx(g, i, j, k)] = U[n * Nz + k];
+    }
+    free(U);
+    free(Tcol);
+    return 0;
+}
+
+int orc_grid_run_columns(const obme_config* cfg, double* const* tracers, const double* T3,
+                         const double* S3, const double* PAR0, int nsteps, double dt, double t0,
+                         const int* col_i, const int* col_j, int ncols, int nthreads, int fast_exit)
+{
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+    for (int c = 0; c < ncols; c++)
+        run_one_column(cfg, tracers, T3, S3, PAR0, col_i[c], col_j[c], nsteps, dt, t0, fast_exit);
+    return 0;
+}
+
+int orc_grid_run(const obme_config* cfg, double* const* tracers, const double* T3, const double* S3,
+                 const double* PAR0, int nsteps, double dt, double t0, int nthreads, int fast_exit)
+{
+    const obme_grid* g = &cfg->grid;
+    long ncols = (long)g->Nx * g->Ny;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
+    for (long c = 0; c < ncols; c++)
+        run_one_column(cfg, tracers, T3, S3, PAR0, (int)(c % g->Nx), (int)(c / g->Nx), nsteps, dt, t0,
+                       fast_exit);
+    return 0;
+}
+
+/* update_biogeochemical_state! for the light model: PAR interior at time t. */
+int orc_grid_update_state(const obme_config* cfg, const double* const* tracers, const double* PAR0,
+                          double* PAR, double t)
+{
+    const obme_grid* g = &cfg->grid;
+    int Nz = g->Nz;
+    int Pidx = (cfg->model == OBME_NPZD) ? 1 : 2;
+    double* P = (double*)malloc(sizeof(double) * Nz);
+    double* col = (double*)malloc(sizeof(double) * Nz);
+    for (int j = 0; j < g->Ny; j++)
+        for (int i = 0; i < g->Nx; i++) {
+            for (int k = 0; k < Nz; k++) P[k] = tracers[Pidx][lidx(g, i, j, k)];
+            double s0 = PAR0 ? PAR0[i + g->Nx * j]
+                             : (cfg->par.surface_mode == OBME_SURFACE_DIURNAL ? orc_default_surface_PAR(t)
+                                                                              : cfg->par.surface_value);
+            orc_two_band_column(&cfg->par, Nz, g->zc, g->zf, P, s0, col);
+            for (int k = 0; k < Nz; k++) PAR[lidx(g, i, j, k)] = col[k];
+        }
+    free(P);
+    free(col);
+    return 0;
+}
+
+/* Tendencies at time t (update-state, transitions, drift, top flux) ADDED
+ * into G (the update_tendencies! hook form). PAR (interior) is written. */
+int orc_grid_tendencies(const obme_config* cfg, const double* const* tracers, double* const* G,
+                        const double* T3, const double* S3, const double* PAR0, double t, double* PAR)
+{
+    const obme_grid* g = &cfg->grid;
+    int Nz = g->Nz;
+    int nt = orc_npd_ntracers(cfg->model, cfg->carbonate_system);
+    double* U = (double*)malloc(sizeof(double) * nt * Nz);
+    double* Gc = (double*)malloc(sizeof(double) * nt * Nz);
+    double* Pc = (double*)malloc(sizeof(double) * Nz);
+    double* Tcol = (double*)malloc(sizeof(double) * Nz);
+    for (int j = 0; j < g->Ny; j++)
+        for (int i = 0; i < g->Nx; i++) {
+            orc_column_env env;
+            env_for_column(cfg, T3, S3, i, j, Tcol, &env, 1);
+            if (PAR0) env.PAR0_const = PAR0[i + g->Nx * j];
+            for (int n = 0; n < nt; n++)
+                for (int k = 0; k < Nz; k++) U[n * Nz + k] = tracers[n][lidx(g, i, j, k)];
+            if (cfg->model == OBME_NPZD) {
+                for (int k = 0; k < Nz; k++) Tcol[k] = tracers[3][lidx(g, i, j, k)];
+                env.Tcol = Tcol;
+            }
+            orc_column_tendencies(&env, U, t, Gc, Pc);
+            for (int n = 0; n < nt; n++)
+                for (int k = 0; k < Nz; k++) G[n][lidx(g, i, j, k)] += Gc[n * Nz + k];
+            if (PAR)
+                for (int k = 0; k < Nz; k++) PAR[lidx(g, i, j, k)] = Pc[k];
+        }
+    free(U);
+    free(Gc);
+    free(Pc);
+    free(Tcol);
+    return 0;
+}
+
+/* ---------------- batched NPZD ensemble ---------------- */
+
+/* Independent NPZD columns with per-column parameters: each default
+ * multiplied by a per-column factor, per-column surface PAR, uniform Δz and
+ * T.  The semantics are N independent column runs of the model above. */
+int orc_batch_run(const obme_config* cfg, int n_columns, const int* offsets, const double* const* params,
+                  double dz, double T_const, double* const* U, int nsteps, double dt, double t0,
+                  int nthreads)
+{
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
+    for (int c = 0; c < n_columns; c++) {
+        int Nz = offsets[c + 1] - offsets[c];
+        obme_config local = *cfg;
+        obme_npd_params* q = &local.npd;
+        const double* const* P = params;
+        q->nitrate_half_saturation = P[OBME_BP_NITRATE_HALF_SATURATION][c];
+        q->light_half_saturation = P[OBME_BP_LIGHT_HALF_SATURATION][c];
+        q->phytoplankton_maximum_growth_rate = P[OBME_BP_MAX_GROWTH_RATE][c];
+        q->phytoplankton_exudation_fraction = P[OBME_BP_EXUDATION_FRACTION][c];
+        q->ammonia_fraction_of_exudate = P[OBME_BP_AMMONIA_FRACTION_OF_EXUDATE][c];
+        q->temperature_coefficient = P[OBME_BP_TEMPERATURE_COEFFICIENT][c];
+        q->phytoplankton_mortality_rate = P[OBME_BP_PHYTO_MORTALITY_RATE][c];
+        q->zooplankton_mortality_rate = P[OBME_BP_ZOO_MORTALITY_RATE][c];
+        q->zooplankton_excretion_rate = P[OBME_BP_ZOO_EXCRETION_RATE][c];
+        q->phytoplankton_solid_waste_fraction = P[OBME_BP_PHYTO_SOLID_WASTE_FRACTION][c];
+        q->excretion_inorganic_fraction = P[OBME_BP_EXCRETION_INORGANIC_FRACTION][c];
+        q->preference_for_phytoplankton = P[OBME_BP_PREFERENCE_FOR_PHYTO][c];
+        q->maximum_grazing_rate = P[OBME_BP_MAX_GRAZING_RATE][c];
+        q->grazing_half_saturation = P[OBME_BP_GRAZING_HALF_SATURATION][c];
+        q->zooplankton_assimilation_fraction = P[OBME_BP_ZOO_ASSIMILATION_FRACTION][c];
+        q->remineralisation_rate = P[OBME_BP_REMINERALISATION_RATE][c];
+        q->small_particle_fraction = P[OBME_BP_SMALL_PARTICLE_FRACTION][c];
+        memset(local.sinking.speed, 0, sizeof(local.sinking.speed));
+        local.sinking.speed[1] = P[OBME_BP_P_SINKING_SPEED][c];
+        local.sinking.speed[4] = P[OBME_BP_D_SINKING_SPEED][c];
+        local.par.surface_mode = OBME_SURFACE_CONSTANT;
+
+        double* zc = (double*)malloc(sizeof(double) * Nz);
+        double* zf = (double*)malloc(sizeof(double) * (Nz + 1));
+        double* dzs = (double*)malloc(sizeof(double) * Nz);
+        double* Tcol = (double*)malloc(sizeof(double) * Nz);
+        double* col = (double*)malloc(sizeof(double) * 5 * Nz);
+        for (int k = 0; k <= Nz; k++) zf[k] = -dz * (Nz - k);
+        for (int k = 0; k < Nz; k++) {
+            zc[k] = -dz * (Nz - k) + dz / 2;
+            dzs[k] = dz;
+            Tcol[k] = T_const;
+        }
+        orc_column_env env;
+        env.cfg = &local;
+        env.Nz = Nz;
+        env.zc = zc;
+        env.zf = zf;
+        env.dz = dzs;
+        env.PAR0_const = P[OBME_BP_SURFACE_PAR][c];
+        env.T_surface = T_const;
+        env.S_surface = 35.0;
+        env.Tcol = Tcol;
+        env.fast_exit = 1;
+        /* tracer order N, P, Z, T, D */
+        const int map[5] = {0, 1, 2, -1, 3};
+        for (int n = 0; n < 5; n++)
+            for (int k = 0; k < Nz; k++)
+                col[n * Nz + k] = map[n] < 0 ? T_const : U[map[n]][offsets[c] + k];
+        orc_column_run(&env, col, nsteps, dt, t0, NULL);
+        for (int n = 0; n < 5; n++)
+            if (map[n] >= 0)
+                for (int k = 0; k < Nz; k++) U[map[n]][offsets[c] + k] = col[n * Nz + k];
+        free(zc);
+        free(zf);
+        free(dzs);
+        free(Tcol);
+        free(col);
+    }
+    return 0;
+}

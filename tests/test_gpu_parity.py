@@ -1,0 +1,116 @@
+"""GPU parity: the HIP library through its C ABI against the FP64 oracle on
+the same inputs.  FP64 bar: relative L2 <= 1e-12 per tracer (reassociation
+only).  FP32-mixed bar: relative L2 <= 1e-5 per tracer."""
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import rel_l2, to_device, to_host
+from oceanbiome import abi, workloads
+from oceanbiome.runtime import HIPBiogeochemistry
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-5
+
+
+def _run_both(wl, precision, nsteps=None):
+    import torch
+    nsteps = wl.nsteps if nsteps is None else nsteps
+    ref = [a.copy() for a in wl.tracers]
+    cfg = wl.bgc.config(abi.OBME_FP64)
+    O.grid_run(cfg, ref, nsteps, wl.dt)
+    dev = HIPBiogeochemistry(wl.bgc, precision)
+    tr = to_device(wl.tracers, dev.dtype)
+    dev.step(tr, nsteps, wl.dt)
+    torch.cuda.synchronize()
+    out = to_host(tr)
+    g = wl.grid
+    errs = {}
+    for name, a, b in zip(wl.bgc.required_biogeochemical_tracers(), out, ref):
+        errs[name] = rel_l2(g.interior(a), g.interior(b))
+    return errs
+
+
+def test_npzd_2d_fp64():
+    print("case npzd_2d fp64", flush=True)
+    errs = _run_both(workloads.npzd_2d(), "fp64")
+    assert all(e <= FP64_TOL for e in errs.values()), errs
+
+
+def test_npzd_2d_fp32_mixed():
+    print("case npzd_2d fp32", flush=True)
+    errs = _run_both(workloads.npzd_2d(), "fp32")
+    assert all(e <= FP32_TOL for e in errs.values()), errs
+
+
+@pytest.mark.parametrize("shape", [(32, 8, 64), (45, 3, 37), (20, 2, 7)])
+def test_lobster_fp64(shape):
+    print("case lobster fp64", shape, flush=True)
+    wl = workloads.lobster_3d(*shape, nsteps=10)
+    errs = _run_both(wl, "fp64")
+    assert all(e <= FP64_TOL for e in errs.values()), errs
+
+
+@pytest.mark.parametrize("shape", [(32, 8, 64), (45, 3, 37)])
+def test_lobster_fp32_mixed(shape):
+    print("case lobster fp32", shape, flush=True)
+    wl = workloads.lobster_3d(*shape, nsteps=20)
+    errs = _run_both(wl, "fp32")
+    assert all(e <= FP32_TOL for e in errs.values()), errs
+
+
+def test_update_state_par_fp64():
+    import torch
+    print("case update_state fp64", flush=True)
+    wl = workloads.lobster_3d(24, 4, 50, nsteps=1)
+    g = wl.grid
+    PAR_ref = g.new_field()
+    O.grid_update_state(wl.bgc.config(abi.OBME_FP64), wl.tracers, PAR_ref, 0.0)
+    dev = HIPBiogeochemistry(wl.bgc, "fp64")
+    tr = to_device(wl.tracers, torch.float64)
+    PAR = torch.zeros(g.parent_shape, dtype=torch.float64, device="cuda")
+    dev.update_biogeochemical_state(tr, PAR, 0.0)
+    torch.cuda.synchronize()
+    assert rel_l2(g.interior(PAR.cpu().numpy()), g.interior(PAR_ref)) <= 1e-13
+
+
+def test_add_tendencies_fp64():
+    import torch
+    print("case add_tendencies fp64", flush=True)
+    wl = workloads.lobster_3d(24, 4, 50, nsteps=1)
+    g = wl.grid
+    rng = np.random.default_rng(3)
+    G0 = [rng.standard_normal(g.parent_shape) * 1e-6 for _ in wl.tracers]
+    Gref = [a.copy() for a in G0]
+    PAR_ref = g.new_field()
+    O.grid_tendencies(wl.bgc.config(abi.OBME_FP64), wl.tracers, Gref, 0.0, PAR=PAR_ref)
+    dev = HIPBiogeochemistry(wl.bgc, "fp64")
+    tr = to_device(wl.tracers, torch.float64)
+    G = to_device(G0, torch.float64)
+    PAR = torch.zeros(g.parent_shape, dtype=torch.float64, device="cuda")
+    dev.update_tendencies(tr, G, PAR, t=0.0)
+    torch.cuda.synchronize()
+    for name, a, b, a0 in zip(wl.bgc.required_biogeochemical_tracers(), to_host(G), Gref, G0):
+        assert rel_l2(g.interior(a - a0), g.interior(b - a0)) <= 1e-12, name
+    assert rel_l2(g.interior(PAR.cpu().numpy()), g.interior(PAR_ref)) <= 1e-13
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_co2_flux_points(mode):
+    import torch
+    print("case co2_flux mode", mode, flush=True)
+    pts = workloads.carbonate_points(4096, seed=7)
+    wl = workloads.lobster_3d(4, 4, 4, nsteps=1)
+    dev = HIPBiogeochemistry(wl.bgc, "fp64")
+    cfg = wl.bgc.config(abi.OBME_FP64)
+    pH_ref, F_ref = O.co2_flux_batch(cfg.co2, cfg.solver, pts["DIC"], pts["Alk"], pts["T"], pts["S"],
+                                     pts["u10"], pts["pCO2_air"])
+    args = [torch.from_numpy(pts[k]).cuda() for k in ("DIC", "Alk", "T", "S", "u10")]
+    pH, F = dev.co2_flux(*args, pCO2_air=pts["pCO2_air"], mode=mode)
+    torch.cuda.synchronize()
+    pH = pH.cpu().numpy()
+    F = F.cpu().numpy()
+    assert np.max(np.abs(pH - pH_ref)) <= 1e-12
+    assert rel_l2(F, F_ref) <= 1e-10
