@@ -8,6 +8,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -38,11 +39,14 @@ extern "C" const char* obme_version(void) { return "obme 0.1.0 gfx950"; }
 
 // ------------------------------------------------------------------ kernels
 
-template <int MODEL, bool CARB, typename FT, int LPT, int MODE, int MAXT>
-__global__ void __launch_bounds__(MAXT) k_column_tile(StepArgs<FT> a)
+// The sweep: one workgroup of NW waves, wave-per-column segments
+// (obme_npd_kernels.h).  __launch_bounds__(NW*64) lets the compiler use
+// 512/(NW/4) VGPRs per lane at most (128 for 16 waves, 256 for 8).
+template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM>
+__global__ void __launch_bounds__(NW * 64) k_column_wave(StepArgs<FT> a)
 {
     extern __shared__ unsigned char smem_raw[];
-    column_tile<MODEL, CARB, FT, LPT, MODE>(a, reinterpret_cast<FT*>(smem_raw));
+    column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, UNIFORM>(a, reinterpret_cast<FT*>(smem_raw));
 }
 
 // CO2 flux and pH on n points (SoA, FP64).
@@ -81,17 +85,12 @@ struct obme_handle {
     void* d_wsink = nullptr;   // MAX_SINK * (Nz+1) of FT
     double* d_Tprof = nullptr;
     double* d_Sprof = nullptr;
+    void* d_srec = nullptr;    // SURF_REC * Nx*Ny of FT (CO2 exchange on)
+    double* d_Hprev = nullptr; // Nx*Ny warm starts of [H+] (mixed path)
     int sink_idx[MAX_SINK];
-    int tx, kg, lpt;
-    // recorded multi-step graph
-    hipGraphExec_t graph = nullptr;
-    int g_nsteps = -1;
-    double g_dt = 0, g_t0 = 0;
-    void* g_stream = nullptr;
-    void* const* g_tr = nullptr;
-    std::vector<void*> g_trv;
-    const void* g_T = nullptr;
-    const void* g_S = nullptr;
+    int cfg_id;                // sweep tile configuration (TileCfg table)
+    bool uniform;
+    double dz_u, zc0_u, w0_u, w1_u;
 };
 
 static int ntracers_of(const obme_config& c)
@@ -132,7 +131,51 @@ static obme_status upload_geometry(obme_handle* h)
     }
     HIP_TRY(hipMalloc(&h->d_wsink, w.size() * sizeof(FT)));
     HIP_TRY(hipMemcpy(h->d_wsink, w.data(), w.size() * sizeof(FT), hipMemcpyHostToDevice));
+
+    // Uniform-grid fast path: the kernel forms zc = zc[0] + k dz, the half
+    // cells dz/2, 1/dz and constant face velocities itself.  Taken only when
+    // those reproduce every per-level value exactly in the arithmetic type
+    // (with or without contraction into an FMA).
+    const FT dzf = (FT)h->dz[0];
+    const FT zc0 = lev[LEV_ZC * Nz + 0];
+    bool uni = sk.open_bottom != 0;
+    for (int k = 0; k < Nz && uni; k++) {
+        FT zk = zc0 + (FT)k * dzf;
+        FT zk_fma = (FT)std::fma((FT)k, dzf, zc0);
+        uni = uni && (zk == lev[LEV_ZC * Nz + k]) && (zk_fma == zk);
+        uni = uni && (lev[LEV_DN * Nz + k] == dzf / (FT)2);
+        uni = uni && (lev[LEV_UP * Nz + k] == ((k < Nz - 1) ? dzf / (FT)2 : (FT)0));
+        uni = uni && (lev[LEV_INVDZ * Nz + k] == (FT)1 / dzf);
+    }
+    h->uniform = uni;
+    h->dz_u = (double)dzf;
+    h->zc0_u = (double)zc0;
+    h->w0_u = (double)w[0];
+    h->w1_u = (double)w[(size_t)(Nz + 1)];
     return OBME_OK;
+}
+
+// Sweep tile configurations: L levels per lane, CPW columns per wave, NW
+// waves per workgroup (TXB = NW * CPW columns, Nz <= L * 64 / CPW).
+struct TileCfg {
+    int L, CPW, NW;
+};
+static const TileCfg TILE_CFGS[] = {
+    {2, 2, 16}, {4, 4, 8}, {2, 2, 8}, {1, 1, 16}, {2, 1, 16}, {4, 2, 8}, {1, 1, 8}, {2, 1, 8},
+};
+static const int N_TILE_CFG = sizeof(TILE_CFGS) / sizeof(TILE_CFGS[0]);
+
+static int choose_tile(int prec, int Nz)
+{
+    // preference lists by precision (indices into TILE_CFGS)
+    static const int pref32[] = {0, 4, 5};
+    static const int pref64[] = {3, 4, 7};
+    const int* p = (prec == OBME_FP64) ? pref64 : pref32;
+    for (int q = 0; q < 3; q++) {
+        const TileCfg& t = TILE_CFGS[p[q]];
+        if (Nz <= t.L * 64 / t.CPW) return p[q];
+    }
+    return -1;
 }
 
 extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
@@ -148,6 +191,12 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
     if (g.Nx < 1 || g.Ny < 1 || g.Nz < 1 || g.Hx < 0 || g.Hy < 0 || g.Hz < 0)
         return fail(OBME_ERR_INVALID, "invalid grid size or halo");
     if (g.Nz > 128) return fail(OBME_ERR_UNSUPPORTED, "Nz > 128 is not supported by the column-tile kernels");
+    {
+        size_t elems = (size_t)(g.Nx + 2 * g.Hx) * (g.Ny + 2 * g.Hy) * (g.Nz + 2 * g.Hz);
+        size_t es = (cfg->precision == OBME_FP64) ? 8 : 4;
+        if (elems * es >= ((size_t)1 << 31))
+            return fail(OBME_ERR_UNSUPPORTED, "a field of 2 GiB or more exceeds the 32-bit buffer offsets");
+    }
     if (!g.zc || !g.zf || !g.dz) return fail(OBME_ERR_INVALID, "grid coordinates missing");
     if (cfg->sinking.scheme != OBME_UPWIND1)
         return fail(OBME_ERR_UNSUPPORTED, "only the first-order upwind sinking scheme runs on the GPU");
@@ -194,15 +243,36 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
         obme_destroy(h);
         return st;
     }
-    // launch shape: LPT levels per thread, KG = ceil(Nz / LPT) level groups,
-    // TX columns per workgroup with TX * KG <= the kernel's bound
-    h->lpt = 2;
-    h->kg = (g.Nz + h->lpt - 1) / h->lpt;
-    int maxt = (h->prec == OBME_FP64) ? 512 : 1024;
-    if (h->kg > maxt) { obme_destroy(h); return fail(OBME_ERR_UNSUPPORTED, "too many levels"); }
-    int tx = 64;
-    while (tx * h->kg > maxt) tx >>= 1;
-    h->tx = tx;
+    // sweep tile configuration: the first of the precision's preference list
+    // whose column fits (Nz <= L * 64 / CPW); OBME_TILE="L,CPW,NW" overrides.
+    h->cfg_id = choose_tile(h->prec, g.Nz);
+    if (const char* e = std::getenv("OBME_TILE")) {
+        int l = 0, c = 0, w = 0;
+        if (std::sscanf(e, "%d,%d,%d", &l, &c, &w) == 3) {
+            for (int t = 0; t < N_TILE_CFG; t++)
+                if (TILE_CFGS[t].L == l && TILE_CFGS[t].CPW == c && TILE_CFGS[t].NW == w &&
+                    g.Nz <= l * 64 / c)
+                    h->cfg_id = t;
+        }
+    }
+    if (h->cfg_id < 0) {
+        obme_destroy(h);
+        return fail(OBME_ERR_UNSUPPORTED, "no tile configuration for this Nz");
+    }
+    if (cfg->model == OBME_LOBSTER && cfg->carbonate_system && cfg->co2.enabled) {
+        size_t ncol = (size_t)g.Nx * g.Ny;
+        size_t es = (h->prec == OBME_FP64) ? 8 : 4;
+        hipError_t e1 = hipMalloc(&h->d_srec, SURF_REC * ncol * es);
+        hipError_t e2 = hipMalloc((void**)&h->d_Hprev, ncol * sizeof(double));
+        if (e1 != hipSuccess || e2 != hipSuccess) {
+            obme_destroy(h);
+            return fail(OBME_ERR_NOMEM, "surface buffers");
+        }
+        if (hipMemset(h->d_Hprev, 0, ncol * sizeof(double)) != hipSuccess) {
+            obme_destroy(h);
+            return fail(OBME_ERR_HIP, "hipMemset");
+        }
+    }
     *out = h;
     return OBME_OK;
 }
@@ -210,11 +280,12 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
 extern "C" obme_status obme_destroy(obme_handle* h)
 {
     if (!h) return OBME_OK;
-    if (h->graph) hipGraphExecDestroy(h->graph);
     if (h->d_lev) (void)hipFree(h->d_lev);
     if (h->d_wsink) (void)hipFree(h->d_wsink);
     if (h->d_Tprof) (void)hipFree(h->d_Tprof);
     if (h->d_Sprof) (void)hipFree(h->d_Sprof);
+    if (h->d_srec) (void)hipFree(h->d_srec);
+    if (h->d_Hprev) (void)hipFree(h->d_Hprev);
     delete h;
     return OBME_OK;
 }
@@ -234,7 +305,7 @@ extern "C" int32_t obme_required_tracers(const obme_handle* h, char* buf, int32_
 }
 
 template <typename FT>
-static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers, const void* T, const void* S)
+static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers)
 {
     const obme_config& c = h->cfg;
     const obme_grid& g = c.grid;
@@ -244,16 +315,12 @@ static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers, const 
     a.lev = (const FT*)h->d_lev;
     a.wsink = (const FT*)h->d_wsink;
     a.surf = (const FT*)c.par.surface_array;
-    a.T3 = (const FT*)T;
-    a.S3 = (const FT*)S;
-    a.Tprof = (c.temperature.mode == OBME_FIELD_PROFILE) ? h->d_Tprof : nullptr;
-    a.Sprof = (c.salinity.mode == OBME_FIELD_PROFILE) ? h->d_Sprof : nullptr;
-    a.Tconst = c.temperature.value;
-    a.Sconst = c.salinity.value;
-    a.Nx = g.Nx; a.Ny = g.Ny; a.Nz = g.Nz; a.Hx = g.Hx; a.Hy = g.Hy; a.Hz = g.Hz;
-    a.sx = g.Nx + 2 * g.Hx;
-    a.sxy = a.sx * (g.Ny + 2 * g.Hy);
-    a.base0 = (long)g.Hx + a.sx * g.Hy + a.sxy * g.Hz;
+    a.srec = (FT*)h->d_srec;
+    a.Nx = g.Nx; a.Ny = g.Ny; a.Nz = g.Nz;
+    a.sx = (unsigned)(g.Nx + 2 * g.Hx);
+    a.sxy = a.sx * (unsigned)(g.Ny + 2 * g.Hy);
+    a.base0 = (unsigned)g.Hx + a.sx * (unsigned)g.Hy + a.sxy * (unsigned)g.Hz;
+    a.field_bytes = (unsigned)((size_t)a.sxy * (size_t)(g.Nz + 2 * g.Hz) * sizeof(FT));
     a.q = convert_params<FT>(c.npd);
     a.kr = (FT)c.par.water_red_attenuation;
     a.kb = (FT)c.par.water_blue_attenuation;
@@ -266,50 +333,154 @@ static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers, const 
     a.Rr = (FT)(c.par.phytoplankton_chlorophyll_ratio / c.par.pigment_ratio);
     a.surface_mode = c.par.surface_mode;
     a.surface_value = (FT)c.par.surface_value;
+    a.dz_u = (FT)h->dz_u;
+    a.zc0_u = (FT)h->zc0_u;
+    a.w0_u = (FT)h->w0_u;
+    a.w1_u = (FT)h->w1_u;
+    a.invdz_u = (FT)1 / (FT)h->dz_u;
     a.co2_on = (c.model == OBME_LOBSTER && c.carbonate_system && c.co2.enabled) ? 1 : 0;
-    a.co2_mode = (h->prec == OBME_FP64) ? 0 : 1;
-    a.co2 = c.co2;
-    a.solver = c.solver;
+    if (sizeof(FT) == 4) {
+        a.f = make_fast_params(c.npd);
+        const double log2e = 1.4426950408889634;
+        a.kr2 = (float)(c.par.water_red_attenuation * log2e);
+        a.kb2 = (float)(c.par.water_blue_attenuation * log2e);
+        a.xr2 = (float)(c.par.chlorophyll_red_attenuation * log2e);
+        a.xb2 = (float)(c.par.chlorophyll_blue_attenuation * log2e);
+    }
     return a;
 }
 
-template <int MODEL, bool CARB, typename FT, int LPT, int MODE, int MAXT>
-static obme_status launch_tile(const obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
+// RK3 coefficients Δt γ_s and Δt ζ_s (BoxModel/timesteppers.jl:79-92) for the FP32 update
+static void rk3_products(double dt, float* A, float* B)
 {
-    dim3 block(h->tx, h->kg);
-    dim3 grid((h->cfg.grid.Nx + h->tx - 1) / h->tx, h->cfg.grid.Ny);
-    size_t lds = (size_t)(4 + MAX_SINK) * h->tx * h->kg * sizeof(FT) + (size_t)h->tx * sizeof(SurfaceCarbon);
-    hipLaunchKernelGGL((k_column_tile<MODEL, CARB, FT, LPT, MODE, MAXT>), grid, block, lds, s, a);
+    A[0] = (float)(dt * (8.0 / 15.0));
+    A[1] = (float)(dt * (5.0 / 12.0));
+    A[2] = (float)(dt * (3.0 / 4.0));
+    B[0] = 0.0f;
+    B[1] = (float)(dt * (-17.0 / 60.0));
+    B[2] = (float)(dt * (-5.0 / 12.0));
+}
+
+template <typename FT>
+static SurfaceArgs<FT> make_surface_args(const obme_handle* h, void* const* tracers, const void* T, const void* S)
+{
+    const obme_config& c = h->cfg;
+    const obme_grid& g = c.grid;
+    SurfaceArgs<FT> s;
+    std::memset(&s, 0, sizeof(s));
+    s.DIC = (FT*)tracers[7];
+    s.Alk = (const FT*)tracers[8];
+    s.srec = (const FT*)h->d_srec;
+    s.Hprev = (h->prec == OBME_FP64) ? nullptr : h->d_Hprev;
+    s.T3 = (const FT*)T;
+    s.S3 = (const FT*)S;
+    s.Tprof = (c.temperature.mode == OBME_FIELD_PROFILE) ? h->d_Tprof : nullptr;
+    s.Sprof = (c.salinity.mode == OBME_FIELD_PROFILE) ? h->d_Sprof : nullptr;
+    s.Tconst = c.temperature.value;
+    s.Sconst = c.salinity.value;
+    s.Nx = g.Nx; s.Ny = g.Ny; s.Nz = g.Nz;
+    s.sx = (unsigned)(g.Nx + 2 * g.Hx);
+    s.sxy = s.sx * (unsigned)(g.Ny + 2 * g.Hy);
+    s.base0 = (unsigned)g.Hx + s.sx * (unsigned)g.Hy + s.sxy * (unsigned)g.Hz;
+    s.invdz_top = (FT)(1.0 / h->dz[g.Nz - 1]);
+    s.mode = (h->prec == OBME_FP64) ? 0 : 1;
+    s.co2 = c.co2;
+    s.solver = c.solver;
+    return s;
+}
+
+template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE>
+static obme_status launch_wave(const obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
+{
+    constexpr int NT = ModelTraits<MODEL, CARB>::NT;
+    constexpr int TXB = NW * CPW;
+    const int ntiles = (MODE == TILE_PAR) ? 1 : (MODE == TILE_TENDENCY ? NT + 1 : NT);
+    const size_t lds = (size_t)ntiles * h->cfg.grid.Nz * (TXB + 1) * sizeof(FT);
+    if (lds > 160 * 1024) return fail(OBME_ERR_UNSUPPORTED, "sweep tile exceeds the 160 KiB LDS");
+    dim3 block(NW * 64);
+    dim3 grid((h->cfg.grid.Nx + TXB - 1) / TXB, h->cfg.grid.Ny);
+    if (h->uniform) {
+        auto k = k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true>;
+        HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k, grid, block, lds, s, a);
+    } else {
+        auto k = k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false>;
+        HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k, grid, block, lds, s, a);
+    }
     HIP_TRY(hipGetLastError());
     return OBME_OK;
 }
 
-template <typename FT, int LPT, int MODE, int MAXT>
-static obme_status dispatch_tile(const obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
+template <typename FT, int L, int CPW, int NW, int MODE>
+static obme_status dispatch_model(const obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
 {
-    if (h->cfg.model == OBME_NPZD) return launch_tile<OBME_NPZD, false, FT, LPT, MODE, MAXT>(h, a, s);
-    if (h->cfg.carbonate_system) return launch_tile<OBME_LOBSTER, true, FT, LPT, MODE, MAXT>(h, a, s);
-    return launch_tile<OBME_LOBSTER, false, FT, LPT, MODE, MAXT>(h, a, s);
+    if (h->cfg.model == OBME_NPZD) return launch_wave<OBME_NPZD, false, FT, L, CPW, NW, MODE>(h, a, s);
+    if (h->cfg.carbonate_system) return launch_wave<OBME_LOBSTER, true, FT, L, CPW, NW, MODE>(h, a, s);
+    return launch_wave<OBME_LOBSTER, false, FT, L, CPW, NW, MODE>(h, a, s);
+}
+
+template <typename FT, int MODE>
+static obme_status dispatch_tile_cfg(const obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
+{
+    switch (h->cfg_id) {
+    case 0: return dispatch_model<FT, 2, 2, 16, MODE>(h, a, s);
+    case 1: return dispatch_model<FT, 4, 4, 8, MODE>(h, a, s);
+    case 2: return dispatch_model<FT, 2, 2, 8, MODE>(h, a, s);
+    case 3: return dispatch_model<FT, 1, 1, 16, MODE>(h, a, s);
+    case 4: return dispatch_model<FT, 2, 1, 16, MODE>(h, a, s);
+    case 5: return dispatch_model<FT, 4, 2, 8, MODE>(h, a, s);
+    case 6: return dispatch_model<FT, 1, 1, 8, MODE>(h, a, s);
+    case 7: return dispatch_model<FT, 2, 1, 8, MODE>(h, a, s);
+    default: return fail(OBME_ERR_INVALID, "bad tile configuration");
+    }
+}
+
+template <typename FT, int MODE>
+static obme_status launch_surface(const obme_handle* h, const SurfaceArgs<FT>& sa, hipStream_t s)
+{
+    const int bs = 256;
+    const long ncol = (long)sa.Nx * sa.Ny;
+    hipLaunchKernelGGL((k_surface<FT, MODE>), dim3((ncol + bs - 1) / bs), dim3(bs), 0, s, sa);
+    HIP_TRY(hipGetLastError());
+    return OBME_OK;
+}
+
+template <typename FT, int MODE>
+static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* G, void* PAR, const void* T,
+                             const void* S, double t, double dt, hipStream_t s)
+{
+    StepArgs<FT> a = make_args<FT>(h, tracers);
+    if (G) for (int n = 0; n < h->ntracers; n++) a.G[n] = (FT*)G[n];
+    a.PAR = (FT*)PAR;
+    a.dtf = (FT)dt;
+    rk3_products(dt, a.rk_a, a.rk_b);
+    if (h->cfg.par.surface_mode == OBME_SURFACE_DIURNAL) {
+        // default_surface_PAR (Light/Light.jl:219) at the stage clocks
+        const double cst[3] = {0.0, 8.0 / 15.0, 8.0 / 15.0 + 5.0 / 12.0 - 17.0 / 60.0};
+        for (int q = 0; q < 3; q++) {
+            double tq = t + cst[q] * dt;
+            double cv = std::cos(tq * 3.141592653589793 / (12 * 3600.0));
+            double d = 0.0 - cv;  // Julia max(0, cos): NaN propagates, ±0 ordered
+            double v = std::isnan(cv) ? d : (std::signbit(d) ? cv : 0.0);
+            a.PAR0_stage[q] = (FT)(100 * v);
+        }
+    }
+    obme_status r = dispatch_tile_cfg<FT, MODE>(h, a, s);
+    if (r != OBME_OK || MODE == TILE_PAR || !a.co2_on) return r;
+    SurfaceArgs<FT> sa = make_surface_args<FT>(h, tracers, T, S);
+    sa.dtf = (FT)dt;
+    rk3_products(dt, sa.rk_a, sa.rk_b);
+    if (MODE == TILE_TENDENCY) sa.GDIC = (FT*)G[7];
+    return launch_surface<FT, MODE>(h, sa, s);
 }
 
 template <int MODE>
 static obme_status run_tile(obme_handle* h, void* const* tracers, void* const* G, void* PAR, const void* T,
                             const void* S, double t, double dt, hipStream_t s)
 {
-    if (h->prec == OBME_FP64) {
-        StepArgs<double> a = make_args<double>(h, tracers, T, S);
-        if (G) for (int n = 0; n < h->ntracers; n++) a.G[n] = (double*)G[n];
-        a.PAR = (double*)PAR;
-        a.t0 = t;
-        a.dt = dt;
-        return dispatch_tile<double, 2, MODE, 512>(h, a, s);
-    }
-    StepArgs<float> a = make_args<float>(h, tracers, T, S);
-    if (G) for (int n = 0; n < h->ntracers; n++) a.G[n] = (float*)G[n];
-    a.PAR = (float*)PAR;
-    a.t0 = t;
-    a.dt = dt;
-    return dispatch_tile<float, 2, MODE, 1024>(h, a, s);
+    if (h->prec == OBME_FP64) return run_typed<double, MODE>(h, tracers, G, PAR, T, S, t, dt, s);
+    return run_typed<float, MODE>(h, tracers, G, PAR, T, S, t, dt, s);
 }
 
 extern "C" obme_status obme_update_state(obme_handle* h, const void* const* tracers, void* PAR, double t,

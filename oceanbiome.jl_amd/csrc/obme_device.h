@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 #include "../../include/obme.h"
 
 namespace obme {
@@ -74,6 +76,35 @@ __device__ __forceinline__ double m_pow(double x, double y) { return pow(x, y); 
 // x^y for x >= 0 as 2^(y log2 x); a negative x gives NaN, as on the GPU
 // reference path.
 __device__ __forceinline__ float m_pow(float x, float y) { return m_exp2(y * m_log2(x)); }
+
+// ---------------------------------------------------------------- buffer I/O
+
+// Raw buffer access: a 128-bit descriptor in SGPRs per array plus a 32-bit
+// byte offset per lane, shared by every array of the same layout.  Offsets
+// at or beyond `bytes` read 0 and drop stores.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ void buf_load(rsrc_t r, unsigned off, float& v)
+{
+    v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void buf_load(rsrc_t r, unsigned off, double& v)
+{
+    v = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ void buf_store(rsrc_t r, unsigned off, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void buf_store(rsrc_t r, unsigned off, double v)
+{
+    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, 0);
+}
 
 // ---------------------------------------------------------------- parameters
 
@@ -227,6 +258,179 @@ __device__ __forceinline__ void lobster_tendencies(const NPDParams<FT>& q, const
         out[7] = -pp + picw + dicw + cdis;                                              // carbonate_system.jl:50-55
         out[8] = dNH4 * FT(0.9375) - dNO3 * FT(1.0625) - FT(2) * cup + FT(2) * cdis;    // :57-62
     }
+}
+
+// ---------------------------------------------------------------- FP32 fast tendencies
+
+// Derived constants of the FP32-mixed path, formed once on the host in
+// FP64 and rounded: the vector ALUs of gfx950 do all float arithmetic
+// (there is no scalar float ALU), so uniform products are precomputed.
+struct FastParams {
+    float nPSI2;       // -psi * log2(e): exp(-psi NH4) = 2^(nPSI2 NH4)
+    float kNO3, kNH4, kPAR, kPAR2;
+    float mu0, mu0h;   // mu0, mu0/2 (LOBSTER's (L_NO3 + L_NH4)/2)
+    float pref, prefc; // pref, 1 - pref
+    float kG, kG2, g;
+    float mP, mZ, muZ;
+    float one_m_gamma, axg;
+    float nitrif;
+    float pinw_z;      // aZ_exc * muZ
+    float one_m_aPs, aPs;
+    float s_rem, b_rem, d_rem, a_rem, one_m_arem;
+    float ponw_mu;     // (1 - alpha_x) gamma
+    float ponw_z;      // (1 - aZ_exc) muZ
+    float one_m_aZass, aZass;
+    float ss, one_m_ss;
+    float ppc;         // (1 + rho (1-gamma) - alpha_x gamma) R
+    float R, Rd, rhoR2, rhoR4;  // R, Rd, 2 rho R, 4 rho R
+    float rem, small_frac, uptake_N;  // Detritus rate, fraction, (1 - alpha_x gamma)
+    float log2Q10_10;  // log2(Q10)/10: Q10^(T/10) = 2^(T log2Q10_10)
+    int light_form, use_Q10, mort_form, graz_form;
+};
+
+inline FastParams make_fast_params(const obme_npd_params& p)
+{
+    FastParams f;
+    const double LOG2E = 1.4426950408889634;
+    f.nPSI2 = (float)(-p.nitrate_ammonia_inhibition * LOG2E);
+    f.kNO3 = (float)p.nitrate_half_saturation;
+    f.kNH4 = (float)p.ammonia_half_saturation;
+    f.kPAR = (float)p.light_half_saturation;
+    f.kPAR2 = (float)(p.light_half_saturation * p.light_half_saturation);
+    f.mu0 = (float)p.phytoplankton_maximum_growth_rate;
+    f.mu0h = (float)(p.phytoplankton_maximum_growth_rate / 2);
+    f.pref = (float)p.preference_for_phytoplankton;
+    f.prefc = (float)(1 - p.preference_for_phytoplankton);
+    f.kG = (float)p.grazing_half_saturation;
+    f.kG2 = (float)(p.grazing_half_saturation * p.grazing_half_saturation);
+    f.g = (float)p.maximum_grazing_rate;
+    f.mP = (float)p.phytoplankton_mortality_rate;
+    f.mZ = (float)p.zooplankton_mortality_rate;
+    f.muZ = (float)p.zooplankton_excretion_rate;
+    f.one_m_gamma = (float)(1 - p.phytoplankton_exudation_fraction);
+    f.axg = (float)(p.ammonia_fraction_of_exudate * p.phytoplankton_exudation_fraction);
+    f.nitrif = (float)p.nitrification_rate;
+    f.pinw_z = (float)(p.excretion_inorganic_fraction * p.zooplankton_excretion_rate);
+    f.one_m_aPs = (float)(1 - p.phytoplankton_solid_waste_fraction);
+    f.aPs = (float)p.phytoplankton_solid_waste_fraction;
+    f.s_rem = (float)p.small_remineralisation_rate;
+    f.b_rem = (float)p.large_remineralisation_rate;
+    f.d_rem = (float)p.dissolved_remineralisation_rate;
+    f.a_rem = (float)p.remineralisation_inorganic_fraction;
+    f.one_m_arem = (float)(1 - p.remineralisation_inorganic_fraction);
+    f.ponw_mu = (float)((1 - p.ammonia_fraction_of_exudate) * p.phytoplankton_exudation_fraction);
+    f.ponw_z = (float)((1 - p.excretion_inorganic_fraction) * p.zooplankton_excretion_rate);
+    f.one_m_aZass = (float)(1 - p.zooplankton_assimilation_fraction);
+    f.aZass = (float)p.zooplankton_assimilation_fraction;
+    f.ss = (float)p.small_solid_waste_fraction;
+    f.one_m_ss = (float)(1 - p.small_solid_waste_fraction);
+    f.ppc = (float)((1 + p.carbon_calcite_ratio * (1 - p.phytoplankton_exudation_fraction)
+                     - p.ammonia_fraction_of_exudate * p.phytoplankton_exudation_fraction) * p.redfield_ratio);
+    f.R = (float)p.redfield_ratio;
+    f.Rd = (float)p.detritus_redfield_ratio;
+    f.rhoR2 = (float)(2 * p.carbon_calcite_ratio * p.redfield_ratio);
+    f.rhoR4 = (float)(4 * p.carbon_calcite_ratio * p.redfield_ratio);
+    f.rem = (float)p.remineralisation_rate;
+    f.small_frac = (float)p.small_particle_fraction;
+    f.uptake_N = (float)(1 - p.ammonia_fraction_of_exudate * p.phytoplankton_exudation_fraction);
+    f.log2Q10_10 = (float)(std::log2(p.temperature_coefficient > 0 ? p.temperature_coefficient : 1.0) / 10);
+    f.light_form = p.light_limitation;
+    f.use_Q10 = p.use_temperature_coefficient;
+    f.mort_form = p.phytoplankton_mortality_formulation;
+    f.graz_form = p.grazing_concentration_formulation;
+    return f;
+}
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+// max(0, x) that keeps a NaN x (Julia's max propagates NaN)
+__device__ __forceinline__ float fmax0(float x) { return (x < 0.0f) ? 0.0f : x; }
+constexpr float F_TINY = 1.17549435e-38f;  // smallest normal float
+
+// Grazing of the PhytoZoo plankton (plankton.jl:118-135, :352-397) in FP32,
+// with the shared factors formed once: g L Z / (food + tiny) scales both
+// grazed fractions.  (The reference's eps(food) only keeps 0/0 away; the
+// smallest normal does the same.)
+struct FGrazing {
+    float total, onP, onSmall;
+};
+__device__ __forceinline__ FGrazing fast_grazing(const FastParams& f, float P, float Z, float sPOM)
+{
+    const float pP = f.pref * P;
+    const float p = pP * frcp(fmaf(f.prefc, sPOM, pP) + F_TINY);
+    const float food = fmaf(p, P - sPOM, sPOM);
+    float L;
+    if (f.graz_form == OBME_LINEAR) L = food * frcp(food + f.kG);
+    else {
+        const float f2 = food * food;
+        L = f2 * frcp(f2 + f.kG2);
+    }
+    FGrazing G;
+    G.total = f.g * L * Z;
+    const float fr = G.total * frcp(food + F_TINY);
+    G.onP = fr * (p * P);
+    G.onSmall = fr * fmaf(-p, sPOM, sPOM);
+    return G;
+}
+
+__device__ __forceinline__ float fast_light(const FastParams& f, float PAR)
+{
+    if (f.light_form == OBME_LIGHT_MONDO) return PAR * frcp(f.kPAR + PAR);
+    return PAR * __builtin_amdgcn_rsqf(fmaf(PAR, PAR, f.kPAR2));
+}
+
+// LOBSTER (+ CarbonateSystem) transitions in FP32; v/out in tracer order.
+template <bool CARB>
+__device__ __forceinline__ void lobster_fast(const FastParams& f, const float* v, float PAR, float* out)
+{
+    const float NO3 = v[0], NH4 = v[1], P = v[2], Z = v[3], sPOM = v[4], bPOM = v[5], DOM = v[6];
+    const float nl = NO3 * __builtin_amdgcn_exp2f(f.nPSI2 * NH4) * frcp(NO3 + f.kNO3);
+    const float al = fmax0(NH4 * frcp(f.kNH4 + NH4));
+    const float muP = f.mu0h * fast_light(f, PAR) * (nl + al) * P;
+    const FGrazing G = fast_grazing(f, P, Z, sPOM);
+    const float nuP = (f.mort_form == OBME_LINEAR) ? f.mP * P : f.mP * P * P;
+    const float nit = f.nitrif * NH4;
+    const float rsh = muP * frcp(nl + al + F_TINY);
+    const float up_NO3 = rsh * nl;
+    const float up_NH4 = fmaf(rsh, al, -f.axg * muP);
+    const float pinw = fmaf(f.pinw_z, Z, f.one_m_aPs * nuP);
+    const float sb = fmaf(f.s_rem, sPOM, f.b_rem * bPOM);
+    const float dinw = fmaf(f.a_rem, sb, f.d_rem * DOM);
+    const float ponw = fmaf(f.ponw_mu, muP, f.ponw_z * Z);
+    const float Z2 = Z * Z;
+    const float solid = fmaf(f.one_m_aZass, G.total, fmaf(f.aPs, nuP, f.mZ * Z2));
+    const float dNO3 = nit - up_NO3;
+    const float dNH4 = pinw + dinw - nit - up_NH4;
+    out[0] = dNO3;
+    out[1] = dNH4;
+    out[2] = fmaf(f.one_m_gamma, muP, -G.onP) - nuP;
+    out[3] = fmaf(f.aZass, G.total, -f.mZ * Z2) - f.muZ * Z;
+    out[4] = fmaf(f.ss, solid, -G.onSmall) - f.s_rem * sPOM;
+    out[5] = fmaf(f.one_m_ss, solid, -f.b_rem * bPOM);
+    out[6] = fmaf(f.one_m_arem, sb, ponw) - f.d_rem * DOM;
+    if constexpr (CARB) {
+        // calcite dissolution (G_P + ν) ρ R (plankton.jl:429-441), uptake 2 ρ μP R (:443-450)
+        const float cdis2 = (G.onP + nuP) * f.rhoR2;            // 2 cdis
+        out[7] = fmaf(-f.ppc, muP, fmaf(f.R, pinw, fmaf(f.Rd, dinw, 0.5f * cdis2)));
+        out[8] = fmaf(0.9375f, dNH4, -1.0625f * dNO3) - f.rhoR4 * muP + cdis2;
+    }
+}
+
+// NPZD transitions in FP32 (N, P, Z, D); Lt = Q10^(T/10).
+__device__ __forceinline__ void npzd_fast(const FastParams& f, float N, float P, float Z, float D, float PAR,
+                                          float Lt, float& dN, float& dP, float& dZ, float& dD)
+{
+    const float muP = f.mu0 * fast_light(f, PAR) * (N * frcp(N + f.kNO3)) * Lt * P;
+    const float sPOM = D * f.small_frac;
+    const FGrazing G = fast_grazing(f, P, Z, sPOM);
+    const float nuP = (f.mort_form == OBME_LINEAR) ? f.mP * P : f.mP * P * P;
+    const float Z2 = Z * Z;
+    dP = fmaf(f.one_m_gamma, muP, -G.onP) - nuP;
+    dZ = fmaf(f.aZass, G.total, -f.mZ * Z2) - f.muZ * Z;
+    const float pinw = fmaf(f.pinw_z, Z, f.one_m_aPs * nuP);
+    dN = fmaf(D, f.rem, pinw) - muP * f.uptake_N;
+    const float ponw = fmaf(f.ponw_mu, muP, f.ponw_z * Z);
+    const float solid = fmaf(f.one_m_aZass, G.total, fmaf(f.aPs, nuP, f.mZ * Z2));
+    dD = ponw + solid - G.onSmall - f.rem * D;
 }
 
 // ---------------------------------------------------------------- carbonate chemistry (FP64)

@@ -1,22 +1,39 @@
-// obme_npd_kernels.h — the fused column-tile kernels of the NPZD / LOBSTER
-// biogeochemistry step on gfx950.
+// obme_npd_kernels.h — the column-sweep and surface kernels of the NPZD /
+// LOBSTER biogeochemistry step on gfx950.
 //
-// Work decomposition.  A workgroup owns TX consecutive i-columns of one j
-// row and the whole depth: thread (tx, kg) holds levels
-// k = kg*LPT .. kg*LPT+LPT-1 (0-based, bottom to top) of column i, so every
-// global load/store of one level is TX consecutive elements of the
-// Oceananigans layout (coalesced along i).  All tracer state of the tile
-// stays in registers through the three RK3 stages of a step; HBM sees one
-// read and one write of each written tracer per step.
-//
-// Column coupling goes through LDS:
+// Column sweep (column_wave): wave per column.  One water column lives
+// across the lanes of one wavefront segment of LPC = 64/CPW lanes; lane r
+// holds L consecutive levels counted from the surface,
+//   k = Nz-1 - (r*L + m),  m = 0..L-1  (0-based, k = 0 at the bottom),
+// so lane 0 holds the top cell.  A workgroup of NW waves owns TXB = NW*CPW
+// consecutive i-columns of one j row.  Tracers are staged through LDS once
+// per step: coalesced loads along i into padded [Nz][TXB+1] tiles, then
+// each lane gathers its levels (and the reverse for the stores), so HBM
+// sees one coalesced read and write of each written tracer per step.  All
+// state stays in registers through the three RK3 stages and the stages need
+// no workgroup barrier:
 //  - the TwoBand chlorophyll integral (Light/2band.jl:18-32) is a suffix sum
-//    over levels: each thread sums its own increments, exchanges the thread
-//    total, and continues the reference recurrence ∫(k) = ∫(k+1) + inc(k)
-//    from the sum of the totals above it;
-//  - upwind sinking needs c(k+1) for a thread's top level: the bottom value
-//    of the thread above is exchanged.
-// Two workgroup barriers per stage.
+//    from the surface: each lane sums its own increments top-down, a
+//    segmented prefix scan over the lanes (DPP row shifts and broadcasts)
+//    gives the sum above it, and the reference recurrence
+//    ∫(k) = ∫(k+1) + inc(k) continues from there;
+//  - upwind sinking needs c(k+1): the level above a lane's top level is the
+//    bottom level of the lane before it, one lane shift;
+//  - FP32-mixed: the step's increment D is accumulated apart from the state
+//    at the start of the step U0 (stage inputs U0 + D, one rounding at the
+//    store), so no stage rounds an increment against the magnitude of the
+//    state; the tendencies use host-folded constant products.  FP64 keeps the
+//    reference's expressions and update order.
+//
+// Surface kernel (k_surface).  DIC and Alk feed no other tendency
+// (NutrientsPlanktonDetritus/carbonate_system.jl: one-way coupled), so the
+// only place the CO2 flux enters is the DIC update of the top cell.  The
+// sweep leaves that one value to a per-column kernel: it records, per
+// column and stage, the surface Alk and the DIC transition without the
+// flux; the surface kernel (one thread per column, no tracer state, high
+// occupancy) solves the carbonate system at each stage and integrates the
+// top-cell DIC with the identical update expressions.  The latency-bound
+// FP64 root solve thereby leaves the bandwidth-bound sweep.
 #pragma once
 
 #include "obme_device.h"
@@ -25,78 +42,62 @@ namespace obme {
 
 constexpr int MAX_SINK = 2;
 
-template <typename FT>
-struct KahanSum {
-    FT s, c;
-    __device__ __forceinline__ void init(FT v) { s = v; c = FT(0); }
-    __device__ __forceinline__ void add(FT v)
-    {
-        if constexpr (sizeof(FT) == 4) {
-            FT y = v - c;
-            FT t = s + y;
-            c = (t - s) - y;
-            s = t;
-        } else {
-            s = s + v;
-        }
-    }
-};
-
 // Per-model tracer layout (tracer order of required_biogeochemical_tracers).
 template <int MODEL, bool CARB>
 struct ModelTraits;
 
 template <>
 struct ModelTraits<OBME_NPZD, false> {
-    static constexpr int NT = 5;         // N, P, Z, T, D
+    static constexpr int NT = 5;            // N, P, Z, T, D
     static constexpr int P_IDX = 1;
-    static constexpr int RO_IDX = 3;     // T: read-only tracer
-    static constexpr int NSINK = 2;
-    static constexpr int SINK[2] = {1, 4};  // P, D
+    static constexpr int RO_IDX = 3;        // T: read-only tracer
+    static constexpr int SINK0 = 1, SINK1 = 4;  // P, D
 };
 template <>
 struct ModelTraits<OBME_LOBSTER, false> {
     static constexpr int NT = 7;
     static constexpr int P_IDX = 2;
     static constexpr int RO_IDX = -1;
-    static constexpr int NSINK = 2;
-    static constexpr int SINK[2] = {4, 5};  // sPOM, bPOM
+    static constexpr int SINK0 = 4, SINK1 = 5;  // sPOM, bPOM
 };
 template <>
 struct ModelTraits<OBME_LOBSTER, true> {
     static constexpr int NT = 9;
     static constexpr int P_IDX = 2;
     static constexpr int RO_IDX = -1;
-    static constexpr int NSINK = 2;
-    static constexpr int SINK[2] = {4, 5};
+    static constexpr int SINK0 = 4, SINK1 = 5;
 };
+
+// Surface record written by the sweep for the surface kernel, per column:
+// [0] DIC at the start of the step, [1+s] DIC transition without the flux
+// at stage s, [4+s] Alk at the start of stage s (s = 0, 1, 2).
+constexpr int SURF_REC = 7;
 
 // Kernel arguments (passed by value).
 template <typename FT>
 struct StepArgs {
     FT* tr[OBME_MAX_TRACERS];      // tracer parent arrays
     FT* G[OBME_MAX_TRACERS];       // tendency parent arrays (hook kernel)
-    FT* PAR;                       // PAR parent array (may be null for the stepper)
-    const FT* lev;                 // per-level geometry, see LEV_* below
+    FT* PAR;                       // PAR parent array (may be null)
+    const FT* lev;                 // per-level geometry rows, see LEV_*
     const FT* wsink;               // [MAX_SINK][Nz+1] face velocities (<= 0)
     const FT* surf;                // surface PAR array (Nx*Ny) or null
-    const FT* T3;                  // 3-D temperature (LOBSTER gas exchange) or null
-    const FT* S3;                  // 3-D salinity or null
-    const double* Tprof;           // temperature profile [Nz] or null
-    const double* Sprof;
-    double Tconst, Sconst;
-    int Nx, Ny, Nz, Hx, Hy, Hz;
-    long sx, sxy, base0;           // strides and interior origin offset
+    FT* srec;                      // surface records [SURF_REC][Nx*Ny] (CO2 on)
+    int Nx, Ny, Nz;
+    unsigned sx, sxy, base0;       // strides and interior origin offset
+    unsigned field_bytes;          // byte size of one parent array
     NPDParams<FT> q;
     FT kr, kb, xr, xb, er, eb, Rr;  // Rr = R_chl / r
     int surface_mode;
     FT surface_value;
-    double dt, t0;
-    int co2_on, co2_mode;
-    obme_co2_exchange co2;
-    obme_carbon_solver solver;
-    double* flux_out;              // optional diagnostics: last-stage CO2 flux (Nx*Ny)
-    double* pH_out;
+    FT PAR0_stage[3];              // diurnal surface PAR at the three stage clocks
+    FT dtf;                        // Δt in the arithmetic type
+    FT dz_u, zc0_u, w0_u, w1_u;    // uniform grid: Δz, zc[0], sinking w
+    FT invdz_u;                    // uniform grid: 1/Δz
+    FastParams f;                  // FP32 path: derived constants
+    float rk_a[3], rk_b[3];        // FP32 path: Δt γ and Δt ζ per stage
+    float kr2, kb2, xr2, xb2;      // FP32 path: attenuations × log2(e)
+    int co2_on;
 };
 
 // Per-level geometry rows in `lev` (each Nz entries).
@@ -104,266 +105,476 @@ enum { LEV_ZC = 0, LEV_UP, LEV_DN, LEV_INVDZ, LEV_COUNT };
 // LEV_UP[k] = zc[k+1] - zf[k+1], LEV_DN[k] = zf[k+1] - zc[k] for k < Nz-1;
 // for k = Nz-1: LEV_UP = 0 and LEV_DN = zf[Nz] - zc[Nz-1] (the top half cell).
 
-__device__ __forceinline__ double default_surface_par(double t)
-{   // Light/Light.jl:219
-    double c = cos(t * 3.141592653589793 / (12 * 3600.0));
-    return 100 * jmax(0.0, c);
-}
-
-template <typename FT>
-__device__ __forceinline__ FT surface_par(const StepArgs<FT>& a, int i, int j, double t)
-{
-    if (a.surface_mode == OBME_SURFACE_ARRAY) return a.surf[i + (long)a.Nx * j];
-    if (a.surface_mode == OBME_SURFACE_DIURNAL) return (FT)default_surface_par(t);
-    return a.surface_value;
-}
-
-// Modes of the column-tile routine.
+// Modes of the column sweep.
 enum { TILE_STEP = 0, TILE_TENDENCY = 1, TILE_PAR = 2 };
 
-template <int MODEL, bool CARB, typename FT, int LPT, int MODE>
-__device__ __forceinline__ void column_tile(const StepArgs<FT>& a, FT* lds)
+// Cross-lane data movement with DPP (GFX9 family): row_shr:d shifts within
+// rows of 16 lanes, row_bcast:15/31 carries row ends into later rows,
+// wave_shr:1 shifts the whole wave by one lane.  bound_ctrl gives 0 for a
+// source lane outside the row (or wave), so a shifted-in 0 is the identity
+// of the sums below.
+template <int CTRL, int ROWMASK, bool BC>
+__device__ __forceinline__ float dpp(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, ROWMASK,
+                                                                 0xf, BC));
+}
+template <int CTRL, int ROWMASK, bool BC>
+__device__ __forceinline__ double dpp(double x)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffull), CTRL, ROWMASK, 0xf, BC);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWMASK, 0xf, BC);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// value of lane r-1 (lane 0 of the wave: 0)
+template <typename T>
+__device__ __forceinline__ T lane_shr1(T x)
+{
+    return dpp<0x138, 0xf, true>(x);
+}
+
+// Inclusive prefix sum over segments of W lanes (W = 16, 32 or 64).
+template <int W, typename T>
+__device__ __forceinline__ T seg_prefix(T x)
+{
+    x = x + dpp<0x111, 0xf, true>(x);
+    x = x + dpp<0x112, 0xf, true>(x);
+    x = x + dpp<0x114, 0xf, true>(x);
+    x = x + dpp<0x118, 0xf, true>(x);
+    if constexpr (W >= 32) x = x + dpp<0x142, 0xa, false>(x);
+    if constexpr (W >= 64) x = x + dpp<0x143, 0xc, false>(x);
+    return x;
+}
+
+template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM>
+__device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
 {
     using TR = ModelTraits<MODEL, CARB>;
     constexpr int NT = TR::NT;
-    const int TX = blockDim.x, KG = blockDim.y;
-    const int tx = threadIdx.x, kg = threadIdx.y;
-    const int i = blockIdx.x * TX + tx;
+    constexpr int RO = TR::RO_IDX;
+    constexpr int LPC = 64 / CPW;          // lanes per column
+    static_assert(LPC == 16 || LPC == 32 || LPC == 64, "segments of 16, 32 or 64 lanes");
+    constexpr int TXB = NW * CPW;          // columns per workgroup
+    constexpr int ROW = TXB + 1;           // padded LDS row: conflict-free gathers
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int r = lane % LPC;              // rank in the column, 0 = top
+    const int c = w * CPW + lane / LPC;    // column in the workgroup
     const int j = blockIdx.y;
-    const bool col_ok = i < a.Nx;
+    const int ibase = blockIdx.x * TXB;
     const int Nz = a.Nz;
-    const int k0 = kg * LPT;
     const int ktop = Nz - 1;
+    const int i_raw = ibase + c;
+    const bool col_ok = i_raw < a.Nx;
+    const int i = col_ok ? i_raw : a.Nx - 1;
 
-    FT* s_pr = lds;                  // [KG][TX]
-    FT* s_pb = s_pr + KG * TX;
-    FT* s_c = s_pb + KG * TX;        // [MAX_SINK][KG][TX]
-    FT* s_tr = s_c + MAX_SINK * KG * TX;
-    FT* s_tb = s_tr + KG * TX;
-    const int me = kg * TX + tx;
-    const int above = (kg + 1) * TX + tx;
+    auto kof = [&](int m) -> int { return ktop - (r * L + m); };
+    auto live = [&](int m) -> bool { return col_ok && kof(m) >= 0; };
+    auto rs = [&](const FT* p) { return make_rsrc(p, a.field_bytes); };
+    // byte offset of (ii, j, k) in a parent array
+    auto goff = [&](int ii, int k) -> unsigned {
+        return (a.base0 + (unsigned)ii + a.sx * (unsigned)j + a.sxy * (unsigned)k) * (unsigned)sizeof(FT);
+    };
 
-    const long colbase = a.base0 + (long)(col_ok ? i : 0) + a.sx * j;
-
-    FT U[NT][LPT];
-    FT Gm[NT][LPT];
-    bool lv[LPT];
+    // ---- stage in: coalesced loads along i into [Nz][ROW] tiles, gather.
+    // Element e of a tile is level e / TXB, column e % TXB; thread t handles
+    // e = t + q * NW*64 for q < L (Nz * TXB <= L * NW * 64).  All loads of
+    // all tracers are issued before the first LDS write.
+    constexpr int NLOAD = (MODE == TILE_PAR) ? 1 : NT;
+    constexpr int BS = NW * 64;
+    auto tile = [&](int t) -> FT* { return lds + t * (Nz * ROW); };
+    const int nel = Nz * TXB;
+    // byte offset of staging element q (beyond the buffer when outside the
+    // tile: reads 0) and its LDS index (-1 outside the tile)
+    auto e_off = [&](int q) -> unsigned {
+        const int e = threadIdx.x + q * BS;
+        const int k = e / TXB, cc = e % TXB;
+        const int ii = (ibase + cc < a.Nx) ? ibase + cc : a.Nx - 1;
+        return (e < nel) ? goff(ii, k) : a.field_bytes;
+    };
+    auto e_idx = [&](int q) -> int {
+        const int e = threadIdx.x + q * BS;
+        return (e < nel) ? (e / TXB) * ROW + e % TXB : -1;
+    };
+    {
+        FT stg[NLOAD][L];
 #pragma unroll
-    for (int l = 0; l < LPT; l++) lv[l] = col_ok && (k0 + l) < Nz;
+        for (int t = 0; t < NLOAD; t++) {
+            const int n = (MODE == TILE_PAR) ? TR::P_IDX : t;
+            const rsrc_t rn = rs(a.tr[n]);
+#pragma unroll
+            for (int q = 0; q < L; q++) buf_load(rn, e_off(q), stg[t][q]);
+        }
+#pragma unroll
+        for (int t = 0; t < NLOAD; t++)
+#pragma unroll
+            for (int q = 0; q < L; q++)
+                if (e_idx(q) >= 0) tile(t)[e_idx(q)] = stg[t][q];
+    }
+    __syncthreads();
+
+    // FP64: U is the state, updated in place (the reference's expressions).
+    // FP32: U is the state at the start of the step and D the step's
+    // accumulated increment; stage inputs are U + D and the sum is rounded
+    // once at the store (a compensated update: the increments are never
+    // rounded against the magnitude of U).
+    constexpr bool F32 = sizeof(FT) == 4;
+    FT U[NT][L];
+    FT Gm[NT][L];
+    FT D[F32 ? NT : 1][L];
 #pragma unroll
     for (int n = 0; n < NT; n++)
 #pragma unroll
-        for (int l = 0; l < LPT; l++) {
-            U[n][l] = lv[l] ? a.tr[n][colbase + a.sxy * (k0 + l)] : FT(0);
-            Gm[n][l] = FT(0);
+        for (int m = 0; m < L; m++) {
+            const int k = kof(m);
+            const bool ok = k >= 0;
+            int t = (MODE == TILE_PAR) ? (n == TR::P_IDX ? 0 : -1) : n;
+            U[n][m] = (ok && t >= 0) ? tile(t)[k * ROW + c] : FT(0);
+            Gm[n][m] = FT(0);
+            if constexpr (F32) D[n][m] = FT(0);
         }
+    // stage value of tracer n at level m
+    auto val = [&](int n, int m) -> FT {
+        if constexpr (F32) return (MODE == TILE_STEP) ? U[n][m] + D[n < NT ? n : 0][m] : U[n][m];
+        else return U[n][m];
+    };
 
-    // T and S are fixed through a step, so everything of the CO2 flux that
-    // depends on them only is computed once per step into LDS (the same
-    // values the per-stage evaluation would produce).
-    const bool owns_top = col_ok && (k0 <= ktop) && (ktop < k0 + LPT);
-    SurfaceCarbon* s_cc = reinterpret_cast<SurfaceCarbon*>(s_tb + KG * TX);
-    if constexpr (CARB) {
-        if (MODE != TILE_PAR && owns_top && a.co2_on) {
-            long ts = colbase + a.sxy * ktop;
-            double Ts = a.T3 ? (double)a.T3[ts] : (a.Tprof ? a.Tprof[ktop] : a.Tconst);
-            double Ss = a.S3 ? (double)a.S3[ts] : (a.Sprof ? a.Sprof[ktop] : a.Sconst);
-            SurfaceCarbon sc;
-            surface_carbon_setup(a.co2, Ts, Ss, a.co2.wind_speed, sc);
-            s_cc[tx] = sc;
-        }
+    // geometry of level k: computed (uniform grid) or read from the per-level rows
+    auto g_zc = [&](int k) -> FT {
+        if constexpr (UNIFORM) return a.zc0_u + FT(k) * a.dz_u;
+        else return a.lev[LEV_ZC * Nz + k];
+    };
+    auto g_up = [&](int k) -> FT {
+        if constexpr (UNIFORM) return a.dz_u * FT(0.5);
+        else return a.lev[LEV_UP * Nz + k];
+    };
+    auto g_dn = [&](int k) -> FT {
+        if constexpr (UNIFORM) return a.dz_u * FT(0.5);
+        else return a.lev[LEV_DN * Nz + k];
+    };
+    auto g_invdz = [&](int k) -> FT {
+        if constexpr (UNIFORM) return a.invdz_u;
+        else return a.lev[LEV_INVDZ * Nz + k];
+    };
+    // face velocities below (face k) and above (face k+1) cell k, sinking tracer s
+    auto g_wlo = [&](int s, int k) -> FT {
+        if constexpr (UNIFORM) return s == 0 ? a.w0_u : a.w1_u;
+        else return a.wsink[s * (Nz + 1) + k];
+    };
+    auto g_whi = [&](int s, int k) -> FT {
+        if constexpr (UNIFORM) return s == 0 ? a.w0_u : a.w1_u;
+        else return a.wsink[s * (Nz + 1) + k + 1];
+    };
+
+    FT PAR0_const = a.surface_value;
+    if (a.surface_mode == OBME_SURFACE_ARRAY) PAR0_const = a.surf[i + a.Nx * j];
+
+    const unsigned col2d = (unsigned)i + (unsigned)a.Nx * (unsigned)j;
+    const unsigned ncol2d = (unsigned)a.Nx * (unsigned)a.Ny;
+    const bool top_lane = col_ok && r == 0;
+    if constexpr (CARB && MODE == TILE_STEP) {
+        if (top_lane && a.co2_on) a.srec[col2d] = U[7][0];
     }
 
-    const FT* zc = a.lev + LEV_ZC * Nz;
-    const FT* up = a.lev + LEV_UP * Nz;
-    const FT* dn = a.lev + LEV_DN * Nz;
-    const FT* invdz = a.lev + LEV_INVDZ * Nz;
-
-    const int nstage = (MODE == TILE_STEP) ? 3 : 1;
+    constexpr int NSTAGE = (MODE == TILE_STEP) ? 3 : 1;
     const double g1 = 8.0 / 15.0, g2 = 5.0 / 12.0, g3 = 3.0 / 4.0, z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
+    FT PARv[L];   // PAR of the own levels (output of the PAR and tendency modes)
 
-    for (int st = 0; st < nstage; st++) {
-        const double t = a.t0 + (st == 0 ? 0.0 : (st == 1 ? g1 : g1 + g2 + z2)) * a.dt;
+#pragma unroll 1
+    for (int st = 0; st < NSTAGE; st++) {
+        FT PAR0 = PAR0_const;
+        if (a.surface_mode == OBME_SURFACE_DIURNAL) PAR0 = a.PAR0_stage[st];
 
-        // ---- phase A: chlorophyll terms (P R/r)^e of the own levels
-        FT pr[LPT], pb[LPT];
+        // chlorophyll terms (P R/r)^e of the own levels
+        FT pr[L], pb[L];
 #pragma unroll
-        for (int l = 0; l < LPT; l++) {
-            FT x = U[TR::P_IDX][l] * a.Rr;
-            if constexpr (sizeof(FT) == 4) {
-                FT lg = m_log2(x);
-                pr[l] = m_exp2(a.er * lg);
-                pb[l] = m_exp2(a.eb * lg);
+        for (int m = 0; m < L; m++) {
+            const FT x = val(TR::P_IDX, m) * a.Rr;
+            if constexpr (F32) {
+                const FT lg = m_log2(x);
+                pr[m] = m_exp2(a.er * lg);
+                pb[m] = m_exp2(a.eb * lg);
             } else {
-                pr[l] = m_pow(x, a.er);
-                pb[l] = m_pow(x, a.eb);
+                pr[m] = m_pow(x, a.er);
+                pb[m] = m_pow(x, a.eb);
             }
         }
-        s_pr[me] = pr[0];
-        s_pb[me] = pb[0];
-#pragma unroll
-        for (int s = 0; s < TR::NSINK; s++) s_c[s * KG * TX + me] = U[TR::SINK[s]][0];
-        __syncthreads();
+        // the level above a lane's top level is the previous lane's bottom level
+        const FT pr_up = lane_shr1(pr[L - 1]);
+        const FT pb_up = lane_shr1(pb[L - 1]);
+        const FT c0_up = lane_shr1(val(TR::SINK0, L - 1));
+        const FT c1_up = lane_shr1(val(TR::SINK1, L - 1));
 
-        // ---- phase B: increments, thread totals, CO2 flux at the top
-        const bool has_above = (kg + 1) < KG;
-        FT pr_up = has_above ? s_pr[above] : FT(0);
-        FT pb_up = has_above ? s_pb[above] : FT(0);
-        FT c_up[MAX_SINK];
+        // increments top-down: inc(k) = up(k) p(k+1) + dn(k) p(k); the top
+        // level has none above: dn p alone
+        FT incr[L], incb[L];
+        FT totr = FT(0), totb = FT(0);
 #pragma unroll
-        for (int s = 0; s < TR::NSINK; s++) c_up[s] = has_above ? s_c[s * KG * TX + above] : FT(0);
-
-        FT incr[LPT], incb[LPT];
-#pragma unroll
-        for (int l = 0; l < LPT; l++) {
-            const int k = k0 + l;
-            if (k < ktop) {
-                FT prn = (l + 1 < LPT) ? pr[(l + 1) < LPT ? l + 1 : l] : pr_up;
-                FT pbn = (l + 1 < LPT) ? pb[(l + 1) < LPT ? l + 1 : l] : pb_up;
-                incr[l] = up[k] * prn + dn[k] * pr[l];
-                incb[l] = up[k] * pbn + dn[k] * pb[l];
-            } else if (k == ktop) {
-                incr[l] = dn[k] * pr[l];
-                incb[l] = dn[k] * pb[l];
-            } else {
-                incr[l] = FT(0);
-                incb[l] = FT(0);
-            }
+        for (int m = 0; m < L; m++) {
+            const int k = kof(m);
+            const FT prn = (m == 0) ? pr_up : pr[m > 0 ? m - 1 : 0];
+            const FT pbn = (m == 0) ? pb_up : pb[m > 0 ? m - 1 : 0];
+            const int kc = k >= 0 ? k : 0;
+            const FT upk = g_up(kc), dnk = g_dn(kc);
+            const FT ir = (k < ktop) ? upk * prn + dnk * pr[m] : dnk * pr[m];
+            const FT ib = (k < ktop) ? upk * pbn + dnk * pb[m] : dnk * pb[m];
+            incr[m] = (k >= 0) ? ir : FT(0);
+            incb[m] = (k >= 0) ? ib : FT(0);
+            totr = totr + incr[m];
+            totb = totb + incb[m];
         }
-        KahanSum<FT> totr, totb;
-        totr.init(incr[LPT - 1]);
-        totb.init(incb[LPT - 1]);
-#pragma unroll
-        for (int l = LPT - 2; l >= 0; l--) {
-            totr.add(incr[l]);
-            totb.add(incb[l]);
+        // sum of the increments above the lane: exclusive segmented prefix
+        // (a lane tree, error growing as log2 of the depth)
+        FT runr = lane_shr1(seg_prefix<LPC>(totr));
+        FT runb = lane_shr1(seg_prefix<LPC>(totb));
+        if (r == 0) {
+            runr = FT(0);
+            runb = FT(0);
         }
-        s_tr[me] = totr.s;
-        s_tb[me] = totb.s;
-
-        double F = 0.0;
-        if constexpr (CARB) {
-            if (MODE != TILE_PAR && owns_top && a.co2_on) {
-                const int lt = ktop - k0;
-                FT dic = FT(0), alk = FT(0);
+        // PAR of the own levels
 #pragma unroll
-                for (int l = 0; l < LPT; l++)
-                    if (l == lt) {
-                        dic = U[7][l];
-                        alk = U[8][l];
-                    }
-                double pH;
-                int ev = 0;
-                if constexpr (sizeof(FT) == 4)
-                    F = surface_flux<false>(s_cc[tx], a.solver, (double)dic, (double)alk, a.co2.air_concentration,
-                                            1, pH, ev);
-                else
-                    F = surface_flux<true>(s_cc[tx], a.solver, (double)dic, (double)alk, a.co2.air_concentration,
-                                           0, pH, ev);
-                if (a.flux_out && (MODE == TILE_TENDENCY || st == 2)) {
-                    a.flux_out[i + (long)a.Nx * j] = F;
-                    if (a.pH_out) a.pH_out[i + (long)a.Nx * j] = pH;
-                }
-            }
-        }
-        __syncthreads();
-
-        // ---- phase C: integral from the top, PAR, transitions, update
-        KahanSum<FT> runr, runb;
-        runr.init(FT(0));
-        runb.init(FT(0));
-        for (int m = KG - 1; m > kg; m--) {
-            runr.add(s_tr[m * TX + tx]);
-            runb.add(s_tb[m * TX + tx]);
-        }
-        const FT PAR0 = col_ok ? surface_par(a, i, j, t) : FT(0);
-        FT PARv[LPT];
-#pragma unroll
-        for (int l = LPT - 1; l >= 0; l--) {
-            runr.add(incr[l]);
-            runb.add(incb[l]);
-            const int k = k0 + l;
-            FT z = lv[l] ? zc[k] : FT(0);
-            PARv[l] = PAR0 * (m_exp(a.kr * z - a.xr * runr.s) + m_exp(a.kb * z - a.xb * runb.s)) / FT(2);
+        for (int m = 0; m < L; m++) {
+            runr = runr + incr[m];
+            runb = runb + incb[m];
+            const int k = kof(m);
+            const FT z = g_zc(k >= 0 ? k : 0);
+            if constexpr (F32)
+                PARv[m] = PAR0 * (m_exp2(fmaf(a.kr2, z, -a.xr2 * runr)) + m_exp2(fmaf(a.kb2, z, -a.xb2 * runb))) * 0.5f;
+            else
+                PARv[m] = PAR0 * (m_exp(a.kr * z - a.xr * runr) + m_exp(a.kb * z - a.xb * runb)) / FT(2);
         }
 
-        if constexpr (MODE == TILE_PAR) {
+        if constexpr (MODE != TILE_PAR) {
+            // bottom-up within the lane (FP64 updates in place: level m-1,
+            // above m, must still hold its stage value when m reads it)
 #pragma unroll
-            for (int l = 0; l < LPT; l++)
-                if (lv[l]) a.PAR[colbase + a.sxy * (k0 + l)] = PARv[l];
-            return;
-        }
-
-        FT Lt_arr[LPT];
-#pragma unroll
-        for (int l = 0; l < LPT; l++) {
-            Lt_arr[l] = FT(1);
-            if constexpr (MODEL == OBME_NPZD) {
-                if (a.q.use_Q10) Lt_arr[l] = m_pow(a.q.Q10, U[3][l] / FT(10));
-            }
-        }
-
-#pragma unroll
-        for (int l = 0; l < LPT; l++) {
-            const int k = k0 + l;
-            FT G[NT];
-            if constexpr (MODEL == OBME_NPZD) {
-                npzd_tendencies(a.q, U[0][l], U[1][l], U[2][l], U[3][l], U[4][l], PARv[l], Lt_arr[l], G[0], G[1],
-                                G[2], G[4]);
-                G[3] = FT(0);
-            } else {
+            for (int m = L - 1; m >= 0; m--) {
+                const int k = kof(m);
+                const int kc = k >= 0 ? k : 0;
+                FT Gv[NT];
                 FT v[NT];
 #pragma unroll
-                for (int n = 0; n < NT; n++) v[n] = U[n][l];
-                lobster_tendencies<FT, CARB>(a.q, v, PARv[l], G);
-            }
-            // minus the upwind drift flux divergence: w(face) c(upwind cell)
-#pragma unroll
-            for (int s = 0; s < TR::NSINK; s++) {
-                const int n = TR::SINK[s];
-                const FT* w = a.wsink + s * (Nz + 1);
-                FT cnext = (l + 1 < LPT) ? U[n][(l + 1) < LPT ? l + 1 : l] : c_up[s];
-                FT Fhi = (k < ktop) ? w[k + 1] * cnext : FT(0);
-                FT Flo = w[k] * U[n][l];
-                FT div = lv[l] ? (Fhi - Flo) * invdz[k] : FT(0);
-                G[n] = G[n] - div;
-            }
-            if constexpr (CARB) {
-                if (k == ktop && a.co2_on) G[7] = G[7] - (FT)F * invdz[k];
-            }
-
-            if constexpr (MODE == TILE_TENDENCY) {
-                if (lv[l]) {
-                    long idx = colbase + a.sxy * k;
-#pragma unroll
-                    for (int n = 0; n < NT; n++)
-                        if (n != TR::RO_IDX) a.G[n][idx] += G[n];
-                    if (a.PAR) a.PAR[idx] = PARv[l];
+                for (int n = 0; n < NT; n++) v[n] = (CARB && n >= 7) ? FT(0) : val(n, m);
+                if constexpr (MODEL == OBME_NPZD) {
+                    if constexpr (F32) {
+                        const float Lt = a.f.use_Q10 ? m_exp2(v[3] * a.f.log2Q10_10) : 1.0f;
+                        npzd_fast(a.f, v[0], v[1], v[2], v[4], PARv[m], Lt, Gv[0], Gv[1], Gv[2], Gv[4]);
+                    } else {
+                        FT Lt = FT(1);
+                        if (a.q.use_Q10) Lt = m_pow(a.q.Q10, v[3] / FT(10));
+                        npzd_tendencies(a.q, v[0], v[1], v[2], v[3], v[4], PARv[m], Lt, Gv[0], Gv[1], Gv[2], Gv[4]);
+                    }
+                    Gv[3] = FT(0);
+                } else {
+                    if constexpr (F32) lobster_fast<CARB>(a.f, v, PARv[m], Gv);
+                    else lobster_tendencies<FT, CARB>(a.q, v, PARv[m], Gv);
                 }
-            } else {
-                // RK3 substep (BoxModel/timesteppers.jl:79-92 form)
-                const FT dtf = (FT)a.dt;
+                // minus the upwind drift flux divergence: w(face) c(upwind cell);
+                // no flux through the top face (w = 0 there)
+                {
+                    const FT cn0 = (m == 0) ? c0_up : val(TR::SINK0, m > 0 ? m - 1 : 0);
+                    const FT cn1 = (m == 0) ? c1_up : val(TR::SINK1, m > 0 ? m - 1 : 0);
+                    const FT idz = g_invdz(kc);
+                    const FT Fhi0 = (kc < ktop) ? g_whi(0, kc) * cn0 : FT(0);
+                    const FT Fhi1 = (kc < ktop) ? g_whi(1, kc) * cn1 : FT(0);
+                    Gv[TR::SINK0] = Gv[TR::SINK0] - (Fhi0 - g_wlo(0, kc) * v[TR::SINK0]) * idz;
+                    Gv[TR::SINK1] = Gv[TR::SINK1] - (Fhi1 - g_wlo(1, kc) * v[TR::SINK1]) * idz;
+                }
+                if constexpr (MODE == TILE_TENDENCY) {
 #pragma unroll
-                for (int n = 0; n < NT; n++) {
-                    if (n == TR::RO_IDX) continue;
-                    if (st == 0) U[n][l] += dtf * (FT)g1 * G[n];
-                    else if (st == 1) U[n][l] += dtf * ((FT)g2 * G[n] + (FT)z2 * Gm[n][l]);
-                    else U[n][l] += dtf * ((FT)g3 * G[n] + (FT)z3 * Gm[n][l]);
-                    Gm[n][l] = G[n];
+                    for (int n = 0; n < NT; n++) Gm[n][m] = Gv[n];   // Gm carries G out
+                } else {
+                    if constexpr (CARB) {
+                        // surface record: DIC transition without the flux, Alk at the stage start
+                        if (top_lane && a.co2_on && m == 0) {
+                            a.srec[(1 + st) * ncol2d + col2d] = Gv[7];
+                            a.srec[(4 + st) * ncol2d + col2d] = val(8, 0);
+                        }
+                    }
+                    // RK3 substep (BoxModel/timesteppers.jl:79-92 form):
+                    // stage 1 U += (Δt γ1) G; stages 2, 3 U += Δt (γ G + ζ G⁻)
+#pragma unroll
+                    for (int n = 0; n < NT; n++) {
+                        if (n == RO) continue;
+                        if constexpr (F32) {
+                            const float A = (st == 0) ? a.rk_a[0] : (st == 1 ? a.rk_a[1] : a.rk_a[2]);
+                            const float B = (st == 0) ? a.rk_b[0] : (st == 1 ? a.rk_b[1] : a.rk_b[2]);
+                            D[n][m] = fmaf(A, Gv[n], fmaf(B, Gm[n][m], D[n][m]));
+                        } else {
+                            const double gs = (st == 1) ? g2 : g3;
+                            const double zs = (st == 1) ? z2 : z3;
+                            const double inc = (st == 0) ? (a.dtf * g1) * Gv[n] : a.dtf * (gs * Gv[n] + zs * Gm[n][m]);
+                            U[n][m] += inc;
+                        }
+                        Gm[n][m] = Gv[n];
+                    }
                 }
             }
         }
     }
 
-    if constexpr (MODE == TILE_STEP) {
+    // ---- stage out: scatter to the tiles, coalesced stores along i.
+    // Outputs: STEP the written tracers; PAR the PAR field; TENDENCY the
+    // written tendencies (added to G) and PAR (tile NT) when requested.
+    __syncthreads();
+    constexpr int NOUT = (MODE == TILE_PAR) ? 1 : (MODE == TILE_TENDENCY ? NT + 1 : NT);
 #pragma unroll
-        for (int n = 0; n < NT; n++) {
-            if (n == TR::RO_IDX) continue;
+    for (int n = 0; n < NOUT; n++) {
+        if (MODE != TILE_PAR && n == RO) continue;
+        FT* T = tile(n);
 #pragma unroll
-            for (int l = 0; l < LPT; l++)
-                if (lv[l]) a.tr[n][colbase + a.sxy * (k0 + l)] = U[n][l];
+        for (int m = 0; m < L; m++) {
+            const int k = kof(m);
+            FT v;
+            if constexpr (MODE == TILE_PAR) v = PARv[m];
+            else if constexpr (MODE == TILE_TENDENCY) v = (n == NT) ? PARv[m] : Gm[n < NT ? n : 0][m];
+            else v = val(n < NT ? n : 0, m);
+            if (k >= 0) T[k * ROW + c] = v;
         }
     }
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < NOUT; n++) {
+        if (MODE != TILE_PAR && n == RO) continue;
+        FT* T = tile(n);
+        FT* dst;
+        if constexpr (MODE == TILE_PAR) dst = a.PAR;
+        else if constexpr (MODE == TILE_TENDENCY) dst = (n == NT) ? a.PAR : a.G[n];
+        else dst = a.tr[n];
+        if (dst == nullptr) continue;
+        const rsrc_t rd = rs(dst);
+        const bool accumulate = (MODE == TILE_TENDENCY) && n < NT;
+        FT v[L];
+#pragma unroll
+        for (int q = 0; q < L; q++) v[q] = (e_idx(q) >= 0) ? T[e_idx(q)] : FT(0);
+        if (accumulate) {
+            FT g0[L];
+#pragma unroll
+            for (int q = 0; q < L; q++) buf_load(rd, e_off(q), g0[q]);
+#pragma unroll
+            for (int q = 0; q < L; q++) v[q] = g0[q] + v[q];
+        }
+#pragma unroll
+        for (int q = 0; q < L; q++) {
+            const int e = threadIdx.x + q * BS;
+            const int k = e / TXB, cc = e % TXB;
+            bool ok = e_idx(q) >= 0 && ibase + cc < a.Nx;
+            if constexpr (CARB && MODE == TILE_STEP) {
+                // the top-cell DIC is written by the surface kernel
+                if (n == 7 && a.co2_on && k == ktop) ok = false;
+            }
+            if (ok) buf_store(rd, e_off(q), v[q]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ surface
+
+template <typename FT>
+struct SurfaceArgs {
+    FT* DIC;                       // DIC parent array (top cell written, step mode)
+    FT* GDIC;                      // DIC tendency parent array (tendency mode)
+    const FT* Alk;                 // Alk parent array (tendency mode)
+    const FT* srec;                // surface records (step mode)
+    double* Hprev;                 // [Nx*Ny] warm start of [H+] (mixed path) or null
+    const FT* T3;                  // 3-D temperature / salinity or null
+    const FT* S3;
+    const double* Tprof;
+    const double* Sprof;
+    double Tconst, Sconst;
+    double* flux_out;              // optional diagnostics (last stage)
+    double* pH_out;
+    int Nx, Ny, Nz;
+    unsigned sx, sxy, base0;
+    FT dtf, invdz_top;
+    float rk_a[3], rk_b[3];        // FP32 path: Δt γ and Δt ζ per stage
+    int mode;                      // 0: reference damped Newton; 1: mixed
+    obme_co2_exchange co2;
+    obme_carbon_solver solver;
+};
+
+// Flux with the mixed path's warm start: H from the previous stage (or an
+// FP32 guess) refined by FP64 Newton, with the reference solve as fallback.
+__device__ __forceinline__ double surface_flux_warm(const SurfaceCarbon& sc, const obme_carbon_solver& s,
+                                                    double DIC, double Alk, double pCO2_air, int mode,
+                                                    double& H, double& pH, int& evals)
+{
+    if (mode == 0) return surface_flux<true>(sc, s, DIC, Alk, pCO2_air, 0, pH, evals);
+    CarbonParams p = sc.p;
+    p.Alk = Alk * sc.conv;
+    p.DIC = DIC * sc.conv;
+    double x = (H > 0.0) ? H : (double)guess_H_fp32(p);
+    bool done = refine_H(p, x, evals);
+    if (!done) {
+        x = (double)guess_H_fp32(p);
+        done = refine_H(p, x, evals);
+        if (!done) x = solve_for_H(p, s, pow(10.0, -s.initial_pH_guess), evals);
+    }
+    H = x;
+    pH = -log10(x);
+    return sc.k * (pco2_from_H(p, x) - pCO2_air);
+}
+
+template <typename FT, int MODE>
+__global__ void __launch_bounds__(256) k_surface(SurfaceArgs<FT> a)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncol = a.Nx * a.Ny;
+    if (col >= ncol) return;
+    const int i = col % a.Nx, j = col / a.Nx;
+    const int ktop = a.Nz - 1;
+    const unsigned top = a.base0 + (unsigned)i + a.sx * (unsigned)j + a.sxy * (unsigned)ktop;
+    const double T = a.T3 ? (double)a.T3[top] : (a.Tprof ? a.Tprof[ktop] : a.Tconst);
+    const double S = a.S3 ? (double)a.S3[top] : (a.Sprof ? a.Sprof[ktop] : a.Sconst);
+    SurfaceCarbon sc;
+    surface_carbon_setup(a.co2, T, S, a.co2.wind_speed, sc);
+    double H = a.Hprev ? a.Hprev[col] : 0.0;
+    double pH = 0.0, F = 0.0;
+    int ev = 0;
+    if constexpr (MODE == TILE_TENDENCY) {
+        F = surface_flux_warm(sc, a.solver, (double)a.DIC[top], (double)a.Alk[top], a.co2.air_concentration, a.mode,
+                              H, pH, ev);
+        a.GDIC[top] = a.GDIC[top] - (FT)F * a.invdz_top;
+    } else {
+        // the identical update expressions of the sweep, with the flux term
+        FT U = a.srec[col];
+        FT Gm = FT(0);
+        if constexpr (sizeof(FT) == 4) {
+            float D = 0.0f;
+#pragma unroll
+            for (int st = 0; st < 3; st++) {
+                const float Gb = a.srec[(1 + st) * ncol + col];
+                const float Alk = a.srec[(4 + st) * ncol + col];
+                F = surface_flux_warm(sc, a.solver, (double)(U + D), (double)Alk, a.co2.air_concentration, a.mode,
+                                      H, pH, ev);
+                const float G = Gb - (float)F * a.invdz_top;
+                D = fmaf(a.rk_a[st], G, fmaf(a.rk_b[st], Gm, D));
+                Gm = G;
+            }
+            U = U + D;
+        } else {
+            const FT g1 = FT(8.0 / 15.0), g2 = FT(5.0 / 12.0), g3 = FT(3.0 / 4.0);
+            const FT z2 = FT(-17.0 / 60.0), z3 = FT(-5.0 / 12.0);
+#pragma unroll
+            for (int st = 0; st < 3; st++) {
+                FT Gb = a.srec[(1 + st) * ncol + col];
+                FT Alk = a.srec[(4 + st) * ncol + col];
+                F = surface_flux_warm(sc, a.solver, (double)U, (double)Alk, a.co2.air_concentration, a.mode, H, pH,
+                                      ev);
+                FT G = Gb - (FT)F * a.invdz_top;
+                if (st == 0) U += a.dtf * g1 * G;
+                else if (st == 1) U += a.dtf * (g2 * G + z2 * Gm);
+                else U += a.dtf * (g3 * G + z3 * Gm);
+                Gm = G;
+            }
+        }
+        a.DIC[top] = U;
+    }
+    if (a.Hprev) a.Hprev[col] = H;
+    if (a.flux_out) a.flux_out[col] = F;
+    if (a.pH_out) a.pH_out[col] = pH;
 }
 
 }  // namespace obme

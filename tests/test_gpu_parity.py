@@ -42,6 +42,7 @@ def test_npzd_2d_fp64():
 def test_npzd_2d_fp32_mixed():
     print("case npzd_2d fp32", flush=True)
     errs = _run_both(workloads.npzd_2d(), "fp32")
+    print("errors", {k: float(v) for k, v in errs.items()}, flush=True)
     assert all(e <= FP32_TOL for e in errs.values()), errs
 
 
@@ -58,6 +59,7 @@ def test_lobster_fp32_mixed(shape):
     print("case lobster fp32", shape, flush=True)
     wl = workloads.lobster_3d(*shape, nsteps=20)
     errs = _run_both(wl, "fp32")
+    print("errors", {k: float(v) for k, v in errs.items()}, flush=True)
     assert all(e <= FP32_TOL for e in errs.values()), errs
 
 
@@ -114,3 +116,26 @@ def test_co2_flux_points(mode):
     F = F.cpu().numpy()
     assert np.max(np.abs(pH - pH_ref)) <= 1e-12
     assert rel_l2(F, F_ref) <= 1e-10
+
+
+TILE_CFGS = ["2,2,16", "4,4,8", "2,2,8", "1,1,16", "2,1,16", "4,2,8", "1,1,8", "2,1,8"]
+
+
+@pytest.mark.parametrize("tile", TILE_CFGS)
+def test_tile_configurations_fp64(tile, monkeypatch):
+    """Every sweep tile configuration gives the same answer (FP64 parity)."""
+    print("case tile fp64", tile, flush=True)
+    monkeypatch.setenv("OBME_TILE", tile)
+    wl = workloads.lobster_3d(40, 3, 29, nsteps=3)
+    errs = _run_both(wl, "fp64")
+    assert all(e <= FP64_TOL for e in errs.values()), errs
+
+
+@pytest.mark.parametrize("tile", TILE_CFGS)
+def test_tile_configurations_fp32(tile, monkeypatch):
+    print("case tile fp32", tile, flush=True)
+    monkeypatch.setenv("OBME_TILE", tile)
+    wl = workloads.lobster_3d(40, 3, 29, nsteps=5)
+    errs = _run_both(wl, "fp32")
+    print("errors", {k: float(v) for k, v in errs.items()}, flush=True)
+    assert all(e <= FP32_TOL for e in errs.values()), errs
