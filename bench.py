@@ -38,8 +38,11 @@ def parse():
     p.add_argument("--nz", type=int, default=64)
     p.add_argument("--check-steps", type=int, default=200, help="steps of the error check run")
     p.add_argument("--sample-columns", type=int, default=256)
-    p.add_argument("--cpu-columns", type=int, default=1024, help="columns of the CPU baseline sample")
+    p.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU work of the baseline sample")
     p.add_argument("--cpu-threads", type=int, default=0)
+    p.add_argument("--calibrate-copy-mb", type=int, default=0,
+                   help="after the timed steps, copy a buffer of this many MiB once (a known-size kernel "
+                        "for scaling HBM counters)")
     p.add_argument("--no-check", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--precisions", default="fp32,fp64")
@@ -51,19 +54,54 @@ def log(*a):
 
 
 def time_steps(dev, tr, steps, dt, warmup):
+    """Time `steps` steps after `warmup` untimed ones.  Kernel timing is on
+    through the timed region: every sweep launch is bracketed by HIP events
+    on the stream it runs on, so the step time includes that instrumentation.
+    Returns (seconds of the region, wall seconds, mean sweep launch ms)."""
     import torch
     if warmup:
         dev.step(tr, warmup, dt)
     torch.cuda.synchronize()
+    dev.set_kernel_timing(True)
+    dev.kernel_timing()          # drop anything recorded before
+    stream = torch.cuda.current_stream()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
-    start.record()
+    start.record(stream)
     dev.step(tr, steps, dt, t0=warmup * dt)
-    stop.record()
+    stop.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    return start.elapsed_time(stop) / 1e3, wall
+    sweep_ms, n = dev.kernel_timing()
+    dev.set_kernel_timing(False)
+    return start.elapsed_time(stop) / 1e3, wall, sweep_ms / max(n, 1)
+
+
+def cpu_threads(req):
+    if req:
+        return req
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def read_traffic(profile_dir, workload, precision):
+    """HBM bytes per sweep launch from a committed counter summary: a line
+    'traffic_bytes_per_launch <workload> <precision> <bytes>'."""
+    if not os.path.isdir(profile_dir):
+        return None
+    for fn in sorted(os.listdir(profile_dir)):
+        if not fn.endswith((".md", ".txt")):
+            continue
+        for line in open(os.path.join(profile_dir, fn)):
+            parts = line.strip().strip("`").split()
+            if len(parts) == 4 and parts[0] == "traffic_bytes_per_launch" and parts[1] == workload \
+                    and parts[2] == precision:
+                return float(parts[3])
+    return None
 
 
 def main():
@@ -87,15 +125,18 @@ def main():
     for prec in args.precisions.split(","):
         dev = HIPBiogeochemistry(wl.bgc, prec)
         tr = [torch.from_numpy(a).to(device="cuda", dtype=dev.dtype) for a in wl.tracers]
-        secs, wall = time_steps(dev, tr, args.steps, wl.dt, args.warmup)
+        secs, wall, sweep_ms = time_steps(dev, tr, args.steps, wl.dt, args.warmup)
         sbytes = 4 if prec == "fp32" else 8
         bytes_step = wl.algorithmic_bytes_per_step(sbytes)
         ms = secs / args.steps * 1e3
         upd = nw * ncell * args.steps / secs
         gbs = bytes_step / (secs / args.steps) / 1e9
+        sweep_gbs = bytes_step / (sweep_ms * 1e-3) / 1e9
         results[prec] = {"ms_per_step": ms, "updates_per_s": upd, "achieved_GBs": gbs,
+                         "sweep_ms_per_launch": sweep_ms, "sweep_achieved_GBs": sweep_gbs,
                          "bytes_per_step": bytes_step, "wall_s": wall}
-        log(f"{prec}: {ms:.4f} ms/step  {upd:.3e} upd/s  {gbs:.1f} GB/s  ({gbs / HBM_PEAK_GBS * 100:.1f}% HBM)")
+        log(f"{prec}: {ms:.4f} ms/step  {upd:.3e} upd/s  {gbs:.1f} GB/s  ({gbs / HBM_PEAK_GBS * 100:.1f}% HBM); "
+            f"sweep {sweep_ms:.4f} ms/launch = {sweep_gbs:.1f} GB/s")
         if not args.no_check:
             tr = [torch.from_numpy(a).to(device="cuda", dtype=dev.dtype) for a in wl.tracers]
             dev.step(tr, args.check_steps, wl.dt)
@@ -103,6 +144,14 @@ def main():
             outputs[prec] = [t.double().cpu().numpy() for t in tr]
         del tr, dev
         torch.cuda.empty_cache()
+    if args.calibrate_copy_mb:
+        n = args.calibrate_copy_mb * (1 << 20) // 4
+        src = torch.ones(n, device="cuda", dtype=torch.float32)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {n * 4} bytes read, {n * 4} bytes written")
+        del src, dst
 
     errors = {}
     cpu = None
@@ -110,7 +159,7 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
         cfg = wl.bgc.config(abi.OBME_FP64)
-        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        nthreads = cpu_threads(args.cpu_threads)
         if not args.no_check:
             rng = np.random.default_rng(1)
             ns = min(args.sample_columns, g.Nx * g.Ny)
@@ -131,25 +180,36 @@ def main():
                 worst = max(v["rel_l2"] for v in e.values())
                 log(f"{prec} vs oracle after {args.check_steps} steps ({ns} columns): worst rel L2 {worst:.3e}")
         if not args.no_cpu:
-            nc = min(args.cpu_columns, g.Nx * g.Ny)
-            ci = (np.arange(nc) % g.Nx).astype(np.int32)
-            cj = (np.arange(nc) // g.Nx).astype(np.int32)
+            # FP64 oracle on every column of the same workload, OpenMP over
+            # columns; the step count is sized from a short probe so that the
+            # sample is about --cpu-seconds of CPU work.
+            allc = np.arange(g.Nx * g.Ny)
+            ci = (allc % g.Nx).astype(np.int32)
+            cj = (allc // g.Nx).astype(np.int32)
             ref = [a.copy() for a in wl.tracers]
-            steps = 20
+            probe = max(1, len(ci) // 16)
+            t0 = time.perf_counter()
+            O.grid_run_columns(cfg, ref, ci[:probe], cj[:probe], 1, wl.dt, nthreads=nthreads)
+            per_col_step = (time.perf_counter() - t0) / probe
+            steps = int(max(1, min(50, round(args.cpu_seconds / (per_col_step * len(ci))))))
             t0 = time.perf_counter()
             O.grid_run_columns(cfg, ref, ci, cj, steps, wl.dt, nthreads=nthreads)
             secs = time.perf_counter() - t0
-            cpu = {"value": nw * g.Nz * nc * steps / secs, "unit": "tracer-cell updates/s", "cores": nthreads,
-                   "kind": "port",
-                   "sample": f"{nc} columns x {g.Nz} levels x {steps} steps of the same workload, FP64 oracle, "
-                             f"OpenMP over columns ({secs:.1f} s)"}
-            log(f"cpu baseline: {cpu['value']:.3e} upd/s on {nthreads} threads")
+            cpu = {"value": nw * g.Nz * len(ci) * steps / secs, "unit": "tracer-cell updates/s",
+                   "cores": nthreads, "kind": "port",
+                   "sample": f"all {len(ci)} columns x {g.Nz} levels x {steps} steps of the same workload, "
+                             f"FP64 oracle (restated reference CPU path), OpenMP over columns, {secs:.1f} s"}
+            log(f"cpu baseline: {cpu['value']:.3e} upd/s on {nthreads} threads ({secs:.1f} s)")
 
-    head = results.get("fp32") or results["fp64"]
-    dom = {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None}
+    head_prec = "fp32" if "fp32" in results else "fp64"
+    head = results[head_prec]
+    traffic = read_traffic(os.path.join(ROOT, "profiles"), wl.name, head_prec)
+    dom = {"bound": "hbm", "achieved": head["sweep_achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": head["sweep_achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
+           "kernel": "k_column_wave (column sweep)", "algorithmic_bytes_per_launch": head["bytes_per_step"],
+           "launch_ms": head["sweep_ms_per_launch"]}
     line = {
-        "metric": "tracer-cell updates/s (FP32-mixed LOBSTER+DIC/Alk step) + % HBM roofline",
+        "metric": "tracer-cell updates/s + % HBM roofline; FP32-mixed vs FP64 at stated error",
         "value": head["updates_per_s"],
         "unit": "tracer-cell updates/s",
         "n_gpus": 1,

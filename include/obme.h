@@ -242,8 +242,7 @@ obme_status obme_step(obme_handle* h, void* const* tracers,
  * mode 0: the reference damped Newton from pH 8 in FP64;
  * mode 1: an FP32 initial guess refined by FP64 Newton.
  * All arrays are double; u10 may be NULL (use cfg wind_speed). out_pH and
- * out_flux may be NULL.  Also counts residual evaluations into
- * *out_evals (device int64, may be NULL). */
+ * out_flux may be NULL. */
 obme_status obme_co2_flux(obme_handle* h, const double* DIC, const double* Alk,
                           const double* T, const double* S, const double* u10,
                           double pCO2_air, double* out_pH, double* out_flux,
@@ -279,6 +278,18 @@ enum {
     OBME_BP_SURFACE_PAR,
     OBME_BP_COUNT
 };
+
+/* ---- kernel timing ------------------------------------------------------ */
+
+/* No counterpart in the reference: measurement instrumentation.  While
+ * enabled, every launch of kernel `which` made by this handle is bracketed
+ * by HIP events recorded on the caller's stream. */
+enum { OBME_KERNEL_SWEEP = 0, OBME_KERNEL_SURFACE = 1, OBME_KERNEL_COUNT };
+obme_status obme_set_kernel_timing(obme_handle* h, int32_t enable);
+/* Waits for the recorded events of kernel `which`, returns their summed
+ * duration in ms and the launch count since the previous call, and
+ * releases them. */
+obme_status obme_kernel_timing(obme_handle* h, int32_t which, double* total_ms, int64_t* launches);
 
 /* Size in bytes of ABI struct `which` as compiled into the library
  * (0 obme_config, 1 obme_grid, 2 obme_npd_params, 3 obme_two_band_par,

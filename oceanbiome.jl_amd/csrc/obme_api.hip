@@ -91,6 +91,39 @@ struct obme_handle {
     int cfg_id;                // sweep tile configuration (TileCfg table)
     bool uniform;
     double dz_u, zc0_u, w0_u, w1_u;
+    // kernel timing: event pairs per kernel kind, and a pool of free events
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> timed[OBME_KERNEL_COUNT];
+    std::vector<hipEvent_t> event_pool;
+};
+
+// Brackets one launch with events on stream s when timing is enabled.
+struct LaunchTimer {
+    obme_handle* h;
+    int which;
+    hipStream_t s;
+    hipEvent_t e0 = nullptr;
+    LaunchTimer(obme_handle* h_, int which_, hipStream_t s_) : h(h_), which(which_), s(s_)
+    {
+        if (h->timing && take(e0)) (void)hipEventRecord(e0, s);
+    }
+    bool take(hipEvent_t& e)
+    {
+        if (!h->event_pool.empty()) {
+            e = h->event_pool.back();
+            h->event_pool.pop_back();
+            return true;
+        }
+        return hipEventCreate(&e) == hipSuccess;
+    }
+    void done()
+    {
+        if (!e0) return;
+        hipEvent_t e1;
+        if (take(e1) && hipEventRecord(e1, s) == hipSuccess) h->timed[which].push_back({e0, e1});
+        else h->event_pool.push_back(e0);
+        e0 = nullptr;
+    }
 };
 
 static int ntracers_of(const obme_config& c)
@@ -286,6 +319,12 @@ extern "C" obme_status obme_destroy(obme_handle* h)
     if (h->d_Sprof) (void)hipFree(h->d_Sprof);
     if (h->d_srec) (void)hipFree(h->d_srec);
     if (h->d_Hprev) (void)hipFree(h->d_Hprev);
+    for (auto& v : h->timed)
+        for (auto& p : v) {
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+    for (hipEvent_t e : h->event_pool) (void)hipEventDestroy(e);
     delete h;
     return OBME_OK;
 }
@@ -390,7 +429,7 @@ static SurfaceArgs<FT> make_surface_args(const obme_handle* h, void* const* trac
 }
 
 template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE>
-static obme_status launch_wave(const obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
+static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
 {
     constexpr int NT = ModelTraits<MODEL, CARB>::NT;
     constexpr int TXB = NW * CPW;
@@ -399,21 +438,18 @@ static obme_status launch_wave(const obme_handle* h, const StepArgs<FT>& a, hipS
     if (lds > 160 * 1024) return fail(OBME_ERR_UNSUPPORTED, "sweep tile exceeds the 160 KiB LDS");
     dim3 block(NW * 64);
     dim3 grid((h->cfg.grid.Nx + TXB - 1) / TXB, h->cfg.grid.Ny);
-    if (h->uniform) {
-        auto k = k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true>;
-        HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k, grid, block, lds, s, a);
-    } else {
-        auto k = k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false>;
-        HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k, grid, block, lds, s, a);
-    }
+    auto k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true>
+                        : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false>;
+    HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    LaunchTimer tm(h, OBME_KERNEL_SWEEP, s);
+    hipLaunchKernelGGL(k, grid, block, lds, s, a);
+    tm.done();
     HIP_TRY(hipGetLastError());
     return OBME_OK;
 }
 
 template <typename FT, int L, int CPW, int NW, int MODE>
-static obme_status dispatch_model(const obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
+static obme_status dispatch_model(obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
 {
     if (h->cfg.model == OBME_NPZD) return launch_wave<OBME_NPZD, false, FT, L, CPW, NW, MODE>(h, a, s);
     if (h->cfg.carbonate_system) return launch_wave<OBME_LOBSTER, true, FT, L, CPW, NW, MODE>(h, a, s);
@@ -421,7 +457,7 @@ static obme_status dispatch_model(const obme_handle* h, const StepArgs<FT>& a, h
 }
 
 template <typename FT, int MODE>
-static obme_status dispatch_tile_cfg(const obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
+static obme_status dispatch_tile_cfg(obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
 {
     switch (h->cfg_id) {
     case 0: return dispatch_model<FT, 2, 2, 16, MODE>(h, a, s);
@@ -437,11 +473,13 @@ static obme_status dispatch_tile_cfg(const obme_handle* h, const StepArgs<FT>& a
 }
 
 template <typename FT, int MODE>
-static obme_status launch_surface(const obme_handle* h, const SurfaceArgs<FT>& sa, hipStream_t s)
+static obme_status launch_surface(obme_handle* h, const SurfaceArgs<FT>& sa, hipStream_t s)
 {
     const int bs = 256;
     const long ncol = (long)sa.Nx * sa.Ny;
+    LaunchTimer tm(h, OBME_KERNEL_SURFACE, s);
     hipLaunchKernelGGL((k_surface<FT, MODE>), dim3((ncol + bs - 1) / bs), dim3(bs), 0, s, sa);
+    tm.done();
     HIP_TRY(hipGetLastError());
     return OBME_OK;
 }
@@ -529,6 +567,38 @@ extern "C" obme_status obme_co2_flux(obme_handle* h, const double* DIC, const do
                        out_pH, out_flux, (long)n, mode, h->cfg.co2, h->cfg.solver, nullptr);
     HIP_TRY(hipGetLastError());
     return OBME_OK;
+}
+
+extern "C" obme_status obme_set_kernel_timing(obme_handle* h, int32_t enable)
+{
+    if (!h) return fail(OBME_ERR_INVALID, "null handle");
+    h->timing = enable != 0;
+    return OBME_OK;
+}
+
+extern "C" obme_status obme_kernel_timing(obme_handle* h, int32_t which, double* total_ms, int64_t* launches)
+{
+    if (!h || !total_ms || !launches) return fail(OBME_ERR_INVALID, "null argument");
+    if (which < 0 || which >= OBME_KERNEL_COUNT) return fail(OBME_ERR_INVALID, "unknown kernel");
+    double sum = 0.0;
+    int64_t n = 0;
+    obme_status st = OBME_OK;
+    for (auto& p : h->timed[which]) {
+        float ms = 0.0f;
+        if (st == OBME_OK) {
+            hipError_t e = hipEventSynchronize(p.second);
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, p.first, p.second);
+            if (e != hipSuccess) st = fail(OBME_ERR_HIP, hipGetErrorString(e));
+        }
+        sum += ms;
+        n++;
+        h->event_pool.push_back(p.first);
+        h->event_pool.push_back(p.second);
+    }
+    h->timed[which].clear();
+    *total_ms = sum;
+    *launches = n;
+    return st;
 }
 
 // Sizes of the ABI structs, so bindings can check their layouts.

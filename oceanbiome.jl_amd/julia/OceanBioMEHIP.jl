@@ -1,0 +1,211 @@
+"""
+    OceanBioMEHIP
+
+Julia side of libobme_hip.so: a `HIPBiogeochemistry <: AbstractBiogeochemistry`
+that Oceananigans drives through the `Oceananigans.Biogeochemistry` hooks
+(src/OceanBioME.jl:122-169), with the light scan, tendencies, sinking and CO₂
+flux computed by the MI355X kernels through `ccall`.
+
+  - `required_biogeochemical_tracers` and `required_biogeochemical_auxiliary_fields`
+    return the wrapped model's names (NutrientsPlanktonDetritus.jl:69-74,
+    Light/2band.jl:160);
+  - `update_biogeochemical_state!` calls `obme_update_state` (PAR scan);
+  - `update_tendencies!` calls `obme_add_tendencies`, which ADDS the
+    biogeochemical transitions, minus the drift flux divergence and the top
+    CO₂ flux, into `model.timestepper.Gⁿ` (the pattern of
+    src/Sediments/tracer_coupling.jl:3-38);
+  - the per-cell functor returns zero and `biogeochemical_drift_velocity`
+    returns `nothing`, so Oceananigans adds nothing twice.
+
+Field data passed to the library is `parent(field)` (Oceananigans layout,
+halos included) of a ROCArray; the pointers are device pointers.
+"""
+module OceanBioMEHIP
+
+using Oceananigans
+using Oceananigans.Biogeochemistry: AbstractBiogeochemistry
+import Oceananigans.Biogeochemistry: required_biogeochemical_tracers,
+                                     required_biogeochemical_auxiliary_fields,
+                                     biogeochemical_auxiliary_fields,
+                                     update_biogeochemical_state!,
+                                     update_tendencies!,
+                                     biogeochemical_drift_velocity
+
+const libobme = get(ENV, "OBME_LIBRARY", joinpath(@__DIR__, "..", "oceanbiome", "libobme_hip.so"))
+
+const OBME_OK = Int32(0)
+const OBME_NPZD = Int32(1)
+const OBME_LOBSTER = Int32(2)
+const OBME_FP64 = Int32(0)
+const OBME_FP32_MIXED = Int32(1)
+const OBME_MAX_TRACERS = 32
+
+# ---- ABI structs (field order and padding as in include/obme.h) -------------
+
+struct ObmeNPDParams
+    values :: NTuple{30, Float64}   # nitrate_half_saturation … small_particle_fraction
+    light_limitation :: Int32
+    use_temperature_coefficient :: Int32
+    phytoplankton_mortality_formulation :: Int32
+    grazing_concentration_formulation :: Int32
+end
+
+struct ObmeTwoBandPAR
+    coefficients :: NTuple{8, Float64}   # kʳ, kᵇ, χʳ, χᵇ, eʳ, eᵇ, r, Rᶜₚ
+    surface_mode :: Int32
+    pad0 :: Int32
+    surface_value :: Float64
+    surface_array :: Ptr{Cvoid}
+end
+
+struct ObmeSinking
+    speed :: NTuple{OBME_MAX_TRACERS, Float64}
+    open_bottom :: Int32
+    scheme :: Int32
+    smoothing_distance :: Float64
+end
+
+struct ObmeCarbonSolver
+    max_iters :: Int32
+    pad0 :: Int32
+    atol :: Float64
+    damping :: Float64
+    armijo_constant :: Float64
+    min_damping :: Float64
+    initial_pH_guess :: Float64
+end
+
+struct ObmeCO2Exchange
+    enabled :: Int32
+    pad0 :: Int32
+    air_concentration :: Float64
+    wind_speed :: Float64
+    air_pressure :: Float64
+    schmidt :: NTuple{5, Float64}
+    transfer_velocity :: NTuple{3, Float64}
+end
+
+struct ObmeGrid
+    Nx :: Int32; Ny :: Int32; Nz :: Int32
+    Hx :: Int32; Hy :: Int32; Hz :: Int32
+    zc :: Ptr{Float64}
+    zf :: Ptr{Float64}
+    dz :: Ptr{Float64}
+end
+
+struct ObmeFieldSource
+    mode :: Int32
+    pad0 :: Int32
+    value :: Float64
+    profile :: Ptr{Float64}
+end
+
+struct ObmeConfig
+    model :: Int32
+    precision :: Int32
+    carbonate_system :: Int32
+    pad0 :: Int32
+    grid :: ObmeGrid
+    npd :: ObmeNPDParams
+    par :: ObmeTwoBandPAR
+    sinking :: ObmeSinking
+    solver :: ObmeCarbonSolver
+    co2 :: ObmeCO2Exchange
+    temperature :: ObmeFieldSource
+    salinity :: ObmeFieldSource
+end
+
+# layout check against the library (obme_abi_sizeof)
+function check_abi()
+    for (idx, T) in enumerate((ObmeConfig, ObmeGrid, ObmeNPDParams, ObmeTwoBandPAR, ObmeSinking,
+                               ObmeCarbonSolver, ObmeCO2Exchange, ObmeFieldSource))
+        n = ccall((:obme_abi_sizeof, libobme), Int64, (Int32,), idx - 1)
+        n == sizeof(T) || error("ABI mismatch for $T: library $n bytes, Julia $(sizeof(T))")
+    end
+    return nothing
+end
+
+last_error() = unsafe_string(ccall((:obme_last_error, libobme), Cstring, ()))
+check(status) = status == OBME_OK || error("libobme_hip: status $status: $(last_error())")
+
+# ---- the wrapped biogeochemistry ----------------------------------------------
+
+mutable struct HIPBiogeochemistry{B, F} <: AbstractBiogeochemistry
+    underlying :: B               # the OceanBioME model it replaces (names, parameters)
+    config :: Base.RefValue{ObmeConfig}
+    handle :: Ptr{Cvoid}
+    PAR :: F                      # CenterField owned by the wrapper
+    keep :: Vector{Any}           # host arrays referenced by `config`
+end
+
+"""
+    HIPBiogeochemistry(underlying, grid, config::ObmeConfig, PAR; keep = [])
+
+Creates the library handle for `config` (built from the OceanBioME model
+`underlying` by the caller, see the Python host package for the field-by-field
+mapping) and owns `PAR`.
+"""
+function HIPBiogeochemistry(underlying, grid, config::ObmeConfig, PAR; keep = Any[])
+    check_abi()
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    cfg = Ref(config)
+    check(ccall((:obme_create, libobme), Int32, (Ref{ObmeConfig}, Ref{Ptr{Cvoid}}), cfg, h))
+    bgc = HIPBiogeochemistry(underlying, cfg, h[], PAR, keep)
+    finalizer(b -> ccall((:obme_destroy, libobme), Int32, (Ptr{Cvoid},), b.handle), bgc)
+    return bgc
+end
+
+required_biogeochemical_tracers(b::HIPBiogeochemistry) = required_biogeochemical_tracers(b.underlying)
+required_biogeochemical_auxiliary_fields(::HIPBiogeochemistry) = (:PAR, )
+biogeochemical_auxiliary_fields(b::HIPBiogeochemistry) = (; PAR = b.PAR)
+biogeochemical_drift_velocity(::HIPBiogeochemistry, ::Val) = nothing
+
+# the per-cell functor: the library adds the transitions in update_tendencies!
+@inline (::HIPBiogeochemistry)(i, j, k, grid, ::Val, clock, fields) = zero(eltype(grid))
+
+device_pointer(field) = Ptr{Cvoid}(UInt(pointer(parent(field))))
+
+function tracer_pointers(b::HIPBiogeochemistry, tracers)
+    names = required_biogeochemical_tracers(b)
+    return Ptr{Cvoid}[device_pointer(tracers[name]) for name in names]
+end
+
+stream_pointer() = C_NULL   # the default stream of the task (AMDGPU's current stream may be passed instead)
+
+function update_biogeochemical_state!(b::HIPBiogeochemistry, model)
+    ptrs = tracer_pointers(b, model.tracers)
+    check(ccall((:obme_update_state, libobme), Int32,
+                (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Float64, Ptr{Cvoid}),
+                b.handle, ptrs, device_pointer(b.PAR), model.clock.time, stream_pointer()))
+    return nothing
+end
+
+function update_tendencies!(b::HIPBiogeochemistry, model)
+    ptrs = tracer_pointers(b, model.tracers)
+    G = tracer_pointers(b, model.timestepper.Gⁿ)
+    T = haskey(model.tracers, :T) && !(:T in required_biogeochemical_tracers(b)) ?
+        device_pointer(model.tracers.T) : C_NULL
+    S = haskey(model.tracers, :S) ? device_pointer(model.tracers.S) : C_NULL
+    check(ccall((:obme_add_tendencies, libobme), Int32,
+                (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Ptr{Cvoid}),
+                b.handle, ptrs, G, device_pointer(b.PAR), T, S, model.clock.time, stream_pointer()))
+    return nothing
+end
+
+"""
+    step!(b::HIPBiogeochemistry, tracers, nsteps, Δt; t₀ = 0)
+
+The library's fused RK3 stepper with frozen physics (update-state,
+tendencies and substep per stage), in place on the tracer fields.
+"""
+function step!(b::HIPBiogeochemistry, tracers, nsteps, Δt; t₀ = 0.0)
+    ptrs = tracer_pointers(b, tracers)
+    check(ccall((:obme_step, libobme), Int32,
+                (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Float64, Float64, Ptr{Cvoid}),
+                b.handle, ptrs, C_NULL, C_NULL, Int32(nsteps), Float64(Δt), Float64(t₀), stream_pointer()))
+    return nothing
+end
+
+export HIPBiogeochemistry, step!
+
+end # module

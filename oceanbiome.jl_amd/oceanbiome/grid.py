@@ -18,7 +18,8 @@ class RectilinearGrid:
     Vertical coordinates follow Oceananigans' regular spacing: faces
     z_F[k] = z_bottom + (k-1) Δz and centres z_F[k] + Δz/2, each the
     correctly rounded value of the exact rational position (Julia's range
-    arithmetic)."""
+    arithmetic).  A z given as Nz+1 face positions makes a stretched grid
+    with centres (z_F[k] + z_F[k+1]) / 2 and Δz = z_F[k+1] - z_F[k]."""
 
     def __init__(self, size, extent=None, z=None, halo=(3, 3, 3),
                  topology=("Periodic", "Periodic", "Bounded"), x=None, y=None):
@@ -41,11 +42,21 @@ class RectilinearGrid:
             self.x = tuple(float(v) for v in (x or (0.0, 1.0)))
             self.y = tuple(float(v) for v in (y or (0.0, 1.0)))
             self.z = tuple(float(v) for v in z)
-        z0, z1 = (Fraction(v) for v in self.z)
-        dz = (z1 - z0) / self.Nz
-        self.zf = np.array([float(z0 + k * dz) for k in range(self.Nz + 1)])
-        self.zc = np.array([float(z0 + k * dz + dz / 2) for k in range(self.Nz)])
-        self.dz = np.full(self.Nz, float(dz))
+        if len(self.z) == 2:
+            z0, z1 = (Fraction(v) for v in self.z)
+            dz = (z1 - z0) / self.Nz
+            self.zf = np.array([float(z0 + k * dz) for k in range(self.Nz + 1)])
+            self.zc = np.array([float(z0 + k * dz + dz / 2) for k in range(self.Nz)])
+            self.dz = np.full(self.Nz, float(dz))
+        else:
+            if len(self.z) != self.Nz + 1:
+                raise ValueError("z must be an interval or Nz+1 face positions")
+            zf = np.array(self.z, dtype=np.float64)
+            if np.any(np.diff(zf) <= 0):
+                raise ValueError("face positions must increase")
+            self.zf = zf
+            self.zc = (zf[:-1] + zf[1:]) / 2
+            self.dz = np.diff(zf)
         x0, x1 = (Fraction(v) for v in self.x)
         dx = (x1 - x0) / self.Nx
         self.xc = np.array([float(x0 + i * dx + dx / 2) for i in range(self.Nx)])

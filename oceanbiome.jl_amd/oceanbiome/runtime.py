@@ -47,6 +47,8 @@ def library():
         "obme_step": (C.c_int32, [vp, C.POINTER(vp), vp, vp, C.c_int32, C.c_double, C.c_double, vp]),
         "obme_co2_flux": (C.c_int32, [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp, C.c_int64, C.c_int32, vp]),
         "obme_abi_sizeof": (C.c_int64, [C.c_int32]),
+        "obme_set_kernel_timing": (C.c_int32, [vp, C.c_int32]),
+        "obme_kernel_timing": (C.c_int32, [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -59,7 +61,11 @@ def library():
 EXPORTED_SYMBOLS = (
     "obme_create", "obme_destroy", "obme_last_error", "obme_version", "obme_required_tracers",
     "obme_update_state", "obme_add_tendencies", "obme_step", "obme_co2_flux", "obme_abi_sizeof",
+    "obme_set_kernel_timing", "obme_kernel_timing",
 )
+
+KERNEL_SWEEP = 0      # OBME_KERNEL_SWEEP: the column sweep
+KERNEL_SURFACE = 1    # OBME_KERNEL_SURFACE: the per-column CO2 surface kernel
 
 
 def _check(status):
@@ -144,6 +150,18 @@ class HIPBiogeochemistry:
         self._check_fields(tracers)
         _check(library().obme_step(self.handle, _ptr_array(tracers), _ptr(T), _ptr(S), int(nsteps), float(dt),
                                    float(t0), _stream()))
+
+    # -- measurement --------------------------------------------------------------
+    def set_kernel_timing(self, enable=True):
+        """Bracket every kernel launch of this handle with HIP events."""
+        _check(library().obme_set_kernel_timing(self.handle, 1 if enable else 0))
+
+    def kernel_timing(self, which=KERNEL_SWEEP):
+        """(summed ms, launches) of kernel `which` since the previous call."""
+        ms = C.c_double()
+        n = C.c_int64()
+        _check(library().obme_kernel_timing(self.handle, int(which), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def co2_flux(self, DIC, Alk, T, S, u10=None, pCO2_air=None, mode=0, pH=None, flux=None):
         """pH (free scale) and CO2 flux on point sets of float64 CUDA tensors."""

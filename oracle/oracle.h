@@ -90,6 +90,13 @@ void orc_two_band_column(const obme_two_band_par* par, int Nz, const double* zc,
                          const double* zf, const double* P, double PAR0, double* PAR);
 double orc_default_surface_PAR(double t);
 
+/* ---- sinking ---- */
+/* w on the Nz+1 faces of a column (bottom face first), Utils/sinking_velocity_fields.jl:10-35 */
+void orc_sinking_profile(const obme_sinking* s, double speed, int Nz, double* w);
+/* (F(k+1) - F(k)) / dz(k) with F = w c(upwind); the tendency is minus this */
+void orc_sinking_divergence(int scheme, int Nz, const double* w, const double* c, const double* dz,
+                            double* div);
+
 /* ---- one column, nsteps of RK3 ---- */
 typedef struct orc_column_env {
     const obme_config* cfg;

@@ -30,6 +30,12 @@ def build(force=False):
     subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
 
 
+class Constants(C.Structure):
+    """orc_constants: equilibrium constants at (T, S[, P])."""
+    _fields_ = [(n, C.c_double) for n in ("K0", "K1", "K2", "KB", "KW", "KS", "KF", "KP1", "KP2", "KP3",
+                                          "KSi", "Is")]
+
+
 _lib = None
 
 
@@ -65,6 +71,9 @@ def lib():
         "orc_npd_ntracers": (I, [I, I]),
         "orc_two_band_column": (None, [C.POINTER(abi.TwoBandPAR), I, DP, DP, DP, D, DP]),
         "orc_default_surface_PAR": (D, [D]),
+        "orc_sinking_profile": (None, [C.POINTER(abi.Sinking), D, I, DP]),
+        "orc_sinking_divergence": (None, [I, I, DP, DP, DP, DP]),
+        "orc_equilibrium_constants": (None, [D, D, I, D, C.POINTER(Constants)]),
         "orc_grid_run": (I, [C.POINTER(abi.Config), C.POINTER(DP), DP, DP, DP, I, D, D, I, I]),
         "orc_grid_run_columns": (I, [C.POINTER(abi.Config), C.POINTER(DP), DP, DP, DP, I, D, D,
                                      C.POINTER(I), C.POINTER(I), I, I, I]),
@@ -112,6 +121,59 @@ def pow_int(x, n):
 
 def density(T, S, Pbar):
     return lib().orc_teos10_density(T, S, Pbar)
+
+
+def equilibrium_constants(Tk, S, P=None):
+    out = Constants()
+    lib().orc_equilibrium_constants(Tk, S, 0 if P is None else 1, 0.0 if P is None else P, C.byref(out))
+    return out
+
+
+def ksp_calcite(Tk, S, P=None):
+    return lib().orc_ksp_calcite(Tk, S, 0 if P is None else 1, 0.0 if P is None else P)
+
+
+def ksp_aragonite(Tk, S, P=None):
+    return lib().orc_ksp_aragonite(Tk, S, 0 if P is None else 1, 0.0 if P is None else P)
+
+
+def virial_B(Tk):
+    return lib().orc_virial_B(Tk)
+
+
+def virial_delta(Tk):
+    return lib().orc_virial_delta(Tk)
+
+
+def schmidt_co2(gas, T):
+    return lib().orc_schmidt_co2(C.byref(gas), T)
+
+
+def transfer_velocity(gas, u10, T, S):
+    return lib().orc_transfer_velocity(C.byref(gas), u10, T, S)
+
+
+def default_surface_PAR(t):
+    return lib().orc_default_surface_PAR(t)
+
+
+def sinking_profile(speed, Nz, open_bottom=True, smoothing_distance=2.0):
+    s = abi.Sinking()
+    s.open_bottom = 1 if open_bottom else 0
+    s.smoothing_distance = smoothing_distance
+    w = np.zeros(Nz + 1)
+    lib().orc_sinking_profile(C.byref(s), speed, Nz, _dp(w))
+    return w
+
+
+def sinking_divergence(w, c, dz, scheme=None):
+    Nz = len(c)
+    out = np.zeros(Nz)
+    lib().orc_sinking_divergence(abi.OBME_UPWIND1 if scheme is None else scheme, Nz,
+                                 _dp(np.ascontiguousarray(w, dtype=np.float64)),
+                                 _dp(np.ascontiguousarray(c, dtype=np.float64)),
+                                 _dp(np.ascontiguousarray(dz, dtype=np.float64)), _dp(out))
+    return out
 
 
 def pressure_correction(which, Tk, P):

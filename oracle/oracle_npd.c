@@ -257,6 +257,17 @@ static void sinking_divergence(int scheme, int Nz, const double* w, const double
     }
 }
 
+void orc_sinking_profile(const obme_sinking* s, double speed, int Nz, double* w)
+{
+    sinking_profile(s, speed, Nz, w);
+}
+
+void orc_sinking_divergence(int scheme, int Nz, const double* w, const double* c, const double* dz,
+                            double* div)
+{
+    sinking_divergence(scheme, Nz, w, c, dz, div);
+}
+
 /* ---------------- column stepper ---------------- */
 
 static double surface_PAR(const orc_column_env* env, double t)
