@@ -46,6 +46,11 @@ def parse():
     p.add_argument("--no-check", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--precisions", default="fp32,fp64")
+    p.add_argument("--workload", default="lobster", choices=["lobster", "columns", "carbonate"],
+                   help="lobster: the headline LOBSTER+DIC/Alk step; columns: the ragged NPZD column "
+                        "ensemble; carbonate: pH and CO2 flux on point sets")
+    p.add_argument("--columns", type=int, default=8192)
+    p.add_argument("--points", type=int, default=1 << 24)
     return p.parse_args()
 
 
@@ -111,6 +116,10 @@ def main():
               f"shards by column tiles with no exchange, and this build measures one GPU's share.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "columns":
+        return bench_columns(args)
+    if args.workload == "carbonate":
+        return bench_carbonate(args)
     import torch
     from oceanbiome import abi, workloads
     from oceanbiome.runtime import HIPBiogeochemistry
@@ -233,6 +242,119 @@ def main():
         "errors_vs_oracle": errors,
     }
     print(json.dumps(line))
+
+
+def bench_columns(args):
+    """The ragged NPZD column ensemble: 8192 columns, Nz ~ U{16..128},
+    per-column parameters and surface PAR, 500 steps of 60 s, FP32-mixed and
+    FP64; packing autotuned first; every column checked against the oracle."""
+    import torch
+    from oceanbiome import abi, workloads
+    from oceanbiome.runtime import HIPBatch
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    wl = workloads.npzd_columns(n_columns=args.columns)
+    off = wl.extra["offsets"]
+    ncell = int(off[-1])
+    steps = args.steps if args.steps != 200 else wl.nsteps
+    nthreads = cpu_threads(args.cpu_threads)
+    t0 = time.perf_counter()
+    ref = [a.copy() for a in wl.tracers]
+    O.batch_run(wl.bgc.config(abi.OBME_FP64), off, list(wl.extra["params"]), wl.extra["dz"], wl.extra["T"], ref,
+                steps, wl.dt, nthreads=nthreads)
+    cpu_s = time.perf_counter() - t0
+    log(f"oracle: {ncell} cells x {steps} steps in {cpu_s:.1f} s on {nthreads} threads")
+    res = {}
+    for prec in args.precisions.split(","):
+        b = HIPBatch(wl.bgc, off, wl.extra["params"], wl.extra["dz"], wl.extra["T"], prec)
+        tr = [torch.from_numpy(a).to(device="cuda", dtype=b.dtype) for a in wl.tracers]
+        b.autotune(tr, 20, wl.dt)
+        plan = b.describe()
+        b.step([t.clone() for t in tr], args.warmup, wl.dt)
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        b.step(tr, steps, wl.dt)
+        e.record()
+        torch.cuda.synchronize()
+        secs = s.elapsed_time(e) / 1e3
+        out = [t.double().cpu().numpy() for t in tr]
+        worst = 0.0
+        tracer_err = []
+        for a, r in zip(out, ref):
+            tracer_err.append(float(np.linalg.norm(a - r) / np.linalg.norm(r)))
+        for c in range(len(off) - 1):
+            sl = slice(off[c], off[c + 1])
+            for a, r in zip(out, ref):
+                nr = np.linalg.norm(r[sl])
+                if nr > 0:
+                    worst = max(worst, float(np.linalg.norm(a[sl] - r[sl]) / nr))
+        res[prec] = {"updates_per_s": 4 * ncell * steps / secs, "ms_per_step": secs / steps * 1e3, "plan": plan,
+                     "rel_l2_per_tracer": tracer_err, "worst_column_rel_l2": worst}
+        log(f"{prec}: {res[prec]['updates_per_s']:.3e} upd/s, {plan}, tracer errors {tracer_err}, worst column {worst:.2e}")
+        del b, tr
+    head = res.get("fp32") or res["fp64"]
+    print(json.dumps({
+        "metric": "tracer-cell updates/s (NPZD column ensemble)", "value": head["updates_per_s"],
+        "unit": "tracer-cell updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if "fp32" in res else "f64", "data": "synthetic (seeded)",
+        "config": {"workload": wl.name, "columns": len(off) - 1, "cells": ncell, "dt": wl.dt},
+        "cpu_baseline": {"value": 4 * ncell * steps / cpu_s, "unit": "tracer-cell updates/s", "cores": nthreads,
+                         "kind": "port", "sample": f"the whole ensemble, {steps} steps, FP64 oracle"},
+        "results": res}))
+
+
+def bench_carbonate(args):
+    """pH and CO2 flux on 2^24 points: the reference damped Newton in FP64
+    (mode 0) and an FP32 guess refined by FP64 Newton (mode 1); errors against
+    the oracle on a 65536-point sample."""
+    import torch
+    from oceanbiome import abi, workloads
+    from oceanbiome.runtime import HIPBiogeochemistry
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    pts = workloads.carbonate_points(args.points)
+    wl = workloads.lobster_3d(4, 4, 4, nsteps=1)
+    dev = HIPBiogeochemistry(wl.bgc, "fp64")
+    cfg = wl.bgc.config(abi.OBME_FP64)
+    nthreads = cpu_threads(args.cpu_threads)
+    ns = min(65536, args.points)
+    t0 = time.perf_counter()
+    pH_ref, F_ref = O.co2_flux_batch(cfg.co2, cfg.solver, pts["DIC"][:ns], pts["Alk"][:ns], pts["T"][:ns],
+                                     pts["S"][:ns], pts["u10"][:ns], pts["pCO2_air"], nthreads=nthreads)
+    cpu_s = time.perf_counter() - t0
+    dargs = [torch.from_numpy(pts[k]).cuda() for k in ("DIC", "Alk", "T", "S", "u10")]
+    pH = torch.empty_like(dargs[0])
+    F = torch.empty_like(dargs[0])
+    res = {}
+    for mode in (0, 1):
+        dev.co2_flux(*dargs, pCO2_air=pts["pCO2_air"], mode=mode, pH=pH, flux=F)
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = max(1, args.steps // 50)
+        s.record()
+        for _ in range(reps):
+            dev.co2_flux(*dargs, pCO2_air=pts["pCO2_air"], mode=mode, pH=pH, flux=F)
+        e.record()
+        torch.cuda.synchronize()
+        ms = s.elapsed_time(e) / reps
+        p = pH[:ns].cpu().numpy()
+        f = F[:ns].cpu().numpy()
+        res[f"mode{mode}"] = {"ms": ms, "points_per_s": args.points / (ms * 1e-3),
+                              "pH_max_abs_err": float(np.max(np.abs(p - pH_ref))),
+                              "flux_rel_l2": float(np.linalg.norm(f - F_ref) / np.linalg.norm(F_ref)),
+                              "flux_max_rel": float(np.max(np.abs(f - F_ref) / np.maximum(np.abs(F_ref), 1e-30)))}
+        log(f"mode {mode}: {ms:.3f} ms, {res[f'mode{mode}']}")
+    print(json.dumps({
+        "metric": "surface points/s (pH solve + CO2 flux)", "value": res["mode1"]["points_per_s"],
+        "unit": "points/s", "n_gpus": 1, "steps": max(1, args.steps // 50), "warmup": 1,
+        "ms_per_step": res["mode1"]["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (seeded)",
+        "config": {"workload": f"carbonate_points_{args.points}"},
+        "cpu_baseline": {"value": ns / cpu_s, "unit": "points/s", "cores": nthreads, "kind": "port",
+                         "sample": f"{ns} points, FP64 oracle"},
+        "results": res}))
 
 
 if __name__ == "__main__":

@@ -279,6 +279,36 @@ enum {
     OBME_BP_COUNT
 };
 
+typedef struct obme_batch obme_batch;
+
+/* An ensemble of independent NPZD water columns (the BoxModel setting of
+ * src/BoxModel/boxmodel.jl and timesteppers.jl:30-92, one column per member):
+ * column c holds cells [offsets[c], offsets[c+1]) of every tracer array,
+ * k = 0 at the bottom, uniform thickness dz, surface at z = 0.  `cfg`
+ * supplies the model (OBME_NPZD), precision, TwoBand coefficients, sinking
+ * (open bottom, upwind) and every parameter that `params` leaves NULL;
+ * params->fields[OBME_BP_*] are device arrays of n_columns doubles (read once
+ * here).  Temperature is uniform.  `offsets` is a host array of n_columns+1. */
+obme_status obme_batch_create(const obme_config* cfg, int32_t n_columns, const int32_t* offsets, double dz,
+                              const obme_batch_params* params, double temperature, obme_batch** out);
+obme_status obme_batch_destroy(obme_batch* b);
+/* nsteps RK3 steps of every column, in place on tracers[0..3] = N, P, Z, D
+ * (device arrays of offsets[n_columns] elements, double for OBME_FP64, float
+ * for OBME_FP32_MIXED).  Each launch advances steps_per_launch steps (0: all
+ * of them) with the column state in registers; several launches per call are
+ * recorded once into a hipGraph and replayed. */
+obme_status obme_batch_step(obme_batch* b, void* const* tracers, int32_t nsteps, double dt, double t0,
+                            void* stream);
+/* Column packing: plan = index of a set of tile shapes (columns go to the
+ * smallest tile that holds them), steps_per_launch as above. */
+obme_status obme_batch_configure(obme_batch* b, int32_t plan, int32_t steps_per_launch);
+/* Times every packing plan and steps-per-launch choice for trial_steps steps
+ * on a scratch copy of `tracers` (left unchanged) and keeps the fastest. */
+obme_status obme_batch_autotune(obme_batch* b, void* const* tracers, int32_t trial_steps, double dt,
+                                void* stream);
+/* Text description of the current packing; returns its length or < 0. */
+int32_t obme_batch_describe(const obme_batch* b, char* buf, int32_t cap);
+
 /* ---- kernel timing ------------------------------------------------------ */
 
 /* No counterpart in the reference: measurement instrumentation.  While
@@ -294,7 +324,7 @@ obme_status obme_kernel_timing(obme_handle* h, int32_t which, double* total_ms, 
 /* Size in bytes of ABI struct `which` as compiled into the library
  * (0 obme_config, 1 obme_grid, 2 obme_npd_params, 3 obme_two_band_par,
  * 4 obme_sinking, 5 obme_carbon_solver, 6 obme_co2_exchange,
- * 7 obme_field_source); -1 for an unknown index.  No reference
+ * 7 obme_field_source, 8 obme_batch_params); -1 for an unknown index.  No reference
  * counterpart: it lets a binding check its struct layouts. */
 int64_t obme_abi_sizeof(int32_t which);
 

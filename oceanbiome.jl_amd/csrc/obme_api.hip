@@ -9,10 +9,12 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "obme_batch_kernels.h"
 #include "obme_npd_kernels.h"
 
 using namespace obme;
@@ -601,6 +603,427 @@ extern "C" obme_status obme_kernel_timing(obme_handle* h, int32_t which, double*
     return st;
 }
 
+// ------------------------------------------------------------------ batched ensemble
+
+template <typename FT, int L, int CPW, int NW>
+__global__ void __launch_bounds__(NW * 64) k_batch(BatchArgs<FT> a)
+{
+    batch_columns<FT, L, CPW, NW>(a);
+}
+
+// Tile shapes of the ensemble kernel: capacity L*64/CPW levels per column.
+struct BatchTile {
+    int L, CPW, NW;
+};
+static const BatchTile BATCH_TILES[] = {
+    {1, 4, 4}, {2, 4, 4}, {1, 2, 4}, {2, 2, 4}, {1, 1, 4}, {2, 1, 4}, {4, 2, 4}, {4, 4, 4},
+};
+static int batch_capacity(int t) { return BATCH_TILES[t].L * 64 / BATCH_TILES[t].CPW; }
+// Packing plans: the tile shapes a plan sorts columns into, by capacity.
+static const std::vector<std::vector<int>> BATCH_PLANS = {
+    {5}, {3, 5}, {1, 3, 5}, {0, 1, 3, 5}, {2, 4, 5}, {7, 6}, {0, 2, 4, 5},
+};
+
+struct obme_batch {
+    obme_config cfg;
+    int prec;
+    int n_columns;
+    std::vector<int> offsets;
+    double dz, T;
+    int* d_offsets = nullptr;
+    FastParams* d_fpar = nullptr;
+    NPDParams<double>* d_dpar = nullptr;
+    BatchColumn* d_bcol = nullptr;
+    int plan = -1;
+    int steps_per_launch = 0;             // 0: every step of a call in one launch per class
+    std::vector<int> class_tile, class_n; // per class of the plan
+    std::vector<int*> class_cols;
+    // captured multi-launch step
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int g_nsteps = -1, g_spl = -1, g_plan = -1;
+    double g_dt = 0.0;
+    void* g_ptrs[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+static void batch_free_classes(obme_batch* b)
+{
+    for (int* p : b->class_cols)
+        if (p) (void)hipFree(p);
+    b->class_cols.clear();
+    b->class_tile.clear();
+    b->class_n.clear();
+}
+
+static void batch_drop_graph(obme_batch* b)
+{
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    b->g_nsteps = -1;
+}
+
+// Sorts the columns into the classes of plan p (smallest capacity that holds
+// the column), deepest columns first within a class.
+static obme_status batch_apply_plan(obme_batch* b, int p)
+{
+    if (p < 0 || p >= (int)BATCH_PLANS.size()) return fail(OBME_ERR_INVALID, "unknown packing plan");
+    std::vector<int> tiles = BATCH_PLANS[p];
+    std::sort(tiles.begin(), tiles.end(), [](int x, int y) { return batch_capacity(x) < batch_capacity(y); });
+    std::vector<std::vector<int>> lists(tiles.size());
+    for (int c = 0; c < b->n_columns; c++) {
+        const int nz = b->offsets[c + 1] - b->offsets[c];
+        int q = -1;
+        for (size_t t = 0; t < tiles.size() && q < 0; t++)
+            if (nz <= batch_capacity(tiles[t])) q = (int)t;
+        if (q < 0) return fail(OBME_ERR_UNSUPPORTED, "a column is deeper than the plan's largest tile");
+        lists[q].push_back(c);
+    }
+    batch_free_classes(b);
+    batch_drop_graph(b);
+    for (size_t t = 0; t < tiles.size(); t++) {
+        std::vector<int>& l = lists[t];
+        if (l.empty()) continue;
+        std::stable_sort(l.begin(), l.end(), [&](int x, int y) {
+            return b->offsets[x + 1] - b->offsets[x] > b->offsets[y + 1] - b->offsets[y];
+        });
+        int* d = nullptr;
+        HIP_TRY(hipMalloc((void**)&d, l.size() * sizeof(int)));
+        HIP_TRY(hipMemcpy(d, l.data(), l.size() * sizeof(int), hipMemcpyHostToDevice));
+        b->class_cols.push_back(d);
+        b->class_tile.push_back(tiles[t]);
+        b->class_n.push_back((int)l.size());
+    }
+    b->plan = p;
+    return OBME_OK;
+}
+
+template <typename FT>
+static BatchArgs<FT> batch_args(const obme_batch* b, void* const* tracers, double dt)
+{
+    BatchArgs<FT> a;
+    std::memset(&a, 0, sizeof(a));
+    for (int n = 0; n < 4; n++) a.U[n] = (FT*)tracers[n];
+    a.offsets = b->d_offsets;
+    a.fpar = b->d_fpar;
+    a.dpar = b->d_dpar;
+    a.bcol = b->d_bcol;
+    a.dtf = (FT)dt;
+    a.dz = (FT)b->dz;
+    a.invdz = (FT)(1.0 / b->dz);
+    rk3_products(dt, a.rk_a, a.rk_b);
+    const obme_two_band_par& par = b->cfg.par;
+    a.kr = par.water_red_attenuation;
+    a.kb = par.water_blue_attenuation;
+    a.xr = par.chlorophyll_red_attenuation;
+    a.xb = par.chlorophyll_blue_attenuation;
+    a.er = par.chlorophyll_red_exponent;
+    a.eb = par.chlorophyll_blue_exponent;
+    a.Rr = par.phytoplankton_chlorophyll_ratio / par.pigment_ratio;
+    const double log2e = 1.4426950408889634;
+    const double lRr = std::log2(a.Rr);
+    a.kr2 = (float)(a.kr * log2e);
+    a.kb2 = (float)(a.kb * log2e);
+    a.er_f = (float)a.er;
+    a.eb_f = (float)a.eb;
+    a.cr = (float)(a.er * lRr);
+    a.cb = (float)(a.eb * lRr);
+    a.xr2h = (float)(a.xr * log2e * b->dz / 2);
+    a.xb2h = (float)(a.xb * log2e * b->dz / 2);
+    return a;
+}
+
+template <typename FT>
+static obme_status batch_launch_class(const obme_batch* b, int q, BatchArgs<FT> a, int nsteps, hipStream_t s)
+{
+    a.cols = b->class_cols[q];
+    a.ncols = b->class_n[q];
+    a.nsteps = nsteps;
+    const BatchTile& t = BATCH_TILES[b->class_tile[q]];
+    const int per_wg = t.NW * t.CPW;
+    dim3 grid((a.ncols + per_wg - 1) / per_wg), block(t.NW * 64);
+    switch (b->class_tile[q]) {
+    case 0: hipLaunchKernelGGL((k_batch<FT, 1, 4, 4>), grid, block, 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_batch<FT, 2, 4, 4>), grid, block, 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_batch<FT, 1, 2, 4>), grid, block, 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_batch<FT, 2, 2, 4>), grid, block, 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_batch<FT, 1, 1, 4>), grid, block, 0, s, a); break;
+    case 5: hipLaunchKernelGGL((k_batch<FT, 2, 1, 4>), grid, block, 0, s, a); break;
+    case 6: hipLaunchKernelGGL((k_batch<FT, 4, 2, 4>), grid, block, 0, s, a); break;
+    case 7: hipLaunchKernelGGL((k_batch<FT, 4, 4, 4>), grid, block, 0, s, a); break;
+    default: return fail(OBME_ERR_INVALID, "bad batch tile");
+    }
+    HIP_TRY(hipGetLastError());
+    return OBME_OK;
+}
+
+// All launches of nsteps steps: per chunk of steps_per_launch steps, one
+// launch per class.
+template <typename FT>
+static obme_status batch_enqueue(const obme_batch* b, void* const* tracers, int nsteps, double dt, hipStream_t s)
+{
+    BatchArgs<FT> a = batch_args<FT>(b, tracers, dt);
+    const int chunk = (b->steps_per_launch > 0) ? b->steps_per_launch : nsteps;
+    for (int done = 0; done < nsteps; done += chunk) {
+        const int n = std::min(chunk, nsteps - done);
+        for (size_t q = 0; q < b->class_tile.size(); q++) {
+            obme_status r = batch_launch_class<FT>(b, (int)q, a, n, s);
+            if (r != OBME_OK) return r;
+        }
+    }
+    return OBME_OK;
+}
+
+static obme_status batch_enqueue_any(const obme_batch* b, void* const* tracers, int nsteps, double dt, hipStream_t s)
+{
+    return (b->prec == OBME_FP64) ? batch_enqueue<double>(b, tracers, nsteps, dt, s)
+                                  : batch_enqueue<float>(b, tracers, nsteps, dt, s);
+}
+
+extern "C" obme_status obme_batch_destroy(obme_batch* b)
+{
+    if (!b) return OBME_OK;
+    batch_drop_graph(b);
+    batch_free_classes(b);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    if (b->d_offsets) (void)hipFree(b->d_offsets);
+    if (b->d_fpar) (void)hipFree(b->d_fpar);
+    if (b->d_dpar) (void)hipFree(b->d_dpar);
+    if (b->d_bcol) (void)hipFree(b->d_bcol);
+    delete b;
+    return OBME_OK;
+}
+
+extern "C" obme_status obme_batch_create(const obme_config* cfg, int32_t n_columns, const int32_t* offsets, double dz,
+                                         const obme_batch_params* params, double temperature, obme_batch** out)
+{
+    if (!cfg || !offsets || !params || !out) return fail(OBME_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (cfg->model != OBME_NPZD) return fail(OBME_ERR_UNSUPPORTED, "the batched ensemble runs NPZD columns");
+    if (cfg->precision != OBME_FP64 && cfg->precision != OBME_FP32_MIXED)
+        return fail(OBME_ERR_INVALID, "unknown precision");
+    if (n_columns < 1) return fail(OBME_ERR_INVALID, "no columns");
+    if (!(dz > 0)) return fail(OBME_ERR_INVALID, "dz must be positive");
+    if (!cfg->sinking.open_bottom) return fail(OBME_ERR_UNSUPPORTED, "the batched ensemble has open-bottom sinking");
+    if (cfg->sinking.scheme != OBME_UPWIND1) return fail(OBME_ERR_UNSUPPORTED, "only upwind sinking on the GPU");
+    if (offsets[0] != 0) return fail(OBME_ERR_INVALID, "offsets[0] must be 0");
+    for (int c = 0; c < n_columns; c++) {
+        const int nz = offsets[c + 1] - offsets[c];
+        if (nz < 1) return fail(OBME_ERR_INVALID, "a column has no cells");
+        if (nz > 128) return fail(OBME_ERR_UNSUPPORTED, "columns deeper than 128 cells");
+    }
+    if ((long long)offsets[n_columns] * 8 >= (1ll << 31)) return fail(OBME_ERR_UNSUPPORTED, "ensemble too large");
+
+    obme_batch* b = new obme_batch();
+    b->cfg = *cfg;
+    b->cfg.grid.zc = b->cfg.grid.zf = b->cfg.grid.dz = nullptr;
+    b->prec = cfg->precision;
+    b->n_columns = n_columns;
+    b->offsets.assign(offsets, offsets + n_columns + 1);
+    b->dz = dz;
+    b->T = temperature;
+    // per-column parameters: device SoA (double) -> host -> derived constants
+    std::vector<std::vector<double>> P(OBME_BP_COUNT);
+    for (int f = 0; f < OBME_BP_COUNT; f++) {
+        if (!params->fields[f]) continue;
+        P[f].resize(n_columns);
+        if (hipMemcpy(P[f].data(), params->fields[f], n_columns * sizeof(double), hipMemcpyDeviceToHost) !=
+            hipSuccess) {
+            obme_batch_destroy(b);
+            return fail(OBME_ERR_HIP, "reading the per-column parameters");
+        }
+    }
+    std::vector<FastParams> fpar(n_columns);
+    std::vector<NPDParams<double>> dpar(n_columns);
+    std::vector<BatchColumn> bcol(n_columns);
+    for (int c = 0; c < n_columns; c++) {
+        obme_npd_params q = cfg->npd;
+        auto get = [&](int f, double dflt) { return P[f].empty() ? dflt : P[f][c]; };
+        q.nitrate_half_saturation = get(OBME_BP_NITRATE_HALF_SATURATION, q.nitrate_half_saturation);
+        q.light_half_saturation = get(OBME_BP_LIGHT_HALF_SATURATION, q.light_half_saturation);
+        q.phytoplankton_maximum_growth_rate = get(OBME_BP_MAX_GROWTH_RATE, q.phytoplankton_maximum_growth_rate);
+        q.phytoplankton_exudation_fraction = get(OBME_BP_EXUDATION_FRACTION, q.phytoplankton_exudation_fraction);
+        q.ammonia_fraction_of_exudate = get(OBME_BP_AMMONIA_FRACTION_OF_EXUDATE, q.ammonia_fraction_of_exudate);
+        q.temperature_coefficient = get(OBME_BP_TEMPERATURE_COEFFICIENT, q.temperature_coefficient);
+        q.phytoplankton_mortality_rate = get(OBME_BP_PHYTO_MORTALITY_RATE, q.phytoplankton_mortality_rate);
+        q.zooplankton_mortality_rate = get(OBME_BP_ZOO_MORTALITY_RATE, q.zooplankton_mortality_rate);
+        q.zooplankton_excretion_rate = get(OBME_BP_ZOO_EXCRETION_RATE, q.zooplankton_excretion_rate);
+        q.phytoplankton_solid_waste_fraction =
+            get(OBME_BP_PHYTO_SOLID_WASTE_FRACTION, q.phytoplankton_solid_waste_fraction);
+        q.excretion_inorganic_fraction = get(OBME_BP_EXCRETION_INORGANIC_FRACTION, q.excretion_inorganic_fraction);
+        q.preference_for_phytoplankton = get(OBME_BP_PREFERENCE_FOR_PHYTO, q.preference_for_phytoplankton);
+        q.maximum_grazing_rate = get(OBME_BP_MAX_GRAZING_RATE, q.maximum_grazing_rate);
+        q.grazing_half_saturation = get(OBME_BP_GRAZING_HALF_SATURATION, q.grazing_half_saturation);
+        q.zooplankton_assimilation_fraction = get(OBME_BP_ZOO_ASSIMILATION_FRACTION, q.zooplankton_assimilation_fraction);
+        q.remineralisation_rate = get(OBME_BP_REMINERALISATION_RATE, q.remineralisation_rate);
+        q.small_particle_fraction = get(OBME_BP_SMALL_PARTICLE_FRACTION, q.small_particle_fraction);
+        fpar[c] = make_fast_params(q);
+        dpar[c] = convert_params<double>(q);
+        BatchColumn& bc = bcol[c];
+        bc.PAR0 = get(OBME_BP_SURFACE_PAR, cfg->par.surface_value);
+        bc.wP = -get(OBME_BP_P_SINKING_SPEED, cfg->sinking.speed[1]);
+        bc.wD = -get(OBME_BP_D_SINKING_SPEED, cfg->sinking.speed[4]);
+        // Q10^(T/10) (plankton.jl:210-214) at the ensemble's uniform temperature
+        bc.Lt = q.use_temperature_coefficient ? std::pow(q.temperature_coefficient, temperature / 10) : 1.0;
+        if (bc.wP > 0 || bc.wD > 0) {
+            obme_batch_destroy(b);
+            return fail(OBME_ERR_INVALID, "negative sinking speed");
+        }
+    }
+    auto up = [&](void** d, const void* h, size_t bytes) -> bool {
+        return hipMalloc(d, bytes) == hipSuccess && hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    bool ok = up((void**)&b->d_offsets, b->offsets.data(), b->offsets.size() * sizeof(int)) &&
+              up((void**)&b->d_bcol, bcol.data(), bcol.size() * sizeof(BatchColumn));
+    if (ok && b->prec == OBME_FP64) ok = up((void**)&b->d_dpar, dpar.data(), dpar.size() * sizeof(NPDParams<double>));
+    if (ok && b->prec != OBME_FP64) ok = up((void**)&b->d_fpar, fpar.data(), fpar.size() * sizeof(FastParams));
+    if (!ok) {
+        obme_batch_destroy(b);
+        return fail(OBME_ERR_NOMEM, "batch buffers");
+    }
+    obme_status st = batch_apply_plan(b, 1);
+    if (st != OBME_OK) st = batch_apply_plan(b, 0);
+    if (st != OBME_OK) {
+        obme_batch_destroy(b);
+        return st;
+    }
+    *out = b;
+    return OBME_OK;
+}
+
+extern "C" obme_status obme_batch_configure(obme_batch* b, int32_t plan, int32_t steps_per_launch)
+{
+    if (!b) return fail(OBME_ERR_INVALID, "null handle");
+    if (steps_per_launch < 0) return fail(OBME_ERR_INVALID, "negative steps_per_launch");
+    if (plan != b->plan) {
+        obme_status st = batch_apply_plan(b, plan);
+        if (st != OBME_OK) return st;
+    }
+    if (steps_per_launch != b->steps_per_launch) batch_drop_graph(b);
+    b->steps_per_launch = steps_per_launch;
+    return OBME_OK;
+}
+
+extern "C" obme_status obme_batch_step(obme_batch* b, void* const* tracers, int32_t nsteps, double dt, double t0,
+                                       void* stream)
+{
+    (void)t0;  // surface PAR is constant per column
+    if (!b || !tracers) return fail(OBME_ERR_INVALID, "null argument");
+    for (int n = 0; n < 4; n++)
+        if (!tracers[n]) return fail(OBME_ERR_INVALID, "null tracer pointer");
+    if (nsteps < 0) return fail(OBME_ERR_INVALID, "negative nsteps");
+    if (nsteps == 0) return OBME_OK;
+    hipStream_t s = (hipStream_t)stream;
+    const bool multi = b->steps_per_launch > 0 && b->steps_per_launch < nsteps;
+    if (!multi) return batch_enqueue_any(b, tracers, nsteps, dt, s);
+    // several launches per call: recorded once into a hipGraph per
+    // (nsteps, steps_per_launch, dt, tracers, plan) and replayed
+    const bool same = b->graph && b->g_nsteps == nsteps && b->g_spl == b->steps_per_launch && b->g_dt == dt &&
+                      b->g_plan == b->plan && std::memcmp(b->g_ptrs, tracers, sizeof(b->g_ptrs)) == 0;
+    if (!same) {
+        batch_drop_graph(b);
+        if (!b->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+        obme_status r = batch_enqueue_any(b, tracers, nsteps, dt, b->cap_stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+        if (r != OBME_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return r;
+        }
+        if (e != hipSuccess) return fail(OBME_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            b->graph = nullptr;
+            return fail(OBME_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        b->g_nsteps = nsteps;
+        b->g_spl = b->steps_per_launch;
+        b->g_dt = dt;
+        b->g_plan = b->plan;
+        std::memcpy(b->g_ptrs, tracers, sizeof(b->g_ptrs));
+    }
+    HIP_TRY(hipGraphLaunch(b->graph, s));
+    return OBME_OK;
+}
+
+// Times each packing plan and steps-per-launch choice on a scratch copy of
+// the state (the caller's tracers are only read) and keeps the fastest.
+extern "C" obme_status obme_batch_autotune(obme_batch* b, void* const* tracers, int32_t trial_steps, double dt,
+                                           void* stream)
+{
+    if (!b || !tracers) return fail(OBME_ERR_INVALID, "null argument");
+    if (trial_steps < 1) return fail(OBME_ERR_INVALID, "trial_steps must be positive");
+    hipStream_t s = (hipStream_t)stream;
+    const size_t es = (b->prec == OBME_FP64) ? 8 : 4;
+    const size_t bytes = (size_t)b->offsets[b->n_columns] * es;
+    void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    obme_status st = OBME_OK;
+    auto cleanup = [&]() {
+        for (void* p : scratch)
+            if (p) (void)hipFree(p);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+    };
+    for (int n = 0; n < 4 && st == OBME_OK; n++)
+        if (hipMalloc(&scratch[n], bytes) != hipSuccess) st = fail(OBME_ERR_NOMEM, "autotune scratch");
+    if (st == OBME_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        st = fail(OBME_ERR_HIP, "autotune events");
+    if (st != OBME_OK) {
+        cleanup();
+        return st;
+    }
+    const int keep_plan = b->plan, keep_spl = b->steps_per_launch;
+    int best_plan = keep_plan, best_spl = keep_spl;
+    float best_ms = 1e30f;
+    std::vector<int> spls = {0, 1, 10};
+    for (int p = 0; p < (int)BATCH_PLANS.size() && st == OBME_OK; p++) {
+        if (batch_apply_plan(b, p) != OBME_OK) continue;  // a plan that cannot hold the deepest column
+        for (int spl : spls) {
+            b->steps_per_launch = spl;
+            for (int n = 0; n < 4; n++)
+                if (hipMemcpyAsync(scratch[n], tracers[n], bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+                    st = fail(OBME_ERR_HIP, "autotune copy");
+            if (st != OBME_OK) break;
+            st = batch_enqueue_any(b, scratch, 1, dt, s);  // warm-up
+            if (st == OBME_OK && hipEventRecord(e0, s) != hipSuccess) st = fail(OBME_ERR_HIP, "event");
+            if (st == OBME_OK) st = batch_enqueue_any(b, scratch, trial_steps, dt, s);
+            if (st == OBME_OK && hipEventRecord(e1, s) != hipSuccess) st = fail(OBME_ERR_HIP, "event");
+            float ms = 0.0f;
+            if (st == OBME_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                st = fail(OBME_ERR_HIP, "autotune timing");
+            if (st != OBME_OK) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best_plan = p;
+                best_spl = spl;
+            }
+        }
+    }
+    cleanup();
+    b->steps_per_launch = keep_spl;
+    obme_status r = batch_apply_plan(b, best_plan);
+    if (r == OBME_OK) b->steps_per_launch = best_spl;
+    (void)keep_plan;
+    return st != OBME_OK ? st : r;
+}
+
+// "plan P steps_per_launch K classes: L,CPW,NW:count ..." (returns length or < 0)
+extern "C" int32_t obme_batch_describe(const obme_batch* b, char* buf, int32_t cap)
+{
+    if (!b || !buf || cap < 1) return -1;
+    std::string s = "plan " + std::to_string(b->plan) + " steps_per_launch " + std::to_string(b->steps_per_launch) +
+                    " classes:";
+    for (size_t q = 0; q < b->class_tile.size(); q++) {
+        const BatchTile& t = BATCH_TILES[b->class_tile[q]];
+        s += " " + std::to_string(t.L) + "," + std::to_string(t.CPW) + "," + std::to_string(t.NW) + ":" +
+             std::to_string(b->class_n[q]);
+    }
+    std::snprintf(buf, cap, "%s", s.c_str());
+    return (int32_t)s.size();
+}
+
 // Sizes of the ABI structs, so bindings can check their layouts.
 extern "C" int64_t obme_abi_sizeof(int32_t which)
 {
@@ -613,6 +1036,7 @@ extern "C" int64_t obme_abi_sizeof(int32_t which)
     case 5: return sizeof(obme_carbon_solver);
     case 6: return sizeof(obme_co2_exchange);
     case 7: return sizeof(obme_field_source);
+    case 8: return sizeof(obme_batch_params);
     default: return -1;
     }
 }

@@ -145,6 +145,21 @@ class Config(C.Structure):
     ]
 
 
+class BatchParams(C.Structure):
+    """obme_batch_params: device pointers to per-column parameter arrays."""
+    _fields_ = [("fields", C.c_void_p * 40)]
+
+
+# per-column parameter rows (OBME_BP_*)
+(OBME_BP_NITRATE_HALF_SATURATION, OBME_BP_LIGHT_HALF_SATURATION, OBME_BP_MAX_GROWTH_RATE,
+ OBME_BP_EXUDATION_FRACTION, OBME_BP_AMMONIA_FRACTION_OF_EXUDATE, OBME_BP_TEMPERATURE_COEFFICIENT,
+ OBME_BP_PHYTO_MORTALITY_RATE, OBME_BP_ZOO_MORTALITY_RATE, OBME_BP_ZOO_EXCRETION_RATE,
+ OBME_BP_PHYTO_SOLID_WASTE_FRACTION, OBME_BP_EXCRETION_INORGANIC_FRACTION, OBME_BP_PREFERENCE_FOR_PHYTO,
+ OBME_BP_MAX_GRAZING_RATE, OBME_BP_GRAZING_HALF_SATURATION, OBME_BP_ZOO_ASSIMILATION_FRACTION,
+ OBME_BP_REMINERALISATION_RATE, OBME_BP_SMALL_PARTICLE_FRACTION, OBME_BP_P_SINKING_SPEED,
+ OBME_BP_D_SINKING_SPEED, OBME_BP_SURFACE_PAR, OBME_BP_COUNT) = range(21)
+
+
 # obme_abi_sizeof(which) indices (see the header)
 ABI_STRUCTS = (
     ("obme_config", Config),
@@ -155,4 +170,5 @@ ABI_STRUCTS = (
     ("obme_carbon_solver", CarbonSolver),
     ("obme_co2_exchange", CO2Exchange),
     ("obme_field_source", FieldSource),
+    ("obme_batch_params", BatchParams),
 )

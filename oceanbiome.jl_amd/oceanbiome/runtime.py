@@ -18,7 +18,8 @@ import numpy as np
 from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libobme_hip.so")
+# OBME_LIBRARY names another build of the library (same ABI) to load instead
+LIB_PATH = os.environ.get("OBME_LIBRARY") or os.path.join(HERE, "libobme_hip.so")
 
 _lib = None
 
@@ -48,6 +49,13 @@ def library():
         "obme_co2_flux": (C.c_int32, [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp, C.c_int64, C.c_int32, vp]),
         "obme_abi_sizeof": (C.c_int64, [C.c_int32]),
         "obme_set_kernel_timing": (C.c_int32, [vp, C.c_int32]),
+        "obme_batch_create": (C.c_int32, [C.POINTER(abi.Config), C.c_int32, C.POINTER(C.c_int32), C.c_double,
+                                          C.POINTER(abi.BatchParams), C.c_double, C.POINTER(vp)]),
+        "obme_batch_destroy": (C.c_int32, [vp]),
+        "obme_batch_step": (C.c_int32, [vp, C.POINTER(vp), C.c_int32, C.c_double, C.c_double, vp]),
+        "obme_batch_configure": (C.c_int32, [vp, C.c_int32, C.c_int32]),
+        "obme_batch_autotune": (C.c_int32, [vp, C.POINTER(vp), C.c_int32, C.c_double, vp]),
+        "obme_batch_describe": (C.c_int32, [vp, C.c_char_p, C.c_int32]),
         "obme_kernel_timing": (C.c_int32, [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
@@ -61,7 +69,8 @@ def library():
 EXPORTED_SYMBOLS = (
     "obme_create", "obme_destroy", "obme_last_error", "obme_version", "obme_required_tracers",
     "obme_update_state", "obme_add_tendencies", "obme_step", "obme_co2_flux", "obme_abi_sizeof",
-    "obme_set_kernel_timing", "obme_kernel_timing",
+    "obme_set_kernel_timing", "obme_kernel_timing", "obme_batch_create", "obme_batch_destroy", "obme_batch_step",
+    "obme_batch_configure", "obme_batch_autotune", "obme_batch_describe",
 )
 
 KERNEL_SWEEP = 0      # OBME_KERNEL_SWEEP: the column sweep
@@ -178,6 +187,80 @@ class HIPBiogeochemistry:
         _check(library().obme_co2_flux(self.handle, _ptr(DIC), _ptr(Alk), _ptr(T), _ptr(S), _ptr(u10), float(air),
                                        _ptr(pH), _ptr(flux), n, int(mode), _stream()))
         return pH, flux
+
+
+class HIPBatch:
+    """Ensemble of independent NPZD columns on the GPU (obme_batch_*).
+
+    bgc: an NPZD Biogeochemistry supplying the TwoBand coefficients and every
+    parameter not given per column; offsets: host int array (n_columns + 1);
+    params: host array [OBME_BP_COUNT, n_columns] (NaN rows = model default);
+    tracers passed to step() are 4 CUDA tensors (N, P, Z, D) of offsets[-1]
+    elements."""
+
+    def __init__(self, bgc, offsets, params, dz, temperature, precision="fp32"):
+        import torch
+        if not torch.cuda.is_available():
+            raise OBMEError("HIPBatch needs a GPU")
+        self.precision = abi.OBME_FP64 if precision == "fp64" else abi.OBME_FP32_MIXED
+        self.dtype = torch.float64 if self.precision == abi.OBME_FP64 else torch.float32
+        self.cfg = bgc.config(self.precision)
+        off = np.ascontiguousarray(offsets, dtype=np.int32)
+        self.n_columns = len(off) - 1
+        self.n_cells = int(off[-1])
+        self._params = []
+        bp = abi.BatchParams()
+        params = np.asarray(params, dtype=np.float64)
+        for f in range(params.shape[0]):
+            if np.all(np.isnan(params[f])):
+                continue
+            t = torch.from_numpy(np.ascontiguousarray(params[f])).to("cuda")
+            self._params.append(t)
+            bp.fields[f] = t.data_ptr()
+        torch.cuda.synchronize()
+        h = C.c_void_p()
+        _check(library().obme_batch_create(C.byref(self.cfg), self.n_columns,
+                                           off.ctypes.data_as(C.POINTER(C.c_int32)), float(dz), C.byref(bp),
+                                           float(temperature), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            library().obme_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check_fields(self, tracers):
+        if len(tracers) != 4:
+            raise OBMEError("the ensemble has 4 tracers (N, P, Z, D)")
+        for t in tracers:
+            if t.dtype != self.dtype or t.numel() != self.n_cells or not t.is_cuda or not t.is_contiguous():
+                raise OBMEError(f"tracers must be contiguous {self.dtype} CUDA tensors of {self.n_cells} elements")
+
+    def step(self, tracers, nsteps, dt, t0=0.0):
+        self._check_fields(tracers)
+        _check(library().obme_batch_step(self.handle, _ptr_array(tracers), int(nsteps), float(dt), float(t0),
+                                         _stream()))
+
+    def configure(self, plan, steps_per_launch=0):
+        _check(library().obme_batch_configure(self.handle, int(plan), int(steps_per_launch)))
+
+    def autotune(self, tracers, trial_steps, dt):
+        self._check_fields(tracers)
+        _check(library().obme_batch_autotune(self.handle, _ptr_array(tracers), int(trial_steps), float(dt),
+                                             _stream()))
+
+    def describe(self):
+        buf = C.create_string_buffer(512)
+        n = library().obme_batch_describe(self.handle, buf, 512)
+        if n < 0:
+            raise OBMEError("obme_batch_describe failed")
+        return buf.value.decode()
 
 
 def abi_layout_mismatches():

@@ -63,6 +63,58 @@ def test_lobster_fp32_mixed(shape):
     assert all(e <= FP32_TOL for e in errs.values()), errs
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_diurnal_surface_PAR(precision):
+    """default_surface_PAR (Light/Light.jl:219) at the three stage clocks."""
+    print("case diurnal", precision, flush=True)
+    wl = workloads.lobster_3d(24, 3, 40, nsteps=12, dt=1800.0, surface_PAR="default")
+    errs = _run_both(wl, precision)
+    tol = FP64_TOL if precision == "fp64" else FP32_TOL
+    assert all(e <= tol for e in errs.values()), errs
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_surface_PAR_array(precision):
+    """A 2-D surface PAR field PAR0[i, j]."""
+    import torch
+    print("case surface array", precision, flush=True)
+    wl = workloads.lobster_3d(30, 5, 33, nsteps=6, surface_PAR=np.zeros((5, 30)))
+    g = wl.grid
+    PAR0 = np.random.default_rng(11).uniform(20.0, 180.0, (g.Ny, g.Nx))
+    ref = [a.copy() for a in wl.tracers]
+    O.grid_run(wl.bgc.config(abi.OBME_FP64), ref, wl.nsteps, wl.dt, PAR0=np.ascontiguousarray(PAR0.ravel()))
+    dev = HIPBiogeochemistry(wl.bgc, precision, surface_array=torch.from_numpy(PAR0.ravel()))
+    tr = to_device(wl.tracers, dev.dtype)
+    dev.step(tr, wl.nsteps, wl.dt)
+    torch.cuda.synchronize()
+    tol = FP64_TOL if precision == "fp64" else FP32_TOL
+    for name, a, b in zip(wl.bgc.required_biogeochemical_tracers(), to_host(tr), ref):
+        assert rel_l2(g.interior(a), g.interior(b)) <= tol, name
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_stretched_grid_closed_bottom(precision):
+    """Non-uniform cell thickness and a closed bottom (w damped to 0 at the
+    bottom face): the per-level geometry and face-velocity tables."""
+    import torch
+    from oceanbiome.grid import RectilinearGrid
+    from oceanbiome.models import LOBSTER, CarbonateSystem, TwoParticleAndDissolved, \
+        TwoBandPhotosyntheticallyActiveRadiation, CarbonDioxideGasExchangeBoundaryCondition
+    print("case stretched closed", precision, flush=True)
+    Nz = 30
+    faces = -200.0 * (1 - np.linspace(0.0, 1.0, Nz + 1) ** 1.5)
+    faces[-1] = 0.0
+    g = RectilinearGrid(size=(20, 4, Nz), x=(0, 100), y=(0, 20), z=faces)
+    bgc = LOBSTER(g, carbonate_system=CarbonateSystem(), detritus=TwoParticleAndDissolved(open_bottom=False),
+                  light_attenuation=TwoBandPhotosyntheticallyActiveRadiation(surface_PAR=120.0),
+                  gas_exchange=CarbonDioxideGasExchangeBoundaryCondition(), temperature=15.0, salinity=34.0)
+    base = workloads.lobster_3d(20, 4, Nz, nsteps=1, seed=5)
+    wl = workloads.Workload("stretched", bgc, g, [a.copy() for a in base.tracers], 120.0, 8)
+    errs = _run_both(wl, precision)
+    tol = FP64_TOL if precision == "fp64" else FP32_TOL
+    assert all(e <= tol for e in errs.values()), errs
+
+
 def test_update_state_par_fp64():
     import torch
     print("case update_state fp64", flush=True)
