@@ -196,11 +196,10 @@ def main():
             ci = (allc % g.Nx).astype(np.int32)
             cj = (allc // g.Nx).astype(np.int32)
             ref = [a.copy() for a in wl.tracers]
-            probe = max(1, len(ci) // 16)
             t0 = time.perf_counter()
-            O.grid_run_columns(cfg, ref, ci[:probe], cj[:probe], 1, wl.dt, nthreads=nthreads)
-            per_col_step = (time.perf_counter() - t0) / probe
-            steps = int(max(1, min(50, round(args.cpu_seconds / (per_col_step * len(ci))))))
+            O.grid_run_columns(cfg, ref, ci, cj, 1, wl.dt, nthreads=nthreads)
+            per_step = time.perf_counter() - t0
+            steps = int(max(1, min(200, round(args.cpu_seconds / per_step))))
             t0 = time.perf_counter()
             O.grid_run_columns(cfg, ref, ci, cj, steps, wl.dt, nthreads=nthreads)
             secs = time.perf_counter() - t0

@@ -309,6 +309,195 @@ obme_status obme_batch_autotune(obme_batch* b, void* const* tracers, int32_t tri
 /* Text description of the current packing; returns its length or < 0. */
 int32_t obme_batch_describe(const obme_batch* b, char* buf, int32_t cap);
 
+
+/* ---- PISCES (src/Models/AdvectedPopulations/PISCES) -------------------- */
+
+/* Tracer order of required_biogeochemical_tracers(PISCES) (PISCES.jl:91-103):
+ * the first 24 are written, T and S are read. */
+enum {
+    OBME_PI_P = 0, OBME_PI_PCHL, OBME_PI_PFE, OBME_PI_D, OBME_PI_DCHL, OBME_PI_DFE, OBME_PI_DSI,
+    OBME_PI_Z, OBME_PI_M, OBME_PI_DOC, OBME_PI_POC, OBME_PI_GOC, OBME_PI_SFE, OBME_PI_BFE, OBME_PI_PSI,
+    OBME_PI_CACO3, OBME_PI_NO3, OBME_PI_NH4, OBME_PI_PO4, OBME_PI_FE, OBME_PI_SI, OBME_PI_DIC,
+    OBME_PI_ALK, OBME_PI_O2, OBME_PI_T, OBME_PI_S, OBME_PI_COUNT
+};
+#define OBME_PISCES_WRITTEN 24
+
+/* Parameters: obme_pisces_config.params holds two phytoplankton blocks
+ * (nano at OBME_PP_NANO, diatoms at OBME_PP_DIATOMS; MixedMondo with
+ * GrowthRespirationLimitedProduction and NitrogenIronPhosphateSilicateLimitation,
+ * the phytoplankton/ sources), two zooplankton blocks (QualityDependantZooplankton,
+ * micro at OBME_PP_MICRO, meso at OBME_PP_MESO) and the scalars at
+ * OBME_PP_SCALARS; within a block the OBME_PH_* / OBME_PZ_* / OBME_PS_*
+ * offsets apply.  Rates in 1/s, concentrations in mmol/m3 (Fe in umol/m3). */
+enum {
+    OBME_PH_MU0 = 0,
+    OBME_PH_BT,
+    OBME_PH_DARK_TOLERANCE,
+    OBME_PH_ALPHA0,
+    OBME_PH_BETA,
+    OBME_PH_BASAL_RESPIRATION,
+    OBME_PH_REFERENCE_GROWTH,
+    OBME_PH_KNH4,
+    OBME_PH_KNO3,
+    OBME_PH_KPO4,
+    OBME_PH_THETA_FE_OPT,
+    OBME_PH_SI_LIMITED,
+    OBME_PH_KSI_MIN,
+    OBME_PH_PK,
+    OBME_PH_DELTA,
+    OBME_PH_BETA1,
+    OBME_PH_BETA2,
+    OBME_PH_BETA3,
+    OBME_PH_MORT_K,
+    OBME_PH_MORT_LIN,
+    OBME_PH_MORT_W0,
+    OBME_PH_MORT_W1,
+    OBME_PH_CHL_MIN,
+    OBME_PH_CHL_MAX,
+    OBME_PH_FE_MAX,
+    OBME_PH_SI_K1,
+    OBME_PH_SI_K2,
+    OBME_PH_SI_THETA0,
+    OBME_PH_KFE_UPTAKE,
+    OBME_PH_SIZE_THRESHOLD,
+    OBME_PH_SIZE_RATIO,
+    OBME_PH_COUNT
+};
+enum {
+    OBME_PZ_B = 0,
+    OBME_PZ_G0,
+    OBME_PZ_PREF_P,
+    OBME_PZ_PREF_D,
+    OBME_PZ_PREF_POC,
+    OBME_PZ_PREF_Z,
+    OBME_PZ_FOOD_THRESHOLD,
+    OBME_PZ_J,
+    OBME_PZ_K,
+    OBME_PZ_FF,
+    OBME_PZ_IRON_RATIO,
+    OBME_PZ_E0,
+    OBME_PZ_SIGMA,
+    OBME_PZ_KM,
+    OBME_PZ_M0,
+    OBME_PZ_R,
+    OBME_PZ_GAMMA,
+    OBME_PZ_ETA,
+    OBME_PZ_COUNT
+};
+enum {
+    OBME_PS_RAIN_RATIO = 0,
+    OBME_PS_BZ,
+    OBME_PS_BM,
+    OBME_PS_BACT_EXP,
+    OBME_PS_BACT_MAX,
+    OBME_PS_K_DOC,
+    OBME_PS_K_NO3_B,
+    OBME_PS_K_NH4_B,
+    OBME_PS_K_PO4_B,
+    OBME_PS_K_FE_B,
+    OBME_PS_DOC_LAMBDA,
+    OBME_PS_DOC_BACT_REF,
+    OBME_PS_DOC_B,
+    OBME_PS_DOC_A1,
+    OBME_PS_DOC_A2,
+    OBME_PS_DOC_A3,
+    OBME_PS_DOC_A4,
+    OBME_PS_DOC_A5,
+    OBME_PS_POM_B,
+    OBME_PS_POM_LAMBDA,
+    OBME_PS_POM_A1,
+    OBME_PS_POM_A2,
+    OBME_PS_POM_A3,
+    OBME_PS_POM_A4,
+    OBME_PS_FE_SCAV_MIN,
+    OBME_PS_FE_SCAV_LOAD,
+    OBME_PS_BACT_FE_EFF,
+    OBME_PS_SMALL_BACT_FE,
+    OBME_PS_LARGE_BACT_FE,
+    OBME_PS_CHI0,
+    OBME_PS_SI_FAST,
+    OBME_PS_SI_SLOW,
+    OBME_PS_CACO3_LAMBDA,
+    OBME_PS_CACO3_EXP,
+    OBME_PS_BACT_FE_MAX,
+    OBME_PS_BACT_FE_K,
+    OBME_PS_BACT_MU0,
+    OBME_PS_NITRIF,
+    OBME_PS_FIX_MAX,
+    OBME_PS_FIX_KFE,
+    OBME_PS_FIX_KPO4,
+    OBME_PS_FIX_E,
+    OBME_PS_FE_EXCESS,
+    OBME_PS_FE_LIGAND_MAX,
+    OBME_PS_FE_LIGAND_RATIO,
+    OBME_PS_O2_RESP,
+    OBME_PS_O2_NITRIF,
+    OBME_PS_ANOXIA1,
+    OBME_PS_ANOXIA2,
+    OBME_PS_THETAN,
+    OBME_PS_THETAP,
+    OBME_PS_MIXED_LAYER_SHEAR,
+    OBME_PS_BACKGROUND_SHEAR,
+    OBME_PS_LATITUDE,
+    OBME_PS_SILICATE_CLIMATOLOGY,
+    OBME_PS_MIXED_LAYER_DEPTH,
+    OBME_PS_KAPPA,
+    OBME_PS_POC_SPEED,
+    OBME_PS_GOC_MIN_SPEED,
+    OBME_PS_GOC_MAX_SPEED,
+    OBME_PS_GRAVITY,
+    OBME_PS_COUNT
+};
+enum {
+    OBME_PP_NANO = 0,
+    OBME_PP_DIATOMS = OBME_PH_COUNT,
+    OBME_PP_MICRO = 2 * OBME_PH_COUNT,
+    OBME_PP_MESO = 2 * OBME_PH_COUNT + OBME_PZ_COUNT,
+    OBME_PP_SCALARS = 2 * OBME_PH_COUNT + 2 * OBME_PZ_COUNT,
+    OBME_PP_COUNT = 2 * OBME_PH_COUNT + 2 * OBME_PZ_COUNT + OBME_PS_COUNT
+};
+
+/* PISCES configuration: grid, parameters (OBME_PP_*), MultiBand light
+ * (MultiBandPhotosyntheticallyActiveRadiation, Light/multi_band.jl: per band
+ * kʷ, e, χ and the share of the surface PAR) and the carbonate solver of the
+ * calcite saturation.  The mixed-layer depth and the mean mixed-layer
+ * diffusivity are prescribed constants (params); the surface PAR is constant. */
+typedef struct obme_pisces_config {
+    int32_t precision;             /* OBME_FP64 | OBME_FP32_MIXED (FP64 calcite solve either way) */
+    int32_t open_bottom;           /* POC sinking: 1 = -speed on every face, 0 = damped at the bottom */
+    obme_grid grid;
+    double params[OBME_PP_COUNT];
+    double band_kw[3], band_e[3], band_chi[3], band_division[3];
+    double surface_PAR;            /* W/m2 */
+    obme_carbon_solver solver;
+} obme_pisces_config;
+
+typedef struct obme_pisces obme_pisces;
+
+/* PISCES(; grid) (PISCES.jl:268-330) on the device. */
+obme_status obme_pisces_create(const obme_pisces_config* cfg, obme_pisces** out);
+obme_status obme_pisces_destroy(obme_pisces* h);
+
+/* update_biogeochemical_state!(model, ::PISCES) with the MultiBand light
+ * (update_state.jl:1-17; multi_band.jl:147-185; compute_euphotic_depth.jl:3-41;
+ * mean_mixed_layer_properties.jl:25-47; compute_calcite_saturation.jl:9-37):
+ * from tracers[OBME_PI_COUNT] at time t writes, when non-NULL, the 3-D PAR
+ * (total) and PAR1..3 fields, the calcite saturation Omega (3-D), and the
+ * 2-D euphotic depth and mixed-layer mean PAR (Nx*Ny, i fastest). */
+obme_status obme_pisces_update_state(obme_pisces* h, const void* const* tracers, void* PAR, void* const* PARn,
+                                     void* Omega, void* euphotic_depth, void* mixed_layer_PAR, double t,
+                                     void* stream);
+/* ADDS the 24 transitions minus the sinking flux divergence into G[0..23]
+ * (the per-tracer `(bgc::PISCES)(i, j, k, grid, Val(name), ...)` methods plus
+ * biogeochemical_drift_velocity of POC/SFe and GOC/BFe/PSi/CaCO3), after the
+ * update-state step at time t. */
+obme_status obme_pisces_add_tendencies(obme_pisces* h, const void* const* tracers, void* const* G, double t,
+                                       void* stream);
+/* nsteps RK3 steps (update-state, tendencies, substep per stage) in place
+ * on tracers[0..23]; T and S (tracers[24], [25]) are read. */
+obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, int32_t nsteps, double dt, double t0,
+                             void* stream);
+
 /* ---- kernel timing ------------------------------------------------------ */
 
 /* No counterpart in the reference: measurement instrumentation.  While
@@ -324,7 +513,8 @@ obme_status obme_kernel_timing(obme_handle* h, int32_t which, double* total_ms, 
 /* Size in bytes of ABI struct `which` as compiled into the library
  * (0 obme_config, 1 obme_grid, 2 obme_npd_params, 3 obme_two_band_par,
  * 4 obme_sinking, 5 obme_carbon_solver, 6 obme_co2_exchange,
- * 7 obme_field_source, 8 obme_batch_params); -1 for an unknown index.  No reference
+ * 7 obme_field_source, 8 obme_batch_params, 9 obme_pisces_config); -1 for an
+ * unknown index.  No reference
  * counterpart: it lets a binding check its struct layouts. */
 int64_t obme_abi_sizeof(int32_t which);
 

@@ -9,6 +9,6 @@ HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
     -Xclang -target-feature -Xclang -packed-fp32-ops \
     -Wall -Wno-unused-function -Wno-unused-variable \
     ${OBME_EXTRA_FLAGS:-} \
-    -o "${OUT}.tmp" "${HERE}/obme_api.hip"
+    -o "${OUT}.tmp" "${HERE}/obme_api.hip" "${HERE}/obme_pisces.hip"
 mv "${OUT}.tmp" "${OUT}"
 echo "built ${OUT}"

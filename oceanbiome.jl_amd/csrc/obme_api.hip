@@ -2,8 +2,8 @@
 //
 // Entry points (declared in include/obme.h) and the reference interface
 // each one stands in for are listed there.  All device work is queued on
-// the caller's stream; the multi-step stepper records its launches into a
-// hipGraph on first use for a given (nsteps, dt, t0) and replays it.
+// the caller's stream.  The batched ensemble records multi-launch calls into
+// a hipGraph and replays it.  PISCES lives in obme_pisces.hip.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "obme_batch_kernels.h"
+#include "obme_internal.h"
 #include "obme_npd_kernels.h"
 
 using namespace obme;
@@ -23,11 +24,13 @@ using namespace obme;
 
 static thread_local std::string g_last_error;
 
-static obme_status fail(obme_status code, const std::string& msg)
+obme_status obme_fail(obme_status code, const std::string& msg)
 {
     g_last_error = msg;
     return code;
 }
+
+static obme_status fail(obme_status code, const std::string& msg) { return obme_fail(code, msg); }
 
 #define HIP_TRY(expr)                                                                       \
     do {                                                                                    \
@@ -44,11 +47,12 @@ extern "C" const char* obme_version(void) { return "obme 0.1.0 gfx950"; }
 // The sweep: one workgroup of NW waves, wave-per-column segments
 // (obme_npd_kernels.h).  __launch_bounds__(NW*64) lets the compiler use
 // 512/(NW/4) VGPRs per lane at most (128 for 16 waves, 256 for 8).
-template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM>
-__global__ void __launch_bounds__(NW * 64) k_column_wave(StepArgs<FT> a)
+// FP32 tiles of up to 2 levels per lane are held to 128 VGPRs (4 waves per SIMD).
+template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL>
+__global__ void __launch_bounds__(NW * 64, (sizeof(FT) == 4 && L <= 2) ? 4 : 1) k_column_wave(StepArgs<FT> a)
 {
     extern __shared__ unsigned char smem_raw[];
-    column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, UNIFORM>(a, reinterpret_cast<FT*>(smem_raw));
+    column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, UNIFORM, FULL>(a, reinterpret_cast<FT*>(smem_raw));
 }
 
 // CO2 flux and pH on n points (SoA, FP64).
@@ -196,15 +200,15 @@ struct TileCfg {
     int L, CPW, NW;
 };
 static const TileCfg TILE_CFGS[] = {
-    {2, 2, 16}, {4, 4, 8}, {2, 2, 8}, {1, 1, 16}, {2, 1, 16}, {4, 2, 8}, {1, 1, 8}, {2, 1, 8},
+    {2, 2, 8}, {1, 1, 16}, {2, 1, 8}, {2, 2, 16}, {1, 1, 8}, {4, 2, 8},
 };
 static const int N_TILE_CFG = sizeof(TILE_CFGS) / sizeof(TILE_CFGS[0]);
 
 static int choose_tile(int prec, int Nz)
 {
     // preference lists by precision (indices into TILE_CFGS)
-    static const int pref32[] = {0, 4, 5};
-    static const int pref64[] = {3, 4, 7};
+    static const int pref32[] = {0, 2, 5};
+    static const int pref64[] = {1, 2, 5};
     const int* p = (prec == OBME_FP64) ? pref64 : pref32;
     for (int q = 0; q < 3; q++) {
         const TileCfg& t = TILE_CFGS[p[q]];
@@ -387,6 +391,13 @@ static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers)
         a.kb2 = (float)(c.par.water_blue_attenuation * log2e);
         a.xr2 = (float)(c.par.chlorophyll_red_attenuation * log2e);
         a.xb2 = (float)(c.par.chlorophyll_blue_attenuation * log2e);
+        const double lRr = std::log2(c.par.phytoplankton_chlorophyll_ratio / c.par.pigment_ratio);
+        a.cr = (float)(c.par.chlorophyll_red_exponent * lRr);
+        a.cb = (float)(c.par.chlorophyll_blue_exponent * lRr);
+        a.xr2h = (float)(c.par.chlorophyll_red_attenuation * log2e * h->dz_u / 2);
+        a.xb2h = (float)(c.par.chlorophyll_blue_attenuation * log2e * h->dz_u / 2);
+        a.wi0 = (float)(h->w0_u / h->dz_u);
+        a.wi1 = (float)(h->w1_u / h->dz_u);
     }
     return a;
 }
@@ -440,8 +451,12 @@ static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_
     if (lds > 160 * 1024) return fail(OBME_ERR_UNSUPPORTED, "sweep tile exceeds the 160 KiB LDS");
     dim3 block(NW * 64);
     dim3 grid((h->cfg.grid.Nx + TXB - 1) / TXB, h->cfg.grid.Ny);
-    auto k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true>
-                        : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false>;
+    // FULL (every lane level is a cell) is compiled for the fused step only
+    const bool full = (MODE == TILE_STEP) && h->cfg.grid.Nz == L * 64 / CPW;
+    auto k = h->uniform ? (full ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, MODE == TILE_STEP>
+                                : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, false>)
+                        : (full ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, MODE == TILE_STEP>
+                                : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, false>);
     HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     LaunchTimer tm(h, OBME_KERNEL_SWEEP, s);
     hipLaunchKernelGGL(k, grid, block, lds, s, a);
@@ -462,14 +477,13 @@ template <typename FT, int MODE>
 static obme_status dispatch_tile_cfg(obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
 {
     switch (h->cfg_id) {
-    case 0: return dispatch_model<FT, 2, 2, 16, MODE>(h, a, s);
-    case 1: return dispatch_model<FT, 4, 4, 8, MODE>(h, a, s);
-    case 2: return dispatch_model<FT, 2, 2, 8, MODE>(h, a, s);
-    case 3: return dispatch_model<FT, 1, 1, 16, MODE>(h, a, s);
-    case 4: return dispatch_model<FT, 2, 1, 16, MODE>(h, a, s);
+    // the entries of TILE_CFGS, in order
+    case 0: return dispatch_model<FT, 2, 2, 8, MODE>(h, a, s);
+    case 1: return dispatch_model<FT, 1, 1, 16, MODE>(h, a, s);
+    case 2: return dispatch_model<FT, 2, 1, 8, MODE>(h, a, s);
+    case 3: return dispatch_model<FT, 2, 2, 16, MODE>(h, a, s);
+    case 4: return dispatch_model<FT, 1, 1, 8, MODE>(h, a, s);
     case 5: return dispatch_model<FT, 4, 2, 8, MODE>(h, a, s);
-    case 6: return dispatch_model<FT, 1, 1, 8, MODE>(h, a, s);
-    case 7: return dispatch_model<FT, 2, 1, 8, MODE>(h, a, s);
     default: return fail(OBME_ERR_INVALID, "bad tile configuration");
     }
 }
@@ -1037,6 +1051,7 @@ extern "C" int64_t obme_abi_sizeof(int32_t which)
     case 6: return sizeof(obme_co2_exchange);
     case 7: return sizeof(obme_field_source);
     case 8: return sizeof(obme_batch_params);
+    case 9: return sizeof(obme_pisces_config);
     default: return -1;
     }
 }

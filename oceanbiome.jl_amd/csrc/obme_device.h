@@ -347,28 +347,33 @@ __device__ __forceinline__ float fmax0(float x) { return (x < 0.0f) ? 0.0f : x; 
 constexpr float F_TINY = 1.17549435e-38f;  // smallest normal float
 
 // Grazing of the PhytoZoo plankton (plankton.jl:118-135, :352-397) in FP32,
-// with the shared factors formed once: g L Z / (food + tiny) scales both
-// grazed fractions.  (The reference's eps(food) only keeps 0/0 away; the
-// smallest normal does the same.)
+// reformulated over n = pref P^2 + (1-pref) sPOM^2 and d = pref P + (1-pref) sPOM:
+// the preference-weighted food is n/d, so the concentration limitation is
+// n^2 / (n^2 + kG^2 d^2) (quadratic) or n / (n + kG d) (linear), and the
+// grazed fractions of the total are pref P^2 / n and (1-pref) sPOM^2 / n.
+// Two reciprocals instead of the reference's three divisions; F_TINY keeps
+// 0/0 away as the reference's eps(food) does.
 struct FGrazing {
     float total, onP, onSmall;
 };
 __device__ __forceinline__ FGrazing fast_grazing(const FastParams& f, float P, float Z, float sPOM)
 {
     const float pP = f.pref * P;
-    const float p = pP * frcp(fmaf(f.prefc, sPOM, pP) + F_TINY);
-    const float food = fmaf(p, P - sPOM, sPOM);
+    const float cS = f.prefc * sPOM;
+    const float n = fmaf(pP, P, cS * sPOM);
+    const float d = pP + cS;
     float L;
-    if (f.graz_form == OBME_LINEAR) L = food * frcp(food + f.kG);
+    if (f.graz_form == OBME_LINEAR) L = n * frcp(fmaf(f.kG, d, n) + F_TINY);
     else {
-        const float f2 = food * food;
-        L = f2 * frcp(f2 + f.kG2);
+        const float n2 = n * n;
+        const float kd = f.kG * d;
+        L = n2 * frcp(fmaf(kd, kd, n2) + F_TINY);
     }
     FGrazing G;
     G.total = f.g * L * Z;
-    const float fr = G.total * frcp(food + F_TINY);
-    G.onP = fr * (p * P);
-    G.onSmall = fr * fmaf(-p, sPOM, sPOM);
+    const float fr = G.total * frcp(n + F_TINY);
+    G.onP = fr * (pP * P);
+    G.onSmall = fr * (cS * sPOM);
     return G;
 }
 
@@ -379,19 +384,26 @@ __device__ __forceinline__ float fast_light(const FastParams& f, float PAR)
 }
 
 // LOBSTER (+ CarbonateSystem) transitions in FP32; v/out in tracer order.
+// The nitrate and ammonia uptakes are mu0/2 L_PAR P times their own
+// limitation (the reference forms them as fractions of the total
+// production, plankton.jl:223-261, which is the same product).
 template <bool CARB>
 __device__ __forceinline__ void lobster_fast(const FastParams& f, const float* v, float PAR, float* out)
 {
     const float NO3 = v[0], NH4 = v[1], P = v[2], Z = v[3], sPOM = v[4], bPOM = v[5], DOM = v[6];
-    const float nl = NO3 * __builtin_amdgcn_exp2f(f.nPSI2 * NH4) * frcp(NO3 + f.kNO3);
-    const float al = fmax0(NH4 * frcp(f.kNH4 + NH4));
-    const float muP = f.mu0h * fast_light(f, PAR) * (nl + al) * P;
+    // one reciprocal for both Monod terms
+    const float a = NO3 + f.kNO3, b = f.kNH4 + NH4;
+    const float rab = frcp(a * b);
+    const float nl = NO3 * __builtin_amdgcn_exp2f(f.nPSI2 * NH4) * (b * rab);
+    const float al = fmax0(NH4 * (a * rab));
+    const float muLP = f.mu0h * fast_light(f, PAR) * P;
+    const float up_NO3 = muLP * nl;
+    const float muA = muLP * al;
+    const float muP = up_NO3 + muA;
     const FGrazing G = fast_grazing(f, P, Z, sPOM);
     const float nuP = (f.mort_form == OBME_LINEAR) ? f.mP * P : f.mP * P * P;
     const float nit = f.nitrif * NH4;
-    const float rsh = muP * frcp(nl + al + F_TINY);
-    const float up_NO3 = rsh * nl;
-    const float up_NH4 = fmaf(rsh, al, -f.axg * muP);
+    const float up_NH4 = fmaf(-f.axg, muP, muA);
     const float pinw = fmaf(f.pinw_z, Z, f.one_m_aPs * nuP);
     const float sb = fmaf(f.s_rem, sPOM, f.b_rem * bPOM);
     const float dinw = fmaf(f.a_rem, sb, f.d_rem * DOM);
@@ -521,9 +533,10 @@ __device__ __forceinline__ void carbon_setup(double DIC, double Alk, double T, d
 }
 
 // alkalinity_residual / ∂ₕ (CarbonChemistry/alkalinity_residual.jl:18-75)
-// with silicate = phosphate = 0 (the defaults).  The phosphate and silicate
+// with phosphate = 0 and, unless SI, silicate = 0 (the defaults).  Those
 // terms are then ±0 and adding them to the nonzero partial sums is exact, so
-// they are left out.
+// they are left out; with SI the silicate term enters at its place in the sum.
+template <bool SI = false>
 __device__ __forceinline__ double alk_residual(double H, const CarbonParams& p)
 {
     double cd = H * H + p.K1 * H + p.K1 * p.K2;
@@ -535,10 +548,12 @@ __device__ __forceinline__ double alk_residual(double H, const CarbonParams& p)
     double free_hydrogen = -H / sd;
     double hydrogen_sulfate = -p.sulfate / (1 + p.KS / H * sd);
     double hydrogen_fluoride = -p.fluoride / (1 + p.KF / H);
-    return bicarbonate + carbonate + borate + hydroxide + free_hydrogen + hydrogen_sulfate + hydrogen_fluoride
-           - p.Alk;
+    double acc = bicarbonate + carbonate + borate + hydroxide;
+    if constexpr (SI) acc = acc + p.silicate / (1 + H / p.KSi);
+    return acc + free_hydrogen + hydrogen_sulfate + hydrogen_fluoride - p.Alk;
 }
 
+template <bool SI = false>
 __device__ __forceinline__ double alk_residual_derivative(double H, const CarbonParams& p)
 {
     double cd = H * H + p.K1 * H + p.K1 * p.K2;
@@ -554,8 +569,12 @@ __device__ __forceinline__ double alk_residual_derivative(double H, const Carbon
     double d_hydrogen_sulfate = p.sulfate / (hsd * hsd) * (p.KS / H * 0.0 - p.KS / (H * H) * sd);
     double fd = 1 + p.KF / H;
     double d_hydrogen_fluoride = -p.fluoride / (fd * fd) * p.KF / (H * H);
-    return d_bicarbonate + d_carbonate + d_borate + d_hydroxide + d_free_hydrogen + d_hydrogen_sulfate
-           + d_hydrogen_fluoride;
+    double acc = d_bicarbonate + d_carbonate + d_borate + d_hydroxide;
+    if constexpr (SI) {
+        const double sdn = 1 + H / p.KSi;
+        acc = acc + -p.silicate / (sdn * sdn) / p.KSi;
+    }
+    return acc + d_free_hydrogen + d_hydrogen_sulfate + d_hydrogen_fluoride;
 }
 
 __device__ __forceinline__ double jsign(double x)
@@ -568,23 +587,24 @@ __device__ __forceinline__ double jsign(double x)
 // exact fixed point of the loop (same x, same f(x), same δ, same λ
 // sequence), so the remaining iterations are skipped with the identical
 // result.  evals counts residual evaluations.
+template <bool SI = false>
 __device__ __forceinline__ double solve_for_H(const CarbonParams& p, const obme_carbon_solver& s, double x0,
                                               int& evals)
 {
     double x = x0;
     int N = 0;
-    double fx = alk_residual(x, p);
+    double fx = alk_residual<SI>(x, p);
     int ne = 1;
     while ((fabs(fx) > s.atol) && (N < s.max_iters)) {
-        double delta = fx / alk_residual_derivative(x, p);
+        double delta = fx / alk_residual_derivative<SI>(x, p);
         double lb = (x - 0.0) / delta;
         lb = (lb < 0) ? 1.0 : lb;
         double lambda = jmin(1.0, lb);
-        double fnew = alk_residual(x - lambda * delta, p);
+        double fnew = alk_residual<SI>(x - lambda * delta, p);
         ne++;
         while ((fabs(fnew) >= fabs(fx) - s.armijo_constant * lambda * fx * jsign(fx)) && (lambda > s.min_damping)) {
             lambda *= s.damping;
-            fnew = alk_residual(x - lambda * delta, p);
+            fnew = alk_residual<SI>(x - lambda * delta, p);
             ne++;
         }
         double xn = x - lambda * delta;

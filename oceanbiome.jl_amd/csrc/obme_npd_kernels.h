@@ -97,6 +97,9 @@ struct StepArgs {
     FastParams f;                  // FP32 path: derived constants
     float rk_a[3], rk_b[3];        // FP32 path: Δt γ and Δt ζ per stage
     float kr2, kb2, xr2, xb2;      // FP32 path: attenuations × log2(e)
+    float cr, cb;                  // FP32 path: e log2(R_chl / r) per band
+    float xr2h, xb2h;              // FP32 uniform grid: xr2 dz/2, xb2 dz/2
+    float wi0, wi1;                // FP32 uniform grid: w / dz of the two sinking tracers
     int co2_on;
 };
 
@@ -148,7 +151,7 @@ __device__ __forceinline__ T seg_prefix(T x)
     return x;
 }
 
-template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM>
+template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL>
 __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
 {
     using TR = ModelTraits<MODEL, CARB>;
@@ -171,7 +174,8 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
     const int i = col_ok ? i_raw : a.Nx - 1;
 
     auto kof = [&](int m) -> int { return ktop - (r * L + m); };
-    auto live = [&](int m) -> bool { return col_ok && kof(m) >= 0; };
+    // FULL: Nz == L * LPC, every lane level is a cell of the column
+    auto has = [&](int m) -> bool { return FULL || kof(m) >= 0; };
     auto rs = [&](const FT* p) { return make_rsrc(p, a.field_bytes); };
     // byte offset of (ii, j, k) in a parent array
     auto goff = [&](int ii, int k) -> unsigned {
@@ -229,7 +233,7 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
 #pragma unroll
         for (int m = 0; m < L; m++) {
             const int k = kof(m);
-            const bool ok = k >= 0;
+            const bool ok = has(m);
             int t = (MODE == TILE_PAR) ? (n == TR::P_IDX ? 0 : -1) : n;
             U[n][m] = (ok && t >= 0) ? tile(t)[k * ROW + c] : FT(0);
             Gm[n][m] = FT(0);
@@ -278,6 +282,17 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
         if (top_lane && a.co2_on) a.srec[col2d] = U[7][0];
     }
 
+    // FP32 on a uniform grid: the water terms kr zc, kb zc of the own levels (log2 scaled)
+    float ezr[L], ezb[L];
+    if constexpr (F32 && UNIFORM) {
+#pragma unroll
+        for (int m = 0; m < L; m++) {
+            const float z = g_zc(has(m) ? kof(m) : 0);
+            ezr[m] = a.kr2 * z;
+            ezb[m] = a.kb2 * z;
+        }
+    }
+
     constexpr int NSTAGE = (MODE == TILE_STEP) ? 3 : 1;
     const double g1 = 8.0 / 15.0, g2 = 5.0 / 12.0, g3 = 3.0 / 4.0, z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
     FT PARv[L];   // PAR of the own levels (output of the PAR and tendency modes)
@@ -291,12 +306,12 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
         FT pr[L], pb[L];
 #pragma unroll
         for (int m = 0; m < L; m++) {
-            const FT x = val(TR::P_IDX, m) * a.Rr;
             if constexpr (F32) {
-                const FT lg = m_log2(x);
-                pr[m] = m_exp2(a.er * lg);
-                pb[m] = m_exp2(a.eb * lg);
+                const FT lg = m_log2(val(TR::P_IDX, m));
+                pr[m] = m_exp2(fmaf(a.er, lg, a.cr));
+                pb[m] = m_exp2(fmaf(a.eb, lg, a.cb));
             } else {
+                const FT x = val(TR::P_IDX, m) * a.Rr;
                 pr[m] = m_pow(x, a.er);
                 pb[m] = m_pow(x, a.eb);
             }
@@ -308,7 +323,8 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
         const FT c1_up = lane_shr1(val(TR::SINK1, L - 1));
 
         // increments top-down: inc(k) = up(k) p(k+1) + dn(k) p(k); the top
-        // level has none above: dn p alone
+        // level has none above: dn p alone.  FP32 on a uniform grid sums
+        // p(k+1) + p(k) and folds dz/2 into the attenuation.
         FT incr[L], incb[L];
         FT totr = FT(0), totb = FT(0);
 #pragma unroll
@@ -316,12 +332,19 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
             const int k = kof(m);
             const FT prn = (m == 0) ? pr_up : pr[m > 0 ? m - 1 : 0];
             const FT pbn = (m == 0) ? pb_up : pb[m > 0 ? m - 1 : 0];
-            const int kc = k >= 0 ? k : 0;
-            const FT upk = g_up(kc), dnk = g_dn(kc);
-            const FT ir = (k < ktop) ? upk * prn + dnk * pr[m] : dnk * pr[m];
-            const FT ib = (k < ktop) ? upk * pbn + dnk * pb[m] : dnk * pb[m];
-            incr[m] = (k >= 0) ? ir : FT(0);
-            incb[m] = (k >= 0) ? ib : FT(0);
+            const bool below_top = (m > 0) || (k < ktop);
+            FT ir, ib;
+            if constexpr (F32 && UNIFORM) {
+                ir = below_top ? prn + pr[m] : pr[m];
+                ib = below_top ? pbn + pb[m] : pb[m];
+            } else {
+                const int kc = has(m) ? k : 0;
+                const FT upk = g_up(kc), dnk = g_dn(kc);
+                ir = below_top ? upk * prn + dnk * pr[m] : dnk * pr[m];
+                ib = below_top ? upk * pbn + dnk * pb[m] : dnk * pb[m];
+            }
+            incr[m] = has(m) ? ir : FT(0);
+            incb[m] = has(m) ? ib : FT(0);
             totr = totr + incr[m];
             totb = totb + incb[m];
         }
@@ -334,16 +357,21 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
             runb = FT(0);
         }
         // PAR of the own levels
+        const FT PAR0h = PAR0 * FT(0.5);
 #pragma unroll
         for (int m = 0; m < L; m++) {
             runr = runr + incr[m];
             runb = runb + incb[m];
-            const int k = kof(m);
-            const FT z = g_zc(k >= 0 ? k : 0);
-            if constexpr (F32)
-                PARv[m] = PAR0 * (m_exp2(fmaf(a.kr2, z, -a.xr2 * runr)) + m_exp2(fmaf(a.kb2, z, -a.xb2 * runb))) * 0.5f;
-            else
-                PARv[m] = PAR0 * (m_exp(a.kr * z - a.xr * runr) + m_exp(a.kb * z - a.xb * runb)) / FT(2);
+            if constexpr (F32 && UNIFORM) {
+                PARv[m] = PAR0h * (m_exp2(fmaf(-a.xr2h, runr, ezr[m])) + m_exp2(fmaf(-a.xb2h, runb, ezb[m])));
+            } else {
+                const int k = kof(m);
+                const FT z = g_zc(has(m) ? k : 0);
+                if constexpr (F32)
+                    PARv[m] = PAR0h * (m_exp2(fmaf(a.kr2, z, -a.xr2 * runr)) + m_exp2(fmaf(a.kb2, z, -a.xb2 * runb)));
+                else
+                    PARv[m] = PAR0 * (m_exp(a.kr * z - a.xr * runr) + m_exp(a.kb * z - a.xb * runb)) / FT(2);
+            }
         }
 
         if constexpr (MODE != TILE_PAR) {
@@ -352,7 +380,7 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
 #pragma unroll
             for (int m = L - 1; m >= 0; m--) {
                 const int k = kof(m);
-                const int kc = k >= 0 ? k : 0;
+                const int kc = has(m) ? k : 0;
                 FT Gv[NT];
                 FT v[NT];
 #pragma unroll
@@ -374,13 +402,22 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
                 // minus the upwind drift flux divergence: w(face) c(upwind cell);
                 // no flux through the top face (w = 0 there)
                 {
+                    const bool below_top = (m > 0) || (k < ktop);
                     const FT cn0 = (m == 0) ? c0_up : val(TR::SINK0, m > 0 ? m - 1 : 0);
                     const FT cn1 = (m == 0) ? c1_up : val(TR::SINK1, m > 0 ? m - 1 : 0);
-                    const FT idz = g_invdz(kc);
-                    const FT Fhi0 = (kc < ktop) ? g_whi(0, kc) * cn0 : FT(0);
-                    const FT Fhi1 = (kc < ktop) ? g_whi(1, kc) * cn1 : FT(0);
-                    Gv[TR::SINK0] = Gv[TR::SINK0] - (Fhi0 - g_wlo(0, kc) * v[TR::SINK0]) * idz;
-                    Gv[TR::SINK1] = Gv[TR::SINK1] - (Fhi1 - g_wlo(1, kc) * v[TR::SINK1]) * idz;
+                    if constexpr (F32 && UNIFORM) {
+                        // w (c(k+1) - c(k)) / dz, c(k+1) = 0 above the top cell
+                        const float d0 = (below_top ? cn0 : 0.0f) - v[TR::SINK0];
+                        const float d1 = (below_top ? cn1 : 0.0f) - v[TR::SINK1];
+                        Gv[TR::SINK0] = fmaf(-a.wi0, d0, Gv[TR::SINK0]);
+                        Gv[TR::SINK1] = fmaf(-a.wi1, d1, Gv[TR::SINK1]);
+                    } else {
+                        const FT idz = g_invdz(kc);
+                        const FT Fhi0 = below_top ? g_whi(0, kc) * cn0 : FT(0);
+                        const FT Fhi1 = below_top ? g_whi(1, kc) * cn1 : FT(0);
+                        Gv[TR::SINK0] = Gv[TR::SINK0] - (Fhi0 - g_wlo(0, kc) * v[TR::SINK0]) * idz;
+                        Gv[TR::SINK1] = Gv[TR::SINK1] - (Fhi1 - g_wlo(1, kc) * v[TR::SINK1]) * idz;
+                    }
                 }
                 if constexpr (MODE == TILE_TENDENCY) {
 #pragma unroll
@@ -431,7 +468,7 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
             if constexpr (MODE == TILE_PAR) v = PARv[m];
             else if constexpr (MODE == TILE_TENDENCY) v = (n == NT) ? PARv[m] : Gm[n < NT ? n : 0][m];
             else v = val(n < NT ? n : 0, m);
-            if (k >= 0) T[k * ROW + c] = v;
+            if (has(m)) T[k * ROW + c] = v;
         }
     }
     __syncthreads();

@@ -1,0 +1,517 @@
+// obme_pisces.hip — PISCES kernels and their C-ABI entry points
+// (obme_pisces_*, include/obme.h).
+//
+// One RK3 stage is three launches on the caller's stream:
+//   k_pisces_columns  MultiBand PAR, euphotic depth, mixed-layer mean PAR;
+//   k_pisces_omega    calcite saturation per cell (FP64);
+//   k_pisces_cell     transitions + sinking + substep (or tendency output).
+// The stage input is read and the stage output written to different
+// arrays (the caller's tracers and a library-owned copy, alternately), so
+// the sinking stencil's k+1 reads never race with the update.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "obme_internal.h"
+#include "obme_pisces.h"
+
+using namespace obme;
+using namespace obme::pisces;
+
+namespace {
+
+constexpr int NW = OBME_PISCES_WRITTEN;  // 24 written tracers
+constexpr int NTR = OBME_PI_COUNT;       // + T, S
+
+template <typename FT>
+struct PiscesArgs {
+    const FT* in[NTR];      // stage input (parent arrays)
+    FT* out[NW];            // stage output (parent arrays) or tendencies (TILE mode)
+    FT* Gm;                 // previous-stage transitions [NW][ncell] (compact)
+    FT* PAR;                // [4][ncell] compact: total, band 1, 2, 3
+    FT* Omega;              // [ncell] compact
+    FT* zeu;                // [Nx*Ny]
+    FT* mlPAR;              // [Nx*Ny]
+    const FT* pp;           // parameters (OBME_PP_*)
+    const FT* lev;          // [4][Nz]: zc, zf (bottom face), dz, 1/dz
+    const FT* w;            // [2][Nz+1]: POC and GOC face velocities
+    double zf_top;          // top face position
+    int Nx, Ny, Nz;
+    unsigned sx, sxy, base0;
+    FT surface_PAR;
+    FT kw[3], e[3], chi[3], division[3];
+    StageConst<FT> sk;
+    FT A, B;                // RK3: Δt γ and Δt ζ of the stage (FP32 path)
+    double dt;
+    int stage;
+    obme_carbon_solver solver;
+};
+
+template <typename FT>
+__device__ __forceinline__ unsigned pidx(const PiscesArgs<FT>& a, int i, int j, int k)
+{
+    return a.base0 + (unsigned)i + a.sx * (unsigned)j + a.sxy * (unsigned)k;
+}
+
+// Thread per column: MultiBand PAR (multi_band.jl:147-165) top-down, the
+// euphotic depth (compute_euphotic_depth.jl:3-33, surface value = the
+// boundary value of PAR at z = 0) and the mixed-layer mean of the total PAR
+// (mean_mixed_layer_properties.jl:25-47) in the same sweep.
+template <typename FT>
+__global__ void __launch_bounds__(256) k_pisces_columns(PiscesArgs<FT> a)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= a.Nx * a.Ny) return;
+    const int i = col % a.Nx, j = col / a.Nx;
+    const int Nz = a.Nz;
+    const long ncell = (long)a.Nx * a.Ny * Nz;
+    const FT* zc = a.lev;
+    const FT* zf = a.lev + Nz;
+    const FT* sc = a.pp + OBME_PP_SCALARS;
+    const FT zmxl = sc[OBME_PS_MIXED_LAYER_DEPTH];
+    const FT thr = a.surface_PAR * FT(1e-3);
+    const FT lthr = m_log(thr);
+    FT band[3];
+    FT zeu = FT(-INFINITY);
+    FT acc = 0, depth = 0;
+    FT PARup = 0, lPARup = 0;
+    for (int k = Nz - 1; k >= 0; k--) {
+        const unsigned p = pidx(a, i, j, k);
+        const FT Chl = a.in[OBME_PI_PCHL][p] + a.in[OBME_PI_DCHL][p];
+#pragma unroll
+        for (int n = 0; n < 3; n++) {
+            const FT att = a.kw[n] + a.chi[n] * m_pow(Chl, a.e[n]);
+            if (k == Nz - 1) band[n] = a.surface_PAR * a.division[n] * m_exp(zc[k] * att);
+            else band[n] = band[n] * m_exp((zc[k] - zc[k + 1]) * att);
+        }
+        const FT PAR = band[0] + band[1] + band[2];
+        const long c = (long)col + (long)a.Nx * a.Ny * k;
+        a.PAR[c] = PAR;
+        a.PAR[ncell + c] = band[0];
+        a.PAR[2 * ncell + c] = band[1];
+        a.PAR[3 * ncell + c] = band[2];
+        const FT lPAR = m_log(PAR);
+        if (k < Nz - 1 && PAR <= thr && zeu == FT(-INFINITY))
+            zeu = zc[k] + (lthr - lPAR) * (zc[k] - zc[k + 1]) / (lPAR - lPARup);
+        PARup = PAR;
+        lPARup = lPAR;
+        // mixed-layer integral, top cell first
+        const FT zk = zf[k];
+        const FT zk1 = (k + 1 < Nz) ? zf[k + 1] : FT(a.zf_top);
+        const FT dzk = zk1 - zk;
+        const FT dzk1 = (zk1 > zmxl) ? zk1 - zmxl : FT(0);
+        const FT dz = (zk >= zmxl) ? dzk : dzk1;
+        acc = acc + PAR * dz;
+        depth = depth + dz;
+    }
+    (void)PARup;
+    if (!(zeu > FT(-INFINITY) && zeu < FT(INFINITY))) zeu = zc[0] - (Nz > 1 ? zc[1] - zc[0] : FT(1));
+    a.zeu[col] = zeu;
+    a.mlPAR[col] = acc / depth;
+}
+
+// Thread per cell: calcite saturation at the hydrostatic pressure
+// |z| g 1026 / 1e5 bar (compute_calcite_saturation.jl:22-36), FP64.
+template <typename FT>
+__global__ void __launch_bounds__(256) k_pisces_omega(PiscesArgs<FT> a)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ncol = (long)a.Nx * a.Ny;
+    if (c >= ncol * a.Nz) return;
+    const int k = (int)(c / ncol);
+    const int col = (int)(c % ncol);
+    const int i = col % a.Nx, j = col / a.Nx;
+    const unsigned p = pidx(a, i, j, k);
+    const double z = (double)a.lev[k];
+    const double g = (double)a.pp[OBME_PP_SCALARS + OBME_PS_GRAVITY];
+    const double P = fabs(z) * g * 1026 / 100000;
+    int ev = 0;
+    a.Omega[c] = (FT)calcite_saturation((double)a.in[OBME_PI_DIC][p], (double)a.in[OBME_PI_ALK][p],
+                                        (double)a.in[OBME_PI_T][p], (double)a.in[OBME_PI_S][p],
+                                        (double)a.in[OBME_PI_SI][p], P, a.solver, ev);
+}
+
+// Thread per cell: transitions, minus the upwind sinking divergence of
+// POC, SFe (wPOC) and GOC, BFe, PSi, CaCO3 (wGOC), then the RK3 substep
+// (STEP) or the addition into the tendency arrays (TENDENCY).
+template <typename FT, int MODE>
+__global__ void __launch_bounds__(256) k_pisces_cell(PiscesArgs<FT> a)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ncol = (long)a.Nx * a.Ny;
+    const long ncell = ncol * a.Nz;
+    if (c >= ncell) return;
+    const int k = (int)(c / ncol);
+    const int col = (int)(c % ncol);
+    const int i = col % a.Nx, j = col / a.Nx;
+    const int Nz = a.Nz;
+    const unsigned p = pidx(a, i, j, k);
+    FT v[NTR];
+#pragma unroll
+    for (int n = 0; n < NTR; n++) v[n] = a.in[n][p];
+    const FT* sc = a.pp + OBME_PP_SCALARS;
+    const FT* wP = a.w;
+    const FT* wG = a.w + (Nz + 1);
+    Aux<FT> A;
+    A.z = a.lev[k];
+    A.zmxl = sc[OBME_PS_MIXED_LAYER_DEPTH];
+    A.zeu = a.zeu[col];
+    A.kappa = sc[OBME_PS_KAPPA];
+    A.mlPAR = a.mlPAR[col];
+    A.Omega = a.Omega[c];
+    A.PAR = a.PAR[c];
+    A.PAR1 = a.PAR[ncell + c];
+    A.PAR2 = a.PAR[2 * ncell + c];
+    A.PAR3 = a.PAR[3 * ncell + c];
+    A.wPOCc = (wP[k] + wP[k + 1]) / 2;
+    A.wGOCc = (wG[k] + wG[k + 1]) / 2;
+    FT G[NW];
+    pisces_tendencies(a.pp, v, A, a.sk, G);
+    // flux-form upwind sinking: (w(k+1) c(k+1) - w(k) c(k)) / dz, w = 0 on the top face
+    const FT idz = a.lev[3 * Nz + k];
+    const unsigned pu = (k + 1 < Nz) ? pidx(a, i, j, k + 1) : p;
+    auto sink = [&](int n, const FT* w) {
+        const FT cu = (k + 1 < Nz) ? a.in[n][pu] : FT(0);
+        G[n] = G[n] - (w[k + 1] * cu - w[k] * v[n]) * idz;
+    };
+    sink(OBME_PI_POC, wP);
+    sink(OBME_PI_SFE, wP);
+    sink(OBME_PI_GOC, wG);
+    sink(OBME_PI_BFE, wG);
+    sink(OBME_PI_PSI, wG);
+    sink(OBME_PI_CACO3, wG);
+    if constexpr (MODE == 1) {
+#pragma unroll
+        for (int n = 0; n < NW; n++) a.out[n][p] = a.out[n][p] + G[n];
+    } else {
+        const double g1 = 8.0 / 15.0, g2 = 5.0 / 12.0, g3 = 3.0 / 4.0, z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
+#pragma unroll
+        for (int n = 0; n < NW; n++) {
+            FT u = v[n];
+            const FT gm = (a.stage == 0) ? FT(0) : a.Gm[(long)n * ncell + c];
+            if constexpr (sizeof(FT) == 8) {
+                // the substep form of BoxModel/timesteppers.jl:79-92
+                if (a.stage == 0) u += (a.dt * g1) * G[n];
+                else if (a.stage == 1) u += a.dt * (g2 * G[n] + z2 * gm);
+                else u += a.dt * (g3 * G[n] + z3 * gm);
+            } else {
+                u = u + fmaf(a.A, G[n], a.B * gm);
+            }
+            a.out[n][p] = u;
+            a.Gm[(long)n * ncell + c] = G[n];
+        }
+    }
+}
+
+}  // namespace
+
+struct obme_pisces {
+    obme_pisces_config cfg;
+    int prec;
+    size_t es;
+    long ncell, ncol;
+    void* d_pp = nullptr;
+    void* d_lev = nullptr;
+    void* d_w = nullptr;
+    void* d_PAR = nullptr;
+    void* d_Omega = nullptr;
+    void* d_zeu = nullptr;
+    void* d_mlPAR = nullptr;
+    void* d_Gm = nullptr;
+    void* d_copy[NW] = {};
+    size_t field_bytes = 0;
+    std::vector<double> zc, zf, dz;
+};
+
+extern "C" obme_status obme_pisces_destroy(obme_pisces* h)
+{
+    if (!h) return OBME_OK;
+    for (void* p : {h->d_pp, h->d_lev, h->d_w, h->d_PAR, h->d_Omega, h->d_zeu, h->d_mlPAR, h->d_Gm})
+        if (p) (void)hipFree(p);
+    for (void* p : h->d_copy)
+        if (p) (void)hipFree(p);
+    delete h;
+    return OBME_OK;
+}
+
+template <typename FT>
+static obme_status pisces_upload(obme_pisces* h)
+{
+    const obme_pisces_config& c = h->cfg;
+    const int Nz = c.grid.Nz;
+    std::vector<FT> pp(OBME_PP_COUNT);
+    for (int n = 0; n < OBME_PP_COUNT; n++) pp[n] = (FT)c.params[n];
+    std::vector<FT> lev(4 * (size_t)Nz);
+    for (int k = 0; k < Nz; k++) {
+        lev[k] = (FT)h->zc[k];
+        lev[Nz + k] = (FT)h->zf[k];
+        lev[2 * Nz + k] = (FT)h->dz[k];
+        lev[3 * Nz + k] = (FT)(1.0 / h->dz[k]);
+    }
+    // face velocities: POC constant speed (Utils/sinking_velocity_fields.jl:10-35);
+    // GOC from DepthDependantSinkingSpeed evaluated once at construction with
+    // zmxl = zeu = 0 (PISCES/common.jl:39-55): face k takes the centre depth of cell k
+    const double* sc = c.params + OBME_PP_SCALARS;
+    const double sPOC = sc[OBME_PS_POC_SPEED], wmin = sc[OBME_PS_GOC_MIN_SPEED], wmax = sc[OBME_PS_GOC_MAX_SPEED];
+    std::vector<FT> w(2 * ((size_t)Nz + 1), (FT)0);
+    for (int k = 1; k <= Nz; k++) {
+        w[k - 1] = (FT)(-sPOC * (c.open_bottom ? 1.0 : (1 - std::exp((1 - k) / 2.0))));
+        const double z = h->zc[k - 1];
+        w[(Nz + 1) + (k - 1)] = (FT)(-wmin + (wmax - wmin) * std::min(0.0, z - std::min(0.0, 0.0)) / 5000);
+    }
+    auto up = [&](void** d, const void* src, size_t bytes) -> obme_status {
+        OBME_HIP_TRY(hipMalloc(d, bytes));
+        OBME_HIP_TRY(hipMemcpy(*d, src, bytes, hipMemcpyHostToDevice));
+        return OBME_OK;
+    };
+    obme_status st = up(&h->d_pp, pp.data(), pp.size() * sizeof(FT));
+    if (st == OBME_OK) st = up(&h->d_lev, lev.data(), lev.size() * sizeof(FT));
+    if (st == OBME_OK) st = up(&h->d_w, w.data(), w.size() * sizeof(FT));
+    return st;
+}
+
+extern "C" obme_status obme_pisces_create(const obme_pisces_config* cfg, obme_pisces** out)
+{
+    if (!cfg || !out) return obme_fail(OBME_ERR_INVALID, "null argument");
+    *out = nullptr;
+    const obme_grid& g = cfg->grid;
+    if (cfg->precision != OBME_FP64 && cfg->precision != OBME_FP32_MIXED)
+        return obme_fail(OBME_ERR_INVALID, "unknown precision");
+    if (g.Nx < 1 || g.Ny < 1 || g.Nz < 2 || g.Hx < 0 || g.Hy < 0 || g.Hz < 0)
+        return obme_fail(OBME_ERR_INVALID, "invalid grid size or halo");
+    if (!g.zc || !g.zf || !g.dz) return obme_fail(OBME_ERR_INVALID, "grid coordinates missing");
+    const size_t es = (cfg->precision == OBME_FP64) ? 8 : 4;
+    const size_t parent = (size_t)(g.Nx + 2 * g.Hx) * (g.Ny + 2 * g.Hy) * (g.Nz + 2 * g.Hz);
+    if (parent >= ((size_t)1 << 31)) return obme_fail(OBME_ERR_UNSUPPORTED, "field too large for 32-bit offsets");
+    if (!(cfg->params[OBME_PP_SCALARS + OBME_PS_MIXED_LAYER_DEPTH] < 0))
+        return obme_fail(OBME_ERR_INVALID, "mixed_layer_depth must be negative (below the surface)");
+    obme_pisces* h = new obme_pisces();
+    h->cfg = *cfg;
+    h->prec = cfg->precision;
+    h->es = es;
+    h->ncol = (long)g.Nx * g.Ny;
+    h->ncell = h->ncol * g.Nz;
+    h->field_bytes = parent * es;
+    h->zc.assign(g.zc, g.zc + g.Nz);
+    h->zf.assign(g.zf, g.zf + g.Nz + 1);
+    h->dz.assign(g.dz, g.dz + g.Nz);
+    h->cfg.grid.zc = h->cfg.grid.zf = h->cfg.grid.dz = nullptr;
+    obme_status st = (h->prec == OBME_FP64) ? pisces_upload<double>(h) : pisces_upload<float>(h);
+    auto alloc = [&](void** d, size_t bytes) {
+        if (st == OBME_OK && hipMalloc(d, bytes) != hipSuccess) st = obme_fail(OBME_ERR_NOMEM, "PISCES buffers");
+    };
+    alloc(&h->d_PAR, 4 * h->ncell * es);
+    alloc(&h->d_Omega, h->ncell * es);
+    alloc(&h->d_zeu, h->ncol * es);
+    alloc(&h->d_mlPAR, h->ncol * es);
+    alloc(&h->d_Gm, (size_t)NW * h->ncell * es);
+    if (st != OBME_OK) {
+        obme_pisces_destroy(h);
+        return st;
+    }
+    *out = h;
+    return OBME_OK;
+}
+
+template <typename FT>
+static PiscesArgs<FT> pisces_args(const obme_pisces* h, double t)
+{
+    const obme_pisces_config& c = h->cfg;
+    const obme_grid& g = c.grid;
+    PiscesArgs<FT> a;
+    std::memset(&a, 0, sizeof(a));
+    a.Gm = (FT*)h->d_Gm;
+    a.PAR = (FT*)h->d_PAR;
+    a.Omega = (FT*)h->d_Omega;
+    a.zeu = (FT*)h->d_zeu;
+    a.mlPAR = (FT*)h->d_mlPAR;
+    a.pp = (const FT*)h->d_pp;
+    a.lev = (const FT*)h->d_lev;
+    a.w = (const FT*)h->d_w;
+    a.zf_top = h->zf[g.Nz];
+    a.Nx = g.Nx;
+    a.Ny = g.Ny;
+    a.Nz = g.Nz;
+    a.sx = (unsigned)(g.Nx + 2 * g.Hx);
+    a.sxy = a.sx * (unsigned)(g.Ny + 2 * g.Hy);
+    a.base0 = (unsigned)g.Hx + a.sx * (unsigned)g.Hy + a.sxy * (unsigned)g.Hz;
+    a.surface_PAR = (FT)c.surface_PAR;
+    for (int n = 0; n < 3; n++) {
+        a.kw[n] = (FT)c.band_kw[n];
+        a.e[n] = (FT)c.band_e[n];
+        a.chi[n] = (FT)c.band_chi[n];
+        a.division[n] = (FT)c.band_division[n];
+    }
+    const double lat = c.params[OBME_PP_SCALARS + OBME_PS_LATITUDE];
+    a.solver = c.solver;
+    (void)t;
+    (void)lat;
+    return a;
+}
+
+// day lengths of the stage clock, on the host in FP64 (the same expressions
+// as the device helper)
+static double host_sind(double x) { return std::sin(std::fmod(x, 360.0) * (3.141592653589793 / 180.0)); }
+static double host_cosd(double x) { return std::cos(std::fmod(x, 360.0) * (3.141592653589793 / 180.0)); }
+static double host_day_length(double t, double phi)
+{
+    const double p = 0.833;
+    double m = std::fmod(t, 365 * 86400.0);
+    if (m < 0) m += 365 * 86400.0;
+    double J = std::floor(m / 86400.0);
+    double theta = 0.216310 + 2 * std::atan(0.9671396 * std::tan(0.00860 * (J - 186)));
+    double dec = std::asin(0.39795 * std::cos(theta)) * (180.0 / 3.141592653589793);
+    double r = (host_sind(p) + host_sind(phi) * host_sind(dec)) / (host_cosd(phi) * host_cosd(dec));
+    double L = std::isnan(r) ? r : std::max(-1.0, std::min(1.0, r));
+    return (24 - 24.0 / 180 * (std::acos(L) * (180.0 / 3.141592653589793))) * 3600.0;
+}
+
+template <typename FT>
+static obme_status pisces_stage(obme_pisces* h, const void* const* in, void* const* out, double t, double dt,
+                                int stage, int mode, hipStream_t s)
+{
+    PiscesArgs<FT> a = pisces_args<FT>(h, t);
+    for (int n = 0; n < NTR; n++) a.in[n] = (const FT*)in[n];
+    for (int n = 0; n < NW; n++) a.out[n] = (FT*)out[n];
+    const double lat = h->cfg.params[OBME_PP_SCALARS + OBME_PS_LATITUDE];
+    a.sk.dl_growth = (FT)host_day_length(lat, t);  // day_length(φ, t), growth_rate.jl:30
+    a.sk.dl_chl = (FT)host_day_length(t, lat);     // day_length(t, φ), growth_rate.jl:128
+    a.dt = dt;
+    a.stage = stage;
+    const double gam[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0}, zet[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+    a.A = (FT)(dt * gam[stage]);
+    a.B = (FT)(dt * zet[stage]);
+    const int bs = 256;
+    hipLaunchKernelGGL(k_pisces_columns<FT>, dim3((unsigned)((h->ncol + bs - 1) / bs)), dim3(bs), 0, s, a);
+    OBME_HIP_TRY(hipGetLastError());
+    const unsigned nb = (unsigned)((h->ncell + bs - 1) / bs);
+    hipLaunchKernelGGL(k_pisces_omega<FT>, dim3(nb), dim3(bs), 0, s, a);
+    OBME_HIP_TRY(hipGetLastError());
+    if (mode == 0) hipLaunchKernelGGL((k_pisces_cell<FT, 0>), dim3(nb), dim3(bs), 0, s, a);
+    else if (mode == 1) hipLaunchKernelGGL((k_pisces_cell<FT, 1>), dim3(nb), dim3(bs), 0, s, a);
+    OBME_HIP_TRY(hipGetLastError());
+    return OBME_OK;
+}
+
+static obme_status pisces_stage_any(obme_pisces* h, const void* const* in, void* const* out, double t, double dt,
+                                    int stage, int mode, hipStream_t s)
+{
+    return (h->prec == OBME_FP64) ? pisces_stage<double>(h, in, out, t, dt, stage, mode, s)
+                                  : pisces_stage<float>(h, in, out, t, dt, stage, mode, s);
+}
+
+template <typename FT>
+static obme_status pisces_copy_aux(obme_pisces* h, void* PAR, void* const* PARn, void* Omega, void* zeu,
+                                   void* mlPAR, hipStream_t s)
+{
+    // compact interior -> parent layout for the 3-D outputs
+    const obme_grid& g = h->cfg.grid;
+    const size_t es = sizeof(FT);
+    const size_t sx = (size_t)(g.Nx + 2 * g.Hx), sxy = sx * (size_t)(g.Ny + 2 * g.Hy);
+    auto put3 = [&](void* dst, const char* src) -> obme_status {
+        if (!dst) return OBME_OK;
+        for (int k = 0; k < g.Nz; k++) {
+            char* d = (char*)dst + ((size_t)g.Hx + sx * (size_t)g.Hy + sxy * (size_t)(k + g.Hz)) * es;
+            const char* c = src + (size_t)k * g.Nx * g.Ny * es;
+            OBME_HIP_TRY(hipMemcpy2DAsync(d, sx * es, c, (size_t)g.Nx * es, (size_t)g.Nx * es, (size_t)g.Ny,
+                                          hipMemcpyDeviceToDevice, s));
+        }
+        return OBME_OK;
+    };
+    const char* P = (const char*)h->d_PAR;
+    obme_status st = put3(PAR, P);
+    for (int n = 0; n < 3 && st == OBME_OK; n++)
+        if (PARn) st = put3(PARn[n], P + (size_t)(n + 1) * h->ncell * es);
+    if (st == OBME_OK) st = put3(Omega, (const char*)h->d_Omega);
+    if (st == OBME_OK && zeu) OBME_HIP_TRY(hipMemcpyAsync(zeu, h->d_zeu, h->ncol * es, hipMemcpyDeviceToDevice, s));
+    if (st == OBME_OK && mlPAR)
+        OBME_HIP_TRY(hipMemcpyAsync(mlPAR, h->d_mlPAR, h->ncol * es, hipMemcpyDeviceToDevice, s));
+    return st;
+}
+
+extern "C" obme_status obme_pisces_update_state(obme_pisces* h, const void* const* tracers, void* PAR,
+                                                void* const* PARn, void* Omega, void* euphotic_depth,
+                                                void* mixed_layer_PAR, double t, void* stream)
+{
+    if (!h || !tracers) return obme_fail(OBME_ERR_INVALID, "null argument");
+    for (int n = 0; n < NTR; n++)
+        if (!tracers[n]) return obme_fail(OBME_ERR_INVALID, "null tracer pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const int bs = 256;
+    if (h->prec == OBME_FP64) {
+        PiscesArgs<double> a = pisces_args<double>(h, t);
+        for (int n = 0; n < NTR; n++) a.in[n] = (const double*)tracers[n];
+        hipLaunchKernelGGL(k_pisces_columns<double>, dim3((unsigned)((h->ncol + bs - 1) / bs)), dim3(bs), 0, s, a);
+        hipLaunchKernelGGL(k_pisces_omega<double>, dim3((unsigned)((h->ncell + bs - 1) / bs)), dim3(bs), 0, s, a);
+        OBME_HIP_TRY(hipGetLastError());
+        return pisces_copy_aux<double>(h, PAR, PARn, Omega, euphotic_depth, mixed_layer_PAR, s);
+    }
+    PiscesArgs<float> a = pisces_args<float>(h, t);
+    for (int n = 0; n < NTR; n++) a.in[n] = (const float*)tracers[n];
+    hipLaunchKernelGGL(k_pisces_columns<float>, dim3((unsigned)((h->ncol + bs - 1) / bs)), dim3(bs), 0, s, a);
+    hipLaunchKernelGGL(k_pisces_omega<float>, dim3((unsigned)((h->ncell + bs - 1) / bs)), dim3(bs), 0, s, a);
+    OBME_HIP_TRY(hipGetLastError());
+    return pisces_copy_aux<float>(h, PAR, PARn, Omega, euphotic_depth, mixed_layer_PAR, s);
+}
+
+extern "C" obme_status obme_pisces_add_tendencies(obme_pisces* h, const void* const* tracers, void* const* G,
+                                                  double t, void* stream)
+{
+    if (!h || !tracers || !G) return obme_fail(OBME_ERR_INVALID, "null argument");
+    for (int n = 0; n < NTR; n++)
+        if (!tracers[n]) return obme_fail(OBME_ERR_INVALID, "null tracer pointer");
+    for (int n = 0; n < NW; n++)
+        if (!G[n]) return obme_fail(OBME_ERR_INVALID, "null tendency pointer");
+    return pisces_stage_any(h, tracers, G, t, 0.0, 0, 1, (hipStream_t)stream);
+}
+
+extern "C" obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, int32_t nsteps, double dt, double t0,
+                                        void* stream)
+{
+    if (!h || !tracers) return obme_fail(OBME_ERR_INVALID, "null argument");
+    if (nsteps < 0) return obme_fail(OBME_ERR_INVALID, "negative nsteps");
+    for (int n = 0; n < NTR; n++)
+        if (!tracers[n]) return obme_fail(OBME_ERR_INVALID, "null tracer pointer");
+    if (nsteps == 0) return OBME_OK;
+    hipStream_t s = (hipStream_t)stream;
+    for (int n = 0; n < NW; n++)
+        if (!h->d_copy[n] && hipMalloc(&h->d_copy[n], h->field_bytes) != hipSuccess)
+            return obme_fail(OBME_ERR_NOMEM, "PISCES stage copy");
+    // Stage inputs and outputs alternate between the caller's arrays (U) and
+    // library copies (C), which start as full copies of U so that their halos
+    // match; the kernels write interiors only.  A step ends on the other side
+    // of where it started; an odd number of steps ends with a copy back.
+    for (int n = 0; n < NW; n++)
+        OBME_HIP_TRY(hipMemcpyAsync(h->d_copy[n], tracers[n], h->field_bytes, hipMemcpyDeviceToDevice, s));
+    const void* inU[NTR];
+    const void* inC[NTR];
+    void* outU[NW];
+    void* outC[NW];
+    for (int n = 0; n < NTR; n++) {
+        inU[n] = tracers[n];
+        inC[n] = (n < NW) ? h->d_copy[n] : tracers[n];
+    }
+    for (int n = 0; n < NW; n++) {
+        outU[n] = tracers[n];
+        outC[n] = h->d_copy[n];
+    }
+    const double cst[3] = {0.0, 8.0 / 15.0, 2.0 / 3.0};
+    bool inCopy = false;  // where the current state lives
+    for (int st = 0; st < nsteps; st++) {
+        const double t = t0 + st * dt;
+        for (int q = 0; q < 3; q++) {
+            const void* const* in = inCopy ? inC : inU;
+            void* const* out = inCopy ? outU : outC;
+            obme_status r = pisces_stage_any(h, in, out, t + cst[q] * dt, dt, q, 0, s);
+            if (r != OBME_OK) return r;
+            inCopy = !inCopy;
+        }
+    }
+    if (inCopy)
+        for (int n = 0; n < NW; n++)
+            OBME_HIP_TRY(hipMemcpyAsync(tracers[n], h->d_copy[n], h->field_bytes, hipMemcpyDeviceToDevice, s));
+    return OBME_OK;
+}

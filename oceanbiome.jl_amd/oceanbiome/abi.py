@@ -150,6 +150,25 @@ class BatchParams(C.Structure):
     _fields_ = [("fields", C.c_void_p * 40)]
 
 
+OBME_PP_COUNT = 159   # obme_pisces_config.params length (2*31 + 2*18 + 61)
+
+
+class PiscesConfig(C.Structure):
+    """obme_pisces_config."""
+    _fields_ = [
+        ("precision", C.c_int32),
+        ("open_bottom", C.c_int32),
+        ("grid", Grid),
+        ("params", C.c_double * OBME_PP_COUNT),
+        ("band_kw", C.c_double * 3),
+        ("band_e", C.c_double * 3),
+        ("band_chi", C.c_double * 3),
+        ("band_division", C.c_double * 3),
+        ("surface_PAR", C.c_double),
+        ("solver", CarbonSolver),
+    ]
+
+
 # per-column parameter rows (OBME_BP_*)
 (OBME_BP_NITRATE_HALF_SATURATION, OBME_BP_LIGHT_HALF_SATURATION, OBME_BP_MAX_GROWTH_RATE,
  OBME_BP_EXUDATION_FRACTION, OBME_BP_AMMONIA_FRACTION_OF_EXUDATE, OBME_BP_TEMPERATURE_COEFFICIENT,
@@ -171,4 +190,5 @@ ABI_STRUCTS = (
     ("obme_co2_exchange", CO2Exchange),
     ("obme_field_source", FieldSource),
     ("obme_batch_params", BatchParams),
+    ("obme_pisces_config", PiscesConfig),
 )

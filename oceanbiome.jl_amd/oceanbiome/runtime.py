@@ -56,6 +56,12 @@ def library():
         "obme_batch_configure": (C.c_int32, [vp, C.c_int32, C.c_int32]),
         "obme_batch_autotune": (C.c_int32, [vp, C.POINTER(vp), C.c_int32, C.c_double, vp]),
         "obme_batch_describe": (C.c_int32, [vp, C.c_char_p, C.c_int32]),
+        "obme_pisces_create": (C.c_int32, [C.POINTER(abi.PiscesConfig), C.POINTER(vp)]),
+        "obme_pisces_destroy": (C.c_int32, [vp]),
+        "obme_pisces_update_state": (C.c_int32, [vp, C.POINTER(vp), vp, C.POINTER(vp), vp, vp, vp, C.c_double,
+                                                 vp]),
+        "obme_pisces_add_tendencies": (C.c_int32, [vp, C.POINTER(vp), C.POINTER(vp), C.c_double, vp]),
+        "obme_pisces_step": (C.c_int32, [vp, C.POINTER(vp), C.c_int32, C.c_double, C.c_double, vp]),
         "obme_kernel_timing": (C.c_int32, [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
@@ -70,7 +76,8 @@ EXPORTED_SYMBOLS = (
     "obme_create", "obme_destroy", "obme_last_error", "obme_version", "obme_required_tracers",
     "obme_update_state", "obme_add_tendencies", "obme_step", "obme_co2_flux", "obme_abi_sizeof",
     "obme_set_kernel_timing", "obme_kernel_timing", "obme_batch_create", "obme_batch_destroy", "obme_batch_step",
-    "obme_batch_configure", "obme_batch_autotune", "obme_batch_describe",
+    "obme_batch_configure", "obme_batch_autotune", "obme_batch_describe", "obme_pisces_create",
+    "obme_pisces_destroy", "obme_pisces_update_state", "obme_pisces_add_tendencies", "obme_pisces_step",
 )
 
 KERNEL_SWEEP = 0      # OBME_KERNEL_SWEEP: the column sweep
@@ -261,6 +268,68 @@ class HIPBatch:
         if n < 0:
             raise OBMEError("obme_batch_describe failed")
         return buf.value.decode()
+
+
+class HIPPISCES:
+    """PISCES on the GPU (obme_pisces_*) for an oceanbiome.pisces.PISCES
+    description.  Tracers: 26 CUDA tensors (tracer order, parent arrays; the
+    last two, T and S, are read only)."""
+
+    def __init__(self, model, precision="fp64"):
+        import torch
+        if not torch.cuda.is_available():
+            raise OBMEError("HIPPISCES needs a GPU")
+        self.model = model
+        self.precision = abi.OBME_FP64 if precision == "fp64" else abi.OBME_FP32_MIXED
+        self.dtype = torch.float64 if self.precision == abi.OBME_FP64 else torch.float32
+        self.cfg = model.config(self.precision)
+        h = C.c_void_p()
+        _check(library().obme_pisces_create(C.byref(self.cfg), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if getattr(self, "handle", None):
+            library().obme_pisces_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check_fields(self, fields, n):
+        g = self.model.grid
+        if len(fields) != n:
+            raise OBMEError(f"expected {n} fields")
+        for f in fields:
+            if f.dtype != self.dtype or tuple(f.shape) != g.parent_shape or not f.is_cuda or not f.is_contiguous():
+                raise OBMEError(f"field must be a contiguous {self.dtype} CUDA tensor of shape {g.parent_shape}")
+
+    def update_biogeochemical_state(self, tracers, t=0.0):
+        """(PAR, [PAR1, PAR2, PAR3], Omega, euphotic_depth, mixed_layer_PAR):
+        3-D parent arrays and 2-D (Ny, Nx) tensors."""
+        import torch
+        self._check_fields(tracers, 26)
+        g = self.model.grid
+        mk3 = lambda: torch.zeros(g.parent_shape, dtype=self.dtype, device="cuda")
+        PAR, bands, Om = mk3(), [mk3() for _ in range(3)], mk3()
+        zeu = torch.zeros((g.Ny, g.Nx), dtype=self.dtype, device="cuda")
+        ml = torch.zeros((g.Ny, g.Nx), dtype=self.dtype, device="cuda")
+        _check(library().obme_pisces_update_state(self.handle, _ptr_array(tracers), _ptr(PAR), _ptr_array(bands),
+                                                  _ptr(Om), _ptr(zeu), _ptr(ml), float(t), _stream()))
+        return PAR, bands, Om, zeu, ml
+
+    def update_tendencies(self, tracers, G, t=0.0):
+        self._check_fields(tracers, 26)
+        self._check_fields(G, 24)
+        _check(library().obme_pisces_add_tendencies(self.handle, _ptr_array(tracers), _ptr_array(G), float(t),
+                                                    _stream()))
+
+    def step(self, tracers, nsteps, dt, t0=0.0):
+        self._check_fields(tracers, 26)
+        _check(library().obme_pisces_step(self.handle, _ptr_array(tracers), int(nsteps), float(dt), float(t0),
+                                          _stream()))
 
 
 def abi_layout_mismatches():

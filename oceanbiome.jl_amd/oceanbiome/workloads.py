@@ -11,6 +11,9 @@ number of steps.  Random draws use numpy's PCG64 with the given seed.
  - npzd_columns: ensemble of independent NPZD columns of ragged depth with
                  per-column parameters and surface PAR.
  - carbonate_points: DIC/Alk/T/S/u10 point sets for the CO2 flux.
+ - pisces_3d:    PISCES on Nx×Ny×Nz (default 256×256×64, 500 m spacing, 1 km
+                 deep), tracers lognormal (σ = 0.3) around the reference's
+                 test values, T = (2 + 26 (j-½)/Ny) exp(z/500), S = 35.
 """
 from dataclasses import dataclass, field
 from typing import Optional
@@ -48,7 +51,9 @@ class Workload:
         """Algorithmic HBM bytes of one step: every written tracer read and
         written once, plus read-only 3-D inputs (NPZD's T tracer)."""
         n = self.grid.n_interior
-        n_ro = 1 if self.bgc.model == abi.OBME_NPZD else 0
+        n_ro = getattr(self.bgc, "read_only_3d", None)
+        if n_ro is None:
+            n_ro = 1 if self.bgc.model == abi.OBME_NPZD else 0
         ro = state_bytes if ro_bytes is None else ro_bytes
         return n * (2 * self.n_written * state_bytes + n_ro * ro)
 
@@ -165,6 +170,40 @@ def npzd_columns(n_columns=8192, nz_min=16, nz_max=128, dz=2.0, nsteps=500, dt=6
                light_attenuation=TwoBandPhotosyntheticallyActiveRadiation(surface_PAR=100.0))
     return Workload(f"npzd_columns_{n_columns}", bgc, bgc.grid, [N, P, Z, D], dt, nsteps,
                     extra={"offsets": offsets, "params": params, "dz": dz, "T": T})
+
+
+# medians of the PISCES tracers (test/test_PISCES.jl:6-16)
+PISCES_MEDIANS = dict(P=0.5, PChl=0.02, PFe=0.005, D=0.1, DChl=0.004, DFe=0.001, DSi=0.01, Z=0.1, M=0.7, DOC=2.1,
+                      POC=7.8, GOC=38.0, SFe=0.206, BFe=1.1, PSi=0.1, CaCO3=0.1, NO3=2.3, NH4=0.9, PO4=0.6,
+                      Fe=0.13, Si=8.5, DIC=2205.0, Alk=2566.0, O2=317.0)
+
+
+def pisces_3d(Nx=256, Ny=256, Nz=64, nsteps=100, dt=300.0, seed=42, Lz=1000.0, spacing=500.0,
+              surface_PAR=100.0, mixed_layer_depth=-50.0):
+    """PISCES on a periodic box: 24 tracers lognormal(σ = 0.3) around the
+    reference's test values, T = (2 + 26 (j-½)/Ny) exp(z/500) °C, S = 35,
+    constant surface PAR, prescribed mixed-layer depth."""
+    from .pisces import PISCES, TRACERS
+    grid = RectilinearGrid(size=(Nx, Ny, Nz), extent=(Nx * spacing, Ny * spacing, Lz), halo=(3, 3, 3),
+                           topology=("Periodic", "Periodic", "Bounded"))
+    model = PISCES(grid, surface_PAR=surface_PAR, mixed_layer_depth=mixed_layer_depth)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    shape = (Nz, Ny, Nx)
+    fields = []
+    keys = list(PISCES_MEDIANS)
+    for name in keys:
+        f = grid.new_field()
+        grid.interior(f)[...] = PISCES_MEDIANS[name] * np.exp(0.3 * rng.standard_normal(shape))
+        fields.append(f)
+    jj = (np.arange(Ny) + 0.5) / Ny
+    T = (2 + 26 * jj)[None, :, None] * np.exp(grid.zc / 500)[:, None, None] * np.ones((1, 1, Nx))
+    fT = grid.new_field()
+    grid.interior(fT)[...] = T
+    fS = grid.new_field(fill=0.0)
+    grid.interior(fS)[...] = 35.0
+    fields += [fT, fS]
+    assert len(fields) == len(TRACERS)
+    return Workload(f"pisces_{Nx}x{Ny}x{Nz}", model, grid, fields, dt, nsteps)
 
 
 def carbonate_points(n=1 << 24, seed=42, pCO2_air=420.0):

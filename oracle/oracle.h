@@ -65,6 +65,12 @@ double orc_carbonate_concentration(double DIC, double Alk, double T, double S,
 double orc_calcite_saturation(double DIC, double Alk, double T, double S,
                               int has_P, double P, const obme_carbon_solver* s);
 
+double orc_calcite_saturation_si(double DIC, double Alk, double T, double S, int has_P, double P,
+                                 double silicate, const obme_carbon_solver* s);
+void orc_calcite_saturation_batch(const double* DIC, const double* Alk, const double* T, const double* S,
+                                  const double* P, const double* Si, double* out, long n,
+                                  const obme_carbon_solver* s, int nthreads);
+
 double orc_schmidt_co2(const obme_co2_exchange* g, double T);
 double orc_transfer_velocity(const obme_co2_exchange* g, double u10, double T, double S);
 /* F = k (pCO2_w - pCO2_air); also returns the free-scale pH of the solve. */

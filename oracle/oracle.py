@@ -61,6 +61,9 @@ def lib():
         "orc_carbon_chemistry": (D, [D, D, D, D, D, I, C.POINTER(abi.CarbonSolver), I, D, D, C.POINTER(I)]),
         "orc_carbonate_concentration": (D, [D, D, D, D, I, D, C.POINTER(abi.CarbonSolver)]),
         "orc_calcite_saturation": (D, [D, D, D, D, I, D, C.POINTER(abi.CarbonSolver)]),
+        "orc_calcite_saturation_si": (D, [D, D, D, D, I, D, D, C.POINTER(abi.CarbonSolver)]),
+        "orc_calcite_saturation_batch": (None, [DP, DP, DP, DP, DP, DP, DP, C.c_long, C.POINTER(abi.CarbonSolver),
+                                                I]),
         "orc_schmidt_co2": (D, [C.POINTER(abi.CO2Exchange), D]),
         "orc_transfer_velocity": (D, [C.POINTER(abi.CO2Exchange), D, D, D]),
         "orc_co2_flux": (D, [C.POINTER(abi.CO2Exchange), C.POINTER(abi.CarbonSolver), D, D, D, D, D, D, I,
@@ -202,6 +205,18 @@ def co2_flux(gas, solver, DIC, Alk, T, S, u10, pCO2_air, fast_exit=1):
     F = lib().orc_co2_flux(C.byref(gas), C.byref(solver), DIC, Alk, T, S, u10, pCO2_air, fast_exit,
                            C.byref(pH), C.byref(ev))
     return F, pH.value, ev.value
+
+
+def calcite_saturation(solver, DIC, Alk, T, S, P=None, silicate=0.0):
+    return lib().orc_calcite_saturation_si(DIC, Alk, T, S, 0 if P is None else 1, 0.0 if P is None else P,
+                                           silicate, C.byref(solver))
+
+
+def calcite_saturation_batch(solver, DIC, Alk, T, S, P, Si, nthreads=8):
+    arrs = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (DIC, Alk, T, S, P, Si)]
+    out = np.empty(arrs[0].size)
+    lib().orc_calcite_saturation_batch(*[_dp(a) for a in arrs], _dp(out), out.size, C.byref(solver), nthreads)
+    return out.reshape(np.shape(DIC))
 
 
 def co2_flux_batch(gas, solver, DIC, Alk, T, S, u10, pCO2_air, nthreads=8):

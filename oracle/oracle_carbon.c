@@ -565,6 +565,31 @@ double orc_calcite_saturation(double DIC, double Alk, double T, double S, int ha
     return ca * co3 / K;
 }
 
+/* calcite_saturation with a silicate concentration (mmol/m3), as PISCES
+ * calls it per cell (PISCES/compute_calcite_saturation.jl:22-36). */
+double orc_calcite_saturation_si(double DIC, double Alk, double T, double S, int has_P, double P, double silicate,
+                                 const obme_carbon_solver* s)
+{
+    orc_carbon_params p;
+    orc_constants k;
+    double Tk;
+    fill_params(DIC, Alk, T, S, 0, has_P ? P : 0.0, has_P, P, silicate, 0.0, &p, &k, &Tk);
+    double H = orc_solve_for_H(&p, s, pow(10.0, -s->initial_pH_guess), 1, NULL, NULL);
+    double denom1 = (H * (H + k.K1));
+    double denom2 = (1.0 + k.K1 * k.K2 / denom1);
+    double co3 = p.DIC * k.K1 * k.K2 / denom1 / denom2;
+    double ca = 0.0103 * S / 35;
+    return ca * co3 / orc_ksp_calcite(T + 273.15, S, has_P, P);
+}
+
+void orc_calcite_saturation_batch(const double* DIC, const double* Alk, const double* T, const double* S,
+                                  const double* P, const double* Si, double* out, long n,
+                                  const obme_carbon_solver* s, int nthreads)
+{
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256)
+    for (long q = 0; q < n; q++) out[q] = orc_calcite_saturation_si(DIC[q], Alk[q], T[q], S[q], 1, P[q], Si[q], s);
+}
+
 /* ---------------- gas exchange ---------------- */
 
 /* CarbonDioxidePolynomialSchmidtNumber (GasExchange/gas_solubility.jl:211-212)

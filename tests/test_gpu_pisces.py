@@ -1,0 +1,90 @@
+"""GPU parity of PISCES (obme_pisces_*) against the numpy FP64 oracle
+(oracle/pisces.py) on the same inputs: the update-state fields (MultiBand
+PAR, euphotic depth, mixed-layer PAR, calcite saturation), the tendencies
+and a few RK3 steps.  FP64 bar: relative L2 <= 1e-12 per tracer after the
+steps (1e-10 for the raw tendencies, where the free-iron root's cancellation
+amplifies one-ulp differences of exp between libraries); FP32-mixed: 1e-5."""
+import numpy as np
+import pytest
+
+import pisces as OP
+from helpers import rel_l2
+from oceanbiome import workloads
+from oceanbiome.runtime import HIPPISCES
+
+pytestmark = pytest.mark.gpu
+
+NAMES = OP.TRACERS
+
+
+def _oracle_inputs(wl):
+    g = wl.grid
+    m = wl.bgc
+    q = OP.params_from_vector(m.param_vector())
+    light = dict(surface_PAR=m.surface_PAR, kw=m.kw, e=m.e, chi=m.chi, division=m.division)
+    F = {n: g.interior(a).copy() for n, a in zip(NAMES, wl.tracers)}
+    cc = m.carbon_chemistry
+    solver = cc.solver.struct(cc.initial_pH_guess)
+    return q, light, F, np.asarray(g.zc), np.asarray(g.zf), np.asarray(g.dz), solver
+
+
+def _device(wl, precision):
+    import torch
+    dev = HIPPISCES(wl.bgc, precision)
+    tr = [torch.from_numpy(a).to(device="cuda", dtype=dev.dtype) for a in wl.tracers]
+    return dev, tr
+
+
+def test_update_state_fp64():
+    import torch
+    print("case pisces update_state fp64", flush=True)
+    wl = workloads.pisces_3d(12, 5, 30, nsteps=1, seed=3)
+    q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
+    A = OP.update_state(q, light, F, zc, zf, 0.0, solver)
+    dev, tr = _device(wl, "fp64")
+    PAR, bands, Om, zeu, ml = dev.update_biogeochemical_state(tr, 0.0)
+    torch.cuda.synchronize()
+    g = wl.grid
+    assert rel_l2(g.interior(PAR.cpu().numpy()), A["PAR"]) <= 1e-13
+    for n in range(3):
+        assert rel_l2(g.interior(bands[n].cpu().numpy()), A[f"PAR{n + 1}"]) <= 1e-13
+    assert rel_l2(g.interior(Om.cpu().numpy()), A["Omega"]) <= 1e-12
+    assert rel_l2(zeu.cpu().numpy(), A["zeu"][0]) <= 1e-13
+    assert rel_l2(ml.cpu().numpy(), A["mlPAR"][0]) <= 1e-13
+
+
+def test_tendencies_fp64():
+    import torch
+    print("case pisces tendencies fp64", flush=True)
+    wl = workloads.pisces_3d(10, 4, 24, nsteps=1, seed=5)
+    q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
+    G, _ = OP.full_tendencies(q, light, F, zc, zf, dz, 1.0e5, solver)
+    dev, tr = _device(wl, "fp64")
+    g = wl.grid
+    Gd = [torch.zeros(g.parent_shape, dtype=torch.float64, device="cuda") for _ in range(24)]
+    dev.update_tendencies(tr, Gd, t=1.0e5)
+    torch.cuda.synchronize()
+    worst = {}
+    for n, name in enumerate(OP.WRITTEN):
+        worst[name] = rel_l2(g.interior(Gd[n].cpu().numpy()), G[name])
+    print("tendency errors", worst, flush=True)
+    assert all(e <= 1e-10 for e in worst.values()), worst
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_step(precision):
+    import torch
+    print("case pisces step", precision, flush=True)
+    wl = workloads.pisces_3d(8, 4, 20, nsteps=3, dt=300.0, seed=7)
+    q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
+    OP.step(q, light, F, zc, zf, dz, wl.nsteps, wl.dt, 0.0, solver)
+    dev, tr = _device(wl, precision)
+    dev.step(tr, wl.nsteps, wl.dt)
+    torch.cuda.synchronize()
+    g = wl.grid
+    errs = {name: rel_l2(g.interior(tr[n].double().cpu().numpy()), F[name]) for n, name in enumerate(OP.WRITTEN)}
+    print("errors", errs, flush=True)
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    assert all(e <= tol for e in errs.values()), errs
+    # T and S untouched
+    assert np.array_equal(tr[24].double().cpu().numpy(), wl.tracers[24].astype(tr[24].cpu().numpy().dtype))
