@@ -3,7 +3,7 @@
 # Packed FP32 instructions are disabled (-packed-fp32-ops).
 set -euo pipefail
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
-OUT="${HERE}/../oceanbiome/libobme_hip.so"
+OUT="${OBME_OUT:-${HERE}/../oceanbiome/libobme_hip.so}"
 HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
 "${HIPCC}" --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared \
     -Xclang -target-feature -Xclang -packed-fp32-ops \

@@ -47,9 +47,14 @@ extern "C" const char* obme_version(void) { return "obme 0.1.0 gfx950"; }
 // The sweep: one workgroup of NW waves, wave-per-column segments
 // (obme_npd_kernels.h).  __launch_bounds__(NW*64) lets the compiler use
 // 512/(NW/4) VGPRs per lane at most (128 for 16 waves, 256 for 8).
-// FP32 tiles of up to 2 levels per lane are held to 128 VGPRs (4 waves per SIMD).
+// FP32 tiles of up to 2 levels per lane are held to 512 / OBME_F32_WAVES_PER_EU
+// VGPRs (128: 4 waves per SIMD).
+#ifndef OBME_F32_WAVES_PER_EU
+#define OBME_F32_WAVES_PER_EU 4
+#endif
 template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL>
-__global__ void __launch_bounds__(NW * 64, (sizeof(FT) == 4 && L <= 2) ? 4 : 1) k_column_wave(StepArgs<FT> a)
+__global__ void __launch_bounds__(NW * 64, (sizeof(FT) == 4 && L <= 2) ? OBME_F32_WAVES_PER_EU : 1)
+    k_column_wave(StepArgs<FT> a)
 {
     extern __shared__ unsigned char smem_raw[];
     column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, UNIFORM, FULL>(a, reinterpret_cast<FT*>(smem_raw));
@@ -450,7 +455,8 @@ static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_
     const size_t lds = (size_t)ntiles * h->cfg.grid.Nz * (TXB + 1) * sizeof(FT);
     if (lds > 160 * 1024) return fail(OBME_ERR_UNSUPPORTED, "sweep tile exceeds the 160 KiB LDS");
     dim3 block(NW * 64);
-    dim3 grid((h->cfg.grid.Nx + TXB - 1) / TXB, h->cfg.grid.Ny);
+    // one-dimensional: tile = xcd_tile(blockIdx.x), row-major over (i tile, j)
+    dim3 grid((unsigned)(((h->cfg.grid.Nx + TXB - 1) / TXB) * h->cfg.grid.Ny));
     // FULL (every lane level is a cell) is compiled for the fused step only
     const bool full = (MODE == TILE_STEP) && h->cfg.grid.Nz == L * 64 / CPW;
     auto k = h->uniform ? (full ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, MODE == TILE_STEP>

@@ -151,6 +151,20 @@ __device__ __forceinline__ T seg_prefix(T x)
     return x;
 }
 
+// Tile of workgroup b of an nb-workgroup launch.  The dispatcher hands
+// workgroup b to XCD b % 8 (round robin over the 8 XCDs of MI355X, each
+// with its own L2).  Each XCD is given a contiguous run of tiles instead,
+// so tiles that are neighbours along i, which share the 64-byte segments
+// at their edges (a tile row is TXB values from an arbitrary offset), are
+// read into and written back from one L2, by workgroups that run at about
+// the same time.  A bijection of [0, nb).
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)
+{
+    constexpr unsigned X = 8;
+    const unsigned x = b % X, s = b / X, q = nb / X, rem = nb % X;
+    return x * q + (x < rem ? x : rem) + s;
+}
+
 template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL>
 __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
 {
@@ -165,8 +179,10 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
     const int w = threadIdx.x >> 6;
     const int r = lane % LPC;              // rank in the column, 0 = top
     const int c = w * CPW + lane / LPC;    // column in the workgroup
-    const int j = blockIdx.y;
-    const int ibase = blockIdx.x * TXB;
+    const unsigned wg_tile = xcd_tile(blockIdx.x, gridDim.x);
+    const unsigned nbx = ((unsigned)a.Nx + TXB - 1) / TXB;
+    const int j = (int)(wg_tile / nbx);
+    const int ibase = (int)(wg_tile % nbx) * TXB;
     const int Nz = a.Nz;
     const int ktop = Nz - 1;
     const int i_raw = ibase + c;
