@@ -46,7 +46,7 @@ def parse():
     p.add_argument("--no-check", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--precisions", default="fp32,fp64")
-    p.add_argument("--workload", default="lobster", choices=["lobster", "columns", "carbonate"],
+    p.add_argument("--workload", default="lobster", choices=["lobster", "columns", "carbonate", "pisces"],
                    help="lobster: the headline LOBSTER+DIC/Alk step; columns: the ragged NPZD column "
                         "ensemble; carbonate: pH and CO2 flux on point sets")
     p.add_argument("--columns", type=int, default=8192)
@@ -120,6 +120,8 @@ def main():
         return bench_columns(args)
     if args.workload == "carbonate":
         return bench_carbonate(args)
+    if args.workload == "pisces":
+        return bench_pisces(args)
     import torch
     from oceanbiome import abi, workloads
     from oceanbiome.runtime import HIPBiogeochemistry
@@ -354,6 +356,117 @@ def bench_carbonate(args):
         "cpu_baseline": {"value": ns / cpu_s, "unit": "points/s", "cores": nthreads, "kind": "port",
                          "sample": f"{ns} points, FP64 oracle"},
         "results": res}))
+
+
+
+def _pisces_oracle_inputs(wl, cols=None):
+    """Oracle arguments for the PISCES workload, on all columns or on the
+    columns `cols` (flat indices j*Nx + i) gathered into one row."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pisces as OP
+    g, m = wl.grid, wl.bgc
+    q = OP.params_from_vector(m.param_vector())
+    light = dict(surface_PAR=m.surface_PAR, kw=m.kw, e=m.e, chi=m.chi, division=m.division)
+    F = {}
+    for name, a in zip(OP.TRACERS, wl.tracers):
+        x = g.interior(a)
+        if cols is not None:
+            x = x.reshape(g.Nz, -1)[:, cols].reshape(g.Nz, 1, len(cols))
+        F[name] = np.ascontiguousarray(x, dtype=np.float64)
+    cc = m.carbon_chemistry
+    solver = cc.solver.struct(cc.initial_pH_guess)
+    return OP, q, light, F, np.asarray(g.zc), np.asarray(g.zf), np.asarray(g.dz), solver
+
+
+def bench_pisces(args):
+    """PISCES on 256x256x64 (MultiBand PAR, Omega, 24 tendencies, sinking,
+    RK3), 300 s steps: FP32-mixed (FP64 Omega) and FP64 timed on the device;
+    both checked against the FP64 oracle on a sample of columns after the
+    check run; the oracle timed on a bounded sample as the CPU baseline."""
+    import torch
+    from oceanbiome import workloads
+    from oceanbiome.runtime import HIPPISCES
+    steps = args.steps if args.steps != 200 else 10
+    nchk = args.check_steps if args.check_steps != 200 else 100
+    wl = workloads.pisces_3d(256, 256, 64, nsteps=nchk, dt=300.0)
+    g = wl.grid
+    ncell = g.n_interior
+    nw = 24
+    res, outputs = {}, {}
+    for prec in args.precisions.split(","):
+        dev = HIPPISCES(wl.bgc, prec)
+        tr = [torch.from_numpy(a).to(device="cuda", dtype=dev.dtype) for a in wl.tracers]
+        if args.warmup:
+            dev.step(tr, args.warmup, wl.dt)
+        torch.cuda.synchronize()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        dev.step(tr, steps, wl.dt, t0=args.warmup * wl.dt)
+        e.record()
+        torch.cuda.synchronize()
+        secs = s.elapsed_time(e) / 1e3
+        sb = 4 if prec == "fp32" else 8
+        bytes_step = wl.algorithmic_bytes_per_step(sb)
+        res[prec] = {"ms_per_step": secs / steps * 1e3, "updates_per_s": nw * ncell * steps / secs,
+                     "achieved_GBs": bytes_step / (secs / steps) / 1e9, "bytes_per_step": bytes_step}
+        log(f"pisces {prec}: {res[prec]['ms_per_step']:.3f} ms/step  {res[prec]['updates_per_s']:.3e} upd/s  "
+            f"{res[prec]['achieved_GBs']:.0f} GB/s")
+        if not args.no_check:
+            tr = [torch.from_numpy(a).to(device="cuda", dtype=dev.dtype) for a in wl.tracers]
+            dev.step(tr, nchk, wl.dt)
+            torch.cuda.synchronize()
+            outputs[prec] = [t.double().cpu().numpy() for t in tr]
+        del tr, dev
+        torch.cuda.empty_cache()
+    errors, cpu = {}, None
+    nthreads = cpu_threads(args.cpu_threads)
+    if not args.no_check:
+        rng = np.random.default_rng(1)
+        cols = np.sort(rng.choice(g.Nx * g.Ny, min(args.sample_columns, g.Nx * g.Ny), replace=False))
+        OP, q, light, F, zc, zf, dz, solver = _pisces_oracle_inputs(wl, cols)
+        t0 = time.perf_counter()
+        OP.step(q, light, F, zc, zf, dz, nchk, wl.dt, 0.0, solver, nthreads=nthreads)
+        log(f"oracle check run: {len(cols)} columns x {nchk} steps in {time.perf_counter() - t0:.1f} s")
+        for prec, out in outputs.items():
+            e = {}
+            for n, name in enumerate(OP.WRITTEN):
+                a = g.interior(out[n]).reshape(g.Nz, -1)[:, cols].reshape(F[name].shape)
+                b = F[name]
+                e[name] = {"rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                           "max_rel": float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))}
+            errors[prec] = e
+            worst = max(v["rel_l2"] for v in e.values())
+            log(f"pisces {prec} vs oracle after {nchk} steps ({len(cols)} columns): worst rel L2 {worst:.3e}")
+    if not args.no_cpu:
+        rng = np.random.default_rng(2)
+        ncol = 4096
+        cols = np.sort(rng.choice(g.Nx * g.Ny, ncol, replace=False))
+        OP, q, light, F, zc, zf, dz, solver = _pisces_oracle_inputs(wl, cols)
+        t0 = time.perf_counter()
+        OP.step(q, light, F, zc, zf, dz, 1, wl.dt, 0.0, solver, nthreads=nthreads)
+        per = time.perf_counter() - t0
+        n = int(max(1, min(100, round(args.cpu_seconds / per))))
+        t0 = time.perf_counter()
+        OP.step(q, light, F, zc, zf, dz, n, wl.dt, wl.dt, solver, nthreads=nthreads)
+        secs = time.perf_counter() - t0
+        cpu = {"value": nw * g.Nz * ncol * n / secs, "unit": "tracer-cell updates/s", "cores": nthreads,
+               "kind": "port",
+               "sample": f"{ncol} columns x {g.Nz} levels x {n} steps of the same workload, FP64 numpy oracle "
+                         f"(calcite saturation in C over {nthreads} threads, the rest single-threaded), {secs:.1f} s"}
+        log(f"cpu baseline: {cpu['value']:.3e} upd/s ({secs:.1f} s)")
+    head_prec = "fp32" if "fp32" in res else "fp64"
+    head = res[head_prec]
+    print(json.dumps({
+        "metric": "tracer-cell updates/s + % HBM roofline (PISCES)", "value": head["updates_per_s"],
+        "unit": "tracer-cell updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if head_prec == "fp32" else "f64", "data": "synthetic (seeded lognormal tracers)",
+        "config": {"workload": wl.name, "grid": [g.Nx, g.Ny, g.Nz], "tracers_written": nw, "dt": wl.dt},
+        "roofline": {"bound": "hbm", "achieved": head["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": head["achieved_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "whole PISCES step", "algorithmic_bytes_per_launch": head["bytes_per_step"],
+                     "launch_ms": head["ms_per_step"]},
+        "cpu_baseline": cpu, "results": res, "errors_vs_oracle": errors}))
 
 
 if __name__ == "__main__":

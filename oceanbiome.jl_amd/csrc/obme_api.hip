@@ -205,7 +205,7 @@ struct TileCfg {
     int L, CPW, NW;
 };
 static const TileCfg TILE_CFGS[] = {
-    {2, 2, 8}, {1, 1, 16}, {2, 1, 8}, {2, 2, 16}, {1, 1, 8}, {4, 2, 8},
+    {2, 2, 8}, {1, 1, 16}, {2, 1, 8}, {2, 2, 16}, {1, 1, 8}, {4, 2, 8}, {2, 2, 4}, {1, 1, 4},
 };
 static const int N_TILE_CFG = sizeof(TILE_CFGS) / sizeof(TILE_CFGS[0]);
 
@@ -213,7 +213,7 @@ static int choose_tile(int prec, int Nz)
 {
     // preference lists by precision (indices into TILE_CFGS)
     static const int pref32[] = {0, 2, 5};
-    static const int pref64[] = {1, 2, 5};
+    static const int pref64[] = {7, 2, 5};
     const int* p = (prec == OBME_FP64) ? pref64 : pref32;
     for (int q = 0; q < 3; q++) {
         const TileCfg& t = TILE_CFGS[p[q]];
@@ -490,6 +490,8 @@ static obme_status dispatch_tile_cfg(obme_handle* h, const StepArgs<FT>& a, hipS
     case 3: return dispatch_model<FT, 2, 2, 16, MODE>(h, a, s);
     case 4: return dispatch_model<FT, 1, 1, 8, MODE>(h, a, s);
     case 5: return dispatch_model<FT, 4, 2, 8, MODE>(h, a, s);
+    case 6: return dispatch_model<FT, 2, 2, 4, MODE>(h, a, s);
+    case 7: return dispatch_model<FT, 1, 1, 4, MODE>(h, a, s);
     default: return fail(OBME_ERR_INVALID, "bad tile configuration");
     }
 }

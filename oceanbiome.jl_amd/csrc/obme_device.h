@@ -652,12 +652,13 @@ __device__ __forceinline__ float guess_H_fp32(const CarbonParams& p)
 
 // Refinement of a close first guess by plain FP64 Newton on the same
 // residual; returns false when it has not converged after 6 steps.
+template <bool SI = false>
 __device__ __forceinline__ bool refine_H(const CarbonParams& p, double& x, int& evals)
 {
 #pragma unroll 1
     for (int it = 0; it < 6; it++) {
-        double f = alk_residual(x, p);
-        double df = alk_residual_derivative(x, p);
+        double f = alk_residual<SI>(x, p);
+        double df = alk_residual_derivative<SI>(x, p);
         evals++;
         double d = f / df;
         double xn = x - d;

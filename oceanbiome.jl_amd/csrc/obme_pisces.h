@@ -478,10 +478,20 @@ __device__ __forceinline__ double pressure_correction(const double* a, double Tk
     return exp((-dV + 0.5 * dk * P) * P / RT);
 }
 
+// Constants of the calcite saturation that depend only on T, S and the
+// pressure (fixed through a step of frozen physics): the unit conversion
+// 1e-3/ρ(T, S, P), the pressure-corrected equilibrium constants that enter
+// the residual (phosphate is 0, so KP1-3 do not), the salinity totals, the
+// calcium concentration and KSP.
+struct OmegaConst {
+    double conv, K1, K2, KB, KW, KS, KF, KSi, boron, sulfate, fluoride, ca, KSP;
+};
+constexpr int OMEGA_CONST_FIELDS = 13;
+
 // calcite_saturation (CarbonChemistry/calcite_concentration.jl:1-79) with
-// silicate and the pressure P in bar: constants pressure-corrected, density at P.
-__device__ __forceinline__ double calcite_saturation(double DIC, double Alk, double T, double S, double Si, double P,
-                                                     const obme_carbon_solver& s, int& evals)
+// silicate and the pressure P in bar: constants pressure-corrected, density
+// at P.  First half: the constants.
+__device__ __forceinline__ void omega_constants(double T, double S, double P, OmegaConst& c)
 {
     static constexpr double PC_K1[5] = {-25.50, 0.1271, 0.0, -0.00308, 0.0000877};
     static constexpr double PC_K2[5] = {-15.82, -0.0219, 0.0, 0.00113, -0.0001475};
@@ -489,42 +499,70 @@ __device__ __forceinline__ double calcite_saturation(double DIC, double Alk, dou
     static constexpr double PC_KW[5] = {-20.02, 0.1119, -0.001409, -0.00513, 0.0000794};
     static constexpr double PC_KS[5] = {-18.03, 0.0466, 0.000316, -0.00453, 0.00009};
     static constexpr double PC_KF[5] = {-9.78, -0.0090, -0.000942, -0.00391, 0.000054};
-    static constexpr double PC_KP1[5] = {-14.51, 0.1211, -0.000321, -0.00267, 0.0000427};
-    static constexpr double PC_KP2[5] = {-23.12, 0.1758, -0.002647, -0.00515, 0.00009};
-    static constexpr double PC_KP3[5] = {-26.57, 0.2020, -0.0030420, -0.00408, 0.0000714};
     static constexpr double PC_CAL[5] = {-48.76, 0.5304, -0.0, -0.01176, 0.0003692};
     CarbonParams p;
-    p.boron = 0.000232 / 10.811 * S / 1.80655;
-    p.sulfate = 0.14 / 96.06 * S / 1.80655;
-    p.fluoride = 0.000067 / 18.9984 * S / 1.80655;
+    c.boron = 0.000232 / 10.811 * S / 1.80655;
+    c.sulfate = 0.14 / 96.06 * S / 1.80655;
+    c.fluoride = 0.000067 / 18.9984 * S / 1.80655;
     const double rho = teos10_density(T, S, P);
-    p.Tk = T + 273.15;
-    p.Alk = Alk * (1e-3 / rho);
-    p.DIC = DIC * (1e-3 / rho);
+    const double Tk = T + 273.15;
+    c.conv = 1e-3 / rho;
+    carbon_constants(Tk, S, p);
+    c.K1 = p.K1 * pressure_correction(PC_K1, Tk, P);
+    c.K2 = p.K2 * pressure_correction(PC_K2, Tk, P);
+    c.KB = p.KB * pressure_correction(PC_KB, Tk, P);
+    c.KW = p.KW * pressure_correction(PC_KW, Tk, P);
+    c.KS = p.KS * pressure_correction(PC_KS, Tk, P);
+    c.KF = p.KF * pressure_correction(PC_KF, Tk, P);
+    c.KSi = p.KSi;
+    c.ca = 0.0103 * S / 35;
+    const double lnK_therm = -171.9065 + -0.077993 * Tk + 2839.319 / Tk + 71.595 * log10(Tk);
+    const double lnK_sea = ((-0.77712 + 0.0028426 * Tk + 178.34 / Tk) * sqrt(S) + -0.07711 * S + 0.0041249 * pow(S, 1.5));
+    c.KSP = pressure_correction(PC_CAL, Tk, P) * pow(10.0, lnK_therm + lnK_sea);
+}
+
+__device__ __forceinline__ CarbonParams omega_params(const OmegaConst& c, double DIC, double Alk, double Si)
+{
+    CarbonParams p;
+    p.boron = c.boron;
+    p.sulfate = c.sulfate;
+    p.fluoride = c.fluoride;
+    p.Alk = Alk * c.conv;
+    p.DIC = DIC * c.conv;
     p.phosphate = 0.0;
-    p.silicate = Si * (1e-3 / rho);
-    carbon_constants(p.Tk, S, p);
-    p.K1 *= pressure_correction(PC_K1, p.Tk, P);
-    p.K2 *= pressure_correction(PC_K2, p.Tk, P);
-    p.KB *= pressure_correction(PC_KB, p.Tk, P);
-    p.KW *= pressure_correction(PC_KW, p.Tk, P);
-    const double KS0 = p.KS;
-    p.KS *= pressure_correction(PC_KS, p.Tk, P);
-    (void)KS0;
-    p.KF *= pressure_correction(PC_KF, p.Tk, P);
-    p.KP1 *= pressure_correction(PC_KP1, p.Tk, P);
-    p.KP2 *= pressure_correction(PC_KP2, p.Tk, P);
-    p.KP3 *= pressure_correction(PC_KP3, p.Tk, P);
-    const double H = solve_for_H<true>(p, s, pow(10.0, -s.initial_pH_guess), evals);
+    p.silicate = Si * c.conv;
+    p.K1 = c.K1;
+    p.K2 = c.K2;
+    p.KB = c.KB;
+    p.KW = c.KW;
+    p.KS = c.KS;
+    p.KF = c.KF;
+    p.KP1 = p.KP2 = p.KP3 = 0.0;
+    p.KSi = c.KSi;
+    p.K0 = 0.0;
+    p.Tk = 0.0;
+    return p;
+}
+
+// Second half: Ω from [H+] (calcite_concentration.jl: CO3 from H, then
+// Ca·CO3/KSP).
+__device__ __forceinline__ double omega_from_H(const OmegaConst& c, const CarbonParams& p, double H)
+{
     const double denom1 = (H * (H + p.K1));
     const double denom2 = (1.0 + p.K1 * p.K2 / denom1);
     const double co3 = p.DIC * p.K1 * p.K2 / denom1 / denom2;
-    const double ca = 0.0103 * S / 35;
-    const double Tk = p.Tk;
-    const double lnK_therm = -171.9065 + -0.077993 * Tk + 2839.319 / Tk + 71.595 * log10(Tk);
-    const double lnK_sea = ((-0.77712 + 0.0028426 * Tk + 178.34 / Tk) * sqrt(S) + -0.07711 * S + 0.0041249 * pow(S, 1.5));
-    const double KSP = pressure_correction(PC_CAL, Tk, P) * pow(10.0, lnK_therm + lnK_sea);
-    return ca * co3 / KSP;
+    return c.ca * co3 / c.KSP;
+}
+
+// The reference solve: damped Newton from 10^-pH₀.
+__device__ __forceinline__ double calcite_saturation(double DIC, double Alk, double T, double S, double Si, double P,
+                                                     const obme_carbon_solver& s, int& evals)
+{
+    OmegaConst c;
+    omega_constants(T, S, P, c);
+    const CarbonParams p = omega_params(c, DIC, Alk, Si);
+    const double H = solve_for_H<true>(p, s, pow(10.0, -s.initial_pH_guess), evals);
+    return omega_from_H(c, p, H);
 }
 
 }  // namespace pisces

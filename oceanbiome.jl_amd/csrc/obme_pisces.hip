@@ -1,10 +1,15 @@
 // obme_pisces.hip — PISCES kernels and their C-ABI entry points
 // (obme_pisces_*, include/obme.h).
 //
-// One RK3 stage is three launches on the caller's stream:
+// obme_pisces_update_state / obme_pisces_add_tendencies launch
 //   k_pisces_columns  MultiBand PAR, euphotic depth, mixed-layer mean PAR;
 //   k_pisces_omega    calcite saturation per cell (FP64);
-//   k_pisces_cell     transitions + sinking + substep (or tendency output).
+//   k_pisces_cell     transitions + sinking (tendency output).
+// obme_pisces_step holds T and S fixed, so it computes the T-, S- and
+// pressure-dependent carbonate constants once per call (k_pisces_prep) and
+// each RK3 stage is two launches: k_pisces_columns, then k_pisces_cell with
+// the calcite saturation solved inline from those constants, and the
+// substep.
 // The stage input is read and the stage output written to different
 // arrays (the caller's tracers and a library-owned copy, alternately), so
 // the sinking stencil's k+1 reads never race with the update.
@@ -33,6 +38,9 @@ struct PiscesArgs {
     FT* Gm;                 // previous-stage transitions [NW][ncell] (compact)
     FT* PAR;                // [4][ncell] compact: total, band 1, 2, 3
     FT* Omega;              // [ncell] compact
+    const double* oc;       // [OMEGA_CONST_FIELDS][ncell]: per-cell carbonate constants
+    double* Hs;             // [ncell]: [H+] of the last stage (FP32-mixed warm start)
+    int have_H;             // Hs holds the previous stage's [H+]
     FT* zeu;                // [Nx*Ny]
     FT* mlPAR;              // [Nx*Ny]
     const FT* pp;           // parameters (OBME_PP_*)
@@ -134,10 +142,40 @@ __global__ void __launch_bounds__(256) k_pisces_omega(PiscesArgs<FT> a)
                                         (double)a.in[OBME_PI_SI][p], P, a.solver, ev);
 }
 
+// Thread per cell: the T-, S- and pressure-dependent constants of the
+// calcite saturation (the first half of calcite_saturation).
+template <typename FT>
+__global__ void __launch_bounds__(256) k_pisces_prep(PiscesArgs<FT> a)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ncol = (long)a.Nx * a.Ny;
+    const long ncell = ncol * a.Nz;
+    if (c >= ncell) return;
+    const int k = (int)(c / ncol);
+    const int col = (int)(c % ncol);
+    const int i = col % a.Nx, j = col / a.Nx;
+    const unsigned p = pidx(a, i, j, k);
+    const double z = (double)a.lev[k];
+    const double g = (double)a.pp[OBME_PP_SCALARS + OBME_PS_GRAVITY];
+    const double P = fabs(z) * g * 1026 / 100000;
+    OmegaConst o;
+    omega_constants((double)a.in[OBME_PI_T][p], (double)a.in[OBME_PI_S][p], P, o);
+    double* out = const_cast<double*>(a.oc);
+    const double f[OMEGA_CONST_FIELDS] = {o.conv, o.K1, o.K2, o.KB, o.KW, o.KS, o.KF, o.KSi,
+                                          o.boron, o.sulfate, o.fluoride, o.ca, o.KSP};
+#pragma unroll
+    for (int n = 0; n < OMEGA_CONST_FIELDS; n++) out[(long)n * ncell + c] = f[n];
+}
+
 // Thread per cell: transitions, minus the upwind sinking divergence of
 // POC, SFe (wPOC) and GOC, BFe, PSi, CaCO3 (wGOC), then the RK3 substep
-// (STEP) or the addition into the tendency arrays (TENDENCY).
-template <typename FT, int MODE>
+// (STEP) or the addition into the tendency arrays (TENDENCY).  OM = 1:
+// the calcite saturation is solved here from the k_pisces_prep constants
+// (FP64: the reference damped Newton from 10^-pH₀, as k_pisces_omega;
+// FP32-mixed: FP64 Newton warm-started from the previous stage's [H+], or
+// from an FP32 first guess, falling back to the reference solve);
+// OM = 0: it is read from the Omega array.
+template <typename FT, int MODE, int OM>
 __global__ void __launch_bounds__(256) k_pisces_cell(PiscesArgs<FT> a)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -161,7 +199,36 @@ __global__ void __launch_bounds__(256) k_pisces_cell(PiscesArgs<FT> a)
     A.zeu = a.zeu[col];
     A.kappa = sc[OBME_PS_KAPPA];
     A.mlPAR = a.mlPAR[col];
-    A.Omega = a.Omega[c];
+    if constexpr (OM == 1) {
+        OmegaConst o;
+        const double* oc = a.oc + c;
+        o.conv = oc[0];
+        o.K1 = oc[ncell];
+        o.K2 = oc[2 * ncell];
+        o.KB = oc[3 * ncell];
+        o.KW = oc[4 * ncell];
+        o.KS = oc[5 * ncell];
+        o.KF = oc[6 * ncell];
+        o.KSi = oc[7 * ncell];
+        o.boron = oc[8 * ncell];
+        o.sulfate = oc[9 * ncell];
+        o.fluoride = oc[10 * ncell];
+        o.ca = oc[11 * ncell];
+        o.KSP = oc[12 * ncell];
+        const CarbonParams cp = omega_params(o, (double)v[OBME_PI_DIC], (double)v[OBME_PI_ALK], (double)v[OBME_PI_SI]);
+        int ev = 0;
+        double H;
+        if constexpr (sizeof(FT) == 8) {
+            H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
+        } else {
+            H = a.have_H ? a.Hs[c] : (double)guess_H_fp32(cp);
+            if (!refine_H<true>(cp, H, ev)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
+            a.Hs[c] = H;
+        }
+        A.Omega = (FT)omega_from_H(o, cp, H);
+    } else {
+        A.Omega = a.Omega[c];
+    }
     A.PAR = a.PAR[c];
     A.PAR1 = a.PAR[ncell + c];
     A.PAR2 = a.PAR[2 * ncell + c];
@@ -221,6 +288,8 @@ struct obme_pisces {
     void* d_zeu = nullptr;
     void* d_mlPAR = nullptr;
     void* d_Gm = nullptr;
+    void* d_oc = nullptr;
+    void* d_H = nullptr;
     void* d_copy[NW] = {};
     size_t field_bytes = 0;
     std::vector<double> zc, zf, dz;
@@ -229,7 +298,7 @@ struct obme_pisces {
 extern "C" obme_status obme_pisces_destroy(obme_pisces* h)
 {
     if (!h) return OBME_OK;
-    for (void* p : {h->d_pp, h->d_lev, h->d_w, h->d_PAR, h->d_Omega, h->d_zeu, h->d_mlPAR, h->d_Gm})
+    for (void* p : {h->d_pp, h->d_lev, h->d_w, h->d_PAR, h->d_Omega, h->d_zeu, h->d_mlPAR, h->d_Gm, h->d_oc, h->d_H})
         if (p) (void)hipFree(p);
     for (void* p : h->d_copy)
         if (p) (void)hipFree(p);
@@ -328,6 +397,8 @@ static PiscesArgs<FT> pisces_args(const obme_pisces* h, double t)
     a.Omega = (FT*)h->d_Omega;
     a.zeu = (FT*)h->d_zeu;
     a.mlPAR = (FT*)h->d_mlPAR;
+    a.oc = (const double*)h->d_oc;
+    a.Hs = (double*)h->d_H;
     a.pp = (const FT*)h->d_pp;
     a.lev = (const FT*)h->d_lev;
     a.w = (const FT*)h->d_w;
@@ -369,9 +440,12 @@ static double host_day_length(double t, double phi)
     return (24 - 24.0 / 180 * (std::acos(L) * (180.0 / 3.141592653589793))) * 3600.0;
 }
 
+// One stage.  mode 0: the substep of obme_pisces_step (Ω inline from the
+// prepared constants; have_H: the [H+] of the previous stage is stored);
+// mode 1: tendencies added into out (Ω by its own kernel).
 template <typename FT>
 static obme_status pisces_stage(obme_pisces* h, const void* const* in, void* const* out, double t, double dt,
-                                int stage, int mode, hipStream_t s)
+                                int stage, int mode, int have_H, hipStream_t s)
 {
     PiscesArgs<FT> a = pisces_args<FT>(h, t);
     for (int n = 0; n < NTR; n++) a.in[n] = (const FT*)in[n];
@@ -384,23 +458,38 @@ static obme_status pisces_stage(obme_pisces* h, const void* const* in, void* con
     const double gam[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0}, zet[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
     a.A = (FT)(dt * gam[stage]);
     a.B = (FT)(dt * zet[stage]);
+    a.have_H = have_H;
     const int bs = 256;
     hipLaunchKernelGGL(k_pisces_columns<FT>, dim3((unsigned)((h->ncol + bs - 1) / bs)), dim3(bs), 0, s, a);
     OBME_HIP_TRY(hipGetLastError());
     const unsigned nb = (unsigned)((h->ncell + bs - 1) / bs);
-    hipLaunchKernelGGL(k_pisces_omega<FT>, dim3(nb), dim3(bs), 0, s, a);
+    if (mode == 0) {
+        hipLaunchKernelGGL((k_pisces_cell<FT, 0, 1>), dim3(nb), dim3(bs), 0, s, a);
+    } else {
+        hipLaunchKernelGGL(k_pisces_omega<FT>, dim3(nb), dim3(bs), 0, s, a);
+        OBME_HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL((k_pisces_cell<FT, 1, 0>), dim3(nb), dim3(bs), 0, s, a);
+    }
     OBME_HIP_TRY(hipGetLastError());
-    if (mode == 0) hipLaunchKernelGGL((k_pisces_cell<FT, 0>), dim3(nb), dim3(bs), 0, s, a);
-    else if (mode == 1) hipLaunchKernelGGL((k_pisces_cell<FT, 1>), dim3(nb), dim3(bs), 0, s, a);
+    return OBME_OK;
+}
+
+template <typename FT>
+static obme_status pisces_prep(obme_pisces* h, const void* const* in, hipStream_t s)
+{
+    PiscesArgs<FT> a = pisces_args<FT>(h, 0.0);
+    for (int n = 0; n < NTR; n++) a.in[n] = (const FT*)in[n];
+    const int bs = 256;
+    hipLaunchKernelGGL(k_pisces_prep<FT>, dim3((unsigned)((h->ncell + bs - 1) / bs)), dim3(bs), 0, s, a);
     OBME_HIP_TRY(hipGetLastError());
     return OBME_OK;
 }
 
 static obme_status pisces_stage_any(obme_pisces* h, const void* const* in, void* const* out, double t, double dt,
-                                    int stage, int mode, hipStream_t s)
+                                    int stage, int mode, int have_H, hipStream_t s)
 {
-    return (h->prec == OBME_FP64) ? pisces_stage<double>(h, in, out, t, dt, stage, mode, s)
-                                  : pisces_stage<float>(h, in, out, t, dt, stage, mode, s);
+    return (h->prec == OBME_FP64) ? pisces_stage<double>(h, in, out, t, dt, stage, mode, have_H, s)
+                                  : pisces_stage<float>(h, in, out, t, dt, stage, mode, have_H, s);
 }
 
 template <typename FT>
@@ -465,7 +554,7 @@ extern "C" obme_status obme_pisces_add_tendencies(obme_pisces* h, const void* co
         if (!tracers[n]) return obme_fail(OBME_ERR_INVALID, "null tracer pointer");
     for (int n = 0; n < NW; n++)
         if (!G[n]) return obme_fail(OBME_ERR_INVALID, "null tendency pointer");
-    return pisces_stage_any(h, tracers, G, t, 0.0, 0, 1, (hipStream_t)stream);
+    return pisces_stage_any(h, tracers, G, t, 0.0, 0, 1, 0, (hipStream_t)stream);
 }
 
 extern "C" obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, int32_t nsteps, double dt, double t0,
@@ -480,6 +569,14 @@ extern "C" obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, in
     for (int n = 0; n < NW; n++)
         if (!h->d_copy[n] && hipMalloc(&h->d_copy[n], h->field_bytes) != hipSuccess)
             return obme_fail(OBME_ERR_NOMEM, "PISCES stage copy");
+    if (!h->d_oc && hipMalloc(&h->d_oc, (size_t)OMEGA_CONST_FIELDS * h->ncell * sizeof(double)) != hipSuccess)
+        return obme_fail(OBME_ERR_NOMEM, "PISCES carbonate constants");
+    if (!h->d_H && hipMalloc(&h->d_H, (size_t)h->ncell * sizeof(double)) != hipSuccess)
+        return obme_fail(OBME_ERR_NOMEM, "PISCES [H+] store");
+    {
+        obme_status r = (h->prec == OBME_FP64) ? pisces_prep<double>(h, tracers, s) : pisces_prep<float>(h, tracers, s);
+        if (r != OBME_OK) return r;
+    }
     // Stage inputs and outputs alternate between the caller's arrays (U) and
     // library copies (C), which start as full copies of U so that their halos
     // match; the kernels write interiors only.  A step ends on the other side
@@ -505,7 +602,7 @@ extern "C" obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, in
         for (int q = 0; q < 3; q++) {
             const void* const* in = inCopy ? inC : inU;
             void* const* out = inCopy ? outU : outC;
-            obme_status r = pisces_stage_any(h, in, out, t + cst[q] * dt, dt, q, 0, s);
+            obme_status r = pisces_stage_any(h, in, out, t + cst[q] * dt, dt, q, 0, (st > 0 || q > 0) ? 1 : 0, s);
             if (r != OBME_OK) return r;
             inCopy = !inCopy;
         }

@@ -170,7 +170,7 @@ def test_co2_flux_points(mode):
     assert rel_l2(F, F_ref) <= 1e-10
 
 
-TILE_CFGS = ["2,2,8", "1,1,16", "2,1,8", "2,2,16", "1,1,8", "4,2,8"]
+TILE_CFGS = ["2,2,8", "1,1,16", "2,1,8", "2,2,16", "1,1,8", "4,2,8", "2,2,4", "1,1,4"]
 
 
 @pytest.mark.parametrize("tile", TILE_CFGS)
