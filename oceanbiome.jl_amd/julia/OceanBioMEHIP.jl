@@ -206,6 +206,85 @@ function step!(b::HIPBiogeochemistry, tracers, nsteps, Δt; t₀ = 0.0)
     return nothing
 end
 
-export HIPBiogeochemistry, step!
+# ---- PISCES ---------------------------------------------------------------------
+
+const OBME_PP_COUNT = 159
+
+struct ObmePiscesConfig
+    precision :: Int32
+    open_bottom :: Int32
+    grid :: ObmeGrid
+    params :: NTuple{OBME_PP_COUNT, Float64}      # OBME_PP_* layout of include/obme.h
+    band_kw :: NTuple{3, Float64}
+    band_e :: NTuple{3, Float64}
+    band_chi :: NTuple{3, Float64}
+    band_division :: NTuple{3, Float64}
+    surface_PAR :: Float64
+    solver :: ObmeCarbonSolver
+end
+
+"""
+    HIPPISCES(underlying, config::ObmePiscesConfig, aux)
+
+`PISCES(; grid)` (PISCES.jl:268-330) on the device.  `aux` holds the
+auxiliary fields the library writes in `update_biogeochemical_state!`:
+`PAR`, `PAR₁`, `PAR₂`, `PAR₃`, `Ω` (CenterFields), `zₑᵤ` and
+`mixed_layer_PAR` (2-D, Nx×Ny).
+"""
+mutable struct HIPPISCES{B, A} <: AbstractBiogeochemistry
+    underlying :: B
+    config :: Base.RefValue{ObmePiscesConfig}
+    handle :: Ptr{Cvoid}
+    aux :: A
+end
+
+function HIPPISCES(underlying, config::ObmePiscesConfig, aux)
+    n = ccall((:obme_abi_sizeof, libobme), Int64, (Int32,), 9)
+    n == sizeof(ObmePiscesConfig) || error("ABI mismatch for ObmePiscesConfig")
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    cfg = Ref(config)
+    check(ccall((:obme_pisces_create, libobme), Int32, (Ref{ObmePiscesConfig}, Ref{Ptr{Cvoid}}), cfg, h))
+    bgc = HIPPISCES(underlying, cfg, h[], aux)
+    finalizer(b -> ccall((:obme_pisces_destroy, libobme), Int32, (Ptr{Cvoid},), b.handle), bgc)
+    return bgc
+end
+
+required_biogeochemical_tracers(b::HIPPISCES) = required_biogeochemical_tracers(b.underlying)
+required_biogeochemical_auxiliary_fields(::HIPPISCES) = (:PAR, :PAR₁, :PAR₂, :PAR₃, :Ω, :zₑᵤ, :mixed_layer_PAR)
+biogeochemical_auxiliary_fields(b::HIPPISCES) = b.aux
+biogeochemical_drift_velocity(::HIPPISCES, ::Val) = nothing
+@inline (::HIPPISCES)(i, j, k, grid, ::Val, clock, fields) = zero(eltype(grid))
+
+function update_biogeochemical_state!(b::HIPPISCES, model)
+    ptrs = Ptr{Cvoid}[device_pointer(model.tracers[name]) for name in required_biogeochemical_tracers(b)]
+    bands = Ptr{Cvoid}[device_pointer(b.aux.PAR₁), device_pointer(b.aux.PAR₂), device_pointer(b.aux.PAR₃)]
+    check(ccall((:obme_pisces_update_state, libobme), Int32,
+                (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                 Float64, Ptr{Cvoid}),
+                b.handle, ptrs, device_pointer(b.aux.PAR), bands, device_pointer(b.aux.Ω),
+                Ptr{Cvoid}(UInt(pointer(b.aux.zₑᵤ))), Ptr{Cvoid}(UInt(pointer(b.aux.mixed_layer_PAR))),
+                model.clock.time, stream_pointer()))
+    return nothing
+end
+
+function update_tendencies!(b::HIPPISCES, model)
+    names = required_biogeochemical_tracers(b)
+    ptrs = Ptr{Cvoid}[device_pointer(model.tracers[name]) for name in names]
+    G = Ptr{Cvoid}[device_pointer(model.timestepper.Gⁿ[name]) for name in names[1:24]]
+    check(ccall((:obme_pisces_add_tendencies, libobme), Int32,
+                (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Float64, Ptr{Cvoid}),
+                b.handle, ptrs, G, model.clock.time, stream_pointer()))
+    return nothing
+end
+
+function step!(b::HIPPISCES, tracers, nsteps, Δt; t₀ = 0.0)
+    ptrs = Ptr{Cvoid}[device_pointer(tracers[name]) for name in required_biogeochemical_tracers(b)]
+    check(ccall((:obme_pisces_step, libobme), Int32,
+                (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Int32, Float64, Float64, Ptr{Cvoid}),
+                b.handle, ptrs, Int32(nsteps), Float64(Δt), Float64(t₀), stream_pointer()))
+    return nothing
+end
+
+export HIPBiogeochemistry, HIPPISCES, step!
 
 end # module
