@@ -52,12 +52,12 @@ extern "C" const char* obme_version(void) { return "obme 0.1.0 gfx950"; }
 #ifndef OBME_F32_WAVES_PER_EU
 #define OBME_F32_WAVES_PER_EU 4
 #endif
-template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL>
+template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL, bool PIPE>
 __global__ void __launch_bounds__(NW * 64, (sizeof(FT) == 4 && L <= 2) ? OBME_F32_WAVES_PER_EU : 1)
     k_column_wave(StepArgs<FT> a)
 {
     extern __shared__ unsigned char smem_raw[];
-    column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, UNIFORM, FULL>(a, reinterpret_cast<FT*>(smem_raw));
+    column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, UNIFORM, FULL, PIPE>(a, reinterpret_cast<FT*>(smem_raw));
 }
 
 // CO2 flux and pH on n points (SoA, FP64).
@@ -101,6 +101,7 @@ struct obme_handle {
     int sink_idx[MAX_SINK];
     int cfg_id;                // sweep tile configuration (TileCfg table)
     bool uniform;
+    bool pipe;                 // pipelined persistent sweep where it applies (OBME_PIPE=1 turns it on)
     double dz_u, zc0_u, w0_u, w1_u;
     // kernel timing: event pairs per kernel kind, and a pool of free events
     bool timing = false;
@@ -290,6 +291,10 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
     // sweep tile configuration: the first of the precision's preference list
     // whose column fits (Nz <= L * 64 / CPW); OBME_TILE="L,CPW,NW" overrides.
     h->cfg_id = choose_tile(h->prec, g.Nz);
+    // the pipelined sweep measured slower than one tile per workgroup on
+    // the 512x512x64 LOBSTER step (0.56 vs 0.44 ms): off unless OBME_PIPE=1
+    h->pipe = false;
+    if (const char* e = std::getenv("OBME_PIPE")) h->pipe = std::atoi(e) != 0;
     if (const char* e = std::getenv("OBME_TILE")) {
         int l = 0, c = 0, w = 0;
         if (std::sscanf(e, "%d,%d,%d", &l, &c, &w) == 3) {
@@ -452,17 +457,40 @@ static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_
     constexpr int NT = ModelTraits<MODEL, CARB>::NT;
     constexpr int TXB = NW * CPW;
     const int ntiles = (MODE == TILE_PAR) ? 1 : (MODE == TILE_TENDENCY ? NT + 1 : NT);
-    const size_t lds = (size_t)ntiles * h->cfg.grid.Nz * (TXB + 1) * sizeof(FT);
+    size_t lds = (size_t)ntiles * h->cfg.grid.Nz * (TXB + 1) * sizeof(FT);
     if (lds > 160 * 1024) return fail(OBME_ERR_UNSUPPORTED, "sweep tile exceeds the 160 KiB LDS");
     dim3 block(NW * 64);
     // one-dimensional: tile = xcd_tile(blockIdx.x), row-major over (i tile, j)
-    dim3 grid((unsigned)(((h->cfg.grid.Nx + TXB - 1) / TXB) * h->cfg.grid.Ny));
+    const unsigned n_tiles = (unsigned)(((h->cfg.grid.Nx + TXB - 1) / TXB) * h->cfg.grid.Ny);
+    dim3 grid(n_tiles);
     // FULL (every lane level is a cell) is compiled for the fused step only
     const bool full = (MODE == TILE_STEP) && h->cfg.grid.Nz == L * 64 / CPW;
-    auto k = h->uniform ? (full ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, MODE == TILE_STEP>
-                                : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, false>)
-                        : (full ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, MODE == TILE_STEP>
-                                : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, false>);
+    auto k = h->uniform ? (full ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, MODE == TILE_STEP, false>
+                                : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, false, false>)
+                        : (full ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, MODE == TILE_STEP, false>
+                                : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, false, false>);
+    // the pipelined persistent sweep (FP32 fused step, FULL columns)
+    constexpr int PB = (64 % TXB == 0) ? 64 / TXB : 1;
+    constexpr int NBLK = (L * 64 / CPW) / PB;
+    constexpr bool PIPE_OK = sizeof(FT) == 4 && MODE == TILE_STEP && 64 % TXB == 0 && NBLK % NW == 0 &&
+                             2 * (NT * NBLK / NW) < 64;
+    if constexpr (PIPE_OK) {
+        if (full && h->pipe && a.surface_mode != OBME_SURFACE_ARRAY) {
+            k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, true, true>
+                           : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, true, true>;
+            lds = 2 * (size_t)NT * NBLK * (64 + PIPE_SKEW) * sizeof(FT);
+            int per_cu = 0, ncu = 0, dev = 0;
+            HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            HIP_TRY(hipGetDevice(&dev));
+            HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, NW * 64, lds));
+            // persistent: one workgroup per slot, a multiple of the 8 XCDs
+            unsigned slots = (unsigned)std::max(1, per_cu) * (unsigned)std::max(1, ncu);
+            unsigned nb = std::min(slots, n_tiles);
+            nb = std::max(8u, nb / 8 * 8);
+            grid = dim3(nb);
+        }
+    }
     HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     LaunchTimer tm(h, OBME_KERNEL_SWEEP, s);
     hipLaunchKernelGGL(k, grid, block, lds, s, a);

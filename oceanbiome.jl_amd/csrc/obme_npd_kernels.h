@@ -165,7 +165,40 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)
     return x * q + (x < rem ? x : rem) + s;
 }
 
-template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL>
+// s_waitcnt with only the vector-memory count constrained (gfx9 encoding:
+// vmcnt bits [3:0] and [15:14], expcnt [6:4] = 7, lgkmcnt [11:8] = 15).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+// s_waitcnt lgkmcnt(0) (LDS and scalar memory), vector memory unconstrained.
+__device__ __forceinline__ void wait_lgkm0()
+{
+    __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4));
+}
+
+// Pipelined sweep (PIPE): a persistent workgroup walks a run of tiles and
+// loads tile n+1 straight into LDS (buffer_load ... lds, no registers)
+// while it computes tile n.  LDS layout of one buffer, per tracer: blocks of
+// 64/TXB levels x TXB columns, 64 elements each (one load instruction of one
+// wave: lane l holds level l / TXB, column l % TXB of the block, so the
+// global reads are coalesced along i), block kb at element kb * (64 + 2): the
+// skew of 2 per block puts the lanes of a column gather (levels two apart,
+// two columns) in distinct banks.
+constexpr int PIPE_SKEW = 2;
+
+// One wave-wide load of 4-byte elements straight into LDS: lane l's element
+// (byte offset voff in the buffer r; past its end reads 0) lands at
+// dst + 4 l.  dst must be the same for every lane of the wave.
+__device__ __forceinline__ void lds_load_b32(rsrc_t r, void* dst, unsigned voff)
+{
+    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)dst, 4, voff, 0, 0, 0);
+}
+
+template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL,
+          bool PIPE = false>
 __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
 {
     using TR = ModelTraits<MODEL, CARB>;
@@ -179,12 +212,74 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
     const int w = threadIdx.x >> 6;
     const int r = lane % LPC;              // rank in the column, 0 = top
     const int c = w * CPW + lane / LPC;    // column in the workgroup
-    const unsigned wg_tile = xcd_tile(blockIdx.x, gridDim.x);
     const unsigned nbx = ((unsigned)a.Nx + TXB - 1) / TXB;
+    const int Nz = FULL ? L * LPC : a.Nz;
+    const int ktop = Nz - 1;
+
+    // ---- PIPE geometry and the workgroup's run of tiles
+    constexpr int PB = (64 % TXB == 0) ? 64 / TXB : 1;    // levels per LDS block
+    constexpr int NBLK = (L * LPC) / PB;                   // blocks per tracer (FULL)
+    constexpr int BLK = 64 + PIPE_SKEW;
+    constexpr int TRS = NBLK * BLK;                        // elements per tracer
+    constexpr int LPW = (ModelTraits<MODEL, CARB>::NT * NBLK) / NW;   // block loads per wave per tile
+    static_assert(!PIPE || (FULL && MODE == TILE_STEP && sizeof(FT) == 4 && 64 % TXB == 0 && NBLK % NW == 0 &&
+                            LPW * 2 < 64),
+                  "pipelined sweep: FP32 fused step, FULL columns, TXB | 64, NW | blocks");
+    unsigned t_first, t_step, t_end;
+    if constexpr (PIPE) {
+        // XCD x (workgroups b = x mod 8) takes a contiguous chunk of tiles;
+        // its workgroups walk it with stride = their number, so the tiles in
+        // flight on one XCD at a time are neighbours.
+        const unsigned ntiles = nbx * (unsigned)a.Ny;
+        const unsigned b = blockIdx.x, nb = gridDim.x, X = 8;
+        const unsigned x = b % X, s = b / X;
+        const unsigned nwx = nb / X + (x < nb % X ? 1u : 0u);
+        const unsigned tq = ntiles / X, trem = ntiles % X;
+        const unsigned start = x * tq + (x < trem ? x : trem);
+        t_first = start + s;
+        t_step = nwx;
+        t_end = start + tq + (x < trem ? 1u : 0u);
+    } else {
+        t_first = xcd_tile(blockIdx.x, gridDim.x);
+        t_step = 1;
+        t_end = t_first + 1;
+    }
+    auto pipe_buf = [&](int n) -> FT* { return lds + (n & 1) * (ModelTraits<MODEL, CARB>::NT * TRS); };
+    // byte offset of (ii, jj, k) in a parent array
+    auto goff2 = [&](int ii, int jj, int k) -> unsigned {
+        return (a.base0 + (unsigned)ii + a.sx * (unsigned)jj + a.sxy * (unsigned)k) * (unsigned)sizeof(FT);
+    };
+    // issue the LDS loads of tile t into buffer B (LPW instructions per wave)
+    auto pipe_issue = [&](unsigned t, FT* B) {
+        const int jj = (int)(t / nbx);
+        const int ib = (int)(t % nbx) * TXB;
+        const int kl = lane / TXB, cc = lane % TXB;
+        const int ii = (ib + cc < a.Nx) ? ib + cc : a.Nx - 1;
+#pragma unroll
+        for (int n = 0; n < ModelTraits<MODEL, CARB>::NT; n++) {
+            const rsrc_t rn = make_rsrc(a.tr[n], a.field_bytes);
+#pragma unroll
+            for (int q = 0; q < NBLK / NW; q++) {
+                const int kb = w + NW * q;
+                lds_load_b32(rn, B + n * TRS + kb * BLK, goff2(ii, jj, kb * PB + kl));
+            }
+        }
+    };
+    bool prev_tile = false;                // a tile was stored before this one
+    bool next_issued = false;              // the tile after this one has been issued
+    if constexpr (PIPE) {
+        if (t_first < t_end) pipe_issue(t_first, pipe_buf(0));
+        if (t_first + t_step < t_end) {
+            pipe_issue(t_first + t_step, pipe_buf(1));
+            next_issued = true;
+        }
+    }
+    const FT surface_value0 = a.surface_value;
+    int it = 0;
+#pragma unroll 1
+    for (unsigned wg_tile = t_first; wg_tile < t_end; wg_tile += t_step, it++) {
     const int j = (int)(wg_tile / nbx);
     const int ibase = (int)(wg_tile % nbx) * TXB;
-    const int Nz = a.Nz;
-    const int ktop = Nz - 1;
     const int i_raw = ibase + c;
     const bool col_ok = i_raw < a.Nx;
     const int i = col_ok ? i_raw : a.Nx - 1;
@@ -218,7 +313,19 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
         const int e = threadIdx.x + q * BS;
         return (e < nel) ? (e / TXB) * ROW + e % TXB : -1;
     };
-    {
+    // PIPE: element (tracer n, level k, column cc) of the current buffer
+    FT* const pbuf = pipe_buf(it);
+    auto pidx = [&](int n, int k, int cc) -> int { return n * TRS + (k / PB) * BLK + (k % PB) * TXB + cc; };
+    if constexpr (PIPE) {
+        // this tile's loads are complete once at most the later ones are
+        // outstanding: the previous tile's LPW stores and, if issued, the
+        // next tile's LPW loads (each wave issues exactly these counts)
+        const int later = (prev_tile ? 1 : 0) + (next_issued ? 1 : 0);
+        if (later == 2) wait_vmcnt<2 * LPW>();
+        else if (later == 1) wait_vmcnt<LPW>();
+        else wait_vmcnt<0>();
+        __builtin_amdgcn_s_barrier();
+    } else {
         FT stg[NLOAD][L];
 #pragma unroll
         for (int t = 0; t < NLOAD; t++) {
@@ -232,8 +339,8 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
 #pragma unroll
             for (int q = 0; q < L; q++)
                 if (e_idx(q) >= 0) tile(t)[e_idx(q)] = stg[t][q];
+        __syncthreads();
     }
-    __syncthreads();
 
     // FP64: U is the state, updated in place (the reference's expressions).
     // FP32: U is the state at the start of the step and D the step's
@@ -251,7 +358,8 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
             const int k = kof(m);
             const bool ok = has(m);
             int t = (MODE == TILE_PAR) ? (n == TR::P_IDX ? 0 : -1) : n;
-            U[n][m] = (ok && t >= 0) ? tile(t)[k * ROW + c] : FT(0);
+            if constexpr (PIPE) U[n][m] = pbuf[pidx(n, k, c)];
+            else U[n][m] = (ok && t >= 0) ? tile(t)[k * ROW + c] : FT(0);
             Gm[n][m] = FT(0);
             if constexpr (F32) D[n][m] = FT(0);
         }
@@ -288,8 +396,13 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
         else return a.wsink[s * (Nz + 1) + k + 1];
     };
 
-    FT PAR0_const = a.surface_value;
-    if (a.surface_mode == OBME_SURFACE_ARRAY) PAR0_const = a.surf[i + a.Nx * j];
+    // (the constant is read before the tile loop.  The pipelined sweep is
+    // not used with a surface PAR array: the wait for a vector load here
+    // would also wait for the prefetch of the next tile.)
+    FT PAR0_const = surface_value0;
+    if constexpr (!PIPE) {
+        if (a.surface_mode == OBME_SURFACE_ARRAY) PAR0_const = a.surf[i + a.Nx * j];
+    }
 
     const unsigned col2d = (unsigned)i + (unsigned)a.Nx * (unsigned)j;
     const unsigned ncol2d = (unsigned)a.Nx * (unsigned)a.Ny;
@@ -468,6 +581,53 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
         }
     }
 
+    if constexpr (PIPE) {
+        // ---- stage out (PIPE): each lane scatters to the elements it
+        // gathered, then every wave reads its blocks in load order
+        // (coalesced), the buffer is refilled with the tile after next, and
+        // the stores are issued; an element that must not be written (past
+        // Nx, or the top-cell DIC left to the surface kernel) gets an offset
+        // past the buffer, which the buffer store drops, so every wave
+        // issues exactly LPW stores.
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+#pragma unroll
+            for (int m = 0; m < L; m++) pbuf[pidx(n, kof(m), c)] = val(n, m);
+        wait_lgkm0();
+        __builtin_amdgcn_s_barrier();
+        FT v[NT][NBLK / NW];
+        unsigned off[NT][NBLK / NW];
+        const int kl = lane / TXB, cc = lane % TXB;
+#pragma unroll
+        for (int n = 0; n < NT; n++)
+#pragma unroll
+            for (int q = 0; q < NBLK / NW; q++) {
+                const int kb = w + NW * q;
+                const int k = kb * PB + kl;
+                v[n][q] = pbuf[n * TRS + kb * BLK + lane];
+                bool ok = ibase + cc < a.Nx && n != RO;
+                if constexpr (CARB) {
+                    if (n == 7 && a.co2_on && k == ktop) ok = false;
+                }
+                off[n][q] = ok ? goff2(ibase + cc, j, k) : a.field_bytes;
+            }
+        wait_lgkm0();
+        __builtin_amdgcn_s_barrier();
+        const unsigned t_next = wg_tile + 2 * t_step;
+        if (t_next < t_end) pipe_issue(t_next, pbuf);
+        // seen from the next tile: a tile was stored before it, and the tile
+        // after it has been issued iff it exists
+        prev_tile = true;
+        next_issued = t_next < t_end;
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            const rsrc_t rd = rs(a.tr[n]);
+#pragma unroll
+            for (int q = 0; q < NBLK / NW; q++) buf_store(rd, off[n][q], v[n][q]);
+        }
+        continue;
+    }
+
     // ---- stage out: scatter to the tiles, coalesced stores along i.
     // Outputs: STEP the written tracers; PAR the PAR field; TENDENCY the
     // written tendencies (added to G) and PAR (tile NT) when requested.
@@ -521,6 +681,7 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
             if (ok) buf_store(rd, e_off(q), v[q]);
         }
     }
+    }   // tile loop
 }
 
 // ------------------------------------------------------------------ surface
