@@ -1,14 +1,22 @@
 #!/bin/bash
-# Builds libobme_hip.so for gfx950 in-tree (next to this script's package).
+# Builds libobme_hip.so for gfx950 in-tree (next to this script's package):
+# each translation unit is compiled to an object in parallel, then linked.
 # Packed FP32 instructions are disabled (-packed-fp32-ops).
 set -euo pipefail
 HERE="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
 OUT="${OBME_OUT:-${HERE}/../oceanbiome/libobme_hip.so}"
 HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
-"${HIPCC}" --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared \
-    -Xclang -target-feature -Xclang -packed-fp32-ops \
-    -Wall -Wno-unused-function -Wno-unused-variable \
-    ${OBME_EXTRA_FLAGS:-} \
-    -o "${OUT}.tmp" "${HERE}/obme_api.hip" "${HERE}/obme_pisces.hip"
+FLAGS=(--offload-arch=gfx950 -O3 -std=c++17 -fPIC
+       -Xclang -target-feature -Xclang -packed-fp32-ops
+       -Wall -Wno-unused-function -Wno-unused-variable)
+TMP="$(mktemp -d)"
+trap 'rm -rf "${TMP}"' EXIT
+pids=()
+for src in obme_api obme_pisces; do
+    "${HIPCC}" "${FLAGS[@]}" ${OBME_EXTRA_FLAGS:-} -c -o "${TMP}/${src}.o" "${HERE}/${src}.hip" &
+    pids+=($!)
+done
+for p in "${pids[@]}"; do wait "$p"; done
+"${HIPCC}" --offload-arch=gfx950 -fPIC -shared -o "${OUT}.tmp" "${TMP}/obme_api.o" "${TMP}/obme_pisces.o"
 mv "${OUT}.tmp" "${OUT}"
 echo "built ${OUT}"

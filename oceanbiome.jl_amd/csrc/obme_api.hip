@@ -52,12 +52,23 @@ extern "C" const char* obme_version(void) { return "obme 0.1.0 gfx950"; }
 #ifndef OBME_F32_WAVES_PER_EU
 #define OBME_F32_WAVES_PER_EU 4
 #endif
-template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL, bool PIPE>
+template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL, bool PIPE,
+          int FM = -1>
 __global__ void __launch_bounds__(NW * 64, (sizeof(FT) == 4 && L <= 2) ? OBME_F32_WAVES_PER_EU : 1)
     k_column_wave(StepArgs<FT> a)
 {
     extern __shared__ unsigned char smem_raw[];
-    column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, UNIFORM, FULL, PIPE>(a, reinterpret_cast<FT*>(smem_raw));
+    column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, UNIFORM, FULL, PIPE, FM>(a, reinterpret_cast<FT*>(smem_raw));
+}
+
+// The model's default formulations (fm_code): LOBSTER Mondo light,
+// quadratic mortality and grazing; NPZD analytical light, linear
+// mortality, quadratic grazing, Q10 on.  The FP32 fused step on full
+// columns is compiled for these with the choices folded in.
+template <int MODEL>
+constexpr int default_fm()
+{
+    return MODEL == OBME_LOBSTER ? (0 | (1 << 1) | (1 << 2)) : (1 | (0 << 1) | (1 << 2) | (1 << 3));
 }
 
 // CO2 flux and pH on n points (SoA, FP64).
@@ -469,6 +480,11 @@ static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_
                                 : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, false, false>)
                         : (full ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, MODE == TILE_STEP, false>
                                 : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, false, false>);
+    if constexpr (sizeof(FT) == 4 && MODE == TILE_STEP) {
+        if (full && fm_code(a.f) == default_fm<MODEL>())
+            k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, true, false, default_fm<MODEL>()>
+                           : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, true, false, default_fm<MODEL>()>;
+    }
     // the pipelined persistent sweep (FP32 fused step, FULL columns)
     constexpr int PB = (64 % TXB == 0) ? 64 / TXB : 1;
     constexpr int NBLK = (L * 64 / CPW) / PB;
@@ -478,6 +494,9 @@ static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_
         if (full && h->pipe && a.surface_mode != OBME_SURFACE_ARRAY) {
             k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, true, true>
                            : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, true, true>;
+            if (fm_code(a.f) == default_fm<MODEL>())
+                k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, true, true, default_fm<MODEL>()>
+                               : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, true, true, default_fm<MODEL>()>;
             lds = 2 * (size_t)NT * NBLK * (64 + PIPE_SKEW) * sizeof(FT);
             int per_cu = 0, ncu = 0, dev = 0;
             HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -489,6 +508,12 @@ static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_
             unsigned nb = std::min(slots, n_tiles);
             nb = std::max(8u, nb / 8 * 8);
             grid = dim3(nb);
+            static bool told = false;
+            if (!told && std::getenv("OBME_VERBOSE")) {
+                std::fprintf(stderr, "obme: pipelined sweep: %d workgroups per CU x %d CUs, grid %u for %u tiles, LDS %zu B\n",
+                             per_cu, ncu, nb, n_tiles, lds);
+                told = true;
+            }
         }
     }
     HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));

@@ -356,6 +356,20 @@ constexpr float F_TINY = 1.17549435e-38f;  // smallest normal float
 struct FGrazing {
     float total, onP, onSmall;
 };
+// Formulation choices of the FP32 path: FM < 0 reads them from FastParams
+// at run time; otherwise they are compiled in, FM = light (bit 0: 0 Mondo,
+// 1 analytical) | phytoplankton mortality << 1 (0 linear, 1 quadratic) |
+// grazing concentration << 2 (0 linear, 1 quadratic) | Q10 in use << 3.
+template <int FM> __device__ __forceinline__ int fm_light(const FastParams& f) { if constexpr (FM < 0) return f.light_form; else return FM & 1; }
+template <int FM> __device__ __forceinline__ int fm_mort(const FastParams& f) { if constexpr (FM < 0) return f.mort_form; else return (FM >> 1) & 1; }
+template <int FM> __device__ __forceinline__ int fm_graz(const FastParams& f) { if constexpr (FM < 0) return f.graz_form; else return (FM >> 2) & 1; }
+template <int FM> __device__ __forceinline__ int fm_q10(const FastParams& f) { if constexpr (FM < 0) return f.use_Q10; else return (FM >> 3) & 1; }
+inline int fm_code(const FastParams& f)
+{
+    return (f.light_form & 1) | ((f.mort_form & 1) << 1) | ((f.graz_form & 1) << 2) | ((f.use_Q10 ? 1 : 0) << 3);
+}
+
+template <int FM = -1>
 __device__ __forceinline__ FGrazing fast_grazing(const FastParams& f, float P, float Z, float sPOM)
 {
     const float pP = f.pref * P;
@@ -363,7 +377,7 @@ __device__ __forceinline__ FGrazing fast_grazing(const FastParams& f, float P, f
     const float n = fmaf(pP, P, cS * sPOM);
     const float d = pP + cS;
     float L;
-    if (f.graz_form == OBME_LINEAR) L = n * frcp(fmaf(f.kG, d, n) + F_TINY);
+    if (fm_graz<FM>(f) == OBME_LINEAR) L = n * frcp(fmaf(f.kG, d, n) + F_TINY);
     else {
         const float n2 = n * n;
         const float kd = f.kG * d;
@@ -377,9 +391,10 @@ __device__ __forceinline__ FGrazing fast_grazing(const FastParams& f, float P, f
     return G;
 }
 
+template <int FM = -1>
 __device__ __forceinline__ float fast_light(const FastParams& f, float PAR)
 {
-    if (f.light_form == OBME_LIGHT_MONDO) return PAR * frcp(f.kPAR + PAR);
+    if (fm_light<FM>(f) == OBME_LIGHT_MONDO) return PAR * frcp(f.kPAR + PAR);
     return PAR * __builtin_amdgcn_rsqf(fmaf(PAR, PAR, f.kPAR2));
 }
 
@@ -387,7 +402,7 @@ __device__ __forceinline__ float fast_light(const FastParams& f, float PAR)
 // The nitrate and ammonia uptakes are mu0/2 L_PAR P times their own
 // limitation (the reference forms them as fractions of the total
 // production, plankton.jl:223-261, which is the same product).
-template <bool CARB>
+template <bool CARB, int FM = -1>
 __device__ __forceinline__ void lobster_fast(const FastParams& f, const float* v, float PAR, float* out)
 {
     const float NO3 = v[0], NH4 = v[1], P = v[2], Z = v[3], sPOM = v[4], bPOM = v[5], DOM = v[6];
@@ -396,12 +411,12 @@ __device__ __forceinline__ void lobster_fast(const FastParams& f, const float* v
     const float rab = frcp(a * b);
     const float nl = NO3 * __builtin_amdgcn_exp2f(f.nPSI2 * NH4) * (b * rab);
     const float al = fmax0(NH4 * (a * rab));
-    const float muLP = f.mu0h * fast_light(f, PAR) * P;
+    const float muLP = f.mu0h * fast_light<FM>(f, PAR) * P;
     const float up_NO3 = muLP * nl;
     const float muA = muLP * al;
     const float muP = up_NO3 + muA;
-    const FGrazing G = fast_grazing(f, P, Z, sPOM);
-    const float nuP = (f.mort_form == OBME_LINEAR) ? f.mP * P : f.mP * P * P;
+    const FGrazing G = fast_grazing<FM>(f, P, Z, sPOM);
+    const float nuP = (fm_mort<FM>(f) == OBME_LINEAR) ? f.mP * P : f.mP * P * P;
     const float nit = f.nitrif * NH4;
     const float up_NH4 = fmaf(-f.axg, muP, muA);
     const float pinw = fmaf(f.pinw_z, Z, f.one_m_aPs * nuP);
@@ -428,13 +443,14 @@ __device__ __forceinline__ void lobster_fast(const FastParams& f, const float* v
 }
 
 // NPZD transitions in FP32 (N, P, Z, D); Lt = Q10^(T/10).
+template <int FM = -1>
 __device__ __forceinline__ void npzd_fast(const FastParams& f, float N, float P, float Z, float D, float PAR,
                                           float Lt, float& dN, float& dP, float& dZ, float& dD)
 {
-    const float muP = f.mu0 * fast_light(f, PAR) * (N * frcp(N + f.kNO3)) * Lt * P;
+    const float muP = f.mu0 * fast_light<FM>(f, PAR) * (N * frcp(N + f.kNO3)) * Lt * P;
     const float sPOM = D * f.small_frac;
-    const FGrazing G = fast_grazing(f, P, Z, sPOM);
-    const float nuP = (f.mort_form == OBME_LINEAR) ? f.mP * P : f.mP * P * P;
+    const FGrazing G = fast_grazing<FM>(f, P, Z, sPOM);
+    const float nuP = (fm_mort<FM>(f) == OBME_LINEAR) ? f.mP * P : f.mP * P * P;
     const float Z2 = Z * Z;
     dP = fmaf(f.one_m_gamma, muP, -G.onP) - nuP;
     dZ = fmaf(f.aZass, G.total, -f.mZ * Z2) - f.muZ * Z;
