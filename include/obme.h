@@ -173,6 +173,26 @@ typedef struct obme_field_source {
     const double* profile;         /* host [Nz], OBME_FIELD_PROFILE (copied)  */
 } obme_field_source;
 
+/* ScaleNegativeTracers (src/Utils/negative_tracers.jl:36-59, :137-276): per
+ * cell and conserved group, t = Σ s·c over the members, p = Σ s·c over the
+ * positive members, t := invalid_fill_value where t < 0, then every member
+ * c := c·t/p where c > 0 or c is not finite, else 0.  Groups apply in order
+ * (a tuple of modifiers); members are tracer indices in tracer order, in
+ * the order of the reference's conserved_tracers (the order of the sums).
+ * It runs first in update_biogeochemical_state! (src/OceanBioME.jl:161-167),
+ * so the fused steppers apply it at the start of every RK3 stage, before the
+ * light and the tendencies.  n_groups = 0: off. */
+#define OBME_MAX_SCALE_GROUPS 5
+#define OBME_MAX_SCALE_MEMBERS 12
+typedef struct obme_negative_scaling {
+    int32_t n_groups;
+    int32_t n_members[OBME_MAX_SCALE_GROUPS];
+    int32_t member[OBME_MAX_SCALE_GROUPS][OBME_MAX_SCALE_MEMBERS];
+    int32_t pad0;
+    double scalefactor[OBME_MAX_SCALE_GROUPS][OBME_MAX_SCALE_MEMBERS];
+    double invalid_fill_value[OBME_MAX_SCALE_GROUPS];
+} obme_negative_scaling;
+
 typedef struct obme_config {
     int32_t model;                 /* OBME_NPZD | OBME_LOBSTER                */
     int32_t precision;             /* OBME_FP64 | OBME_FP32_MIXED             */
@@ -186,6 +206,7 @@ typedef struct obme_config {
     obme_co2_exchange co2;
     obme_field_source temperature; /* LOBSTER gas exchange; NPZD reads T as a tracer */
     obme_field_source salinity;
+    obme_negative_scaling scaling; /* NPZD(grid; scale_negatives = true) etc. */
 } obme_config;
 
 typedef struct obme_handle obme_handle;
@@ -208,9 +229,11 @@ int32_t obme_required_tracers(const obme_handle* h, char* buf, int32_t cap);
 /* ---- the Oceananigans.Biogeochemistry hooks ----------------------------- */
 
 /* update_biogeochemical_state!(bgc, model) (src/OceanBioME.jl:161-167):
- * the light step, i.e. the TwoBand PAR column scan (Light/2band.jl:1-33,
+ * the modifiers first — with negative scaling configured, the tracers are
+ * rescaled in place (src/Utils/negative_tracers.jl:137-251) — then the
+ * light step, i.e. the TwoBand PAR column scan (Light/2band.jl:1-33,
  * :148-155). Reads tracers[P], writes the interior of PAR at time t. */
-obme_status obme_update_state(obme_handle* h, const void* const* tracers,
+obme_status obme_update_state(obme_handle* h, void* const* tracers,
                               void* PAR, double t, void* stream);
 
 /* update_tendencies!(bgc, model) (src/OceanBioME.jl:148-155), in the form
@@ -470,6 +493,8 @@ typedef struct obme_pisces_config {
     double band_kw[3], band_e[3], band_chi[3], band_division[3];
     double surface_PAR;            /* W/m2 */
     obme_carbon_solver solver;
+    obme_negative_scaling scaling; /* PISCES(; scale_negatives = true): the five groups of
+                                      PISCES/coupling_utils.jl:11-35 */
 } obme_pisces_config;
 
 typedef struct obme_pisces obme_pisces;
@@ -513,7 +538,8 @@ obme_status obme_kernel_timing(obme_handle* h, int32_t which, double* total_ms, 
 /* Size in bytes of ABI struct `which` as compiled into the library
  * (0 obme_config, 1 obme_grid, 2 obme_npd_params, 3 obme_two_band_par,
  * 4 obme_sinking, 5 obme_carbon_solver, 6 obme_co2_exchange,
- * 7 obme_field_source, 8 obme_batch_params, 9 obme_pisces_config); -1 for an
+ * 7 obme_field_source, 8 obme_batch_params, 9 obme_pisces_config,
+ * 10 obme_negative_scaling); -1 for an
  * unknown index.  No reference
  * counterpart: it lets a binding check its struct layouts. */
 int64_t obme_abi_sizeof(int32_t which);

@@ -53,12 +53,36 @@ extern "C" const char* obme_version(void) { return "obme 0.1.0 gfx950"; }
 #define OBME_F32_WAVES_PER_EU 4
 #endif
 template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL, bool PIPE,
-          int FM = -1>
+          int FM = -1, int EXTRA = 0>
 __global__ void __launch_bounds__(NW * 64, (sizeof(FT) == 4 && L <= 2) ? OBME_F32_WAVES_PER_EU : 1)
     k_column_wave(StepArgs<FT> a)
 {
     extern __shared__ unsigned char smem_raw[];
-    column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, UNIFORM, FULL, PIPE, FM>(a, reinterpret_cast<FT*>(smem_raw));
+    column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, UNIFORM, FULL, PIPE, FM, EXTRA>(a,
+                                                                                   reinterpret_cast<FT*>(smem_raw));
+}
+
+// ScaleNegativeTracers on every interior cell, in place (the modifier that
+// runs first in update_biogeochemical_state!, src/OceanBioME.jl:161-167;
+// src/Utils/negative_tracers.jl:198-251): one thread per cell.
+template <typename FT, int NT>
+__global__ void __launch_bounds__(256) k_scale_grid(StepArgs<FT> a)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ncol = (long)a.Nx * a.Ny;
+    if (c >= ncol * a.Nz) return;
+    const int k = (int)(c / ncol);
+    const int col = (int)(c % ncol);
+    const unsigned p = a.base0 + (unsigned)(col % a.Nx) + a.sx * (unsigned)(col / a.Nx) + a.sxy * (unsigned)k;
+    unsigned members = 0;
+    for (int g = 0; g < a.ns.n_groups; g++) members |= a.ns.mask[g];
+    FT x[NT];
+#pragma unroll
+    for (int n = 0; n < NT; n++) x[n] = (members & (1u << n)) ? a.tr[n][p] : FT(0);
+    scale_negatives_cell<NT>(a.ns, x);
+#pragma unroll
+    for (int n = 0; n < NT; n++)
+        if (members & (1u << n)) a.tr[n][p] = x[n];
 }
 
 // The model's default formulations (fm_code): LOBSTER Mondo light,
@@ -113,6 +137,8 @@ struct obme_handle {
     int cfg_id;                // sweep tile configuration (TileCfg table)
     bool uniform;
     bool pipe;                 // pipelined persistent sweep where it applies (OBME_PIPE=1 turns it on)
+    bool scaling;              // ScaleNegativeTracers groups configured
+    bool co2_inline;           // scaling with the CO2 flux: the flux is computed in the sweep
     double dz_u, zc0_u, w0_u, w1_u;
     // kernel timing: event pairs per kernel kind, and a pool of free events
     bool timing = false;
@@ -148,6 +174,41 @@ struct LaunchTimer {
         e0 = nullptr;
     }
 };
+
+// ScaleNegativeTracers groups: counts within the limits, members valid
+// tracer indices, no member twice in a group.
+static obme_status validate_scaling(const obme_negative_scaling& s, int ntracers)
+{
+    if (s.n_groups < 0 || s.n_groups > OBME_MAX_SCALE_GROUPS) return fail(OBME_ERR_INVALID, "bad number of scaling groups");
+    for (int g = 0; g < s.n_groups; g++) {
+        if (s.n_members[g] < 1 || s.n_members[g] > OBME_MAX_SCALE_MEMBERS)
+            return fail(OBME_ERR_INVALID, "bad number of scaling group members");
+        unsigned long long seen = 0;
+        for (int m = 0; m < s.n_members[g]; m++) {
+            const int t = s.member[g][m];
+            if (t < 0 || t >= ntracers || t >= 64) return fail(OBME_ERR_INVALID, "scaling member is not a tracer");
+            if (seen & (1ull << t)) return fail(OBME_ERR_INVALID, "scaling member listed twice in a group");
+            seen |= 1ull << t;
+        }
+    }
+    return OBME_OK;
+}
+
+template <typename FT, int NTMAX>
+static NegScale<FT, NTMAX> convert_scaling(const obme_negative_scaling& s)
+{
+    NegScale<FT, NTMAX> d;
+    std::memset(&d, 0, sizeof(d));
+    d.n_groups = s.n_groups;
+    for (int g = 0; g < s.n_groups; g++) {
+        for (int m = 0; m < s.n_members[g]; m++) {
+            d.mask[g] |= 1u << s.member[g][m];
+            d.sf[g][s.member[g][m]] = (FT)s.scalefactor[g][m];
+        }
+        d.fill[g] = (FT)s.invalid_fill_value[g];
+    }
+    return d;
+}
 
 static int ntracers_of(const obme_config& c)
 {
@@ -254,6 +315,10 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
             return fail(OBME_ERR_UNSUPPORTED, "a field of 2 GiB or more exceeds the 32-bit buffer offsets");
     }
     if (!g.zc || !g.zf || !g.dz) return fail(OBME_ERR_INVALID, "grid coordinates missing");
+    {
+        obme_status sv = validate_scaling(cfg->scaling, ntracers_of(*cfg));
+        if (sv != OBME_OK) return sv;
+    }
     if (cfg->sinking.scheme != OBME_UPWIND1)
         return fail(OBME_ERR_UNSUPPORTED, "only the first-order upwind sinking scheme runs on the GPU");
     if (cfg->par.surface_mode == OBME_SURFACE_ARRAY && !cfg->par.surface_array)
@@ -302,6 +367,8 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
     // sweep tile configuration: the first of the precision's preference list
     // whose column fits (Nz <= L * 64 / CPW); OBME_TILE="L,CPW,NW" overrides.
     h->cfg_id = choose_tile(h->prec, g.Nz);
+    h->scaling = cfg->scaling.n_groups > 0;
+    h->co2_inline = h->scaling && cfg->model == OBME_LOBSTER && cfg->carbonate_system && cfg->co2.enabled;
     // the pipelined sweep measured slower than one tile per workgroup on
     // the 512x512x64 LOBSTER step (0.56 vs 0.44 ms): off unless OBME_PIPE=1
     h->pipe = false;
@@ -404,7 +471,18 @@ static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers)
     a.w0_u = (FT)h->w0_u;
     a.w1_u = (FT)h->w1_u;
     a.invdz_u = (FT)1 / (FT)h->dz_u;
-    a.co2_on = (c.model == OBME_LOBSTER && c.carbonate_system && c.co2.enabled) ? 1 : 0;
+    a.co2_on = (c.model == OBME_LOBSTER && c.carbonate_system && c.co2.enabled && !h->co2_inline) ? 1 : 0;
+    a.ns = convert_scaling<FT, 16>(c.scaling);
+    if (h->co2_inline) {
+        a.Tprof = (c.temperature.mode == OBME_FIELD_PROFILE) ? h->d_Tprof : nullptr;
+        a.Sprof = (c.salinity.mode == OBME_FIELD_PROFILE) ? h->d_Sprof : nullptr;
+        a.Tconst = c.temperature.value;
+        a.Sconst = c.salinity.value;
+        a.Hprev = (h->prec == OBME_FP64) ? nullptr : h->d_Hprev;
+        a.invdz_top = (FT)(1.0 / h->dz[g.Nz - 1]);
+        a.co2 = c.co2;
+        a.solver = c.solver;
+    }
     if (sizeof(FT) == 4) {
         a.f = make_fast_params(c.npd);
         const double log2e = 1.4426950408889634;
@@ -485,13 +563,32 @@ static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_
             k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, true, false, default_fm<MODEL>()>
                            : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, true, false, default_fm<MODEL>()>;
     }
+    // negative scaling (and the inline CO2 flux) in the fused step: the
+    // general (non-FULL) column code with the extra work compiled in
+    bool extra = false;
+    if constexpr (MODE == TILE_STEP) {
+        if (h->scaling) {
+            extra = true;
+            if constexpr (CARB) {
+                if (h->co2_inline)
+                    k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, false, false, -1, 2>
+                                   : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, false, false, -1, 2>;
+                else
+                    k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, false, false, -1, 1>
+                                   : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, false, false, -1, 1>;
+            } else {
+                k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, false, false, -1, 1>
+                               : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, false, false, -1, 1>;
+            }
+        }
+    }
     // the pipelined persistent sweep (FP32 fused step, FULL columns)
     constexpr int PB = (64 % TXB == 0) ? 64 / TXB : 1;
     constexpr int NBLK = (L * 64 / CPW) / PB;
     constexpr bool PIPE_OK = sizeof(FT) == 4 && MODE == TILE_STEP && 64 % TXB == 0 && NBLK % NW == 0 &&
                              2 * (NT * NBLK / NW) < 64;
     if constexpr (PIPE_OK) {
-        if (full && h->pipe && a.surface_mode != OBME_SURFACE_ARRAY) {
+        if (full && h->pipe && !extra && a.surface_mode != OBME_SURFACE_ARRAY) {
             k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, true, true>
                            : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, true, true>;
             if (fm_code(a.f) == default_fm<MODEL>())
@@ -581,6 +678,21 @@ static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* 
             a.PAR0_stage[q] = (FT)(100 * v);
         }
     }
+    if (MODE != TILE_STEP) {
+        // the hooks: the flux by the surface kernel in tendency mode
+        const obme_config& c = h->cfg;
+        a.co2_on = (c.model == OBME_LOBSTER && c.carbonate_system && c.co2.enabled) ? 1 : 0;
+    }
+    a.T3 = (const FT*)T;
+    a.S3 = (const FT*)S;
+    if (MODE == TILE_PAR && h->scaling) {
+        // update_biogeochemical_state!: the scaling modifier first, in place
+        const int bs = 256;
+        const long ncell = (long)h->cfg.grid.Nx * h->cfg.grid.Ny * h->cfg.grid.Nz;
+        auto ks = (h->ntracers == 5) ? k_scale_grid<FT, 5> : (h->ntracers == 7 ? k_scale_grid<FT, 7> : k_scale_grid<FT, 9>);
+        hipLaunchKernelGGL(ks, dim3((unsigned)((ncell + bs - 1) / bs)), dim3(bs), 0, s, a);
+        HIP_TRY(hipGetLastError());
+    }
     obme_status r = dispatch_tile_cfg<FT, MODE>(h, a, s);
     if (r != OBME_OK || MODE == TILE_PAR || !a.co2_on) return r;
     SurfaceArgs<FT> sa = make_surface_args<FT>(h, tracers, T, S);
@@ -598,10 +710,12 @@ static obme_status run_tile(obme_handle* h, void* const* tracers, void* const* G
     return run_typed<float, MODE>(h, tracers, G, PAR, T, S, t, dt, s);
 }
 
-extern "C" obme_status obme_update_state(obme_handle* h, const void* const* tracers, void* PAR, double t,
-                                         void* stream)
+extern "C" obme_status obme_update_state(obme_handle* h, void* const* tracers, void* PAR, double t, void* stream)
 {
     if (!h || !tracers || !PAR) return fail(OBME_ERR_INVALID, "null argument");
+    if (h->scaling)
+        for (int n = 0; n < h->ntracers; n++)
+            if (!tracers[n]) return fail(OBME_ERR_INVALID, "null tracer pointer");
     return run_tile<TILE_PAR>(h, (void* const*)tracers, nullptr, PAR, nullptr, nullptr, t, 0.0,
                               (hipStream_t)stream);
 }
@@ -874,6 +988,8 @@ extern "C" obme_status obme_batch_create(const obme_config* cfg, int32_t n_colum
     if (!cfg || !offsets || !params || !out) return fail(OBME_ERR_INVALID, "null argument");
     *out = nullptr;
     if (cfg->model != OBME_NPZD) return fail(OBME_ERR_UNSUPPORTED, "the batched ensemble runs NPZD columns");
+    if (cfg->scaling.n_groups != 0)
+        return fail(OBME_ERR_UNSUPPORTED, "negative scaling is not built for the batched ensemble");
     if (cfg->precision != OBME_FP64 && cfg->precision != OBME_FP32_MIXED)
         return fail(OBME_ERR_INVALID, "unknown precision");
     if (n_columns < 1) return fail(OBME_ERR_INVALID, "no columns");
@@ -1113,6 +1229,7 @@ extern "C" int64_t obme_abi_sizeof(int32_t which)
     case 7: return sizeof(obme_field_source);
     case 8: return sizeof(obme_batch_params);
     case 9: return sizeof(obme_pisces_config);
+    case 10: return sizeof(obme_negative_scaling);
     default: return -1;
     }
 }

@@ -73,6 +73,47 @@ struct ModelTraits<OBME_LOBSTER, true> {
 // at stage s, [4+s] Alk at the start of stage s (s = 0, 1, 2).
 constexpr int SURF_REC = 7;
 
+// ScaleNegativeTracers groups for the kernels: members as bits of the
+// tracer index and a scalefactor per tracer index (obme_negative_scaling
+// converted on the host).
+template <typename FT, int NTMAX>
+struct NegScale {
+    int n_groups;
+    unsigned mask[OBME_MAX_SCALE_GROUPS];
+    FT sf[OBME_MAX_SCALE_GROUPS][NTMAX];
+    FT fill[OBME_MAX_SCALE_GROUPS];
+};
+
+// The scaling of one cell's NT tracer values x[] (src/Utils/negative_tracers.jl:
+// 198-251): per group t = Σ s·c, p = Σ s·c over c > 0 (summed in tracer
+// order: a reassociation of the reference's member order), t := fill where
+// t < 0, then c := c·t/p where c > 0 or c is not finite, else 0.
+template <int NT, typename FT, int NTMAX>
+__device__ __forceinline__ void scale_negatives_cell(const NegScale<FT, NTMAX>& s, FT* x)
+{
+#pragma unroll 1
+    for (int g = 0; g < s.n_groups; g++) {
+        const unsigned mask = s.mask[g];
+        FT t = FT(0), p = FT(0);
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            if (mask & (1u << n)) {
+                const FT vs = x[n] * s.sf[g][n];
+                t = t + vs;
+                if (x[n] > FT(0)) p = p + vs;
+            }
+        }
+        t = (t < FT(0)) ? s.fill[g] : t;
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            if (mask & (1u << n)) {
+                const FT v = x[n];
+                x[n] = (!isfinite(v) || v > FT(0)) ? v * t / p : FT(0);
+            }
+        }
+    }
+}
+
 // Kernel arguments (passed by value).
 template <typename FT>
 struct StepArgs {
@@ -100,7 +141,19 @@ struct StepArgs {
     float cr, cb;                  // FP32 path: e log2(R_chl / r) per band
     float xr2h, xb2h;              // FP32 uniform grid: xr2 dz/2, xb2 dz/2
     float wi0, wi1;                // FP32 uniform grid: w / dz of the two sinking tracers
-    int co2_on;
+    int co2_on;                    // CO2 flux by the surface kernel (surface records written)
+    NegScale<FT, 16> ns;           // negative scaling at the start of every stage (EXTRA >= 1)
+    // CO2 flux computed in the sweep by the top lane (EXTRA == 2: with the
+    // scaling, the top cell's DIC feeds back into the other tracers)
+    const FT* T3;
+    const FT* S3;
+    const double* Tprof;
+    const double* Sprof;
+    double Tconst, Sconst;
+    double* Hprev;                 // FP32-mixed warm start of [H+] per column
+    FT invdz_top;
+    obme_co2_exchange co2;
+    obme_carbon_solver solver;
 };
 
 // Per-level geometry rows in `lev` (each Nz entries).
@@ -165,6 +218,28 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)
     return x * q + (x < rem ? x : rem) + s;
 }
 
+// Flux with the mixed path's warm start: H from the previous stage (or an
+// FP32 guess) refined by FP64 Newton, with the reference solve as fallback.
+__device__ __forceinline__ double surface_flux_warm(const SurfaceCarbon& sc, const obme_carbon_solver& s,
+                                                    double DIC, double Alk, double pCO2_air, int mode,
+                                                    double& H, double& pH, int& evals)
+{
+    if (mode == 0) return surface_flux<true>(sc, s, DIC, Alk, pCO2_air, 0, pH, evals);
+    CarbonParams p = sc.p;
+    p.Alk = Alk * sc.conv;
+    p.DIC = DIC * sc.conv;
+    double x = (H > 0.0) ? H : (double)guess_H_fp32(p);
+    bool done = refine_H(p, x, evals);
+    if (!done) {
+        x = (double)guess_H_fp32(p);
+        done = refine_H(p, x, evals);
+        if (!done) x = solve_for_H(p, s, pow(10.0, -s.initial_pH_guess), evals);
+    }
+    H = x;
+    pH = -log10(x);
+    return sc.k * (pco2_from_H(p, x) - pCO2_air);
+}
+
 // s_waitcnt with only the vector-memory count constrained (gfx9 encoding:
 // vmcnt bits [3:0] and [15:14], expcnt [6:4] = 7, lgkmcnt [11:8] = 15).
 template <int N>
@@ -197,8 +272,10 @@ __device__ __forceinline__ void lds_load_b32(rsrc_t r, void* dst, unsigned voff)
     __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)dst, 4, voff, 0, 0, 0);
 }
 
+// EXTRA (fused step only): 0 plain; 1 negative scaling at the start of every
+// stage; 2 scaling and the CO2 flux computed here by the top lane.
 template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL,
-          bool PIPE = false, int FM = -1>
+          bool PIPE = false, int FM = -1, int EXTRA = 0>
 __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
 {
     using TR = ModelTraits<MODEL, CARB>;
@@ -425,10 +502,54 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
     constexpr int NSTAGE = (MODE == TILE_STEP) ? 3 : 1;
     const double g1 = 8.0 / 15.0, g2 = 5.0 / 12.0, g3 = 3.0 / 4.0, z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
     FT PARv[L];   // PAR of the own levels (output of the PAR and tendency modes)
+    static_assert(EXTRA == 0 || (MODE == TILE_STEP && !PIPE), "scaling and inline CO2: fused step, direct sweep");
+    static_assert(EXTRA < 2 || CARB, "inline CO2 needs the carbonate system");
+    // inline CO2 (EXTRA 2): [H+] of the previous stage (FP32-mixed warm start)
+    double Hc = 0.0;
+    if constexpr (EXTRA == 2 && F32) {
+        if (top_lane && a.Hprev) Hc = a.Hprev[col2d];
+    }
 
     auto run_stage = [&](const int st) {
         FT PAR0 = PAR0_const;
         if (a.surface_mode == OBME_SURFACE_DIURNAL) PAR0 = a.PAR0_stage[st];
+
+        if constexpr (EXTRA >= 1) {
+            // ScaleNegativeTracers first (update_biogeochemical_state! order,
+            // src/OceanBioME.jl:161-167).  FP64 applies it to every cell as
+            // the reference does; FP32-mixed where a member is negative or
+            // not finite (elsewhere it is the identity up to rounding), and
+            // the scaled value becomes the new U with no pending increment.
+            unsigned members = 0;
+#pragma unroll 1
+            for (int g = 0; g < a.ns.n_groups; g++) members |= a.ns.mask[g];
+#pragma unroll
+            for (int m = 0; m < L; m++) {
+                FT x[NT];
+#pragma unroll
+                for (int n = 0; n < NT; n++) x[n] = val(n, m);
+                if constexpr (F32) {
+                    bool need = false;
+#pragma unroll
+                    for (int n = 0; n < NT; n++)
+                        if (members & (1u << n)) need = need || !(x[n] >= 0.0f) || isinf(x[n]);
+                    if (need && has(m)) {
+                        scale_negatives_cell<NT>(a.ns, x);
+#pragma unroll
+                        for (int n = 0; n < NT; n++)
+                            if (members & (1u << n)) {
+                                U[n][m] = x[n];
+                                D[n][m] = 0.0f;
+                            }
+                    }
+                } else {
+                    scale_negatives_cell<NT>(a.ns, x);
+#pragma unroll
+                    for (int n = 0; n < NT; n++)
+                        if (members & (1u << n)) U[n][m] = x[n];
+                }
+            }
+        }
 
         // chlorophyll terms (P R/r)^e of the own levels
         FT pr[L], pb[L];
@@ -547,6 +668,23 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
                         Gv[TR::SINK1] = Gv[TR::SINK1] - (Fhi1 - g_wlo(1, kc) * v[TR::SINK1]) * idz;
                     }
                 }
+                if constexpr (EXTRA == 2) {
+                    // the top-cell CO2 flux (GasExchange, gas_exchange.jl:26-38) of
+                    // this stage's DIC and Alk, the surface kernel's expressions
+                    if (top_lane && m == 0) {
+                        const unsigned top = (unsigned)a.base0 + (unsigned)i + a.sx * (unsigned)j +
+                                             a.sxy * (unsigned)ktop;
+                        const double T = a.T3 ? (double)a.T3[top] : (a.Tprof ? a.Tprof[ktop] : a.Tconst);
+                        const double S = a.S3 ? (double)a.S3[top] : (a.Sprof ? a.Sprof[ktop] : a.Sconst);
+                        SurfaceCarbon sc;
+                        surface_carbon_setup(a.co2, T, S, a.co2.wind_speed, sc);
+                        double pH = 0.0;
+                        int ev = 0;
+                        const double F = surface_flux_warm(sc, a.solver, (double)val(7, 0), (double)val(8, 0),
+                                                           a.co2.air_concentration, F32 ? 1 : 0, Hc, pH, ev);
+                        Gv[7] = Gv[7] - (FT)F * a.invdz_top;
+                    }
+                }
                 if constexpr (MODE == TILE_TENDENCY) {
 #pragma unroll
                     for (int n = 0; n < NT; n++) Gm[n][m] = Gv[n];   // Gm carries G out
@@ -582,6 +720,9 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
     // (the stages stay a loop: unrolled, the FP32 step measured 6 % slower)
 #pragma unroll 1
     for (int st = 0; st < NSTAGE; st++) run_stage(st);
+    if constexpr (EXTRA == 2 && F32) {
+        if (top_lane && a.Hprev) a.Hprev[col2d] = Hc;
+    }
 
     if constexpr (PIPE) {
         // ---- stage out (PIPE): each lane scatters to the elements it
@@ -712,28 +853,6 @@ struct SurfaceArgs {
     obme_co2_exchange co2;
     obme_carbon_solver solver;
 };
-
-// Flux with the mixed path's warm start: H from the previous stage (or an
-// FP32 guess) refined by FP64 Newton, with the reference solve as fallback.
-__device__ __forceinline__ double surface_flux_warm(const SurfaceCarbon& sc, const obme_carbon_solver& s,
-                                                    double DIC, double Alk, double pCO2_air, int mode,
-                                                    double& H, double& pH, int& evals)
-{
-    if (mode == 0) return surface_flux<true>(sc, s, DIC, Alk, pCO2_air, 0, pH, evals);
-    CarbonParams p = sc.p;
-    p.Alk = Alk * sc.conv;
-    p.DIC = DIC * sc.conv;
-    double x = (H > 0.0) ? H : (double)guess_H_fp32(p);
-    bool done = refine_H(p, x, evals);
-    if (!done) {
-        x = (double)guess_H_fp32(p);
-        done = refine_H(p, x, evals);
-        if (!done) x = solve_for_H(p, s, pow(10.0, -s.initial_pH_guess), evals);
-    }
-    H = x;
-    pH = -log10(x);
-    return sc.k * (pco2_from_H(p, x) - pCO2_air);
-}
 
 template <typename FT, int MODE>
 __global__ void __launch_bounds__(256) k_surface(SurfaceArgs<FT> a)

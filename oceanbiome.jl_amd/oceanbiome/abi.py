@@ -128,6 +128,40 @@ class FieldSource(C.Structure):
     ]
 
 
+OBME_MAX_SCALE_GROUPS = 5
+OBME_MAX_SCALE_MEMBERS = 12
+
+
+class NegativeScaling(C.Structure):
+    """obme_negative_scaling (ScaleNegativeTracers groups)."""
+    _fields_ = [
+        ("n_groups", C.c_int32),
+        ("n_members", C.c_int32 * OBME_MAX_SCALE_GROUPS),
+        ("member", (C.c_int32 * OBME_MAX_SCALE_MEMBERS) * OBME_MAX_SCALE_GROUPS),
+        ("pad0", C.c_int32),
+        ("scalefactor", (C.c_double * OBME_MAX_SCALE_MEMBERS) * OBME_MAX_SCALE_GROUPS),
+        ("invalid_fill_value", C.c_double * OBME_MAX_SCALE_GROUPS),
+    ]
+
+
+def negative_scaling(groups, names):
+    """NegativeScaling from [(member names, scalefactors, invalid_fill_value), ...]
+    with member names resolved against the tracer-name tuple `names`."""
+    s = NegativeScaling()
+    if len(groups) > OBME_MAX_SCALE_GROUPS:
+        raise ValueError("too many conserved groups")
+    s.n_groups = len(groups)
+    for g, (members, factors, fill) in enumerate(groups):
+        if len(members) > OBME_MAX_SCALE_MEMBERS or len(members) != len(factors):
+            raise ValueError("bad conserved group")
+        s.n_members[g] = len(members)
+        for m, (name, f) in enumerate(zip(members, factors)):
+            s.member[g][m] = names.index(name)
+            s.scalefactor[g][m] = float(f)
+        s.invalid_fill_value[g] = float(fill)
+    return s
+
+
 class Config(C.Structure):
     _fields_ = [
         ("model", C.c_int32),
@@ -142,6 +176,7 @@ class Config(C.Structure):
         ("co2", CO2Exchange),
         ("temperature", FieldSource),
         ("salinity", FieldSource),
+        ("scaling", NegativeScaling),
     ]
 
 
@@ -166,6 +201,7 @@ class PiscesConfig(C.Structure):
         ("band_division", C.c_double * 3),
         ("surface_PAR", C.c_double),
         ("solver", CarbonSolver),
+        ("scaling", NegativeScaling),
     ]
 
 
@@ -191,4 +227,5 @@ ABI_STRUCTS = (
     ("obme_field_source", FieldSource),
     ("obme_batch_params", BatchParams),
     ("obme_pisces_config", PiscesConfig),
+    ("obme_negative_scaling", NegativeScaling),
 )

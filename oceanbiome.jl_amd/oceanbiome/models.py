@@ -202,8 +202,10 @@ class Biogeochemistry:
 
     def __init__(self, grid, nutrients, plankton, detritus, carbonate_system=None,
                  light_attenuation=None, gas_exchange=None, sinking_scheme="upwind",
-                 temperature=None, salinity=35.0):
+                 temperature=None, salinity=35.0, scale_negatives=False, invalid_fill_value=float("nan")):
         self.grid = grid
+        self.scale_negatives = bool(scale_negatives)
+        self.invalid_fill_value = float(invalid_fill_value)
         self.nutrients = nutrients
         self.plankton = plankton
         self.detritus = detritus
@@ -236,6 +238,27 @@ class Biogeochemistry:
 
     def required_biogeochemical_auxiliary_fields(self):
         return ("PAR",)
+
+    def conserved_tracers(self):
+        """conserved_tracers (NutrientsPlanktonDetritus/coupling_utils.jl:1-51):
+        [(names, scalefactors)] — nitrogen, then carbon with the carbonate
+        system, members in the reference's order."""
+        nitrogen = ["P", "Z"] + list(self.nutrients.tracers) + list(self.detritus.tracers)
+        groups = [(nitrogen, [1.0] * len(nitrogen))]
+        if self.carbonate_system is not None:
+            R = self.plankton.redfield_ratio
+            rho = self.plankton.carbon_calcite_ratio
+            groups.append((["P", "Z", "DIC"] + list(self.detritus.tracers),
+                           [(1 + rho) * R, R, 1.0] + [R] * len(self.detritus.tracers)))
+        return groups
+
+    def scaling_struct(self):
+        """obme_negative_scaling of NPZD/LOBSTER(grid; scale_negatives = true)
+        (NutrientsPlanktonDetritus.jl:46-58): one group per conserved set."""
+        if not self.scale_negatives:
+            return abi.NegativeScaling()
+        groups = [(n, f, self.invalid_fill_value) for n, f in self.conserved_tracers()]
+        return abi.negative_scaling(groups, self.required_biogeochemical_tracers())
 
     def written_tracers(self):
         """Tracers the step writes (T is read-only: zero tendency, no drift)."""
@@ -330,6 +353,7 @@ class Biogeochemistry:
             cfg.co2 = self.gas_exchange.struct()
         cfg.temperature = self._field_source(self.temperature)
         cfg.salinity = self._field_source(self.salinity)
+        cfg.scaling = self.scaling_struct()
         return cfg
 
     def _field_source(self, v):

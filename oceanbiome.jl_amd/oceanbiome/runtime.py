@@ -151,7 +151,9 @@ class HIPBiogeochemistry:
                 raise OBMEError(f"field must be a contiguous {self.dtype} CUDA tensor of shape {g.parent_shape}")
 
     def update_biogeochemical_state(self, tracers, PAR, t=0.0):
-        """The light step (Light/2band.jl:148-155): PAR interior from P."""
+        """update_biogeochemical_state!: the negative-scaling modifier first
+        (in place on the tracers, when configured), then the light step
+        (Light/2band.jl:148-155): PAR interior from P."""
         self._check_fields(list(tracers) + [PAR])
         _check(library().obme_update_state(self.handle, _ptr_array(tracers), _ptr(PAR), t, _stream()))
 

@@ -121,6 +121,10 @@ typedef struct orc_column_env {
  * in place; if PAR_out != NULL writes the PAR of the last stage evaluated. */
 void orc_column_run(const orc_column_env* env, double* U, int nsteps, double dt,
                     double t0, double* PAR_out);
+/* ScaleNegativeTracers on one cell's tracer values (indexed by tracer). */
+void orc_scale_negatives(const obme_negative_scaling* s, double* vals);
+/* ... in place on every interior cell of a grid. */
+int orc_grid_scale_negatives(const obme_config* cfg, double* const* tracers);
 /* Tendencies of one column at time t (PAR scan + BGC + sinking + CO2 flux). */
 void orc_column_tendencies(const orc_column_env* env, const double* U, double t,
                            double* G, double* PAR);

@@ -81,6 +81,8 @@ def lib():
         "orc_grid_run_columns": (I, [C.POINTER(abi.Config), C.POINTER(DP), DP, DP, DP, I, D, D,
                                      C.POINTER(I), C.POINTER(I), I, I, I]),
         "orc_grid_update_state": (I, [C.POINTER(abi.Config), C.POINTER(DP), DP, DP, D]),
+        "orc_scale_negatives": (None, [C.POINTER(abi.NegativeScaling), DP]),
+        "orc_grid_scale_negatives": (I, [C.POINTER(abi.Config), C.POINTER(DP)]),
         "orc_grid_tendencies": (I, [C.POINTER(abi.Config), C.POINTER(DP), C.POINTER(DP), DP, DP, DP, D, DP]),
         "orc_batch_run": (I, [C.POINTER(abi.Config), I, C.POINTER(I), C.POINTER(DP), D, D, C.POINTER(DP), I, D,
                               D, I]),
@@ -259,6 +261,18 @@ def grid_run_columns(cfg, tracers, cols_i, cols_j, nsteps, dt, t0=0.0, T3=None, 
     lib().orc_grid_run_columns(C.byref(cfg), _ptrs(tracers), _dp(T3), _dp(S3), _dp(PAR0), nsteps, dt, t0,
                                ci.ctypes.data_as(C.POINTER(I)), cj.ctypes.data_as(C.POINTER(I)), len(ci),
                                nthreads, fast_exit)
+
+
+def scale_negatives(scaling, values):
+    """ScaleNegativeTracers on one cell: `values` indexed by tracer; returns the new values."""
+    v = np.ascontiguousarray(values, dtype=np.float64).copy()
+    lib().orc_scale_negatives(C.byref(scaling), _dp(v))
+    return v
+
+
+def grid_scale_negatives(cfg, tracers):
+    """The scaling modifier in place on Oceananigans-layout float64 arrays."""
+    lib().orc_grid_scale_negatives(C.byref(cfg), _ptrs(tracers))
 
 
 def grid_update_state(cfg, tracers, PAR, t=0.0, PAR0=None):

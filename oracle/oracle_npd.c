@@ -270,6 +270,39 @@ void orc_sinking_divergence(int scheme, int Nz, const double* w, const double* c
 
 /* ---------------- column stepper ---------------- */
 
+/* ScaleNegativeTracers on the tracer values of one cell, group after group
+ * (src/Utils/negative_tracers.jl: scale_for_negs_cpu! :255-276, the same
+ * arithmetic as the GPU kernel :198-251): t = Σ s·c, p = Σ s·c over c > 0,
+ * t := fill if t < 0, c := c·t/p if c > 0 or c is not finite, else 0. */
+void orc_scale_negatives(const obme_negative_scaling* s, double* vals)
+{
+    for (int g = 0; g < s->n_groups; g++) {
+        double t = 0.0, p = 0.0;
+        for (int m = 0; m < s->n_members[g]; m++) {
+            double value = vals[s->member[g][m]];
+            double sf = s->scalefactor[g][m];
+            t += value * sf;
+            if (value > 0) p += value * sf;
+        }
+        t = (t < 0) ? s->invalid_fill_value[g] : t;
+        for (int m = 0; m < s->n_members[g]; m++) {
+            double value = vals[s->member[g][m]];
+            vals[s->member[g][m]] = (!isfinite(value) || value > 0) ? value * t / p : 0.0;
+        }
+    }
+}
+
+static void scale_column(const obme_config* cfg, int Nz, int nt, double* U)
+{
+    if (cfg->scaling.n_groups <= 0) return;
+    double vals[OBME_MAX_TRACERS];
+    for (int k = 0; k < Nz; k++) {
+        for (int n = 0; n < nt; n++) vals[n] = U[n * Nz + k];
+        orc_scale_negatives(&cfg->scaling, vals);
+        for (int n = 0; n < nt; n++) U[n * Nz + k] = vals[n];
+    }
+}
+
 static double surface_PAR(const orc_column_env* env, double t)
 {
     if (env->cfg->par.surface_mode == OBME_SURFACE_DIURNAL) return orc_default_surface_PAR(t);
@@ -334,20 +367,25 @@ void orc_column_run(const orc_column_env* env, double* U, int nsteps, double dt,
     const double g1 = 8.0 / 15.0, g2 = 5.0 / 12.0, g3 = 3.0 / 4.0;
     const double z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
     int readonly = (env->cfg->model == OBME_NPZD) ? 3 : -1;
+    /* the modifiers run first in update_biogeochemical_state!
+     * (src/OceanBioME.jl:161-167): scaling before every tendency evaluation */
     for (int s = 0; s < nsteps; s++) {
         double t = t0 + s * dt;
+        scale_column(env->cfg, Nz, nt, U);
         orc_column_tendencies(env, U, t, G, PAR);
         for (int i = 0; i < ncell; i++) {
             if (i / Nz == readonly) continue;
             U[i] += dt * g1 * G[i];
         }
         memcpy(Gm, G, sizeof(double) * ncell);
+        scale_column(env->cfg, Nz, nt, U);
         orc_column_tendencies(env, U, t + g1 * dt, G, PAR);
         for (int i = 0; i < ncell; i++) {
             if (i / Nz == readonly) continue;
             U[i] += dt * (g2 * G[i] + z2 * Gm[i]);
         }
         memcpy(Gm, G, sizeof(double) * ncell);
+        scale_column(env->cfg, Nz, nt, U);
         orc_column_tendencies(env, U, t + (g1 + g2 + z2) * dt, G, PAR);
         for (int i = 0; i < ncell; i++) {
             if (i / Nz == readonly) continue;
@@ -440,6 +478,24 @@ int orc_grid_run(const obme_config* cfg, double* const* tracers, const double* T
     for (long c = 0; c < ncols; c++)
         run_one_column(cfg, tracers, T3, S3, PAR0, (int)(c % g->Nx), (int)(c / g->Nx), nsteps, dt, t0,
                        fast_exit);
+    return 0;
+}
+
+/* The scaling modifier in place on every interior cell (the first part of
+ * update_biogeochemical_state!). */
+int orc_grid_scale_negatives(const obme_config* cfg, double* const* tracers)
+{
+    const obme_grid* g = &cfg->grid;
+    int nt = orc_npd_ntracers(cfg->model, cfg->carbonate_system);
+    double vals[OBME_MAX_TRACERS];
+    for (int k = 0; k < g->Nz; k++)
+        for (int j = 0; j < g->Ny; j++)
+            for (int i = 0; i < g->Nx; i++) {
+                long q = lidx(g, i, j, k);
+                for (int n = 0; n < nt; n++) vals[n] = tracers[n][q];
+                orc_scale_negatives(&cfg->scaling, vals);
+                for (int n = 0; n < nt; n++) tracers[n][q] = vals[n];
+            }
     return 0;
 }
 
