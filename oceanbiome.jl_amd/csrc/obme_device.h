@@ -461,6 +461,47 @@ __device__ __forceinline__ void npzd_fast(const FastParams& f, float N, float P,
     dD = ponw + solid - G.onSmall - f.rem * D;
 }
 
+// ScaleNegativeTracers groups for the kernels: members as bits of the
+// tracer index and a scalefactor per tracer index (obme_negative_scaling
+// converted on the host).
+template <typename FT, int NTMAX>
+struct NegScale {
+    int n_groups;
+    unsigned mask[OBME_MAX_SCALE_GROUPS];
+    FT sf[OBME_MAX_SCALE_GROUPS][NTMAX];
+    FT fill[OBME_MAX_SCALE_GROUPS];
+};
+
+// The scaling of one cell's NT tracer values x[] (src/Utils/negative_tracers.jl:
+// 198-251): per group t = Σ s·c, p = Σ s·c over c > 0 (summed in tracer
+// order: a reassociation of the reference's member order), t := fill where
+// t < 0, then c := c·t/p where c > 0 or c is not finite, else 0.
+template <int NT, typename FT, int NTMAX>
+__device__ __forceinline__ void scale_negatives_cell(const NegScale<FT, NTMAX>& s, FT* x)
+{
+#pragma unroll 1
+    for (int g = 0; g < s.n_groups; g++) {
+        const unsigned mask = s.mask[g];
+        FT t = FT(0), p = FT(0);
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            if (mask & (1u << n)) {
+                const FT vs = x[n] * s.sf[g][n];
+                t = t + vs;
+                if (x[n] > FT(0)) p = p + vs;
+            }
+        }
+        t = (t < FT(0)) ? s.fill[g] : t;
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            if (mask & (1u << n)) {
+                const FT v = x[n];
+                x[n] = (!isfinite(v) || v > FT(0)) ? v * t / p : FT(0);
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- carbonate chemistry (FP64)
 
 // teos10_polynomial_approximation (Models/seawater_density.jl:33-39) over

@@ -142,6 +142,30 @@ __global__ void __launch_bounds__(256) k_pisces_omega(PiscesArgs<FT> a)
                                         (double)a.in[OBME_PI_SI][p], P, a.solver, ev);
 }
 
+// Thread per cell: ScaleNegativeTracers in place on the stage input (the
+// modifier runs first in update_biogeochemical_state!, src/OceanBioME.jl:
+// 161-167; src/Utils/negative_tracers.jl:198-251), the groups of
+// PISCES/coupling_utils.jl:11-35.
+template <typename FT>
+__global__ void __launch_bounds__(256) k_pisces_scale(PiscesArgs<FT> a, NegScale<FT, 32> ns)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long ncol = (long)a.Nx * a.Ny;
+    if (c >= ncol * a.Nz) return;
+    const int k = (int)(c / ncol);
+    const int col = (int)(c % ncol);
+    const unsigned p = pidx(a, col % a.Nx, col / a.Nx, k);
+    unsigned members = 0;
+    for (int g = 0; g < ns.n_groups; g++) members |= ns.mask[g];
+    FT x[NW];
+#pragma unroll
+    for (int n = 0; n < NW; n++) x[n] = (members & (1u << n)) ? a.in[n][p] : FT(0);
+    scale_negatives_cell<NW>(ns, x);
+#pragma unroll
+    for (int n = 0; n < NW; n++)
+        if (members & (1u << n)) const_cast<FT*>(a.in[n])[p] = x[n];
+}
+
 // Thread per cell: the T-, S- and pressure-dependent constants of the
 // calcite saturation (the first half of calcite_saturation).
 template <typename FT>
@@ -293,6 +317,7 @@ struct obme_pisces {
     void* d_copy[NW] = {};
     size_t field_bytes = 0;
     std::vector<double> zc, zf, dz;
+    bool scaling = false;
 };
 
 extern "C" obme_status obme_pisces_destroy(obme_pisces* h)
@@ -357,6 +382,22 @@ extern "C" obme_status obme_pisces_create(const obme_pisces_config* cfg, obme_pi
     if (parent >= ((size_t)1 << 31)) return obme_fail(OBME_ERR_UNSUPPORTED, "field too large for 32-bit offsets");
     if (!(cfg->params[OBME_PP_SCALARS + OBME_PS_MIXED_LAYER_DEPTH] < 0))
         return obme_fail(OBME_ERR_INVALID, "mixed_layer_depth must be negative (below the surface)");
+    {
+        const obme_negative_scaling& s = cfg->scaling;
+        if (s.n_groups < 0 || s.n_groups > OBME_MAX_SCALE_GROUPS)
+            return obme_fail(OBME_ERR_INVALID, "bad number of scaling groups");
+        for (int gi = 0; gi < s.n_groups; gi++) {
+            if (s.n_members[gi] < 1 || s.n_members[gi] > OBME_MAX_SCALE_MEMBERS)
+                return obme_fail(OBME_ERR_INVALID, "bad number of scaling group members");
+            unsigned seen = 0;
+            for (int m = 0; m < s.n_members[gi]; m++) {
+                const int t = s.member[gi][m];
+                if (t < 0 || t >= NW) return obme_fail(OBME_ERR_INVALID, "scaling member is not a written tracer");
+                if (seen & (1u << t)) return obme_fail(OBME_ERR_INVALID, "scaling member listed twice in a group");
+                seen |= 1u << t;
+            }
+        }
+    }
     obme_pisces* h = new obme_pisces();
     h->cfg = *cfg;
     h->prec = cfg->precision;
@@ -368,6 +409,7 @@ extern "C" obme_status obme_pisces_create(const obme_pisces_config* cfg, obme_pi
     h->zf.assign(g.zf, g.zf + g.Nz + 1);
     h->dz.assign(g.dz, g.dz + g.Nz);
     h->cfg.grid.zc = h->cfg.grid.zf = h->cfg.grid.dz = nullptr;
+    h->scaling = cfg->scaling.n_groups > 0;
     obme_status st = (h->prec == OBME_FP64) ? pisces_upload<double>(h) : pisces_upload<float>(h);
     auto alloc = [&](void** d, size_t bytes) {
         if (st == OBME_OK && hipMalloc(d, bytes) != hipSuccess) st = obme_fail(OBME_ERR_NOMEM, "PISCES buffers");
@@ -440,6 +482,9 @@ static double host_day_length(double t, double phi)
     return (24 - 24.0 / 180 * (std::acos(L) * (180.0 / 3.141592653589793))) * 3600.0;
 }
 
+template <typename FT>
+static obme_status pisces_scale(obme_pisces* h, const void* const* in, hipStream_t s);
+
 // One stage.  mode 0: the substep of obme_pisces_step (Ω inline from the
 // prepared constants; have_H: the [H+] of the previous stage is stored);
 // mode 1: tendencies added into out (Ω by its own kernel).
@@ -459,6 +504,11 @@ static obme_status pisces_stage(obme_pisces* h, const void* const* in, void* con
     a.A = (FT)(dt * gam[stage]);
     a.B = (FT)(dt * zet[stage]);
     a.have_H = have_H;
+    if (mode == 0) {
+        // the fused step: the scaling modifier first, on the stage input
+        obme_status r = pisces_scale<FT>(h, in, s);
+        if (r != OBME_OK) return r;
+    }
     const int bs = 256;
     hipLaunchKernelGGL(k_pisces_columns<FT>, dim3((unsigned)((h->ncol + bs - 1) / bs)), dim3(bs), 0, s, a);
     OBME_HIP_TRY(hipGetLastError());
@@ -470,6 +520,36 @@ static obme_status pisces_stage(obme_pisces* h, const void* const* in, void* con
         OBME_HIP_TRY(hipGetLastError());
         hipLaunchKernelGGL((k_pisces_cell<FT, 1, 0>), dim3(nb), dim3(bs), 0, s, a);
     }
+    OBME_HIP_TRY(hipGetLastError());
+    return OBME_OK;
+}
+
+template <typename FT>
+static NegScale<FT, 32> pisces_scaling(const obme_negative_scaling& s)
+{
+    NegScale<FT, 32> d;
+    std::memset(&d, 0, sizeof(d));
+    d.n_groups = s.n_groups;
+    for (int g = 0; g < s.n_groups; g++) {
+        for (int m = 0; m < s.n_members[g]; m++) {
+            d.mask[g] |= 1u << s.member[g][m];
+            d.sf[g][s.member[g][m]] = (FT)s.scalefactor[g][m];
+        }
+        d.fill[g] = (FT)s.invalid_fill_value[g];
+    }
+    return d;
+}
+
+// the scaling modifier in place on the arrays `in` (no-op without groups)
+template <typename FT>
+static obme_status pisces_scale(obme_pisces* h, const void* const* in, hipStream_t s)
+{
+    if (!h->scaling) return OBME_OK;
+    PiscesArgs<FT> a = pisces_args<FT>(h, 0.0);
+    for (int n = 0; n < NTR; n++) a.in[n] = (const FT*)in[n];
+    const int bs = 256;
+    hipLaunchKernelGGL(k_pisces_scale<FT>, dim3((unsigned)((h->ncell + bs - 1) / bs)), dim3(bs), 0, s, a,
+                       pisces_scaling<FT>(h->cfg.scaling));
     OBME_HIP_TRY(hipGetLastError());
     return OBME_OK;
 }
@@ -530,6 +610,11 @@ extern "C" obme_status obme_pisces_update_state(obme_pisces* h, const void* cons
         if (!tracers[n]) return obme_fail(OBME_ERR_INVALID, "null tracer pointer");
     hipStream_t s = (hipStream_t)stream;
     const int bs = 256;
+    {
+        // the modifiers first: the tracers are rescaled in place
+        obme_status r = (h->prec == OBME_FP64) ? pisces_scale<double>(h, tracers, s) : pisces_scale<float>(h, tracers, s);
+        if (r != OBME_OK) return r;
+    }
     if (h->prec == OBME_FP64) {
         PiscesArgs<double> a = pisces_args<double>(h, t);
         for (int n = 0; n < NTR; n++) a.in[n] = (const double*)tracers[n];

@@ -119,8 +119,10 @@ class PISCES:
 
     def __init__(self, grid, surface_PAR=100.0, mixed_layer_depth=-50.0, kappa=1.0, latitude=45.0,
                  bands=((400, 500), (500, 600), (600, 700)), surface_PAR_division=None, POC_speed=2 / day,
-                 open_bottom=True, **overrides):
+                 open_bottom=True, scale_negatives=False, invalid_fill_value=float("nan"), **overrides):
         self.grid = grid
+        self.scale_negatives = bool(scale_negatives)
+        self.invalid_fill_value = float(invalid_fill_value)
         self.surface_PAR = float(surface_PAR)
         self.nano = _phyto(3 * day, 0.013, 0.13, 0.8, False, 2.1, 0.42, 0.4, 0.0, 0.033, 1.0)
         self.diatoms = _phyto(4 * day, 0.039, 0.39, 2.4, True, 1.6, 0.69, 0.7, 0.03 / day, 0.05, 3.0)
@@ -169,6 +171,25 @@ class PISCES:
     def required_biogeochemical_auxiliary_fields(self):
         return ("zₘₓₗ", "zₑᵤ", "Si′", "Ω", "κ", "mixed_layer_PAR", "wPOC", "wGOC", "PAR", "PAR₁", "PAR₂", "PAR₃")
 
+    def conserved_tracers(self):
+        """conserved_tracers(bgc::PISCES) (PISCES/coupling_utils.jl:11-35):
+        carbon, iron, phosphate, silicon, nitrogen — [(names, scalefactors)]."""
+        thP, thN = self.scalars["thetaP"], self.scalars["thetaN"]
+        carbon = ["P", "D", "Z", "M", "DOC", "POC", "GOC", "DIC", "CaCO₃"]
+        iron = (["PFe", "DFe", "Z", "M", "SFe", "BFe", "Fe"],
+                [1.0, 1.0, self.micro["iron_ratio"], self.meso["iron_ratio"], 1.0, 1.0, 1.0])
+        phosphate = (["P", "D", "Z", "M", "DOC", "POC", "GOC", "PO₄"], [thP] * 7 + [1.0])
+        silicon = ["DSi", "Si", "PSi"]
+        nitrogen = (["NH₄", "NO₃", "P", "D", "Z", "M", "DOC", "POC", "GOC"], [1.0, 1.0] + [thN] * 7)
+        return [(carbon, [1.0] * len(carbon)), iron, phosphate, (silicon, [1.0] * 3), nitrogen]
+
+    def scaling_struct(self):
+        """obme_negative_scaling of PISCES(; scale_negatives = true) (PISCES.jl:392-401)."""
+        if not self.scale_negatives:
+            return abi.NegativeScaling()
+        groups = [(n, f, self.invalid_fill_value) for n, f in self.conserved_tracers()]
+        return abi.negative_scaling(groups, TRACERS)
+
     def param_vector(self):
         v = [self.nano[f] for f in PHYTO_FIELDS] + [self.diatoms[f] for f in PHYTO_FIELDS]
         v += [self.micro[f] for f in ZOO_FIELDS] + [self.meso[f] for f in ZOO_FIELDS]
@@ -199,6 +220,7 @@ class PISCES:
         cfg.open_bottom = 1 if self.open_bottom else 0
         cc = self.carbon_chemistry
         cfg.solver = cc.solver.struct(cc.initial_pH_guess)
+        cfg.scaling = self.scaling_struct()
         return cfg
 
     def __repr__(self):

@@ -680,9 +680,45 @@ def full_tendencies(q, light, F, zc, zf, dz, t, solver, open_bottom=True, nthrea
     return G, A
 
 
-def step(q, light, F, zc, zf, dz, nsteps, dt, t0, solver, open_bottom=True, nthreads=8):
+def conserved_groups(q, fill=float("nan")):
+    """conserved_tracers(bgc::PISCES) (PISCES/coupling_utils.jl:11-35) as
+    (member names, scalefactors, invalid_fill_value), in the reference's
+    group and member order."""
+    thP, thN = q["thetaP"], q["thetaN"]
+    return [
+        (("P", "D", "Z", "M", "DOC", "POC", "GOC", "DIC", "CaCO3"), (1.0,) * 9, fill),
+        (("PFe", "DFe", "Z", "M", "SFe", "BFe", "Fe"),
+         (1.0, 1.0, q["micro"]["iron_ratio"], q["meso"]["iron_ratio"], 1.0, 1.0, 1.0), fill),
+        (("P", "D", "Z", "M", "DOC", "POC", "GOC", "PO4"), (thP,) * 7 + (1.0,), fill),
+        (("DSi", "Si", "PSi"), (1.0,) * 3, fill),
+        (("NH4", "NO3", "P", "D", "Z", "M", "DOC", "POC", "GOC"), (1.0, 1.0) + (thN,) * 7, fill),
+    ]
+
+
+def scale_negatives(F, groups):
+    """ScaleNegativeTracers in place on the arrays F (src/Utils/negative_tracers.jl:
+    255-276): per cell and group t = Σ s·c, p = Σ s·c over c > 0 (member
+    order), t := fill where t < 0, c := c·t/p where c > 0 or c is not finite,
+    else 0."""
+    for names, factors, fill in groups:
+        t = np.zeros_like(F[names[0]])
+        p = np.zeros_like(t)
+        for n, s in zip(names, factors):
+            vs = F[n] * s
+            t = t + vs
+            p = p + np.where(F[n] > 0, vs, 0.0)
+        t = np.where(t < 0, fill, t)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            for n in names:
+                v = F[n]
+                F[n] = np.where(~np.isfinite(v) | (v > 0), v * t / p, 0.0)
+    return F
+
+
+def step(q, light, F, zc, zf, dz, nsteps, dt, t0, solver, open_bottom=True, nthreads=8, scaling=None):
     """nsteps RK3 steps (Oceananigans γ, ζ; the substep form of
-    BoxModel/timesteppers.jl:79-92) of the 24 written tracers, in place."""
+    BoxModel/timesteppers.jl:79-92) of the 24 written tracers, in place;
+    `scaling` (conserved_groups) runs at the start of every stage."""
     g = (8 / 15, 5 / 12, 3 / 4)
     z = (None, -17 / 60, -5 / 12)
     cst = (0.0, 8 / 15, 2 / 3)
@@ -690,6 +726,8 @@ def step(q, light, F, zc, zf, dz, nsteps, dt, t0, solver, open_bottom=True, nthr
         Gm = None
         for st in range(3):
             t = t0 + s * dt + cst[st] * dt
+            if scaling:
+                scale_negatives(F, scaling)
             G, _ = full_tendencies(q, light, F, zc, zf, dz, t, solver, open_bottom, nthreads)
             for name in WRITTEN:
                 if st == 0:

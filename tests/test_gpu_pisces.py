@@ -88,3 +88,49 @@ def test_step(precision):
     assert all(e <= tol for e in errs.values()), errs
     # T and S untouched
     assert np.array_equal(tr[24].double().cpu().numpy(), wl.tracers[24].astype(tr[24].cpu().numpy().dtype))
+
+
+def _scaled_run(precision, flipped, seed=9):
+    import torch
+    from oceanbiome import pisces as HP
+    wl = workloads.pisces_3d(8, 4, 20, nsteps=3, dt=300.0, seed=seed)
+    wl.bgc = HP.PISCES(wl.grid, scale_negatives=True)
+    rng = np.random.default_rng(3)
+    g = wl.grid
+    for name in flipped:
+        a = g.interior(wl.tracers[HP.TRACERS.index(name)])
+        flip = rng.random(a.shape) < 0.1
+        a[flip] = -0.2 * np.abs(a[flip])
+    q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
+    with np.errstate(all="ignore"):
+        OP.step(q, light, F, zc, zf, dz, wl.nsteps, wl.dt, 0.0, solver, scaling=OP.conserved_groups(q))
+    dev, tr = _device(wl, precision)
+    dev.step(tr, wl.nsteps, wl.dt)
+    torch.cuda.synchronize()
+    return {name: (g.interior(tr[n].double().cpu().numpy()), F[name]) for n, name in enumerate(OP.WRITTEN)}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_step_with_negative_scaling(precision):
+    """PISCES(; scale_negatives = true): the five groups applied at the start
+    of every stage, against the oracle (negative nutrients and DOC; every
+    group total stays positive, so no fill value enters)."""
+    print("case pisces scaling", precision, flush=True)
+    res = _scaled_run(precision, ("NH₄", "NO₃", "Fe", "DSi", "PO₄", "DOC"))
+    errs = {name: rel_l2(a, b) for name, (a, b) in res.items()}
+    print("errors", errs, flush=True)
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    assert all(e <= tol for e in errs.values()), errs
+
+
+def test_scaling_zeroed_phytoplankton_gives_the_same_nans():
+    """Zeroing a negative P leaves PChl and PFe (members of no group with P,
+    or of another group) positive, so the ratios PChl/P and PFe/P divide by
+    P + eps(0): the reference's arithmetic gives NaN there.  The device
+    gives NaN in exactly the same cells and agrees elsewhere (FP64)."""
+    print("case pisces scaling nan", flush=True)
+    res = _scaled_run("fp64", ("P", "Z", "POC"))
+    for name, (a, b) in res.items():
+        assert np.array_equal(np.isnan(a), np.isnan(b)), name
+        ok = ~np.isnan(b)
+        assert rel_l2(a[ok], b[ok]) <= 1e-12, name

@@ -170,3 +170,45 @@ def test_day_length():
     assert 14.5 * 3600 < OP.cbm_day_length(172 * d, 45.0) < 16.5 * 3600
     assert 7.5 * 3600 < OP.cbm_day_length(355 * d, 45.0) < 9.5 * 3600
     assert abs(OP.cbm_day_length(80 * d, 0.0) - 12.1 * 3600) < 0.3 * 3600
+
+
+def test_pisces_conserved_groups_match_host():
+    """The oracle's conserved groups (PISCES/coupling_utils.jl:11-35) and the
+    host description's (two restatements) agree in order and factors."""
+    grid = RectilinearGrid(size=(2, 2, 4), extent=(1, 1, 40))
+    m = HP.PISCES(grid, scale_negatives=True)
+    q = OP.params_from_vector(m.param_vector())
+    ascii_ = {"CaCO₃": "CaCO3", "NO₃": "NO3", "NH₄": "NH4", "PO₄": "PO4", "O₂": "O2"}
+    host = [([ascii_.get(n, n) for n in names], list(f)) for names, f in m.conserved_tracers()]
+    orc = [(list(names), list(f)) for names, f, _ in OP.conserved_groups(q)]
+    assert [h[0] for h in host] == [o[0] for o in orc]
+    for (hn, hf), (on, of) in zip(host, orc):
+        assert hf == pytest.approx(of, rel=1e-15)
+    s = m.config().scaling
+    assert s.n_groups == 5 and [s.n_members[g] for g in range(5)] == [9, 7, 8, 3, 9]
+
+
+def test_pisces_scaling_zeroes_negatives_and_keeps_the_last_group_total():
+    """Groups share members (P is in the carbon, phosphate and nitrogen
+    groups), so each later group rescales members of the earlier ones: a
+    group keeps the total it sees when it runs (checked for the last,
+    nitrogen), and no member is left negative."""
+    rng = np.random.default_rng(2)
+    q = OP.default_parameters()
+    base = REF["pisces_conservation"]["initial_values"]
+    n = 256
+    F = {k: float(base.get(k, 1.0)) * np.exp(0.3 * rng.standard_normal(n)) for k in OP.TRACERS}
+    for k in ("P", "Z", "POC", "NH4", "Fe", "DSi"):
+        flip = rng.random(n) < 0.2
+        F[k][flip] = -0.1 * F[k][flip]
+    groups = OP.conserved_groups(q)
+    names, factors, _ = groups[-1]
+    OP.scale_negatives(F, groups[:-1])
+    before = sum(F[k] * s for k, s in zip(names, factors))   # the total the last group sees
+    OP.scale_negatives(F, groups[-1:])
+    after = sum(F[k] * s for k, s in zip(names, factors))
+    ok = before > 0
+    assert np.allclose(after[ok], before[ok], rtol=1e-13)
+    for g_names, _, _ in groups:
+        for k in g_names:
+            assert np.all((F[k] >= 0) | np.isnan(F[k])), k
