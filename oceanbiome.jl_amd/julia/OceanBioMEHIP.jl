@@ -100,6 +100,22 @@ struct ObmeFieldSource
     profile :: Ptr{Float64}
 end
 
+const OBME_MAX_SCALE_GROUPS = 5
+const OBME_MAX_SCALE_MEMBERS = 12
+
+# ScaleNegativeTracers groups (src/Utils/negative_tracers.jl); n_groups = 0: off.
+# `member` and `scalefactor` flatten the C arrays [group][member]: entry
+# g * OBME_MAX_SCALE_MEMBERS + m + 1 (0-based g, m); members are 0-based
+# tracer indices in required_biogeochemical_tracers order.
+struct ObmeNegativeScaling
+    n_groups :: Int32
+    n_members :: NTuple{OBME_MAX_SCALE_GROUPS, Int32}
+    member :: NTuple{OBME_MAX_SCALE_GROUPS * OBME_MAX_SCALE_MEMBERS, Int32}
+    pad0 :: Int32
+    scalefactor :: NTuple{OBME_MAX_SCALE_GROUPS * OBME_MAX_SCALE_MEMBERS, Float64}
+    invalid_fill_value :: NTuple{OBME_MAX_SCALE_GROUPS, Float64}
+end
+
 struct ObmeConfig
     model :: Int32
     precision :: Int32
@@ -113,6 +129,7 @@ struct ObmeConfig
     co2 :: ObmeCO2Exchange
     temperature :: ObmeFieldSource
     salinity :: ObmeFieldSource
+    scaling :: ObmeNegativeScaling
 end
 
 # layout check against the library (obme_abi_sizeof)
@@ -120,6 +137,10 @@ function check_abi()
     for (idx, T) in enumerate((ObmeConfig, ObmeGrid, ObmeNPDParams, ObmeTwoBandPAR, ObmeSinking,
                                ObmeCarbonSolver, ObmeCO2Exchange, ObmeFieldSource))
         n = ccall((:obme_abi_sizeof, libobme), Int64, (Int32,), idx - 1)
+        n == sizeof(T) || error("ABI mismatch for $T: library $n bytes, Julia $(sizeof(T))")
+    end
+    for (idx, T) in ((10, ObmeNegativeScaling),)
+        n = ccall((:obme_abi_sizeof, libobme), Int64, (Int32,), idx)
         n == sizeof(T) || error("ABI mismatch for $T: library $n bytes, Julia $(sizeof(T))")
     end
     return nothing
@@ -221,6 +242,7 @@ struct ObmePiscesConfig
     band_division :: NTuple{3, Float64}
     surface_PAR :: Float64
     solver :: ObmeCarbonSolver
+    scaling :: ObmeNegativeScaling
 end
 
 """
