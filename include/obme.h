@@ -480,6 +480,22 @@ enum {
     OBME_PP_COUNT = 2 * OBME_PH_COUNT + 2 * OBME_PZ_COUNT + OBME_PS_COUNT
 };
 
+/* OxygenGasExchangeBoundaryCondition (GasExchange/GasExchange.jl:127-146):
+ * F = k (O2 - C_air α) with k = k660(u10) / sqrt(Sc(T)/660)
+ * (gas_transfer_velocity.jl:24-34; Ho06 :61), Sc the Wanninkhof (2014)
+ * quartic (schmidt_number.jl:15-16), α = β/Tk the Wanninkhof (1992)
+ * solubility (gas_solubility.jl:28-54, with its S (B1 + B2 T/100 +
+ * B2 (T/100)^2) term as written there). */
+typedef struct obme_o2_exchange {
+    int32_t enabled;
+    int32_t pad0;
+    double air_concentration;      /* mmol O2 / m3 (9352.7) */
+    double wind_speed;             /* m/s */
+    double schmidt[5];             /* a, b, c, d, e */
+    double transfer_velocity[3];   /* k660 = c0 + c1 u + c2 u^2 */
+    double solubility[6];          /* A1, A2, A3, B1, B2, B3 */
+} obme_o2_exchange;
+
 /* PISCES configuration: grid, parameters (OBME_PP_*), MultiBand light
  * (MultiBandPhotosyntheticallyActiveRadiation, Light/multi_band.jl: per band
  * kʷ, e, χ and the share of the surface PAR) and the carbonate solver of the
@@ -495,6 +511,8 @@ typedef struct obme_pisces_config {
     obme_carbon_solver solver;
     obme_negative_scaling scaling; /* PISCES(; scale_negatives = true): the five groups of
                                       PISCES/coupling_utils.jl:11-35 */
+    obme_co2_exchange co2;         /* top DIC flux (CarbonDioxideGasExchangeBoundaryCondition, no Si/PO4) */
+    obme_o2_exchange o2;           /* top O2 flux */
 } obme_pisces_config;
 
 typedef struct obme_pisces obme_pisces;
@@ -539,7 +557,7 @@ obme_status obme_kernel_timing(obme_handle* h, int32_t which, double* total_ms, 
  * (0 obme_config, 1 obme_grid, 2 obme_npd_params, 3 obme_two_band_par,
  * 4 obme_sinking, 5 obme_carbon_solver, 6 obme_co2_exchange,
  * 7 obme_field_source, 8 obme_batch_params, 9 obme_pisces_config,
- * 10 obme_negative_scaling); -1 for an
+ * 10 obme_negative_scaling, 11 obme_o2_exchange); -1 for an
  * unknown index.  No reference
  * counterpart: it lets a binding check its struct layouts. */
 int64_t obme_abi_sizeof(int32_t which);

@@ -1230,6 +1230,7 @@ extern "C" int64_t obme_abi_sizeof(int32_t which)
     case 8: return sizeof(obme_batch_params);
     case 9: return sizeof(obme_pisces_config);
     case 10: return sizeof(obme_negative_scaling);
+    case 11: return sizeof(obme_o2_exchange);
     default: return -1;
     }
 }

@@ -832,4 +832,48 @@ __device__ __forceinline__ double co2_flux(const obme_co2_exchange& g, const obm
     return surface_flux(sc, s, DIC, Alk, pCO2_air, mode, pH, evals);
 }
 
+// Flux with the mixed path's warm start: H from the previous stage (or an
+// FP32 guess) refined by FP64 Newton, with the reference solve as fallback.
+__device__ __forceinline__ double surface_flux_warm(const SurfaceCarbon& sc, const obme_carbon_solver& s,
+                                                    double DIC, double Alk, double pCO2_air, int mode,
+                                                    double& H, double& pH, int& evals)
+{
+    if (mode == 0) return surface_flux<true>(sc, s, DIC, Alk, pCO2_air, 0, pH, evals);
+    CarbonParams p = sc.p;
+    p.Alk = Alk * sc.conv;
+    p.DIC = DIC * sc.conv;
+    double x = (H > 0.0) ? H : (double)guess_H_fp32(p);
+    bool done = refine_H(p, x, evals);
+    if (!done) {
+        x = (double)guess_H_fp32(p);
+        done = refine_H(p, x, evals);
+        if (!done) x = solve_for_H(p, s, pow(10.0, -s.initial_pH_guess), evals);
+    }
+    H = x;
+    pH = -log10(x);
+    return sc.k * (pco2_from_H(p, x) - pCO2_air);
+}
+
+// OxygenGasExchangeBoundaryCondition (GasExchange/GasExchange.jl:127-146):
+// F = k (O2 - C_air α), k = k660(u10) / sqrt(Sc(T) / 660) · 1
+// (gas_transfer_velocity.jl:33-34), Sc the quartic of schmidt_number.jl:15-16
+// (generic_parameterisations.jl:31-32), α = β / Tk with
+// β = exp(A1 + A2/(Tk/100) + A3 log(Tk/100) + S (B1 + B2 Tk/100 + B2 (Tk/100)²))
+// (gas_solubility.jl:37-48: B2 in both salinity terms, as there).
+__device__ __forceinline__ double o2_flux(const obme_o2_exchange& g, double O2, double T, double S)
+{
+    const double* c = g.schmidt;
+    const double Sc = c[0] + c[1] * T + c[2] * (T * T) + c[3] * (T * T * T) + c[4] * pow4_compensated(T);
+    const double* v = g.transfer_velocity;
+    const double u = g.wind_speed;
+    const double k660 = v[0] + v[1] * u + v[2] * (u * u);
+    const double k = k660 / sqrt(Sc / 660.0) * 1.0;
+    const double Tk = T + 273.15;
+    const double Tk100 = Tk / 100.0;
+    const double* a = g.solubility;
+    const double beta = exp(a[0] + a[1] / Tk100 + a[2] * log(Tk100) + S * (a[3] + a[4] * Tk100 + a[4] * (Tk100 * Tk100)));
+    const double air = g.air_concentration * (beta / Tk);
+    return k * (O2 - air);
+}
+
 }  // namespace obme

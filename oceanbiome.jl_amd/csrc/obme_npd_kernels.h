@@ -177,28 +177,6 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)
     return x * q + (x < rem ? x : rem) + s;
 }
 
-// Flux with the mixed path's warm start: H from the previous stage (or an
-// FP32 guess) refined by FP64 Newton, with the reference solve as fallback.
-__device__ __forceinline__ double surface_flux_warm(const SurfaceCarbon& sc, const obme_carbon_solver& s,
-                                                    double DIC, double Alk, double pCO2_air, int mode,
-                                                    double& H, double& pH, int& evals)
-{
-    if (mode == 0) return surface_flux<true>(sc, s, DIC, Alk, pCO2_air, 0, pH, evals);
-    CarbonParams p = sc.p;
-    p.Alk = Alk * sc.conv;
-    p.DIC = DIC * sc.conv;
-    double x = (H > 0.0) ? H : (double)guess_H_fp32(p);
-    bool done = refine_H(p, x, evals);
-    if (!done) {
-        x = (double)guess_H_fp32(p);
-        done = refine_H(p, x, evals);
-        if (!done) x = solve_for_H(p, s, pow(10.0, -s.initial_pH_guess), evals);
-    }
-    H = x;
-    pH = -log10(x);
-    return sc.k * (pco2_from_H(p, x) - pCO2_air);
-}
-
 // s_waitcnt with only the vector-memory count constrained (gfx9 encoding:
 // vmcnt bits [3:0] and [15:14], expcnt [6:4] = 7, lgkmcnt [11:8] = 15).
 template <int N>

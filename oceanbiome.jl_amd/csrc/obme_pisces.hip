@@ -41,6 +41,10 @@ struct PiscesArgs {
     const double* oc;       // [OMEGA_CONST_FIELDS][ncell]: per-cell carbonate constants
     double* Hs;             // [ncell]: [H+] of the last stage (FP32-mixed warm start)
     int have_H;             // Hs holds the previous stage's [H+]
+    double* sflux;          // [2][Nx*Ny]: top CO2 and O2 fluxes of the stage (gas exchange on)
+    double* Hsurf;          // [Nx*Ny]: [H+] of the surface pCO2 solve (FP32-mixed warm start)
+    obme_co2_exchange co2;
+    obme_o2_exchange o2;
     FT* zeu;                // [Nx*Ny]
     FT* mlPAR;              // [Nx*Ny]
     const FT* pp;           // parameters (OBME_PP_*)
@@ -166,6 +170,36 @@ __global__ void __launch_bounds__(256) k_pisces_scale(PiscesArgs<FT> a, NegScale
         if (members & (1u << n)) const_cast<FT*>(a.in[n])[p] = x[n];
 }
 
+// Thread per column: the top-cell CO2 and O2 gas exchange fluxes of the
+// stage input (GasExchange, gas_exchange.jl:26-38: F = k (c_water - c_air),
+// positive out of the ocean), for the cell kernel to take out of the top
+// cell's tendencies.  CO2: CarbonDioxideGasExchangeBoundaryCondition with no
+// silicate or phosphate (validation/PISCES/column.jl:63), the surface solve
+// of the LOBSTER path; O2: OxygenGasExchangeBoundaryCondition (:64).
+template <typename FT>
+__global__ void __launch_bounds__(256) k_pisces_surface(PiscesArgs<FT> a)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncol = a.Nx * a.Ny;
+    if (col >= ncol) return;
+    const unsigned p = pidx(a, col % a.Nx, col / a.Nx, a.Nz - 1);
+    const double T = (double)a.in[OBME_PI_T][p], S = (double)a.in[OBME_PI_S][p];
+    double Fc = 0.0, Fo = 0.0;
+    if (a.co2.enabled) {
+        SurfaceCarbon sc;
+        surface_carbon_setup(a.co2, T, S, a.co2.wind_speed, sc);
+        double H = (sizeof(FT) == 4) ? a.Hsurf[col] : 0.0;
+        double pH = 0.0;
+        int ev = 0;
+        Fc = surface_flux_warm(sc, a.solver, (double)a.in[OBME_PI_DIC][p], (double)a.in[OBME_PI_ALK][p],
+                               a.co2.air_concentration, sizeof(FT) == 4 ? 1 : 0, H, pH, ev);
+        if (sizeof(FT) == 4) a.Hsurf[col] = H;
+    }
+    if (a.o2.enabled) Fo = o2_flux(a.o2, (double)a.in[OBME_PI_O2][p], T, S);
+    a.sflux[col] = Fc;
+    a.sflux[ncol + col] = Fo;
+}
+
 // Thread per cell: the T-, S- and pressure-dependent constants of the
 // calcite saturation (the first half of calcite_saturation).
 template <typename FT>
@@ -274,6 +308,11 @@ __global__ void __launch_bounds__(256) k_pisces_cell(PiscesArgs<FT> a)
     sink(OBME_PI_BFE, wG);
     sink(OBME_PI_PSI, wG);
     sink(OBME_PI_CACO3, wG);
+    if (k == Nz - 1 && a.sflux) {
+        // the top boundary fluxes of DIC and O2
+        G[OBME_PI_DIC] = G[OBME_PI_DIC] - (FT)a.sflux[col] * idz;
+        G[OBME_PI_O2] = G[OBME_PI_O2] - (FT)a.sflux[ncol + col] * idz;
+    }
     if constexpr (MODE == 1) {
 #pragma unroll
         for (int n = 0; n < NW; n++) a.out[n][p] = a.out[n][p] + G[n];
@@ -314,6 +353,8 @@ struct obme_pisces {
     void* d_Gm = nullptr;
     void* d_oc = nullptr;
     void* d_H = nullptr;
+    void* d_sflux = nullptr;
+    void* d_Hsurf = nullptr;
     void* d_copy[NW] = {};
     size_t field_bytes = 0;
     std::vector<double> zc, zf, dz;
@@ -323,7 +364,8 @@ struct obme_pisces {
 extern "C" obme_status obme_pisces_destroy(obme_pisces* h)
 {
     if (!h) return OBME_OK;
-    for (void* p : {h->d_pp, h->d_lev, h->d_w, h->d_PAR, h->d_Omega, h->d_zeu, h->d_mlPAR, h->d_Gm, h->d_oc, h->d_H})
+    for (void* p : {h->d_pp, h->d_lev, h->d_w, h->d_PAR, h->d_Omega, h->d_zeu, h->d_mlPAR, h->d_Gm, h->d_oc, h->d_H,
+                    h->d_sflux, h->d_Hsurf})
         if (p) (void)hipFree(p);
     for (void* p : h->d_copy)
         if (p) (void)hipFree(p);
@@ -419,6 +461,12 @@ extern "C" obme_status obme_pisces_create(const obme_pisces_config* cfg, obme_pi
     alloc(&h->d_zeu, h->ncol * es);
     alloc(&h->d_mlPAR, h->ncol * es);
     alloc(&h->d_Gm, (size_t)NW * h->ncell * es);
+    if (cfg->co2.enabled || cfg->o2.enabled) {
+        alloc(&h->d_sflux, 2 * (size_t)h->ncol * sizeof(double));
+        alloc(&h->d_Hsurf, (size_t)h->ncol * sizeof(double));
+        if (st == OBME_OK && hipMemset(h->d_Hsurf, 0, (size_t)h->ncol * sizeof(double)) != hipSuccess)
+            st = obme_fail(OBME_ERR_HIP, "hipMemset");
+    }
     if (st != OBME_OK) {
         obme_pisces_destroy(h);
         return st;
@@ -441,6 +489,10 @@ static PiscesArgs<FT> pisces_args(const obme_pisces* h, double t)
     a.mlPAR = (FT*)h->d_mlPAR;
     a.oc = (const double*)h->d_oc;
     a.Hs = (double*)h->d_H;
+    a.sflux = (double*)h->d_sflux;
+    a.Hsurf = (double*)h->d_Hsurf;
+    a.co2 = c.co2;
+    a.o2 = c.o2;
     a.pp = (const FT*)h->d_pp;
     a.lev = (const FT*)h->d_lev;
     a.w = (const FT*)h->d_w;
@@ -512,6 +564,10 @@ static obme_status pisces_stage(obme_pisces* h, const void* const* in, void* con
     const int bs = 256;
     hipLaunchKernelGGL(k_pisces_columns<FT>, dim3((unsigned)((h->ncol + bs - 1) / bs)), dim3(bs), 0, s, a);
     OBME_HIP_TRY(hipGetLastError());
+    if (a.sflux) {
+        hipLaunchKernelGGL(k_pisces_surface<FT>, dim3((unsigned)((h->ncol + bs - 1) / bs)), dim3(bs), 0, s, a);
+        OBME_HIP_TRY(hipGetLastError());
+    }
     const unsigned nb = (unsigned)((h->ncell + bs - 1) / bs);
     if (mode == 0) {
         hipLaunchKernelGGL((k_pisces_cell<FT, 0, 1>), dim3(nb), dim3(bs), 0, s, a);

@@ -188,6 +188,19 @@ class BatchParams(C.Structure):
 OBME_PP_COUNT = 159   # obme_pisces_config.params length (2*31 + 2*18 + 61)
 
 
+class O2Exchange(C.Structure):
+    """obme_o2_exchange."""
+    _fields_ = [
+        ("enabled", C.c_int32),
+        ("pad0", C.c_int32),
+        ("air_concentration", C.c_double),
+        ("wind_speed", C.c_double),
+        ("schmidt", C.c_double * 5),
+        ("transfer_velocity", C.c_double * 3),
+        ("solubility", C.c_double * 6),
+    ]
+
+
 class PiscesConfig(C.Structure):
     """obme_pisces_config."""
     _fields_ = [
@@ -202,6 +215,8 @@ class PiscesConfig(C.Structure):
         ("surface_PAR", C.c_double),
         ("solver", CarbonSolver),
         ("scaling", NegativeScaling),
+        ("co2", CO2Exchange),
+        ("o2", O2Exchange),
     ]
 
 
@@ -228,4 +243,5 @@ ABI_STRUCTS = (
     ("obme_batch_params", BatchParams),
     ("obme_pisces_config", PiscesConfig),
     ("obme_negative_scaling", NegativeScaling),
+    ("obme_o2_exchange", O2Exchange),
 )

@@ -161,6 +161,30 @@ class CarbonDioxideGasExchangeBoundaryCondition:
 
 
 @dataclass
+class OxygenGasExchangeBoundaryCondition:
+    """OxygenGasExchangeBoundaryCondition (GasExchange.jl:127-146): Ho06 k660
+    scaled by the Wanninkhof (2014) O2 Schmidt number (schmidt_number.jl:15-16);
+    air side 9352.7 mmol/m3 times the Wanninkhof (1992) solubility β/Tk
+    (gas_solubility.jl:28-54)."""
+    air_concentration: float = 9352.7
+    wind_speed: float = 2.0
+    schmidt: tuple = (1920.4, -135.6, 5.2122, -0.10939, 0.00093777)
+    transfer_velocity: tuple = (0.0, 0.0, 0.266 / hour / 100)
+    solubility: tuple = (-58.3877, 85.8079, 23.8439, -0.034892, 0.015578, -0.0019387)
+
+    def struct(self):
+        s = abi.O2Exchange(enabled=1, air_concentration=float(self.air_concentration),
+                           wind_speed=float(self.wind_speed))
+        for i, v in enumerate(self.schmidt):
+            s.schmidt[i] = float(v)
+        for i, v in enumerate(self.transfer_velocity):
+            s.transfer_velocity[i] = float(v)
+        for i, v in enumerate(self.solubility):
+            s.solubility[i] = float(v)
+        return s
+
+
+@dataclass
 class TwoBandPhotosyntheticallyActiveRadiation:
     """TwoBandPhotosyntheticallyActiveRadiation (Light/2band.jl:106-146).
     surface_PAR: a number (constant), "default" (default_surface_PAR,

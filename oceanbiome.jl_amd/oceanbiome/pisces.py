@@ -119,8 +119,15 @@ class PISCES:
 
     def __init__(self, grid, surface_PAR=100.0, mixed_layer_depth=-50.0, kappa=1.0, latitude=45.0,
                  bands=((400, 500), (500, 600), (600, 700)), surface_PAR_division=None, POC_speed=2 / day,
-                 open_bottom=True, scale_negatives=False, invalid_fill_value=float("nan"), **overrides):
+                 open_bottom=True, scale_negatives=False, invalid_fill_value=float("nan"),
+                 co2_exchange=None, o2_exchange=None, **overrides):
+        """co2_exchange / o2_exchange: the top boundary conditions of DIC and O2
+        (a CarbonDioxideGasExchangeBoundaryCondition / OxygenGasExchangeBoundaryCondition,
+        or True for the defaults; validation/PISCES/column.jl:63-64)."""
+        from .models import CarbonDioxideGasExchangeBoundaryCondition, OxygenGasExchangeBoundaryCondition
         self.grid = grid
+        self.co2_exchange = CarbonDioxideGasExchangeBoundaryCondition() if co2_exchange is True else co2_exchange
+        self.o2_exchange = OxygenGasExchangeBoundaryCondition() if o2_exchange is True else o2_exchange
         self.scale_negatives = bool(scale_negatives)
         self.invalid_fill_value = float(invalid_fill_value)
         self.surface_PAR = float(surface_PAR)
@@ -221,6 +228,10 @@ class PISCES:
         cc = self.carbon_chemistry
         cfg.solver = cc.solver.struct(cc.initial_pH_guess)
         cfg.scaling = self.scaling_struct()
+        if self.co2_exchange is not None:
+            cfg.co2 = self.co2_exchange.struct()
+        if self.o2_exchange is not None:
+            cfg.o2 = self.o2_exchange.struct()
         return cfg
 
     def __repr__(self):

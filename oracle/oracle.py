@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Only tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
 """
 import ctypes as C
+import math
 import os
 import subprocess
 import sys
@@ -72,6 +73,7 @@ def lib():
                                       DP, D, DP, DP, C.c_long, I]),
         "orc_npd_tendencies": (None, [I, I, C.POINTER(abi.NPDParams), DP, D, DP]),
         "orc_npd_ntracers": (I, [I, I]),
+        "orc_pow_int4": (D, [D]),
         "orc_two_band_column": (None, [C.POINTER(abi.TwoBandPAR), I, DP, DP, DP, D, DP]),
         "orc_default_surface_PAR": (D, [D]),
         "orc_sinking_profile": (None, [C.POINTER(abi.Sinking), D, I, DP]),
@@ -207,6 +209,34 @@ def co2_flux(gas, solver, DIC, Alk, T, S, u10, pCO2_air, fast_exit=1):
     F = lib().orc_co2_flux(C.byref(gas), C.byref(solver), DIC, Alk, T, S, u10, pCO2_air, fast_exit,
                            C.byref(pH), C.byref(ev))
     return F, pH.value, ev.value
+
+
+def o2_schmidt(o2, T):
+    """OxygenPolynomialSchmidtNumber (GasExchange/schmidt_number.jl:15-16) as
+    the quartic PolynomialParameterisation{4} (generic_parameterisations.jl:31-32)."""
+    c = o2.schmidt
+    return c[0] + c[1] * T + c[2] * (T * T) + c[3] * (T * T * T) + c[4] * lib().orc_pow_int4(T)
+
+
+def o2_air_concentration(o2, T, S):
+    """PartiallySolubleGas surface value (gas_solubility.jl:23-24): the air
+    concentration times the Wanninkhof (1992) solubility β/Tk (:37-48; the
+    salinity polynomial uses B2 twice, as written there)."""
+    a = o2.solubility
+    Tk = T + 273.15
+    Tk100 = Tk / 100
+    beta = math.exp(a[0] + a[1] / Tk100 + a[2] * math.log(Tk100) + S * (a[3] + a[4] * Tk100 + a[4] * Tk100 ** 2))
+    return o2.air_concentration * (beta / Tk)
+
+
+def o2_flux(o2, O2, T, S):
+    """GasExchange (gas_exchange.jl:26-38) for O2: k (O2 - C_air α), k =
+    k660(u) / sqrt(Sc(T) / 660) (gas_transfer_velocity.jl:33-34)."""
+    v = o2.transfer_velocity
+    u = o2.wind_speed
+    k660 = v[0] + v[1] * u + v[2] * (u * u)
+    k = k660 / math.sqrt(o2_schmidt(o2, T) / 660.0) * 1.0
+    return k * (O2 - o2_air_concentration(o2, T, S))
 
 
 def calcite_saturation(solver, DIC, Alk, T, S, P=None, silicate=0.0):

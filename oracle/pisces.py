@@ -662,9 +662,30 @@ def face_velocities(q, zc, open_bottom=True):
 SINKING = (("POC", 0), ("SFe", 0), ("GOC", 1), ("BFe", 1), ("PSi", 1), ("CaCO3", 1))
 
 
-def full_tendencies(q, light, F, zc, zf, dz, t, solver, open_bottom=True, nthreads=8):
+def surface_fluxes(F, gas, solver):
+    """Top-cell gas exchange fluxes (GasExchange, gas_exchange.jl:26-38) per
+    column: CO2 (CarbonDioxideGasExchangeBoundaryCondition, no silicate or
+    phosphate) and O2 (OxygenGasExchangeBoundaryCondition); 0 where off."""
+    import oracle as O
+    top = lambda name: F[name][-1]
+    shp = top("DIC").shape
+    Fc = np.zeros(shp)
+    Fo = np.zeros(shp)
+    co2, o2 = gas.get("co2"), gas.get("o2")
+    for idx in np.ndindex(shp):
+        T, S = float(top("T")[idx]), float(top("S")[idx])
+        if co2 is not None and co2.enabled:
+            Fc[idx] = O.co2_flux(co2, solver, float(top("DIC")[idx]), float(top("Alk")[idx]), T, S,
+                                 co2.wind_speed, co2.air_concentration)[0]
+        if o2 is not None and o2.enabled:
+            Fo[idx] = O.o2_flux(o2, float(top("O2")[idx]), T, S)
+    return Fc, Fo
+
+
+def full_tendencies(q, light, F, zc, zf, dz, t, solver, open_bottom=True, nthreads=8, gas=None):
     """Transitions minus the upwind sinking divergence at every cell, with
-    the update state at time t."""
+    the update state at time t; `gas` ({"co2": ..., "o2": ...}) adds the top
+    boundary fluxes of DIC and O2 (−F/Δz in the top cell)."""
     A = update_state(q, light, F, zc, zf, t, solver, nthreads)
     wP, wG = face_velocities(q, zc, open_bottom)
     A["wPOCc"] = ((wP[:-1] + wP[1:]) / 2)[:, None, None]
@@ -677,6 +698,10 @@ def full_tendencies(q, light, F, zc, zf, dz, t, solver, open_bottom=True, nthrea
         cu[:-1] = c[1:]
         div = (w[1:, None, None] * cu - w[:-1, None, None] * c) / dz[:, None, None]
         G[name] = G[name] - div
+    if gas:
+        Fc, Fo = surface_fluxes(F, gas, solver)
+        G["DIC"][-1] = G["DIC"][-1] - Fc / dz[-1]
+        G["O2"][-1] = G["O2"][-1] - Fo / dz[-1]
     return G, A
 
 
@@ -715,7 +740,7 @@ def scale_negatives(F, groups):
     return F
 
 
-def step(q, light, F, zc, zf, dz, nsteps, dt, t0, solver, open_bottom=True, nthreads=8, scaling=None):
+def step(q, light, F, zc, zf, dz, nsteps, dt, t0, solver, open_bottom=True, nthreads=8, scaling=None, gas=None):
     """nsteps RK3 steps (Oceananigans γ, ζ; the substep form of
     BoxModel/timesteppers.jl:79-92) of the 24 written tracers, in place;
     `scaling` (conserved_groups) runs at the start of every stage."""
@@ -728,7 +753,7 @@ def step(q, light, F, zc, zf, dz, nsteps, dt, t0, solver, open_bottom=True, nthr
             t = t0 + s * dt + cst[st] * dt
             if scaling:
                 scale_negatives(F, scaling)
-            G, _ = full_tendencies(q, light, F, zc, zf, dz, t, solver, open_bottom, nthreads)
+            G, _ = full_tendencies(q, light, F, zc, zf, dz, t, solver, open_bottom, nthreads, gas)
             for name in WRITTEN:
                 if st == 0:
                     F[name] = F[name] + (dt * g[0]) * G[name]

@@ -134,3 +134,49 @@ def test_scaling_zeroed_phytoplankton_gives_the_same_nans():
         assert np.array_equal(np.isnan(a), np.isnan(b)), name
         ok = ~np.isnan(b)
         assert rel_l2(a[ok], b[ok]) <= 1e-12, name
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_step_with_gas_exchange(precision):
+    """PISCES with the CO2 and O2 top boundary fluxes of
+    validation/PISCES/column.jl:63-64 (DIC and O2 lose F/Δz in the top cell)."""
+    import torch
+    from oceanbiome import pisces as HP
+    print("case pisces gas exchange", precision, flush=True)
+    wl = workloads.pisces_3d(8, 4, 20, nsteps=3, dt=300.0, seed=13)
+    wl.bgc = HP.PISCES(wl.grid, co2_exchange=True, o2_exchange=True)
+    q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
+    cfg = wl.bgc.config(0)
+    gas = {"co2": cfg.co2, "o2": cfg.o2}
+    F0 = {k: v.copy() for k, v in F.items()}
+    OP.step(q, light, F, zc, zf, dz, wl.nsteps, wl.dt, 0.0, solver, gas=gas)
+    dev, tr = _device(wl, precision)
+    dev.step(tr, wl.nsteps, wl.dt)
+    torch.cuda.synchronize()
+    g = wl.grid
+    errs = {name: rel_l2(g.interior(tr[n].double().cpu().numpy()), F[name]) for n, name in enumerate(OP.WRITTEN)}
+    print("errors", errs, flush=True)
+    tol = 1e-12 if precision == "fp64" else 1e-5
+    assert all(e <= tol for e in errs.values()), errs
+    # the fluxes act: the top-cell O2 moves towards saturation differently from a run without them
+    F1 = {k: v.copy() for k, v in F0.items()}
+    OP.step(q, light, F1, zc, zf, dz, wl.nsteps, wl.dt, 0.0, solver)
+    assert not np.allclose(F1["O2"][-1], F["O2"][-1], rtol=1e-9)
+
+
+def test_tendencies_with_gas_exchange_fp64():
+    import torch
+    from oceanbiome import pisces as HP
+    print("case pisces gas exchange tendencies", flush=True)
+    wl = workloads.pisces_3d(10, 4, 24, nsteps=1, seed=5)
+    wl.bgc = HP.PISCES(wl.grid, co2_exchange=True, o2_exchange=True)
+    q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
+    cfg = wl.bgc.config(0)
+    G, _ = OP.full_tendencies(q, light, F, zc, zf, dz, 1.0e5, solver, gas={"co2": cfg.co2, "o2": cfg.o2})
+    dev, tr = _device(wl, "fp64")
+    g = wl.grid
+    Gd = [torch.zeros(g.parent_shape, dtype=torch.float64, device="cuda") for _ in range(24)]
+    dev.update_tendencies(tr, Gd, t=1.0e5)
+    torch.cuda.synchronize()
+    for n, name in enumerate(OP.WRITTEN):
+        assert rel_l2(g.interior(Gd[n].cpu().numpy()), G[name]) <= 1e-10, name

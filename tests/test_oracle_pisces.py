@@ -212,3 +212,23 @@ def test_pisces_scaling_zeroes_negatives_and_keeps_the_last_group_total():
     for g_names, _, _ in groups:
         for k in g_names:
             assert np.all((F[k] >= 0) | np.isnan(F[k])), k
+
+
+def test_oxygen_gas_exchange_known_answers():
+    """test_gasexchange_carbon_chem.jl:157-185: the O2 Schmidt number at
+    20 °C is 568 ± 1 and the air-side O2 at 25 °C, S = 35 is 200 ± 50; the
+    flux is k (O2 - C_air α) with k from the CO2 path's Ho06 k660."""
+    from oceanbiome import OxygenGasExchangeBoundaryCondition, CarbonDioxideGasExchangeBoundaryCondition
+    o2 = OxygenGasExchangeBoundaryCondition().struct()
+    assert abs(O.o2_schmidt(o2, 20.0) - 568) <= 1
+    air = O.o2_air_concentration(o2, 25.0, 35.0)
+    assert abs(air - 200) <= 50
+    # zero flux at saturation, linear in the water concentration
+    assert O.o2_flux(o2, air, 25.0, 35.0) == pytest.approx(0.0, abs=1e-18)
+    f1, f2 = O.o2_flux(o2, air + 10, 25.0, 35.0), O.o2_flux(o2, air + 20, 25.0, 35.0)
+    assert f2 == pytest.approx(2 * f1, rel=1e-12)
+    # k equals the CO2 path's k660/√(Sc/660) with the O2 Schmidt number
+    co2 = CarbonDioxideGasExchangeBoundaryCondition().struct()
+    k660 = 0.266 / 3600 / 100 * 2.0 ** 2
+    assert f1 / 10 == pytest.approx(k660 / math.sqrt(O.o2_schmidt(o2, 25.0) / 660), rel=1e-12)
+    assert co2.transfer_velocity[2] == pytest.approx(0.266 / 3600 / 100, rel=1e-15)
