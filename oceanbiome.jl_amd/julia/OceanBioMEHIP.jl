@@ -231,6 +231,17 @@ end
 
 const OBME_PP_COUNT = 159
 
+# OxygenGasExchangeBoundaryCondition (GasExchange.jl:127-146)
+struct ObmeO2Exchange
+    enabled :: Int32
+    pad0 :: Int32
+    air_concentration :: Float64
+    wind_speed :: Float64
+    schmidt :: NTuple{5, Float64}
+    transfer_velocity :: NTuple{3, Float64}
+    solubility :: NTuple{6, Float64}
+end
+
 struct ObmePiscesConfig
     precision :: Int32
     open_bottom :: Int32
@@ -243,6 +254,8 @@ struct ObmePiscesConfig
     surface_PAR :: Float64
     solver :: ObmeCarbonSolver
     scaling :: ObmeNegativeScaling
+    co2 :: ObmeCO2Exchange           # top DIC flux (validation/PISCES/column.jl:63)
+    o2 :: ObmeO2Exchange             # top O₂ flux (:64)
 end
 
 """
@@ -263,6 +276,8 @@ end
 function HIPPISCES(underlying, config::ObmePiscesConfig, aux)
     n = ccall((:obme_abi_sizeof, libobme), Int64, (Int32,), 9)
     n == sizeof(ObmePiscesConfig) || error("ABI mismatch for ObmePiscesConfig")
+    n = ccall((:obme_abi_sizeof, libobme), Int64, (Int32,), 11)
+    n == sizeof(ObmeO2Exchange) || error("ABI mismatch for ObmeO2Exchange")
     h = Ref{Ptr{Cvoid}}(C_NULL)
     cfg = Ref(config)
     check(ccall((:obme_pisces_create, libobme), Int32, (Ref{ObmePiscesConfig}, Ref{Ptr{Cvoid}}), cfg, h))
