@@ -233,8 +233,11 @@ __global__ void __launch_bounds__(256) k_pisces_prep(PiscesArgs<FT> a)
 // FP32-mixed: FP64 Newton warm-started from the previous stage's [H+], or
 // from an FP32 first guess, falling back to the reference solve);
 // OM = 0: it is read from the Omega array.
+#ifndef OBME_PISCES_F32_WAVES
+#define OBME_PISCES_F32_WAVES 4
+#endif
 template <typename FT, int MODE, int OM>
-__global__ void __launch_bounds__(256) k_pisces_cell(PiscesArgs<FT> a)
+__global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES : 2) k_pisces_cell(PiscesArgs<FT> a)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long ncol = (long)a.Nx * a.Ny;
