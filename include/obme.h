@@ -193,6 +193,31 @@ typedef struct obme_negative_scaling {
     double invalid_fill_value[OBME_MAX_SCALE_GROUPS];
 } obme_negative_scaling;
 
+/* Sediment models (src/Models/Sediments). */
+enum { OBME_SEDIMENT_NONE = 0, OBME_SEDIMENT_INSTANT_REMINERALISATION = 1 };
+enum { OBME_SEDIMENT_AB2 = 0 };   /* QuasiAdamsBashforth2, the sediment's default time stepper */
+
+/* InstantRemineralisationSediment (Models/Sediments/instant_remineralisation.jl:60-123)
+ * coupled through BiogeochemicalSediment (src/Sediments): at the bottom face
+ * the sinking flux of the listed tracers, f = Σ -w c (tracked_fields.jl:53-70,
+ * first-order upwind: c of the bottom cell), burial efficiency
+ * e = a + b (f / (k + f))^2; the receiver's bottom cell gains (1 - e) f / Δz
+ * (tracer_coupling.jl:3-38, Δz at the bottom face = the bottom cell's
+ * thickness) and the 2-D storage field gains e f through the sediment's own
+ * time stepper (update_state.jl:6-17, timesteppers.jl:16-43). */
+#define OBME_MAX_SEDIMENT_TRACERS 6
+typedef struct obme_sediment {
+    int32_t model;                 /* OBME_SEDIMENT_*                                   */
+    int32_t receiver;              /* tracer index of remineralisation_reciever (N, NH4) */
+    int32_t n_sinking;
+    int32_t sinking[OBME_MAX_SEDIMENT_TRACERS];   /* tracer indices of sinking_tracers  */
+    int32_t timestepper;           /* OBME_SEDIMENT_AB2                                 */
+    double burial_efficiency_constant1;           /* a, 0.013                           */
+    double burial_efficiency_constant2;           /* b, 0.53                            */
+    double burial_efficiency_half_saturation;     /* k, 7 / 6.56                        */
+    double ab2_chi;                /* χ of QuasiAdamsBashforth2, 0.1                     */
+} obme_sediment;
+
 typedef struct obme_config {
     int32_t model;                 /* OBME_NPZD | OBME_LOBSTER                */
     int32_t precision;             /* OBME_FP64 | OBME_FP32_MIXED             */
@@ -207,6 +232,7 @@ typedef struct obme_config {
     obme_field_source temperature; /* LOBSTER gas exchange; NPZD reads T as a tracer */
     obme_field_source salinity;
     obme_negative_scaling scaling; /* NPZD(grid; scale_negatives = true) etc. */
+    obme_sediment sediment;        /* NPZD(; grid, sediment_model) etc.          */
 } obme_config;
 
 typedef struct obme_handle obme_handle;
@@ -256,6 +282,34 @@ obme_status obme_add_tendencies(obme_handle* h, const void* const* tracers,
 obme_status obme_step(obme_handle* h, void* const* tracers,
                       const void* T, const void* S,
                       int32_t nsteps, double dt, double t0, void* stream);
+
+/* ---- sediment (BiogeochemicalSediment, src/Sediments) -------------------
+ * The sediment's state is its storage field (device, Nx*Ny doubles, i
+ * fastest; the caller's) and its time stepper's tendencies, which the handle
+ * keeps.  With a sediment configured, obme_add_tendencies also adds the
+ * coupled tendency of the receiver (update_tendencies!(bgc, sediment, model),
+ * tracer_coupling.jl:3-38) from the sinking flux of the tracers it is given,
+ * and obme_step applies it at every stage (storage not advanced). */
+
+/* update_biogeochemical_state!(model, sediment) (Sediments/update_state.jl:6-17):
+ * the tracked sinking flux f of the given tracers (tracked_fields.jl:8-70)
+ * and, for a finite dt (the model's last stage Δt), one sediment time step:
+ * QuasiAdamsBashforth2 (timesteppers.jl:16-43 in Oceananigans' AB2
+ * time_step!): a forward Euler step when dt differs from the previous step's
+ * (always, between RK3 stages), else χ-weighted with the previous tendency;
+ * the tendency e f is evaluated after each step from the tracked flux, and
+ * before the first step (iteration 0) from the flux of that call. */
+obme_status obme_sediment_update_state(obme_handle* h, const void* const* tracers, double* storage,
+                                       double dt, void* stream);
+
+/* obme_step with the sediment's storage advanced in the same pass: at every
+ * RK3 stage the coupled tendency of the stage's flux, and after every stage
+ * the sediment time step of obme_sediment_update_state with the stage Δt
+ * (Δt γ1, Δt (γ2 + ζ2), Δt (γ3 + ζ3)), as the reference's
+ * update_biogeochemical_state! does after each substep.  storage may be
+ * NULL (the coupling alone). */
+obme_status obme_step_with_sediment(obme_handle* h, void* const* tracers, const void* T, const void* S,
+                                    double* storage, int32_t nsteps, double dt, double t0, void* stream);
 
 /* ---- carbonate chemistry / gas exchange on point sets (SoA, n points) --- */
 
@@ -557,7 +611,7 @@ obme_status obme_kernel_timing(obme_handle* h, int32_t which, double* total_ms, 
  * (0 obme_config, 1 obme_grid, 2 obme_npd_params, 3 obme_two_band_par,
  * 4 obme_sinking, 5 obme_carbon_solver, 6 obme_co2_exchange,
  * 7 obme_field_source, 8 obme_batch_params, 9 obme_pisces_config,
- * 10 obme_negative_scaling, 11 obme_o2_exchange); -1 for an
+ * 10 obme_negative_scaling, 11 obme_o2_exchange, 12 obme_sediment); -1 for an
  * unknown index.  No reference
  * counterpart: it lets a binding check its struct layouts. */
 int64_t obme_abi_sizeof(int32_t which);

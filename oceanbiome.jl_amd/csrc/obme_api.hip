@@ -85,6 +85,52 @@ __global__ void __launch_bounds__(256) k_scale_grid(StepArgs<FT> a)
         if (members & (1u << n)) a.tr[n][p] = x[n];
 }
 
+// The InstantRemineralisation sediment at the bottom cell of column col:
+// the sinking flux f through the bottom face and the tendency e f
+// (instant_remineralisation.jl:101-123; tracked_fields.jl:53-70).
+template <typename FT>
+__device__ __forceinline__ void sediment_flux(const StepArgs<FT>& a, int s0, int s1, unsigned col, FT& f, FT& e)
+{
+    const unsigned p = a.base0 + col % (unsigned)a.Nx + a.sx * (col / (unsigned)a.Nx);
+    f = a.sed_w0 * a.tr[s0][p] + a.sed_w1 * a.tr[s1][p];
+    const FT qf = f / (a.sed_k + f);
+    e = a.sed_a + a.sed_b * (qf * qf);
+}
+
+// update_tendencies!(bgc, sediment, model) (Sediments/tracer_coupling.jl:3-38):
+// G[receiver] of the bottom cell += (1 - e) f / Δz, one thread per column.
+template <typename FT>
+__global__ void __launch_bounds__(256) k_sediment_couple(StepArgs<FT> a, int s0, int s1)
+{
+    const unsigned col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (unsigned)a.Nx * (unsigned)a.Ny) return;
+    FT f, e;
+    sediment_flux(a, s0, s1, col, f, e);
+    const unsigned p = a.base0 + col % (unsigned)a.Nx + a.sx * (col / (unsigned)a.Nx);
+    a.G[a.sed_recv][p] += ((FT(1) - e) * f) / a.sed_dz0;
+}
+
+// One time step of the sediment's storage (Sediments/update_state.jl:6-17;
+// QuasiAdamsBashforth2, timesteppers.jl:16-43): the tendency Gⁿ (at the
+// first step, e f of the tracked flux), a forward Euler step or the
+// χ-weighted step with G⁻, then G⁻ = Gⁿ and Gⁿ = e f.  One thread per column.
+template <typename FT>
+__global__ void __launch_bounds__(256) k_sediment_state(StepArgs<FT> a, int s0, int s1, double dt, int euler)
+{
+    const unsigned col = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned ncol = (unsigned)a.Nx * (unsigned)a.Ny;
+    if (col >= ncol) return;
+    FT f, e;
+    sediment_flux(a, s0, s1, col, f, e);
+    const double ge = (double)(e * f);
+    const double Gn = a.sed_init ? ge : a.sedG[col];
+    // Euler: (1.5 + χ) Gⁿ with χ = -1/2, and G⁻ multiplied by a strong zero
+    const double Gu = euler ? Gn : a.sed_cn * Gn - a.sed_cm * a.sedG[ncol + col];
+    if (a.storage) a.storage[col] = a.storage[col] + dt * Gu;
+    a.sedG[ncol + col] = Gn;
+    a.sedG[col] = ge;
+}
+
 // The model's default formulations (fm_code): LOBSTER Mondo light,
 // quadratic mortality and grazing; NPZD analytical light, linear
 // mortality, quadratic grazing, Q10 on.  The FP32 fused step on full
@@ -139,6 +185,10 @@ struct obme_handle {
     bool pipe;                 // pipelined persistent sweep where it applies (OBME_PIPE=1 turns it on)
     bool scaling;              // ScaleNegativeTracers groups configured
     bool co2_inline;           // scaling with the CO2 flux: the flux is computed in the sweep
+    bool sediment;             // InstantRemineralisation configured
+    double* d_sedG = nullptr;  // the sediment time stepper's Gⁿ, G⁻ (2 * Nx*Ny)
+    long long sed_iter = 0;    // sediment steps taken (its clock's iteration)
+    double sed_last_dt = HUGE_VAL;   // Δt of its last step (Inf before the first)
     double dz_u, zc0_u, w0_u, w1_u;
     // kernel timing: event pairs per kernel kind, and a pool of free events
     bool timing = false;
@@ -174,6 +224,28 @@ struct LaunchTimer {
         e0 = nullptr;
     }
 };
+
+// InstantRemineralisation: receiver a written tracer, sinking tracers valid
+// and distinct, the default time stepper.
+static obme_status validate_sediment(const obme_sediment& s, int ntracers, int readonly)
+{
+    if (s.model == OBME_SEDIMENT_NONE) return OBME_OK;
+    if (s.model != OBME_SEDIMENT_INSTANT_REMINERALISATION) return fail(OBME_ERR_INVALID, "unknown sediment model");
+    if (s.receiver < 0 || s.receiver >= ntracers || s.receiver == readonly)
+        return fail(OBME_ERR_INVALID, "sediment receiver is not a written tracer");
+    if (s.n_sinking < 1 || s.n_sinking > OBME_MAX_SEDIMENT_TRACERS)
+        return fail(OBME_ERR_INVALID, "bad number of sediment sinking tracers");
+    unsigned seen = 0;
+    for (int q = 0; q < s.n_sinking; q++) {
+        const int t = s.sinking[q];
+        if (t < 0 || t >= ntracers) return fail(OBME_ERR_INVALID, "sediment sinking tracer is not a tracer");
+        if (seen & (1u << t)) return fail(OBME_ERR_INVALID, "sediment sinking tracer listed twice");
+        seen |= 1u << t;
+    }
+    if (s.timestepper != OBME_SEDIMENT_AB2)
+        return fail(OBME_ERR_UNSUPPORTED, "only the QuasiAdamsBashforth2 sediment time stepper is supported");
+    return OBME_OK;
+}
 
 // ScaleNegativeTracers groups: counts within the limits, members valid
 // tracer indices, no member twice in a group.
@@ -318,6 +390,8 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
     {
         obme_status sv = validate_scaling(cfg->scaling, ntracers_of(*cfg));
         if (sv != OBME_OK) return sv;
+        sv = validate_sediment(cfg->sediment, ntracers_of(*cfg), cfg->model == OBME_NPZD ? 3 : -1);
+        if (sv != OBME_OK) return sv;
     }
     if (cfg->sinking.scheme != OBME_UPWIND1)
         return fail(OBME_ERR_UNSUPPORTED, "only the first-order upwind sinking scheme runs on the GPU");
@@ -400,6 +474,18 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
             return fail(OBME_ERR_HIP, "hipMemset");
         }
     }
+    h->sediment = cfg->sediment.model == OBME_SEDIMENT_INSTANT_REMINERALISATION;
+    if (h->sediment) {
+        size_t bytes = 2 * (size_t)g.Nx * g.Ny * sizeof(double);
+        if (hipMalloc((void**)&h->d_sedG, bytes) != hipSuccess) {
+            obme_destroy(h);
+            return fail(OBME_ERR_NOMEM, "sediment buffers");
+        }
+        if (hipMemset(h->d_sedG, 0, bytes) != hipSuccess) {
+            obme_destroy(h);
+            return fail(OBME_ERR_HIP, "hipMemset");
+        }
+    }
     *out = h;
     return OBME_OK;
 }
@@ -413,6 +499,7 @@ extern "C" obme_status obme_destroy(obme_handle* h)
     if (h->d_Sprof) (void)hipFree(h->d_Sprof);
     if (h->d_srec) (void)hipFree(h->d_srec);
     if (h->d_Hprev) (void)hipFree(h->d_Hprev);
+    if (h->d_sedG) (void)hipFree(h->d_sedG);
     for (auto& v : h->timed)
         for (auto& p : v) {
             (void)hipEventDestroy(p.first);
@@ -482,6 +569,27 @@ static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers)
         a.invdz_top = (FT)(1.0 / h->dz[g.Nz - 1]);
         a.co2 = c.co2;
         a.solver = c.solver;
+    }
+    if (h->sediment) {
+        // -w at the bottom face (Utils/sinking_velocity_fields.jl:16, k = 1):
+        // the speed with an open bottom, 0 with a closed one
+        const obme_sediment& sd = c.sediment;
+        auto listed = [&](int t) {
+            for (int q = 0; q < sd.n_sinking; q++)
+                if (sd.sinking[q] == t) return true;
+            return false;
+        };
+        const int s0 = h->sink_idx[0], s1 = h->sink_idx[1];
+        a.sed_w0 = (FT)((listed(s0) && c.sinking.open_bottom) ? c.sinking.speed[s0] : 0.0);
+        a.sed_w1 = (FT)((listed(s1) && c.sinking.open_bottom) ? c.sinking.speed[s1] : 0.0);
+        a.sed_recv = sd.receiver;
+        a.sed_a = (FT)sd.burial_efficiency_constant1;
+        a.sed_b = (FT)sd.burial_efficiency_constant2;
+        a.sed_k = (FT)sd.burial_efficiency_half_saturation;
+        a.sed_dz0 = (FT)h->dz[0];
+        a.sedG = h->d_sedG;
+        a.sed_cn = 1.5 + sd.ab2_chi;
+        a.sed_cm = 0.5 + sd.ab2_chi;
     }
     if (sizeof(FT) == 4) {
         a.f = make_fast_params(c.npd);
@@ -563,24 +671,24 @@ static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_
             k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, true, false, default_fm<MODEL>()>
                            : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, true, false, default_fm<MODEL>()>;
     }
-    // negative scaling (and the inline CO2 flux) in the fused step: the
-    // general (non-FULL) column code with the extra work compiled in
+    // negative scaling (1), the inline CO2 flux (2, with 1) and the sediment
+    // (4) in the fused step: the general (non-FULL) column code with the
+    // extra work compiled in
     bool extra = false;
     if constexpr (MODE == TILE_STEP) {
-        if (h->scaling) {
-            extra = true;
-            if constexpr (CARB) {
-                if (h->co2_inline)
-                    k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, false, false, -1, 2>
-                                   : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, false, false, -1, 2>;
-                else
-                    k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, false, false, -1, 1>
-                                   : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, false, false, -1, 1>;
-            } else {
-                k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, false, false, -1, 1>
-                               : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, false, false, -1, 1>;
-            }
+        const int ex = (h->scaling ? 1 : 0) | (h->co2_inline ? 2 : 0) | (h->sediment ? 4 : 0);
+        extra = ex != 0;
+#define OBME_EXTRA_KERNEL(E)                                                                              \
+    (h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, false, false, -1, E>            \
+                : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, false, false, -1, E>)
+        if (ex == 1) k = OBME_EXTRA_KERNEL(1);
+        if (ex == 4) k = OBME_EXTRA_KERNEL(4);
+        if (ex == 5) k = OBME_EXTRA_KERNEL(5);
+        if constexpr (CARB) {
+            if (ex == 3) k = OBME_EXTRA_KERNEL(3);
+            if (ex == 7) k = OBME_EXTRA_KERNEL(7);
         }
+#undef OBME_EXTRA_KERNEL
     }
     // the pipelined persistent sweep (FP32 fused step, FULL columns)
     constexpr int PB = (64 % TXB == 0) ? 64 / TXB : 1;
@@ -660,7 +768,7 @@ static obme_status launch_surface(obme_handle* h, const SurfaceArgs<FT>& sa, hip
 
 template <typename FT, int MODE>
 static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* G, void* PAR, const void* T,
-                             const void* S, double t, double dt, hipStream_t s)
+                             const void* S, double t, double dt, hipStream_t s, double* storage = nullptr)
 {
     StepArgs<FT> a = make_args<FT>(h, tracers);
     if (G) for (int n = 0; n < h->ntracers; n++) a.G[n] = (FT*)G[n];
@@ -693,7 +801,31 @@ static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* 
         hipLaunchKernelGGL(ks, dim3((unsigned)((ncell + bs - 1) / bs)), dim3(bs), 0, s, a);
         HIP_TRY(hipGetLastError());
     }
+    double sed_last = 0.0;
+    if (MODE == TILE_STEP && h->sediment) {
+        // the sediment steps after the three stages: Δt γ1, Δt (γ2 + ζ2),
+        // Δt (γ3 + ζ3) (the stage Δt of Oceananigans' RK3, its clock's
+        // last_stage_Δt); an AB2 step only where a step's Δt repeats the
+        // previous one, which within a step never happens
+        a.storage = storage;
+        a.sed_dt[0] = (8.0 / 15.0) * dt;
+        a.sed_dt[1] = (5.0 / 12.0 + -17.0 / 60.0) * dt;
+        a.sed_dt[2] = (3.0 / 4.0 + -5.0 / 12.0) * dt;
+        a.sed_init = h->sed_iter == 0 ? 1 : 0;
+        a.sed_ab2 = (!a.sed_init && a.sed_dt[0] == h->sed_last_dt) ? 1 : 0;
+        sed_last = a.sed_dt[2];
+    }
     obme_status r = dispatch_tile_cfg<FT, MODE>(h, a, s);
+    if (r == OBME_OK && MODE == TILE_STEP && h->sediment) {
+        h->sed_iter += 3;
+        h->sed_last_dt = sed_last;
+    }
+    if (r == OBME_OK && MODE == TILE_TENDENCY && h->sediment) {
+        const unsigned ncol = (unsigned)(h->cfg.grid.Nx * h->cfg.grid.Ny);
+        hipLaunchKernelGGL(k_sediment_couple<FT>, dim3((ncol + 255) / 256), dim3(256), 0, s, a, h->sink_idx[0],
+                           h->sink_idx[1]);
+        HIP_TRY(hipGetLastError());
+    }
     if (r != OBME_OK || MODE == TILE_PAR || !a.co2_on) return r;
     SurfaceArgs<FT> sa = make_surface_args<FT>(h, tracers, T, S);
     sa.dtf = (FT)dt;
@@ -704,10 +836,10 @@ static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* 
 
 template <int MODE>
 static obme_status run_tile(obme_handle* h, void* const* tracers, void* const* G, void* PAR, const void* T,
-                            const void* S, double t, double dt, hipStream_t s)
+                            const void* S, double t, double dt, hipStream_t s, double* storage = nullptr)
 {
-    if (h->prec == OBME_FP64) return run_typed<double, MODE>(h, tracers, G, PAR, T, S, t, dt, s);
-    return run_typed<float, MODE>(h, tracers, G, PAR, T, S, t, dt, s);
+    if (h->prec == OBME_FP64) return run_typed<double, MODE>(h, tracers, G, PAR, T, S, t, dt, s, storage);
+    return run_typed<float, MODE>(h, tracers, G, PAR, T, S, t, dt, s, storage);
 }
 
 extern "C" obme_status obme_update_state(obme_handle* h, void* const* tracers, void* PAR, double t, void* stream)
@@ -729,18 +861,56 @@ extern "C" obme_status obme_add_tendencies(obme_handle* h, const void* const* tr
     return run_tile<TILE_TENDENCY>(h, (void* const*)tracers, G, (void*)PAR, T, S, t, 0.0, (hipStream_t)stream);
 }
 
-extern "C" obme_status obme_step(obme_handle* h, void* const* tracers, const void* T, const void* S, int32_t nsteps,
-                                 double dt, double t0, void* stream)
+extern "C" obme_status obme_step_with_sediment(obme_handle* h, void* const* tracers, const void* T, const void* S,
+                                               double* storage, int32_t nsteps, double dt, double t0, void* stream)
 {
     if (!h || !tracers) return fail(OBME_ERR_INVALID, "null argument");
     if (nsteps < 0) return fail(OBME_ERR_INVALID, "negative nsteps");
     for (int n = 0; n < h->ntracers; n++)
         if (!tracers[n]) return fail(OBME_ERR_INVALID, "null tracer pointer");
+    if (storage && !h->sediment) return fail(OBME_ERR_INVALID, "no sediment configured");
+    if (h->sediment && !std::isfinite(dt)) return fail(OBME_ERR_INVALID, "non-finite time step with a sediment");
     hipStream_t s = (hipStream_t)stream;
     for (int st = 0; st < nsteps; st++) {
-        obme_status r = run_tile<TILE_STEP>(h, tracers, nullptr, nullptr, T, S, t0 + st * dt, dt, s);
+        obme_status r = run_tile<TILE_STEP>(h, tracers, nullptr, nullptr, T, S, t0 + st * dt, dt, s, storage);
         if (r != OBME_OK) return r;
     }
+    return OBME_OK;
+}
+
+extern "C" obme_status obme_step(obme_handle* h, void* const* tracers, const void* T, const void* S, int32_t nsteps,
+                                 double dt, double t0, void* stream)
+{
+    return obme_step_with_sediment(h, tracers, T, S, nullptr, nsteps, dt, t0, stream);
+}
+
+extern "C" obme_status obme_sediment_update_state(obme_handle* h, const void* const* tracers, double* storage,
+                                                  double dt, void* stream)
+{
+    if (!h || !tracers) return fail(OBME_ERR_INVALID, "null argument");
+    if (!h->sediment) return fail(OBME_ERR_INVALID, "no sediment configured");
+    const int s0 = h->sink_idx[0], s1 = h->sink_idx[1];
+    if (!tracers[s0] || !tracers[s1]) return fail(OBME_ERR_INVALID, "null tracer pointer");
+    // the tracked fields only: no step before the model's first stage
+    if (!std::isfinite(dt)) return OBME_OK;
+    const unsigned ncol = (unsigned)(h->cfg.grid.Nx * h->cfg.grid.Ny);
+    const bool euler = !(dt == h->sed_last_dt);
+    if (h->prec == OBME_FP64) {
+        StepArgs<double> a = make_args<double>(h, (void* const*)tracers);
+        a.storage = storage;
+        a.sed_init = h->sed_iter == 0 ? 1 : 0;
+        hipLaunchKernelGGL(k_sediment_state<double>, dim3((ncol + 255) / 256), dim3(256), 0, (hipStream_t)stream, a,
+                           s0, s1, dt, euler ? 1 : 0);
+    } else {
+        StepArgs<float> a = make_args<float>(h, (void* const*)tracers);
+        a.storage = storage;
+        a.sed_init = h->sed_iter == 0 ? 1 : 0;
+        hipLaunchKernelGGL(k_sediment_state<float>, dim3((ncol + 255) / 256), dim3(256), 0, (hipStream_t)stream, a,
+                           s0, s1, dt, euler ? 1 : 0);
+    }
+    HIP_TRY(hipGetLastError());
+    h->sed_iter += 1;
+    h->sed_last_dt = dt;
     return OBME_OK;
 }
 
@@ -990,6 +1160,8 @@ extern "C" obme_status obme_batch_create(const obme_config* cfg, int32_t n_colum
     if (cfg->model != OBME_NPZD) return fail(OBME_ERR_UNSUPPORTED, "the batched ensemble runs NPZD columns");
     if (cfg->scaling.n_groups != 0)
         return fail(OBME_ERR_UNSUPPORTED, "negative scaling is not built for the batched ensemble");
+    if (cfg->sediment.model != OBME_SEDIMENT_NONE)
+        return fail(OBME_ERR_UNSUPPORTED, "a sediment is not built for the batched ensemble");
     if (cfg->precision != OBME_FP64 && cfg->precision != OBME_FP32_MIXED)
         return fail(OBME_ERR_INVALID, "unknown precision");
     if (n_columns < 1) return fail(OBME_ERR_INVALID, "no columns");
@@ -1231,6 +1403,7 @@ extern "C" int64_t obme_abi_sizeof(int32_t which)
     case 9: return sizeof(obme_pisces_config);
     case 10: return sizeof(obme_negative_scaling);
     case 11: return sizeof(obme_o2_exchange);
+    case 12: return sizeof(obme_sediment);
     default: return -1;
     }
 }

@@ -101,8 +101,8 @@ struct StepArgs {
     float xr2h, xb2h;              // FP32 uniform grid: xr2 dz/2, xb2 dz/2
     float wi0, wi1;                // FP32 uniform grid: w / dz of the two sinking tracers
     int co2_on;                    // CO2 flux by the surface kernel (surface records written)
-    NegScale<FT, 16> ns;           // negative scaling at the start of every stage (EXTRA >= 1)
-    // CO2 flux computed in the sweep by the top lane (EXTRA == 2: with the
+    NegScale<FT, 16> ns;           // negative scaling at the start of every stage (EXTRA & 1)
+    // CO2 flux computed in the sweep by the top lane (EXTRA & 2: with the
     // scaling, the top cell's DIC feeds back into the other tracers)
     const FT* T3;
     const FT* S3;
@@ -113,6 +113,18 @@ struct StepArgs {
     FT invdz_top;
     obme_co2_exchange co2;
     obme_carbon_solver solver;
+    // InstantRemineralisation sediment (EXTRA & 4), worked by the lane that
+    // holds a column's bottom cell
+    int sed_recv;                  // receiver tracer
+    FT sed_w0, sed_w1;             // -w at the bottom face of SINK0 / SINK1 when listed, else 0
+    FT sed_a, sed_b, sed_k;        // burial efficiency a + b (f / (k + f))^2
+    FT sed_dz0;                    // Δz at the bottom face
+    double* storage;               // the storage field (Nx*Ny) or null
+    double* sedG;                  // the sediment's tendencies [2][Nx*Ny]: Gⁿ, G⁻
+    double sed_dt[3];              // Δt of the sediment step after each stage
+    double sed_cn, sed_cm;         // AB2 weights 1.5 + χ, 0.5 + χ of the first stage's step
+    int sed_ab2;                   // the first stage's step is an AB2 (not an Euler) step
+    int sed_init;                  // the sediment's first step ever is in this launch
 };
 
 // Per-level geometry rows in `lev` (each Nz entries).
@@ -209,8 +221,9 @@ __device__ __forceinline__ void lds_load_b32(rsrc_t r, void* dst, unsigned voff)
     __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)dst, 4, voff, 0, 0, 0);
 }
 
-// EXTRA (fused step only): 0 plain; 1 negative scaling at the start of every
-// stage; 2 scaling and the CO2 flux computed here by the top lane.
+// EXTRA (fused step only), a bit set: 1 negative scaling at the start of
+// every stage; 2 the CO2 flux computed here by the top lane (with 1);
+// 4 the InstantRemineralisation sediment at the bottom cell.
 template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL,
           bool PIPE = false, int FM = -1, int EXTRA = 0>
 __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
@@ -439,19 +452,29 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
     constexpr int NSTAGE = (MODE == TILE_STEP) ? 3 : 1;
     const double g1 = 8.0 / 15.0, g2 = 5.0 / 12.0, g3 = 3.0 / 4.0, z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
     FT PARv[L];   // PAR of the own levels (output of the PAR and tendency modes)
-    static_assert(EXTRA == 0 || (MODE == TILE_STEP && !PIPE), "scaling and inline CO2: fused step, direct sweep");
-    static_assert(EXTRA < 2 || CARB, "inline CO2 needs the carbonate system");
-    // inline CO2 (EXTRA 2): [H+] of the previous stage (FP32-mixed warm start)
+    static_assert(EXTRA == 0 || (MODE == TILE_STEP && !PIPE), "scaling, inline CO2, sediment: fused step, direct sweep");
+    static_assert(!(EXTRA & 2) || CARB, "inline CO2 needs the carbonate system");
+    // inline CO2 (EXTRA & 2): [H+] of the previous stage (FP32-mixed warm start)
     double Hc = 0.0;
-    if constexpr (EXTRA == 2 && F32) {
+    if constexpr ((EXTRA & 2) && F32) {
         if (top_lane && a.Hprev) Hc = a.Hprev[col2d];
+    }
+    // sediment (EXTRA & 4): the bottom lane's storage value, the tendency e f
+    // of the last stage start and the G⁻ of the sediment's previous step
+    const bool sed_lane = col_ok && r * L <= ktop && ktop < r * L + L;
+    double sed_st = 0.0, sed_ge = 0.0, sed_gm = 0.0;
+    if constexpr ((EXTRA & 4) != 0) {
+        if (sed_lane) {
+            if (a.storage) sed_st = a.storage[col2d];
+            if (a.sed_ab2) sed_gm = a.sedG[ncol2d + col2d];
+        }
     }
 
     auto run_stage = [&](const int st) {
         FT PAR0 = PAR0_const;
         if (a.surface_mode == OBME_SURFACE_DIURNAL) PAR0 = a.PAR0_stage[st];
 
-        if constexpr (EXTRA >= 1) {
+        if constexpr ((EXTRA & 1) != 0) {
             // ScaleNegativeTracers first (update_biogeochemical_state! order,
             // src/OceanBioME.jl:161-167).  FP64 applies it to every cell as
             // the reference does; FP32-mixed where a member is negative or
@@ -605,7 +628,7 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
                         Gv[TR::SINK1] = Gv[TR::SINK1] - (Fhi1 - g_wlo(1, kc) * v[TR::SINK1]) * idz;
                     }
                 }
-                if constexpr (EXTRA == 2) {
+                if constexpr ((EXTRA & 2) != 0) {
                     // the top-cell CO2 flux (GasExchange, gas_exchange.jl:26-38) of
                     // this stage's DIC and Alk, the surface kernel's expressions
                     if (top_lane && m == 0) {
@@ -620,6 +643,37 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
                         const double F = surface_flux_warm(sc, a.solver, (double)val(7, 0), (double)val(8, 0),
                                                            a.co2.air_concentration, F32 ? 1 : 0, Hc, pH, ev);
                         Gv[7] = Gv[7] - (FT)F * a.invdz_top;
+                    }
+                }
+                if constexpr ((EXTRA & 4) != 0) {
+                    // the sediment at the bottom cell: the sinking flux through
+                    // the bottom face, f = Σ -w c (Sediments/tracked_fields.jl:53-70),
+                    // burial e = a + b (f / (k + f))^2 and the receiver's coupled
+                    // tendency (1 - e) f / Δz (instant_remineralisation.jl:101-123,
+                    // Sediments/tracer_coupling.jl:30-38)
+                    if (col_ok && k == 0 && has(m)) {
+                        const FT f = a.sed_w0 * v[TR::SINK0] + a.sed_w1 * v[TR::SINK1];
+                        const FT qf = f / (a.sed_k + f);
+                        const FT e = a.sed_a + a.sed_b * (qf * qf);
+                        const FT cpl = ((FT(1) - e) * f) / a.sed_dz0;
+#pragma unroll
+                        for (int n = 0; n < NT; n++)
+                            if (n == a.sed_recv) Gv[n] = Gv[n] + cpl;
+                        // the sediment step after this stage's substep
+                        // (update_state.jl:6-17) takes the tendency evaluated
+                        // after the previous step, from this stage-start
+                        // state: e f here.  Its first step ever evaluates
+                        // the tendency lazily, after the first substep: the
+                        // tendency of the second stage start serves both.
+                        const double ge = (double)(e * f);
+                        if (st == 0) {
+                            if (!a.sed_init)
+                                sed_st += a.sed_dt[0] * (a.sed_ab2 ? a.sed_cn * ge - a.sed_cm * sed_gm : ge);
+                        } else {
+                            if (st == 1 && a.sed_init) sed_st += a.sed_dt[0] * ge;
+                            sed_st += a.sed_dt[st] * ge;
+                        }
+                        sed_ge = ge;
                     }
                 }
                 if constexpr (MODE == TILE_TENDENCY) {
@@ -657,8 +711,23 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
     // (the stages stay a loop: unrolled, the FP32 step measured 6 % slower)
 #pragma unroll 1
     for (int st = 0; st < NSTAGE; st++) run_stage(st);
-    if constexpr (EXTRA == 2 && F32) {
+    if constexpr ((EXTRA & 2) && F32) {
         if (top_lane && a.Hprev) a.Hprev[col2d] = Hc;
+    }
+    if constexpr ((EXTRA & 4) != 0) {
+        // the sediment after the last step: storage, G⁻ = the tendency of its
+        // last step, Gⁿ = e f of the final state
+#pragma unroll
+        for (int m = 0; m < L; m++) {
+            if (sed_lane && kof(m) == 0) {
+                const FT f = a.sed_w0 * val(TR::SINK0, m) + a.sed_w1 * val(TR::SINK1, m);
+                const FT qf = f / (a.sed_k + f);
+                const FT e = a.sed_a + a.sed_b * (qf * qf);
+                if (a.storage) a.storage[col2d] = sed_st;
+                a.sedG[col2d] = (double)(e * f);
+                a.sedG[ncol2d + col2d] = sed_ge;
+            }
+        }
     }
 
     if constexpr (PIPE) {

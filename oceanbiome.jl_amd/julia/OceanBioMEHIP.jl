@@ -116,6 +116,24 @@ struct ObmeNegativeScaling
     invalid_fill_value :: NTuple{OBME_MAX_SCALE_GROUPS, Float64}
 end
 
+const OBME_MAX_SEDIMENT_TRACERS = 6
+const OBME_SEDIMENT_INSTANT_REMINERALISATION = Int32(1)
+
+# InstantRemineralisationSediment (Models/Sediments/instant_remineralisation.jl):
+# receiver and sinking tracers as 0-based indices in required_biogeochemical_tracers
+# order; model = 0: no sediment.
+struct ObmeSediment
+    model :: Int32
+    receiver :: Int32
+    n_sinking :: Int32
+    sinking :: NTuple{OBME_MAX_SEDIMENT_TRACERS, Int32}
+    timestepper :: Int32             # 0: QuasiAdamsBashforth2
+    burial_efficiency_constant1 :: Float64
+    burial_efficiency_constant2 :: Float64
+    burial_efficiency_half_saturation :: Float64
+    ab2_chi :: Float64
+end
+
 struct ObmeConfig
     model :: Int32
     precision :: Int32
@@ -130,6 +148,7 @@ struct ObmeConfig
     temperature :: ObmeFieldSource
     salinity :: ObmeFieldSource
     scaling :: ObmeNegativeScaling
+    sediment :: ObmeSediment
 end
 
 # layout check against the library (obme_abi_sizeof)
@@ -139,7 +158,7 @@ function check_abi()
         n = ccall((:obme_abi_sizeof, libobme), Int64, (Int32,), idx - 1)
         n == sizeof(T) || error("ABI mismatch for $T: library $n bytes, Julia $(sizeof(T))")
     end
-    for (idx, T) in ((10, ObmeNegativeScaling),)
+    for (idx, T) in ((10, ObmeNegativeScaling), (12, ObmeSediment))
         n = ccall((:obme_abi_sizeof, libobme), Int64, (Int32,), idx)
         n == sizeof(T) || error("ABI mismatch for $T: library $n bytes, Julia $(sizeof(T))")
     end
@@ -157,6 +176,7 @@ mutable struct HIPBiogeochemistry{B, F} <: AbstractBiogeochemistry
     handle :: Ptr{Cvoid}
     PAR :: F                      # CenterField owned by the wrapper
     keep :: Vector{Any}           # host arrays referenced by `config`
+    storage :: Any                # the sediment's storage (device, Nx*Ny Float64) or nothing
 end
 
 """
@@ -166,12 +186,12 @@ Creates the library handle for `config` (built from the OceanBioME model
 `underlying` by the caller, see the Python host package for the field-by-field
 mapping) and owns `PAR`.
 """
-function HIPBiogeochemistry(underlying, grid, config::ObmeConfig, PAR; keep = Any[])
+function HIPBiogeochemistry(underlying, grid, config::ObmeConfig, PAR; keep = Any[], storage = nothing)
     check_abi()
     h = Ref{Ptr{Cvoid}}(C_NULL)
     cfg = Ref(config)
     check(ccall((:obme_create, libobme), Int32, (Ref{ObmeConfig}, Ref{Ptr{Cvoid}}), cfg, h))
-    bgc = HIPBiogeochemistry(underlying, cfg, h[], PAR, keep)
+    bgc = HIPBiogeochemistry(underlying, cfg, h[], PAR, keep, storage)
     finalizer(b -> ccall((:obme_destroy, libobme), Int32, (Ptr{Cvoid},), b.handle), bgc)
     return bgc
 end
@@ -198,6 +218,13 @@ function update_biogeochemical_state!(b::HIPBiogeochemistry, model)
     check(ccall((:obme_update_state, libobme), Int32,
                 (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Float64, Ptr{Cvoid}),
                 b.handle, ptrs, device_pointer(b.PAR), model.clock.time, stream_pointer()))
+    if b.config[].sediment.model == OBME_SEDIMENT_INSTANT_REMINERALISATION
+        # update_biogeochemical_state!(model, sediment) (Sediments/update_state.jl:6-17)
+        check(ccall((:obme_sediment_update_state, libobme), Int32,
+                    (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Float64, Ptr{Cvoid}),
+                    b.handle, ptrs, Ptr{Cvoid}(UInt(pointer(b.storage))), model.clock.last_stage_Δt,
+                    stream_pointer()))
+    end
     return nothing
 end
 
@@ -221,6 +248,14 @@ tendencies and substep per stage), in place on the tracer fields.
 """
 function step!(b::HIPBiogeochemistry, tracers, nsteps, Δt; t₀ = 0.0)
     ptrs = tracer_pointers(b, tracers)
+    if b.config[].sediment.model == OBME_SEDIMENT_INSTANT_REMINERALISATION
+        check(ccall((:obme_step_with_sediment, libobme), Int32,
+                    (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Float64, Float64,
+                     Ptr{Cvoid}),
+                    b.handle, ptrs, C_NULL, C_NULL, Ptr{Cvoid}(UInt(pointer(b.storage))), Int32(nsteps),
+                    Float64(Δt), Float64(t₀), stream_pointer()))
+        return nothing
+    end
     check(ccall((:obme_step, libobme), Int32,
                 (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Float64, Float64, Ptr{Cvoid}),
                 b.handle, ptrs, C_NULL, C_NULL, Int32(nsteps), Float64(Δt), Float64(t₀), stream_pointer()))

@@ -3,13 +3,14 @@
 Mirrors the OceanBioME.jl interface for the NutrientsPlanktonDetritus models
 (NPZD, LOBSTER + CarbonateSystem) and PISCES (oceanbiome.pisces), TwoBand and
 MultiBand PAR, sinking, CarbonChemistry, the CO2 and O2 GasExchange boundary
-conditions and ScaleNegativeTracers; the compute runs in libobme_hip.so.
+conditions, ScaleNegativeTracers and the InstantRemineralisation sediment;
+the compute runs in libobme_hip.so.
 """
 from . import abi, units
 from .grid import ColumnGrid, RectilinearGrid
 from .models import (LOBSTER, NPZD, AnalyticalLightLimitation, Biogeochemistry, CarbonateSystem,
                      CarbonChemistry, CarbonDioxideGasExchangeBoundaryCondition, DampedNewtonRaphsonSolver,
-                     Detritus, Linear, MondoLightLimitation, NitrateAmmonia, Nutrient,
+                     Detritus, InstantRemineralisationSediment, Linear, MondoLightLimitation, NitrateAmmonia, Nutrient,
                      OxygenGasExchangeBoundaryCondition, PhytoZoo, Quadratic,
                      TwoBandPhotosyntheticallyActiveRadiation, TwoParticleAndDissolved,
                      biogeochemical_drift_velocity, default_surface_PAR, required_biogeochemical_tracers)
@@ -20,7 +21,7 @@ __all__ = [
     "CarbonDioxideGasExchangeBoundaryCondition", "OxygenGasExchangeBoundaryCondition", "DampedNewtonRaphsonSolver",
     "TwoBandPhotosyntheticallyActiveRadiation", "Linear", "Quadratic", "MondoLightLimitation",
     "AnalyticalLightLimitation", "default_surface_PAR", "required_biogeochemical_tracers",
-    "biogeochemical_drift_velocity",
+    "biogeochemical_drift_velocity", "InstantRemineralisationSediment",
 ]
 
 

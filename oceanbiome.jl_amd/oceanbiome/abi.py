@@ -162,6 +162,26 @@ def negative_scaling(groups, names):
     return s
 
 
+OBME_SEDIMENT_NONE, OBME_SEDIMENT_INSTANT_REMINERALISATION = 0, 1
+OBME_SEDIMENT_AB2 = 0
+OBME_MAX_SEDIMENT_TRACERS = 6
+
+
+class Sediment(C.Structure):
+    """obme_sediment (InstantRemineralisation through BiogeochemicalSediment)."""
+    _fields_ = [
+        ("model", C.c_int32),
+        ("receiver", C.c_int32),
+        ("n_sinking", C.c_int32),
+        ("sinking", C.c_int32 * OBME_MAX_SEDIMENT_TRACERS),
+        ("timestepper", C.c_int32),
+        ("burial_efficiency_constant1", C.c_double),
+        ("burial_efficiency_constant2", C.c_double),
+        ("burial_efficiency_half_saturation", C.c_double),
+        ("ab2_chi", C.c_double),
+    ]
+
+
 class Config(C.Structure):
     _fields_ = [
         ("model", C.c_int32),
@@ -177,6 +197,7 @@ class Config(C.Structure):
         ("temperature", FieldSource),
         ("salinity", FieldSource),
         ("scaling", NegativeScaling),
+        ("sediment", Sediment),
     ]
 
 
@@ -244,4 +265,5 @@ ABI_STRUCTS = (
     ("obme_pisces_config", PiscesConfig),
     ("obme_negative_scaling", NegativeScaling),
     ("obme_o2_exchange", O2Exchange),
+    ("obme_sediment", Sediment),
 )

@@ -215,6 +215,58 @@ class TwoBandPhotosyntheticallyActiveRadiation:
         return s
 
 
+class InstantRemineralisationSediment:
+    """InstantRemineralisationSediment(grid; sinking_tracers = (:P, :D),
+    remineralisation_reciever = :N, ...) (Models/Sediments/instant_remineralisation.jl:60-98):
+    the sinking flux f of `sinking_tracers` through the bottom face is
+    remineralised into `remineralisation_reciever`'s bottom cell, a fraction
+    e = a + b (f / (k + f))^2 buried in the 2-D `storage` field.  The only
+    sediment time stepper built is the default QuasiAdamsBashforth2."""
+
+    def __init__(self, grid=None, sinking_tracers=("P", "D"), remineralisation_reciever="N",
+                 burial_efficiency_constant1=0.013, burial_efficiency_constant2=0.53,
+                 burial_efficiency_half_saturation=7.0 / 6.56, timestepper="QuasiAdamsBashforth2"):
+        if timestepper != "QuasiAdamsBashforth2":
+            raise NotImplementedError("only the QuasiAdamsBashforth2 sediment time stepper is built")
+        self.grid = grid
+        self.sinking_tracers = tuple(sinking_tracers)
+        self.remineralisation_reciever = remineralisation_reciever
+        self.burial_efficiency_constant1 = float(burial_efficiency_constant1)
+        self.burial_efficiency_constant2 = float(burial_efficiency_constant2)
+        self.burial_efficiency_half_saturation = float(burial_efficiency_half_saturation)
+        self.chi = 0.1   # QuasiAdamsBashforth2TimeStepper default χ
+
+    def required_sediment_fields(self):
+        return ("storage",)
+
+    def sinking_fluxes(self):
+        return self.sinking_tracers
+
+    def coupled_tracers(self):
+        return (self.remineralisation_reciever,)
+
+    def struct(self, names):
+        """obme_sediment with tracer names resolved against `names`."""
+        s = abi.Sediment()
+        s.model = abi.OBME_SEDIMENT_INSTANT_REMINERALISATION
+        if self.remineralisation_reciever not in names:
+            raise ValueError(f"{self.remineralisation_reciever} is not a tracer of the model")
+        s.receiver = names.index(self.remineralisation_reciever)
+        if len(self.sinking_tracers) > abi.OBME_MAX_SEDIMENT_TRACERS:
+            raise ValueError("too many sinking tracers")
+        for q, t in enumerate(self.sinking_tracers):
+            if t not in names:
+                raise ValueError(f"{t} is not a tracer of the model")
+            s.sinking[q] = names.index(t)
+        s.n_sinking = len(self.sinking_tracers)
+        s.timestepper = abi.OBME_SEDIMENT_AB2
+        s.burial_efficiency_constant1 = self.burial_efficiency_constant1
+        s.burial_efficiency_constant2 = self.burial_efficiency_constant2
+        s.burial_efficiency_half_saturation = self.burial_efficiency_half_saturation
+        s.ab2_chi = self.chi
+        return s
+
+
 def default_surface_PAR(t):
     """default_surface_PAR (Light/Light.jl:219)."""
     return 100 * max(0.0, np.cos(t * np.pi / (12 * hour)))
@@ -226,8 +278,10 @@ class Biogeochemistry:
 
     def __init__(self, grid, nutrients, plankton, detritus, carbonate_system=None,
                  light_attenuation=None, gas_exchange=None, sinking_scheme="upwind",
-                 temperature=None, salinity=35.0, scale_negatives=False, invalid_fill_value=float("nan")):
+                 temperature=None, salinity=35.0, scale_negatives=False, invalid_fill_value=float("nan"),
+                 sediment_model=None):
         self.grid = grid
+        self.sediment = sediment_model
         self.scale_negatives = bool(scale_negatives)
         self.invalid_fill_value = float(invalid_fill_value)
         self.nutrients = nutrients
@@ -378,6 +432,8 @@ class Biogeochemistry:
         cfg.temperature = self._field_source(self.temperature)
         cfg.salinity = self._field_source(self.salinity)
         cfg.scaling = self.scaling_struct()
+        if self.sediment is not None:
+            cfg.sediment = self.sediment.struct(list(self.required_biogeochemical_tracers()))
         return cfg
 
     def _field_source(self, v):

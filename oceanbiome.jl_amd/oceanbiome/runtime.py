@@ -46,6 +46,9 @@ def library():
         "obme_update_state": (C.c_int32, [vp, C.POINTER(vp), vp, C.c_double, vp]),
         "obme_add_tendencies": (C.c_int32, [vp, C.POINTER(vp), C.POINTER(vp), vp, vp, vp, C.c_double, vp]),
         "obme_step": (C.c_int32, [vp, C.POINTER(vp), vp, vp, C.c_int32, C.c_double, C.c_double, vp]),
+        "obme_step_with_sediment": (C.c_int32, [vp, C.POINTER(vp), vp, vp, vp, C.c_int32, C.c_double, C.c_double,
+                                                vp]),
+        "obme_sediment_update_state": (C.c_int32, [vp, C.POINTER(vp), vp, C.c_double, vp]),
         "obme_co2_flux": (C.c_int32, [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp, C.c_int64, C.c_int32, vp]),
         "obme_abi_sizeof": (C.c_int64, [C.c_int32]),
         "obme_set_kernel_timing": (C.c_int32, [vp, C.c_int32]),
@@ -74,7 +77,8 @@ def library():
 
 EXPORTED_SYMBOLS = (
     "obme_create", "obme_destroy", "obme_last_error", "obme_version", "obme_required_tracers",
-    "obme_update_state", "obme_add_tendencies", "obme_step", "obme_co2_flux", "obme_abi_sizeof",
+    "obme_update_state", "obme_add_tendencies", "obme_step", "obme_step_with_sediment",
+    "obme_sediment_update_state", "obme_co2_flux", "obme_abi_sizeof",
     "obme_set_kernel_timing", "obme_kernel_timing", "obme_batch_create", "obme_batch_destroy", "obme_batch_step",
     "obme_batch_configure", "obme_batch_autotune", "obme_batch_describe", "obme_pisces_create",
     "obme_pisces_destroy", "obme_pisces_update_state", "obme_pisces_add_tendencies", "obme_pisces_step",
@@ -124,6 +128,11 @@ class HIPBiogeochemistry:
         h = C.c_void_p()
         _check(library().obme_create(C.byref(self.cfg), C.byref(h)))
         self.handle = h
+        # the sediment's prognostic field (BiogeochemicalSediment.fields.storage)
+        self.sediment_storage = None
+        if getattr(bgc, "sediment", None) is not None:
+            g = bgc.grid
+            self.sediment_storage = torch.zeros((g.Ny, g.Nx), dtype=torch.float64, device="cuda")
 
     def close(self):
         if getattr(self, "handle", None):
@@ -164,10 +173,26 @@ class HIPBiogeochemistry:
                                              _ptr(T), _ptr(S), t, _stream()))
 
     def step(self, tracers, nsteps, dt, t0=0.0, T=None, S=None):
-        """nsteps fused RK3 steps in place (physics frozen)."""
+        """nsteps fused RK3 steps in place (physics frozen); with a sediment,
+        its storage field (self.sediment_storage) is advanced in the same pass."""
         self._check_fields(tracers)
+        if self.sediment_storage is not None:
+            _check(library().obme_step_with_sediment(self.handle, _ptr_array(tracers), _ptr(T), _ptr(S),
+                                                     _ptr(self.sediment_storage), int(nsteps), float(dt), float(t0),
+                                                     _stream()))
+            return
         _check(library().obme_step(self.handle, _ptr_array(tracers), _ptr(T), _ptr(S), int(nsteps), float(dt),
                                    float(t0), _stream()))
+
+    def update_sediment_state(self, tracers, dt):
+        """update_biogeochemical_state!(model, sediment) (Sediments/update_state.jl:6-17):
+        one sediment time step of dt (the model's last stage Δt; none when
+        dt is not finite) from the sinking flux of the tracers."""
+        if self.sediment_storage is None:
+            raise OBMEError("no sediment configured")
+        self._check_fields(tracers)
+        _check(library().obme_sediment_update_state(self.handle, _ptr_array(tracers), _ptr(self.sediment_storage),
+                                                    float(dt), _stream()))
 
     # -- measurement --------------------------------------------------------------
     def set_kernel_timing(self, enable=True):

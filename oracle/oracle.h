@@ -115,7 +115,24 @@ typedef struct orc_column_env {
     double S_surface;
     const double* Tcol; /* NPZD: T tracer column (read-only) or NULL */
     int fast_exit;
+    /* the sediment of this column (cfg->sediment): its storage, Gⁿ and G⁻,
+     * and the sediment clock (last Δt, iteration), or NULL */
+    struct orc_sediment_column* sed;
 } orc_column_env;
+
+/* One column's share of a BiogeochemicalSediment's state. */
+typedef struct orc_sediment_column {
+    double storage, Gn, Gm;
+    double last_dt;
+    long iteration;
+} orc_sediment_column;
+
+/* InstantRemineralisation: the sinking flux through the bottom face of a
+ * column (U[n*Nz + k] layout), its burial efficiency, and one sediment
+ * time step (QuasiAdamsBashforth2) from the tracked flux f. */
+double orc_sediment_flux(const obme_config* cfg, const double* U, int Nz);
+double orc_burial_efficiency(const obme_sediment* s, double f);
+void orc_sediment_time_step(const obme_sediment* s, orc_sediment_column* c, double f, double dt);
 
 /* U[n*Nz + k]: tracer n at level k (0-based, bottom to top).  Advances nsteps
  * in place; if PAR_out != NULL writes the PAR of the last stage evaluated. */
@@ -136,6 +153,17 @@ void orc_column_tendencies(const orc_column_env* env, const double* U, double t,
 int orc_grid_run(const obme_config* cfg, double* const* tracers, const double* T3,
                  const double* S3, const double* PAR0, int nsteps, double dt, double t0,
                  int nthreads, int fast_exit);
+/* orc_grid_run with the sediment: storage, Gn, Gm host Nx*Ny (i fastest),
+ * the sediment clock in *last_dt, *iteration (Inf and 0 before its first step). */
+int orc_grid_run_sediment(const obme_config* cfg, double* const* tracers, const double* T3,
+                          const double* S3, const double* PAR0, int nsteps, double dt, double t0,
+                          int nthreads, int fast_exit, double* storage, double* Gn, double* Gm,
+                          double* last_dt, long* iteration);
+/* update_biogeochemical_state!(model, sediment): one sediment step of dt
+ * (none for a non-finite dt) from the flux of the given tracers. */
+int orc_grid_sediment_update_state(const obme_config* cfg, const double* const* tracers,
+                                   double* storage, double* Gn, double* Gm, double* last_dt,
+                                   long* iteration, double dt);
 int orc_grid_run_columns(const obme_config* cfg, double* const* tracers, const double* T3,
                          const double* S3, const double* PAR0, int nsteps, double dt,
                          double t0, const int* col_i, const int* col_j, int ncols,

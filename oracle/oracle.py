@@ -86,6 +86,12 @@ def lib():
         "orc_scale_negatives": (None, [C.POINTER(abi.NegativeScaling), DP]),
         "orc_grid_scale_negatives": (I, [C.POINTER(abi.Config), C.POINTER(DP)]),
         "orc_grid_tendencies": (I, [C.POINTER(abi.Config), C.POINTER(DP), C.POINTER(DP), DP, DP, DP, D, DP]),
+        "orc_grid_run_sediment": (I, [C.POINTER(abi.Config), C.POINTER(DP), DP, DP, DP, I, D, D, I, I, DP, DP, DP,
+                                      DP, C.POINTER(C.c_long)]),
+        "orc_grid_sediment_update_state": (I, [C.POINTER(abi.Config), C.POINTER(DP), DP, DP, DP, DP,
+                                               C.POINTER(C.c_long), D]),
+        "orc_sediment_flux": (D, [C.POINTER(abi.Config), DP, I]),
+        "orc_burial_efficiency": (D, [C.POINTER(abi.Sediment), D]),
         "orc_batch_run": (I, [C.POINTER(abi.Config), I, C.POINTER(I), C.POINTER(DP), D, D, C.POINTER(DP), I, D,
                               D, I]),
     }
@@ -282,6 +288,51 @@ def grid_run(cfg, tracers, nsteps, dt, t0=0.0, T3=None, S3=None, PAR0=None, nthr
     """Advance Oceananigans-layout float64 arrays in place."""
     lib().orc_grid_run(C.byref(cfg), _ptrs(tracers), _dp(T3), _dp(S3), _dp(PAR0), nsteps, dt, t0,
                        nthreads, fast_exit)
+
+
+class SedimentState:
+    """A BiogeochemicalSediment's state on the oracle side: the storage field
+    and its time stepper's Gⁿ, G⁻ (Ny, Nx), and its clock (last Δt, iteration)."""
+
+    def __init__(self, Nx, Ny):
+        self.storage = np.zeros((Ny, Nx))
+        self.Gn = np.zeros((Ny, Nx))
+        self.Gm = np.zeros((Ny, Nx))
+        self.last_dt = math.inf
+        self.iteration = 0
+
+
+def grid_run_sediment(cfg, tracers, state, nsteps, dt, t0=0.0, T3=None, S3=None, PAR0=None, nthreads=8,
+                      fast_exit=1):
+    """grid_run with the sediment: after every substep, the sediment's time
+    step of the stage Δt (update_biogeochemical_state!, Sediments/update_state.jl:6-17)."""
+    last = C.c_double(state.last_dt)
+    it = C.c_long(state.iteration)
+    lib().orc_grid_run_sediment(C.byref(cfg), _ptrs(tracers), _dp(T3), _dp(S3), _dp(PAR0), nsteps, dt, t0,
+                                nthreads, fast_exit, _dp(state.storage), _dp(state.Gn), _dp(state.Gm),
+                                C.byref(last), C.byref(it))
+    state.last_dt = last.value
+    state.iteration = it.value
+
+
+def grid_sediment_update_state(cfg, tracers, state, dt):
+    """update_biogeochemical_state!(model, sediment): one sediment step of dt."""
+    last = C.c_double(state.last_dt)
+    it = C.c_long(state.iteration)
+    lib().orc_grid_sediment_update_state(C.byref(cfg), _ptrs(tracers), _dp(state.storage), _dp(state.Gn),
+                                         _dp(state.Gm), C.byref(last), C.byref(it), dt)
+    state.last_dt = last.value
+    state.iteration = it.value
+
+
+def sediment_flux(cfg, U):
+    """The sinking flux through the bottom face of one column, U[n, k]."""
+    U = np.ascontiguousarray(U, dtype=np.float64)
+    return lib().orc_sediment_flux(C.byref(cfg), _dp(U), U.shape[1])
+
+
+def burial_efficiency(sediment, f):
+    return lib().orc_burial_efficiency(C.byref(sediment), float(f))
 
 
 def grid_run_columns(cfg, tracers, cols_i, cols_j, nsteps, dt, t0=0.0, T3=None, S3=None, PAR0=None,

@@ -303,6 +303,77 @@ static void scale_column(const obme_config* cfg, int Nz, int nt, double* U)
     }
 }
 
+/* ---------------- sediment ---------------- */
+
+/* The tracked sinking flux (Sediments/tracked_fields.jl:53-70): for each of
+ * the sediment's sinking tracers, -w c at the bottom face (k = 1, 1-based)
+ * per unit area, w from the drift velocity profile and c the upwind (bottom)
+ * cell, summed in the listed order (instant_remineralisation.jl:104).  A
+ * tracer without a drift velocity has a zero flux.  (The reference divides
+ * the advective flux Az w c by Az: the same value to rounding.) */
+double orc_sediment_flux(const obme_config* cfg, const double* U, int Nz)
+{
+    const obme_sediment* s = &cfg->sediment;
+    double f = 0.0;
+    double* w = (double*)malloc(sizeof(double) * (Nz + 1));
+    for (int q = 0; q < s->n_sinking; q++) {
+        int n = s->sinking[q];
+        double sp = cfg->sinking.speed[n];
+        double fq = 0.0;
+        if (sp != 0.0) {
+            sinking_profile(&cfg->sinking, sp, Nz, w);
+            fq = -(w[0] * U[n * Nz + 0]);
+        }
+        f = (q == 0) ? fq : f + fq;
+    }
+    free(w);
+    return f;
+}
+
+/* e = a + b (f / (k + f))^2 (instant_remineralisation.jl:101-123; the
+ * docstring's b f / (k + f)^2 is not what the code computes). */
+double orc_burial_efficiency(const obme_sediment* s, double f)
+{
+    double x = f / (s->burial_efficiency_half_saturation + f);
+    return s->burial_efficiency_constant1 + s->burial_efficiency_constant2 * (x * x);
+}
+
+/* time_step!(sediment, Δt) of update_biogeochemical_state!
+ * (Sediments/update_state.jl:6-17) with its QuasiAdamsBashforth2 stepper:
+ * at iteration 0 the tendencies are computed first (update_state!); the
+ * step is forward Euler (χ = -1/2, G⁻ multiplied by a strong zero) when Δt
+ * differs from the clock's last Δt, else (3/2 + χ) Gⁿ - (1/2 + χ) G⁻
+ * (timesteppers.jl:16-43); then G⁻ = Gⁿ (cache_previous_tendencies!,
+ * :81-91) and Gⁿ = e f from the tracked flux (compute_tendencies.jl:7-31). */
+void orc_sediment_time_step(const obme_sediment* s, orc_sediment_column* c, double f, double dt)
+{
+    if (!isfinite(dt)) return;
+    double ge = orc_burial_efficiency(s, f) * f;
+    if (c->iteration == 0) c->Gn = ge;
+    int euler = !(dt == c->last_dt);
+    double chi = euler ? -0.5 : s->ab2_chi;
+    double Gu = (1.5 + chi) * c->Gn - (euler ? 0.0 : (0.5 + chi) * c->Gm);
+    c->storage = c->storage + dt * Gu;
+    c->Gm = c->Gn;
+    c->Gn = ge;
+    c->last_dt = dt;
+    c->iteration += 1;
+}
+
+/* The flux of the bottom cell after the modifiers (the scaling runs first
+ * in update_biogeochemical_state!, before the sediment's tracked fields). */
+static double sediment_flux_scaled(const obme_config* cfg, const double* U, int Nz, int nt)
+{
+    if (cfg->scaling.n_groups <= 0) return orc_sediment_flux(cfg, U, Nz);
+    /* the bottom cell alone, as a one-level column */
+    double* B = (double*)malloc(sizeof(double) * nt);
+    for (int n = 0; n < nt; n++) B[n] = U[n * Nz + 0];
+    orc_scale_negatives(&cfg->scaling, B);
+    double f = orc_sediment_flux(cfg, B, 1);
+    free(B);
+    return f;
+}
+
 static double surface_PAR(const orc_column_env* env, double t)
 {
     if (env->cfg->par.surface_mode == OBME_SURFACE_DIURNAL) return orc_default_surface_PAR(t);
@@ -350,6 +421,16 @@ void orc_column_tendencies(const orc_column_env* env, const double* U, double t,
                                 NULL, NULL);
         G[7 * Nz + top] = G[7 * Nz + top] - F / env->dz[top];
     }
+
+    /* the sediment's coupled tendency (Sediments/tracer_coupling.jl:3-38):
+     * the receiver's bottom cell gains (1 - e) f / Δz, Δz at the bottom face
+     * taken as the bottom cell's thickness */
+    if (cfg->sediment.model == OBME_SEDIMENT_INSTANT_REMINERALISATION) {
+        double f = orc_sediment_flux(cfg, U, Nz);
+        double e = orc_burial_efficiency(&cfg->sediment, f);
+        int r = cfg->sediment.receiver;
+        G[r * Nz + 0] = G[r * Nz + 0] + ((1 - e) * f) / env->dz[0];
+    }
 }
 
 /* RungeKutta3 (Oceananigans): γ = 8/15, 5/12, 3/4; ζ = -17/60, -5/12; the
@@ -367,8 +448,13 @@ void orc_column_run(const orc_column_env* env, double* U, int nsteps, double dt,
     const double g1 = 8.0 / 15.0, g2 = 5.0 / 12.0, g3 = 3.0 / 4.0;
     const double z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
     int readonly = (env->cfg->model == OBME_NPZD) ? 3 : -1;
+    const obme_sediment* sd = &env->cfg->sediment;
+    int sed = env->sed && sd->model == OBME_SEDIMENT_INSTANT_REMINERALISATION;
     /* the modifiers run first in update_biogeochemical_state!
-     * (src/OceanBioME.jl:161-167): scaling before every tendency evaluation */
+     * (src/OceanBioME.jl:161-167): scaling before every tendency evaluation;
+     * after every substep the update-state hooks end with the sediment's
+     * time step of the stage Δt (Sediments/update_state.jl:6-17), from the
+     * flux of the substep's (scaled) state */
     for (int s = 0; s < nsteps; s++) {
         double t = t0 + s * dt;
         scale_column(env->cfg, Nz, nt, U);
@@ -377,6 +463,7 @@ void orc_column_run(const orc_column_env* env, double* U, int nsteps, double dt,
             if (i / Nz == readonly) continue;
             U[i] += dt * g1 * G[i];
         }
+        if (sed) orc_sediment_time_step(sd, env->sed, sediment_flux_scaled(env->cfg, U, Nz, nt), g1 * dt);
         memcpy(Gm, G, sizeof(double) * ncell);
         scale_column(env->cfg, Nz, nt, U);
         orc_column_tendencies(env, U, t + g1 * dt, G, PAR);
@@ -384,6 +471,7 @@ void orc_column_run(const orc_column_env* env, double* U, int nsteps, double dt,
             if (i / Nz == readonly) continue;
             U[i] += dt * (g2 * G[i] + z2 * Gm[i]);
         }
+        if (sed) orc_sediment_time_step(sd, env->sed, sediment_flux_scaled(env->cfg, U, Nz, nt), (g2 + z2) * dt);
         memcpy(Gm, G, sizeof(double) * ncell);
         scale_column(env->cfg, Nz, nt, U);
         orc_column_tendencies(env, U, t + (g1 + g2 + z2) * dt, G, PAR);
@@ -391,6 +479,7 @@ void orc_column_run(const orc_column_env* env, double* U, int nsteps, double dt,
             if (i / Nz == readonly) continue;
             U[i] += dt * (g3 * G[i] + z3 * Gm[i]);
         }
+        if (sed) orc_sediment_time_step(sd, env->sed, sediment_flux_scaled(env->cfg, U, Nz, nt), (g3 + z3) * dt);
     }
     if (PAR_out) memcpy(PAR_out, PAR, sizeof(double) * Nz);
     free(G);
@@ -428,12 +517,13 @@ static void env_for_column(const obme_config* cfg, const double* T3, const doubl
     env->T_surface = field_value(&cfg->temperature, T3, g, i, j, g->Nz - 1);
     env->S_surface = field_value(&cfg->salinity, S3, g, i, j, g->Nz - 1);
     env->Tcol = NULL;
+    env->sed = NULL;
     (void)Tcol;
 }
 
 static int run_one_column(const obme_config* cfg, double* const* tracers, const double* T3,
                           const double* S3, const double* PAR0, int i, int j, int nsteps, double dt,
-                          double t0, int fast_exit)
+                          double t0, int fast_exit, orc_sediment_column* sed)
 {
     const obme_grid* g = &cfg->grid;
     int Nz = g->Nz;
@@ -443,6 +533,7 @@ static int run_one_column(const obme_config* cfg, double* const* tracers, const 
     orc_column_env env;
     env_for_column(cfg, T3, S3, i, j, Tcol, &env, fast_exit);
     if (PAR0) env.PAR0_const = PAR0[i + g->Nx * j];
+    env.sed = sed;
     for (int n = 0; n < nt; n++)
         for (int k = 0; k < Nz; k++) U[n * Nz + k] = tracers[n][lidx(g, i, j, k)];
     if (cfg->model == OBME_NPZD) {
@@ -465,19 +556,84 @@ int orc_grid_run_columns(const obme_config* cfg, double* const* tracers, const d
 {
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
     for (int c = 0; c < ncols; c++)
-        run_one_column(cfg, tracers, T3, S3, PAR0, col_i[c], col_j[c], nsteps, dt, t0, fast_exit);
+        run_one_column(cfg, tracers, T3, S3, PAR0, col_i[c], col_j[c], nsteps, dt, t0, fast_exit, NULL);
+    return 0;
+}
+
+int orc_grid_run_sediment(const obme_config* cfg, double* const* tracers, const double* T3,
+                          const double* S3, const double* PAR0, int nsteps, double dt, double t0,
+                          int nthreads, int fast_exit, double* storage, double* Gn, double* Gm,
+                          double* last_dt, long* iteration)
+{
+    const obme_grid* g = &cfg->grid;
+    long ncols = (long)g->Nx * g->Ny;
+    int sed = cfg->sediment.model == OBME_SEDIMENT_INSTANT_REMINERALISATION && storage && Gn && Gm &&
+              last_dt && iteration;
+    double last_dt_after = sed ? *last_dt : 0.0;
+    long iteration_after = sed ? *iteration : 0;
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
+    for (long c = 0; c < ncols; c++) {
+        orc_sediment_column sc;
+        if (sed) {
+            sc.storage = storage[c];
+            sc.Gn = Gn[c];
+            sc.Gm = Gm[c];
+            sc.last_dt = *last_dt;
+            sc.iteration = *iteration;
+        }
+        run_one_column(cfg, tracers, T3, S3, PAR0, (int)(c % g->Nx), (int)(c / g->Nx), nsteps, dt, t0,
+                       fast_exit, sed ? &sc : NULL);
+        if (sed) {
+            storage[c] = sc.storage;
+            Gn[c] = sc.Gn;
+            Gm[c] = sc.Gm;
+            if (c == 0) {
+                last_dt_after = sc.last_dt;
+                iteration_after = sc.iteration;
+            }
+        }
+    }
+    if (sed) {
+        *last_dt = last_dt_after;
+        *iteration = iteration_after;
+    }
     return 0;
 }
 
 int orc_grid_run(const obme_config* cfg, double* const* tracers, const double* T3, const double* S3,
                  const double* PAR0, int nsteps, double dt, double t0, int nthreads, int fast_exit)
 {
+    return orc_grid_run_sediment(cfg, tracers, T3, S3, PAR0, nsteps, dt, t0, nthreads, fast_exit, NULL, NULL,
+                                 NULL, NULL, NULL);
+}
+
+int orc_grid_sediment_update_state(const obme_config* cfg, const double* const* tracers,
+                                   double* storage, double* Gn, double* Gm, double* last_dt,
+                                   long* iteration, double dt)
+{
     const obme_grid* g = &cfg->grid;
+    int nt = orc_npd_ntracers(cfg->model, cfg->carbonate_system);
+    double* B = (double*)malloc(sizeof(double) * nt);
     long ncols = (long)g->Nx * g->Ny;
-#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
-    for (long c = 0; c < ncols; c++)
-        run_one_column(cfg, tracers, T3, S3, PAR0, (int)(c % g->Nx), (int)(c / g->Nx), nsteps, dt, t0,
-                       fast_exit);
+    orc_sediment_column sc = {0};
+    for (long c = 0; c < ncols; c++) {
+        int i = (int)(c % g->Nx), j = (int)(c / g->Nx);
+        for (int n = 0; n < nt; n++) B[n] = tracers[n][lidx(g, i, j, 0)];
+        sc.storage = storage[c];
+        sc.Gn = Gn[c];
+        sc.Gm = Gm[c];
+        sc.last_dt = *last_dt;
+        sc.iteration = *iteration;
+        orc_sediment_time_step(&cfg->sediment, &sc, orc_sediment_flux(cfg, B, 1), dt);
+        storage[c] = sc.storage;
+        Gn[c] = sc.Gn;
+        Gm[c] = sc.Gm;
+    }
+    if (ncols > 0) {
+        *last_dt = sc.last_dt;
+        *iteration = sc.iteration;
+    }
+    free(B);
     return 0;
 }
 
