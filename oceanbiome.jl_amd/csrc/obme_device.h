@@ -750,9 +750,13 @@ __device__ __forceinline__ void alk_residual_and_derivative_fast(double H, const
 }
 
 // Refinement of a close first guess by plain FP64 Newton on the same
-// residual; returns false when it has not converged after 6 steps.
+// residual; returns false when it has not converged after 6 steps.  It
+// stops after the step that is at most tol relative to the iterate: Newton
+// converges quadratically here, so the error left after a step of relative
+// size tol is of order tol^2 (tol = 1e-8 leaves ~1e-16, the FP64 rounding
+// of H; the default 1e-14 takes one confirming step more).
 template <bool SI = false>
-__device__ __forceinline__ bool refine_H(const CarbonParams& p, double& x, int& evals)
+__device__ __forceinline__ bool refine_H(const CarbonParams& p, double& x, int& evals, double tol = 1e-14)
 {
 #pragma unroll 1
     for (int it = 0; it < 6; it++) {
@@ -763,7 +767,7 @@ __device__ __forceinline__ bool refine_H(const CarbonParams& p, double& x, int& 
         double xn = x - d;
         if (!(xn > 0.0)) return false;
         x = xn;
-        if (fabs(d) <= 1e-14 * x) return true;
+        if (fabs(d) <= tol * x) return true;
     }
     return false;
 }
@@ -902,20 +906,125 @@ __device__ __forceinline__ double surface_flux_warm(const SurfaceCarbon& sc, con
 // (generic_parameterisations.jl:31-32), α = β / Tk with
 // β = exp(A1 + A2/(Tk/100) + A3 log(Tk/100) + S (B1 + B2 Tk/100 + B2 (Tk/100)²))
 // (gas_solubility.jl:37-48: B2 in both salinity terms, as there).
-__device__ __forceinline__ double o2_flux(const obme_o2_exchange& g, double O2, double T, double S)
+// The T- and S-dependent part: the transfer velocity k and the air-side
+// concentration C_air α.
+__device__ __forceinline__ void o2_exchange_coefficients(const obme_o2_exchange& g, double T, double S, double& k,
+                                                         double& air)
 {
     const double* c = g.schmidt;
     const double Sc = c[0] + c[1] * T + c[2] * (T * T) + c[3] * (T * T * T) + c[4] * pow4_compensated(T);
     const double* v = g.transfer_velocity;
     const double u = g.wind_speed;
     const double k660 = v[0] + v[1] * u + v[2] * (u * u);
-    const double k = k660 / sqrt(Sc / 660.0) * 1.0;
+    k = k660 / sqrt(Sc / 660.0) * 1.0;
     const double Tk = T + 273.15;
     const double Tk100 = Tk / 100.0;
     const double* a = g.solubility;
     const double beta = exp(a[0] + a[1] / Tk100 + a[2] * log(Tk100) + S * (a[3] + a[4] * Tk100 + a[4] * (Tk100 * Tk100)));
-    const double air = g.air_concentration * (beta / Tk);
+    air = g.air_concentration * (beta / Tk);
+}
+
+__device__ __forceinline__ double o2_flux(const obme_o2_exchange& g, double O2, double T, double S)
+{
+    double k, air;
+    o2_exchange_coefficients(g, T, S, k, air);
     return k * (O2 - air);
+}
+
+// SurfaceCarbon (without DIC and Alk) and the O2 coefficients of a column
+// as 21 doubles, for a stepper with T and S fixed.
+constexpr int SURF_PACK = 21;
+__device__ __forceinline__ void surface_pack(const SurfaceCarbon& sc, double kO2, double airO2, double* o)
+{
+    const CarbonParams& p = sc.p;
+    const double f[SURF_PACK] = {p.boron, p.sulfate, p.fluoride, p.silicate, p.phosphate, p.K1, p.K2,
+                                 p.KB, p.KW, p.KS, p.KF, p.KP1, p.KP2, p.KP3, p.KSi, p.K0, p.Tk,
+                                 sc.conv, sc.k, kO2, airO2};
+#pragma unroll
+    for (int q = 0; q < SURF_PACK; q++) o[q] = f[q];
+}
+__device__ __forceinline__ void surface_unpack(const double* o, SurfaceCarbon& sc, double& kO2, double& airO2)
+{
+    CarbonParams& p = sc.p;
+    p.DIC = 0.0;
+    p.Alk = 0.0;
+    p.boron = o[0];
+    p.sulfate = o[1];
+    p.fluoride = o[2];
+    p.silicate = o[3];
+    p.phosphate = o[4];
+    p.K1 = o[5];
+    p.K2 = o[6];
+    p.KB = o[7];
+    p.KW = o[8];
+    p.KS = o[9];
+    p.KF = o[10];
+    p.KP1 = o[11];
+    p.KP2 = o[12];
+    p.KP3 = o[13];
+    p.KSi = o[14];
+    p.K0 = o[15];
+    p.Tk = o[16];
+    sc.conv = o[17];
+    sc.k = o[18];
+    kO2 = o[19];
+    airO2 = o[20];
+}
+
+// ---- wave-level helpers (the column sweeps)
+
+// Cross-lane data movement with DPP (GFX9 family): row_shr:d shifts within
+// rows of 16 lanes, row_bcast:15/31 carries row ends into later rows,
+// wave_shr:1 shifts the whole wave by one lane.  bound_ctrl gives 0 for a
+// source lane outside the row (or wave), so a shifted-in 0 is the identity
+// of the sums below.
+template <int CTRL, int ROWMASK, bool BC>
+__device__ __forceinline__ float dpp(float x)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, ROWMASK,
+                                                                 0xf, BC));
+}
+template <int CTRL, int ROWMASK, bool BC>
+__device__ __forceinline__ double dpp(double x)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffull), CTRL, ROWMASK, 0xf, BC);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWMASK, 0xf, BC);
+    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// value of lane r-1 (lane 0 of the wave: 0)
+template <typename T>
+__device__ __forceinline__ T lane_shr1(T x)
+{
+    return dpp<0x138, 0xf, true>(x);
+}
+
+// Inclusive prefix sum over segments of W lanes (W = 16, 32 or 64).
+template <int W, typename T>
+__device__ __forceinline__ T seg_prefix(T x)
+{
+    x = x + dpp<0x111, 0xf, true>(x);
+    x = x + dpp<0x112, 0xf, true>(x);
+    x = x + dpp<0x114, 0xf, true>(x);
+    x = x + dpp<0x118, 0xf, true>(x);
+    if constexpr (W >= 32) x = x + dpp<0x142, 0xa, false>(x);
+    if constexpr (W >= 64) x = x + dpp<0x143, 0xc, false>(x);
+    return x;
+}
+
+// Tile of workgroup b of an nb-workgroup launch.  The dispatcher hands
+// workgroup b to XCD b % 8 (round robin over the 8 XCDs of MI355X, each
+// with its own L2).  Each XCD is given a contiguous run of tiles instead,
+// so tiles that are neighbours along i, which share the 64-byte segments
+// at their edges (a tile row is TXB values from an arbitrary offset), are
+// read into and written back from one L2, by workgroups that run at about
+// the same time.  A bijection of [0, nb).
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)
+{
+    constexpr unsigned X = 8;
+    const unsigned x = b % X, s = b / X, q = nb / X, rem = nb % X;
+    return x * q + (x < rem ? x : rem) + s;
 }
 
 }  // namespace obme

@@ -135,60 +135,6 @@ enum { LEV_ZC = 0, LEV_UP, LEV_DN, LEV_INVDZ, LEV_COUNT };
 // Modes of the column sweep.
 enum { TILE_STEP = 0, TILE_TENDENCY = 1, TILE_PAR = 2 };
 
-// Cross-lane data movement with DPP (GFX9 family): row_shr:d shifts within
-// rows of 16 lanes, row_bcast:15/31 carries row ends into later rows,
-// wave_shr:1 shifts the whole wave by one lane.  bound_ctrl gives 0 for a
-// source lane outside the row (or wave), so a shifted-in 0 is the identity
-// of the sums below.
-template <int CTRL, int ROWMASK, bool BC>
-__device__ __forceinline__ float dpp(float x)
-{
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, ROWMASK,
-                                                                 0xf, BC));
-}
-template <int CTRL, int ROWMASK, bool BC>
-__device__ __forceinline__ double dpp(double x)
-{
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffull), CTRL, ROWMASK, 0xf, BC);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROWMASK, 0xf, BC);
-    return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
-
-// value of lane r-1 (lane 0 of the wave: 0)
-template <typename T>
-__device__ __forceinline__ T lane_shr1(T x)
-{
-    return dpp<0x138, 0xf, true>(x);
-}
-
-// Inclusive prefix sum over segments of W lanes (W = 16, 32 or 64).
-template <int W, typename T>
-__device__ __forceinline__ T seg_prefix(T x)
-{
-    x = x + dpp<0x111, 0xf, true>(x);
-    x = x + dpp<0x112, 0xf, true>(x);
-    x = x + dpp<0x114, 0xf, true>(x);
-    x = x + dpp<0x118, 0xf, true>(x);
-    if constexpr (W >= 32) x = x + dpp<0x142, 0xa, false>(x);
-    if constexpr (W >= 64) x = x + dpp<0x143, 0xc, false>(x);
-    return x;
-}
-
-// Tile of workgroup b of an nb-workgroup launch.  The dispatcher hands
-// workgroup b to XCD b % 8 (round robin over the 8 XCDs of MI355X, each
-// with its own L2).  Each XCD is given a contiguous run of tiles instead,
-// so tiles that are neighbours along i, which share the 64-byte segments
-// at their edges (a tile row is TXB values from an arbitrary offset), are
-// read into and written back from one L2, by workgroups that run at about
-// the same time.  A bijection of [0, nb).
-__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned nb)
-{
-    constexpr unsigned X = 8;
-    const unsigned x = b % X, s = b / X, q = nb / X, rem = nb % X;
-    return x * q + (x < rem ? x : rem) + s;
-}
-
 // s_waitcnt with only the vector-memory count constrained (gfx9 encoding:
 // vmcnt bits [3:0] and [15:14], expcnt [6:4] = 7, lgkmcnt [11:8] = 15).
 template <int N>

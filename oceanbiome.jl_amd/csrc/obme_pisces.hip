@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -283,7 +284,9 @@ __global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES :
             H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
         } else {
             H = a.have_H ? a.Hs[c] : (double)guess_H_fp32(cp);
-            if (!refine_H<true>(cp, H, ev)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
+            // Ω enters the FP32 transitions: a step below 1e-8 relative leaves H
+            // accurate to FP64 rounding
+            if (!refine_H<true>(cp, H, ev, 1e-8)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
             a.Hs[c] = H;
         }
         A.Omega = (FT)omega_from_H(o, cp, H);
@@ -339,6 +342,296 @@ __global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES :
     }
 }
 
+// Thread per column: the surface constants of the CO2 and O2 exchange
+// (T and S fixed within a stepper call), SURF_PACK doubles per column.
+template <typename FT>
+__global__ void __launch_bounds__(256) k_pisces_surface_prep(PiscesArgs<FT> a, double* out)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncol = a.Nx * a.Ny;
+    if (col >= ncol) return;
+    const unsigned p = pidx(a, col % a.Nx, col / a.Nx, a.Nz - 1);
+    const double T = (double)a.in[OBME_PI_T][p], S = (double)a.in[OBME_PI_S][p];
+    SurfaceCarbon sc;
+    std::memset(&sc, 0, sizeof(sc));
+    if (a.co2.enabled) surface_carbon_setup(a.co2, T, S, a.co2.wind_speed, sc);
+    double kO2 = 0.0, airO2 = 0.0;
+    if (a.o2.enabled) o2_exchange_coefficients(a.o2, T, S, kO2, airO2);
+    double o[SURF_PACK];
+    surface_pack(sc, kO2, airO2, o);
+#pragma unroll
+    for (int q = 0; q < SURF_PACK; q++) out[(long)q * ncol + col] = o[q];
+}
+
+// Per-call constants of the fused step.
+struct PiscesFused {
+    NegScale<float, 32> ns;         // ScaleNegativeTracers groups (SCALE)
+    StageConst<float> sk[3];        // day lengths at the three stage clocks
+    float A[3], B[3];               // Δt γ, Δt ζ per stage
+    float PAR0div[3];               // surface PAR × the share of each band
+    int have_H;                     // Hs holds the [H+] of the previous step
+    const double* surf;             // [SURF_PACK][Nx*Ny] from k_pisces_surface_prep
+};
+
+// The fused FP32-mixed PISCES step: one wave per water column, lane r
+// holding level k = Nz - 1 - r (r = 0 at the top; Nz <= 64, lanes r >= Nz
+// idle), the three RK3 stages in registers, one read and one write of each
+// tracer per step, in place.  Per stage, as the per-stage kernels do:
+// the scaling modifier (SCALE); the MultiBand PAR as an inclusive prefix
+// sum from the top of the per-level log attenuation (multi_band.jl:147-165);
+// the euphotic depth (the first level from the top where PAR falls to 1e-3
+// of the surface value, by ballot; compute_euphotic_depth.jl:3-33) and the
+// mixed-layer mean PAR (a wave sum; mean_mixed_layer_properties.jl:25-47);
+// Ω from the prepared constants (FP64 Newton warm-started from the
+// previous stage's [H+]); the top fluxes of CO2 and O2 by the top lane; the
+// 24 transitions; the upwind sinking with the value of the level above
+// from the neighbouring lane; the substep.  The workgroup's PF_WAVES
+// columns are neighbours along i, in the XCD-aware tile order.
+#ifndef OBME_PISCES_PF_WAVES
+#define OBME_PISCES_PF_WAVES 4
+#endif
+constexpr int PF_WAVES = OBME_PISCES_PF_WAVES;
+// waves per SIMD the register allocation is held to
+#ifndef OBME_PISCES_PF_OCC
+#define OBME_PISCES_PF_OCC 2
+#endif
+
+template <bool SCALE>
+__global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fused(PiscesArgs<float> a, PiscesFused f)
+{
+    using FT = float;
+    const int r = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const unsigned nbx = ((unsigned)a.Nx + PF_WAVES - 1) / PF_WAVES;
+    const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int j = (int)(tile / nbx);
+    const int ib = (int)(tile % nbx) * PF_WAVES;
+    const int i = ib + wv;
+    // a wave past the last column computes on zeros and stores nothing
+    // (it stays for the workgroup barriers)
+    const bool colok = i < a.Nx;
+    const int Nz = a.Nz;
+    const bool act = colok && r < Nz;
+    const int k = (r < Nz) ? Nz - 1 - r : 0;
+    const unsigned ncol = (unsigned)a.Nx * (unsigned)a.Ny;
+    const unsigned col = (unsigned)(colok ? i : a.Nx - 1) + (unsigned)a.Nx * (unsigned)j;
+    const unsigned long ncell = (unsigned long)ncol * (unsigned)Nz;
+    const int tid = threadIdx.x;
+
+    // the lane's carbonate constants (T, S and pressure fixed) and the
+    // previous stage's transitions live in LDS, each lane in its own slots,
+    // to leave the registers to the state and the transitions
+    __shared__ double s_oc[OMEGA_CONST_FIELDS][PF_WAVES * 64];
+    __shared__ FT s_gm[NW][PF_WAVES * 64];
+    __shared__ FT s_ts[2][PF_WAVES * 64];
+    __shared__ double s_h[PF_WAVES * 64];
+    __shared__ double s_surf[PF_WAVES][SURF_PACK];   // the top lane's surface constants
+
+    // stage in: thread t takes level kt = t / PF_WAVES of column ct = t %
+    // PF_WAVES (PF_WAVES neighbouring values along i per level: coalesced)
+    // and puts it into the LDS slot of the lane that owns that cell
+    const int kt = tid / PF_WAVES, ct = tid % PF_WAVES;
+    const bool tok = kt < Nz && ib + ct < a.Nx;
+    const int slot = ct * 64 + (Nz - 1 - (kt < Nz ? kt : 0));
+    const unsigned pt = pidx(a, tok ? ib + ct : 0, j, tok ? kt : 0);
+    const unsigned long ct_cell = (unsigned long)(tok ? ib + ct : 0) + (unsigned long)a.Nx * (unsigned)j +
+                                  (unsigned long)ncol * (unsigned)(tok ? kt : 0);
+    {
+        FT stg[NTR];
+#pragma unroll
+        for (int n = 0; n < NTR; n++) stg[n] = tok ? a.in[n][pt] : FT(0);
+        double so[OMEGA_CONST_FIELDS];
+#pragma unroll
+        for (int q = 0; q < OMEGA_CONST_FIELDS; q++) so[q] = tok ? a.oc[(unsigned long)q * ncell + ct_cell] : 1.0;
+        const double sh = (tok && f.have_H) ? a.Hs[ct_cell] : -1.0;
+        if (tok) {
+#pragma unroll
+            for (int n = 0; n < NW; n++) s_gm[n][slot] = stg[n];
+            s_ts[0][slot] = stg[OBME_PI_T];
+            s_ts[1][slot] = stg[OBME_PI_S];
+#pragma unroll
+            for (int q = 0; q < OMEGA_CONST_FIELDS; q++) s_oc[q][slot] = so[q];
+            s_h[slot] = sh;
+        }
+    }
+    __syncthreads();
+    FT v[NTR];
+#pragma unroll
+    for (int n = 0; n < NW; n++) v[n] = act ? s_gm[n][tid] : FT(0);
+    v[OBME_PI_T] = act ? s_ts[0][tid] : FT(0);
+    v[OBME_PI_S] = act ? s_ts[1][tid] : FT(0);
+    double H = act ? s_h[tid] : -1.0;      // < 0: the FP32 first guess
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < NW; n++) s_gm[n][tid] = FT(0);
+
+    // the level's geometry
+    const FT* zcv = a.lev;
+    const FT* zfv = a.lev + Nz;
+    const FT zc = zcv[k];
+    const FT idz = a.lev[3 * Nz + k];
+    const FT* wP = a.w;
+    const FT* wG = a.w + (Nz + 1);
+    const FT wPlo = wP[k], wPhi = wP[k + 1], wGlo = wG[k], wGhi = wG[k + 1];
+    // the exponent increment of the light from the level above (the top
+    // level: from the surface, zc of the top cell)
+    const FT dzl = act ? (r == 0 ? zc : zc - zcv[k + 1 < Nz ? k + 1 : k]) : FT(0);
+    const FT* sc = a.pp + OBME_PP_SCALARS;
+    const FT zmxl = sc[OBME_PS_MIXED_LAYER_DEPTH];
+    FT wml;
+    {
+        const FT zk = zfv[k];
+        const FT zk1 = (k + 1 < Nz) ? zfv[k + 1] : FT(a.zf_top);
+        const FT dzk = zk1 - zk;
+        const FT dzk1 = (zk1 > zmxl) ? zk1 - zmxl : FT(0);
+        wml = act ? ((zk >= zmxl) ? dzk : dzk1) : FT(0);
+    }
+    const FT depth = __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, seg_prefix<64>(wml)), 63));
+    const FT thr = a.surface_PAR * FT(1e-3);
+    const FT lthr = m_log(thr);
+    FT zeu_none;
+    {
+        const FT z0 = zcv[0];
+        zeu_none = z0 - (Nz > 1 ? zcv[1] - z0 : FT(1));
+    }
+
+    const bool top = r == 0 && colok;
+    double Hsurf = (top && a.co2.enabled) ? a.Hsurf[col] : 0.0;
+    const bool gas = a.co2.enabled || a.o2.enabled;
+    if (top && gas) {
+#pragma unroll
+        for (int q = 0; q < SURF_PACK; q++) s_surf[wv][q] = f.surf[(unsigned long)q * ncol + col];
+    }
+
+#pragma unroll 1
+    for (int st = 0; st < 3; st++) {
+        // the parameter pointer is made opaque per stage, so that the
+        // compiler reloads the parameters (scalar loads) where the
+        // transitions use them instead of holding them across the loop
+        const FT* pps = a.pp;
+        asm volatile("" : "+s"(pps));
+        if constexpr (SCALE) scale_negatives_cell<NW>(f.ns, v);
+
+        // the top boundary fluxes of DIC and O2 (the surface kernel's solve),
+        // first, while few values are live
+        double Fc = 0.0, Fo = 0.0;
+        if (top && gas) {
+            SurfaceCarbon scb;
+            double kO2, airO2;
+            surface_unpack(s_surf[wv], scb, kO2, airO2);
+            if (a.co2.enabled) {
+                double pH = 0.0;
+                int ev = 0;
+                Fc = surface_flux_warm(scb, a.solver, (double)v[OBME_PI_DIC], (double)v[OBME_PI_ALK],
+                                       a.co2.air_concentration, 1, Hsurf, pH, ev);
+            }
+            if (a.o2.enabled) Fo = kO2 * ((double)v[OBME_PI_O2] - airO2);
+        }
+
+        // MultiBand PAR: band n = PAR0 share · exp(Σ from the top of Δz·att)
+        const FT Chl = v[OBME_PI_PCHL] + v[OBME_PI_DCHL];
+        FT band[3];
+#pragma unroll
+        for (int n = 0; n < 3; n++) {
+            const FT att = a.kw[n] + a.chi[n] * m_pow(Chl, a.e[n]);
+            const FT S = seg_prefix<64>(act ? dzl * att : FT(0));
+            band[n] = f.PAR0div[n] * m_exp(S);
+        }
+        const FT PAR = band[0] + band[1] + band[2];
+
+        // euphotic depth: the first level below the top cell, from the top,
+        // with PAR <= thr, interpolated in log PAR with the level above
+        const FT lPAR = m_log(PAR);
+        const unsigned long long cand = __ballot(act && r >= 1 && PAR <= thr);
+        FT zeu = zeu_none;
+        if (cand) {
+            const int rs = __ffsll((long long)cand) - 1;
+            const FT zk = __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, zc), rs));
+            const FT zu = __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, zc), rs - 1));
+            const FT lk = __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, lPAR), rs));
+            const FT lu = __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, lPAR), rs - 1));
+            zeu = zk + (lthr - lk) * (zk - zu) / (lk - lu);
+            if (!(zeu > FT(-INFINITY) && zeu < FT(INFINITY))) zeu = zeu_none;
+        }
+        // mixed-layer mean of the total PAR
+        const FT acc =
+            __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, seg_prefix<64>(PAR * wml)), 63));
+
+        Aux<FT> A;
+        A.z = zc;
+        A.zmxl = zmxl;
+        A.zeu = zeu;
+        A.kappa = pps[OBME_PP_SCALARS + OBME_PS_KAPPA];
+        A.mlPAR = acc / depth;
+        A.PAR = PAR;
+        A.PAR1 = band[0];
+        A.PAR2 = band[1];
+        A.PAR3 = band[2];
+        A.wPOCc = (wPlo + wPhi) / 2;
+        A.wGOCc = (wGlo + wGhi) / 2;
+        A.Omega = FT(0);
+        if (act) {
+            OmegaConst o;
+            o.conv = s_oc[0][tid];
+            o.K1 = s_oc[1][tid];
+            o.K2 = s_oc[2][tid];
+            o.KB = s_oc[3][tid];
+            o.KW = s_oc[4][tid];
+            o.KS = s_oc[5][tid];
+            o.KF = s_oc[6][tid];
+            o.KSi = s_oc[7][tid];
+            o.boron = s_oc[8][tid];
+            o.sulfate = s_oc[9][tid];
+            o.fluoride = s_oc[10][tid];
+            o.ca = s_oc[11][tid];
+            o.KSP = s_oc[12][tid];
+            const CarbonParams cp =
+                omega_params(o, (double)v[OBME_PI_DIC], (double)v[OBME_PI_ALK], (double)v[OBME_PI_SI]);
+            int ev = 0;
+            if (H < 0) H = (double)guess_H_fp32(cp);
+            if (!refine_H<true>(cp, H, ev, 1e-8)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
+            A.Omega = (FT)omega_from_H(o, cp, H);
+        }
+        FT G[NW];
+        pisces_tendencies(pps, v, A, f.sk[st], G);
+        // flux-form upwind sinking: (w(k+1) c(k+1) - w(k) c(k)) / dz, the
+        // level above from lane r - 1, w = 0 on the top face
+        auto sink = [&](int n, FT wlo, FT whi) {
+            const FT cu = lane_shr1(v[n]);
+            G[n] = G[n] - (whi * cu - wlo * v[n]) * idz;
+        };
+        sink(OBME_PI_POC, wPlo, wPhi);
+        sink(OBME_PI_SFE, wPlo, wPhi);
+        sink(OBME_PI_GOC, wGlo, wGhi);
+        sink(OBME_PI_BFE, wGlo, wGhi);
+        sink(OBME_PI_PSI, wGlo, wGhi);
+        sink(OBME_PI_CACO3, wGlo, wGhi);
+        if (top) {
+            G[OBME_PI_DIC] = G[OBME_PI_DIC] - (FT)Fc * idz;
+            G[OBME_PI_O2] = G[OBME_PI_O2] - (FT)Fo * idz;
+        }
+        const FT As = f.A[st], Bs = f.B[st];
+#pragma unroll
+        for (int n = 0; n < NW; n++) {
+            v[n] = v[n] + fmaf(As, G[n], Bs * s_gm[n][tid]);
+            s_gm[n][tid] = G[n];
+        }
+    }
+
+    // stage out: each lane into its own slots (G⁻ is no longer needed),
+    // then the coalesced stores of the stage-in pattern
+#pragma unroll
+    for (int n = 0; n < NW; n++) s_gm[n][tid] = v[n];
+    s_h[tid] = H;
+    __syncthreads();
+    if (tok) {
+#pragma unroll
+        for (int n = 0; n < NW; n++) a.out[n][pt] = s_gm[n][slot];
+        a.Hs[ct_cell] = s_h[slot];
+    }
+    if (top && a.co2.enabled) a.Hsurf[col] = Hsurf;
+}
+
 }  // namespace
 
 struct obme_pisces {
@@ -358,17 +651,19 @@ struct obme_pisces {
     void* d_H = nullptr;
     void* d_sflux = nullptr;
     void* d_Hsurf = nullptr;
+    void* d_surfc = nullptr;   // [SURF_PACK][Nx*Ny] surface constants of the fused step
     void* d_copy[NW] = {};
     size_t field_bytes = 0;
     std::vector<double> zc, zf, dz;
     bool scaling = false;
+    bool fused = false;        // FP32-mixed with Nz <= 64: k_pisces_fused (OBME_PISCES_FUSED=0 turns it off)
 };
 
 extern "C" obme_status obme_pisces_destroy(obme_pisces* h)
 {
     if (!h) return OBME_OK;
     for (void* p : {h->d_pp, h->d_lev, h->d_w, h->d_PAR, h->d_Omega, h->d_zeu, h->d_mlPAR, h->d_Gm, h->d_oc, h->d_H,
-                    h->d_sflux, h->d_Hsurf})
+                    h->d_sflux, h->d_Hsurf, h->d_surfc})
         if (p) (void)hipFree(p);
     for (void* p : h->d_copy)
         if (p) (void)hipFree(p);
@@ -455,6 +750,9 @@ extern "C" obme_status obme_pisces_create(const obme_pisces_config* cfg, obme_pi
     h->dz.assign(g.dz, g.dz + g.Nz);
     h->cfg.grid.zc = h->cfg.grid.zf = h->cfg.grid.dz = nullptr;
     h->scaling = cfg->scaling.n_groups > 0;
+    // the fused step: FP32-mixed, one wave per column of at most 64 levels
+    h->fused = h->prec == OBME_FP32_MIXED && g.Nz <= 64;
+    if (const char* e = std::getenv("OBME_PISCES_FUSED")) h->fused = h->fused && std::atoi(e) != 0;
     obme_status st = (h->prec == OBME_FP64) ? pisces_upload<double>(h) : pisces_upload<float>(h);
     auto alloc = [&](void** d, size_t bytes) {
         if (st == OBME_OK && hipMalloc(d, bytes) != hipSuccess) st = obme_fail(OBME_ERR_NOMEM, "PISCES buffers");
@@ -701,6 +999,52 @@ extern "C" obme_status obme_pisces_add_tendencies(obme_pisces* h, const void* co
     return pisces_stage_any(h, tracers, G, t, 0.0, 0, 1, 0, (hipStream_t)stream);
 }
 
+// nsteps of the fused FP32-mixed step (k_pisces_fused), one launch per step,
+// in place on the caller's tracers
+static obme_status pisces_fused_steps(obme_pisces* h, void* const* tracers, int nsteps, double dt, double t0,
+                                      hipStream_t s)
+{
+    PiscesArgs<float> a = pisces_args<float>(h, t0);
+    for (int n = 0; n < NTR; n++) a.in[n] = (const float*)tracers[n];
+    for (int n = 0; n < NW; n++) a.out[n] = (float*)tracers[n];
+    PiscesFused f;
+    std::memset(&f, 0, sizeof(f));
+    if (h->scaling) f.ns = pisces_scaling<float>(h->cfg.scaling);
+    const double gam[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0}, zet[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
+    const double cst[3] = {0.0, 8.0 / 15.0, 2.0 / 3.0};
+    for (int q = 0; q < 3; q++) {
+        f.A[q] = (float)(dt * gam[q]);
+        f.B[q] = (float)(dt * zet[q]);
+        f.PAR0div[q] = (float)h->cfg.surface_PAR * (float)h->cfg.band_division[q];
+    }
+    const double lat = h->cfg.params[OBME_PP_SCALARS + OBME_PS_LATITUDE];
+    if (h->cfg.co2.enabled || h->cfg.o2.enabled) {
+        if (!h->d_surfc && hipMalloc(&h->d_surfc, (size_t)SURF_PACK * h->ncol * sizeof(double)) != hipSuccess)
+            return obme_fail(OBME_ERR_NOMEM, "PISCES surface constants");
+        hipLaunchKernelGGL(k_pisces_surface_prep<float>, dim3((unsigned)((h->ncol + 255) / 256)), dim3(256), 0, s, a,
+                           (double*)h->d_surfc);
+        OBME_HIP_TRY(hipGetLastError());
+        f.surf = (const double*)h->d_surfc;
+    }
+    const unsigned nbx = (unsigned)((h->cfg.grid.Nx + PF_WAVES - 1) / PF_WAVES);
+    const unsigned nb = nbx * (unsigned)h->cfg.grid.Ny;
+    for (int st = 0; st < nsteps; st++) {
+        const double t = t0 + st * dt;
+        for (int q = 0; q < 3; q++) {
+            const double tq = t + cst[q] * dt;
+            f.sk[q].dl_growth = (float)host_day_length(lat, tq);  // day_length(φ, t), growth_rate.jl:30
+            f.sk[q].dl_chl = (float)host_day_length(tq, lat);     // day_length(t, φ), growth_rate.jl:128
+        }
+        f.have_H = st > 0 ? 1 : 0;
+        if (h->scaling)
+            hipLaunchKernelGGL(k_pisces_fused<true>, dim3(nb), dim3(PF_WAVES * 64), 0, s, a, f);
+        else
+            hipLaunchKernelGGL(k_pisces_fused<false>, dim3(nb), dim3(PF_WAVES * 64), 0, s, a, f);
+        OBME_HIP_TRY(hipGetLastError());
+    }
+    return OBME_OK;
+}
+
 extern "C" obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, int32_t nsteps, double dt, double t0,
                                         void* stream)
 {
@@ -710,9 +1054,6 @@ extern "C" obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, in
         if (!tracers[n]) return obme_fail(OBME_ERR_INVALID, "null tracer pointer");
     if (nsteps == 0) return OBME_OK;
     hipStream_t s = (hipStream_t)stream;
-    for (int n = 0; n < NW; n++)
-        if (!h->d_copy[n] && hipMalloc(&h->d_copy[n], h->field_bytes) != hipSuccess)
-            return obme_fail(OBME_ERR_NOMEM, "PISCES stage copy");
     if (!h->d_oc && hipMalloc(&h->d_oc, (size_t)OMEGA_CONST_FIELDS * h->ncell * sizeof(double)) != hipSuccess)
         return obme_fail(OBME_ERR_NOMEM, "PISCES carbonate constants");
     if (!h->d_H && hipMalloc(&h->d_H, (size_t)h->ncell * sizeof(double)) != hipSuccess)
@@ -721,6 +1062,10 @@ extern "C" obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, in
         obme_status r = (h->prec == OBME_FP64) ? pisces_prep<double>(h, tracers, s) : pisces_prep<float>(h, tracers, s);
         if (r != OBME_OK) return r;
     }
+    if (h->fused) return pisces_fused_steps(h, tracers, nsteps, dt, t0, s);
+    for (int n = 0; n < NW; n++)
+        if (!h->d_copy[n] && hipMalloc(&h->d_copy[n], h->field_bytes) != hipSuccess)
+            return obme_fail(OBME_ERR_NOMEM, "PISCES stage copy");
     // Stage inputs and outputs alternate between the caller's arrays (U) and
     // library copies (C), which start as full copies of U so that their halos
     // match; the kernels write interiors only.  A step ends on the other side

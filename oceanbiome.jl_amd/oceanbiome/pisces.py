@@ -126,8 +126,11 @@ class PISCES:
         or True for the defaults; validation/PISCES/column.jl:63-64)."""
         from .models import CarbonDioxideGasExchangeBoundaryCondition, OxygenGasExchangeBoundaryCondition
         self.grid = grid
-        self.co2_exchange = CarbonDioxideGasExchangeBoundaryCondition() if co2_exchange is True else co2_exchange
-        self.o2_exchange = OxygenGasExchangeBoundaryCondition() if o2_exchange is True else o2_exchange
+        def choose(x, make):
+            return make() if x is True else (None if x is False else x)
+
+        self.co2_exchange = choose(co2_exchange, CarbonDioxideGasExchangeBoundaryCondition)
+        self.o2_exchange = choose(o2_exchange, OxygenGasExchangeBoundaryCondition)
         self.scale_negatives = bool(scale_negatives)
         self.invalid_fill_value = float(invalid_fill_value)
         self.surface_PAR = float(surface_PAR)
