@@ -155,14 +155,7 @@ def main():
             outputs[prec] = [t.double().cpu().numpy() for t in tr]
         del tr, dev
         torch.cuda.empty_cache()
-    if args.calibrate_copy_mb:
-        n = args.calibrate_copy_mb * (1 << 20) // 4
-        src = torch.ones(n, device="cuda", dtype=torch.float32)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"calibration copy: {n * 4} bytes read, {n * 4} bytes written")
-        del src, dst
+    calibration_copy(args)
 
     errors = {}
     cpu = None
@@ -403,6 +396,19 @@ def _pisces_oracle_inputs(wl, cols=None):
     return OP, q, light, F, np.asarray(g.zc), np.asarray(g.zf), np.asarray(g.dz), solver
 
 
+def calibration_copy(args):
+    """A device copy of known size (--calibrate-copy-mb) for the HBM counters."""
+    import torch
+    if args.calibrate_copy_mb:
+        n = args.calibrate_copy_mb * (1 << 20) // 4
+        src = torch.ones(n, device="cuda", dtype=torch.float32)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {n * 4} bytes read, {n * 4} bytes written")
+        del src, dst
+
+
 def bench_pisces(args):
     """PISCES on 256x256x64 (MultiBand PAR, Omega, 24 tendencies, sinking,
     RK3), 300 s steps: FP32-mixed (FP64 Omega) and FP64 timed on the device;
@@ -417,6 +423,7 @@ def bench_pisces(args):
     g = wl.grid
     ncell = g.n_interior
     nw = 24
+    calibration_copy(args)
     res, outputs = {}, {}
     for prec in args.precisions.split(","):
         dev = HIPPISCES(wl.bgc, prec)

@@ -505,11 +505,7 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
 
 #pragma unroll 1
     for (int st = 0; st < 3; st++) {
-        // the parameter pointer is made opaque per stage, so that the
-        // compiler reloads the parameters (scalar loads) where the
-        // transitions use them instead of holding them across the loop
         const FT* pps = a.pp;
-        asm volatile("" : "+s"(pps));
         if constexpr (SCALE) scale_negatives_cell<NW>(f.ns, v);
 
         // the top boundary fluxes of DIC and O2 (the surface kernel's solve),
