@@ -96,3 +96,16 @@ def test_deep_columns_take_the_per_stage_path():
     errs = {name: rel_l2(g.interior(out[n]), F[name]) for n, name in enumerate(OP.WRITTEN)}
     print("errors", {k: float(v) for k, v in errs.items()}, flush=True)
     assert all(e <= TOL for e in errs.values()), errs
+
+
+def test_fused_step_with_non_default_parameters():
+    """A parameter set other than the defaults (a shallower mixed layer)."""
+    print("case pisces fused non-default parameters", flush=True)
+    wl = workloads.pisces_3d(5, 3, 64, nsteps=2, dt=300.0, seed=31, mixed_layer_depth=-30.0)
+    q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
+    OP.step(q, light, F, zc, zf, dz, wl.nsteps, wl.dt, 0.0, solver)
+    out = _device_run(wl, 1, wl.nsteps)
+    g = wl.grid
+    errs = {name: rel_l2(g.interior(out[n]), F[name]) for n, name in enumerate(OP.WRITTEN)}
+    print("errors", {k: float(v) for k, v in errs.items()}, flush=True)
+    assert all(e <= TOL for e in errs.values()), errs
