@@ -8,6 +8,14 @@
  * to the OceanBioME.jl source tree).  Evaluation order follows the Julia
  * expressions left to right; the build uses -ffp-contract=off so no FMA is
  * formed where Julia would not form one.
+ *
+ * Parity unpinned (no reference-held value or closed form fixes the bits):
+ * orc_teos10_density away from the reference's doctest point; the exact
+ * per-tracer tendency values of orc_npd_tendencies away from the budgets
+ * the tests check; the RK3 coefficients of orc_column_run beyond the
+ * amplification polynomial; the step sequence of orc_sediment_time_step
+ * (Oceananigans' QuasiAdamsBashforth2) beyond the hand-worked sequence and
+ * the nitrogen budget.
  */
 #ifndef OBME_ORACLE_H
 #define OBME_ORACLE_H

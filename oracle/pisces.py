@@ -11,6 +11,10 @@ np.maximum), eps(0.0) = 5e-324 as the division guard, literal x^2 and x^3 as
 products, float^float through pow, left-to-right evaluation of products.
 The calcite saturation per cell uses the C oracle's carbonate chemistry
 (pinned separately by the reference's doctests and known answers).
+
+Parity unpinned: the exact per-tracer transition values away from the
+conservation balances and known answers the tests check, and the Morel
+coefficient table beyond the ranges of its band means.
 """
 import math
 
