@@ -284,9 +284,10 @@ __global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES :
             H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
         } else {
             H = a.have_H ? a.Hs[c] : (double)guess_H_fp32(cp);
-            // Ω enters the FP32 transitions: a step below 1e-8 relative leaves H
-            // accurate to FP64 rounding
-            if (!refine_H<true>(cp, H, ev, 1e-8)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
+            // Ω enters the FP32 transitions: after a step below 1e-4 relative,
+            // quadratic convergence leaves H within ~1e-8 of the root, below
+            // the FP32 rounding of Ω
+            if (!refine_H<true>(cp, H, ev, 1e-4)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
             a.Hs[c] = H;
         }
         A.Omega = (FT)omega_from_H(o, cp, H);
@@ -585,7 +586,7 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
                 omega_params(o, (double)v[OBME_PI_DIC], (double)v[OBME_PI_ALK], (double)v[OBME_PI_SI]);
             int ev = 0;
             if (H < 0) H = (double)guess_H_fp32(cp);
-            if (!refine_H<true>(cp, H, ev, 1e-8)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
+            if (!refine_H<true>(cp, H, ev, 1e-4)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
             A.Omega = (FT)omega_from_H(o, cp, H);
         }
         FT G[NW];
