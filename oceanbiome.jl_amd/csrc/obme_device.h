@@ -755,6 +755,10 @@ __device__ __forceinline__ void alk_residual_and_derivative_fast(double H, const
 // converges quadratically here, so the error left after a step of relative
 // size tol is of order tol^2 (tol = 1e-8 leaves ~1e-16, the FP64 rounding
 // of H; the default 1e-14 takes one confirming step more).
+// The stop of the FP32-mixed paths, whose [H+] feeds FP32 arithmetic: after
+// a step below 1e-4 relative, H is within ~1e-8 of the root (measured on
+// the CPU oracle from warm starts off by up to 1e-2: at most 9e-9).
+constexpr double FP32_NEWTON_TOL = 1e-4;
 template <bool SI = false>
 __device__ __forceinline__ bool refine_H(const CarbonParams& p, double& x, int& evals, double tol = 1e-14)
 {
@@ -879,20 +883,21 @@ __device__ __forceinline__ double co2_flux(const obme_co2_exchange& g, const obm
 }
 
 // Flux with the mixed path's warm start: H from the previous stage (or an
-// FP32 guess) refined by FP64 Newton, with the reference solve as fallback.
+// FP32 guess) refined by FP64 Newton (stopping after a step below tol
+// relative, see refine_H), with the reference solve as fallback.
 __device__ __forceinline__ double surface_flux_warm(const SurfaceCarbon& sc, const obme_carbon_solver& s,
                                                     double DIC, double Alk, double pCO2_air, int mode,
-                                                    double& H, double& pH, int& evals)
+                                                    double& H, double& pH, int& evals, double tol = 1e-14)
 {
     if (mode == 0) return surface_flux<true>(sc, s, DIC, Alk, pCO2_air, 0, pH, evals);
     CarbonParams p = sc.p;
     p.Alk = Alk * sc.conv;
     p.DIC = DIC * sc.conv;
     double x = (H > 0.0) ? H : (double)guess_H_fp32(p);
-    bool done = refine_H(p, x, evals);
+    bool done = refine_H(p, x, evals, tol);
     if (!done) {
         x = (double)guess_H_fp32(p);
-        done = refine_H(p, x, evals);
+        done = refine_H(p, x, evals, tol);
         if (!done) x = solve_for_H(p, s, pow(10.0, -s.initial_pH_guess), evals);
     }
     H = x;

@@ -587,7 +587,8 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
                         double pH = 0.0;
                         int ev = 0;
                         const double F = surface_flux_warm(sc, a.solver, (double)val(7, 0), (double)val(8, 0),
-                                                           a.co2.air_concentration, F32 ? 1 : 0, Hc, pH, ev);
+                                                           a.co2.air_concentration, F32 ? 1 : 0, Hc, pH, ev,
+                                                           F32 ? FP32_NEWTON_TOL : 1e-14);
                         Gv[7] = Gv[7] - (FT)F * a.invdz_top;
                     }
                 }
@@ -824,7 +825,7 @@ __global__ void __launch_bounds__(256) k_surface(SurfaceArgs<FT> a)
     int ev = 0;
     if constexpr (MODE == TILE_TENDENCY) {
         F = surface_flux_warm(sc, a.solver, (double)a.DIC[top], (double)a.Alk[top], a.co2.air_concentration, a.mode,
-                              H, pH, ev);
+                              H, pH, ev, sizeof(FT) == 4 ? FP32_NEWTON_TOL : 1e-14);
         a.GDIC[top] = a.GDIC[top] - (FT)F * a.invdz_top;
     } else {
         // the identical update expressions of the sweep, with the flux term
@@ -837,7 +838,7 @@ __global__ void __launch_bounds__(256) k_surface(SurfaceArgs<FT> a)
                 const float Gb = a.srec[(1 + st) * ncol + col];
                 const float Alk = a.srec[(4 + st) * ncol + col];
                 F = surface_flux_warm(sc, a.solver, (double)(U + D), (double)Alk, a.co2.air_concentration, a.mode,
-                                      H, pH, ev);
+                                      H, pH, ev, FP32_NEWTON_TOL);
                 const float G = Gb - (float)F * a.invdz_top;
                 D = fmaf(a.rk_a[st], G, fmaf(a.rk_b[st], Gm, D));
                 Gm = G;

@@ -193,7 +193,8 @@ __global__ void __launch_bounds__(256) k_pisces_surface(PiscesArgs<FT> a)
         double pH = 0.0;
         int ev = 0;
         Fc = surface_flux_warm(sc, a.solver, (double)a.in[OBME_PI_DIC][p], (double)a.in[OBME_PI_ALK][p],
-                               a.co2.air_concentration, sizeof(FT) == 4 ? 1 : 0, H, pH, ev);
+                               a.co2.air_concentration, sizeof(FT) == 4 ? 1 : 0, H, pH, ev,
+                               sizeof(FT) == 4 ? FP32_NEWTON_TOL : 1e-14);
         if (sizeof(FT) == 4) a.Hsurf[col] = H;
     }
     if (a.o2.enabled) Fo = o2_flux(a.o2, (double)a.in[OBME_PI_O2][p], T, S);
@@ -287,7 +288,7 @@ __global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES :
             // Ω enters the FP32 transitions: after a step below 1e-4 relative,
             // quadratic convergence leaves H within ~1e-8 of the root, below
             // the FP32 rounding of Ω
-            if (!refine_H<true>(cp, H, ev, 1e-4)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
+            if (!refine_H<true>(cp, H, ev, FP32_NEWTON_TOL)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
             a.Hs[c] = H;
         }
         A.Omega = (FT)omega_from_H(o, cp, H);
@@ -520,7 +521,7 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
                 double pH = 0.0;
                 int ev = 0;
                 Fc = surface_flux_warm(scb, a.solver, (double)v[OBME_PI_DIC], (double)v[OBME_PI_ALK],
-                                       a.co2.air_concentration, 1, Hsurf, pH, ev);
+                                       a.co2.air_concentration, 1, Hsurf, pH, ev, FP32_NEWTON_TOL);
             }
             if (a.o2.enabled) Fo = kO2 * ((double)v[OBME_PI_O2] - airO2);
         }
@@ -586,7 +587,7 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
                 omega_params(o, (double)v[OBME_PI_DIC], (double)v[OBME_PI_ALK], (double)v[OBME_PI_SI]);
             int ev = 0;
             if (H < 0) H = (double)guess_H_fp32(cp);
-            if (!refine_H<true>(cp, H, ev, 1e-4)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
+            if (!refine_H<true>(cp, H, ev, FP32_NEWTON_TOL)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
             A.Omega = (FT)omega_from_H(o, cp, H);
         }
         FT G[NW];
