@@ -121,8 +121,7 @@ __device__ __forceinline__ void batch_columns(const BatchArgs<FT>& a)
                         pb[m] = m_exp2(fmaf(a.eb_f, lg, a.cb));
                     } else {
                         const double x = val(1, m) * a.Rr;
-                        pr[m] = m_pow(x, a.er);
-                        pb[m] = m_pow(x, a.eb);
+                        pow_pair(x, a.er, a.eb, pr[m], pb[m]);
                     }
                 }
                 const FT pr_up = lane_shr1(pr[L - 1]);

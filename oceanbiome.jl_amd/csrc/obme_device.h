@@ -73,6 +73,21 @@ __device__ __forceinline__ float m_exp(float x) { return __builtin_amdgcn_exp2f(
 __device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float m_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double m_pow(double x, double y) { return pow(x, y); }
+// x^e1 and x^e2 of one base in FP64 as exp(e log x) with the logarithm
+// shared: within a few ulp of pow (the rounding of e log x, |e log x| of
+// order 10, is what exp amplifies), at a third of two pow calls.  x <= 0,
+// NaN and a zero exponent take pow's own special cases.
+__device__ __forceinline__ void pow_pair(double x, double e1, double e2, double& r1, double& r2)
+{
+    if (x > 0.0 && e1 != 0.0 && e2 != 0.0) {
+        const double L = log(x);
+        r1 = exp(e1 * L);
+        r2 = exp(e2 * L);
+    } else {
+        r1 = pow(x, e1);
+        r2 = pow(x, e2);
+    }
+}
 // x^y for x >= 0 as 2^(y log2 x); a negative x gives NaN, as on the GPU
 // reference path.
 __device__ __forceinline__ float m_pow(float x, float y) { return m_exp2(y * m_log2(x)); }
