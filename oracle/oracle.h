@@ -1,7 +1,7 @@
 /*
  * oracle.h — FP64 CPU restatement of OceanBioME.jl v0.17.6's biogeochemistry
- * step.  TEST INFRASTRUCTURE ONLY: this code is the checker for the HIP
- * library.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * step.  TEST INFRASTRUCTURE ONLY: this code is the oracle the HIP library
+ * is tested against.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
  * leg may load it; nothing in the product links or calls it.
  *
  * Every function cites the reference file:line it restates (paths relative

@@ -1,6 +1,6 @@
 """ctypes wrapper of the FP64 oracle (liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP library.  Only tests/,
+TEST INFRASTRUCTURE ONLY: the oracle of the HIP library.  Only tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
 """
 import ctypes as C

@@ -1,6 +1,6 @@
 /*
  * oracle_carbon.c — FP64 restatement of OceanBioME's CarbonChemistry and CO2
- * GasExchange path.  TEST INFRASTRUCTURE ONLY (the checker; see oracle.h).
+ * GasExchange path.  TEST INFRASTRUCTURE ONLY (the oracle; see oracle.h).
  *
  * Third-party arithmetic restated here:
  *  - SeawaterPolynomials.jl 0.3 (Project.toml:28), TEOS10._ρ(τ, s, ζ): the

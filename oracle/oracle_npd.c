@@ -2,7 +2,7 @@
  * oracle_npd.c — FP64 restatement of OceanBioME's NutrientsPlanktonDetritus
  * tendencies (NPZD and LOBSTER + CarbonateSystem), the TwoBand PAR column
  * recurrence, the sinking flux divergence and the RK3 stage sequence.
- * TEST INFRASTRUCTURE ONLY (the checker; see oracle.h).
+ * TEST INFRASTRUCTURE ONLY (the oracle; see oracle.h).
  *
  * Reference files (relative to OceanBioME.jl/src):
  *   Models/AdvectedPopulations/NutrientsPlanktonDetritus/{plankton,nutrients,

@@ -2,7 +2,7 @@
 v0.17.6 (src/Models/AdvectedPopulations/PISCES/**, src/Light/multi_band.jl,
 src/Light/compute_euphotic_depth.jl, src/Utils/Utils.jl CBMDayLength).
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP library's PISCES path.
+TEST INFRASTRUCTURE ONLY: the oracle of the HIP library's PISCES path.
 Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use it.
 
 Every function is vectorised over cells: arguments are numpy arrays of any
