@@ -74,6 +74,37 @@ struct Aux {
     FT z, zmxl, zeu, kappa, mlPAR, Omega, PAR, PAR1, PAR2, PAR3, wPOCc, wGOCc;
 };
 
+// The factors of the transitions that depend on the temperature alone: the
+// growth rates' bᵀ of the two phytoplankton and two zooplankton classes
+// (growth_rate.jl:25, food_quality_dependant.jl:157), of DOC and POM
+// degradation (dissolved_organic_carbon.jl:62, two_size_class.jl:154), the
+// rain ratio's 1 + exp(-(T-10)²/25) (nano_diatom_coupling.jl), the ligand
+// constant of the free iron (iron/iron.jl:29), the silicate equilibrium
+// 10^(6.44 - 968/T_K) and (1 + T/400)^4 (silicate.jl:30-34).  The same
+// expressions whether evaluated per stage or once per stepper call.
+template <typename FT>
+struct TFac {
+    FT btN, btD, bZ, bM, doc, pom, rain, kfe, eq, t4;
+};
+constexpr int TFAC_FIELDS = 10;
+
+template <typename FT>
+__device__ __forceinline__ void temperature_factors(const FT* pp, FT T, TFac<FT>& f)
+{
+    const FT* sc = pp + OBME_PP_SCALARS;
+    f.btN = m_pow(pp[OBME_PP_NANO + OBME_PH_BT], T);
+    f.btD = m_pow(pp[OBME_PP_DIATOMS + OBME_PH_BT], T);
+    f.bZ = m_pow(pp[OBME_PP_MICRO + OBME_PZ_B], T);
+    f.bM = m_pow(pp[OBME_PP_MESO + OBME_PZ_B], T);
+    f.doc = m_pow(sc[OBME_PS_DOC_B], T);
+    f.pom = m_pow(sc[OBME_PS_POM_B], T);
+    const FT Tm = T - 10;
+    f.rain = 1 + m_exp(-(Tm * Tm) / 25);
+    f.kfe = m_exp(FT(16.27) - FT(1565.7) / jmax(T + FT(273.15), FT(5)));
+    f.eq = m_pow(FT(10), FT(6.44) - 968 / (T + FT(273.15)));
+    f.t4 = m_pow(1 + T / 400, FT(4));
+}
+
 // One phytoplankton class (parameters at block b of pp).
 template <typename FT>
 struct Phyto {
@@ -101,10 +132,9 @@ __device__ __forceinline__ FT nitrogen_lim(FT N1, FT N2, FT K1, FT K2)
 // the growth rate (growth_rate.jl:3-47, GrowthRespirationLimitedProduction :104-111).
 template <typename FT>
 __device__ __forceinline__ void phyto_state(const FT* ph, const FT* sc, const FT* v, const Aux<FT>& A,
-                                            const StageConst<FT>& sk, FT I, FT IChl, FT IFe, Phyto<FT>& s)
+                                            const StageConst<FT>& sk, FT btT, FT I, FT IChl, FT IFe, Phyto<FT>& s)
 {
     const FT NO3 = v[OBME_PI_NO3], NH4 = v[OBME_PI_NH4], PO4 = v[OBME_PI_PO4], Si = v[OBME_PI_SI];
-    const FT T = v[OBME_PI_T];
     s.I = I;
     s.IChl = IChl;
     s.IFe = IFe;
@@ -132,7 +162,7 @@ __device__ __forceinline__ void phyto_state(const FT* ph, const FT* sc, const FT
     const FT dl = sk.dl_growth;
     const FT d = jmax(FT(0), A.zeu - A.zmxl);
     const FT drt = d * d / A.kappa;
-    const FT mui = ph[OBME_PH_MU0] * m_pow(ph[OBME_PH_BT], T);
+    const FT mui = ph[OBME_PH_MU0] * btT;
     const FT f1 = FT(1.5) * dl / (dl + FT(0.5 * DAYS));
     const FT f2 = 1 - drt / (drt + ph[OBME_PH_DARK_TOLERANCE]);
     const FT alpha = ph[OBME_PH_ALPHA0] * (1 + ph[OBME_PH_BETA] * m_exp(-PAR));
@@ -172,11 +202,11 @@ __device__ __forceinline__ FT chlorophyll_growth(const FT* ph, const Phyto<FT>& 
 
 // iron uptake (mixed_mondo.jl:161-181)
 template <typename FT>
-__device__ __forceinline__ FT iron_uptake(const FT* ph, const Phyto<FT>& s, FT T, FT Fe)
+__device__ __forceinline__ FT iron_uptake(const FT* ph, const Phyto<FT>& s, FT btT, FT Fe)
 {
     const FT fe_max = ph[OBME_PH_FE_MAX];
     const FT thFe = s.IFe / (s.I + eps0<FT>());
-    const FT mui = ph[OBME_PH_MU0] * m_pow(ph[OBME_PH_BT], T);
+    const FT mui = ph[OBME_PH_MU0] * btT;
     const FT K = ph[OBME_PH_KFE_UPTAKE] * size_factor(ph, s.I);
     const FT L1 = Fe / (Fe + K + eps0<FT>());
     const FT L2 = 4 - FT(4.5) * s.LFe / (s.LFe + 1);
@@ -186,9 +216,9 @@ __device__ __forceinline__ FT iron_uptake(const FT* ph, const Phyto<FT>& s, FT T
 
 // silicate uptake (mixed_mondo.jl:195-226)
 template <typename FT>
-__device__ __forceinline__ FT silicate_uptake(const FT* ph, const FT* sc, const Phyto<FT>& s, FT T, FT Si)
+__device__ __forceinline__ FT silicate_uptake(const FT* ph, const FT* sc, const Phyto<FT>& s, FT btT, FT Si)
 {
-    const FT mui = ph[OBME_PH_MU0] * m_pow(ph[OBME_PH_BT], T);
+    const FT mui = ph[OBME_PH_MU0] * btT;
     const FT L1 = Si / (Si + ph[OBME_PH_SI_K1] + eps0<FT>());
     const FT K2 = ph[OBME_PH_SI_K2];
     const FT L2 = (sc[OBME_PS_LATITUDE] < FT(0)) ? (Si * Si * Si) / (Si * Si * Si + K2 * K2 * K2) : FT(0);
@@ -206,11 +236,11 @@ struct Zoo {
 };
 
 template <typename FT, int NPREY>
-__device__ __forceinline__ void zoo_state(const FT* zp, const FT* v, FT micro_iron, FT wPOCc, FT wGOCc, FT I, Zoo<FT>& g)
+__device__ __forceinline__ void zoo_state(const FT* zp, const FT* v, FT bT, FT micro_iron, FT wPOCc, FT wGOCc, FT I,
+                                          Zoo<FT>& g)
 {
-    const FT T = v[OBME_PI_T];
     g.I = I;
-    g.tf = m_pow(zp[OBME_PZ_B], T);
+    g.tf = bT;
     const FT base = zp[OBME_PZ_G0] * g.tf;
     const FT food[4] = {v[OBME_PI_P], v[OBME_PI_D], v[OBME_PI_POC], v[OBME_PI_Z]};
     const FT iron[4] = {v[OBME_PI_PFE] / (v[OBME_PI_P] + eps0<FT>()), v[OBME_PI_DFE] / (v[OBME_PI_D] + eps0<FT>()),
@@ -258,7 +288,7 @@ __device__ __forceinline__ FT anoxia(const FT* sc, FT O2)
 // The 24 transitions of one cell (v: tracers in OBME_PI order).
 template <typename FT>
 __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, const Aux<FT>& A,
-                                                  const StageConst<FT>& sk, FT* G)
+                                                  const StageConst<FT>& sk, const TFac<FT>& tf, FT* G)
 {
     const FT* phN = pp + OBME_PP_NANO;
     const FT* phD = pp + OBME_PP_DIATOMS;
@@ -269,12 +299,12 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
     const FT e0 = eps0<FT>();
 
     Phyto<FT> sP, sD;
-    phyto_state(phN, sc, v, A, sk, v[OBME_PI_P], v[OBME_PI_PCHL], v[OBME_PI_PFE], sP);
-    phyto_state(phD, sc, v, A, sk, v[OBME_PI_D], v[OBME_PI_DCHL], v[OBME_PI_DFE], sD);
+    phyto_state(phN, sc, v, A, sk, tf.btN, v[OBME_PI_P], v[OBME_PI_PCHL], v[OBME_PI_PFE], sP);
+    phyto_state(phD, sc, v, A, sk, tf.btD, v[OBME_PI_D], v[OBME_PI_DCHL], v[OBME_PI_DFE], sD);
     Zoo<FT> gZ, gM;
     const FT micro_iron = zpZ[OBME_PZ_IRON_RATIO];
-    zoo_state<FT, 3>(zpZ, v, micro_iron, A.wPOCc, A.wGOCc, v[OBME_PI_Z], gZ);
-    zoo_state<FT, 4>(zpM, v, micro_iron, A.wPOCc, A.wGOCc, v[OBME_PI_M], gM);
+    zoo_state<FT, 3>(zpZ, v, tf.bZ, micro_iron, A.wPOCc, A.wGOCc, v[OBME_PI_Z], gZ);
+    zoo_state<FT, 4>(zpM, v, tf.bM, micro_iron, A.wPOCc, A.wGOCc, v[OBME_PI_M], gM);
     const FT O2 = v[OBME_PI_O2];
     const FT an = anoxia(sc, O2);
     // zooplankton mortality (food_quality_dependant.jl:201-227)
@@ -304,7 +334,7 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
         const FT thChl = sP.IChl / (12 * sP.I + e0);
         G[OBME_PI_PCHL] = chlorophyll_growth(phN, sP, A, sk) - (death + grazP) * thChl * 12;
         const FT thFe = sP.IFe / (sP.I + e0);
-        G[OBME_PI_PFE] = iron_uptake(phN, sP, T, v[OBME_PI_FE]) - (death + grazP) * thFe;
+        G[OBME_PI_PFE] = iron_uptake(phN, sP, tf.btN, v[OBME_PI_FE]) - (death + grazP) * thFe;
     }
     {
         const FT death = Dlin + Dquad;
@@ -312,9 +342,9 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
         const FT thChl = sD.IChl / (12 * sD.I + e0);
         G[OBME_PI_DCHL] = chlorophyll_growth(phD, sD, A, sk) - (death + grazD) * thChl * 12;
         const FT thFe = sD.IFe / (sD.I + e0);
-        G[OBME_PI_DFE] = iron_uptake(phD, sD, T, v[OBME_PI_FE]) - (death + grazD) * thFe;
+        G[OBME_PI_DFE] = iron_uptake(phD, sD, tf.btD, v[OBME_PI_FE]) - (death + grazD) * thFe;
     }
-    const FT si_uptake = silicate_uptake(phD, sc, sD, T, v[OBME_PI_SI]);
+    const FT si_uptake = silicate_uptake(phD, sc, sD, tf.btD, v[OBME_PI_SI]);
     const FT thSi = v[OBME_PI_DSI] / (v[OBME_PI_D] + e0);
     G[OBME_PI_DSI] = si_uptake - (Dlin + Dquad + grazD) * thSi;
 
@@ -346,13 +376,13 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
                      * (DOC / (DOC + sc[OBME_PS_K_DOC]));
 
     // DOC (dissolved_organic_carbon.jl:39-106)
-    const FT doc_deg = sc[OBME_PS_DOC_LAMBDA] * m_pow(sc[OBME_PS_DOC_B], T) * LBact * Bact / sc[OBME_PS_DOC_BACT_REF] * DOC;
+    const FT doc_deg = sc[OBME_PS_DOC_LAMBDA] * tf.doc * LBact * Bact / sc[OBME_PS_DOC_BACT_REF] * DOC;
     const FT POC = v[OBME_PI_POC], GOC = v[OBME_PI_GOC];
     const FT Phi1 = shear_lt * (sc[OBME_PS_DOC_A1] * DOC + sc[OBME_PS_DOC_A2] * POC) * DOC;
     const FT Phi2 = shear_lt * (sc[OBME_PS_DOC_A3] * GOC) * DOC;
     const FT Phi3 = (sc[OBME_PS_DOC_A4] * POC + sc[OBME_PS_DOC_A5] * DOC) * DOC;
     const FT doc_agg = Phi1 + Phi2 + Phi3;
-    const FT lam_pom = sc[OBME_PS_POM_LAMBDA] * m_pow(sc[OBME_PS_POM_B], T) * (1 - FT(0.45) * an);
+    const FT lam_pom = sc[OBME_PS_POM_LAMBDA] * tf.pom * (1 - FT(0.45) * an);
     const FT pom_deg_POC = lam_pom * POC;
     const FT exudate = phN[OBME_PH_DELTA] * sP.muI + phD[OBME_PH_DELTA] * sD.muI;
     G[OBME_PI_DOC] = exudate + ut_excretion + org_exc + pom_deg_POC - doc_deg - doc_agg;
@@ -365,10 +395,9 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
                        + sc[OBME_PS_POM_A3] * POC * GOC + sc[OBME_PS_POM_A4] * (POC * POC);
     const FT PAR = A.PAR;
     const FT L_CaCO3 = jmin3(sP.LN, Fe / (Fe + FT(0.05)), sP.LPO4);
-    const FT Tm = T - 10;
     const FT R = sc[OBME_PS_RAIN_RATIO] * L_CaCO3 * jmax(FT(1), v[OBME_PI_P] / 2)
                  * (jmax(FT(0), PAR - 1) / (4 + PAR)) * (30 / (30 + PAR)) * jmax(FT(0), T / (T + FT(0.1)))
-                 * (1 + m_exp(-(Tm * Tm) / 25)) * jmin(FT(1), -50 / A.zmxl);
+                 * tf.rain * jmin(FT(1), -50 / A.zmxl);
     const FT small_mort = (1 - R / 2) * (Plin + Pquad) + Dlin / 2;
     const FT large_mort = R / 2 * (Plin + Pquad) + Dlin / 2 + Dquad;
     const FT thP = v[OBME_PI_PFE] / (v[OBME_PI_P] + e0);
@@ -387,14 +416,14 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
 
     // iron in particles (iron.jl:2-89) with the free iron of SimpleIron (iron/iron.jl:25-37)
     const FT ligands = jmax(FT(0.6), FT(0.09) * (DOC + 40) - 3);
-    const FT Kfe = m_exp(FT(16.27) - FT(1565.7) / jmax(T + FT(273.15), FT(5)));
+    const FT Kfe = tf.kfe;
     const FT Dl = 1 + Kfe * ligands - Kfe * Fe;
     const FT Fef = (-Dl + m_sqrt(Dl * Dl + 4 * Kfe * Fe)) / (2 * Kfe);
     const FT colloidal = FT(0.5) * (Fe - Fef);
     const FT CgFe1 = (Phi1 + Phi3) * colloidal / (DOC + e0);
     const FT CgFe2 = Phi2 * colloidal / (DOC + e0);
     const FT lamFe = sc[OBME_PS_FE_SCAV_MIN] + sc[OBME_PS_FE_SCAV_LOAD] * (POC + GOC + v[OBME_PI_CACO3] + v[OBME_PI_PSI]);
-    const FT BactFe = sc[OBME_PS_BACT_MU0] * m_pow(sc[OBME_PS_POM_B], T) * LBact * sc[OBME_PS_BACT_FE_MAX] * Fe
+    const FT BactFe = sc[OBME_PS_BACT_MU0] * tf.pom * LBact * sc[OBME_PS_BACT_FE_MAX] * Fe
                       / (Fe + sc[OBME_PS_BACT_FE_K]) * Bact * sc[OBME_PS_BACT_FE_EFF];
     const FT thS = v[OBME_PI_SFE] / (POC + e0);
     const FT thB = v[OBME_PI_BFE] / (GOC + e0);
@@ -409,9 +438,9 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
     const FT chi = sc[OBME_PS_CHI0] * ((A.z >= zm) ? FT(1)
                                                    : m_exp((sc[OBME_PS_SI_FAST] - sc[OBME_PS_SI_SLOW]) * (zm - A.z) / A.wGOCc));
     const FT lam0 = chi * sc[OBME_PS_SI_FAST] + (1 - chi) * sc[OBME_PS_SI_SLOW];
-    const FT eq = m_pow(FT(10), FT(6.44) - 968 / (T + FT(273.15)));
+    const FT eq = tf.eq;
     const FT sat = (eq - v[OBME_PI_SI]) / eq;
-    const FT t4 = m_pow(1 + T / 400, FT(4));
+    const FT t4 = tf.t4;
     const FT lam_si = lam0 * (FT(0.225) * (1 + T / 15) * sat + FT(0.775) * m_pow(t4 * sat, FT(9)));
     const FT si_diss = lam_si * v[OBME_PI_PSI];
     G[OBME_PI_PSI] = (grazD + Dlin + Dquad) * thSi - si_diss;
@@ -428,7 +457,7 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
     const FT thN = sc[OBME_PS_THETAN];
     const FT nitrif = sc[OBME_PS_NITRIF] * NH4 / (1 + A.mlPAR) * (1 - an);
     const FT fix_limit = (sP.LN >= FT(0.8)) ? FT(0.01) : 1 - sP.LN;
-    const FT mu_nano = phN[OBME_PH_MU0] * m_pow(phN[OBME_PH_BT], T);
+    const FT mu_nano = phN[OBME_PH_MU0] * tf.btN;
     const FT growth_req = jmax(FT(0), mu_nano - FT(2.15));
     const FT nut_lim = jmin(Fe / (Fe + sc[OBME_PS_FIX_KFE]), PO4 / (PO4 + sc[OBME_PS_FIX_KPO4]));
     const FT light_lim = 1 - m_exp(-PAR / sc[OBME_PS_FIX_E]);
@@ -448,7 +477,7 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
     const FT scav = lamFe * (POC + GOC) * Fef;
     const FT naZ = (gZ.gIFe + gZ.ffFe) - sigZ * (gZ.gIFe + gZ.ffFe) - zpZ[OBME_PZ_IRON_RATIO] * gZ.ge * (gZ.gI + gZ.ff);
     const FT naM = (gM.gIFe + gM.ffFe) - sigM * (gM.gIFe + gM.ffFe) - zpM[OBME_PZ_IRON_RATIO] * gM.ge * (gM.gI + gM.ff);
-    const FT up_Fe = iron_uptake(phN, sP, T, Fe) + iron_uptake(phD, sD, T, Fe);
+    const FT up_Fe = iron_uptake(phN, sP, tf.btN, Fe) + iron_uptake(phD, sD, tf.btD, Fe);
     G[OBME_PI_FE] = (lam_pom * v[OBME_PI_SFE] + (naZ + naM) + zpM[OBME_PZ_IRON_RATIO] * R_ut
                      - up_Fe - lig_agg - (CgFe1 + CgFe2) - scav - BactFe);
 

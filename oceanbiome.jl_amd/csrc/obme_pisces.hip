@@ -40,6 +40,7 @@ struct PiscesArgs {
     FT* PAR;                // [4][ncell] compact: total, band 1, 2, 3
     FT* Omega;              // [ncell] compact
     const double* oc;       // [OMEGA_CONST_FIELDS][ncell]: per-cell carbonate constants
+    FT* tfac;               // [TFAC_FIELDS][ncell]: temperature factors of the FP64 step, or null
     double* Hs;             // [ncell]: [H+] of the last stage (FP32-mixed warm start)
     int have_H;             // Hs holds the previous stage's [H+]
     double* sflux;          // [2][Nx*Ny]: top CO2 and O2 fluxes of the stage (gas exchange on)
@@ -225,6 +226,14 @@ __global__ void __launch_bounds__(256) k_pisces_prep(PiscesArgs<FT> a)
                                           o.boron, o.sulfate, o.fluoride, o.ca, o.KSP};
 #pragma unroll
     for (int n = 0; n < OMEGA_CONST_FIELDS; n++) out[(long)n * ncell + c] = f[n];
+    if (a.tfac) {
+        // the temperature factors of the transitions, with T fixed
+        TFac<FT> t;
+        temperature_factors(a.pp, a.in[OBME_PI_T][p], t);
+        const FT tv[TFAC_FIELDS] = {t.btN, t.btD, t.bZ, t.bM, t.doc, t.pom, t.rain, t.kfe, t.eq, t.t4};
+#pragma unroll
+        for (int n = 0; n < TFAC_FIELDS; n++) a.tfac[(long)n * ncell + c] = tv[n];
+    }
 }
 
 // Thread per cell: transitions, minus the upwind sinking divergence of
@@ -302,7 +311,25 @@ __global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES :
     A.wPOCc = (wP[k] + wP[k + 1]) / 2;
     A.wGOCc = (wG[k] + wG[k + 1]) / 2;
     FT G[NW];
-    pisces_tendencies(a.pp, v, A, a.sk, G);
+    // the FP64 step reads the temperature factors k_pisces_prep stored; the
+    // other modes evaluate them here
+    TFac<FT> tf;
+    if constexpr (sizeof(FT) == 8 && MODE == 0) {
+        const FT* t = a.tfac + c;
+        tf.btN = t[0];
+        tf.btD = t[ncell];
+        tf.bZ = t[2 * ncell];
+        tf.bM = t[3 * ncell];
+        tf.doc = t[4 * ncell];
+        tf.pom = t[5 * ncell];
+        tf.rain = t[6 * ncell];
+        tf.kfe = t[7 * ncell];
+        tf.eq = t[8 * ncell];
+        tf.t4 = t[9 * ncell];
+    } else {
+        temperature_factors(a.pp, v[OBME_PI_T], tf);
+    }
+    pisces_tendencies(a.pp, v, A, a.sk, tf, G);
     // flux-form upwind sinking: (w(k+1) c(k+1) - w(k) c(k)) / dz, w = 0 on the top face
     const FT idz = a.lev[3 * Nz + k];
     const unsigned pu = (k + 1 < Nz) ? pidx(a, i, j, k + 1) : p;
@@ -591,7 +618,9 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
             A.Omega = (FT)omega_from_H(o, cp, H);
         }
         FT G[NW];
-        pisces_tendencies(pps, v, A, f.sk[st], G);
+        TFac<FT> tf;
+        temperature_factors(pps, v[OBME_PI_T], tf);
+        pisces_tendencies(pps, v, A, f.sk[st], tf, G);
         // flux-form upwind sinking: (w(k+1) c(k+1) - w(k) c(k)) / dz, the
         // level above from lane r - 1, w = 0 on the top face
         auto sink = [&](int n, FT wlo, FT whi) {
@@ -650,6 +679,7 @@ struct obme_pisces {
     void* d_sflux = nullptr;
     void* d_Hsurf = nullptr;
     void* d_surfc = nullptr;   // [SURF_PACK][Nx*Ny] surface constants of the fused step
+    void* d_tfac = nullptr;    // [TFAC_FIELDS][ncell] temperature factors of the FP64 step
     void* d_copy[NW] = {};
     size_t field_bytes = 0;
     std::vector<double> zc, zf, dz;
@@ -661,7 +691,7 @@ extern "C" obme_status obme_pisces_destroy(obme_pisces* h)
 {
     if (!h) return OBME_OK;
     for (void* p : {h->d_pp, h->d_lev, h->d_w, h->d_PAR, h->d_Omega, h->d_zeu, h->d_mlPAR, h->d_Gm, h->d_oc, h->d_H,
-                    h->d_sflux, h->d_Hsurf, h->d_surfc})
+                    h->d_sflux, h->d_Hsurf, h->d_surfc, h->d_tfac})
         if (p) (void)hipFree(p);
     for (void* p : h->d_copy)
         if (p) (void)hipFree(p);
@@ -787,6 +817,7 @@ static PiscesArgs<FT> pisces_args(const obme_pisces* h, double t)
     a.zeu = (FT*)h->d_zeu;
     a.mlPAR = (FT*)h->d_mlPAR;
     a.oc = (const double*)h->d_oc;
+    a.tfac = (FT*)h->d_tfac;
     a.Hs = (double*)h->d_H;
     a.sflux = (double*)h->d_sflux;
     a.Hsurf = (double*)h->d_Hsurf;
@@ -1056,6 +1087,9 @@ extern "C" obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, in
         return obme_fail(OBME_ERR_NOMEM, "PISCES carbonate constants");
     if (!h->d_H && hipMalloc(&h->d_H, (size_t)h->ncell * sizeof(double)) != hipSuccess)
         return obme_fail(OBME_ERR_NOMEM, "PISCES [H+] store");
+    if (h->prec == OBME_FP64 && !h->d_tfac &&
+        hipMalloc(&h->d_tfac, (size_t)TFAC_FIELDS * h->ncell * sizeof(double)) != hipSuccess)
+        return obme_fail(OBME_ERR_NOMEM, "PISCES temperature factors");
     {
         obme_status r = (h->prec == OBME_FP64) ? pisces_prep<double>(h, tracers, s) : pisces_prep<float>(h, tracers, s);
         if (r != OBME_OK) return r;
