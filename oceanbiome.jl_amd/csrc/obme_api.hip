@@ -179,6 +179,8 @@ struct obme_handle {
     double* d_Sprof = nullptr;
     void* d_srec = nullptr;    // SURF_REC * Nx*Ny of FT (CO2 exchange on)
     double* d_Hprev = nullptr; // Nx*Ny warm starts of [H+] (mixed path)
+    double* d_surfc = nullptr; // [SURF_PACK][Nx*Ny] surface constants of the current stepper call
+    bool surfc_ready = false;  // d_surfc holds the constants of this call's T and S
     int sink_idx[MAX_SINK];
     int cfg_id;                // sweep tile configuration (TileCfg table)
     bool uniform;
@@ -500,6 +502,7 @@ extern "C" obme_status obme_destroy(obme_handle* h)
     if (h->d_srec) (void)hipFree(h->d_srec);
     if (h->d_Hprev) (void)hipFree(h->d_Hprev);
     if (h->d_sedG) (void)hipFree(h->d_sedG);
+    if (h->d_surfc) (void)hipFree(h->d_surfc);
     for (auto& v : h->timed)
         for (auto& p : v) {
             (void)hipEventDestroy(p.first);
@@ -828,6 +831,7 @@ static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* 
     }
     if (r != OBME_OK || MODE == TILE_PAR || !a.co2_on) return r;
     SurfaceArgs<FT> sa = make_surface_args<FT>(h, tracers, T, S);
+    if (MODE == TILE_STEP && h->surfc_ready) sa.surfc = h->d_surfc;
     sa.dtf = (FT)dt;
     rk3_products(dt, sa.rk_a, sa.rk_b);
     if (MODE == TILE_TENDENCY) sa.GDIC = (FT*)G[7];
@@ -871,11 +875,29 @@ extern "C" obme_status obme_step_with_sediment(obme_handle* h, void* const* trac
     if (storage && !h->sediment) return fail(OBME_ERR_INVALID, "no sediment configured");
     if (h->sediment && !std::isfinite(dt)) return fail(OBME_ERR_INVALID, "non-finite time step with a sediment");
     hipStream_t s = (hipStream_t)stream;
-    for (int st = 0; st < nsteps; st++) {
-        obme_status r = run_tile<TILE_STEP>(h, tracers, nullptr, nullptr, T, S, t0 + st * dt, dt, s, storage);
-        if (r != OBME_OK) return r;
+    // the CO2 surface kernel's constants depend on the top cell's T and S
+    // alone, which hold for the whole call: set them up once
+    const obme_config& c = h->cfg;
+    h->surfc_ready = false;
+    if (nsteps > 1 && c.model == OBME_LOBSTER && c.carbonate_system && c.co2.enabled && !h->co2_inline) {
+        const size_t ncol = (size_t)c.grid.Nx * c.grid.Ny;
+        if (!h->d_surfc && hipMalloc((void**)&h->d_surfc, SURF_PACK * ncol * sizeof(double)) != hipSuccess)
+            return fail(OBME_ERR_NOMEM, "surface constants");
+        const unsigned nb = (unsigned)((ncol + 255) / 256);
+        if (h->prec == OBME_FP64)
+            hipLaunchKernelGGL(k_surface_prep<double>, dim3(nb), dim3(256), 0, s,
+                               make_surface_args<double>(h, tracers, T, S), h->d_surfc);
+        else
+            hipLaunchKernelGGL(k_surface_prep<float>, dim3(nb), dim3(256), 0, s,
+                               make_surface_args<float>(h, tracers, T, S), h->d_surfc);
+        HIP_TRY(hipGetLastError());
+        h->surfc_ready = true;
     }
-    return OBME_OK;
+    obme_status r = OBME_OK;
+    for (int st = 0; st < nsteps && r == OBME_OK; st++)
+        r = run_tile<TILE_STEP>(h, tracers, nullptr, nullptr, T, S, t0 + st * dt, dt, s, storage);
+    h->surfc_ready = false;
+    return r;
 }
 
 extern "C" obme_status obme_step(obme_handle* h, void* const* tracers, const void* T, const void* S, int32_t nsteps,

@@ -791,6 +791,7 @@ struct SurfaceArgs {
     const FT* Alk;                 // Alk parent array (tendency mode)
     const FT* srec;                // surface records (step mode)
     double* Hprev;                 // [Nx*Ny] warm start of [H+] (mixed path) or null
+    const double* surfc;           // [SURF_PACK][Nx*Ny] surface constants of the call (step mode) or null
     const FT* T3;                  // 3-D temperature / salinity or null
     const FT* S3;
     const double* Tprof;
@@ -807,8 +808,10 @@ struct SurfaceArgs {
     obme_carbon_solver solver;
 };
 
-template <typename FT, int MODE>
-__global__ void __launch_bounds__(256) k_surface(SurfaceArgs<FT> a)
+// The surface constants of every column (T and S of the top cell), once per
+// stepper call, for k_surface.
+template <typename FT>
+__global__ void __launch_bounds__(256) k_surface_prep(SurfaceArgs<FT> a, double* out)
 {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int ncol = a.Nx * a.Ny;
@@ -820,6 +823,33 @@ __global__ void __launch_bounds__(256) k_surface(SurfaceArgs<FT> a)
     const double S = a.S3 ? (double)a.S3[top] : (a.Sprof ? a.Sprof[ktop] : a.Sconst);
     SurfaceCarbon sc;
     surface_carbon_setup(a.co2, T, S, a.co2.wind_speed, sc);
+    double o[SURF_PACK];
+    surface_pack(sc, 0.0, 0.0, o);
+#pragma unroll
+    for (int q = 0; q < SURF_PACK; q++) out[(long)q * ncol + col] = o[q];
+}
+
+template <typename FT, int MODE>
+__global__ void __launch_bounds__(256) k_surface(SurfaceArgs<FT> a)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncol = a.Nx * a.Ny;
+    if (col >= ncol) return;
+    const int i = col % a.Nx, j = col / a.Nx;
+    const int ktop = a.Nz - 1;
+    const unsigned top = a.base0 + (unsigned)i + a.sx * (unsigned)j + a.sxy * (unsigned)ktop;
+    SurfaceCarbon sc;
+    if (a.surfc) {
+        // T and S are fixed within a stepper call: the constants were set up once
+        double o[SURF_PACK], kO2, airO2;
+#pragma unroll
+        for (int q = 0; q < SURF_PACK; q++) o[q] = a.surfc[(long)q * ncol + col];
+        surface_unpack(o, sc, kO2, airO2);
+    } else {
+        const double T = a.T3 ? (double)a.T3[top] : (a.Tprof ? a.Tprof[ktop] : a.Tconst);
+        const double S = a.S3 ? (double)a.S3[top] : (a.Sprof ? a.Sprof[ktop] : a.Sconst);
+        surface_carbon_setup(a.co2, T, S, a.co2.wind_speed, sc);
+    }
     double H = a.Hprev ? a.Hprev[col] : 0.0;
     double pH = 0.0, F = 0.0;
     int ev = 0;
