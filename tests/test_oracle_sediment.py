@@ -166,3 +166,21 @@ def test_hook_sequence_by_hand():
     assert state.storage[0, 0] == pytest.approx(st3 + 4.0 * g2, rel=1e-14)
     assert state.iteration == 4 and state.last_dt == 4.0
     np.testing.assert_array_equal(state.storage[0, 0], state.storage[0, 1])
+
+
+def test_host_description_of_the_sediment():
+    """InstantRemineralisationSediment on the host: the defaults of
+    instant_remineralisation.jl:84-98, names resolved in tracer order, bad
+    names and the RungeKutta3 sediment stepper rejected."""
+    names = ["NO₃", "NH₄", "P", "Z", "sPOM", "bPOM", "DOM"]
+    s = InstantRemineralisationSediment(sinking_tracers=("sPOM", "bPOM"), remineralisation_reciever="NH₄").struct(names)
+    assert s.model == abi.OBME_SEDIMENT_INSTANT_REMINERALISATION and s.receiver == 1
+    assert s.n_sinking == 2 and list(s.sinking)[:2] == [4, 5]
+    assert (s.burial_efficiency_constant1, s.burial_efficiency_constant2) == (A, B)
+    assert s.burial_efficiency_half_saturation == K and s.ab2_chi == 0.1
+    with pytest.raises(ValueError):
+        InstantRemineralisationSediment(remineralisation_reciever="N").struct(names)
+    with pytest.raises(ValueError):
+        InstantRemineralisationSediment(sinking_tracers=("D",), remineralisation_reciever="NH₄").struct(names)
+    with pytest.raises(NotImplementedError):
+        InstantRemineralisationSediment(timestepper="RungeKutta3")
