@@ -1,0 +1,88 @@
+"""The benchmark configurations at their full sizes on the GPU, through
+properties that do not need the whole grid on the CPU: columns are
+independent, so a sample of columns stepped by the oracle is an exact check
+of those columns; the halos and the read-only fields stay untouched; every
+written value stays finite.  LOBSTER + DIC/Alk with the CO2 exchange on
+512x512x64 (the headline workload) in both precisions, and PISCES on
+256x256x64 in FP32-mixed.  Bars as everywhere: FP64 <= 1e-12 relative L2
+per tracer, FP32-mixed <= 1e-5."""
+import numpy as np
+import pytest
+
+import oracle as O
+import pisces as OP
+from oceanbiome import abi, workloads
+from oceanbiome.runtime import HIPBiogeochemistry, HIPPISCES
+
+pytestmark = pytest.mark.gpu
+TOL = {"fp64": 1e-12, "fp32": 1e-5}
+
+
+def _sample(g, n, seed):
+    rng = np.random.default_rng(seed)
+    flat = rng.choice(g.Nx * g.Ny, n, replace=False)
+    return (flat % g.Nx).astype(np.int32), (flat // g.Nx).astype(np.int32)
+
+
+def _columns(g, arr, ci, cj):
+    ks = np.arange(g.Nz) + g.Hz
+    return arr[ks[:, None], (cj + g.Hy)[None, :], (ci + g.Hx)[None, :]]
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_lobster_512x512x64_column_sample(precision):
+    import torch
+    print("case full-size lobster", precision, flush=True)
+    wl = workloads.lobster_3d(512, 512, 64, nsteps=6)
+    g = wl.grid
+    ci, cj = _sample(g, 96, 5)
+    ref = [a.copy() for a in wl.tracers]
+    O.grid_run_columns(wl.bgc.config(abi.OBME_FP64), ref, ci, cj, wl.nsteps, wl.dt)
+    dev = HIPBiogeochemistry(wl.bgc, precision)
+    tr = [torch.from_numpy(a).to(device="cuda", dtype=dev.dtype) for a in wl.tracers]
+    dev.step(tr, wl.nsteps, wl.dt)
+    torch.cuda.synchronize()
+    names = wl.bgc.required_biogeochemical_tracers()
+    errs = {}
+    for n, name in enumerate(names):
+        out = tr[n].double().cpu().numpy()
+        a, b = _columns(g, out, ci, cj), _columns(g, ref[n], ci, cj)
+        errs[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        assert np.isfinite(g.interior(out)).all(), name
+        # halos untouched (the bottom halo plane and the first halo row)
+        np.testing.assert_array_equal(out[0], wl.tracers[n][0].astype(out.dtype) if precision == "fp64"
+                                      else wl.tracers[n][0].astype(np.float32).astype(np.float64))
+        del out
+    print("errors", errs, flush=True)
+    assert all(e <= TOL[precision] for e in errs.values()), errs
+
+
+def test_pisces_256x256x64_column_sample_fp32():
+    import torch
+    print("case full-size pisces fp32", flush=True)
+    wl = workloads.pisces_3d(256, 256, 64, nsteps=2, dt=300.0)
+    g = wl.grid
+    m = wl.bgc
+    ci, cj = _sample(g, 48, 7)
+    q = OP.params_from_vector(m.param_vector())
+    light = dict(surface_PAR=m.surface_PAR, kw=m.kw, e=m.e, chi=m.chi, division=m.division)
+    F = {n: _columns(g, a, ci, cj)[:, None, :].copy() for n, a in zip(OP.TRACERS, wl.tracers)}
+    cc = m.carbon_chemistry
+    with np.errstate(all="ignore"):
+        OP.step(q, light, F, np.asarray(g.zc), np.asarray(g.zf), np.asarray(g.dz), wl.nsteps, wl.dt, 0.0,
+                cc.solver.struct(cc.initial_pH_guess))
+    dev = HIPPISCES(m, "fp32")
+    tr = [torch.from_numpy(a).to(device="cuda", dtype=dev.dtype) for a in wl.tracers]
+    dev.step(tr, wl.nsteps, wl.dt)
+    torch.cuda.synchronize()
+    errs = {}
+    for n, name in enumerate(OP.WRITTEN):
+        out = tr[n].double().cpu().numpy()
+        a = _columns(g, out, ci, cj)
+        b = F[name][:, 0, :]
+        errs[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        assert np.isfinite(g.interior(out)).all(), name
+    print("errors", errs, flush=True)
+    assert all(e <= TOL["fp32"] for e in errs.values()), errs
+    # T and S are read only
+    np.testing.assert_array_equal(tr[24].cpu().numpy(), wl.tracers[24].astype(np.float32))
