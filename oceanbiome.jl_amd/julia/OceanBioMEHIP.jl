@@ -180,11 +180,12 @@ mutable struct HIPBiogeochemistry{B, F} <: AbstractBiogeochemistry
 end
 
 """
-    HIPBiogeochemistry(underlying, grid, config::ObmeConfig, PAR; keep = [])
+    HIPBiogeochemistry(underlying, grid, config::ObmeConfig, PAR; keep = [], storage = nothing)
 
 Creates the library handle for `config` (built from the OceanBioME model
 `underlying` by the caller, see the Python host package for the field-by-field
-mapping) and owns `PAR`.
+mapping) and owns `PAR`.  With a sediment in `config`, `storage` is the
+sediment's storage field: a device array of Nx*Ny Float64, i fastest.
 """
 function HIPBiogeochemistry(underlying, grid, config::ObmeConfig, PAR; keep = Any[], storage = nothing)
     check_abi()
