@@ -218,6 +218,17 @@ typedef struct obme_sediment {
     double ab2_chi;                /* χ of QuasiAdamsBashforth2, 0.1                     */
 } obme_sediment;
 
+/* Oxygen() (NutrientsPlanktonDetritus/oxygen.jl:14-33): the O₂ tracer of
+ * LOBSTER, last in the tracer order (NutrientsPlanktonDetritus.jl:69-74).
+ * One-way coupled: its tendency Rp μP - (Rp - Rn) G_NH₄ - Rp nitrification
+ * reads the other tracers and changes none of them. */
+typedef struct obme_oxygen {
+    int32_t enabled;               /* 1 adds O₂                                          */
+    int32_t pad0;
+    double respiration_oxygen_nitrogen_ratio;     /* Rp, 10.75 mol O / mol N            */
+    double nitrification_oxygen_nitrogen_ratio;   /* Rn, 2.0 mol O / mol N              */
+} obme_oxygen;
+
 typedef struct obme_config {
     int32_t model;                 /* OBME_NPZD | OBME_LOBSTER                */
     int32_t precision;             /* OBME_FP64 | OBME_FP32_MIXED             */
@@ -233,6 +244,7 @@ typedef struct obme_config {
     obme_field_source salinity;
     obme_negative_scaling scaling; /* NPZD(grid; scale_negatives = true) etc. */
     obme_sediment sediment;        /* NPZD(; grid, sediment_model) etc.          */
+    obme_oxygen oxygen;            /* LOBSTER(grid; oxygen = Oxygen())          */
 } obme_config;
 
 typedef struct obme_handle obme_handle;
@@ -611,7 +623,8 @@ obme_status obme_kernel_timing(obme_handle* h, int32_t which, double* total_ms, 
  * (0 obme_config, 1 obme_grid, 2 obme_npd_params, 3 obme_two_band_par,
  * 4 obme_sinking, 5 obme_carbon_solver, 6 obme_co2_exchange,
  * 7 obme_field_source, 8 obme_batch_params, 9 obme_pisces_config,
- * 10 obme_negative_scaling, 11 obme_o2_exchange, 12 obme_sediment); -1 for an
+ * 10 obme_negative_scaling, 11 obme_o2_exchange, 12 obme_sediment,
+ * 13 obme_oxygen); -1 for an
  * unknown index.  No reference
  * counterpart: it lets a binding check its struct layouts. */
 int64_t obme_abi_sizeof(int32_t which);

@@ -52,7 +52,7 @@ extern "C" const char* obme_version(void) { return "obme 0.1.0 gfx950"; }
 #ifndef OBME_F32_WAVES_PER_EU
 #define OBME_F32_WAVES_PER_EU 4
 #endif
-template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL, bool PIPE,
+template <int MODEL, int CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL, bool PIPE,
           int FM = -1, int EXTRA = 0>
 __global__ void __launch_bounds__(NW * 64, (sizeof(FT) == 4 && L <= 2) ? OBME_F32_WAVES_PER_EU : 1)
     k_column_wave(StepArgs<FT> a)
@@ -287,7 +287,7 @@ static NegScale<FT, NTMAX> convert_scaling(const obme_negative_scaling& s)
 static int ntracers_of(const obme_config& c)
 {
     if (c.model == OBME_NPZD) return 5;
-    if (c.model == OBME_LOBSTER) return c.carbonate_system ? 9 : 7;
+    if (c.model == OBME_LOBSTER) return (c.carbonate_system ? 9 : 7) + (c.oxygen.enabled ? 1 : 0);
     return 0;
 }
 
@@ -379,6 +379,8 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
         return fail(OBME_ERR_INVALID, "unknown precision");
     if (cfg->model == OBME_NPZD && cfg->carbonate_system)
         return fail(OBME_ERR_UNSUPPORTED, "the carbonate system is supported with LOBSTER");
+    if (cfg->model == OBME_NPZD && cfg->oxygen.enabled)
+        return fail(OBME_ERR_UNSUPPORTED, "the oxygen component is supported with LOBSTER");
     if (g.Nx < 1 || g.Ny < 1 || g.Nz < 1 || g.Hx < 0 || g.Hy < 0 || g.Hz < 0)
         return fail(OBME_ERR_INVALID, "invalid grid size or halo");
     if (g.Nz > 128) return fail(OBME_ERR_UNSUPPORTED, "Nz > 128 is not supported by the column-tile kernels");
@@ -518,8 +520,9 @@ extern "C" int32_t obme_required_tracers(const obme_handle* h, char* buf, int32_
     if (!h) return -1;
     const char* names;
     if (h->cfg.model == OBME_NPZD) names = "N;P;Z;T;D";
-    else if (h->cfg.carbonate_system) names = "NO₃;NH₄;P;Z;sPOM;bPOM;DOM;DIC;Alk";
-    else names = "NO₃;NH₄;P;Z;sPOM;bPOM;DOM";
+    else if (h->cfg.carbonate_system)
+        names = h->cfg.oxygen.enabled ? "NO₃;NH₄;P;Z;sPOM;bPOM;DOM;DIC;Alk;O₂" : "NO₃;NH₄;P;Z;sPOM;bPOM;DOM;DIC;Alk";
+    else names = h->cfg.oxygen.enabled ? "NO₃;NH₄;P;Z;sPOM;bPOM;DOM;O₂" : "NO₃;NH₄;P;Z;sPOM;bPOM;DOM";
     if (buf && cap > 0) {
         std::strncpy(buf, names, cap - 1);
         buf[cap - 1] = 0;
@@ -545,6 +548,10 @@ static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers)
     a.base0 = (unsigned)g.Hx + a.sx * (unsigned)g.Hy + a.sxy * (unsigned)g.Hz;
     a.field_bytes = (unsigned)((size_t)a.sxy * (size_t)(g.Nz + 2 * g.Hz) * sizeof(FT));
     a.q = convert_params<FT>(c.npd);
+    if (c.oxygen.enabled) {
+        a.q.Rp = (FT)c.oxygen.respiration_oxygen_nitrogen_ratio;
+        a.q.Rn = (FT)c.oxygen.nitrification_oxygen_nitrogen_ratio;
+    }
     a.kr = (FT)c.par.water_red_attenuation;
     a.kb = (FT)c.par.water_blue_attenuation;
     a.xr = (FT)c.par.chlorophyll_red_attenuation;
@@ -596,6 +603,10 @@ static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers)
     }
     if (sizeof(FT) == 4) {
         a.f = make_fast_params(c.npd);
+        if (c.oxygen.enabled) {
+            a.f.oRp = (float)c.oxygen.respiration_oxygen_nitrogen_ratio;
+            a.f.oRpn = (float)(c.oxygen.respiration_oxygen_nitrogen_ratio - c.oxygen.nitrification_oxygen_nitrogen_ratio);
+        }
         const double log2e = 1.4426950408889634;
         a.kr2 = (float)(c.par.water_red_attenuation * log2e);
         a.kb2 = (float)(c.par.water_blue_attenuation * log2e);
@@ -651,7 +662,7 @@ static SurfaceArgs<FT> make_surface_args(const obme_handle* h, void* const* trac
     return s;
 }
 
-template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE>
+template <int MODEL, int CARB, typename FT, int L, int CPW, int NW, int MODE>
 static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
 {
     constexpr int NT = ModelTraits<MODEL, CARB>::NT;
@@ -687,7 +698,7 @@ static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_
         if (ex == 1) k = OBME_EXTRA_KERNEL(1);
         if (ex == 4) k = OBME_EXTRA_KERNEL(4);
         if (ex == 5) k = OBME_EXTRA_KERNEL(5);
-        if constexpr (CARB) {
+        if constexpr ((CARB & 1) != 0) {
             if (ex == 3) k = OBME_EXTRA_KERNEL(3);
             if (ex == 7) k = OBME_EXTRA_KERNEL(7);
         }
@@ -696,7 +707,7 @@ static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_
     // the pipelined persistent sweep (FP32 fused step, FULL columns)
     constexpr int PB = (64 % TXB == 0) ? 64 / TXB : 1;
     constexpr int NBLK = (L * 64 / CPW) / PB;
-    constexpr bool PIPE_OK = sizeof(FT) == 4 && MODE == TILE_STEP && 64 % TXB == 0 && NBLK % NW == 0 &&
+    constexpr bool PIPE_OK = CARB < 2 && sizeof(FT) == 4 && MODE == TILE_STEP && 64 % TXB == 0 && NBLK % NW == 0 &&
                              2 * (NT * NBLK / NW) < 64;
     if constexpr (PIPE_OK) {
         if (full && h->pipe && !extra && a.surface_mode != OBME_SURFACE_ARRAY) {
@@ -735,9 +746,13 @@ static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_
 template <typename FT, int L, int CPW, int NW, int MODE>
 static obme_status dispatch_model(obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
 {
-    if (h->cfg.model == OBME_NPZD) return launch_wave<OBME_NPZD, false, FT, L, CPW, NW, MODE>(h, a, s);
-    if (h->cfg.carbonate_system) return launch_wave<OBME_LOBSTER, true, FT, L, CPW, NW, MODE>(h, a, s);
-    return launch_wave<OBME_LOBSTER, false, FT, L, CPW, NW, MODE>(h, a, s);
+    if (h->cfg.model == OBME_NPZD) return launch_wave<OBME_NPZD, 0, FT, L, CPW, NW, MODE>(h, a, s);
+    // LOBSTER's components: 1 CarbonateSystem, 2 Oxygen
+    const int comp = (h->cfg.carbonate_system ? 1 : 0) | (h->cfg.oxygen.enabled ? 2 : 0);
+    if (comp == 1) return launch_wave<OBME_LOBSTER, 1, FT, L, CPW, NW, MODE>(h, a, s);
+    if (comp == 2) return launch_wave<OBME_LOBSTER, 2, FT, L, CPW, NW, MODE>(h, a, s);
+    if (comp == 3) return launch_wave<OBME_LOBSTER, 3, FT, L, CPW, NW, MODE>(h, a, s);
+    return launch_wave<OBME_LOBSTER, 0, FT, L, CPW, NW, MODE>(h, a, s);
 }
 
 template <typename FT, int MODE>
@@ -800,7 +815,11 @@ static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* 
         // update_biogeochemical_state!: the scaling modifier first, in place
         const int bs = 256;
         const long ncell = (long)h->cfg.grid.Nx * h->cfg.grid.Ny * h->cfg.grid.Nz;
-        auto ks = (h->ntracers == 5) ? k_scale_grid<FT, 5> : (h->ntracers == 7 ? k_scale_grid<FT, 7> : k_scale_grid<FT, 9>);
+        auto ks = k_scale_grid<FT, 5>;
+        if (h->ntracers == 7) ks = k_scale_grid<FT, 7>;
+        if (h->ntracers == 8) ks = k_scale_grid<FT, 8>;
+        if (h->ntracers == 9) ks = k_scale_grid<FT, 9>;
+        if (h->ntracers == 10) ks = k_scale_grid<FT, 10>;
         hipLaunchKernelGGL(ks, dim3((unsigned)((ncell + bs - 1) / bs)), dim3(bs), 0, s, a);
         HIP_TRY(hipGetLastError());
     }
@@ -1426,6 +1445,7 @@ extern "C" int64_t obme_abi_sizeof(int32_t which)
     case 10: return sizeof(obme_negative_scaling);
     case 11: return sizeof(obme_o2_exchange);
     case 12: return sizeof(obme_sediment);
+    case 13: return sizeof(obme_oxygen);
     default: return -1;
     }
 }

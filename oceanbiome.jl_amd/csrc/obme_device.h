@@ -129,6 +129,7 @@ struct NPDParams {
     FT kNO3, kNH4, psi, kPAR, mu0, gamma, alpha_x, Q10;
     FT mP, mZ, muZ, aP_solid, aZ_exc, pref, g, kG, aZ_ass, R, rho_c, R_chl;
     FT nitrif, a_rem, s_rem, b_rem, d_rem, small_solid, Rd, rem, small_frac;
+    FT Rp, Rn;    // Oxygen(): respiration and nitrification O:N ratios (set from obme_oxygen)
     int light_form, use_Q10, mort_form, graz_form;
 };
 
@@ -136,6 +137,7 @@ template <typename FT>
 inline NPDParams<FT> convert_params(const obme_npd_params& p)
 {
     NPDParams<FT> q;
+    q.Rp = q.Rn = FT(0);
     q.kNO3 = (FT)p.nitrate_half_saturation;
     q.kNH4 = (FT)p.ammonia_half_saturation;
     q.psi = (FT)p.nitrate_ammonia_inhibition;
@@ -232,8 +234,9 @@ __device__ __forceinline__ void npzd_tendencies(const NPDParams<FT>& q, FT N, FT
     dD = ponw + solid - G.onSmall - q.rem * D;                     // detritus.jl:373-379
 }
 
-// LOBSTER transitions; v/out in tracer order NO3, NH4, P, Z, sPOM, bPOM, DOM[, DIC, Alk].
-template <typename FT, bool CARB>
+// LOBSTER transitions; v/out in tracer order NO3, NH4, P, Z, sPOM, bPOM, DOM[, DIC, Alk][, O2].
+// CARB: bit 1 CarbonateSystem, bit 2 Oxygen.
+template <typename FT, int CARB>
 __device__ __forceinline__ void lobster_tendencies(const NPDParams<FT>& q, const FT* v, FT PAR, FT* out)
 {
     FT NO3 = v[0], NH4 = v[1], P = v[2], Z = v[3], sPOM = v[4], bPOM = v[5], DOM = v[6];
@@ -264,7 +267,7 @@ __device__ __forceinline__ void lobster_tendencies(const NPDParams<FT>& q, const
     out[4] = q.small_solid * solid - G.onSmall - q.s_rem * sPOM;       // detritus.jl:240-244
     out[5] = (FT(1) - q.small_solid) * solid - q.b_rem * bPOM;        // :246-249
     out[6] = ponw + donw - q.d_rem * DOM;                              // :234-238
-    if constexpr (CARB) {
+    if constexpr ((CARB & 1) != 0) {
         FT pp = (FT(1) + q.rho_c * (FT(1) - q.gamma) - q.alpha_x * q.gamma) * muP * q.R;  // plankton.jl:280-290
         FT picw = q.R * pinw;                                                           // :308-309
         FT dicw = q.a_rem * (q.s_rem * (sPOM * q.Rd) + q.b_rem * (bPOM * q.Rd)) + q.d_rem * (DOM * q.Rd);
@@ -272,6 +275,10 @@ __device__ __forceinline__ void lobster_tendencies(const NPDParams<FT>& q, const
         FT cup = FT(2) * q.rho_c * muP * q.R;                                           // :443-450
         out[7] = -pp + picw + dicw + cdis;                                              // carbonate_system.jl:50-55
         out[8] = dNH4 * FT(0.9375) - dNO3 * FT(1.0625) - FT(2) * cup + FT(2) * cdis;    // :57-62
+    }
+    if constexpr ((CARB & 2) != 0) {
+        // oxygen.jl:21-31: the NH4 tendency stands for "nitrate_production"
+        out[(CARB & 1) ? 9 : 7] = q.Rp * muP - (q.Rp - q.Rn) * dNH4 - q.Rp * nitrification;
     }
 }
 
@@ -300,12 +307,14 @@ struct FastParams {
     float R, Rd, rhoR2, rhoR4;  // R, Rd, 2 rho R, 4 rho R
     float rem, small_frac, uptake_N;  // Detritus rate, fraction, (1 - alpha_x gamma)
     float log2Q10_10;  // log2(Q10)/10: Q10^(T/10) = 2^(T log2Q10_10)
+    float oRp, oRpn;   // Oxygen(): Rp, Rp - Rn (set from obme_oxygen)
     int light_form, use_Q10, mort_form, graz_form;
 };
 
 inline FastParams make_fast_params(const obme_npd_params& p)
 {
     FastParams f;
+    f.oRp = f.oRpn = 0.0f;
     const double LOG2E = 1.4426950408889634;
     f.nPSI2 = (float)(-p.nitrate_ammonia_inhibition * LOG2E);
     f.kNO3 = (float)p.nitrate_half_saturation;
@@ -417,7 +426,7 @@ __device__ __forceinline__ float fast_light(const FastParams& f, float PAR)
 // The nitrate and ammonia uptakes are mu0/2 L_PAR P times their own
 // limitation (the reference forms them as fractions of the total
 // production, plankton.jl:223-261, which is the same product).
-template <bool CARB, int FM = -1>
+template <int CARB, int FM = -1>
 __device__ __forceinline__ void lobster_fast(const FastParams& f, const float* v, float PAR, float* out)
 {
     const float NO3 = v[0], NH4 = v[1], P = v[2], Z = v[3], sPOM = v[4], bPOM = v[5], DOM = v[6];
@@ -449,11 +458,15 @@ __device__ __forceinline__ void lobster_fast(const FastParams& f, const float* v
     out[4] = fmaf(f.ss, solid, -G.onSmall) - f.s_rem * sPOM;
     out[5] = fmaf(f.one_m_ss, solid, -f.b_rem * bPOM);
     out[6] = fmaf(f.one_m_arem, sb, ponw) - f.d_rem * DOM;
-    if constexpr (CARB) {
+    if constexpr ((CARB & 1) != 0) {
         // calcite dissolution (G_P + ν) ρ R (plankton.jl:429-441), uptake 2 ρ μP R (:443-450)
         const float cdis2 = (G.onP + nuP) * f.rhoR2;            // 2 cdis
         out[7] = fmaf(-f.ppc, muP, fmaf(f.R, pinw, fmaf(f.Rd, dinw, 0.5f * cdis2)));
         out[8] = fmaf(0.9375f, dNH4, -1.0625f * dNO3) - f.rhoR4 * muP + cdis2;
+    }
+    if constexpr ((CARB & 2) != 0) {
+        // Rp (μP - nitrification) - (Rp - Rn) G_NH4 (oxygen.jl:21-31)
+        out[(CARB & 1) ? 9 : 7] = fmaf(f.oRp, muP - nit, -f.oRpn * dNH4);
     }
 }
 

@@ -43,7 +43,9 @@ namespace obme {
 constexpr int MAX_SINK = 2;
 
 // Per-model tracer layout (tracer order of required_biogeochemical_tracers).
-template <int MODEL, bool CARB>
+// CARB is a bit set of LOBSTER's optional components: 1 CarbonateSystem
+// (DIC, Alk at 7, 8), 2 Oxygen (O₂, last).
+template <int MODEL, int CARB>
 struct ModelTraits;
 
 template <>
@@ -61,8 +63,22 @@ struct ModelTraits<OBME_LOBSTER, false> {
     static constexpr int SINK0 = 4, SINK1 = 5;  // sPOM, bPOM
 };
 template <>
-struct ModelTraits<OBME_LOBSTER, true> {
+struct ModelTraits<OBME_LOBSTER, 1> {
     static constexpr int NT = 9;
+    static constexpr int P_IDX = 2;
+    static constexpr int RO_IDX = -1;
+    static constexpr int SINK0 = 4, SINK1 = 5;
+};
+template <>
+struct ModelTraits<OBME_LOBSTER, 2> {
+    static constexpr int NT = 8;            // ..., DOM, O₂
+    static constexpr int P_IDX = 2;
+    static constexpr int RO_IDX = -1;
+    static constexpr int SINK0 = 4, SINK1 = 5;
+};
+template <>
+struct ModelTraits<OBME_LOBSTER, 3> {
+    static constexpr int NT = 10;           // ..., DOM, DIC, Alk, O₂
     static constexpr int P_IDX = 2;
     static constexpr int RO_IDX = -1;
     static constexpr int SINK0 = 4, SINK1 = 5;
@@ -170,7 +186,7 @@ __device__ __forceinline__ void lds_load_b32(rsrc_t r, void* dst, unsigned voff)
 // EXTRA (fused step only), a bit set: 1 negative scaling at the start of
 // every stage; 2 the CO2 flux computed here by the top lane (with 1);
 // 4 the InstantRemineralisation sediment at the bottom cell.
-template <int MODEL, bool CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL,
+template <int MODEL, int CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL,
           bool PIPE = false, int FM = -1, int EXTRA = 0>
 __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
 {
@@ -380,7 +396,7 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
     const unsigned col2d = (unsigned)i + (unsigned)a.Nx * (unsigned)j;
     const unsigned ncol2d = (unsigned)a.Nx * (unsigned)a.Ny;
     const bool top_lane = col_ok && r == 0;
-    if constexpr (CARB && MODE == TILE_STEP) {
+    if constexpr ((CARB & 1) && MODE == TILE_STEP) {
         if (top_lane && a.co2_on) a.srec[col2d] = U[7][0];
     }
 
@@ -399,7 +415,7 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
     const double g1 = 8.0 / 15.0, g2 = 5.0 / 12.0, g3 = 3.0 / 4.0, z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
     FT PARv[L];   // PAR of the own levels (output of the PAR and tendency modes)
     static_assert(EXTRA == 0 || (MODE == TILE_STEP && !PIPE), "scaling, inline CO2, sediment: fused step, direct sweep");
-    static_assert(!(EXTRA & 2) || CARB, "inline CO2 needs the carbonate system");
+    static_assert(!(EXTRA & 2) || (CARB & 1), "inline CO2 needs the carbonate system");
     // inline CO2 (EXTRA & 2): [H+] of the previous stage (FP32-mixed warm start)
     double Hc = 0.0;
     if constexpr ((EXTRA & 2) && F32) {
@@ -539,7 +555,7 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
                 FT Gv[NT];
                 FT v[NT];
 #pragma unroll
-                for (int n = 0; n < NT; n++) v[n] = (CARB && n >= 7) ? FT(0) : val(n, m);
+                for (int n = 0; n < NT; n++) v[n] = (CARB != 0 && n >= 7) ? FT(0) : val(n, m);
                 if constexpr (MODEL == OBME_NPZD) {
                     if constexpr (F32) {
                         const float Lt = fm_q10<FM>(a.f) ? m_exp2(v[3] * a.f.log2Q10_10) : 1.0f;
@@ -627,7 +643,7 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
 #pragma unroll
                     for (int n = 0; n < NT; n++) Gm[n][m] = Gv[n];   // Gm carries G out
                 } else {
-                    if constexpr (CARB) {
+                    if constexpr (CARB & 1) {
                         // surface record: DIC transition without the flux, Alk at the stage start
                         if (top_lane && a.co2_on && m == 0) {
                             a.srec[(1 + st) * ncol2d + col2d] = Gv[7];
@@ -702,7 +718,7 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
                 const int k = kb * PB + kl;
                 v[n][q] = pbuf[n * TRS + kb * BLK + lane];
                 bool ok = ibase + cc < a.Nx && n != RO;
-                if constexpr (CARB) {
+                if constexpr (CARB & 1) {
                     if (n == 7 && a.co2_on && k == ktop) ok = false;
                 }
                 off[n][q] = ok ? goff2(ibase + cc, j, k) : a.field_bytes;
@@ -770,7 +786,7 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
             const int e = threadIdx.x + q * BS;
             const int k = e / TXB, cc = e % TXB;
             bool ok = e_idx(q) >= 0 && ibase + cc < a.Nx;
-            if constexpr (CARB && MODE == TILE_STEP) {
+            if constexpr ((CARB & 1) && MODE == TILE_STEP) {
                 // the top-cell DIC is written by the surface kernel
                 if (n == 7 && a.co2_on && k == ktop) ok = false;
             }

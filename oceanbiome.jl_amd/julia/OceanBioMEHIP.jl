@@ -134,6 +134,15 @@ struct ObmeSediment
     ab2_chi :: Float64
 end
 
+# Oxygen() (NutrientsPlanktonDetritus/oxygen.jl): LOBSTER's O₂ tracer, last in
+# the tracer order; enabled = 0: no O₂.
+struct ObmeOxygen
+    enabled :: Int32
+    pad0 :: Int32
+    respiration_oxygen_nitrogen_ratio :: Float64     # 10.75
+    nitrification_oxygen_nitrogen_ratio :: Float64   # 2.0
+end
+
 struct ObmeConfig
     model :: Int32
     precision :: Int32
@@ -149,6 +158,7 @@ struct ObmeConfig
     salinity :: ObmeFieldSource
     scaling :: ObmeNegativeScaling
     sediment :: ObmeSediment
+    oxygen :: ObmeOxygen
 end
 
 # layout check against the library (obme_abi_sizeof)
@@ -158,7 +168,7 @@ function check_abi()
         n = ccall((:obme_abi_sizeof, libobme), Int64, (Int32,), idx - 1)
         n == sizeof(T) || error("ABI mismatch for $T: library $n bytes, Julia $(sizeof(T))")
     end
-    for (idx, T) in ((10, ObmeNegativeScaling), (12, ObmeSediment))
+    for (idx, T) in ((10, ObmeNegativeScaling), (12, ObmeSediment), (13, ObmeOxygen))
         n = ccall((:obme_abi_sizeof, libobme), Int64, (Int32,), idx)
         n == sizeof(T) || error("ABI mismatch for $T: library $n bytes, Julia $(sizeof(T))")
     end

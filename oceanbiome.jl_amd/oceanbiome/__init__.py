@@ -1,7 +1,7 @@
 """oceanbiome — host side of the MI355X biogeochemistry step.
 
 Mirrors the OceanBioME.jl interface for the NutrientsPlanktonDetritus models
-(NPZD, LOBSTER + CarbonateSystem) and PISCES (oceanbiome.pisces), TwoBand and
+(NPZD, LOBSTER + CarbonateSystem + Oxygen) and PISCES (oceanbiome.pisces), TwoBand and
 MultiBand PAR, sinking, CarbonChemistry, the CO2 and O2 GasExchange boundary
 conditions, ScaleNegativeTracers and the InstantRemineralisation sediment;
 the compute runs in libobme_hip.so.
@@ -11,7 +11,7 @@ from .grid import ColumnGrid, RectilinearGrid
 from .models import (LOBSTER, NPZD, AnalyticalLightLimitation, Biogeochemistry, CarbonateSystem,
                      CarbonChemistry, CarbonDioxideGasExchangeBoundaryCondition, DampedNewtonRaphsonSolver,
                      Detritus, InstantRemineralisationSediment, Linear, MondoLightLimitation, NitrateAmmonia, Nutrient,
-                     OxygenGasExchangeBoundaryCondition, PhytoZoo, Quadratic,
+                     Oxygen, OxygenGasExchangeBoundaryCondition, PhytoZoo, Quadratic,
                      TwoBandPhotosyntheticallyActiveRadiation, TwoParticleAndDissolved,
                      biogeochemical_drift_velocity, default_surface_PAR, required_biogeochemical_tracers)
 
@@ -21,7 +21,7 @@ __all__ = [
     "CarbonDioxideGasExchangeBoundaryCondition", "OxygenGasExchangeBoundaryCondition", "DampedNewtonRaphsonSolver",
     "TwoBandPhotosyntheticallyActiveRadiation", "Linear", "Quadratic", "MondoLightLimitation",
     "AnalyticalLightLimitation", "default_surface_PAR", "required_biogeochemical_tracers",
-    "biogeochemical_drift_velocity", "InstantRemineralisationSediment",
+    "biogeochemical_drift_velocity", "InstantRemineralisationSediment", "Oxygen",
 ]
 
 

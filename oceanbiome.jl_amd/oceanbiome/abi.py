@@ -182,6 +182,16 @@ class Sediment(C.Structure):
     ]
 
 
+class Oxygen(C.Structure):
+    """obme_oxygen (the Oxygen() component of LOBSTER)."""
+    _fields_ = [
+        ("enabled", C.c_int32),
+        ("pad0", C.c_int32),
+        ("respiration_oxygen_nitrogen_ratio", C.c_double),
+        ("nitrification_oxygen_nitrogen_ratio", C.c_double),
+    ]
+
+
 class Config(C.Structure):
     _fields_ = [
         ("model", C.c_int32),
@@ -198,6 +208,7 @@ class Config(C.Structure):
         ("salinity", FieldSource),
         ("scaling", NegativeScaling),
         ("sediment", Sediment),
+        ("oxygen", Oxygen),
     ]
 
 
@@ -266,4 +277,5 @@ ABI_STRUCTS = (
     ("obme_negative_scaling", NegativeScaling),
     ("obme_o2_exchange", O2Exchange),
     ("obme_sediment", Sediment),
+    ("obme_oxygen", Oxygen),
 )

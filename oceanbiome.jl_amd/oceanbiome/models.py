@@ -115,6 +115,20 @@ class CarbonateSystem:
 
 
 @dataclass
+class Oxygen:
+    """Oxygen (NutrientsPlanktonDetritus/oxygen.jl:14-17): the O₂ tracer of
+    LOBSTER, one-way coupled (its tendency, oxygen.jl:21-31, changes no other
+    tracer)."""
+    respiration_oxygen_nitrogen_ratio: float = 10.75
+    nitrification_oxygen_nitrogen_ratio: float = 2.0
+    tracers = ("O₂",)
+
+    def struct(self):
+        return abi.Oxygen(enabled=1, respiration_oxygen_nitrogen_ratio=self.respiration_oxygen_nitrogen_ratio,
+                          nitrification_oxygen_nitrogen_ratio=self.nitrification_oxygen_nitrogen_ratio)
+
+
+@dataclass
 class DampedNewtonRaphsonSolver:
     """DampedNewtonRaphsonSolver (Utils/solvers.jl:81-88) with the
     CarbonChemistry bounds (lower = 0) (carbon_chemistry.jl:81)."""
@@ -276,7 +290,7 @@ class Biogeochemistry:
     """The DiscreteBiogeochemistry wrapper (src/OceanBioME.jl:72-120) around a
     NutrientsPlanktonDetritus composition."""
 
-    def __init__(self, grid, nutrients, plankton, detritus, carbonate_system=None,
+    def __init__(self, grid, nutrients, plankton, detritus, carbonate_system=None, oxygen=None,
                  light_attenuation=None, gas_exchange=None, sinking_scheme="upwind",
                  temperature=None, salinity=35.0, scale_negatives=False, invalid_fill_value=float("nan"),
                  sediment_model=None):
@@ -288,6 +302,7 @@ class Biogeochemistry:
         self.plankton = plankton
         self.detritus = detritus
         self.carbonate_system = carbonate_system
+        self.oxygen = oxygen
         self.light_attenuation = light_attenuation or TwoBandPhotosyntheticallyActiveRadiation()
         self.gas_exchange = gas_exchange
         self.sinking_scheme = sinking_scheme
@@ -301,6 +316,14 @@ class Biogeochemistry:
             raise NotImplementedError("supported compositions: NPZD and LOBSTER")
         if self.model == abi.OBME_NPZD and carbonate_system is not None:
             raise NotImplementedError("the carbonate system is supported with LOBSTER")
+        if oxygen is not None and not isinstance(oxygen, Oxygen):
+            raise TypeError("oxygen must be Oxygen() or None")
+        if gas_exchange is not None and not isinstance(gas_exchange, CarbonDioxideGasExchangeBoundaryCondition):
+            # the NPZD/LOBSTER step couples the CO2 flux only; the O2 flux runs with PISCES
+            raise NotImplementedError("NPZD and LOBSTER take the CO2 gas exchange")
+        if self.model == abi.OBME_NPZD and oxygen is not None:
+            # NPZD's Oxygen method (oxygen.jl:35-46) is not built
+            raise NotImplementedError("the oxygen component is supported with LOBSTER")
         self._keep = []
 
     # -- Oceananigans.Biogeochemistry API -------------------------------------
@@ -312,6 +335,8 @@ class Biogeochemistry:
         t += list(self.detritus.tracers)
         if self.carbonate_system is not None:
             t += list(self.carbonate_system.tracers)
+        if self.oxygen is not None:
+            t += list(self.oxygen.tracers)
         return tuple(t)
 
     def required_biogeochemical_auxiliary_fields(self):
@@ -406,6 +431,8 @@ class Biogeochemistry:
         cfg.model = self.model
         cfg.precision = precision
         cfg.carbonate_system = 1 if self.carbonate_system is not None else 0
+        if self.oxygen is not None:
+            cfg.oxygen = self.oxygen.struct()
         # host arrays referenced by the struct stay alive with this object
         zc = np.ascontiguousarray(g.zc, dtype=np.float64)
         zf = np.ascontiguousarray(g.zf, dtype=np.float64)
@@ -488,7 +515,7 @@ def NPZD(grid, nutrients=None, plankton=None, detritus=None, light_attenuation=N
                            light_attenuation=light_attenuation, **kw)
 
 
-def LOBSTER(grid, nutrients=None, plankton=None, detritus=None, carbonate_system=None,
+def LOBSTER(grid, nutrients=None, plankton=None, detritus=None, carbonate_system=None, oxygen=None,
             light_attenuation=None, surface_photosynthetically_active_radiation="default", **kw):
     """LOBSTER(grid) (NutrientsPlanktonDetritus/constructors.jl:65-96)."""
     if light_attenuation is None:
@@ -496,7 +523,7 @@ def LOBSTER(grid, nutrients=None, plankton=None, detritus=None, carbonate_system
             surface_PAR=surface_photosynthetically_active_radiation)
     return Biogeochemistry(grid, nutrients or NitrateAmmonia(), plankton or PhytoZoo(),
                            detritus or TwoParticleAndDissolved(), carbonate_system=carbonate_system,
-                           light_attenuation=light_attenuation, **kw)
+                           oxygen=oxygen, light_attenuation=light_attenuation, **kw)
 
 
 def required_biogeochemical_tracers(bgc):

@@ -110,10 +110,11 @@ static void npzd_tendencies(const obme_npd_params* q, const double* v, double PA
     out[4] = ponw + solid - G.onSmall - q->remineralisation_rate * D;
 }
 
-/* LOBSTER: NitrateAmmonia + PhytoZoo + TwoParticleAndDissolved [+ CarbonateSystem];
- * tracers (NO3, NH4, P, Z, sPOM, bPOM, DOM [, DIC, Alk]). */
-static void lobster_tendencies(const obme_npd_params* q, int carbonate, const double* v, double PAR,
-                               double* out)
+/* LOBSTER: NitrateAmmonia + PhytoZoo + TwoParticleAndDissolved [+ CarbonateSystem]
+ * [+ Oxygen]; tracers (NO3, NH4, P, Z, sPOM, bPOM, DOM [, DIC, Alk] [, O2]).
+ * ox is NULL without the Oxygen component. */
+static void lobster_tendencies(const obme_npd_params* q, int carbonate, const obme_oxygen* ox,
+                               const double* v, double PAR, double* out)
 {
     double NO3 = v[0], NH4 = v[1], P = v[2], Z = v[3], sPOM = v[4], bPOM = v[5], DOM = v[6];
 
@@ -186,13 +187,41 @@ static void lobster_tendencies(const obme_npd_params* q, int carbonate, const do
         out[7] = -pp + picw + dicw + cdis;
         out[8] = dNH4 * 0.9375 - dNO3 * 1.0625 - 2.0 * cup + 2.0 * cdis;
     }
+    if (ox) {
+        /* Oxygen (oxygen.jl:21-31), NitrateAmmonia method: phytoplankton_growth
+         * is muP, "nitrate_production" is the NH4 tendency, and the
+         * nitrification term is muNH4 = nitrification_rate NH4 */
+        double Rp = ox->respiration_oxygen_nitrogen_ratio;
+        double Rn = ox->nitrification_oxygen_nitrogen_ratio;
+        out[carbonate ? 9 : 7] = Rp * muP - (Rp - Rn) * dNH4 - Rp * nitrification;
+    }
 }
 
 void orc_npd_tendencies(int model, int carbonate, const obme_npd_params* p, const double* vals,
                         double PAR, double* out)
 {
     if (model == OBME_NPZD) npzd_tendencies(p, vals, PAR, out);
-    else lobster_tendencies(p, carbonate, vals, PAR, out);
+    else lobster_tendencies(p, carbonate, NULL, vals, PAR, out);
+}
+
+void orc_npd_tendencies_oxygen(int carbonate, const obme_npd_params* p, const obme_oxygen* ox,
+                               const double* vals, double PAR, double* out)
+{
+    lobster_tendencies(p, carbonate, ox, vals, PAR, out);
+}
+
+/* tracer count and tendencies of a configuration (its optional components included) */
+static int cfg_ntracers(const obme_config* cfg)
+{
+    int n = orc_npd_ntracers(cfg->model, cfg->carbonate_system);
+    return n + ((cfg->model == OBME_LOBSTER && cfg->oxygen.enabled) ? 1 : 0);
+}
+
+static void cfg_tendencies(const obme_config* cfg, const double* vals, double PAR, double* out)
+{
+    if (cfg->model == OBME_NPZD) npzd_tendencies(&cfg->npd, vals, PAR, out);
+    else lobster_tendencies(&cfg->npd, cfg->carbonate_system, cfg->oxygen.enabled ? &cfg->oxygen : NULL,
+                            vals, PAR, out);
 }
 
 /* ---------------- light ---------------- */
@@ -384,7 +413,7 @@ void orc_column_tendencies(const orc_column_env* env, const double* U, double t,
 {
     const obme_config* cfg = env->cfg;
     int Nz = env->Nz;
-    int nt = orc_npd_ntracers(cfg->model, cfg->carbonate_system);
+    int nt = cfg_ntracers(cfg);
     int Pidx = (cfg->model == OBME_NPZD) ? 1 : 2;
 
     /* update_biogeochemical_state!: light (src/OceanBioME.jl:161-167) */
@@ -395,7 +424,7 @@ void orc_column_tendencies(const orc_column_env* env, const double* U, double t,
     for (int k = 0; k < Nz; k++) {
         for (int n = 0; n < nt; n++) vals[n] = U[n * Nz + k];
         if (cfg->model == OBME_NPZD && env->Tcol) vals[3] = env->Tcol[k];
-        orc_npd_tendencies(cfg->model, cfg->carbonate_system, &cfg->npd, vals, PAR[k], out);
+        cfg_tendencies(cfg, vals, PAR[k], out);
         for (int n = 0; n < nt; n++) G[n * Nz + k] = out[n];
     }
 
@@ -440,7 +469,7 @@ void orc_column_run(const orc_column_env* env, double* U, int nsteps, double dt,
                     double* PAR_out)
 {
     int Nz = env->Nz;
-    int nt = orc_npd_ntracers(env->cfg->model, env->cfg->carbonate_system);
+    int nt = cfg_ntracers(env->cfg);
     int ncell = nt * Nz;
     double* G = (double*)malloc(sizeof(double) * ncell);
     double* Gm = (double*)malloc(sizeof(double) * ncell);
@@ -527,7 +556,7 @@ static int run_one_column(const obme_config* cfg, double* const* tracers, const 
 {
     const obme_grid* g = &cfg->grid;
     int Nz = g->Nz;
-    int nt = orc_npd_ntracers(cfg->model, cfg->carbonate_system);
+    int nt = cfg_ntracers(cfg);
     double* U = (double*)malloc(sizeof(double) * nt * Nz);
     double* Tcol = (double*)malloc(sizeof(double) * Nz);
     orc_column_env env;
@@ -612,7 +641,7 @@ int orc_grid_sediment_update_state(const obme_config* cfg, const double* const* 
                                    long* iteration, double dt)
 {
     const obme_grid* g = &cfg->grid;
-    int nt = orc_npd_ntracers(cfg->model, cfg->carbonate_system);
+    int nt = cfg_ntracers(cfg);
     double* B = (double*)malloc(sizeof(double) * nt);
     long ncols = (long)g->Nx * g->Ny;
     orc_sediment_column sc = {0};
@@ -642,7 +671,7 @@ int orc_grid_sediment_update_state(const obme_config* cfg, const double* const* 
 int orc_grid_scale_negatives(const obme_config* cfg, double* const* tracers)
 {
     const obme_grid* g = &cfg->grid;
-    int nt = orc_npd_ntracers(cfg->model, cfg->carbonate_system);
+    int nt = cfg_ntracers(cfg);
     double vals[OBME_MAX_TRACERS];
     for (int k = 0; k < g->Nz; k++)
         for (int j = 0; j < g->Ny; j++)
@@ -685,7 +714,7 @@ int orc_grid_tendencies(const obme_config* cfg, const double* const* tracers, do
 {
     const obme_grid* g = &cfg->grid;
     int Nz = g->Nz;
-    int nt = orc_npd_ntracers(cfg->model, cfg->carbonate_system);
+    int nt = cfg_ntracers(cfg);
     double* U = (double*)malloc(sizeof(double) * nt * Nz);
     double* Gc = (double*)malloc(sizeof(double) * nt * Nz);
     double* Pc = (double*)malloc(sizeof(double) * Nz);

@@ -1,0 +1,140 @@
+"""The Oxygen component of LOBSTER (NutrientsPlanktonDetritus/oxygen.jl) in
+the oracle.  Pinned by: the reference's own test properties
+(test/test_NutrientsPlanktonDetritus.jl:41-145 with oxygen = Oxygen(): the
+O2 tracer is materialised last, a zero state stays zero, nitrogen is
+conserved over 100 steps of 1 s); the one-way coupling the component
+documents (oxygen.jl:1-12: every other tendency is unchanged, bit for bit);
+and a stoichiometric identity that holds for any state and any ratios Rp,
+Rn: G_O2 + Rp G_NO3 + (Rp - Rn) G_NH4 = Rp mu0 L_PAR P L_NH4 / 2, the
+ammonium-limited share of growth, with L_PAR = PAR / (kPAR + PAR) and
+L_NH4 = NH4 / (kNH4 + NH4) formed here from the parameters (a dropped term,
+a wrong sign or swapped ratios fail it).  CPU only."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oceanbiome import LOBSTER, NPZD, CarbonateSystem, Oxygen, RectilinearGrid, abi
+from oceanbiome.models import TwoBandPhotosyntheticallyActiveRadiation, TwoParticleAndDissolved
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF = json.load(open(os.path.join(HERE, "golden", "reference_values.json")))
+
+
+def _box():
+    return RectilinearGrid(size=(1, 1, 1), extent=(1.0, 1.0, 2.0), halo=(1, 1, 1),
+                           topology=("Periodic", "Periodic", "Bounded"))
+
+
+def _lobster(grid, carbonate, oxygen=True, sinking=False):
+    det = TwoParticleAndDissolved()
+    if not sinking:
+        det.small_particle_sinking_speed = 0.0
+        det.large_particle_sinking_speed = 0.0
+    return LOBSTER(grid, detritus=det, carbonate_system=CarbonateSystem() if carbonate else None,
+                   oxygen=Oxygen() if oxygen else None,
+                   light_attenuation=TwoBandPhotosyntheticallyActiveRadiation(surface_PAR=100.0),
+                   temperature=12.0, salinity=35.0)
+
+
+def _state(rng, carbonate):
+    v = [10 * rng.random(), rng.random(), rng.random(), rng.random(), rng.random(), rng.random(), rng.random()]
+    if carbonate:
+        v += [2000 * rng.random(), 2000 * rng.random()]
+    return np.array(v + [300 * rng.random()])
+
+
+def test_host_description():
+    """oxygen.jl:14-19: defaults Rp = 10.75, Rn = 2, tracer O2 after DIC, Alk."""
+    g = _box()
+    ox = Oxygen()
+    assert (ox.respiration_oxygen_nitrogen_ratio, ox.nitrification_oxygen_nitrogen_ratio) == (10.75, 2.0)
+    assert _lobster(g, True).required_biogeochemical_tracers()[-3:] == ("DIC", "Alk", "O₂")
+    assert _lobster(g, False).required_biogeochemical_tracers()[-2:] == ("DOM", "O₂")
+    cfg = _lobster(g, True).config()
+    assert cfg.oxygen.enabled == 1 and cfg.oxygen.respiration_oxygen_nitrogen_ratio == 10.75
+    assert _lobster(g, True, oxygen=False).config().oxygen.enabled == 0
+    with pytest.raises(NotImplementedError):
+        NPZD(g, oxygen=Oxygen())
+
+
+@pytest.mark.parametrize("carbonate", [False, True])
+def test_zero_state_stays_zero(carbonate):
+    """test_NutrientsPlanktonDetritus.jl:105-117 with oxygen = Oxygen()."""
+    g = _box()
+    bgc = _lobster(g, carbonate)
+    n = len(bgc.required_biogeochemical_tracers())
+    out = O.npd_tendencies_oxygen(int(carbonate), bgc.npd_struct(), Oxygen().struct(), np.zeros(n), 100.0)
+    assert out.shape == (n,) and np.all(out == 0.0)
+    tr = [g.new_field() for _ in range(n)]
+    O.grid_run(bgc.config(), tr, 10, 60.0, nthreads=1)
+    assert all(np.all(t == 0.0) for t in tr)
+
+
+@pytest.mark.parametrize("carbonate", [False, True])
+def test_one_way_coupling(carbonate):
+    """oxygen.jl:9-12: O2 does not affect any other group."""
+    rng = np.random.default_rng(7)
+    g = _box()
+    bgc = _lobster(g, carbonate)
+    for _ in range(50):
+        v = _state(rng, carbonate)
+        with_o2 = O.npd_tendencies_oxygen(int(carbonate), bgc.npd_struct(), Oxygen().struct(), v, 80.0)
+        without = O.npd_tendencies(abi.OBME_LOBSTER, int(carbonate), bgc.npd_struct(), v[:-1], 80.0)
+        np.testing.assert_array_equal(with_o2[:-1], without)
+        # the O2 value itself is not read
+        v2 = v.copy()
+        v2[-1] = -1e6
+        np.testing.assert_array_equal(
+            O.npd_tendencies_oxygen(int(carbonate), bgc.npd_struct(), Oxygen().struct(), v2, 80.0), with_o2)
+
+
+@pytest.mark.parametrize("carbonate", [False, True])
+def test_stoichiometric_identity(carbonate):
+    rng = np.random.default_rng(11)
+    g = _box()
+    bgc = _lobster(g, carbonate)
+    q = bgc.npd_struct()
+    for it in range(200):
+        Rp, Rn = 20 * rng.random(), 5 * rng.random()
+        ox = abi.Oxygen(enabled=1, respiration_oxygen_nitrogen_ratio=Rp, nitrification_oxygen_nitrogen_ratio=Rn)
+        v = _state(rng, carbonate)
+        PAR = 200 * rng.random()
+        G = O.npd_tendencies_oxygen(int(carbonate), q, ox, v, PAR)
+        NO3, NH4, P = v[0], v[1], v[2]
+        L_PAR = PAR / (q.light_half_saturation + PAR)          # MondoLightLimitation, LOBSTER's default
+        L_NH4 = NH4 / (q.ammonia_half_saturation + NH4)
+        expect = Rp * q.phytoplankton_maximum_growth_rate * L_PAR * P * L_NH4 / 2
+        lhs = G[-1] + Rp * G[0] + (Rp - Rn) * G[1]
+        scale = abs(G[-1]) + Rp * abs(G[0]) + abs(Rp - Rn) * abs(G[1]) + abs(expect)
+        assert abs(lhs - expect) <= 1e-13 * scale, (it, lhs, expect)
+    # Rp = 0, Rn = 1: the O2 tendency is the NH4 tendency itself
+    ox = abi.Oxygen(enabled=1, respiration_oxygen_nitrogen_ratio=0.0, nitrification_oxygen_nitrogen_ratio=1.0)
+    v = _state(rng, carbonate)
+    G = O.npd_tendencies_oxygen(int(carbonate), q, ox, v, 50.0)
+    assert G[-1] == G[1]
+
+
+@pytest.mark.parametrize("carbonate", [False, True])
+def test_box_nitrogen_conservation_with_oxygen(carbonate):
+    """test_NutrientsPlanktonDetritus.jl:119-145 with oxygen: one cell, no
+    sinking, 100 steps of 1 s from random values; the nitrogen sum holds and
+    O2 moves."""
+    rng = np.random.default_rng(42)
+    g = _box()
+    bgc = _lobster(g, carbonate)
+    v = _state(rng, carbonate)
+    tr = []
+    for x in v:
+        f = g.new_field()
+        g.interior(f)[...] = x
+        tr.append(f)
+    names = [n.replace("₃", "3").replace("₄", "4") for n in bgc.required_biogeochemical_tracers()]
+    Nn = REF["conservation"]["nitrogen_LOBSTER"]
+    total = lambda: sum(float(g.interior(tr[names.index(n)])[0, 0, 0]) for n in Nn)  # noqa: E731
+    N0 = total()
+    O.grid_run(bgc.config(), tr, 100, 1.0, nthreads=1)
+    assert total() == pytest.approx(N0, rel=1e-13)
+    assert g.interior(tr[-1])[0, 0, 0] != v[-1]
