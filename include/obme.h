@@ -218,6 +218,22 @@ typedef struct obme_sediment {
     double ab2_chi;                /* χ of QuasiAdamsBashforth2, 0.1                     */
 } obme_sediment;
 
+/* OxygenGasExchangeBoundaryCondition (GasExchange/GasExchange.jl:127-146):
+ * F = k (O2 - C_air α) with k = k660(u10) / sqrt(Sc(T)/660)
+ * (gas_transfer_velocity.jl:24-34; Ho06 :61), Sc the Wanninkhof (2014)
+ * quartic (schmidt_number.jl:15-16), α = β/Tk the Wanninkhof (1992)
+ * solubility (gas_solubility.jl:28-54, with its S (B1 + B2 T/100 +
+ * B2 (T/100)^2) term as written there). */
+typedef struct obme_o2_exchange {
+    int32_t enabled;
+    int32_t pad0;
+    double air_concentration;      /* mmol O2 / m3 (9352.7) */
+    double wind_speed;             /* m/s */
+    double schmidt[5];             /* a, b, c, d, e */
+    double transfer_velocity[3];   /* k660 = c0 + c1 u + c2 u^2 */
+    double solubility[6];          /* A1, A2, A3, B1, B2, B3 */
+} obme_o2_exchange;
+
 /* Oxygen() (NutrientsPlanktonDetritus/oxygen.jl:14-33): the O₂ tracer of
  * LOBSTER, last in the tracer order (NutrientsPlanktonDetritus.jl:69-74).
  * One-way coupled: its tendency Rp μP - (Rp - Rn) G_NH₄ - Rp nitrification
@@ -245,6 +261,7 @@ typedef struct obme_config {
     obme_negative_scaling scaling; /* NPZD(grid; scale_negatives = true) etc. */
     obme_sediment sediment;        /* NPZD(; grid, sediment_model) etc.          */
     obme_oxygen oxygen;            /* LOBSTER(grid; oxygen = Oxygen())          */
+    obme_o2_exchange o2;           /* LOBSTER with O₂: the O₂ gas exchange at the top */
 } obme_config;
 
 typedef struct obme_handle obme_handle;
@@ -545,22 +562,6 @@ enum {
     OBME_PP_SCALARS = 2 * OBME_PH_COUNT + 2 * OBME_PZ_COUNT,
     OBME_PP_COUNT = 2 * OBME_PH_COUNT + 2 * OBME_PZ_COUNT + OBME_PS_COUNT
 };
-
-/* OxygenGasExchangeBoundaryCondition (GasExchange/GasExchange.jl:127-146):
- * F = k (O2 - C_air α) with k = k660(u10) / sqrt(Sc(T)/660)
- * (gas_transfer_velocity.jl:24-34; Ho06 :61), Sc the Wanninkhof (2014)
- * quartic (schmidt_number.jl:15-16), α = β/Tk the Wanninkhof (1992)
- * solubility (gas_solubility.jl:28-54, with its S (B1 + B2 T/100 +
- * B2 (T/100)^2) term as written there). */
-typedef struct obme_o2_exchange {
-    int32_t enabled;
-    int32_t pad0;
-    double air_concentration;      /* mmol O2 / m3 (9352.7) */
-    double wind_speed;             /* m/s */
-    double schmidt[5];             /* a, b, c, d, e */
-    double transfer_velocity[3];   /* k660 = c0 + c1 u + c2 u^2 */
-    double solubility[6];          /* A1, A2, A3, B1, B2, B3 */
-} obme_o2_exchange;
 
 /* PISCES configuration: grid, parameters (OBME_PP_*), MultiBand light
  * (MultiBandPhotosyntheticallyActiveRadiation, Light/multi_band.jl: per band

@@ -381,6 +381,8 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
         return fail(OBME_ERR_UNSUPPORTED, "the carbonate system is supported with LOBSTER");
     if (cfg->model == OBME_NPZD && cfg->oxygen.enabled)
         return fail(OBME_ERR_UNSUPPORTED, "the oxygen component is supported with LOBSTER");
+    if (cfg->o2.enabled && !(cfg->model == OBME_LOBSTER && cfg->oxygen.enabled))
+        return fail(OBME_ERR_INVALID, "the O2 gas exchange needs LOBSTER with the oxygen component");
     if (g.Nx < 1 || g.Ny < 1 || g.Nz < 1 || g.Hx < 0 || g.Hy < 0 || g.Hz < 0)
         return fail(OBME_ERR_INVALID, "invalid grid size or halo");
     if (g.Nz > 128) return fail(OBME_ERR_UNSUPPORTED, "Nz > 128 is not supported by the column-tile kernels");
@@ -530,6 +532,9 @@ extern "C" int32_t obme_required_tracers(const obme_handle* h, char* buf, int32_
     return h->ntracers;
 }
 
+// the sweep takes its arguments by value: within the 4 KiB kernel-argument limit
+static_assert(sizeof(StepArgs<double>) <= 4096 && sizeof(StepArgs<float>) <= 4096, "StepArgs exceeds 4 KiB");
+
 template <typename FT>
 static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers)
 {
@@ -570,7 +575,9 @@ static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers)
     a.invdz_u = (FT)1 / (FT)h->dz_u;
     a.co2_on = (c.model == OBME_LOBSTER && c.carbonate_system && c.co2.enabled && !h->co2_inline) ? 1 : 0;
     a.ns = convert_scaling<FT, 16>(c.scaling);
-    if (h->co2_inline) {
+    a.o2_on = (c.model == OBME_LOBSTER && c.oxygen.enabled && c.o2.enabled) ? 1 : 0;
+    if (a.o2_on) a.o2 = c.o2;
+    if (h->co2_inline || a.o2_on) {
         a.Tprof = (c.temperature.mode == OBME_FIELD_PROFILE) ? h->d_Tprof : nullptr;
         a.Sprof = (c.salinity.mode == OBME_FIELD_PROFILE) ? h->d_Sprof : nullptr;
         a.Tconst = c.temperature.value;

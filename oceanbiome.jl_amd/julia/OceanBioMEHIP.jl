@@ -85,6 +85,17 @@ struct ObmeCO2Exchange
     transfer_velocity :: NTuple{3, Float64}
 end
 
+# OxygenGasExchangeBoundaryCondition (GasExchange/GasExchange.jl:127-146)
+struct ObmeO2Exchange
+    enabled :: Int32
+    pad0 :: Int32
+    air_concentration :: Float64     # 9352.7 mmol O₂ / m³
+    wind_speed :: Float64
+    schmidt :: NTuple{5, Float64}
+    transfer_velocity :: NTuple{3, Float64}
+    solubility :: NTuple{6, Float64}
+end
+
 struct ObmeGrid
     Nx :: Int32; Ny :: Int32; Nz :: Int32
     Hx :: Int32; Hy :: Int32; Hz :: Int32
@@ -159,6 +170,7 @@ struct ObmeConfig
     scaling :: ObmeNegativeScaling
     sediment :: ObmeSediment
     oxygen :: ObmeOxygen
+    o2 :: ObmeO2Exchange             # the O₂ gas exchange of LOBSTER's O₂
 end
 
 # layout check against the library (obme_abi_sizeof)
@@ -168,7 +180,7 @@ function check_abi()
         n = ccall((:obme_abi_sizeof, libobme), Int64, (Int32,), idx - 1)
         n == sizeof(T) || error("ABI mismatch for $T: library $n bytes, Julia $(sizeof(T))")
     end
-    for (idx, T) in ((10, ObmeNegativeScaling), (12, ObmeSediment), (13, ObmeOxygen))
+    for (idx, T) in ((10, ObmeNegativeScaling), (11, ObmeO2Exchange), (12, ObmeSediment), (13, ObmeOxygen))
         n = ccall((:obme_abi_sizeof, libobme), Int64, (Int32,), idx)
         n == sizeof(T) || error("ABI mismatch for $T: library $n bytes, Julia $(sizeof(T))")
     end

@@ -182,6 +182,19 @@ class Sediment(C.Structure):
     ]
 
 
+class O2Exchange(C.Structure):
+    """obme_o2_exchange."""
+    _fields_ = [
+        ("enabled", C.c_int32),
+        ("pad0", C.c_int32),
+        ("air_concentration", C.c_double),
+        ("wind_speed", C.c_double),
+        ("schmidt", C.c_double * 5),
+        ("transfer_velocity", C.c_double * 3),
+        ("solubility", C.c_double * 6),
+    ]
+
+
 class Oxygen(C.Structure):
     """obme_oxygen (the Oxygen() component of LOBSTER)."""
     _fields_ = [
@@ -209,6 +222,7 @@ class Config(C.Structure):
         ("scaling", NegativeScaling),
         ("sediment", Sediment),
         ("oxygen", Oxygen),
+        ("o2", O2Exchange),
     ]
 
 
@@ -218,19 +232,6 @@ class BatchParams(C.Structure):
 
 
 OBME_PP_COUNT = 159   # obme_pisces_config.params length (2*31 + 2*18 + 61)
-
-
-class O2Exchange(C.Structure):
-    """obme_o2_exchange."""
-    _fields_ = [
-        ("enabled", C.c_int32),
-        ("pad0", C.c_int32),
-        ("air_concentration", C.c_double),
-        ("wind_speed", C.c_double),
-        ("schmidt", C.c_double * 5),
-        ("transfer_velocity", C.c_double * 3),
-        ("solubility", C.c_double * 6),
-    ]
 
 
 class PiscesConfig(C.Structure):

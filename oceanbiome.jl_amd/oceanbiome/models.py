@@ -304,7 +304,11 @@ class Biogeochemistry:
         self.carbonate_system = carbonate_system
         self.oxygen = oxygen
         self.light_attenuation = light_attenuation or TwoBandPhotosyntheticallyActiveRadiation()
-        self.gas_exchange = gas_exchange
+        # a CO2 and/or an O2 boundary condition (one or a tuple of both)
+        ges = () if gas_exchange is None else (tuple(gas_exchange) if isinstance(gas_exchange, (tuple, list))
+                                                else (gas_exchange,))
+        self.gas_exchange = next((g for g in ges if isinstance(g, CarbonDioxideGasExchangeBoundaryCondition)), None)
+        self.o2_exchange = next((g for g in ges if isinstance(g, OxygenGasExchangeBoundaryCondition)), None)
         self.sinking_scheme = sinking_scheme
         self.temperature = temperature
         self.salinity = salinity
@@ -318,9 +322,11 @@ class Biogeochemistry:
             raise NotImplementedError("the carbonate system is supported with LOBSTER")
         if oxygen is not None and not isinstance(oxygen, Oxygen):
             raise TypeError("oxygen must be Oxygen() or None")
-        if gas_exchange is not None and not isinstance(gas_exchange, CarbonDioxideGasExchangeBoundaryCondition):
-            # the NPZD/LOBSTER step couples the CO2 flux only; the O2 flux runs with PISCES
-            raise NotImplementedError("NPZD and LOBSTER take the CO2 gas exchange")
+        if len(ges) != (self.gas_exchange is not None) + (self.o2_exchange is not None):
+            raise NotImplementedError("gas_exchange takes one CO2 and one O2 boundary condition at most")
+        if self.o2_exchange is not None and (self.model != abi.OBME_LOBSTER or oxygen is None):
+            # the O2 flux needs the O2 tracer (LOBSTER with Oxygen())
+            raise ValueError("the O2 gas exchange needs LOBSTER with oxygen=Oxygen()")
         if self.model == abi.OBME_NPZD and oxygen is not None:
             # NPZD's Oxygen method (oxygen.jl:35-46) is not built
             raise NotImplementedError("the oxygen component is supported with LOBSTER")
@@ -456,6 +462,8 @@ class Biogeochemistry:
         cfg.solver = cc.solver.struct(cc.initial_pH_guess)
         if self.gas_exchange is not None and self.carbonate_system is not None:
             cfg.co2 = self.gas_exchange.struct()
+        if self.o2_exchange is not None:
+            cfg.o2 = self.o2_exchange.struct()
         cfg.temperature = self._field_source(self.temperature)
         cfg.salinity = self._field_source(self.salinity)
         cfg.scaling = self.scaling_struct()

@@ -80,6 +80,7 @@ void orc_calcite_saturation_batch(const double* DIC, const double* Alk, const do
                                   const obme_carbon_solver* s, int nthreads);
 
 double orc_schmidt_co2(const obme_co2_exchange* g, double T);
+double orc_o2_flux(const obme_o2_exchange* g, double O2, double T, double S);
 double orc_transfer_velocity(const obme_co2_exchange* g, double u10, double T, double S);
 /* F = k (pCO2_w - pCO2_air); also returns the free-scale pH of the solve. */
 double orc_co2_flux(const obme_co2_exchange* g, const obme_carbon_solver* s,

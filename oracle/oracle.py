@@ -73,6 +73,7 @@ def lib():
                                       DP, D, DP, DP, C.c_long, I]),
         "orc_npd_tendencies": (None, [I, I, C.POINTER(abi.NPDParams), DP, D, DP]),
         "orc_npd_ntracers": (I, [I, I]),
+        "orc_o2_flux": (D, [C.POINTER(abi.O2Exchange), D, D, D]),
         "orc_npd_tendencies_oxygen": (None, [I, C.POINTER(abi.NPDParams), C.POINTER(abi.Oxygen), DP, D, DP]),
         "orc_pow_int4": (D, [D]),
         "orc_two_band_column": (None, [C.POINTER(abi.TwoBandPAR), I, DP, DP, DP, D, DP]),

@@ -612,6 +612,25 @@ double orc_transfer_velocity(const obme_co2_exchange* g, double u10, double T, d
     return k660 / sqrt(sc / 660.0) * sol;
 }
 
+/* GasExchange functor (gas_exchange.jl:26-38) for O2
+ * (OxygenGasExchangeBoundaryCondition, GasExchange.jl:127-146):
+ * F = k (O2 - C_air beta / Tk), k = k660(u) / sqrt(Sc(T) / 660)
+ * (gas_transfer_velocity.jl:33-34), Sc the quartic of schmidt_number.jl:15-16,
+ * beta the solubility of gas_solubility.jl:37-48 (B2 in both salinity terms,
+ * as written there). */
+double orc_o2_flux(const obme_o2_exchange* g, double O2, double T, double S)
+{
+    const double* c = g->schmidt;
+    double Sc = c[0] + c[1] * T + c[2] * (T * T) + c[3] * (T * T * T) + c[4] * orc_pow_int4(T);
+    const double* v = g->transfer_velocity;
+    double u = g->wind_speed;
+    double k = (v[0] + v[1] * u + v[2] * (u * u)) / sqrt(Sc / 660.0) * 1.0;
+    const double* a = g->solubility;
+    double Tk = T + 273.15, Tk100 = Tk / 100.0;
+    double beta = exp(a[0] + a[1] / Tk100 + a[2] * log(Tk100) + S * (a[3] + a[4] * Tk100 + a[4] * (Tk100 * Tk100)));
+    return k * (O2 - g->air_concentration * (beta / Tk));
+}
+
 /* GasExchange functor (gas_exchange.jl:26-38) with CarbonDioxideConcentration
  * surface_value (carbon_dioxide_concentration.jl:48-60). */
 double orc_co2_flux(const obme_co2_exchange* g, const obme_carbon_solver* s, double DIC, double Alk,

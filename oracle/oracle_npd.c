@@ -451,6 +451,13 @@ void orc_column_tendencies(const orc_column_env* env, const double* U, double t,
         G[7 * Nz + top] = G[7 * Nz + top] - F / env->dz[top];
     }
 
+    /* top O2 flux boundary condition of LOBSTER's O2: G[Nz] -= F/Δz */
+    if (cfg->model == OBME_LOBSTER && cfg->oxygen.enabled && cfg->o2.enabled) {
+        int top = Nz - 1, n = nt - 1;
+        double F = orc_o2_flux(&cfg->o2, U[n * Nz + top], env->T_surface, env->S_surface);
+        G[n * Nz + top] = G[n * Nz + top] - F / env->dz[top];
+    }
+
     /* the sediment's coupled tendency (Sediments/tracer_coupling.jl:3-38):
      * the receiver's bottom cell gains (1 - e) f / Δz, Δz at the bottom face
      * taken as the bottom cell's thickness */

@@ -2,7 +2,8 @@
 fused step with and without the carbonate system, with the CO2 exchange,
 negative scaling and the InstantRemineralisation sediment compiled in beside
 it, at a ragged depth and at full 64-level columns (the FP32 default-
-formulation kernel), and the update_tendencies! hook.  The other tracers
+formulation kernel), with the O2 gas exchange at the top, and the
+update_tendencies! hook.  The other tracers
 match the same model without O2 (one-way coupling).  FP64 <= 1e-12
 relative L2 per tracer; FP32-mixed <= 1e-5."""
 import numpy as np
@@ -11,7 +12,7 @@ import pytest
 import oracle as O
 from helpers import rel_l2, to_device, to_host
 from oceanbiome import (LOBSTER, CarbonateSystem, CarbonDioxideGasExchangeBoundaryCondition,
-                        InstantRemineralisationSediment, Oxygen, RectilinearGrid,
+                        InstantRemineralisationSediment, Oxygen, OxygenGasExchangeBoundaryCondition, RectilinearGrid,
                         TwoBandPhotosyntheticallyActiveRadiation, abi)
 from oceanbiome.runtime import HIPBiogeochemistry
 
@@ -20,7 +21,7 @@ TOL = {"fp64": 1e-12, "fp32": 1e-5}
 RANGES = ((5.0, 12.0), (0.1, 1.0), (0.05, 0.5), (0.05, 0.3), (0.5, 3.0), (0.5, 3.0), (0.1, 1.0))
 
 
-def _model(shape, carbonate, co2=False, scale=False, sediment=False, oxygen=True, seed=3):
+def _model(shape, carbonate, co2=False, scale=False, sediment=False, oxygen=True, o2x=False, seed=3):
     Nx, Ny, Nz = shape
     grid = RectilinearGrid(size=shape, extent=(10.0 * Nx, 10.0 * Ny, 4.0 * Nz), halo=(3, 3, 3),
                            topology=("Periodic", "Periodic", "Bounded"))
@@ -30,7 +31,8 @@ def _model(shape, carbonate, co2=False, scale=False, sediment=False, oxygen=True
     bgc = LOBSTER(grid, carbonate_system=CarbonateSystem() if carbonate else None,
                   oxygen=Oxygen() if oxygen else None,
                   light_attenuation=TwoBandPhotosyntheticallyActiveRadiation(surface_PAR=80.0),
-                  gas_exchange=CarbonDioxideGasExchangeBoundaryCondition() if co2 else None,
+                  gas_exchange=tuple([CarbonDioxideGasExchangeBoundaryCondition()] if co2 else []) +
+                  tuple([OxygenGasExchangeBoundaryCondition()] if o2x else []),
                   temperature=20 - 0.05 * np.abs(grid.zc), salinity=35.0, scale_negatives=scale,
                   sediment_model=sed)
     rng = np.random.default_rng(seed)
@@ -87,6 +89,33 @@ def test_oxygen_with_co2_scaling_sediment(precision):
     bgc, grid, fields = _model((12, 3, 24), True, co2=True, scale=True, sediment=True)
     errs, _ = _run(bgc, grid, fields, precision, 3, 300.0)
     assert all(e <= TOL[precision] for e in errs.values()), errs
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("carbonate,co2,scale", [(False, False, False), (True, True, True), (True, False, False)])
+@pytest.mark.parametrize("Nz", [23, 64])
+def test_oxygen_gas_exchange(precision, carbonate, co2, scale, Nz):
+    print("case oxygen gas exchange", precision, carbonate, co2, scale, Nz, flush=True)
+    bgc, grid, fields = _model((9, 2, Nz), carbonate, co2=co2, scale=scale, o2x=True, seed=13)
+    errs, _ = _run(bgc, grid, fields, precision, 3, 600.0)
+    assert all(e <= TOL[precision] for e in errs.values()), errs
+
+
+def test_add_tendencies_with_oxygen_gas_exchange():
+    import torch
+    print("case oxygen gas exchange tendencies", flush=True)
+    bgc, grid, fields = _model((7, 3, 18), True, o2x=True, seed=17)
+    cfg = bgc.config(abi.OBME_FP64)
+    G_ref = [np.zeros_like(a) for a in fields]
+    O.grid_tendencies(cfg, fields, G_ref)
+    dev = HIPBiogeochemistry(bgc, "fp64")
+    tr = to_device(fields, torch.float64)
+    G = [torch.zeros_like(t) for t in tr]
+    dev.update_tendencies(tr, G)
+    torch.cuda.synchronize()
+    out = to_host(G)
+    for n, name in enumerate(bgc.required_biogeochemical_tracers()):
+        assert rel_l2(grid.interior(out[n]), grid.interior(G_ref[n])) <= 1e-12, name
 
 
 def test_other_tracers_match_the_model_without_oxygen():

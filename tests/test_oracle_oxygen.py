@@ -8,7 +8,10 @@ and a stoichiometric identity that holds for any state and any ratios Rp,
 Rn: G_O2 + Rp G_NO3 + (Rp - Rn) G_NH4 = Rp mu0 L_PAR P L_NH4 / 2, the
 ammonium-limited share of growth, with L_PAR = PAR / (kPAR + PAR) and
 L_NH4 = NH4 / (kNH4 + NH4) formed here from the parameters (a dropped term,
-a wrong sign or swapped ratios fail it).  CPU only."""
+a wrong sign or swapped ratios fail it).  The O2 gas exchange on that
+tracer: the C flux against the restatement the PISCES coupling uses, zero
+at saturation, and the top cell's O2 tendency as the only change, by
+-F / dz.  CPU only."""
 import json
 import os
 
@@ -138,3 +141,66 @@ def test_box_nitrogen_conservation_with_oxygen(carbonate):
     O.grid_run(bgc.config(), tr, 100, 1.0, nthreads=1)
     assert total() == pytest.approx(N0, rel=1e-13)
     assert g.interior(tr[-1])[0, 0, 0] != v[-1]
+
+
+# ---- the O2 gas exchange on LOBSTER's O2 (GasExchange.jl:127-146) ----------
+
+def test_o2_flux_in_c_matches_the_pisces_restatement():
+    """orc_o2_flux (the NPD coupling) against oracle.o2_flux (the PISCES
+    coupling, pinned in the PISCES tests); zero at saturation; slope k."""
+    from oceanbiome import OxygenGasExchangeBoundaryCondition
+    g = OxygenGasExchangeBoundaryCondition().struct()
+    rng = np.random.default_rng(5)
+    for _ in range(100):
+        O2, T, S = 400 * rng.random(), 30 * rng.random(), 30 + 8 * rng.random()
+        f = O.lib().orc_o2_flux(g, O2, T, S)
+        assert f == pytest.approx(O.o2_flux(g, O2, T, S), rel=1e-14, abs=1e-18)
+        sat = O.o2_air_concentration(g, T, S)
+        assert abs(O.lib().orc_o2_flux(g, sat, T, S)) <= 1e-14 * sat
+        k = O.lib().orc_o2_flux(g, O2 + 1.0, T, S) - f
+        assert k > 0
+
+
+def test_o2_flux_is_the_only_difference_at_the_top():
+    from oceanbiome import OxygenGasExchangeBoundaryCondition
+    grid = RectilinearGrid(size=(3, 2, 6), extent=(3.0, 2.0, 30.0), halo=(1, 1, 1),
+                           topology=("Periodic", "Periodic", "Bounded"))
+    rng = np.random.default_rng(9)
+
+    def model(o2):
+        return LOBSTER(grid, carbonate_system=CarbonateSystem(), oxygen=Oxygen(),
+                       gas_exchange=OxygenGasExchangeBoundaryCondition() if o2 else None,
+                       light_attenuation=TwoBandPhotosyntheticallyActiveRadiation(surface_PAR=50.0),
+                       temperature=14.0, salinity=34.0)
+    tr = []
+    for lo, hi in [(5, 12), (0.1, 1), (0.05, 0.5), (0.05, 0.3), (0.5, 3), (0.5, 3), (0.1, 1),
+                   (2000, 2200), (2300, 2400), (150, 300)]:
+        f = grid.new_field()
+        grid.interior(f)[...] = rng.uniform(lo, hi, (6, 2, 3))
+        tr.append(f)
+    G0 = [np.zeros_like(a) for a in tr]
+    G1 = [np.zeros_like(a) for a in tr]
+    O.grid_tendencies(model(False).config(), tr, G0)
+    O.grid_tendencies(model(True).config(), tr, G1)
+    for n in range(9):
+        np.testing.assert_array_equal(G1[n], G0[n])
+    d = grid.interior(G1[9]) - grid.interior(G0[9])
+    assert np.all(d[:-1] == 0)
+    g = OxygenGasExchangeBoundaryCondition().struct()
+    top = grid.interior(tr[9])[-1]
+    expect = -np.vectorize(lambda c: O.o2_flux(g, c, 14.0, 34.0))(top) / 5.0
+    np.testing.assert_allclose(d[-1], expect, rtol=1e-12)
+
+
+def test_host_gas_exchange_pairs():
+    from oceanbiome import CarbonDioxideGasExchangeBoundaryCondition, OxygenGasExchangeBoundaryCondition
+    g = _box()
+    bgc = LOBSTER(g, carbonate_system=CarbonateSystem(), oxygen=Oxygen(),
+                  gas_exchange=(CarbonDioxideGasExchangeBoundaryCondition(), OxygenGasExchangeBoundaryCondition()))
+    cfg = bgc.config()
+    assert cfg.co2.enabled == 1 and cfg.o2.enabled == 1 and cfg.o2.air_concentration == 9352.7
+    with pytest.raises(ValueError):
+        LOBSTER(g, gas_exchange=OxygenGasExchangeBoundaryCondition())
+    with pytest.raises(NotImplementedError):
+        LOBSTER(g, oxygen=Oxygen(),
+                gas_exchange=(OxygenGasExchangeBoundaryCondition(), OxygenGasExchangeBoundaryCondition()))
