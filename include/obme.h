@@ -234,10 +234,11 @@ typedef struct obme_o2_exchange {
     double solubility[6];          /* A1, A2, A3, B1, B2, B3 */
 } obme_o2_exchange;
 
-/* Oxygen() (NutrientsPlanktonDetritus/oxygen.jl:14-33): the O₂ tracer of
- * LOBSTER, last in the tracer order (NutrientsPlanktonDetritus.jl:69-74).
- * One-way coupled: its tendency Rp μP - (Rp - Rn) G_NH₄ - Rp nitrification
- * reads the other tracers and changes none of them. */
+/* Oxygen() (NutrientsPlanktonDetritus/oxygen.jl:14-46): the O₂ tracer of
+ * LOBSTER or NPZD, last in the tracer order (NutrientsPlanktonDetritus.jl:69-74).
+ * One-way coupled: its tendency (LOBSTER: Rp μP - (Rp - Rn) G_NH₄ - Rp
+ * nitrification; NPZD: Rp μP - (Rp - Rn) × the plankton's inorganic nitrogen
+ * waste) reads the other tracers and changes none of them. */
 typedef struct obme_oxygen {
     int32_t enabled;               /* 1 adds O₂                                          */
     int32_t pad0;
@@ -260,7 +261,7 @@ typedef struct obme_config {
     obme_field_source salinity;
     obme_negative_scaling scaling; /* NPZD(grid; scale_negatives = true) etc. */
     obme_sediment sediment;        /* NPZD(; grid, sediment_model) etc.          */
-    obme_oxygen oxygen;            /* LOBSTER(grid; oxygen = Oxygen())          */
+    obme_oxygen oxygen;            /* NPZD/LOBSTER(grid; oxygen = Oxygen())     */
     obme_o2_exchange o2;           /* LOBSTER with O₂: the O₂ gas exchange at the top */
 } obme_config;
 

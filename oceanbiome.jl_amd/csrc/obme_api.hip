@@ -286,7 +286,7 @@ static NegScale<FT, NTMAX> convert_scaling(const obme_negative_scaling& s)
 
 static int ntracers_of(const obme_config& c)
 {
-    if (c.model == OBME_NPZD) return 5;
+    if (c.model == OBME_NPZD) return c.oxygen.enabled ? 6 : 5;
     if (c.model == OBME_LOBSTER) return (c.carbonate_system ? 9 : 7) + (c.oxygen.enabled ? 1 : 0);
     return 0;
 }
@@ -379,8 +379,6 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
         return fail(OBME_ERR_INVALID, "unknown precision");
     if (cfg->model == OBME_NPZD && cfg->carbonate_system)
         return fail(OBME_ERR_UNSUPPORTED, "the carbonate system is supported with LOBSTER");
-    if (cfg->model == OBME_NPZD && cfg->oxygen.enabled)
-        return fail(OBME_ERR_UNSUPPORTED, "the oxygen component is supported with LOBSTER");
     if (cfg->o2.enabled && !(cfg->model == OBME_LOBSTER && cfg->oxygen.enabled))
         return fail(OBME_ERR_INVALID, "the O2 gas exchange needs LOBSTER with the oxygen component");
     if (g.Nx < 1 || g.Ny < 1 || g.Nz < 1 || g.Hx < 0 || g.Hy < 0 || g.Hz < 0)
@@ -521,7 +519,7 @@ extern "C" int32_t obme_required_tracers(const obme_handle* h, char* buf, int32_
 {
     if (!h) return -1;
     const char* names;
-    if (h->cfg.model == OBME_NPZD) names = "N;P;Z;T;D";
+    if (h->cfg.model == OBME_NPZD) names = h->cfg.oxygen.enabled ? "N;P;Z;T;D;O₂" : "N;P;Z;T;D";
     else if (h->cfg.carbonate_system)
         names = h->cfg.oxygen.enabled ? "NO₃;NH₄;P;Z;sPOM;bPOM;DOM;DIC;Alk;O₂" : "NO₃;NH₄;P;Z;sPOM;bPOM;DOM;DIC;Alk";
     else names = h->cfg.oxygen.enabled ? "NO₃;NH₄;P;Z;sPOM;bPOM;DOM;O₂" : "NO₃;NH₄;P;Z;sPOM;bPOM;DOM";
@@ -753,7 +751,10 @@ static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_
 template <typename FT, int L, int CPW, int NW, int MODE>
 static obme_status dispatch_model(obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
 {
-    if (h->cfg.model == OBME_NPZD) return launch_wave<OBME_NPZD, 0, FT, L, CPW, NW, MODE>(h, a, s);
+    if (h->cfg.model == OBME_NPZD) {
+        if (h->cfg.oxygen.enabled) return launch_wave<OBME_NPZD, 2, FT, L, CPW, NW, MODE>(h, a, s);
+        return launch_wave<OBME_NPZD, 0, FT, L, CPW, NW, MODE>(h, a, s);
+    }
     // LOBSTER's components: 1 CarbonateSystem, 2 Oxygen
     const int comp = (h->cfg.carbonate_system ? 1 : 0) | (h->cfg.oxygen.enabled ? 2 : 0);
     if (comp == 1) return launch_wave<OBME_LOBSTER, 1, FT, L, CPW, NW, MODE>(h, a, s);
@@ -823,6 +824,7 @@ static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* 
         const int bs = 256;
         const long ncell = (long)h->cfg.grid.Nx * h->cfg.grid.Ny * h->cfg.grid.Nz;
         auto ks = k_scale_grid<FT, 5>;
+        if (h->ntracers == 6) ks = k_scale_grid<FT, 6>;
         if (h->ntracers == 7) ks = k_scale_grid<FT, 7>;
         if (h->ntracers == 8) ks = k_scale_grid<FT, 8>;
         if (h->ntracers == 9) ks = k_scale_grid<FT, 9>;
@@ -1210,6 +1212,8 @@ extern "C" obme_status obme_batch_create(const obme_config* cfg, int32_t n_colum
         return fail(OBME_ERR_UNSUPPORTED, "negative scaling is not built for the batched ensemble");
     if (cfg->sediment.model != OBME_SEDIMENT_NONE)
         return fail(OBME_ERR_UNSUPPORTED, "a sediment is not built for the batched ensemble");
+    if (cfg->oxygen.enabled || cfg->o2.enabled)
+        return fail(OBME_ERR_UNSUPPORTED, "the oxygen component is not built for the batched ensemble");
     if (cfg->precision != OBME_FP64 && cfg->precision != OBME_FP32_MIXED)
         return fail(OBME_ERR_INVALID, "unknown precision");
     if (n_columns < 1) return fail(OBME_ERR_INVALID, "no columns");

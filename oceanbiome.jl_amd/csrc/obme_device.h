@@ -211,10 +211,15 @@ __device__ __forceinline__ FT mortality(const NPDParams<FT>& q, FT P)
     return q.mort_form == OBME_LINEAR ? q.mP * P : q.mP * (P * P);
 }
 
-// NPZD transitions (N, P, Z, D); T has zero tendency.
+// NPZD transitions (N, P, Z, D); T has zero tendency.  With dO2 the O2
+// transition of the Oxygen component's Nutrient method (oxygen.jl:35-46):
+// Rp μP - (Rp - Rn) respiration, where respiration is the plankton's
+// inorganic nitrogen waste alone (the line that adds the detritus waste,
+// oxygen.jl:42, begins with a unary + and is a separate, discarded
+// expression in Julia).
 template <typename FT>
 __device__ __forceinline__ void npzd_tendencies(const NPDParams<FT>& q, FT N, FT P, FT Z, FT T, FT D, FT PAR,
-                                                FT Lt, FT& dN, FT& dP, FT& dZ, FT& dD)
+                                                FT Lt, FT& dN, FT& dP, FT& dZ, FT& dD, FT* dO2 = nullptr)
 {
     // phytoplankton_growth (plankton.jl:191-206), Nutrient limitation (:180-188)
     FT Ln = m_div(N, N + q.kNO3);
@@ -232,6 +237,7 @@ __device__ __forceinline__ void npzd_tendencies(const NPDParams<FT>& q, FT N, FT
     FT ponw = (FT(1) - q.alpha_x) * q.gamma * muP + (FT(1) - q.aZ_exc) * q.muZ * Z;  // :312-325
     FT solid = (FT(1) - q.aZ_ass) * G.total + q.aP_solid * nuP + q.mZ * (Z * Z);    // :330-346
     dD = ponw + solid - G.onSmall - q.rem * D;                     // detritus.jl:373-379
+    if (dO2) *dO2 = q.Rp * muP - (q.Rp - q.Rn) * pinw;
 }
 
 // LOBSTER transitions; v/out in tracer order NO3, NH4, P, Z, sPOM, bPOM, DOM[, DIC, Alk][, O2].
@@ -470,10 +476,11 @@ __device__ __forceinline__ void lobster_fast(const FastParams& f, const float* v
     }
 }
 
-// NPZD transitions in FP32 (N, P, Z, D); Lt = Q10^(T/10).
+// NPZD transitions in FP32 (N, P, Z, D); Lt = Q10^(T/10); dO2 as in npzd_tendencies.
 template <int FM = -1>
 __device__ __forceinline__ void npzd_fast(const FastParams& f, float N, float P, float Z, float D, float PAR,
-                                          float Lt, float& dN, float& dP, float& dZ, float& dD)
+                                          float Lt, float& dN, float& dP, float& dZ, float& dD,
+                                          float* dO2 = nullptr)
 {
     const float muP = f.mu0 * fast_light<FM>(f, PAR) * (N * frcp(N + f.kNO3)) * Lt * P;
     const float sPOM = D * f.small_frac;
@@ -487,6 +494,7 @@ __device__ __forceinline__ void npzd_fast(const FastParams& f, float N, float P,
     const float ponw = fmaf(f.ponw_mu, muP, f.ponw_z * Z);
     const float solid = fmaf(f.one_m_aZass, G.total, fmaf(f.aPs, nuP, f.mZ * Z2));
     dD = ponw + solid - G.onSmall - f.rem * D;
+    if (dO2) *dO2 = fmaf(f.oRp, muP, -f.oRpn * pinw);
 }
 
 // ScaleNegativeTracers groups for the kernels: members as bits of the

@@ -43,8 +43,8 @@ namespace obme {
 constexpr int MAX_SINK = 2;
 
 // Per-model tracer layout (tracer order of required_biogeochemical_tracers).
-// CARB is a bit set of LOBSTER's optional components: 1 CarbonateSystem
-// (DIC, Alk at 7, 8), 2 Oxygen (O₂, last).
+// CARB is a bit set of the optional components: 1 CarbonateSystem (LOBSTER:
+// DIC, Alk at 7, 8), 2 Oxygen (O₂, last).
 template <int MODEL, int CARB>
 struct ModelTraits;
 
@@ -54,6 +54,13 @@ struct ModelTraits<OBME_NPZD, false> {
     static constexpr int P_IDX = 1;
     static constexpr int RO_IDX = 3;        // T: read-only tracer
     static constexpr int SINK0 = 1, SINK1 = 4;  // P, D
+};
+template <>
+struct ModelTraits<OBME_NPZD, 2> {
+    static constexpr int NT = 6;            // N, P, Z, T, D, O₂
+    static constexpr int P_IDX = 1;
+    static constexpr int RO_IDX = 3;
+    static constexpr int SINK0 = 1, SINK1 = 4;
 };
 template <>
 struct ModelTraits<OBME_LOBSTER, false> {
@@ -563,11 +570,13 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
                 if constexpr (MODEL == OBME_NPZD) {
                     if constexpr (F32) {
                         const float Lt = fm_q10<FM>(a.f) ? m_exp2(v[3] * a.f.log2Q10_10) : 1.0f;
-                        npzd_fast<FM>(a.f, v[0], v[1], v[2], v[4], PARv[m], Lt, Gv[0], Gv[1], Gv[2], Gv[4]);
+                        npzd_fast<FM>(a.f, v[0], v[1], v[2], v[4], PARv[m], Lt, Gv[0], Gv[1], Gv[2], Gv[4],
+                                      (CARB & 2) ? &Gv[NT - 1] : nullptr);
                     } else {
                         FT Lt = FT(1);
                         if (a.q.use_Q10) Lt = m_pow(a.q.Q10, v[3] / FT(10));
-                        npzd_tendencies(a.q, v[0], v[1], v[2], v[3], v[4], PARv[m], Lt, Gv[0], Gv[1], Gv[2], Gv[4]);
+                        npzd_tendencies(a.q, v[0], v[1], v[2], v[3], v[4], PARv[m], Lt, Gv[0], Gv[1], Gv[2], Gv[4],
+                                        (CARB & 2) ? &Gv[NT - 1] : nullptr);
                     }
                     Gv[3] = FT(0);
                 } else {

@@ -145,7 +145,7 @@ struct ObmeSediment
     ab2_chi :: Float64
 end
 
-# Oxygen() (NutrientsPlanktonDetritus/oxygen.jl): LOBSTER's O₂ tracer, last in
+# Oxygen() (NutrientsPlanktonDetritus/oxygen.jl): the O₂ tracer of NPZD or LOBSTER, last in
 # the tracer order; enabled = 0: no O₂.
 struct ObmeOxygen
     enabled :: Int32

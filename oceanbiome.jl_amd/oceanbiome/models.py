@@ -117,8 +117,8 @@ class CarbonateSystem:
 @dataclass
 class Oxygen:
     """Oxygen (NutrientsPlanktonDetritus/oxygen.jl:14-17): the O₂ tracer of
-    LOBSTER, one-way coupled (its tendency, oxygen.jl:21-31, changes no other
-    tracer)."""
+    LOBSTER or NPZD, one-way coupled (its tendencies, oxygen.jl:21-46, change
+    no other tracer)."""
     respiration_oxygen_nitrogen_ratio: float = 10.75
     nitrification_oxygen_nitrogen_ratio: float = 2.0
     tracers = ("O₂",)
@@ -327,9 +327,6 @@ class Biogeochemistry:
         if self.o2_exchange is not None and (self.model != abi.OBME_LOBSTER or oxygen is None):
             # the O2 flux needs the O2 tracer (LOBSTER with Oxygen())
             raise ValueError("the O2 gas exchange needs LOBSTER with oxygen=Oxygen()")
-        if self.model == abi.OBME_NPZD and oxygen is not None:
-            # NPZD's Oxygen method (oxygen.jl:35-46) is not built
-            raise NotImplementedError("the oxygen component is supported with LOBSTER")
         self._keep = []
 
     # -- Oceananigans.Biogeochemistry API -------------------------------------

@@ -97,8 +97,8 @@ void orc_co2_flux_batch(const obme_co2_exchange* g, const obme_carbon_solver* s,
 void orc_npd_tendencies(int model, int carbonate, const obme_npd_params* p,
                         const double* vals, double PAR, double* out);
 int orc_npd_ntracers(int model, int carbonate);
-/* LOBSTER with the Oxygen component: tracer order as above with O2 last. */
-void orc_npd_tendencies_oxygen(int carbonate, const obme_npd_params* p, const obme_oxygen* ox,
+/* NPZD or LOBSTER with the Oxygen component: tracer order as above with O2 last. */
+void orc_npd_tendencies_oxygen(int model, int carbonate, const obme_npd_params* p, const obme_oxygen* ox,
                                const double* vals, double PAR, double* out);
 
 /* ---- light ---- */

@@ -74,7 +74,7 @@ def lib():
         "orc_npd_tendencies": (None, [I, I, C.POINTER(abi.NPDParams), DP, D, DP]),
         "orc_npd_ntracers": (I, [I, I]),
         "orc_o2_flux": (D, [C.POINTER(abi.O2Exchange), D, D, D]),
-        "orc_npd_tendencies_oxygen": (None, [I, C.POINTER(abi.NPDParams), C.POINTER(abi.Oxygen), DP, D, DP]),
+        "orc_npd_tendencies_oxygen": (None, [I, I, C.POINTER(abi.NPDParams), C.POINTER(abi.Oxygen), DP, D, DP]),
         "orc_pow_int4": (D, [D]),
         "orc_two_band_column": (None, [C.POINTER(abi.TwoBandPAR), I, DP, DP, DP, D, DP]),
         "orc_default_surface_PAR": (D, [D]),
@@ -276,12 +276,12 @@ def npd_tendencies(model, carbonate, params, vals, PAR):
     return out
 
 
-def npd_tendencies_oxygen(carbonate, params, oxygen, vals, PAR):
-    """LOBSTER transitions with the Oxygen component (O2 last)."""
-    n = lib().orc_npd_ntracers(abi.OBME_LOBSTER, carbonate) + 1
+def npd_tendencies_oxygen(carbonate, params, oxygen, vals, PAR, model=abi.OBME_LOBSTER):
+    """NPZD or LOBSTER transitions with the Oxygen component (O2 last)."""
+    n = lib().orc_npd_ntracers(model, carbonate) + 1
     v = np.ascontiguousarray(vals, dtype=np.float64)
     out = np.zeros(n)
-    lib().orc_npd_tendencies_oxygen(carbonate, C.byref(params), C.byref(oxygen), _dp(v), PAR, _dp(out))
+    lib().orc_npd_tendencies_oxygen(model, carbonate, C.byref(params), C.byref(oxygen), _dp(v), PAR, _dp(out))
     return out
 
 

@@ -70,7 +70,7 @@ static grazing_terms grazing(const obme_npd_params* q, double P, double Z, doubl
 }
 
 /* NPZD: Nutrient + PhytoZoo + Detritus; tracers (N, P, Z, T, D). */
-static void npzd_tendencies(const obme_npd_params* q, const double* v, double PAR, double* out)
+static void npzd_tendencies(const obme_npd_params* q, const double* v, double PAR, double* out, const obme_oxygen* ox)
 {
     double N = v[0], P = v[1], Z = v[2], T = v[3], D = v[4];
 
@@ -108,6 +108,16 @@ static void npzd_tendencies(const obme_npd_params* q, const double* v, double PA
     double solid = (1 - q->zooplankton_assimilation_fraction) * G.total
                    + q->phytoplankton_solid_waste_fraction * nuP + q->zooplankton_mortality_rate * (Z * Z);
     out[4] = ponw + solid - G.onSmall - q->remineralisation_rate * D;
+    if (ox) {
+        /* Oxygen, Nutrient method (oxygen.jl:35-46). Julia ends the statement
+         * `respiration = plankton_inorganic_nitrogen_waste(...)` at the line
+         * break; the next line, `+ detritus_inorganic_nitrogen_waste(...)`,
+         * is a separate unary-plus expression whose value is discarded. So
+         * respiration is the plankton's inorganic nitrogen waste alone. */
+        double Rp = ox->respiration_oxygen_nitrogen_ratio;
+        double Rn = ox->nitrification_oxygen_nitrogen_ratio;
+        out[5] = Rp * muP - (Rp - Rn) * pinw;
+    }
 }
 
 /* LOBSTER: NitrateAmmonia + PhytoZoo + TwoParticleAndDissolved [+ CarbonateSystem]
@@ -200,26 +210,27 @@ static void lobster_tendencies(const obme_npd_params* q, int carbonate, const ob
 void orc_npd_tendencies(int model, int carbonate, const obme_npd_params* p, const double* vals,
                         double PAR, double* out)
 {
-    if (model == OBME_NPZD) npzd_tendencies(p, vals, PAR, out);
+    if (model == OBME_NPZD) npzd_tendencies(p, vals, PAR, out, NULL);
     else lobster_tendencies(p, carbonate, NULL, vals, PAR, out);
 }
 
-void orc_npd_tendencies_oxygen(int carbonate, const obme_npd_params* p, const obme_oxygen* ox,
+void orc_npd_tendencies_oxygen(int model, int carbonate, const obme_npd_params* p, const obme_oxygen* ox,
                                const double* vals, double PAR, double* out)
 {
-    lobster_tendencies(p, carbonate, ox, vals, PAR, out);
+    if (model == OBME_NPZD) npzd_tendencies(p, vals, PAR, out, ox);
+    else lobster_tendencies(p, carbonate, ox, vals, PAR, out);
 }
 
 /* tracer count and tendencies of a configuration (its optional components included) */
 static int cfg_ntracers(const obme_config* cfg)
 {
     int n = orc_npd_ntracers(cfg->model, cfg->carbonate_system);
-    return n + ((cfg->model == OBME_LOBSTER && cfg->oxygen.enabled) ? 1 : 0);
+    return n + (cfg->oxygen.enabled ? 1 : 0);
 }
 
 static void cfg_tendencies(const obme_config* cfg, const double* vals, double PAR, double* out)
 {
-    if (cfg->model == OBME_NPZD) npzd_tendencies(&cfg->npd, vals, PAR, out);
+    if (cfg->model == OBME_NPZD) npzd_tendencies(&cfg->npd, vals, PAR, out, cfg->oxygen.enabled ? &cfg->oxygen : NULL);
     else lobster_tendencies(&cfg->npd, cfg->carbonate_system, cfg->oxygen.enabled ? &cfg->oxygen : NULL,
                             vals, PAR, out);
 }

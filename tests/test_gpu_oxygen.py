@@ -1,4 +1,4 @@
-"""LOBSTER with the Oxygen component on the GPU against the oracle: the
+"""LOBSTER (and NPZD) with the Oxygen component on the GPU against the oracle: the
 fused step with and without the carbonate system, with the CO2 exchange,
 negative scaling and the InstantRemineralisation sediment compiled in beside
 it, at a ragged depth and at full 64-level columns (the FP32 default-
@@ -116,6 +116,27 @@ def test_add_tendencies_with_oxygen_gas_exchange():
     out = to_host(G)
     for n, name in enumerate(bgc.required_biogeochemical_tracers()):
         assert rel_l2(grid.interior(out[n]), grid.interior(G_ref[n])) <= 1e-12, name
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("Nz,scale,sediment", [(31, False, False), (64, False, False), (40, True, True)])
+def test_npzd_with_oxygen(precision, Nz, scale, sediment):
+    from oceanbiome import NPZD
+    print("case npzd oxygen", precision, Nz, scale, sediment, flush=True)
+    grid = RectilinearGrid(size=(11, 2, Nz), extent=(110.0, 20.0, 4.0 * Nz), halo=(3, 3, 3),
+                           topology=("Periodic", "Periodic", "Bounded"))
+    bgc = NPZD(grid, oxygen=Oxygen(), scale_negatives=scale,
+               light_attenuation=TwoBandPhotosyntheticallyActiveRadiation(surface_PAR=80.0),
+               sediment_model=InstantRemineralisationSediment(grid) if sediment else None)
+    rng = np.random.default_rng(19)
+    fields = []
+    for lo, hi in ((1.0, 10.0), (0.05, 0.5), (0.05, 0.3), (8.0, 20.0), (0.5, 3.0), (150.0, 300.0)):
+        f = grid.new_field()
+        grid.interior(f)[...] = rng.uniform(lo, hi, (Nz, 2, 11))
+        fields.append(f)
+    errs, out = _run(bgc, grid, fields, precision, 3, 600.0)
+    assert all(e <= TOL[precision] for e in errs.values()), errs
+    assert not np.array_equal(grid.interior(out[-1]), grid.interior(fields[-1]))
 
 
 def test_other_tracers_match_the_model_without_oxygen():

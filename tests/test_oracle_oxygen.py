@@ -8,7 +8,11 @@ and a stoichiometric identity that holds for any state and any ratios Rp,
 Rn: G_O2 + Rp G_NO3 + (Rp - Rn) G_NH4 = Rp mu0 L_PAR P L_NH4 / 2, the
 ammonium-limited share of growth, with L_PAR = PAR / (kPAR + PAR) and
 L_NH4 = NH4 / (kNH4 + NH4) formed here from the parameters (a dropped term,
-a wrong sign or swapped ratios fail it).  The O2 gas exchange on that
+a wrong sign or swapped ratios fail it).  NPZD's method (oxygen.jl:35-46,
+respiration = the plankton's inorganic nitrogen waste alone, as Julia
+parses it): G_O2 + (Rp - Rn) G_N = Rp muP + (Rp - Rn) (r D - (1 - ax g) muP),
+muP formed here from the parameters; a respiration that also held the
+detritus waste r D fails it.  The O2 gas exchange on that
 tracer: the C flux against the restatement the PISCES coupling uses, zero
 at saturation, and the top cell's O2 tendency as the only change, by
 -F / dz.  CPU only."""
@@ -59,8 +63,7 @@ def test_host_description():
     cfg = _lobster(g, True).config()
     assert cfg.oxygen.enabled == 1 and cfg.oxygen.respiration_oxygen_nitrogen_ratio == 10.75
     assert _lobster(g, True, oxygen=False).config().oxygen.enabled == 0
-    with pytest.raises(NotImplementedError):
-        NPZD(g, oxygen=Oxygen())
+    assert NPZD(g, oxygen=Oxygen()).required_biogeochemical_tracers() == ("N", "P", "Z", "T", "D", "O₂")
 
 
 @pytest.mark.parametrize("carbonate", [False, True])
@@ -204,3 +207,31 @@ def test_host_gas_exchange_pairs():
     with pytest.raises(NotImplementedError):
         LOBSTER(g, oxygen=Oxygen(),
                 gas_exchange=(OxygenGasExchangeBoundaryCondition(), OxygenGasExchangeBoundaryCondition()))
+
+
+def test_npzd_oxygen_identity_and_coupling():
+    rng = np.random.default_rng(23)
+    g = _box()
+    bgc = NPZD(g, oxygen=Oxygen())
+    q = bgc.npd_struct()
+    for it in range(200):
+        Rp, Rn = 20 * rng.random(), 5 * rng.random()
+        ox = abi.Oxygen(enabled=1, respiration_oxygen_nitrogen_ratio=Rp, nitrification_oxygen_nitrogen_ratio=Rn)
+        N, P, Z, T, D = 10 * rng.random(), rng.random(), rng.random(), 5 + 20 * rng.random(), rng.random()
+        PAR = 200 * rng.random()
+        v = np.array([N, P, Z, T, D, 300 * rng.random()])
+        G = O.npd_tendencies_oxygen(0, q, ox, v, PAR, model=abi.OBME_NPZD)
+        without = O.npd_tendencies(abi.OBME_NPZD, 0, q, v[:-1], PAR)
+        np.testing.assert_array_equal(G[:-1], without)
+        L_PAR = PAR / np.sqrt(PAR ** 2 + q.light_half_saturation ** 2)   # AnalyticalLightLimitation, NPZD's default
+        L_N = N / (N + q.nitrate_half_saturation)
+        Lt = q.temperature_coefficient ** (T / 10) if q.use_temperature_coefficient else 1.0
+        muP = q.phytoplankton_maximum_growth_rate * L_PAR * L_N * Lt * P
+        uptake = (1 - q.ammonia_fraction_of_exudate * q.phytoplankton_exudation_fraction) * muP
+        expect = Rp * muP + (Rp - Rn) * (q.remineralisation_rate * D - uptake)
+        lhs = G[-1] + (Rp - Rn) * G[0]
+        scale = abs(G[-1]) + abs(Rp - Rn) * (abs(G[0]) + q.remineralisation_rate * D + uptake) + Rp * muP
+        assert abs(lhs - expect) <= 1e-13 * scale, (it, lhs, expect)
+    tr = [g.new_field() for _ in range(6)]
+    O.grid_run(bgc.config(), tr, 10, 60.0, nthreads=1)
+    assert all(np.all(t == 0.0) for t in tr)
