@@ -443,7 +443,7 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
         return st;
     }
     // sweep tile configuration: the first of the precision's preference list
-    // whose column fits (Nz <= L * 64 / CPW); OBME_TILE="L,CPW,NW" overrides.
+    // whose column fits (Nz <= L * 64 / CPW); OBME_TILE="L,CPW,NW" overrides (not with O₂).
     h->cfg_id = choose_tile(h->prec, g.Nz);
     h->scaling = cfg->scaling.n_groups > 0;
     h->co2_inline = h->scaling && cfg->model == OBME_LOBSTER && cfg->carbonate_system && cfg->co2.enabled;
@@ -451,7 +451,8 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
     // the 512x512x64 LOBSTER step (0.56 vs 0.44 ms): off unless OBME_PIPE=1
     h->pipe = false;
     if (const char* e = std::getenv("OBME_PIPE")) h->pipe = std::atoi(e) != 0;
-    if (const char* e = std::getenv("OBME_TILE")) {
+    const char* tile_env = cfg->oxygen.enabled ? nullptr : std::getenv("OBME_TILE");
+    if (const char* e = tile_env) {
         int l = 0, c = 0, w = 0;
         if (std::sscanf(e, "%d,%d,%d", &l, &c, &w) == 3) {
             for (int t = 0; t < N_TILE_CFG; t++)
@@ -748,18 +749,33 @@ static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_
     return OBME_OK;
 }
 
+// The tile shapes choose_tile can pick for a precision (its preference
+// lists); the other entries of TILE_CFGS are reached only through OBME_TILE.
+template <typename FT, int L, int CPW, int NW>
+constexpr bool preferred_tile()
+{
+    constexpr bool shared = (L == 2 && CPW == 1 && NW == 8) || (L == 4 && CPW == 2 && NW == 8);
+    if constexpr (sizeof(FT) == 4) return shared || (L == 2 && CPW == 2 && NW == 8);
+    else return shared || (L == 1 && CPW == 1 && NW == 4);
+}
+
 template <typename FT, int L, int CPW, int NW, int MODE>
 static obme_status dispatch_model(obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
 {
-    if (h->cfg.model == OBME_NPZD) {
-        if (h->cfg.oxygen.enabled) return launch_wave<OBME_NPZD, 2, FT, L, CPW, NW, MODE>(h, a, s);
-        return launch_wave<OBME_NPZD, 0, FT, L, CPW, NW, MODE>(h, a, s);
+    if (h->cfg.oxygen.enabled) {
+        // the oxygen variants are compiled for the preferred tile shapes only
+        // (obme_create ignores OBME_TILE for them), which keeps the build time down
+        if constexpr (preferred_tile<FT, L, CPW, NW>()) {
+            if (h->cfg.model == OBME_NPZD) return launch_wave<OBME_NPZD, 2, FT, L, CPW, NW, MODE>(h, a, s);
+            if (h->cfg.carbonate_system) return launch_wave<OBME_LOBSTER, 3, FT, L, CPW, NW, MODE>(h, a, s);
+            return launch_wave<OBME_LOBSTER, 2, FT, L, CPW, NW, MODE>(h, a, s);
+        } else {
+            return fail(OBME_ERR_UNSUPPORTED, "tile configuration not compiled for the oxygen component");
+        }
     }
-    // LOBSTER's components: 1 CarbonateSystem, 2 Oxygen
-    const int comp = (h->cfg.carbonate_system ? 1 : 0) | (h->cfg.oxygen.enabled ? 2 : 0);
-    if (comp == 1) return launch_wave<OBME_LOBSTER, 1, FT, L, CPW, NW, MODE>(h, a, s);
-    if (comp == 2) return launch_wave<OBME_LOBSTER, 2, FT, L, CPW, NW, MODE>(h, a, s);
-    if (comp == 3) return launch_wave<OBME_LOBSTER, 3, FT, L, CPW, NW, MODE>(h, a, s);
+    if (h->cfg.model == OBME_NPZD) return launch_wave<OBME_NPZD, 0, FT, L, CPW, NW, MODE>(h, a, s);
+    // LOBSTER's components: 1 CarbonateSystem, 2 Oxygen (above)
+    if (h->cfg.carbonate_system) return launch_wave<OBME_LOBSTER, 1, FT, L, CPW, NW, MODE>(h, a, s);
     return launch_wave<OBME_LOBSTER, 0, FT, L, CPW, NW, MODE>(h, a, s);
 }
 
