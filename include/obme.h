@@ -298,8 +298,10 @@ obme_status obme_update_state(obme_handle* h, void* const* tracers,
  * ((bgc)(i,j,k,grid,Val(name),clock,fields), src/OceanBioME.jl:157-159),
  * minus the drift-velocity flux divergence (biogeochemical_drift_velocity,
  * src/OceanBioME.jl:126) and, with the carbonate system and gas exchange
- * enabled, minus F/dz of the top-cell CO2 flux (GasExchange/gas_exchange.jl:26-38).
- * T and S: optional 3-D fields (NULL = the configured source). */
+ * enabled, minus F/dz of the top-cell CO2 flux (GasExchange/gas_exchange.jl:26-38);
+ * with LOBSTER's O2 and its gas exchange, minus F/dz of the top-cell O2 flux
+ * (GasExchange.jl:127-146).  T and S: optional 3-D fields (NULL = the
+ * configured source). */
 obme_status obme_add_tendencies(obme_handle* h, const void* const* tracers,
                                 void* const* G, const void* PAR,
                                 const void* T, const void* S,
