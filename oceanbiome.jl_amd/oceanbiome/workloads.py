@@ -22,7 +22,8 @@ import numpy as np
 
 from . import abi
 from .grid import RectilinearGrid
-from .models import (LOBSTER, NPZD, CarbonateSystem, CarbonDioxideGasExchangeBoundaryCondition,
+from .models import (LOBSTER, NPZD, CarbonateSystem, CarbonDioxideGasExchangeBoundaryCondition, Oxygen,
+                     OxygenGasExchangeBoundaryCondition,
                      TwoBandPhotosyntheticallyActiveRadiation)
 from .units import day
 
@@ -80,15 +81,18 @@ def npzd_2d(nsteps=100, dt=50.0, surface_PAR=100.0):
 
 
 def lobster_3d(Nx=512, Ny=512, Nz=64, nsteps=200, dt=60.0, seed=42, surface_PAR=100.0, gas_exchange=True,
-               Lx=1000.0, Ly=1000.0, Lz=200.0):
-    """LOBSTER + CarbonateSystem on a periodic box with CO2 exchange."""
+               Lx=1000.0, Ly=1000.0, Lz=200.0, oxygen=False):
+    """LOBSTER + CarbonateSystem on a periodic box with CO2 exchange; with
+    oxygen, also the O2 tracer (last) and its gas exchange."""
     grid = RectilinearGrid(size=(Nx, Ny, Nz), extent=(Lx, Ly, Lz), halo=(3, 3, 3),
                            topology=("Periodic", "Periodic", "Bounded"))
     T_prof = 20 - 0.05 * np.abs(grid.zc)
-    bgc = LOBSTER(grid, carbonate_system=CarbonateSystem(),
+    ges = [CarbonDioxideGasExchangeBoundaryCondition()] if gas_exchange else []
+    if oxygen and gas_exchange:
+        ges.append(OxygenGasExchangeBoundaryCondition())
+    bgc = LOBSTER(grid, carbonate_system=CarbonateSystem(), oxygen=Oxygen() if oxygen else None,
                   light_attenuation=TwoBandPhotosyntheticallyActiveRadiation(surface_PAR=surface_PAR),
-                  gas_exchange=CarbonDioxideGasExchangeBoundaryCondition() if gas_exchange else None,
-                  temperature=T_prof, salinity=35.0)
+                  gas_exchange=tuple(ges), temperature=T_prof, salinity=35.0)
     rng = np.random.Generator(np.random.PCG64(seed))
     shape = (Nz, Ny, Nx)
     medians = (10.0, 0.1, 0.1, 0.1, 0.05, 0.05, 0.05)
@@ -97,11 +101,12 @@ def lobster_3d(Nx=512, Ny=512, Nz=64, nsteps=200, dt=60.0, seed=42, surface_PAR=
         f = grid.new_field()
         grid.interior(f)[...] = m * np.exp(0.5 * rng.standard_normal(shape))
         fields.append(f)
-    for lo, hi in ((2000.0, 2200.0), (2250.0, 2400.0)):
+    for lo, hi in ((2000.0, 2200.0), (2250.0, 2400.0)) + (((180.0, 320.0),) if oxygen else ()):
         f = grid.new_field()
         grid.interior(f)[...] = rng.uniform(lo, hi, shape)
         fields.append(f)
-    return Workload(f"lobster_dic_alk_{Nx}x{Ny}x{Nz}", bgc, grid, fields, dt, nsteps,
+    name = f"lobster_dic_alk{'_o2' if oxygen else ''}_{Nx}x{Ny}x{Nz}"
+    return Workload(name, bgc, grid, fields, dt, nsteps,
                     extra={"T_profile": T_prof, "S": 35.0})
 
 

@@ -3,7 +3,8 @@ properties that do not need the whole grid on the CPU: columns are
 independent, so a sample of columns stepped by the oracle is an exact check
 of those columns; the halos and the read-only fields stay untouched; every
 written value stays finite.  LOBSTER + DIC/Alk with the CO2 exchange on
-512x512x64 (the headline workload) in both precisions, and PISCES on
+512x512x64 (the headline workload) in both precisions, the same with the
+O2 tracer and its gas exchange in FP32-mixed, and PISCES on
 256x256x64 in FP32-mixed.  Bars as everywhere: FP64 <= 1e-12 relative L2
 per tracer, FP32-mixed <= 1e-5."""
 import numpy as np
@@ -29,11 +30,11 @@ def _columns(g, arr, ci, cj):
     return arr[ks[:, None], (cj + g.Hy)[None, :], (ci + g.Hx)[None, :]]
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp64"])
-def test_lobster_512x512x64_column_sample(precision):
+@pytest.mark.parametrize("precision,oxygen", [("fp32", False), ("fp64", False), ("fp32", True)])
+def test_lobster_512x512x64_column_sample(precision, oxygen):
     import torch
-    print("case full-size lobster", precision, flush=True)
-    wl = workloads.lobster_3d(512, 512, 64, nsteps=6)
+    print("case full-size lobster", precision, "oxygen", oxygen, flush=True)
+    wl = workloads.lobster_3d(512, 512, 64, nsteps=6, oxygen=oxygen)
     g = wl.grid
     ci, cj = _sample(g, 96, 5)
     ref = [a.copy() for a in wl.tracers]
