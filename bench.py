@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--workload", default="lobster", choices=["lobster", "columns", "carbonate", "pisces"],
                    help="lobster: the headline LOBSTER+DIC/Alk step; columns: the ragged NPZD column "
                         "ensemble; carbonate: pH and CO2 flux on point sets")
+    p.add_argument("--oxygen", action="store_true",
+                   help="lobster workload with the O2 tracer and its gas exchange (10 written tracers)")
     p.add_argument("--columns", type=int, default=8192)
     p.add_argument("--points", type=int, default=1 << 24)
     return p.parse_args()
@@ -127,7 +129,7 @@ def main():
     from oceanbiome.runtime import HIPBiogeochemistry
 
     log("building workload", args.nx, args.ny, args.nz)
-    wl = workloads.lobster_3d(args.nx, args.ny, args.nz, nsteps=args.check_steps)
+    wl = workloads.lobster_3d(args.nx, args.ny, args.nz, nsteps=args.check_steps, oxygen=args.oxygen)
     g = wl.grid
     ncell = g.n_interior
     nw = wl.n_written

@@ -62,6 +62,24 @@ __global__ void __launch_bounds__(NW * 64, (sizeof(FT) == 4 && L <= 2) ? OBME_F3
                                                                                    reinterpret_cast<FT*>(smem_raw));
 }
 
+// The O2 gas exchange's per-column coefficients (GasExchange.jl:127-146):
+// k / Δz_top and the air-side concentration C_air α at the column's surface
+// T and S (T and S are fixed during a call), one thread per column.
+template <typename FT>
+__global__ void __launch_bounds__(256) k_o2_prep(StepArgs<FT> a, obme_o2_exchange g, double invdz_top, double* out)
+{
+    const long col = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= (long)a.Nx * a.Ny) return;
+    const int i = (int)(col % a.Nx), j = (int)(col / a.Nx), ktop = a.Nz - 1;
+    const unsigned top = a.base0 + (unsigned)i + a.sx * (unsigned)j + a.sxy * (unsigned)ktop;
+    const double T = a.T3 ? (double)a.T3[top] : (a.Tprof ? a.Tprof[ktop] : a.Tconst);
+    const double S = a.S3 ? (double)a.S3[top] : (a.Sprof ? a.Sprof[ktop] : a.Sconst);
+    double k, air;
+    o2_exchange_coefficients(g, T, S, k, air);
+    out[2 * col] = k * invdz_top;
+    out[2 * col + 1] = air;
+}
+
 // ScaleNegativeTracers on every interior cell, in place (the modifier that
 // runs first in update_biogeochemical_state!, src/OceanBioME.jl:161-167;
 // src/Utils/negative_tracers.jl:198-251): one thread per cell.
@@ -178,6 +196,8 @@ struct obme_handle {
     double* d_Tprof = nullptr;
     double* d_Sprof = nullptr;
     void* d_srec = nullptr;    // SURF_REC * Nx*Ny of FT (CO2 exchange on)
+    double* d_o2c = nullptr;   // 2*Nx*Ny: k / Δz_top and C_air α of the O2 exchange
+    bool o2c_ready = false;    // d_o2c holds the current obme_step call's coefficients
     double* d_Hprev = nullptr; // Nx*Ny warm starts of [H+] (mixed path)
     double* d_surfc = nullptr; // [SURF_PACK][Nx*Ny] surface constants of the current stepper call
     bool surfc_ready = false;  // d_surfc holds the constants of this call's T and S
@@ -465,6 +485,12 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
         obme_destroy(h);
         return fail(OBME_ERR_UNSUPPORTED, "no tile configuration for this Nz");
     }
+    if (cfg->model == OBME_LOBSTER && cfg->oxygen.enabled && cfg->o2.enabled) {
+        if (hipMalloc((void**)&h->d_o2c, 2 * (size_t)g.Nx * g.Ny * sizeof(double)) != hipSuccess) {
+            obme_destroy(h);
+            return fail(OBME_ERR_HIP, "hipMalloc of the O2 exchange coefficients failed");
+        }
+    }
     if (cfg->model == OBME_LOBSTER && cfg->carbonate_system && cfg->co2.enabled) {
         size_t ncol = (size_t)g.Nx * g.Ny;
         size_t es = (h->prec == OBME_FP64) ? 8 : 4;
@@ -504,6 +530,7 @@ extern "C" obme_status obme_destroy(obme_handle* h)
     if (h->d_Sprof) (void)hipFree(h->d_Sprof);
     if (h->d_srec) (void)hipFree(h->d_srec);
     if (h->d_Hprev) (void)hipFree(h->d_Hprev);
+    if (h->d_o2c) (void)hipFree(h->d_o2c);
     if (h->d_sedG) (void)hipFree(h->d_sedG);
     if (h->d_surfc) (void)hipFree(h->d_surfc);
     for (auto& v : h->timed)
@@ -575,7 +602,7 @@ static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers)
     a.co2_on = (c.model == OBME_LOBSTER && c.carbonate_system && c.co2.enabled && !h->co2_inline) ? 1 : 0;
     a.ns = convert_scaling<FT, 16>(c.scaling);
     a.o2_on = (c.model == OBME_LOBSTER && c.oxygen.enabled && c.o2.enabled) ? 1 : 0;
-    if (a.o2_on) a.o2 = c.o2;
+    a.o2c = h->d_o2c;
     if (h->co2_inline || a.o2_on) {
         a.Tprof = (c.temperature.mode == OBME_FIELD_PROFILE) ? h->d_Tprof : nullptr;
         a.Sprof = (c.salinity.mode == OBME_FIELD_PROFILE) ? h->d_Sprof : nullptr;
@@ -835,6 +862,15 @@ static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* 
     }
     a.T3 = (const FT*)T;
     a.S3 = (const FT*)S;
+    if (MODE != TILE_PAR && a.o2_on && !h->o2c_ready) {
+        // the O2 exchange coefficients of this call's surface T and S (once
+        // per obme_step call, which sets o2c_ready around its steps)
+        const long ncol = (long)h->cfg.grid.Nx * h->cfg.grid.Ny;
+        hipLaunchKernelGGL(k_o2_prep<FT>, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, s, a, h->cfg.o2,
+                           1.0 / h->dz[h->cfg.grid.Nz - 1], h->d_o2c);
+        HIP_TRY(hipGetLastError());
+        if (MODE == TILE_STEP) h->o2c_ready = true;
+    }
     if (MODE == TILE_PAR && h->scaling) {
         // update_biogeochemical_state!: the scaling modifier first, in place
         const int bs = 256;
@@ -938,9 +974,11 @@ extern "C" obme_status obme_step_with_sediment(obme_handle* h, void* const* trac
         h->surfc_ready = true;
     }
     obme_status r = OBME_OK;
+    h->o2c_ready = false;
     for (int st = 0; st < nsteps && r == OBME_OK; st++)
         r = run_tile<TILE_STEP>(h, tracers, nullptr, nullptr, T, S, t0 + st * dt, dt, s, storage);
     h->surfc_ready = false;
+    h->o2c_ready = false;
     return r;
 }
 

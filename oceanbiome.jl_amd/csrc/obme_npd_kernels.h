@@ -136,10 +136,11 @@ struct StepArgs {
     FT invdz_top;
     obme_co2_exchange co2;
     obme_carbon_solver solver;
-    // LOBSTER's O2 gas exchange at the top cell (CARB & 2; T, S and
-    // invdz_top as above), worked by the top lane at every stage
+    // LOBSTER's O2 gas exchange at the top cell (CARB & 2): per column
+    // k / Δz_top and the air-side concentration C_air α, formed from the
+    // surface T and S once per call (k_o2_prep); applied by the top lane
     int o2_on;
-    obme_o2_exchange o2;
+    const double* o2c;             // [2 * (i + Nx j)]
     // InstantRemineralisation sediment (EXTRA & 4), worked by the lane that
     // holds a column's bottom cell
     int sed_recv;                  // receiver tracer
@@ -625,14 +626,8 @@ __device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
                     // the top-cell O2 flux (GasExchange for O2, gas_exchange.jl:26-38,
                     // GasExchange.jl:127-146) of this stage's O2: G -= F / Δz
                     if (a.o2_on && top_lane && m == 0) {
-                        const unsigned top = (unsigned)a.base0 + (unsigned)i + a.sx * (unsigned)j +
-                                             a.sxy * (unsigned)ktop;
-                        const double T = a.T3 ? (double)a.T3[top] : (a.Tprof ? a.Tprof[ktop] : a.Tconst);
-                        const double S = a.S3 ? (double)a.S3[top] : (a.Sprof ? a.Sprof[ktop] : a.Sconst);
-                        double kO2, air;
-                        o2_exchange_coefficients(a.o2, T, S, kO2, air);
-                        const double F = kO2 * ((double)val(NT - 1, 0) - air);
-                        Gv[NT - 1] = Gv[NT - 1] - (FT)F * a.invdz_top;
+                        const FT kdz = (FT)a.o2c[2 * col2d], air = (FT)a.o2c[2 * col2d + 1];
+                        Gv[NT - 1] = Gv[NT - 1] - kdz * (val(NT - 1, 0) - air);
                     }
                 }
                 if constexpr ((EXTRA & 4) != 0) {
