@@ -204,6 +204,8 @@ def test_host_gas_exchange_pairs():
     assert cfg.co2.enabled == 1 and cfg.o2.enabled == 1 and cfg.o2.air_concentration == 9352.7
     with pytest.raises(ValueError):
         LOBSTER(g, gas_exchange=OxygenGasExchangeBoundaryCondition())
+    with pytest.raises(ValueError):   # NPZD's T is a tracer: its O2 flux is not built
+        NPZD(g, oxygen=Oxygen(), gas_exchange=OxygenGasExchangeBoundaryCondition())
     with pytest.raises(NotImplementedError):
         LOBSTER(g, oxygen=Oxygen(),
                 gas_exchange=(OxygenGasExchangeBoundaryCondition(), OxygenGasExchangeBoundaryCondition()))
