@@ -214,7 +214,7 @@ def main():
     traffic = read_traffic(os.path.join(ROOT, "profiles"), wl.name, head_prec)
     dom = {"bound": "hbm", "achieved": head["sweep_achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": head["sweep_achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
-           "kernel": "k_column_wave (column sweep)", "algorithmic_bytes_per_launch": head["bytes_per_step"],
+           "kernel": "k_column_lane (column sweep)", "algorithmic_bytes_per_launch": head["bytes_per_step"],
            "launch_ms": head["sweep_ms_per_launch"]}
     line = {
         "metric": "tracer-cell updates/s + % HBM roofline; FP32-mixed vs FP64 at stated error",

@@ -44,22 +44,15 @@ extern "C" const char* obme_version(void) { return "obme 0.1.0 gfx950"; }
 
 // ------------------------------------------------------------------ kernels
 
-// The sweep: one workgroup of NW waves, wave-per-column segments
-// (obme_npd_kernels.h).  __launch_bounds__(NW*64) lets the compiler use
-// 512/(NW/4) VGPRs per lane at most (128 for 16 waves, 256 for 8).
-// FP32 tiles of up to 2 levels per lane are held to 512 / OBME_F32_WAVES_PER_EU
-// VGPRs (128: 4 waves per SIMD).
-#ifndef OBME_F32_WAVES_PER_EU
-#define OBME_F32_WAVES_PER_EU 4
-#endif
-template <int MODEL, int CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL, bool PIPE,
-          int FM = -1, int EXTRA = 0>
-__global__ void __launch_bounds__(NW * 64, (sizeof(FT) == 4 && L <= 2) ? OBME_F32_WAVES_PER_EU : 1)
-    k_column_wave(StepArgs<FT> a)
+// The sweep: lane per water column, workgroups of blockDim.x = 64 NW
+// consecutive columns of one row (obme_npd_kernels.h).  FP32 is held to 128
+// VGPRs (4 waves per SIMD, the occupancy at which the 512x512 grid's 4096
+// waves fill the chip once).
+constexpr int SWEEP_MAX_THREADS = 512;
+template <int MODEL, int CARB, typename FT, int MODE, bool UNIFORM, int FM = -1, int EXTRA = 0>
+__global__ void __launch_bounds__(SWEEP_MAX_THREADS, sizeof(FT) == 4 ? 4 : 1) k_column_lane(StepArgs<FT> a)
 {
-    extern __shared__ unsigned char smem_raw[];
-    column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, UNIFORM, FULL, PIPE, FM, EXTRA>(a,
-                                                                                   reinterpret_cast<FT*>(smem_raw));
+    column_lane<MODEL, CARB, FT, MODE, UNIFORM, FM, EXTRA>(a, (int)blockDim.x);
 }
 
 // The O2 gas exchange's per-column coefficients (GasExchange.jl:127-146):
@@ -202,9 +195,8 @@ struct obme_handle {
     double* d_surfc = nullptr; // [SURF_PACK][Nx*Ny] surface constants of the current stepper call
     bool surfc_ready = false;  // d_surfc holds the constants of this call's T and S
     int sink_idx[MAX_SINK];
-    int cfg_id;                // sweep tile configuration (TileCfg table)
+    int sweep_waves;           // waves per workgroup of the sweep (1, 2, 4 or 8)
     bool uniform;
-    bool pipe;                 // pipelined persistent sweep where it applies (OBME_PIPE=1 turns it on)
     bool scaling;              // ScaleNegativeTracers groups configured
     bool co2_inline;           // scaling with the CO2 flux: the flux is computed in the sweep
     bool sediment;             // InstantRemineralisation configured
@@ -366,27 +358,16 @@ static obme_status upload_geometry(obme_handle* h)
     return OBME_OK;
 }
 
-// Sweep tile configurations: L levels per lane, CPW columns per wave, NW
-// waves per workgroup (TXB = NW * CPW columns, Nz <= L * 64 / CPW).
-struct TileCfg {
-    int L, CPW, NW;
-};
-static const TileCfg TILE_CFGS[] = {
-    {2, 2, 8}, {1, 1, 16}, {2, 1, 8}, {2, 2, 16}, {1, 1, 8}, {4, 2, 8}, {2, 2, 4}, {1, 1, 4},
-};
-static const int N_TILE_CFG = sizeof(TILE_CFGS) / sizeof(TILE_CFGS[0]);
-
-static int choose_tile(int prec, int Nz)
+// Waves per workgroup of the sweep: a workgroup covers 64 NW consecutive
+// columns of a row.  4 measured fastest on the 512x512x64 LOBSTER step;
+// obme_autotune times every choice.
+static const int SWEEP_WAVES[] = {1, 2, 4, 8};
+static const int N_SWEEP_WAVES = sizeof(SWEEP_WAVES) / sizeof(SWEEP_WAVES[0]);
+static bool valid_sweep_waves(int nw)
 {
-    // preference lists by precision (indices into TILE_CFGS)
-    static const int pref32[] = {0, 2, 5};
-    static const int pref64[] = {7, 2, 5};
-    const int* p = (prec == OBME_FP64) ? pref64 : pref32;
-    for (int q = 0; q < 3; q++) {
-        const TileCfg& t = TILE_CFGS[p[q]];
-        if (Nz <= t.L * 64 / t.CPW) return p[q];
-    }
-    return -1;
+    for (int q = 0; q < N_SWEEP_WAVES; q++)
+        if (SWEEP_WAVES[q] == nw) return true;
+    return false;
 }
 
 extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
@@ -403,7 +384,6 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
         return fail(OBME_ERR_INVALID, "the O2 gas exchange needs LOBSTER with the oxygen component");
     if (g.Nx < 1 || g.Ny < 1 || g.Nz < 1 || g.Hx < 0 || g.Hy < 0 || g.Hz < 0)
         return fail(OBME_ERR_INVALID, "invalid grid size or halo");
-    if (g.Nz > 128) return fail(OBME_ERR_UNSUPPORTED, "Nz > 128 is not supported by the column-tile kernels");
     {
         size_t elems = (size_t)(g.Nx + 2 * g.Hx) * (g.Ny + 2 * g.Hy) * (g.Nz + 2 * g.Hz);
         size_t es = (cfg->precision == OBME_FP64) ? 8 : 4;
@@ -462,29 +442,13 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
         obme_destroy(h);
         return st;
     }
-    // sweep tile configuration: the first of the precision's preference list
-    // whose column fits (Nz <= L * 64 / CPW); OBME_TILE="L,CPW,NW" overrides (not with O₂).
-    h->cfg_id = choose_tile(h->prec, g.Nz);
+    h->sweep_waves = 4;
+    if (const char* e = std::getenv("OBME_SWEEP_WAVES")) {
+        const int nw = std::atoi(e);
+        if (valid_sweep_waves(nw)) h->sweep_waves = nw;
+    }
     h->scaling = cfg->scaling.n_groups > 0;
     h->co2_inline = h->scaling && cfg->model == OBME_LOBSTER && cfg->carbonate_system && cfg->co2.enabled;
-    // the pipelined sweep measured slower than one tile per workgroup on
-    // the 512x512x64 LOBSTER step (0.56 vs 0.44 ms): off unless OBME_PIPE=1
-    h->pipe = false;
-    if (const char* e = std::getenv("OBME_PIPE")) h->pipe = std::atoi(e) != 0;
-    const char* tile_env = cfg->oxygen.enabled ? nullptr : std::getenv("OBME_TILE");
-    if (const char* e = tile_env) {
-        int l = 0, c = 0, w = 0;
-        if (std::sscanf(e, "%d,%d,%d", &l, &c, &w) == 3) {
-            for (int t = 0; t < N_TILE_CFG; t++)
-                if (TILE_CFGS[t].L == l && TILE_CFGS[t].CPW == c && TILE_CFGS[t].NW == w &&
-                    g.Nz <= l * 64 / c)
-                    h->cfg_id = t;
-        }
-    }
-    if (h->cfg_id < 0) {
-        obme_destroy(h);
-        return fail(OBME_ERR_UNSUPPORTED, "no tile configuration for this Nz");
-    }
     if (cfg->model == OBME_LOBSTER && cfg->oxygen.enabled && cfg->o2.enabled) {
         if (hipMalloc((void**)&h->d_o2c, 2 * (size_t)g.Nx * g.Ny * sizeof(double)) != hipSuccess) {
             obme_destroy(h);
@@ -695,39 +659,26 @@ static SurfaceArgs<FT> make_surface_args(const obme_handle* h, void* const* trac
     return s;
 }
 
-template <int MODEL, int CARB, typename FT, int L, int CPW, int NW, int MODE>
-static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
+template <int MODEL, int CARB, typename FT, int MODE>
+static obme_status launch_lane(obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
 {
-    constexpr int NT = ModelTraits<MODEL, CARB>::NT;
-    constexpr int TXB = NW * CPW;
-    const int ntiles = (MODE == TILE_PAR) ? 1 : (MODE == TILE_TENDENCY ? NT + 1 : NT);
-    size_t lds = (size_t)ntiles * h->cfg.grid.Nz * (TXB + 1) * sizeof(FT);
-    if (lds > 160 * 1024) return fail(OBME_ERR_UNSUPPORTED, "sweep tile exceeds the 160 KiB LDS");
-    dim3 block(NW * 64);
+    const int cw = 64 * h->sweep_waves;
     // one-dimensional: tile = xcd_tile(blockIdx.x), row-major over (i tile, j)
-    const unsigned n_tiles = (unsigned)(((h->cfg.grid.Nx + TXB - 1) / TXB) * h->cfg.grid.Ny);
-    dim3 grid(n_tiles);
-    // FULL (every lane level is a cell) is compiled for the fused step only
-    const bool full = (MODE == TILE_STEP) && h->cfg.grid.Nz == L * 64 / CPW;
-    auto k = h->uniform ? (full ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, MODE == TILE_STEP, false>
-                                : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, false, false>)
-                        : (full ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, MODE == TILE_STEP, false>
-                                : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, false, false>);
+    const unsigned n_tiles = (unsigned)(((h->cfg.grid.Nx + cw - 1) / cw) * h->cfg.grid.Ny);
+    auto k = h->uniform ? k_column_lane<MODEL, CARB, FT, MODE, true> : k_column_lane<MODEL, CARB, FT, MODE, false>;
     if constexpr (sizeof(FT) == 4 && MODE == TILE_STEP) {
-        if (full && fm_code(a.f) == default_fm<MODEL>())
-            k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, true, false, default_fm<MODEL>()>
-                           : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, true, false, default_fm<MODEL>()>;
+        // the FP32 fused step compiled with the model's default formulations
+        if (fm_code(a.f) == default_fm<MODEL>())
+            k = h->uniform ? k_column_lane<MODEL, CARB, FT, MODE, true, default_fm<MODEL>()>
+                           : k_column_lane<MODEL, CARB, FT, MODE, false, default_fm<MODEL>()>;
     }
-    // negative scaling (1), the inline CO2 flux (2, with 1) and the sediment
-    // (4) in the fused step: the general (non-FULL) column code with the
-    // extra work compiled in
-    bool extra = false;
     if constexpr (MODE == TILE_STEP) {
+        // negative scaling (1), the inline CO2 flux (2, with 1) and the
+        // sediment (4) in the fused step, with the extra work compiled in
         const int ex = (h->scaling ? 1 : 0) | (h->co2_inline ? 2 : 0) | (h->sediment ? 4 : 0);
-        extra = ex != 0;
-#define OBME_EXTRA_KERNEL(E)                                                                              \
-    (h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, false, false, -1, E>            \
-                : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, false, false, -1, E>)
+#define OBME_EXTRA_KERNEL(E)                                                               \
+    (h->uniform ? k_column_lane<MODEL, CARB, FT, MODE, true, -1, E>                       \
+                : k_column_lane<MODEL, CARB, FT, MODE, false, -1, E>)
         if (ex == 1) k = OBME_EXTRA_KERNEL(1);
         if (ex == 4) k = OBME_EXTRA_KERNEL(4);
         if (ex == 5) k = OBME_EXTRA_KERNEL(5);
@@ -737,90 +688,25 @@ static obme_status launch_wave(obme_handle* h, const StepArgs<FT>& a, hipStream_
         }
 #undef OBME_EXTRA_KERNEL
     }
-    // the pipelined persistent sweep (FP32 fused step, FULL columns)
-    constexpr int PB = (64 % TXB == 0) ? 64 / TXB : 1;
-    constexpr int NBLK = (L * 64 / CPW) / PB;
-    constexpr bool PIPE_OK = CARB < 2 && sizeof(FT) == 4 && MODE == TILE_STEP && 64 % TXB == 0 && NBLK % NW == 0 &&
-                             2 * (NT * NBLK / NW) < 64;
-    if constexpr (PIPE_OK) {
-        if (full && h->pipe && !extra && a.surface_mode != OBME_SURFACE_ARRAY) {
-            k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, true, true>
-                           : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, true, true>;
-            if (fm_code(a.f) == default_fm<MODEL>())
-                k = h->uniform ? k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, true, true, true, default_fm<MODEL>()>
-                               : k_column_wave<MODEL, CARB, FT, L, CPW, NW, MODE, false, true, true, default_fm<MODEL>()>;
-            lds = 2 * (size_t)NT * NBLK * (64 + PIPE_SKEW) * sizeof(FT);
-            int per_cu = 0, ncu = 0, dev = 0;
-            HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            HIP_TRY(hipGetDevice(&dev));
-            HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-            HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k, NW * 64, lds));
-            // persistent: one workgroup per slot, a multiple of the 8 XCDs
-            unsigned slots = (unsigned)std::max(1, per_cu) * (unsigned)std::max(1, ncu);
-            unsigned nb = std::min(slots, n_tiles);
-            nb = std::max(8u, nb / 8 * 8);
-            grid = dim3(nb);
-            static bool told = false;
-            if (!told && std::getenv("OBME_VERBOSE")) {
-                std::fprintf(stderr, "obme: pipelined sweep: %d workgroups per CU x %d CUs, grid %u for %u tiles, LDS %zu B\n",
-                             per_cu, ncu, nb, n_tiles, lds);
-                told = true;
-            }
-        }
-    }
-    HIP_TRY(hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     LaunchTimer tm(h, OBME_KERNEL_SWEEP, s);
-    hipLaunchKernelGGL(k, grid, block, lds, s, a);
+    hipLaunchKernelGGL(k, dim3(n_tiles), dim3(cw), 0, s, a);
     tm.done();
     HIP_TRY(hipGetLastError());
     return OBME_OK;
 }
 
-// The tile shapes choose_tile can pick for a precision (its preference
-// lists); the other entries of TILE_CFGS are reached only through OBME_TILE.
-template <typename FT, int L, int CPW, int NW>
-constexpr bool preferred_tile()
-{
-    constexpr bool shared = (L == 2 && CPW == 1 && NW == 8) || (L == 4 && CPW == 2 && NW == 8);
-    if constexpr (sizeof(FT) == 4) return shared || (L == 2 && CPW == 2 && NW == 8);
-    else return shared || (L == 1 && CPW == 1 && NW == 4);
-}
-
-template <typename FT, int L, int CPW, int NW, int MODE>
+template <typename FT, int MODE>
 static obme_status dispatch_model(obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
 {
     if (h->cfg.oxygen.enabled) {
-        // the oxygen variants are compiled for the preferred tile shapes only
-        // (obme_create ignores OBME_TILE for them), which keeps the build time down
-        if constexpr (preferred_tile<FT, L, CPW, NW>()) {
-            if (h->cfg.model == OBME_NPZD) return launch_wave<OBME_NPZD, 2, FT, L, CPW, NW, MODE>(h, a, s);
-            if (h->cfg.carbonate_system) return launch_wave<OBME_LOBSTER, 3, FT, L, CPW, NW, MODE>(h, a, s);
-            return launch_wave<OBME_LOBSTER, 2, FT, L, CPW, NW, MODE>(h, a, s);
-        } else {
-            return fail(OBME_ERR_UNSUPPORTED, "tile configuration not compiled for the oxygen component");
-        }
+        if (h->cfg.model == OBME_NPZD) return launch_lane<OBME_NPZD, 2, FT, MODE>(h, a, s);
+        if (h->cfg.carbonate_system) return launch_lane<OBME_LOBSTER, 3, FT, MODE>(h, a, s);
+        return launch_lane<OBME_LOBSTER, 2, FT, MODE>(h, a, s);
     }
-    if (h->cfg.model == OBME_NPZD) return launch_wave<OBME_NPZD, 0, FT, L, CPW, NW, MODE>(h, a, s);
+    if (h->cfg.model == OBME_NPZD) return launch_lane<OBME_NPZD, 0, FT, MODE>(h, a, s);
     // LOBSTER's components: 1 CarbonateSystem, 2 Oxygen (above)
-    if (h->cfg.carbonate_system) return launch_wave<OBME_LOBSTER, 1, FT, L, CPW, NW, MODE>(h, a, s);
-    return launch_wave<OBME_LOBSTER, 0, FT, L, CPW, NW, MODE>(h, a, s);
-}
-
-template <typename FT, int MODE>
-static obme_status dispatch_tile_cfg(obme_handle* h, const StepArgs<FT>& a, hipStream_t s)
-{
-    switch (h->cfg_id) {
-    // the entries of TILE_CFGS, in order
-    case 0: return dispatch_model<FT, 2, 2, 8, MODE>(h, a, s);
-    case 1: return dispatch_model<FT, 1, 1, 16, MODE>(h, a, s);
-    case 2: return dispatch_model<FT, 2, 1, 8, MODE>(h, a, s);
-    case 3: return dispatch_model<FT, 2, 2, 16, MODE>(h, a, s);
-    case 4: return dispatch_model<FT, 1, 1, 8, MODE>(h, a, s);
-    case 5: return dispatch_model<FT, 4, 2, 8, MODE>(h, a, s);
-    case 6: return dispatch_model<FT, 2, 2, 4, MODE>(h, a, s);
-    case 7: return dispatch_model<FT, 1, 1, 4, MODE>(h, a, s);
-    default: return fail(OBME_ERR_INVALID, "bad tile configuration");
-    }
+    if (h->cfg.carbonate_system) return launch_lane<OBME_LOBSTER, 1, FT, MODE>(h, a, s);
+    return launch_lane<OBME_LOBSTER, 0, FT, MODE>(h, a, s);
 }
 
 template <typename FT, int MODE>
@@ -898,7 +784,7 @@ static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* 
         a.sed_ab2 = (!a.sed_init && a.sed_dt[0] == h->sed_last_dt) ? 1 : 0;
         sed_last = a.sed_dt[2];
     }
-    obme_status r = dispatch_tile_cfg<FT, MODE>(h, a, s);
+    obme_status r = dispatch_model<FT, MODE>(h, a, s);
     if (r == OBME_OK && MODE == TILE_STEP && h->sediment) {
         h->sed_iter += 3;
         h->sed_last_dt = sed_last;

@@ -1,24 +1,19 @@
 // obme_npd_kernels.h — the column-sweep and surface kernels of the NPZD /
 // LOBSTER biogeochemistry step on gfx950.
 //
-// Column sweep (column_wave): wave per column.  One water column lives
-// across the lanes of one wavefront segment of LPC = 64/CPW lanes; lane r
-// holds L consecutive levels counted from the surface,
-//   k = Nz-1 - (r*L + m),  m = 0..L-1  (0-based, k = 0 at the bottom),
-// so lane 0 holds the top cell.  A workgroup of NW waves owns TXB = NW*CPW
-// consecutive i-columns of one j row.  Tracers are staged through LDS once
-// per step: coalesced loads along i into padded [Nz][TXB+1] tiles, then
-// each lane gathers its levels (and the reverse for the stores), so HBM
-// sees one coalesced read and write of each written tracer per step.  All
-// state stays in registers through the three RK3 stages and the stages need
-// no workgroup barrier:
-//  - the TwoBand chlorophyll integral (Light/2band.jl:18-32) is a suffix sum
-//    from the surface: each lane sums its own increments top-down, a
-//    segmented prefix scan over the lanes (DPP row shifts and broadcasts)
-//    gives the sum above it, and the reference recurrence
-//    ∫(k) = ∫(k+1) + inc(k) continues from there;
-//  - upwind sinking needs c(k+1): the level above a lane's top level is the
-//    bottom level of the lane before it, one lane shift;
+// Column sweep (column_lane): lane per water column.  A workgroup of NW
+// waves owns 64 NW consecutive i-columns of one j row; each lane walks its
+// column from the surface down, one level at a time, and runs the three RK3
+// stages at each level before it moves on.  That order is legal because
+// stage s at level k reads only stage-s values of the levels above it:
+//  - the TwoBand chlorophyll integral (Light/2band.jl:18-32) is the running
+//    sum ∫(k) = ∫(k+1) + inc(k), carried down per stage (in the reference's
+//    own top-down order; Kahan-compensated in FP32);
+//  - upwind sinking needs c(k+1) of the same stage, one level of history.
+// Every load and store of a level is one contiguous run of 64 values per
+// wave, straight from and to HBM (no LDS staging, no cross-lane traffic,
+// no barriers); the next level's loads are issued before the current
+// level's arithmetic.  Each tracer is read and written once per step.
 //  - FP32-mixed: the step's increment D is accumulated apart from the state
 //    at the start of the step U0 (stage inputs U0 + D, one rounding at the
 //    store), so no stage rounds an increment against the magnitude of the
@@ -163,661 +158,396 @@ enum { LEV_ZC = 0, LEV_UP, LEV_DN, LEV_INVDZ, LEV_COUNT };
 // Modes of the column sweep.
 enum { TILE_STEP = 0, TILE_TENDENCY = 1, TILE_PAR = 2 };
 
-// s_waitcnt with only the vector-memory count constrained (gfx9 encoding:
-// vmcnt bits [3:0] and [15:14], expcnt [6:4] = 7, lgkmcnt [11:8] = 15).
-template <int N>
-__device__ __forceinline__ void wait_vmcnt()
-{
-    static_assert(N >= 0 && N < 64, "vmcnt is 6 bits");
-    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | (7 << 4) | (15 << 8));
-}
-// s_waitcnt lgkmcnt(0) (LDS and scalar memory), vector memory unconstrained.
-__device__ __forceinline__ void wait_lgkm0()
-{
-    __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4));
-}
-
-// Pipelined sweep (PIPE): a persistent workgroup walks a run of tiles and
-// loads tile n+1 straight into LDS (buffer_load ... lds, no registers)
-// while it computes tile n.  LDS layout of one buffer, per tracer: blocks of
-// 64/TXB levels x TXB columns, 64 elements each (one load instruction of one
-// wave: lane l holds level l / TXB, column l % TXB of the block, so the
-// global reads are coalesced along i), block kb at element kb * (64 + 2): the
-// skew of 2 per block puts the lanes of a column gather (levels two apart,
-// two columns) in distinct banks.
-constexpr int PIPE_SKEW = 2;
-
-// One wave-wide load of 4-byte elements straight into LDS: lane l's element
-// (byte offset voff in the buffer r; past its end reads 0) lands at
-// dst + 4 l.  dst must be the same for every lane of the wave.
-__device__ __forceinline__ void lds_load_b32(rsrc_t r, void* dst, unsigned voff)
-{
-    __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)dst, 4, voff, 0, 0, 0);
-}
-
+// The column sweep.  Lane = water column: a workgroup of NW waves owns
+// NW*64 consecutive i-columns of one j row, so every load and store of a
+// level is one contiguous run of 64 values per wave (coalesced straight
+// from HBM, no staging).  Each lane walks its column from the surface down,
+// one level at a time, and at each level runs the RK3 stages 1, 2, 3 in
+// turn: stage s at level k needs only stage-s values of the levels above
+// it (the chlorophyll integral, the upwind sinking inflow), which the lane
+// carries down as running sums and one level of history.  The loads of the
+// next level are issued before the current level's arithmetic.
+//
 // EXTRA (fused step only), a bit set: 1 negative scaling at the start of
-// every stage; 2 the CO2 flux computed here by the top lane (with 1);
-// 4 the InstantRemineralisation sediment at the bottom cell.
-template <int MODEL, int CARB, typename FT, int L, int CPW, int NW, int MODE, bool UNIFORM, bool FULL,
-          bool PIPE = false, int FM = -1, int EXTRA = 0>
-__device__ __forceinline__ void column_wave(const StepArgs<FT>& a, FT* lds)
+// every stage; 2 the CO2 flux computed here at the top level (with 1);
+// 4 the InstantRemineralisation sediment at the bottom level.  PARK: the
+// FP32 chlorophyll integrals are Kahan-compensated running sums.
+template <int MODEL, int CARB, typename FT, int MODE, bool UNIFORM, int FM = -1, int EXTRA = 0, bool PARK = true>
+__device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
 {
     using TR = ModelTraits<MODEL, CARB>;
     constexpr int NT = TR::NT;
     constexpr int RO = TR::RO_IDX;
-    constexpr int LPC = 64 / CPW;          // lanes per column
-    static_assert(LPC == 16 || LPC == 32 || LPC == 64, "segments of 16, 32 or 64 lanes");
-    constexpr int TXB = NW * CPW;          // columns per workgroup
-    constexpr int ROW = TXB + 1;           // padded LDS row: conflict-free gathers
-    const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    const int r = lane % LPC;              // rank in the column, 0 = top
-    const int c = w * CPW + lane / LPC;    // column in the workgroup
-    const unsigned nbx = ((unsigned)a.Nx + TXB - 1) / TXB;
-    const int Nz = FULL ? L * LPC : a.Nz;
-    const int ktop = Nz - 1;
-
-    // ---- PIPE geometry and the workgroup's run of tiles
-    constexpr int PB = (64 % TXB == 0) ? 64 / TXB : 1;    // levels per LDS block
-    constexpr int NBLK = (L * LPC) / PB;                   // blocks per tracer (FULL)
-    constexpr int BLK = 64 + PIPE_SKEW;
-    constexpr int TRS = NBLK * BLK;                        // elements per tracer
-    constexpr int LPW = (ModelTraits<MODEL, CARB>::NT * NBLK) / NW;   // block loads per wave per tile
-    static_assert(!PIPE || (FULL && MODE == TILE_STEP && sizeof(FT) == 4 && 64 % TXB == 0 && NBLK % NW == 0 &&
-                            LPW * 2 < 64),
-                  "pipelined sweep: FP32 fused step, FULL columns, TXB | 64, NW | blocks");
-    unsigned t_first, t_step, t_end;
-    if constexpr (PIPE) {
-        // XCD x (workgroups b = x mod 8) takes a contiguous chunk of tiles;
-        // its workgroups walk it with stride = their number, so the tiles in
-        // flight on one XCD at a time are neighbours.
-        const unsigned ntiles = nbx * (unsigned)a.Ny;
-        const unsigned b = blockIdx.x, nb = gridDim.x, X = 8;
-        const unsigned x = b % X, s = b / X;
-        const unsigned nwx = nb / X + (x < nb % X ? 1u : 0u);
-        const unsigned tq = ntiles / X, trem = ntiles % X;
-        const unsigned start = x * tq + (x < trem ? x : trem);
-        t_first = start + s;
-        t_step = nwx;
-        t_end = start + tq + (x < trem ? 1u : 0u);
-    } else {
-        t_first = xcd_tile(blockIdx.x, gridDim.x);
-        t_step = 1;
-        t_end = t_first + 1;
-    }
-    auto pipe_buf = [&](int n) -> FT* { return lds + (n & 1) * (ModelTraits<MODEL, CARB>::NT * TRS); };
-    // byte offset of (ii, jj, k) in a parent array
-    auto goff2 = [&](int ii, int jj, int k) -> unsigned {
-        return (a.base0 + (unsigned)ii + a.sx * (unsigned)jj + a.sxy * (unsigned)k) * (unsigned)sizeof(FT);
-    };
-    // issue the LDS loads of tile t into buffer B (LPW instructions per wave)
-    auto pipe_issue = [&](unsigned t, FT* B) {
-        const int jj = (int)(t / nbx);
-        const int ib = (int)(t % nbx) * TXB;
-        const int kl = lane / TXB, cc = lane % TXB;
-        const int ii = (ib + cc < a.Nx) ? ib + cc : a.Nx - 1;
-#pragma unroll
-        for (int n = 0; n < ModelTraits<MODEL, CARB>::NT; n++) {
-            const rsrc_t rn = make_rsrc(a.tr[n], a.field_bytes);
-#pragma unroll
-            for (int q = 0; q < NBLK / NW; q++) {
-                const int kb = w + NW * q;
-                lds_load_b32(rn, B + n * TRS + kb * BLK, goff2(ii, jj, kb * PB + kl));
-            }
-        }
-    };
-    bool prev_tile = false;                // a tile was stored before this one
-    bool next_issued = false;              // the tile after this one has been issued
-    if constexpr (PIPE) {
-        if (t_first < t_end) pipe_issue(t_first, pipe_buf(0));
-        if (t_first + t_step < t_end) {
-            pipe_issue(t_first + t_step, pipe_buf(1));
-            next_issued = true;
-        }
-    }
-    const FT surface_value0 = a.surface_value;
-    int it = 0;
-#pragma unroll 1
-    for (unsigned wg_tile = t_first; wg_tile < t_end; wg_tile += t_step, it++) {
-    const int j = (int)(wg_tile / nbx);
-    const int ibase = (int)(wg_tile % nbx) * TXB;
-    const int i_raw = ibase + c;
-    const bool col_ok = i_raw < a.Nx;
-    const int i = col_ok ? i_raw : a.Nx - 1;
-
-    auto kof = [&](int m) -> int { return ktop - (r * L + m); };
-    // FULL: Nz == L * LPC, every lane level is a cell of the column
-    auto has = [&](int m) -> bool { return FULL || kof(m) >= 0; };
-    auto rs = [&](const FT* p) { return make_rsrc(p, a.field_bytes); };
-    // byte offset of (ii, j, k) in a parent array
-    auto goff = [&](int ii, int k) -> unsigned {
-        return (a.base0 + (unsigned)ii + a.sx * (unsigned)j + a.sxy * (unsigned)k) * (unsigned)sizeof(FT);
-    };
-
-    // ---- stage in: coalesced loads along i into [Nz][ROW] tiles, gather.
-    // Element e of a tile is level e / TXB, column e % TXB; thread t handles
-    // e = t + q * NW*64 for q < L (Nz * TXB <= L * NW * 64).  All loads of
-    // all tracers are issued before the first LDS write.
-    constexpr int NLOAD = (MODE == TILE_PAR) ? 1 : NT;
-    constexpr int BS = NW * 64;
-    auto tile = [&](int t) -> FT* { return lds + t * (Nz * ROW); };
-    const int nel = Nz * TXB;
-    // byte offset of staging element q (beyond the buffer when outside the
-    // tile: reads 0) and its LDS index (-1 outside the tile)
-    auto e_off = [&](int q) -> unsigned {
-        const int e = threadIdx.x + q * BS;
-        const int k = e / TXB, cc = e % TXB;
-        const int ii = (ibase + cc < a.Nx) ? ibase + cc : a.Nx - 1;
-        return (e < nel) ? goff(ii, k) : a.field_bytes;
-    };
-    auto e_idx = [&](int q) -> int {
-        const int e = threadIdx.x + q * BS;
-        return (e < nel) ? (e / TXB) * ROW + e % TXB : -1;
-    };
-    // PIPE: element (tracer n, level k, column cc) of the current buffer
-    FT* const pbuf = pipe_buf(it);
-    auto pidx = [&](int n, int k, int cc) -> int { return n * TRS + (k / PB) * BLK + (k % PB) * TXB + cc; };
-    if constexpr (PIPE) {
-        // this tile's loads are complete once at most the later ones are
-        // outstanding: the previous tile's LPW stores and, if issued, the
-        // next tile's LPW loads (each wave issues exactly these counts)
-        const int later = (prev_tile ? 1 : 0) + (next_issued ? 1 : 0);
-        if (later == 2) wait_vmcnt<2 * LPW>();
-        else if (later == 1) wait_vmcnt<LPW>();
-        else wait_vmcnt<0>();
-        __builtin_amdgcn_s_barrier();
-    } else {
-        FT stg[NLOAD][L];
-#pragma unroll
-        for (int t = 0; t < NLOAD; t++) {
-            const int n = (MODE == TILE_PAR) ? TR::P_IDX : t;
-            const rsrc_t rn = rs(a.tr[n]);
-#pragma unroll
-            for (int q = 0; q < L; q++) buf_load(rn, e_off(q), stg[t][q]);
-        }
-#pragma unroll
-        for (int t = 0; t < NLOAD; t++)
-#pragma unroll
-            for (int q = 0; q < L; q++)
-                if (e_idx(q) >= 0) tile(t)[e_idx(q)] = stg[t][q];
-        __syncthreads();
-    }
-
-    // FP64: U is the state, updated in place (the reference's expressions).
-    // FP32: U is the state at the start of the step and D the step's
-    // accumulated increment; stage inputs are U + D and the sum is rounded
-    // once at the store (a compensated update: the increments are never
-    // rounded against the magnitude of U).
+    constexpr int S0 = TR::SINK0, S1 = TR::SINK1, PI = TR::P_IDX;
     constexpr bool F32 = sizeof(FT) == 4;
-    FT U[NT][L];
-    FT Gm[NT][L];
-    FT D[F32 ? NT : 1][L];
-#pragma unroll
-    for (int n = 0; n < NT; n++)
-#pragma unroll
-        for (int m = 0; m < L; m++) {
-            const int k = kof(m);
-            const bool ok = has(m);
-            int t = (MODE == TILE_PAR) ? (n == TR::P_IDX ? 0 : -1) : n;
-            if constexpr (PIPE) U[n][m] = pbuf[pidx(n, k, c)];
-            else U[n][m] = (ok && t >= 0) ? tile(t)[k * ROW + c] : FT(0);
-            Gm[n][m] = FT(0);
-            if constexpr (F32) D[n][m] = FT(0);
-        }
-    // stage value of tracer n at level m
-    auto val = [&](int n, int m) -> FT {
-        if constexpr (F32) return (MODE == TILE_STEP) ? U[n][m] + D[n < NT ? n : 0][m] : U[n][m];
-        else return U[n][m];
-    };
+    constexpr int NSTAGE = (MODE == TILE_STEP) ? 3 : 1;
+    static_assert(EXTRA == 0 || MODE == TILE_STEP, "scaling, inline CO2, sediment: fused step");
+    static_assert(!(EXTRA & 2) || (CARB & 1), "inline CO2 needs the carbonate system");
 
-    // geometry of level k: computed (uniform grid) or read from the per-level rows
-    auto g_zc = [&](int k) -> FT {
-        if constexpr (UNIFORM) return a.zc0_u + FT(k) * a.dz_u;
-        else return a.lev[LEV_ZC * Nz + k];
-    };
-    auto g_up = [&](int k) -> FT {
-        if constexpr (UNIFORM) return a.dz_u * FT(0.5);
-        else return a.lev[LEV_UP * Nz + k];
-    };
-    auto g_dn = [&](int k) -> FT {
-        if constexpr (UNIFORM) return a.dz_u * FT(0.5);
-        else return a.lev[LEV_DN * Nz + k];
-    };
-    auto g_invdz = [&](int k) -> FT {
-        if constexpr (UNIFORM) return a.invdz_u;
-        else return a.lev[LEV_INVDZ * Nz + k];
-    };
-    // face velocities below (face k) and above (face k+1) cell k, sinking tracer s
-    auto g_wlo = [&](int s, int k) -> FT {
-        if constexpr (UNIFORM) return s == 0 ? a.w0_u : a.w1_u;
-        else return a.wsink[s * (Nz + 1) + k];
-    };
-    auto g_whi = [&](int s, int k) -> FT {
-        if constexpr (UNIFORM) return s == 0 ? a.w0_u : a.w1_u;
-        else return a.wsink[s * (Nz + 1) + k + 1];
-    };
-
-    // (the constant is read before the tile loop.  The pipelined sweep is
-    // not used with a surface PAR array: the wait for a vector load here
-    // would also wait for the prefetch of the next tile.)
-    FT PAR0_const = surface_value0;
-    if constexpr (!PIPE) {
-        if (a.surface_mode == OBME_SURFACE_ARRAY) PAR0_const = a.surf[i + a.Nx * j];
-    }
-
+    const unsigned nbx = ((unsigned)a.Nx + (unsigned)cw - 1) / (unsigned)cw;
+    const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int j = (int)(tile / nbx);
+    const int i_raw = (int)(tile % nbx) * cw + (int)threadIdx.x;
+    const bool col_ok = i_raw < a.Nx;
+    // a lane past the last column computes on the last column and stores nothing
+    const int i = col_ok ? i_raw : a.Nx - 1;
+    const int Nz = a.Nz;
+    const int ktop = Nz - 1;
+    constexpr unsigned ES = sizeof(FT);
+    const unsigned dk = a.sxy * ES;
+    const unsigned off_top = (a.base0 + (unsigned)i + a.sx * (unsigned)j + a.sxy * (unsigned)ktop) * ES;
     const unsigned col2d = (unsigned)i + (unsigned)a.Nx * (unsigned)j;
     const unsigned ncol2d = (unsigned)a.Nx * (unsigned)a.Ny;
-    const bool top_lane = col_ok && r == 0;
-    if constexpr ((CARB & 1) && MODE == TILE_STEP) {
-        if (top_lane && a.co2_on) a.srec[col2d] = U[7][0];
-    }
+    auto rs = [&](const FT* p) { return make_rsrc(p, a.field_bytes); };
+    auto lev_off = [&](int k) -> unsigned { return off_top - (unsigned)(ktop - k) * dk; };
 
-    // FP32 on a uniform grid: the water terms kr zc, kb zc of the own levels (log2 scaled)
-    float ezr[L], ezb[L];
-    if constexpr (F32 && UNIFORM) {
+    // loads of one level: the PAR mode reads P alone
+    auto load_level = [&](FT (&dst)[NT], int k) {
+        const unsigned off = lev_off(k);
 #pragma unroll
-        for (int m = 0; m < L; m++) {
-            const float z = g_zc(has(m) ? kof(m) : 0);
-            ezr[m] = a.kr2 * z;
-            ezb[m] = a.kb2 * z;
+        for (int n = 0; n < NT; n++) {
+            if (MODE == TILE_PAR && n != PI) dst[n] = FT(0);
+            else buf_load(rs(a.tr[n]), off, dst[n]);
         }
+    };
+
+    FT PAR0_const = a.surface_value;
+    if (a.surface_mode == OBME_SURFACE_ARRAY) PAR0_const = a.surf[col2d];
+
+    // carried down the column, per stage: the stage values of the sinking
+    // tracers and the chlorophyll terms (P R/r)^e of the level above, and
+    // the running chlorophyll integrals (with their compensation)
+    FT Xup[NSTAGE][2], pup[NSTAGE][2], Ir[NSTAGE], Ib[NSTAGE], Cr[NSTAGE], Cb[NSTAGE];
+#pragma unroll
+    for (int s = 0; s < NSTAGE; s++) {
+        Xup[s][0] = Xup[s][1] = FT(0);
+        pup[s][0] = pup[s][1] = FT(0);
+        Ir[s] = Ib[s] = Cr[s] = Cb[s] = FT(0);
     }
 
-    constexpr int NSTAGE = (MODE == TILE_STEP) ? 3 : 1;
-    const double g1 = 8.0 / 15.0, g2 = 5.0 / 12.0, g3 = 3.0 / 4.0, z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
-    FT PARv[L];   // PAR of the own levels (output of the PAR and tendency modes)
-    static_assert(EXTRA == 0 || (MODE == TILE_STEP && !PIPE), "scaling, inline CO2, sediment: fused step, direct sweep");
-    static_assert(!(EXTRA & 2) || (CARB & 1), "inline CO2 needs the carbonate system");
     // inline CO2 (EXTRA & 2): [H+] of the previous stage (FP32-mixed warm start)
     double Hc = 0.0;
     if constexpr ((EXTRA & 2) && F32) {
-        if (top_lane && a.Hprev) Hc = a.Hprev[col2d];
+        if (col_ok && a.Hprev) Hc = a.Hprev[col2d];
     }
-    // sediment (EXTRA & 4): the bottom lane's storage value, the tendency e f
-    // of the last stage start and the G⁻ of the sediment's previous step
-    const bool sed_lane = col_ok && r * L <= ktop && ktop < r * L + L;
+    // sediment (EXTRA & 4): the storage value, the tendency e f of the last
+    // stage start and the G⁻ of the sediment's previous step
     double sed_st = 0.0, sed_ge = 0.0, sed_gm = 0.0;
     if constexpr ((EXTRA & 4) != 0) {
-        if (sed_lane) {
+        if (col_ok) {
             if (a.storage) sed_st = a.storage[col2d];
             if (a.sed_ab2) sed_gm = a.sedG[ncol2d + col2d];
         }
     }
 
-    auto run_stage = [&](const int st) {
-        FT PAR0 = PAR0_const;
-        if (a.surface_mode == OBME_SURFACE_DIURNAL) PAR0 = a.PAR0_stage[st];
+    const double g1 = 8.0 / 15.0, g2 = 5.0 / 12.0, g3 = 3.0 / 4.0, z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
 
-        if constexpr ((EXTRA & 1) != 0) {
-            // ScaleNegativeTracers first (update_biogeochemical_state! order,
-            // src/OceanBioME.jl:161-167).  FP64 applies it to every cell as
-            // the reference does; FP32-mixed where a member is negative or
-            // not finite (elsewhere it is the identity up to rounding), and
-            // the scaled value becomes the new U with no pending increment.
-            unsigned members = 0;
+    FT nxt[NT];
+    load_level(nxt, ktop);
 #pragma unroll 1
-            for (int g = 0; g < a.ns.n_groups; g++) members |= a.ns.mask[g];
+    for (int k = ktop; k >= 0; k--) {
+        // FP64: U is the state, updated in place (the reference's
+        // expressions).  FP32: U is the state at the start of the step and D
+        // the step's accumulated increment; stage inputs are U + D and the
+        // sum is rounded once at the store (the increments are never rounded
+        // against the magnitude of U).
+        FT U[NT], D[F32 ? NT : 1], Gm[NT];
 #pragma unroll
-            for (int m = 0; m < L; m++) {
+        for (int n = 0; n < NT; n++) {
+            U[n] = nxt[n];
+            Gm[n] = FT(0);
+            if constexpr (F32) D[n] = FT(0);
+        }
+        if (k > 0) load_level(nxt, k - 1);
+        const bool top = k == ktop;
+
+        // geometry of level k (uniform across the wave)
+        FT zc, up, dn, invdz, wlo0, whi0, wlo1, whi1;
+        if constexpr (UNIFORM) {
+            zc = a.zc0_u + FT(k) * a.dz_u;
+            up = dn = a.dz_u * FT(0.5);
+            invdz = a.invdz_u;
+            wlo0 = whi0 = a.w0_u;
+            wlo1 = whi1 = a.w1_u;
+        } else {
+            zc = a.lev[LEV_ZC * Nz + k];
+            up = a.lev[LEV_UP * Nz + k];
+            dn = a.lev[LEV_DN * Nz + k];
+            invdz = a.lev[LEV_INVDZ * Nz + k];
+            wlo0 = a.wsink[k];
+            whi0 = a.wsink[k + 1];
+            wlo1 = a.wsink[(Nz + 1) + k];
+            whi1 = a.wsink[(Nz + 1) + k + 1];
+        }
+        float ezr = 0.0f, ezb = 0.0f;   // FP32: kr zc, kb zc (log2 scaled)
+        if constexpr (F32) {
+            ezr = a.kr2 * (float)zc;
+            ezb = a.kb2 * (float)zc;
+        }
+        if constexpr ((CARB & 1) && MODE == TILE_STEP) {
+            if (top && col_ok && a.co2_on) a.srec[col2d] = U[7];
+        }
+
+        FT PARv = FT(0);
+        FT Gout[NT];   // TILE_TENDENCY: the transitions
+#pragma unroll
+        for (int st = 0; st < NSTAGE; st++) {
+            FT PAR0 = PAR0_const;
+            if (a.surface_mode == OBME_SURFACE_DIURNAL) PAR0 = a.PAR0_stage[st];
+
+            if constexpr ((EXTRA & 1) != 0) {
+                // ScaleNegativeTracers first (update_biogeochemical_state!
+                // order, src/OceanBioME.jl:161-167).  FP64 applies it to every
+                // cell as the reference does; FP32-mixed where a member is
+                // negative or not finite (elsewhere it is the identity up to
+                // rounding), and the scaled value becomes the new U with no
+                // pending increment.
+                unsigned members = 0;
+#pragma unroll 1
+                for (int g = 0; g < a.ns.n_groups; g++) members |= a.ns.mask[g];
                 FT x[NT];
 #pragma unroll
-                for (int n = 0; n < NT; n++) x[n] = val(n, m);
+                for (int n = 0; n < NT; n++) x[n] = (F32 && st > 0) ? U[n] + D[F32 ? n : 0] : U[n];
                 if constexpr (F32) {
                     bool need = false;
 #pragma unroll
                     for (int n = 0; n < NT; n++)
                         if (members & (1u << n)) need = need || !(x[n] >= 0.0f) || isinf(x[n]);
-                    if (need && has(m)) {
+                    if (need) {
                         scale_negatives_cell<NT>(a.ns, x);
 #pragma unroll
                         for (int n = 0; n < NT; n++)
                             if (members & (1u << n)) {
-                                U[n][m] = x[n];
-                                D[n][m] = 0.0f;
+                                U[n] = x[n];
+                                D[F32 ? n : 0] = FT(0);
                             }
                     }
                 } else {
                     scale_negatives_cell<NT>(a.ns, x);
 #pragma unroll
                     for (int n = 0; n < NT; n++)
-                        if (members & (1u << n)) U[n][m] = x[n];
+                        if (members & (1u << n)) U[n] = x[n];
                 }
             }
-        }
 
-        // chlorophyll terms (P R/r)^e of the own levels
-        FT pr[L], pb[L];
+            // stage values of this level
+            FT X[NT];
 #pragma unroll
-        for (int m = 0; m < L; m++) {
+            for (int n = 0; n < NT; n++) X[n] = (F32 && st > 0) ? U[n] + D[F32 ? n : 0] : U[n];
+
+            // ---- TwoBand PAR (Light/2band.jl:18-32): the chlorophyll
+            // integral from the surface, ∫(k) = ∫(k+1) + up(k) p(k+1) + dn(k) p(k)
+            // (the top level: dn p alone), PAR = PAR0 (e^(kr zc - χr ∫r) + e^(kb zc - χb ∫b)) / 2.
+            FT pr, pb;
             if constexpr (F32) {
-                const FT lg = m_log2(val(TR::P_IDX, m));
-                pr[m] = m_exp2(fmaf(a.er, lg, a.cr));
-                pb[m] = m_exp2(fmaf(a.eb, lg, a.cb));
+                const float lg = m_log2(X[PI]);
+                pr = m_exp2(fmaf(a.er, lg, a.cr));
+                pb = m_exp2(fmaf(a.eb, lg, a.cb));
             } else {
-                const FT x = val(TR::P_IDX, m) * a.Rr;
-                pr[m] = m_pow(x, a.er);
-                pb[m] = m_pow(x, a.eb);
+                const FT x = X[PI] * a.Rr;
+                pr = m_pow(x, a.er);
+                pb = m_pow(x, a.eb);
             }
-        }
-        // the level above a lane's top level is the previous lane's bottom level
-        const FT pr_up = lane_shr1(pr[L - 1]);
-        const FT pb_up = lane_shr1(pb[L - 1]);
-        const FT c0_up = lane_shr1(val(TR::SINK0, L - 1));
-        const FT c1_up = lane_shr1(val(TR::SINK1, L - 1));
-
-        // increments top-down: inc(k) = up(k) p(k+1) + dn(k) p(k); the top
-        // level has none above: dn p alone.  FP32 on a uniform grid sums
-        // p(k+1) + p(k) and folds dz/2 into the attenuation.
-        FT incr[L], incb[L];
-        FT totr = FT(0), totb = FT(0);
-#pragma unroll
-        for (int m = 0; m < L; m++) {
-            const int k = kof(m);
-            const FT prn = (m == 0) ? pr_up : pr[m > 0 ? m - 1 : 0];
-            const FT pbn = (m == 0) ? pb_up : pb[m > 0 ? m - 1 : 0];
-            const bool below_top = (m > 0) || (k < ktop);
             FT ir, ib;
             if constexpr (F32 && UNIFORM) {
-                ir = below_top ? prn + pr[m] : pr[m];
-                ib = below_top ? pbn + pb[m] : pb[m];
+                // dz/2 is folded into the attenuations (xr2h, xb2h)
+                ir = top ? pr : pup[st][0] + pr;
+                ib = top ? pb : pup[st][1] + pb;
             } else {
-                const int kc = has(m) ? k : 0;
-                const FT upk = g_up(kc), dnk = g_dn(kc);
-                ir = below_top ? upk * prn + dnk * pr[m] : dnk * pr[m];
-                ib = below_top ? upk * pbn + dnk * pb[m] : dnk * pb[m];
+                ir = top ? dn * pr : up * pup[st][0] + dn * pr;
+                ib = top ? dn * pb : up * pup[st][1] + dn * pb;
             }
-            incr[m] = has(m) ? ir : FT(0);
-            incb[m] = has(m) ? ib : FT(0);
-            totr = totr + incr[m];
-            totb = totb + incb[m];
-        }
-        // sum of the increments above the lane: exclusive segmented prefix
-        // (a lane tree, error growing as log2 of the depth)
-        FT runr = lane_shr1(seg_prefix<LPC>(totr));
-        FT runb = lane_shr1(seg_prefix<LPC>(totb));
-        if (r == 0) {
-            runr = FT(0);
-            runb = FT(0);
-        }
-        // PAR of the own levels
-        const FT PAR0h = PAR0 * FT(0.5);
-#pragma unroll
-        for (int m = 0; m < L; m++) {
-            runr = runr + incr[m];
-            runb = runb + incb[m];
+            if constexpr (F32 && PARK) {
+                // Kahan summation: every increment is >= 0, so the running
+                // sum dominates and its lost low part is recovered exactly
+                const FT yr = ir - Cr[st], yb = ib - Cb[st];
+                const FT tr = Ir[st] + yr, tb = Ib[st] + yb;
+                Cr[st] = (tr - Ir[st]) - yr;
+                Cb[st] = (tb - Ib[st]) - yb;
+                Ir[st] = tr;
+                Ib[st] = tb;
+            } else {
+                Ir[st] = Ir[st] + ir;
+                Ib[st] = Ib[st] + ib;
+            }
+            pup[st][0] = pr;
+            pup[st][1] = pb;
             if constexpr (F32 && UNIFORM) {
-                PARv[m] = PAR0h * (m_exp2(fmaf(-a.xr2h, runr, ezr[m])) + m_exp2(fmaf(-a.xb2h, runb, ezb[m])));
+                PARv = (PAR0 * 0.5f) * (m_exp2(fmaf(-a.xr2h, Ir[st], ezr)) + m_exp2(fmaf(-a.xb2h, Ib[st], ezb)));
+            } else if constexpr (F32) {
+                PARv = (PAR0 * 0.5f) * (m_exp2(fmaf(-a.xr2, Ir[st], ezr)) + m_exp2(fmaf(-a.xb2, Ib[st], ezb)));
             } else {
-                const int k = kof(m);
-                const FT z = g_zc(has(m) ? k : 0);
-                if constexpr (F32)
-                    PARv[m] = PAR0h * (m_exp2(fmaf(a.kr2, z, -a.xr2 * runr)) + m_exp2(fmaf(a.kb2, z, -a.xb2 * runb)));
-                else
-                    PARv[m] = PAR0 * (m_exp(a.kr * z - a.xr * runr) + m_exp(a.kb * z - a.xb * runb)) / FT(2);
+                PARv = PAR0 * (m_exp(a.kr * zc - a.xr * Ir[st]) + m_exp(a.kb * zc - a.xb * Ib[st])) / FT(2);
             }
-        }
+            if constexpr (MODE == TILE_PAR) continue;
 
-        if constexpr (MODE != TILE_PAR) {
-            // bottom-up within the lane (FP64 updates in place: level m-1,
-            // above m, must still hold its stage value when m reads it)
-#pragma unroll
-            for (int m = L - 1; m >= 0; m--) {
-                const int k = kof(m);
-                const int kc = has(m) ? k : 0;
-                FT Gv[NT];
+            // ---- the transitions
+            FT Gv[NT];
+            {
                 FT v[NT];
 #pragma unroll
-                for (int n = 0; n < NT; n++) v[n] = (CARB != 0 && n >= 7) ? FT(0) : val(n, m);
+                for (int n = 0; n < NT; n++) v[n] = (CARB != 0 && n >= 7) ? FT(0) : X[n];
                 if constexpr (MODEL == OBME_NPZD) {
                     if constexpr (F32) {
                         const float Lt = fm_q10<FM>(a.f) ? m_exp2(v[3] * a.f.log2Q10_10) : 1.0f;
-                        npzd_fast<FM>(a.f, v[0], v[1], v[2], v[4], PARv[m], Lt, Gv[0], Gv[1], Gv[2], Gv[4],
+                        npzd_fast<FM>(a.f, v[0], v[1], v[2], v[4], PARv, Lt, Gv[0], Gv[1], Gv[2], Gv[4],
                                       (CARB & 2) ? &Gv[NT - 1] : nullptr);
                     } else {
                         FT Lt = FT(1);
                         if (a.q.use_Q10) Lt = m_pow(a.q.Q10, v[3] / FT(10));
-                        npzd_tendencies(a.q, v[0], v[1], v[2], v[3], v[4], PARv[m], Lt, Gv[0], Gv[1], Gv[2], Gv[4],
+                        npzd_tendencies(a.q, v[0], v[1], v[2], v[3], v[4], PARv, Lt, Gv[0], Gv[1], Gv[2], Gv[4],
                                         (CARB & 2) ? &Gv[NT - 1] : nullptr);
                     }
                     Gv[3] = FT(0);
                 } else {
-                    if constexpr (F32) lobster_fast<CARB, FM>(a.f, v, PARv[m], Gv);
-                    else lobster_tendencies<FT, CARB>(a.q, v, PARv[m], Gv);
+                    if constexpr (F32) lobster_fast<CARB, FM>(a.f, v, PARv, Gv);
+                    else lobster_tendencies<FT, CARB>(a.q, v, PARv, Gv);
                 }
-                // minus the upwind drift flux divergence: w(face) c(upwind cell);
-                // no flux through the top face (w = 0 there)
-                {
-                    const bool below_top = (m > 0) || (k < ktop);
-                    const FT cn0 = (m == 0) ? c0_up : val(TR::SINK0, m > 0 ? m - 1 : 0);
-                    const FT cn1 = (m == 0) ? c1_up : val(TR::SINK1, m > 0 ? m - 1 : 0);
-                    if constexpr (F32 && UNIFORM) {
-                        // w (c(k+1) - c(k)) / dz, c(k+1) = 0 above the top cell
-                        const float d0 = (below_top ? cn0 : 0.0f) - v[TR::SINK0];
-                        const float d1 = (below_top ? cn1 : 0.0f) - v[TR::SINK1];
-                        Gv[TR::SINK0] = fmaf(-a.wi0, d0, Gv[TR::SINK0]);
-                        Gv[TR::SINK1] = fmaf(-a.wi1, d1, Gv[TR::SINK1]);
+            }
+            // minus the upwind drift flux divergence, w(face) c(upwind cell):
+            // the inflow through the top face is w c of the level above
+            // (no flux through the surface: w = 0 there, and Xup = 0)
+            if constexpr (F32 && UNIFORM) {
+                Gv[S0] = fmaf(-a.wi0, (top ? 0.0f : Xup[st][0]) - X[S0], Gv[S0]);
+                Gv[S1] = fmaf(-a.wi1, (top ? 0.0f : Xup[st][1]) - X[S1], Gv[S1]);
+            } else {
+                const FT Fhi0 = top ? FT(0) : whi0 * Xup[st][0];
+                const FT Fhi1 = top ? FT(0) : whi1 * Xup[st][1];
+                Gv[S0] = Gv[S0] - (Fhi0 - wlo0 * X[S0]) * invdz;
+                Gv[S1] = Gv[S1] - (Fhi1 - wlo1 * X[S1]) * invdz;
+            }
+            Xup[st][0] = X[S0];
+            Xup[st][1] = X[S1];
+
+            if constexpr ((EXTRA & 2) != 0) {
+                // the top-cell CO2 flux (GasExchange, gas_exchange.jl:26-38) of
+                // this stage's DIC and Alk, the surface kernel's expressions
+                if (top) {
+                    const unsigned tc = a.base0 + (unsigned)i + a.sx * (unsigned)j + a.sxy * (unsigned)ktop;
+                    const double T = a.T3 ? (double)a.T3[tc] : (a.Tprof ? a.Tprof[ktop] : a.Tconst);
+                    const double S = a.S3 ? (double)a.S3[tc] : (a.Sprof ? a.Sprof[ktop] : a.Sconst);
+                    SurfaceCarbon sc;
+                    surface_carbon_setup(a.co2, T, S, a.co2.wind_speed, sc);
+                    double pH = 0.0;
+                    int ev = 0;
+                    const double F = surface_flux_warm(sc, a.solver, (double)X[7], (double)X[8],
+                                                       a.co2.air_concentration, F32 ? 1 : 0, Hc, pH, ev,
+                                                       F32 ? FP32_NEWTON_TOL : 1e-14);
+                    Gv[7] = Gv[7] - (FT)F * a.invdz_top;
+                }
+            }
+            if constexpr ((CARB & 2) != 0) {
+                // the top-cell O2 flux (GasExchange for O2, gas_exchange.jl:26-38,
+                // GasExchange.jl:127-146) of this stage's O2: G -= F / Δz
+                if (a.o2_on && top) {
+                    const FT kdz = (FT)a.o2c[2 * col2d], air = (FT)a.o2c[2 * col2d + 1];
+                    Gv[NT - 1] = Gv[NT - 1] - kdz * (X[NT - 1] - air);
+                }
+            }
+            if constexpr ((EXTRA & 4) != 0) {
+                // the sediment at the bottom cell: the sinking flux through
+                // the bottom face, f = Σ -w c (Sediments/tracked_fields.jl:53-70),
+                // burial e = a + b (f / (k + f))^2 and the receiver's coupled
+                // tendency (1 - e) f / Δz (instant_remineralisation.jl:101-123,
+                // Sediments/tracer_coupling.jl:30-38)
+                if (k == 0) {
+                    const FT f = a.sed_w0 * X[S0] + a.sed_w1 * X[S1];
+                    const FT qf = f / (a.sed_k + f);
+                    const FT e = a.sed_a + a.sed_b * (qf * qf);
+                    const FT cpl = ((FT(1) - e) * f) / a.sed_dz0;
+#pragma unroll
+                    for (int n = 0; n < NT; n++)
+                        if (n == a.sed_recv) Gv[n] = Gv[n] + cpl;
+                    // the sediment step after this stage's substep
+                    // (update_state.jl:6-17) takes the tendency evaluated
+                    // after the previous step, from this stage-start state:
+                    // e f here.  Its first step ever evaluates the tendency
+                    // lazily, after the first substep: the tendency of the
+                    // second stage start serves both.
+                    const double ge = (double)(e * f);
+                    if (st == 0) {
+                        if (!a.sed_init)
+                            sed_st += a.sed_dt[0] * (a.sed_ab2 ? a.sed_cn * ge - a.sed_cm * sed_gm : ge);
                     } else {
-                        const FT idz = g_invdz(kc);
-                        const FT Fhi0 = below_top ? g_whi(0, kc) * cn0 : FT(0);
-                        const FT Fhi1 = below_top ? g_whi(1, kc) * cn1 : FT(0);
-                        Gv[TR::SINK0] = Gv[TR::SINK0] - (Fhi0 - g_wlo(0, kc) * v[TR::SINK0]) * idz;
-                        Gv[TR::SINK1] = Gv[TR::SINK1] - (Fhi1 - g_wlo(1, kc) * v[TR::SINK1]) * idz;
+                        if (st == 1 && a.sed_init) sed_st += a.sed_dt[0] * ge;
+                        sed_st += a.sed_dt[st] * ge;
                     }
-                }
-                if constexpr ((EXTRA & 2) != 0) {
-                    // the top-cell CO2 flux (GasExchange, gas_exchange.jl:26-38) of
-                    // this stage's DIC and Alk, the surface kernel's expressions
-                    if (top_lane && m == 0) {
-                        const unsigned top = (unsigned)a.base0 + (unsigned)i + a.sx * (unsigned)j +
-                                             a.sxy * (unsigned)ktop;
-                        const double T = a.T3 ? (double)a.T3[top] : (a.Tprof ? a.Tprof[ktop] : a.Tconst);
-                        const double S = a.S3 ? (double)a.S3[top] : (a.Sprof ? a.Sprof[ktop] : a.Sconst);
-                        SurfaceCarbon sc;
-                        surface_carbon_setup(a.co2, T, S, a.co2.wind_speed, sc);
-                        double pH = 0.0;
-                        int ev = 0;
-                        const double F = surface_flux_warm(sc, a.solver, (double)val(7, 0), (double)val(8, 0),
-                                                           a.co2.air_concentration, F32 ? 1 : 0, Hc, pH, ev,
-                                                           F32 ? FP32_NEWTON_TOL : 1e-14);
-                        Gv[7] = Gv[7] - (FT)F * a.invdz_top;
-                    }
-                }
-                if constexpr ((CARB & 2) != 0) {
-                    // the top-cell O2 flux (GasExchange for O2, gas_exchange.jl:26-38,
-                    // GasExchange.jl:127-146) of this stage's O2: G -= F / Δz
-                    if (a.o2_on && top_lane && m == 0) {
-                        const FT kdz = (FT)a.o2c[2 * col2d], air = (FT)a.o2c[2 * col2d + 1];
-                        Gv[NT - 1] = Gv[NT - 1] - kdz * (val(NT - 1, 0) - air);
-                    }
-                }
-                if constexpr ((EXTRA & 4) != 0) {
-                    // the sediment at the bottom cell: the sinking flux through
-                    // the bottom face, f = Σ -w c (Sediments/tracked_fields.jl:53-70),
-                    // burial e = a + b (f / (k + f))^2 and the receiver's coupled
-                    // tendency (1 - e) f / Δz (instant_remineralisation.jl:101-123,
-                    // Sediments/tracer_coupling.jl:30-38)
-                    if (col_ok && k == 0 && has(m)) {
-                        const FT f = a.sed_w0 * v[TR::SINK0] + a.sed_w1 * v[TR::SINK1];
-                        const FT qf = f / (a.sed_k + f);
-                        const FT e = a.sed_a + a.sed_b * (qf * qf);
-                        const FT cpl = ((FT(1) - e) * f) / a.sed_dz0;
-#pragma unroll
-                        for (int n = 0; n < NT; n++)
-                            if (n == a.sed_recv) Gv[n] = Gv[n] + cpl;
-                        // the sediment step after this stage's substep
-                        // (update_state.jl:6-17) takes the tendency evaluated
-                        // after the previous step, from this stage-start
-                        // state: e f here.  Its first step ever evaluates
-                        // the tendency lazily, after the first substep: the
-                        // tendency of the second stage start serves both.
-                        const double ge = (double)(e * f);
-                        if (st == 0) {
-                            if (!a.sed_init)
-                                sed_st += a.sed_dt[0] * (a.sed_ab2 ? a.sed_cn * ge - a.sed_cm * sed_gm : ge);
-                        } else {
-                            if (st == 1 && a.sed_init) sed_st += a.sed_dt[0] * ge;
-                            sed_st += a.sed_dt[st] * ge;
-                        }
-                        sed_ge = ge;
-                    }
-                }
-                if constexpr (MODE == TILE_TENDENCY) {
-#pragma unroll
-                    for (int n = 0; n < NT; n++) Gm[n][m] = Gv[n];   // Gm carries G out
-                } else {
-                    if constexpr (CARB & 1) {
-                        // surface record: DIC transition without the flux, Alk at the stage start
-                        if (top_lane && a.co2_on && m == 0) {
-                            a.srec[(1 + st) * ncol2d + col2d] = Gv[7];
-                            a.srec[(4 + st) * ncol2d + col2d] = val(8, 0);
-                        }
-                    }
-                    // RK3 substep (BoxModel/timesteppers.jl:79-92 form):
-                    // stage 1 U += (Δt γ1) G; stages 2, 3 U += Δt (γ G + ζ G⁻)
-#pragma unroll
-                    for (int n = 0; n < NT; n++) {
-                        if (n == RO) continue;
-                        if constexpr (F32) {
-                            const float A = (st == 0) ? a.rk_a[0] : (st == 1 ? a.rk_a[1] : a.rk_a[2]);
-                            const float B = (st == 0) ? a.rk_b[0] : (st == 1 ? a.rk_b[1] : a.rk_b[2]);
-                            D[n][m] = fmaf(A, Gv[n], fmaf(B, Gm[n][m], D[n][m]));
-                        } else {
-                            const double gs = (st == 1) ? g2 : g3;
-                            const double zs = (st == 1) ? z2 : z3;
-                            const double inc = (st == 0) ? (a.dtf * g1) * Gv[n] : a.dtf * (gs * Gv[n] + zs * Gm[n][m]);
-                            U[n][m] += inc;
-                        }
-                        Gm[n][m] = Gv[n];
-                    }
+                    sed_ge = ge;
                 }
             }
-        }
-    };
-    // (the stages stay a loop: unrolled, the FP32 step measured 6 % slower)
-#pragma unroll 1
-    for (int st = 0; st < NSTAGE; st++) run_stage(st);
-    if constexpr ((EXTRA & 2) && F32) {
-        if (top_lane && a.Hprev) a.Hprev[col2d] = Hc;
-    }
-    if constexpr ((EXTRA & 4) != 0) {
-        // the sediment after the last step: storage, G⁻ = the tendency of its
-        // last step, Gⁿ = e f of the final state
+            if constexpr (MODE == TILE_TENDENCY) {
 #pragma unroll
-        for (int m = 0; m < L; m++) {
-            if (sed_lane && kof(m) == 0) {
-                const FT f = a.sed_w0 * val(TR::SINK0, m) + a.sed_w1 * val(TR::SINK1, m);
-                const FT qf = f / (a.sed_k + f);
-                const FT e = a.sed_a + a.sed_b * (qf * qf);
-                if (a.storage) a.storage[col2d] = sed_st;
-                a.sedG[col2d] = (double)(e * f);
-                a.sedG[ncol2d + col2d] = sed_ge;
-            }
-        }
-    }
-
-    if constexpr (PIPE) {
-        // ---- stage out (PIPE): each lane scatters to the elements it
-        // gathered, then every wave reads its blocks in load order
-        // (coalesced), the buffer is refilled with the tile after next, and
-        // the stores are issued; an element that must not be written (past
-        // Nx, or the top-cell DIC left to the surface kernel) gets an offset
-        // past the buffer, which the buffer store drops, so every wave
-        // issues exactly LPW stores.
-#pragma unroll
-        for (int n = 0; n < NT; n++)
-#pragma unroll
-            for (int m = 0; m < L; m++) pbuf[pidx(n, kof(m), c)] = val(n, m);
-        wait_lgkm0();
-        __builtin_amdgcn_s_barrier();
-        FT v[NT][NBLK / NW];
-        unsigned off[NT][NBLK / NW];
-        const int kl = lane / TXB, cc = lane % TXB;
-#pragma unroll
-        for (int n = 0; n < NT; n++)
-#pragma unroll
-            for (int q = 0; q < NBLK / NW; q++) {
-                const int kb = w + NW * q;
-                const int k = kb * PB + kl;
-                v[n][q] = pbuf[n * TRS + kb * BLK + lane];
-                bool ok = ibase + cc < a.Nx && n != RO;
+                for (int n = 0; n < NT; n++) Gout[n] = Gv[n];
+            } else {
                 if constexpr (CARB & 1) {
-                    if (n == 7 && a.co2_on && k == ktop) ok = false;
+                    // surface record: DIC transition without the flux, Alk at the stage start
+                    if (top && col_ok && a.co2_on) {
+                        a.srec[(1 + st) * ncol2d + col2d] = Gv[7];
+                        a.srec[(4 + st) * ncol2d + col2d] = X[8];
+                    }
                 }
-                off[n][q] = ok ? goff2(ibase + cc, j, k) : a.field_bytes;
+                // RK3 substep (BoxModel/timesteppers.jl:79-92 form):
+                // stage 1 U += (Δt γ1) G; stages 2, 3 U += Δt (γ G + ζ G⁻)
+#pragma unroll
+                for (int n = 0; n < NT; n++) {
+                    if (n == RO) continue;
+                    if constexpr (F32) {
+                        if (st == 0) D[n] = a.rk_a[0] * Gv[n];
+                        else D[n] = fmaf(a.rk_a[st], Gv[n], fmaf(a.rk_b[st], Gm[n], D[n]));
+                    } else {
+                        const double gs = (st == 1) ? g2 : g3;
+                        const double zs = (st == 1) ? z2 : z3;
+                        const double inc = (st == 0) ? (a.dtf * g1) * Gv[n] : a.dtf * (gs * Gv[n] + zs * Gm[n]);
+                        U[n] += inc;
+                    }
+                    Gm[n] = Gv[n];
+                }
             }
-        wait_lgkm0();
-        __builtin_amdgcn_s_barrier();
-        const unsigned t_next = wg_tile + 2 * t_step;
-        if (t_next < t_end) pipe_issue(t_next, pbuf);
-        // seen from the next tile: a tile was stored before it, and the tile
-        // after it has been issued iff it exists
-        prev_tile = true;
-        next_issued = t_next < t_end;
-#pragma unroll
-        for (int n = 0; n < NT; n++) {
-            const rsrc_t rd = rs(a.tr[n]);
-#pragma unroll
-            for (int q = 0; q < NBLK / NW; q++) buf_store(rd, off[n][q], v[n][q]);
         }
-        continue;
-    }
 
-    // ---- stage out: scatter to the tiles, coalesced stores along i.
-    // Outputs: STEP the written tracers; PAR the PAR field; TENDENCY the
-    // written tendencies (added to G) and PAR (tile NT) when requested.
-    __syncthreads();
-    constexpr int NOUT = (MODE == TILE_PAR) ? 1 : (MODE == TILE_TENDENCY ? NT + 1 : NT);
+        // ---- the level's outputs
+        const unsigned off = col_ok ? lev_off(k) : a.field_bytes;
+        if constexpr (MODE == TILE_PAR) {
+            buf_store(rs(a.PAR), off, PARv);
+        } else if constexpr (MODE == TILE_TENDENCY) {
+            FT g0[NT];
 #pragma unroll
-    for (int n = 0; n < NOUT; n++) {
-        if (MODE != TILE_PAR && n == RO) continue;
-        FT* T = tile(n);
+            for (int n = 0; n < NT; n++)
+                if (n != RO) buf_load(rs(a.G[n]), off, g0[n]);
 #pragma unroll
-        for (int m = 0; m < L; m++) {
-            const int k = kof(m);
-            FT v;
-            if constexpr (MODE == TILE_PAR) v = PARv[m];
-            else if constexpr (MODE == TILE_TENDENCY) v = (n == NT) ? PARv[m] : Gm[n < NT ? n : 0][m];
-            else v = val(n < NT ? n : 0, m);
-            if (has(m)) T[k * ROW + c] = v;
-        }
-    }
-    __syncthreads();
+            for (int n = 0; n < NT; n++)
+                if (n != RO) buf_store(rs(a.G[n]), off, g0[n] + Gout[n]);
+            if (a.PAR) buf_store(rs(a.PAR), off, PARv);
+        } else {
 #pragma unroll
-    for (int n = 0; n < NOUT; n++) {
-        if (MODE != TILE_PAR && n == RO) continue;
-        FT* T = tile(n);
-        FT* dst;
-        if constexpr (MODE == TILE_PAR) dst = a.PAR;
-        else if constexpr (MODE == TILE_TENDENCY) dst = (n == NT) ? a.PAR : a.G[n];
-        else dst = a.tr[n];
-        if (dst == nullptr) continue;
-        const rsrc_t rd = rs(dst);
-        const bool accumulate = (MODE == TILE_TENDENCY) && n < NT;
-        FT v[L];
-#pragma unroll
-        for (int q = 0; q < L; q++) v[q] = (e_idx(q) >= 0) ? T[e_idx(q)] : FT(0);
-        if (accumulate) {
-            FT g0[L];
-#pragma unroll
-            for (int q = 0; q < L; q++) buf_load(rd, e_off(q), g0[q]);
-#pragma unroll
-            for (int q = 0; q < L; q++) v[q] = g0[q] + v[q];
-        }
-#pragma unroll
-        for (int q = 0; q < L; q++) {
-            const int e = threadIdx.x + q * BS;
-            const int k = e / TXB, cc = e % TXB;
-            bool ok = e_idx(q) >= 0 && ibase + cc < a.Nx;
-            if constexpr ((CARB & 1) && MODE == TILE_STEP) {
+            for (int n = 0; n < NT; n++) {
+                if (n == RO) continue;
+                unsigned o = off;
                 // the top-cell DIC is written by the surface kernel
-                if (n == 7 && a.co2_on && k == ktop) ok = false;
+                if constexpr (CARB & 1) {
+                    if (n == 7 && top && a.co2_on) o = a.field_bytes;
+                }
+                buf_store(rs(a.tr[n]), o, F32 ? U[n] + D[F32 ? n : 0] : U[n]);
             }
-            // an element not to be written gets an offset past the buffer,
-            // which the buffer store drops (no branch around the store)
-            buf_store(rd, ok ? e_off(q) : a.field_bytes, v[q]);
+            if constexpr ((EXTRA & 4) != 0) {
+                // the sediment after the last stage: storage, G⁻ = the tendency
+                // of its last step, Gⁿ = e f of the final state
+                if (k == 0 && col_ok) {
+                    const FT x0 = F32 ? U[S0] + D[F32 ? S0 : 0] : U[S0];
+                    const FT x1 = F32 ? U[S1] + D[F32 ? S1 : 0] : U[S1];
+                    const FT f = a.sed_w0 * x0 + a.sed_w1 * x1;
+                    const FT qf = f / (a.sed_k + f);
+                    const FT e = a.sed_a + a.sed_b * (qf * qf);
+                    if (a.storage) a.storage[col2d] = sed_st;
+                    a.sedG[col2d] = (double)(e * f);
+                    a.sedG[ncol2d + col2d] = sed_ge;
+                }
+            }
         }
     }
-    }   // tile loop
+    if constexpr ((EXTRA & 2) && F32) {
+        if (col_ok && a.Hprev) a.Hprev[col2d] = Hc;
+    }
 }
 
 // ------------------------------------------------------------------ surface

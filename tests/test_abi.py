@@ -90,10 +90,6 @@ def test_invalid_configs_are_rejected_before_device_work():
     st, _, msg = _create(cfg)
     assert st == abi.OBME_ERR_INVALID
 
-    g2 = RectilinearGrid(size=(2, 2, 200), extent=(1, 1, 200))
-    st, _, msg = _create(_cfg(NPZD(g2)))
-    assert st == abi.OBME_ERR_UNSUPPORTED and "Nz" in msg
-
     cfg = _cfg(LOBSTER(g, carbonate_system=CarbonateSystem()))
     cfg.sinking.speed[0] = 1e-4   # NO3 does not sink
     st, _, msg = _create(cfg)

@@ -170,23 +170,23 @@ def test_co2_flux_points(mode):
     assert rel_l2(F, F_ref) <= 1e-10
 
 
-TILE_CFGS = ["2,2,8", "1,1,16", "2,1,8", "2,2,16", "1,1,8", "4,2,8", "2,2,4", "1,1,4"]
+SWEEP_WAVES = ["1", "2", "4", "8"]
 
 
-@pytest.mark.parametrize("tile", TILE_CFGS)
-def test_tile_configurations_fp64(tile, monkeypatch):
-    """Every sweep tile configuration gives the same answer (FP64 parity)."""
-    print("case tile fp64", tile, flush=True)
-    monkeypatch.setenv("OBME_TILE", tile)
+@pytest.mark.parametrize("waves", SWEEP_WAVES)
+def test_sweep_waves_fp64(waves, monkeypatch):
+    """Every workgroup size of the sweep gives the same answer (FP64 parity)."""
+    print("case sweep waves fp64", waves, flush=True)
+    monkeypatch.setenv("OBME_SWEEP_WAVES", waves)
     wl = workloads.lobster_3d(40, 3, 29, nsteps=3)
     errs = _run_both(wl, "fp64")
     assert all(e <= FP64_TOL for e in errs.values()), errs
 
 
-@pytest.mark.parametrize("tile", TILE_CFGS)
-def test_tile_configurations_fp32(tile, monkeypatch):
-    print("case tile fp32", tile, flush=True)
-    monkeypatch.setenv("OBME_TILE", tile)
+@pytest.mark.parametrize("waves", SWEEP_WAVES)
+def test_sweep_waves_fp32(waves, monkeypatch):
+    print("case sweep waves fp32", waves, flush=True)
+    monkeypatch.setenv("OBME_SWEEP_WAVES", waves)
     wl = workloads.lobster_3d(40, 3, 29, nsteps=5)
     errs = _run_both(wl, "fp32")
     print("errors", {k: float(v) for k, v in errs.items()}, flush=True)

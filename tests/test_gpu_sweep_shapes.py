@@ -1,10 +1,9 @@
-"""The pipelined persistent sweep (FP32-mixed fused step on full columns:
-each workgroup walks a run of tiles and prefetches the next tile into LDS
-while it computes the current one) against the one-tile-per-workgroup
-sweep (OBME_PIPE=0) and the FP64 oracle.  Both sweeps run the same
-arithmetic, so their results must be bit-identical; the shapes cover a
-partial last tile along i, fewer tiles than XCDs (idle workgroups), and
-several tiles per workgroup."""
+"""The column sweep (lane per water column) over workgroup sizes and grid
+shapes: every workgroup size of 1, 2, 4 or 8 waves runs the same
+arithmetic per column, so the results must be bit-identical.  The shapes
+cover a partial last workgroup along i, fewer workgroups than XCDs, many
+workgroups per row and columns deeper than a wave is wide; the small ones
+are checked against the FP64 oracle too."""
 import numpy as np
 import pytest
 
@@ -17,9 +16,9 @@ from oceanbiome.workloads import Workload
 pytestmark = pytest.mark.gpu
 
 
-def _step(wl, pipe, monkeypatch, nsteps):
+def _step(wl, waves, monkeypatch, nsteps):
     import torch
-    monkeypatch.setenv("OBME_PIPE", "1" if pipe else "0")
+    monkeypatch.setenv("OBME_SWEEP_WAVES", str(waves))
     dev = HIPBiogeochemistry(wl.bgc, "fp32")
     tr = to_device(wl.tracers, dev.dtype)
     dev.step(tr, nsteps, wl.dt)
@@ -40,15 +39,16 @@ def _npzd_3d(Nx, Ny, Nz, seed=3):
     return Workload(f"npzd_{Nx}x{Ny}x{Nz}", bgc, grid, fields, 60.0, 10)
 
 
-@pytest.mark.parametrize("shape", [(45, 20, 64), (7, 3, 64), (512, 64, 64)])
-def test_pipelined_sweep_lobster(shape, monkeypatch):
-    print("case pipe lobster", shape, flush=True)
+@pytest.mark.parametrize("shape", [(45, 20, 64), (7, 3, 64), (512, 64, 64), (300, 5, 200)])
+def test_sweep_waves_lobster(shape, monkeypatch):
+    print("case sweep waves lobster", shape, flush=True)
     wl = workloads.lobster_3d(*shape, nsteps=4)
-    a = _step(wl, True, monkeypatch, 4)
-    b = _step(wl, False, monkeypatch, 4)
-    for n, (x, y) in enumerate(zip(a, b)):
-        assert np.array_equal(x, y), (n, float(np.max(np.abs(x - y))))
-    if shape[0] * shape[1] <= 1024:      # the oracle on the small shapes
+    a = _step(wl, 4, monkeypatch, 4)
+    for waves in (1, 2, 8):
+        b = _step(wl, waves, monkeypatch, 4)
+        for n, (x, y) in enumerate(zip(a, b)):
+            assert np.array_equal(x, y), (waves, n, float(np.max(np.abs(x - y))))
+    if shape[0] * shape[1] <= 1500:      # the oracle on the small shapes
         ref = [t.copy() for t in wl.tracers]
         O.grid_run(wl.bgc.config(abi.OBME_FP64), ref, 4, wl.dt)
         g = wl.grid
@@ -56,11 +56,11 @@ def test_pipelined_sweep_lobster(shape, monkeypatch):
         assert max(errs) <= 1e-5, errs
 
 
-def test_pipelined_sweep_npzd_keeps_temperature(monkeypatch):
-    print("case pipe npzd", flush=True)
+def test_sweep_npzd_keeps_temperature(monkeypatch):
+    print("case sweep npzd", flush=True)
     wl = _npzd_3d(40, 9, 64)
-    a = _step(wl, True, monkeypatch, 10)
-    b = _step(wl, False, monkeypatch, 10)
+    a = _step(wl, 4, monkeypatch, 10)
+    b = _step(wl, 1, monkeypatch, 10)
     for n, (x, y) in enumerate(zip(a, b)):
         assert np.array_equal(x, y), (n, float(np.max(np.abs(x - y))))
     # T (tracer 3) is read-only: unchanged, halos included
