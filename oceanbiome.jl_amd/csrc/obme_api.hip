@@ -49,10 +49,10 @@ extern "C" const char* obme_version(void) { return "obme 0.1.0 gfx950"; }
 // VGPRs (4 waves per SIMD, the occupancy at which the 512x512 grid's 4096
 // waves fill the chip once).
 constexpr int SWEEP_MAX_THREADS = 512;
-template <int MODEL, int CARB, typename FT, int MODE, bool UNIFORM, int FM = -1, int EXTRA = 0>
+template <int MODEL, int CARB, typename FT, int MODE, bool UNIFORM, int FM = -1, int EXTRA = 0, bool MASKX = true>
 __global__ void __launch_bounds__(SWEEP_MAX_THREADS, sizeof(FT) == 4 ? 4 : 1) k_column_lane(StepArgs<FT> a)
 {
-    column_lane<MODEL, CARB, FT, MODE, UNIFORM, FM, EXTRA>(a, (int)blockDim.x);
+    column_lane<MODEL, CARB, FT, MODE, UNIFORM, FM, EXTRA, MASKX>(a, (int)blockDim.x);
 }
 
 // The O2 gas exchange's per-column coefficients (GasExchange.jl:127-146):
@@ -192,7 +192,8 @@ struct obme_handle {
     double* d_o2c = nullptr;   // 2*Nx*Ny: k / Δz_top and C_air α of the O2 exchange
     bool o2c_ready = false;    // d_o2c holds the current obme_step call's coefficients
     double* d_Hprev = nullptr; // Nx*Ny warm starts of [H+] (mixed path)
-    double* d_surfc = nullptr; // [SURF_PACK][Nx*Ny] surface constants of the current stepper call
+    double* d_surfc = nullptr; // [SURF_PACK][surfc_cols] surface constants of the current stepper call
+    int surfc_cols = 0;        // Nx*Ny, or 1 when the call's T and S are the same in every column
     bool surfc_ready = false;  // d_surfc holds the constants of this call's T and S
     int sink_idx[MAX_SINK];
     int sweep_waves;           // waves per workgroup of the sweep (1, 2, 4 or 8)
@@ -666,11 +667,22 @@ static obme_status launch_lane(obme_handle* h, const StepArgs<FT>& a, hipStream_
     // one-dimensional: tile = xcd_tile(blockIdx.x), row-major over (i tile, j)
     const unsigned n_tiles = (unsigned)(((h->cfg.grid.Nx + cw - 1) / cw) * h->cfg.grid.Ny);
     auto k = h->uniform ? k_column_lane<MODEL, CARB, FT, MODE, true> : k_column_lane<MODEL, CARB, FT, MODE, false>;
-    if constexpr (sizeof(FT) == 4 && MODE == TILE_STEP) {
-        // the FP32 fused step compiled with the model's default formulations
-        if (fm_code(a.f) == default_fm<MODEL>())
-            k = h->uniform ? k_column_lane<MODEL, CARB, FT, MODE, true, default_fm<MODEL>()>
-                           : k_column_lane<MODEL, CARB, FT, MODE, false, default_fm<MODEL>()>;
+    // the fused step on rows of whole workgroups (Nx a multiple of 64 NW):
+    // stores without a mask; FP32 compiled with the model's default formulations
+    const bool whole = h->cfg.grid.Nx % cw == 0;
+    if constexpr (MODE == TILE_STEP) {
+        if constexpr (sizeof(FT) == 4) {
+            if (fm_code(a.f) == default_fm<MODEL>()) {
+                constexpr int FM = default_fm<MODEL>();
+                k = whole ? (h->uniform ? k_column_lane<MODEL, CARB, FT, MODE, true, FM, 0, false>
+                                        : k_column_lane<MODEL, CARB, FT, MODE, false, FM, 0, false>)
+                          : (h->uniform ? k_column_lane<MODEL, CARB, FT, MODE, true, FM>
+                                        : k_column_lane<MODEL, CARB, FT, MODE, false, FM>);
+            }
+        } else if (whole) {
+            k = h->uniform ? k_column_lane<MODEL, CARB, FT, MODE, true, -1, 0, false>
+                           : k_column_lane<MODEL, CARB, FT, MODE, false, -1, 0, false>;
+        }
     }
     if constexpr (MODE == TILE_STEP) {
         // negative scaling (1), the inline CO2 flux (2, with 1) and the
@@ -797,7 +809,10 @@ static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* 
     }
     if (r != OBME_OK || MODE == TILE_PAR || !a.co2_on) return r;
     SurfaceArgs<FT> sa = make_surface_args<FT>(h, tracers, T, S);
-    if (MODE == TILE_STEP && h->surfc_ready) sa.surfc = h->d_surfc;
+    if (MODE == TILE_STEP && h->surfc_ready) {
+        sa.surfc = h->d_surfc;
+        sa.surfc_cols = h->surfc_cols;
+    }
     sa.dtf = (FT)dt;
     rk3_products(dt, sa.rk_a, sa.rk_b);
     if (MODE == TILE_TENDENCY) sa.GDIC = (FT*)G[7];
@@ -849,13 +864,19 @@ extern "C" obme_status obme_step_with_sediment(obme_handle* h, void* const* trac
         const size_t ncol = (size_t)c.grid.Nx * c.grid.Ny;
         if (!h->d_surfc && hipMalloc((void**)&h->d_surfc, SURF_PACK * ncol * sizeof(double)) != hipSuccess)
             return fail(OBME_ERR_NOMEM, "surface constants");
-        const unsigned nb = (unsigned)((ncol + 255) / 256);
-        if (h->prec == OBME_FP64)
-            hipLaunchKernelGGL(k_surface_prep<double>, dim3(nb), dim3(256), 0, s,
-                               make_surface_args<double>(h, tracers, T, S), h->d_surfc);
-        else
-            hipLaunchKernelGGL(k_surface_prep<float>, dim3(nb), dim3(256), 0, s,
-                               make_surface_args<float>(h, tracers, T, S), h->d_surfc);
+        // T and S from the configuration (a profile or a constant) are the
+        // same in every column: one column of constants serves them all
+        h->surfc_cols = (!T && !S) ? 1 : (int)ncol;
+        const unsigned nb = (unsigned)((h->surfc_cols + 255) / 256);
+        if (h->prec == OBME_FP64) {
+            SurfaceArgs<double> sa = make_surface_args<double>(h, tracers, T, S);
+            sa.surfc_cols = h->surfc_cols;
+            hipLaunchKernelGGL(k_surface_prep<double>, dim3(nb), dim3(256), 0, s, sa, h->d_surfc);
+        } else {
+            SurfaceArgs<float> sa = make_surface_args<float>(h, tracers, T, S);
+            sa.surfc_cols = h->surfc_cols;
+            hipLaunchKernelGGL(k_surface_prep<float>, dim3(nb), dim3(256), 0, s, sa, h->d_surfc);
+        }
         HIP_TRY(hipGetLastError());
         h->surfc_ready = true;
     }

@@ -121,6 +121,20 @@ __device__ __forceinline__ void buf_store(rsrc_t r, unsigned off, double v)
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, 0);
 }
 
+// Plain global access at a 32-bit byte offset from an array's base: the
+// base stays in scalar registers and one per-lane offset serves every array
+// of the same layout (global_load/store with a scalar base address).
+template <typename FT>
+__device__ __forceinline__ FT gload(const FT* p, unsigned boff)
+{
+    return *reinterpret_cast<const FT*>(reinterpret_cast<const char*>(p) + boff);
+}
+template <typename FT>
+__device__ __forceinline__ void gstore(FT* p, unsigned boff, FT v)
+{
+    *reinterpret_cast<FT*>(reinterpret_cast<char*>(p) + boff) = v;
+}
+
 // ---------------------------------------------------------------- parameters
 
 // obme_npd_params converted to the arithmetic type.
@@ -378,11 +392,13 @@ constexpr float F_TINY = 1.17549435e-38f;  // smallest normal float
 
 // Grazing of the PhytoZoo plankton (plankton.jl:118-135, :352-397) in FP32,
 // reformulated over n = pref P^2 + (1-pref) sPOM^2 and d = pref P + (1-pref) sPOM:
-// the preference-weighted food is n/d, so the concentration limitation is
-// n^2 / (n^2 + kG^2 d^2) (quadratic) or n / (n + kG d) (linear), and the
-// grazed fractions of the total are pref P^2 / n and (1-pref) sPOM^2 / n.
-// Two reciprocals instead of the reference's three divisions; F_TINY keeps
-// 0/0 away as the reference's eps(food) does.
+// the preference-weighted food is n/d, so the total grazing is
+// g Z n^2 / (n^2 + kG^2 d^2) (quadratic) or g Z n / (n + kG d) (linear), and
+// the grazed fractions of the total are pref P^2 / n and (1-pref) sPOM^2 / n.
+// With fr = total / n = g Z n / (n^2 + kG^2 d^2) (or g Z / (n + kG d)) the
+// three terms are fr n, fr pref P^2 and fr (1-pref) sPOM^2: one reciprocal
+// instead of the reference's three divisions; F_TINY keeps 0/0 away as the
+// reference's eps(food) does.
 struct FGrazing {
     float total, onP, onSmall;
 };
@@ -404,20 +420,20 @@ __device__ __forceinline__ FGrazing fast_grazing(const FastParams& f, float P, f
 {
     const float pP = f.pref * P;
     const float cS = f.prefc * sPOM;
-    const float n = fmaf(pP, P, cS * sPOM);
+    const float sS = cS * sPOM;
+    const float n = fmaf(pP, P, sS);
     const float d = pP + cS;
-    float L;
-    if (fm_graz<FM>(f) == OBME_LINEAR) L = n * frcp(fmaf(f.kG, d, n) + F_TINY);
+    const float gZ = f.g * Z;
+    float fr;
+    if (fm_graz<FM>(f) == OBME_LINEAR) fr = gZ * frcp(fmaf(f.kG, d, n) + F_TINY);
     else {
-        const float n2 = n * n;
         const float kd = f.kG * d;
-        L = n2 * frcp(fmaf(kd, kd, n2) + F_TINY);
+        fr = (gZ * n) * frcp(fmaf(kd, kd, n * n) + F_TINY);
     }
     FGrazing G;
-    G.total = f.g * L * Z;
-    const float fr = G.total * frcp(n + F_TINY);
+    G.total = fr * n;
     G.onP = fr * (pP * P);
-    G.onSmall = fr * (cS * sPOM);
+    G.onSmall = fr * sS;
     return G;
 }
 
@@ -436,14 +452,27 @@ template <int CARB, int FM = -1>
 __device__ __forceinline__ void lobster_fast(const FastParams& f, const float* v, float PAR, float* out)
 {
     const float NO3 = v[0], NH4 = v[1], P = v[2], Z = v[3], sPOM = v[4], bPOM = v[5], DOM = v[6];
-    // one reciprocal for both Monod terms
     const float a = NO3 + f.kNO3, b = f.kNH4 + NH4;
-    const float rab = frcp(a * b);
-    const float nl = NO3 * __builtin_amdgcn_exp2f(f.nPSI2 * NH4) * (b * rab);
-    const float al = fmax0(NH4 * (a * rab));
-    const float muLP = f.mu0h * fast_light<FM>(f, PAR) * P;
-    const float up_NO3 = muLP * nl;
-    const float muA = muLP * al;
+    float up_NO3, muA;
+    if (fm_light<FM>(f) == OBME_LIGHT_MONDO) {
+        // one reciprocal for both Monod terms and the light limitation
+        // PAR / (kPAR + PAR): with c = kPAR + PAR > 0, 1/(a c) = b r and
+        // 1/(b c) = a r for r = 1/(a b c), and max(0, x / c) = max(0, x) / c
+        const float r = frcp((a * b) * (f.kPAR + PAR));
+        const float nlc = NO3 * __builtin_amdgcn_exp2f(f.nPSI2 * NH4) * (b * r);
+        const float alc = fmax0(NH4 * (a * r));
+        const float muLPc = f.mu0h * PAR * P;
+        up_NO3 = muLPc * nlc;
+        muA = muLPc * alc;
+    } else {
+        // one reciprocal for both Monod terms
+        const float rab = frcp(a * b);
+        const float nl = NO3 * __builtin_amdgcn_exp2f(f.nPSI2 * NH4) * (b * rab);
+        const float al = fmax0(NH4 * (a * rab));
+        const float muLP = f.mu0h * fast_light<FM>(f, PAR) * P;
+        up_NO3 = muLP * nl;
+        muA = muLP * al;
+    }
     const float muP = up_NO3 + muA;
     const FGrazing G = fast_grazing<FM>(f, P, Z, sPOM);
     const float nuP = (fm_mort<FM>(f) == OBME_LINEAR) ? f.mP * P : f.mP * P * P;

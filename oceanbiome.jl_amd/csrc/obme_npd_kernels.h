@@ -31,6 +31,8 @@
 // FP64 root solve thereby leaves the bandwidth-bound sweep.
 #pragma once
 
+#include <type_traits>
+
 #include "obme_device.h"
 
 namespace obme {
@@ -170,9 +172,12 @@ enum { TILE_STEP = 0, TILE_TENDENCY = 1, TILE_PAR = 2 };
 //
 // EXTRA (fused step only), a bit set: 1 negative scaling at the start of
 // every stage; 2 the CO2 flux computed here at the top level (with 1);
-// 4 the InstantRemineralisation sediment at the bottom level.  PARK: the
-// FP32 chlorophyll integrals are Kahan-compensated running sums.
-template <int MODEL, int CARB, typename FT, int MODE, bool UNIFORM, int FM = -1, int EXTRA = 0, bool PARK = true>
+// 4 the InstantRemineralisation sediment at the bottom level.  MASKX: some
+// lanes lie past the last column (Nx is not a multiple of the workgroup's
+// width); false compiles the stores without a mask.  PARK: the FP32
+// chlorophyll integrals are Kahan-compensated running sums.
+template <int MODEL, int CARB, typename FT, int MODE, bool UNIFORM, int FM = -1, int EXTRA = 0, bool MASKX = true,
+          bool PARK = true>
 __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
 {
     using TR = ModelTraits<MODEL, CARB>;
@@ -188,7 +193,8 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int j = (int)(tile / nbx);
     const int i_raw = (int)(tile % nbx) * cw + (int)threadIdx.x;
-    const bool col_ok = i_raw < a.Nx;
+    // (MASKX false: the launch guarantees every lane holds a column)
+    const bool col_ok = !MASKX || i_raw < a.Nx;
     // a lane past the last column computes on the last column and stores nothing
     const int i = col_ok ? i_raw : a.Nx - 1;
     const int Nz = a.Nz;
@@ -198,7 +204,9 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
     const unsigned off_top = (a.base0 + (unsigned)i + a.sx * (unsigned)j + a.sxy * (unsigned)ktop) * ES;
     const unsigned col2d = (unsigned)i + (unsigned)a.Nx * (unsigned)j;
     const unsigned ncol2d = (unsigned)a.Nx * (unsigned)a.Ny;
-    auto rs = [&](const FT* p) { return make_rsrc(p, a.field_bytes); };
+    // byte offset of level k of this column in every parent array (the
+    // same for all of them): a 32-bit per-lane offset from each array's
+    // base address, which stays in two scalar registers
     auto lev_off = [&](int k) -> unsigned { return off_top - (unsigned)(ktop - k) * dk; };
 
     // loads of one level: the PAR mode reads P alone
@@ -207,7 +215,7 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
 #pragma unroll
         for (int n = 0; n < NT; n++) {
             if (MODE == TILE_PAR && n != PI) dst[n] = FT(0);
-            else buf_load(rs(a.tr[n]), off, dst[n]);
+            else dst[n] = gload(a.tr[n], off);
         }
     };
 
@@ -244,8 +252,14 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
 
     FT nxt[NT];
     load_level(nxt, ktop);
-#pragma unroll 1
-    for (int k = ktop; k >= 0; k--) {
+    // one level, all stages; top: the surface level, bot: the bottom level.
+    // The plain fused step passes them as compile-time constants and
+    // compiles the surface and the bottom level apart, so the levels between
+    // carry none of their special cases and prefetch the next level
+    // unconditionally; the other variants pass run-time flags.
+    auto level = [&](const int k, auto top_c, auto bot_c) {
+        const bool top = top_c;
+        const bool bot = bot_c;
         // FP64: U is the state, updated in place (the reference's
         // expressions).  FP32: U is the state at the start of the step and D
         // the step's accumulated increment; stage inputs are U + D and the
@@ -258,8 +272,7 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
             Gm[n] = FT(0);
             if constexpr (F32) D[n] = FT(0);
         }
-        if (k > 0) load_level(nxt, k - 1);
-        const bool top = k == ktop;
+        if (!bot) load_level(nxt, k - 1);
 
         // geometry of level k (uniform across the wave)
         FT zc, up, dn, invdz, wlo0, whi0, wlo1, whi1;
@@ -450,7 +463,7 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
                 // burial e = a + b (f / (k + f))^2 and the receiver's coupled
                 // tendency (1 - e) f / Δz (instant_remineralisation.jl:101-123,
                 // Sediments/tracer_coupling.jl:30-38)
-                if (k == 0) {
+                if (bot) {
                     const FT f = a.sed_w0 * X[S0] + a.sed_w1 * X[S1];
                     const FT qf = f / (a.sed_k + f);
                     const FT e = a.sed_a + a.sed_b * (qf * qf);
@@ -505,34 +518,41 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
             }
         }
 
-        // ---- the level's outputs
-        const unsigned off = col_ok ? lev_off(k) : a.field_bytes;
+        // ---- the level's outputs.  A lane past the last column stores
+        // nothing: with MASKX its stores go through a buffer resource at an
+        // offset past the array's end, which drops them.  No store sits behind
+        // a branch, so the next level's wait for its prefetched loads never
+        // has to wait for these stores as well.
+        const unsigned off = lev_off(k);
+        auto put = [&](FT* p, FT v) {
+            if constexpr (MASKX) buf_store(make_rsrc(p, a.field_bytes), col_ok ? off : a.field_bytes, v);
+            else gstore(p, off, v);
+        };
         if constexpr (MODE == TILE_PAR) {
-            buf_store(rs(a.PAR), off, PARv);
+            put(a.PAR, PARv);
         } else if constexpr (MODE == TILE_TENDENCY) {
             FT g0[NT];
 #pragma unroll
             for (int n = 0; n < NT; n++)
-                if (n != RO) buf_load(rs(a.G[n]), off, g0[n]);
+                if (n != RO) g0[n] = gload(a.G[n], off);
 #pragma unroll
             for (int n = 0; n < NT; n++)
-                if (n != RO) buf_store(rs(a.G[n]), off, g0[n] + Gout[n]);
-            if (a.PAR) buf_store(rs(a.PAR), off, PARv);
+                if (n != RO) put(a.G[n], g0[n] + Gout[n]);
+            if (a.PAR) put(a.PAR, PARv);
         } else {
 #pragma unroll
             for (int n = 0; n < NT; n++) {
                 if (n == RO) continue;
-                unsigned o = off;
                 // the top-cell DIC is written by the surface kernel
-                if constexpr (CARB & 1) {
-                    if (n == 7 && top && a.co2_on) o = a.field_bytes;
+                if constexpr ((CARB & 1) != 0) {
+                    if (top && n == 7 && a.co2_on) continue;
                 }
-                buf_store(rs(a.tr[n]), o, F32 ? U[n] + D[F32 ? n : 0] : U[n]);
+                put(a.tr[n], F32 ? U[n] + D[F32 ? n : 0] : U[n]);
             }
             if constexpr ((EXTRA & 4) != 0) {
                 // the sediment after the last stage: storage, G⁻ = the tendency
                 // of its last step, Gⁿ = e f of the final state
-                if (k == 0 && col_ok) {
+                if (bot && col_ok) {
                     const FT x0 = F32 ? U[S0] + D[F32 ? S0 : 0] : U[S0];
                     const FT x1 = F32 ? U[S1] + D[F32 ? S1 : 0] : U[S1];
                     const FT f = a.sed_w0 * x0 + a.sed_w1 * x1;
@@ -544,6 +564,23 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
                 }
             }
         }
+    };
+    using T_ = std::integral_constant<bool, true>;
+    using F_ = std::integral_constant<bool, false>;
+    if constexpr (MODE == TILE_STEP && EXTRA == 0) {
+        if (Nz == 1) {
+            level(0, T_{}, T_{});
+        } else {
+            level(ktop, T_{}, F_{});
+            // (unrolled by two: the prefetched and the current level's
+            // registers alternate instead of being copied)
+#pragma unroll 2
+            for (int k = ktop - 1; k >= 1; k--) level(k, F_{}, F_{});
+            level(0, F_{}, T_{});
+        }
+    } else {
+#pragma unroll 1
+        for (int k = ktop; k >= 0; k--) level(k, k == ktop, k == 0);
     }
     if constexpr ((EXTRA & 2) && F32) {
         if (col_ok && a.Hprev) a.Hprev[col2d] = Hc;
@@ -559,7 +596,8 @@ struct SurfaceArgs {
     const FT* Alk;                 // Alk parent array (tendency mode)
     const FT* srec;                // surface records (step mode)
     double* Hprev;                 // [Nx*Ny] warm start of [H+] (mixed path) or null
-    const double* surfc;           // [SURF_PACK][Nx*Ny] surface constants of the call (step mode) or null
+    const double* surfc;           // [SURF_PACK][surfc_cols] surface constants of the call (step mode) or null
+    int surfc_cols;                // Nx*Ny, or 1 where T and S are the same in every column (profile, constant)
     const FT* T3;                  // 3-D temperature / salinity or null
     const FT* S3;
     const double* Tprof;
@@ -577,12 +615,13 @@ struct SurfaceArgs {
 };
 
 // The surface constants of every column (T and S of the top cell), once per
-// stepper call, for k_surface.
+// stepper call, for k_surface: a.surfc_cols columns (one where T and S do
+// not vary across the columns).
 template <typename FT>
 __global__ void __launch_bounds__(256) k_surface_prep(SurfaceArgs<FT> a, double* out)
 {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ncol = a.Nx * a.Ny;
+    const int ncol = a.surfc_cols;
     if (col >= ncol) return;
     const int i = col % a.Nx, j = col / a.Nx;
     const int ktop = a.Nz - 1;
@@ -610,8 +649,9 @@ __global__ void __launch_bounds__(256) k_surface(SurfaceArgs<FT> a)
     if (a.surfc) {
         // T and S are fixed within a stepper call: the constants were set up once
         double o[SURF_PACK], kO2, airO2;
+        const long cidx = (a.surfc_cols == 1) ? 0 : col;
 #pragma unroll
-        for (int q = 0; q < SURF_PACK; q++) o[q] = a.surfc[(long)q * ncol + col];
+        for (int q = 0; q < SURF_PACK; q++) o[q] = a.surfc[(long)q * a.surfc_cols + cidx];
         surface_unpack(o, sc, kO2, airO2);
     } else {
         const double T = a.T3 ? (double)a.T3[top] : (a.Tprof ? a.Tprof[ktop] : a.Tconst);
