@@ -11,6 +11,7 @@
 #include <cstdlib>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -181,6 +182,7 @@ __global__ void __launch_bounds__(256) k_co2_flux(const double* __restrict__ DIC
 
 struct obme_handle {
     obme_config cfg;
+    int device = 0;            // the device current at obme_create (its buffers live there)
     int ntracers;
     int prec;
     std::vector<double> zc, zf, dz, Tprof, Sprof;
@@ -426,6 +428,10 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
     }
     h->sink_idx[0] = s0;
     h->sink_idx[1] = s1;
+    if (hipGetDevice(&h->device) != hipSuccess) {
+        delete h;
+        return fail(OBME_ERR_HIP, "hipGetDevice failed");
+    }
     // field sources
     auto copy_prof = [&](const obme_field_source& f, std::vector<double>& v, double** d) -> obme_status {
         if (f.mode == OBME_FIELD_PROFILE) {
@@ -893,6 +899,115 @@ extern "C" obme_status obme_step(obme_handle* h, void* const* tracers, const voi
                                  double dt, double t0, void* stream)
 {
     return obme_step_with_sediment(h, tracers, T, S, nullptr, nsteps, dt, t0, stream);
+}
+
+// ------------------------------------------------------------------ allocation-free stepper
+
+// obme_step_noalloc keeps obme_step's per-call scratch (the surface records,
+// the [H+] warm starts, the surface constants and the O2 exchange
+// coefficients) in this static device array instead of buffers the library
+// allocates.  Each device the library runs on holds its own copy until the
+// process ends.  In host code `scratch` names a host-side stand-in: its
+// device address comes from hipGetSymbolAddress, once per device, and only
+// that address is ever handed to a kernel.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mutex;
+static void* g_scratch_addr[NOALLOC_MAX_DEVICES] = {};
+
+// p[0 .. n) = 0 (a kernel rather than hipMemsetAsync, which HIP graph
+// capture does not take reliably)
+__global__ void __launch_bounds__(256) k_zero_f64(double* p, long n)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = 0.0;
+}
+
+static size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// The device address of this device's scratch array (set up on the first
+// call on the device, with the device current).
+static obme_status scratch_address(int dev, unsigned char** out)
+{
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(OBME_ERR_UNSUPPORTED, "device index beyond 63");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(OBME_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_addr[dev] = p;
+    }
+    *out = (unsigned char*)g_scratch_addr[dev];
+    return OBME_OK;
+}
+
+extern "C" obme_status obme_step_noalloc(obme_handle* h, void* const* tracers, const void* T, const void* S,
+                                         int32_t nsteps, double dt, double t0, void* stream)
+{
+    if (!h || !tracers) return fail(OBME_ERR_INVALID, "null argument");
+    if (nsteps < 0) return fail(OBME_ERR_INVALID, "negative nsteps");
+    for (int n = 0; n < h->ntracers; n++)
+        if (!tracers[n]) return fail(OBME_ERR_INVALID, "null tracer pointer");
+    const obme_config& c = h->cfg;
+    const size_t ncol = (size_t)c.grid.Nx * c.grid.Ny;
+    const size_t es = (h->prec == OBME_FP64) ? 8 : 4;
+    const bool co2 = c.model == OBME_LOBSTER && c.carbonate_system && c.co2.enabled;
+    const bool o2 = c.model == OBME_LOBSTER && c.oxygen.enabled && c.o2.enabled;
+    // the regions of this call: surface records, warm starts, surface
+    // constants (one column when T and S come from the configuration), O2
+    const size_t b_srec = co2 ? round256(SURF_REC * ncol * es) : 0;
+    const size_t b_hprev = co2 ? round256(ncol * sizeof(double)) : 0;
+    const size_t b_surfc = co2 ? round256(SURF_PACK * ((!T && !S) ? 1 : ncol) * sizeof(double)) : 0;
+    const size_t b_o2c = o2 ? round256(2 * ncol * sizeof(double)) : 0;
+    const size_t need = b_srec + b_hprev + b_surfc + b_o2c;
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return fail(OBME_ERR_INVALID, "obme_step_noalloc: this grid needs " + std::to_string(need) +
+                                          " bytes of scratch; the static scratch holds " +
+                                          std::to_string(NOALLOC_SCRATCH_BYTES) + " (64 MiB)");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != h->device) return fail(OBME_ERR_INVALID, "obme_step_noalloc: not the device the handle was created on");
+    unsigned char* base = nullptr;
+    obme_status st = scratch_address(dev, &base);
+    if (st != OBME_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+
+    // run obme_step with the handle's scratch pointers on the static array
+    // (restored on every return), and without kernel timing, whose events
+    // the library would have to create
+    struct Swap {
+        obme_handle* h;
+        void* srec;
+        double *hprev, *surfc, *o2c;
+        bool timing;
+        ~Swap()
+        {
+            h->d_srec = srec;
+            h->d_Hprev = hprev;
+            h->d_surfc = surfc;
+            h->d_o2c = o2c;
+            h->timing = timing;
+        }
+    } keep{h, h->d_srec, h->d_Hprev, h->d_surfc, h->d_o2c, h->timing};
+    h->timing = false;
+    size_t off = 0;
+    if (co2) {
+        h->d_srec = base + off;
+        off += b_srec;
+        h->d_Hprev = (double*)(base + off);
+        off += b_hprev;
+        h->d_surfc = (double*)(base + off);
+        off += b_surfc;
+        // the array keeps the last call's contents: the warm starts begin at
+        // zero (the FP32-mixed solve starts from its FP32 guess)
+        hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, s, h->d_Hprev, (long)ncol);
+        HIP_TRY(hipGetLastError());
+    }
+    if (o2) h->d_o2c = (double*)(base + off);
+    return obme_step_with_sediment(h, tracers, T, S, nullptr, nsteps, dt, t0, s);
 }
 
 extern "C" obme_status obme_sediment_update_state(obme_handle* h, const void* const* tracers, double* storage,

@@ -48,6 +48,7 @@ def library():
         "obme_step": (C.c_int32, [vp, C.POINTER(vp), vp, vp, C.c_int32, C.c_double, C.c_double, vp]),
         "obme_step_with_sediment": (C.c_int32, [vp, C.POINTER(vp), vp, vp, vp, C.c_int32, C.c_double, C.c_double,
                                                 vp]),
+        "obme_step_noalloc": (C.c_int32, [vp, C.POINTER(vp), vp, vp, C.c_int32, C.c_double, C.c_double, vp]),
         "obme_sediment_update_state": (C.c_int32, [vp, C.POINTER(vp), vp, C.c_double, vp]),
         "obme_co2_flux": (C.c_int32, [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp, C.c_int64, C.c_int32, vp]),
         "obme_abi_sizeof": (C.c_int64, [C.c_int32]),
@@ -77,7 +78,7 @@ def library():
 
 EXPORTED_SYMBOLS = (
     "obme_create", "obme_destroy", "obme_last_error", "obme_version", "obme_required_tracers",
-    "obme_update_state", "obme_add_tendencies", "obme_step", "obme_step_with_sediment",
+    "obme_update_state", "obme_add_tendencies", "obme_step", "obme_step_with_sediment", "obme_step_noalloc",
     "obme_sediment_update_state", "obme_co2_flux", "obme_abi_sizeof",
     "obme_set_kernel_timing", "obme_kernel_timing", "obme_batch_create", "obme_batch_destroy", "obme_batch_step",
     "obme_batch_configure", "obme_batch_autotune", "obme_batch_describe", "obme_pisces_create",
@@ -183,6 +184,14 @@ class HIPBiogeochemistry:
             return
         _check(library().obme_step(self.handle, _ptr_array(tracers), _ptr(T), _ptr(S), int(nsteps), float(dt),
                                    float(t0), _stream()))
+
+    def step_noalloc(self, tracers, nsteps, dt, t0=0.0, T=None, S=None):
+        """step() through obme_step_noalloc: the same fused RK3 steps with the
+        per-call scratch in the library's static device array (no device
+        allocation in the call)."""
+        self._check_fields(tracers)
+        _check(library().obme_step_noalloc(self.handle, _ptr_array(tracers), _ptr(T), _ptr(S), int(nsteps),
+                                           float(dt), float(t0), _stream()))
 
     def update_sediment_state(self, tracers, dt):
         """update_biogeochemical_state!(model, sediment) (Sediments/update_state.jl:6-17):
