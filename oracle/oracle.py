@@ -46,7 +46,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         build()
-    L = C.CDLL(LIB_PATH)
+    _lib = load_library(LIB_PATH)
+    return _lib
+
+
+def load_library(path):
+    """An oracle library at `path` with its function signatures set (the
+    committed oracle, or a variant the tests compile from its sources)."""
+    L = C.CDLL(path)
     sig = {
         "orc_julia_min": (D, [D, D]),
         "orc_julia_max": (D, [D, D]),
@@ -101,7 +108,6 @@ def lib():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
-    _lib = L
     return L
 
 
