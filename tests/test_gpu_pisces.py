@@ -1,9 +1,10 @@
 """GPU parity of PISCES (obme_pisces_*) against the numpy FP64 oracle
 (oracle/pisces.py) on the same inputs: the update-state fields (MultiBand
 PAR, euphotic depth, mixed-layer PAR, calcite saturation), the tendencies
-and a few RK3 steps.  FP64 bar: relative L2 <= 1e-12 per tracer after the
-steps (1e-10 for the raw tendencies, where the free-iron root's cancellation
-amplifies one-ulp differences of exp between libraries); FP32-mixed: 1e-5."""
+and a few RK3 steps.  FP64 bar: relative L2 <= 1e-12 per tracer, for the
+raw tendencies and after the steps (the raw tendencies measured at most
+3.2e-15, Fe, whose free-iron root moves that much when exp(...) changes by
+one ulp); FP32-mixed: 1e-5."""
 import numpy as np
 import pytest
 
@@ -68,7 +69,7 @@ def test_tendencies_fp64():
     for n, name in enumerate(OP.WRITTEN):
         worst[name] = rel_l2(g.interior(Gd[n].cpu().numpy()), G[name])
     print("tendency errors", worst, flush=True)
-    assert all(e <= 1e-10 for e in worst.values()), worst
+    assert all(e <= 1e-12 for e in worst.values()), worst
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
