@@ -1119,6 +1119,7 @@ struct obme_batch {
     FastParams* d_fpar = nullptr;
     NPDParams<double>* d_dpar = nullptr;
     BatchColumn* d_bcol = nullptr;
+    float* d_comp = nullptr;              // FP32: the residual arrays, 4 x offsets[n_columns]
     int plan = -1;
     int steps_per_launch = 0;             // 0: every step of a call in one launch per class
     std::vector<int> class_tile, class_n; // per class of the plan
@@ -1248,8 +1249,15 @@ static obme_status batch_enqueue(const obme_batch* b, void* const* tracers, int 
 {
     BatchArgs<FT> a = batch_args<FT>(b, tracers, dt);
     const int chunk = (b->steps_per_launch > 0) ? b->steps_per_launch : nsteps;
+    const size_t ncell = (size_t)b->offsets[b->n_columns];
+    if (b->d_comp)
+        for (int n = 0; n < 4; n++) a.C[n] = (FT*)(b->d_comp + n * ncell);
     for (int done = 0; done < nsteps; done += chunk) {
         const int n = std::min(chunk, nsteps - done);
+        // FP32: the residuals start at zero with each call and pass from
+        // launch to launch within it
+        a.c_in = (done > 0 && b->d_comp) ? 1 : 0;
+        a.c_out = (done + n < nsteps && b->d_comp) ? 1 : 0;
         for (size_t q = 0; q < b->class_tile.size(); q++) {
             obme_status r = batch_launch_class<FT>(b, (int)q, a, n, s);
             if (r != OBME_OK) return r;
@@ -1274,6 +1282,7 @@ extern "C" obme_status obme_batch_destroy(obme_batch* b)
     if (b->d_fpar) (void)hipFree(b->d_fpar);
     if (b->d_dpar) (void)hipFree(b->d_dpar);
     if (b->d_bcol) (void)hipFree(b->d_bcol);
+    if (b->d_comp) (void)hipFree(b->d_comp);
     delete b;
     return OBME_OK;
 }
@@ -1367,6 +1376,8 @@ extern "C" obme_status obme_batch_create(const obme_config* cfg, int32_t n_colum
               up((void**)&b->d_bcol, bcol.data(), bcol.size() * sizeof(BatchColumn));
     if (ok && b->prec == OBME_FP64) ok = up((void**)&b->d_dpar, dpar.data(), dpar.size() * sizeof(NPDParams<double>));
     if (ok && b->prec != OBME_FP64) ok = up((void**)&b->d_fpar, fpar.data(), fpar.size() * sizeof(FastParams));
+    if (ok && b->prec != OBME_FP64)
+        ok = hipMalloc((void**)&b->d_comp, 4 * (size_t)b->offsets[n_columns] * sizeof(float)) == hipSuccess;
     if (!ok) {
         obme_batch_destroy(b);
         return fail(OBME_ERR_NOMEM, "batch buffers");

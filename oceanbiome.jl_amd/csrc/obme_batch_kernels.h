@@ -8,6 +8,15 @@
 // NutrientsPlanktonDetritus/*.jl; the RK3 substep, BoxModel/timesteppers.jl:79-92),
 // with the column's parameters in place of the model defaults.
 //
+// FP32-mixed: the state is U plus an increment D.  Within a step D gathers
+// the stage increments (stage inputs U + D); at the end of the step U + D is
+// split exactly (TwoSum) into its rounding s and the residual, U = s and D =
+// the residual, so no step's increment is lost to the rounding of U: the
+// state is carried to about twice FP32's precision for as long as it stays
+// in registers.  A call split over several launches hands the residual to
+// the next launch through a residual array; the stored state is rounded
+// once, at the end of the call.
+//
 // Layout: tracer n (N, P, Z, D) is one packed array; column c occupies
 // [offsets[c], offsets[c+1]) with k = 0 at the bottom.  Columns are packed
 // into wave segments of LPC = 64/CPW lanes, lane r of a segment holding the
@@ -31,6 +40,8 @@ struct BatchColumn {
 template <typename FT>
 struct BatchArgs {
     FT* U[4];                       // N, P, Z, D packed arrays
+    FT* C[4];                       // FP32: the residuals of U, packed alike (launch to launch)
+    int c_in, c_out;                // FP32: start from the stored residuals; store them at the end
     const int* offsets;             // [n_columns + 1]
     const int* cols;                // this class's column list
     int ncols;                      // entries of `cols`
@@ -75,7 +86,7 @@ __device__ __forceinline__ void batch_columns(const BatchArgs<FT>& a)
             for (int m = 0; m < L; m++) {
                 U[n][m] = has(m) ? a.U[n][idx(m)] : FT(0);
                 Gm[n][m] = FT(0);
-                if constexpr (F32) D[n][m] = FT(0);
+                if constexpr (F32) D[n][m] = (a.c_in && has(m)) ? a.C[n][idx(m)] : FT(0);
             }
         auto val = [&](int n, int m) -> FT {
             if constexpr (F32) return U[n][m] + D[n][m];
@@ -200,12 +211,16 @@ __device__ __forceinline__ void batch_columns(const BatchArgs<FT>& a)
                 }
             }
             if constexpr (F32) {
+                // U + D = s + e exactly (TwoSum): U = s, and e carries on
 #pragma unroll
                 for (int n = 0; n < 4; n++)
 #pragma unroll
                     for (int m = 0; m < L; m++) {
-                        U[n][m] = U[n][m] + D[n][m];
-                        D[n][m] = 0.0f;
+                        const float u = U[n][m], d = D[n][m];
+                        const float sum = u + d;
+                        const float dv = sum - u;
+                        D[n][m] = (u - (sum - dv)) + (d - dv);
+                        U[n][m] = sum;
                     }
             }
         }
@@ -213,7 +228,12 @@ __device__ __forceinline__ void batch_columns(const BatchArgs<FT>& a)
         for (int n = 0; n < 4; n++)
 #pragma unroll
             for (int m = 0; m < L; m++)
-                if (has(m)) a.U[n][idx(m)] = U[n][m];
+                if (has(m)) {
+                    a.U[n][idx(m)] = U[n][m];
+                    if constexpr (F32) {
+                        if (a.c_out) a.C[n][idx(m)] = D[n][m];
+                    }
+                }
     }
 }
 
