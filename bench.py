@@ -90,6 +90,36 @@ def time_steps(dev, tr, steps, dt, warmup):
     return secs, wall, sweep_ms / max(n, 1)
 
 
+def surface_errors(wl, cfg, out, ref, ci, cj, O):
+    """pH (free scale) and the CO2 flux of the sampled columns' surface cells
+    after the check run: the GPU's point kernel on the GPU state against the
+    FP64 oracle on the oracle's state (carbon_chemistry.jl:168,
+    gas_exchange.jl:26-38)."""
+    import torch
+    from oceanbiome.runtime import HIPBiogeochemistry
+    g = wl.grid
+    k = g.Nz - 1 + g.Hz
+    top = lambda arr: np.ascontiguousarray(arr[k, cj + g.Hy, ci + g.Hx], dtype=np.float64)
+    T = np.full(len(ci), float(wl.extra["T_profile"][-1]))
+    S = np.full(len(ci), float(wl.extra["S"]))
+    u10 = np.full(len(ci), cfg.co2.wind_speed)
+    pH_ref, F_ref = O.co2_flux_batch(cfg.co2, cfg.solver, top(ref[7]), top(ref[8]), T, S, u10,
+                                     cfg.co2.air_concentration)
+    dev = HIPBiogeochemistry(wl.bgc, "fp64")
+    cuda = lambda x: torch.from_numpy(x).cuda()
+    pH, F = dev.co2_flux(cuda(top(out[7])), cuda(top(out[8])), cuda(T), cuda(S), cuda(u10), mode=0)
+    pH, F = pH.cpu().numpy(), F.cpu().numpy()
+    # the floor FP32 storage sets: the oracle's own state rounded to FP32
+    r32 = lambda x: x.astype(np.float32).astype(np.float64)
+    pH_fl, _ = O.co2_flux_batch(cfg.co2, cfg.solver, r32(top(ref[7])), r32(top(ref[8])), T, S, u10,
+                                cfg.co2.air_concentration)
+    return {"pH (surface)": {"max_abs": float(np.max(np.abs(pH - pH_ref))),
+                             "rms": float(np.sqrt(np.mean((pH - pH_ref) ** 2))),
+                             "fp32_storage_floor_max_abs": float(np.max(np.abs(pH_fl - pH_ref)))},
+            "CO2 flux (surface)": {"rel_l2": float(np.linalg.norm(F - F_ref) / np.linalg.norm(F_ref)),
+                                   "max_rel": float(np.max(np.abs(F - F_ref) / np.abs(F_ref)))}}
+
+
 def cpu_threads(req):
     if req:
         return req
@@ -187,8 +217,9 @@ def main():
                     b = ref[n][ks[:, None], (cj + g.Hy)[None, :], (ci + g.Hx)[None, :]]
                     e[name] = {"rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
                                "max_rel": float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))}
+                e.update(surface_errors(wl, cfg, out, ref, ci, cj, O))
                 errors[prec] = e
-                worst = max(v["rel_l2"] for v in e.values())
+                worst = max(v["rel_l2"] for v in e.values() if "rel_l2" in v)
                 log(f"{prec} vs oracle after {args.check_steps} steps ({ns} columns): worst rel L2 {worst:.3e}")
         if not args.no_cpu:
             # FP64 oracle on every column of the same workload, OpenMP over

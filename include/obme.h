@@ -318,12 +318,13 @@ obme_status obme_step(obme_handle* h, void* const* tracers,
 /* No counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc).  obme_step with its per-call scratch (surface records,
- * [H+] warm starts, surface constants, O2 exchange coefficients) in a static
+ * [H+] warm starts, surface constants, O2 exchange coefficients, top-cell
+ * DIC and Alk residuals) in a static
  * device array of 64 MiB per device instead of buffers the library
  * allocates; it calls no hipMalloc or hipFree (the handle itself comes from
  * obme_create).  A grid whose scratch exceeds 64 MiB is rejected with
  * OBME_ERR_INVALID (FP32-mixed LOBSTER + DIC/Alk with CO2 exchange takes
- * 36 B per column, about 1.8 million columns).  The warm starts begin at
+ * 44 B per column, about 1.5 million columns).  The warm starts begin at
  * zero on every call, so FP32-mixed results differ from obme_step's within
  * the root solve's tolerance; FP64 results are the same.  Every call on a
  * device shares that device's array: calls that overlap on one device race.

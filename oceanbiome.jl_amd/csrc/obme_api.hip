@@ -194,6 +194,7 @@ struct obme_handle {
     double* d_o2c = nullptr;   // 2*Nx*Ny: k / Δz_top and C_air α of the O2 exchange
     bool o2c_ready = false;    // d_o2c holds the current obme_step call's coefficients
     double* d_Hprev = nullptr; // Nx*Ny warm starts of [H+] (mixed path)
+    float* d_topres = nullptr; // 2*Nx*Ny residuals of the top-cell DIC and Alk (mixed path, CO2 exchange)
     double* d_surfc = nullptr; // [SURF_PACK][surfc_cols] surface constants of the current stepper call
     int surfc_cols = 0;        // Nx*Ny, or 1 when the call's T and S are the same in every column
     bool surfc_ready = false;  // d_surfc holds the constants of this call's T and S
@@ -467,7 +468,8 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
         size_t es = (h->prec == OBME_FP64) ? 8 : 4;
         hipError_t e1 = hipMalloc(&h->d_srec, SURF_REC * ncol * es);
         hipError_t e2 = hipMalloc((void**)&h->d_Hprev, ncol * sizeof(double));
-        if (e1 != hipSuccess || e2 != hipSuccess) {
+        hipError_t e3 = (h->prec == OBME_FP64) ? hipSuccess : hipMalloc((void**)&h->d_topres, 2 * ncol * sizeof(float));
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
             obme_destroy(h);
             return fail(OBME_ERR_NOMEM, "surface buffers");
         }
@@ -501,6 +503,7 @@ extern "C" obme_status obme_destroy(obme_handle* h)
     if (h->d_Sprof) (void)hipFree(h->d_Sprof);
     if (h->d_srec) (void)hipFree(h->d_srec);
     if (h->d_Hprev) (void)hipFree(h->d_Hprev);
+    if (h->d_topres) (void)hipFree(h->d_topres);
     if (h->d_o2c) (void)hipFree(h->d_o2c);
     if (h->d_sedG) (void)hipFree(h->d_sedG);
     if (h->d_surfc) (void)hipFree(h->d_surfc);
@@ -766,6 +769,9 @@ static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* 
     }
     a.T3 = (const FT*)T;
     a.S3 = (const FT*)S;
+    // the top-cell DIC and Alk residuals (FP32 fused step, CO2 by the surface kernel)
+    float* topres = (MODE == TILE_STEP && a.co2_on) ? h->d_topres : nullptr;
+    a.topres = topres;
     if (MODE != TILE_PAR && a.o2_on && !h->o2c_ready) {
         // the O2 exchange coefficients of this call's surface T and S (once
         // per obme_step call, which sets o2c_ready around its steps)
@@ -819,6 +825,7 @@ static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* 
         sa.surfc = h->d_surfc;
         sa.surfc_cols = h->surfc_cols;
     }
+    sa.topres = topres;
     sa.dtf = (FT)dt;
     rk3_products(dt, sa.rk_a, sa.rk_b);
     if (MODE == TILE_TENDENCY) sa.GDIC = (FT*)G[7];
@@ -850,6 +857,14 @@ extern "C" obme_status obme_add_tendencies(obme_handle* h, const void* const* tr
     for (int n = 0; n < h->ntracers; n++)
         if (!tracers[n] || !G[n]) return fail(OBME_ERR_INVALID, "null tracer or tendency pointer");
     return run_tile<TILE_TENDENCY>(h, (void* const*)tracers, G, (void*)PAR, T, S, t, 0.0, (hipStream_t)stream);
+}
+
+// p[0 .. n) = 0 (a kernel rather than hipMemsetAsync, which HIP graph
+// capture does not take reliably)
+__global__ void __launch_bounds__(256) k_zero_f64(double* p, long n)
+{
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) p[q] = 0.0;
 }
 
 extern "C" obme_status obme_step_with_sediment(obme_handle* h, void* const* tracers, const void* T, const void* S,
@@ -886,6 +901,14 @@ extern "C" obme_status obme_step_with_sediment(obme_handle* h, void* const* trac
         HIP_TRY(hipGetLastError());
         h->surfc_ready = true;
     }
+    if (nsteps > 0 && h->d_topres && c.model == OBME_LOBSTER && c.carbonate_system && c.co2.enabled &&
+        !h->co2_inline) {
+        // the top-cell DIC and Alk residuals start at zero in every call
+        const size_t ncol = (size_t)c.grid.Nx * c.grid.Ny;
+        hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, s, (double*)h->d_topres,
+                           (long)ncol);
+        HIP_TRY(hipGetLastError());
+    }
     obme_status r = OBME_OK;
     h->o2c_ready = false;
     for (int st = 0; st < nsteps && r == OBME_OK; st++)
@@ -904,8 +927,8 @@ extern "C" obme_status obme_step(obme_handle* h, void* const* tracers, const voi
 // ------------------------------------------------------------------ allocation-free stepper
 
 // obme_step_noalloc keeps obme_step's per-call scratch (the surface records,
-// the [H+] warm starts, the surface constants and the O2 exchange
-// coefficients) in this static device array instead of buffers the library
+// the [H+] warm starts, the surface constants, the O2 exchange coefficients
+// and the top-cell residuals) in this static device array instead of buffers the library
 // allocates.  Each device the library runs on holds its own copy until the
 // process ends.  In host code `scratch` names a host-side stand-in: its
 // device address comes from hipGetSymbolAddress, once per device, and only
@@ -916,14 +939,6 @@ static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 constexpr int NOALLOC_MAX_DEVICES = 64;
 static std::mutex g_scratch_mutex;
 static void* g_scratch_addr[NOALLOC_MAX_DEVICES] = {};
-
-// p[0 .. n) = 0 (a kernel rather than hipMemsetAsync, which HIP graph
-// capture does not take reliably)
-__global__ void __launch_bounds__(256) k_zero_f64(double* p, long n)
-{
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < n) p[q] = 0.0;
-}
 
 static size_t round256(size_t x) { return (x + 255) & ~(size_t)255; }
 
@@ -962,7 +977,8 @@ extern "C" obme_status obme_step_noalloc(obme_handle* h, void* const* tracers, c
     const size_t b_hprev = co2 ? round256(ncol * sizeof(double)) : 0;
     const size_t b_surfc = co2 ? round256(SURF_PACK * ((!T && !S) ? 1 : ncol) * sizeof(double)) : 0;
     const size_t b_o2c = o2 ? round256(2 * ncol * sizeof(double)) : 0;
-    const size_t need = b_srec + b_hprev + b_surfc + b_o2c;
+    const size_t b_topres = (co2 && h->prec != OBME_FP64) ? round256(2 * ncol * sizeof(float)) : 0;
+    const size_t need = b_srec + b_hprev + b_surfc + b_o2c + b_topres;
     if (need > NOALLOC_SCRATCH_BYTES)
         return fail(OBME_ERR_INVALID, "obme_step_noalloc: this grid needs " + std::to_string(need) +
                                           " bytes of scratch; the static scratch holds " +
@@ -982,6 +998,7 @@ extern "C" obme_status obme_step_noalloc(obme_handle* h, void* const* tracers, c
         obme_handle* h;
         void* srec;
         double *hprev, *surfc, *o2c;
+        float* topres;
         bool timing;
         ~Swap()
         {
@@ -989,9 +1006,10 @@ extern "C" obme_status obme_step_noalloc(obme_handle* h, void* const* tracers, c
             h->d_Hprev = hprev;
             h->d_surfc = surfc;
             h->d_o2c = o2c;
+            h->d_topres = topres;
             h->timing = timing;
         }
-    } keep{h, h->d_srec, h->d_Hprev, h->d_surfc, h->d_o2c, h->timing};
+    } keep{h, h->d_srec, h->d_Hprev, h->d_surfc, h->d_o2c, h->d_topres, h->timing};
     h->timing = false;
     size_t off = 0;
     if (co2) {
@@ -1006,7 +1024,12 @@ extern "C" obme_status obme_step_noalloc(obme_handle* h, void* const* tracers, c
         hipLaunchKernelGGL(k_zero_f64, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, s, h->d_Hprev, (long)ncol);
         HIP_TRY(hipGetLastError());
     }
-    if (o2) h->d_o2c = (double*)(base + off);
+    if (o2) {
+        h->d_o2c = (double*)(base + off);
+        off += b_o2c;
+    }
+    // (obme_step_with_sediment clears the residuals)
+    h->d_topres = b_topres ? (float*)(base + off) : nullptr;
     return obme_step_with_sediment(h, tracers, T, S, nullptr, nsteps, dt, t0, s);
 }
 

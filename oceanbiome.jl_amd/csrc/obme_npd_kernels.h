@@ -103,6 +103,10 @@ struct StepArgs {
     const FT* wsink;               // [MAX_SINK][Nz+1] face velocities (<= 0)
     const FT* surf;                // surface PAR array (Nx*Ny) or null
     FT* srec;                      // surface records [SURF_REC][Nx*Ny] (CO2 on)
+    // FP32 fused step with the CO2 exchange: the TwoSum residuals of the
+    // top-cell DIC [0, Nx*Ny) (surface kernel) and Alk [Nx*Ny, 2 Nx*Ny)
+    // (sweep), carried from step to step within a call, or null
+    float* topres;
     int Nx, Ny, Nz;
     unsigned sx, sxy, base0;       // strides and interior origin offset
     unsigned field_bytes;          // byte size of one parent array
@@ -299,6 +303,10 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
         }
         if constexpr ((CARB & 1) && MODE == TILE_STEP) {
             if (top && col_ok && a.co2_on) a.srec[col2d] = U[7];
+            // the top-cell Alk carries its residual from the previous step
+            if constexpr (F32) {
+                if (top && a.topres) D[8] = a.topres[ncol2d + col2d];
+            }
         }
 
         FT PARv = FT(0);
@@ -346,7 +354,11 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
             // stage values of this level
             FT X[NT];
 #pragma unroll
-            for (int n = 0; n < NT; n++) X[n] = (F32 && st > 0) ? U[n] + D[F32 ? n : 0] : U[n];
+            for (int n = 0; n < NT; n++) {
+                // (the top-cell Alk includes its residual from the first stage on)
+                const bool inc = F32 && (st > 0 || ((CARB & 1) && MODE == TILE_STEP && top && n == 8));
+                X[n] = inc ? U[n] + D[F32 ? n : 0] : U[n];
+            }
 
             // ---- TwoBand PAR (Light/2band.jl:18-32): the chlorophyll
             // integral from the surface, ∫(k) = ∫(k+1) + up(k) p(k+1) + dn(k) p(k)
@@ -505,7 +517,7 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
                 for (int n = 0; n < NT; n++) {
                     if (n == RO) continue;
                     if constexpr (F32) {
-                        if (st == 0) D[n] = a.rk_a[0] * Gv[n];
+                        if (st == 0) D[n] = fmaf(a.rk_a[0], Gv[n], D[n]);
                         else D[n] = fmaf(a.rk_a[st], Gv[n], fmaf(a.rk_b[st], Gm[n], D[n]));
                     } else {
                         const double gs = (st == 1) ? g2 : g3;
@@ -546,6 +558,15 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
                 // the top-cell DIC is written by the surface kernel
                 if constexpr ((CARB & 1) != 0) {
                     if (top && n == 7 && a.co2_on) continue;
+                    // the top-cell Alk: U + D = s + e exactly (TwoSum), e to the next step
+                    if constexpr (F32) {
+                        if (top && n == 8 && a.topres) {
+                            const float u = U[8], d = D[8], sum = u + d, dv = sum - u;
+                            if (col_ok) a.topres[ncol2d + col2d] = (u - (sum - dv)) + (d - dv);
+                            put(a.tr[8], sum);
+                            continue;
+                        }
+                    }
                 }
                 put(a.tr[n], F32 ? U[n] + D[F32 ? n : 0] : U[n]);
             }
@@ -595,6 +616,7 @@ struct SurfaceArgs {
     FT* GDIC;                      // DIC tendency parent array (tendency mode)
     const FT* Alk;                 // Alk parent array (tendency mode)
     const FT* srec;                // surface records (step mode)
+    float* topres;                 // FP32 step mode: the top-cell DIC residuals [Nx*Ny], or null
     double* Hprev;                 // [Nx*Ny] warm start of [H+] (mixed path) or null
     const double* surfc;           // [SURF_PACK][surfc_cols] surface constants of the call (step mode) or null
     int surfc_cols;                // Nx*Ny, or 1 where T and S are the same in every column (profile, constant)
@@ -670,18 +692,23 @@ __global__ void __launch_bounds__(256) k_surface(SurfaceArgs<FT> a)
         FT U = a.srec[col];
         FT Gm = FT(0);
         if constexpr (sizeof(FT) == 4) {
-            float D = 0.0f;
+            // the increment starts from the residual the previous step left
+            // (DIC = U + D carried to about twice FP32's precision in a call)
+            float D = a.topres ? a.topres[col] : 0.0f;
 #pragma unroll
             for (int st = 0; st < 3; st++) {
                 const float Gb = a.srec[(1 + st) * ncol + col];
                 const float Alk = a.srec[(4 + st) * ncol + col];
-                F = surface_flux_warm(sc, a.solver, (double)(U + D), (double)Alk, a.co2.air_concentration, a.mode,
-                                      H, pH, ev, FP32_NEWTON_TOL);
+                F = surface_flux_warm(sc, a.solver, (double)U + (double)D, (double)Alk, a.co2.air_concentration,
+                                      a.mode, H, pH, ev, FP32_NEWTON_TOL);
                 const float G = Gb - (float)F * a.invdz_top;
                 D = fmaf(a.rk_a[st], G, fmaf(a.rk_b[st], Gm, D));
                 Gm = G;
             }
-            U = U + D;
+            // U + D = s + e exactly (TwoSum): s stored, e to the next step
+            const float sum = U + D, dv = sum - U;
+            if (a.topres) a.topres[col] = (U - (sum - dv)) + (D - dv);
+            U = sum;
         } else {
             const FT g1 = FT(8.0 / 15.0), g2 = FT(5.0 / 12.0), g3 = FT(3.0 / 4.0);
             const FT z2 = FT(-17.0 / 60.0), z3 = FT(-5.0 / 12.0);

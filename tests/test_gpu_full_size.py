@@ -6,7 +6,9 @@ written value stays finite.  LOBSTER + DIC/Alk with the CO2 exchange on
 512x512x64 (the headline workload) in both precisions, the same with the
 O2 tracer and its gas exchange in FP32-mixed, and PISCES on
 256x256x64 in FP32-mixed.  Bars as everywhere: FP64 <= 1e-12 relative L2
-per tracer, FP32-mixed <= 1e-5."""
+per tracer, FP32-mixed <= 1e-5; the surface pH of the sampled columns
+within 1e-6 (FP32-mixed) and 1e-12 (FP64) of the oracle's, and their CO2
+flux within 1e-5 and 1e-10 relative L2."""
 import numpy as np
 import pytest
 
@@ -17,6 +19,8 @@ from oceanbiome.runtime import HIPBiogeochemistry, HIPPISCES
 
 pytestmark = pytest.mark.gpu
 TOL = {"fp64": 1e-12, "fp32": 1e-5}
+PH_TOL = {"fp64": 1e-12, "fp32": 1e-6}
+FLUX_TOL = {"fp64": 1e-10, "fp32": 1e-5}
 
 
 def _sample(g, n, seed):
@@ -45,9 +49,11 @@ def test_lobster_512x512x64_column_sample(precision, oxygen):
     torch.cuda.synchronize()
     names = wl.bgc.required_biogeochemical_tracers()
     errs = {}
+    top = {}
     for n, name in enumerate(names):
         out = tr[n].double().cpu().numpy()
         a, b = _columns(g, out, ci, cj), _columns(g, ref[n], ci, cj)
+        top[n] = (a[-1].copy(), b[-1].copy())
         errs[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
         assert np.isfinite(g.interior(out)).all(), name
         # halos untouched (the bottom halo plane and the first halo row)
@@ -56,6 +62,18 @@ def test_lobster_512x512x64_column_sample(precision, oxygen):
         del out
     print("errors", errs, flush=True)
     assert all(e <= TOL[precision] for e in errs.values()), errs
+    # the surface pH and CO2 flux (carbon_chemistry.jl:168, gas_exchange.jl:26-38)
+    # of the GPU state and of the oracle's, both by the oracle's point function
+    cfg = wl.bgc.config(abi.OBME_FP64)
+    T = np.full(len(ci), float(wl.extra["T_profile"][-1]))
+    S = np.full(len(ci), float(wl.extra["S"]))
+    u10 = np.full(len(ci), cfg.co2.wind_speed)
+    pH_g, F_g = O.co2_flux_batch(cfg.co2, cfg.solver, top[7][0], top[8][0], T, S, u10, cfg.co2.air_concentration)
+    pH_r, F_r = O.co2_flux_batch(cfg.co2, cfg.solver, top[7][1], top[8][1], T, S, u10, cfg.co2.air_concentration)
+    dpH = float(np.max(np.abs(pH_g - pH_r)))
+    dF = float(np.linalg.norm(F_g - F_r) / np.linalg.norm(F_r))
+    print("surface pH max abs", dpH, "CO2 flux rel L2", dF, flush=True)
+    assert dpH <= PH_TOL[precision] and dF <= FLUX_TOL[precision], (dpH, dF)
 
 
 def test_pisces_256x256x64_column_sample_fp32():

@@ -72,7 +72,7 @@ def test_noalloc_oxygen_and_3d_temperature(prec):
 
 
 def test_noalloc_rejects_a_grid_beyond_the_scratch():
-    """2048 x 2048 columns in FP32 need 36 B x 4 Mi > 64 MiB of scratch."""
+    """2048 x 2048 columns in FP32 need 44 B x 4 Mi > 64 MiB of scratch."""
     import torch
     print("case noalloc oversize", flush=True)
     wl = workloads.lobster_3d(2048, 2048, 1, nsteps=1)
