@@ -19,7 +19,11 @@
  *  - Element type of tracer / tendency / PAR arrays: double for OBME_FP64,
  *    float for OBME_FP32_MIXED.  Carbonate-chemistry arrays are always double.
  *  - All calls are asynchronous on the given stream (NULL = default stream),
- *    one handle per host thread; a handle is not re-entrant.
+ *    one handle per host thread; a handle is not re-entrant.  One stream per
+ *    handle: the calls on a handle share its scratch buffers (surface
+ *    records, warm starts, exchange coefficients, residuals), so calls on a
+ *    handle made on different streams must be ordered by the caller (an
+ *    event, or a stream synchronisation) or they race.
  *  - Numerical failures are not errors: NaN propagates (min/max propagate NaN
  *    as Julia's do; a non-integer power of a negative value gives NaN).
  */
@@ -331,6 +335,23 @@ obme_status obme_step(obme_handle* h, void* const* tracers,
  * Must be called on the device the handle was created on. */
 obme_status obme_step_noalloc(obme_handle* h, void* const* tracers, const void* T, const void* S,
                               int32_t nsteps, double dt, double t0, void* stream);
+
+/* No counterpart in the reference: the launch shape of the column sweep.
+ * waves = waves per workgroup (1, 2, 4 or 8; a workgroup covers 64*waves
+ * consecutive columns of one row).  Results do not depend on it, bit for
+ * bit.  The default is 4, or the environment's OBME_SWEEP_WAVES. */
+obme_status obme_sweep_configure(obme_handle* h, int32_t waves);
+/* The current waves per workgroup, or < 0 for a null handle. */
+int32_t obme_sweep_waves(const obme_handle* h);
+/* No counterpart in the reference: the sweep's autotuner.  Times obme_step
+ * (one warm-up step, then trial_steps steps, best of two rounds) with every
+ * launch shape on a scratch copy of `tracers`, which are only read (T and S
+ * as in obme_step), and keeps the fastest.  The handle's carried state (the
+ * sediment stepper's history, the [H+] warm starts, the kernel-timing
+ * records) is left as it was.  Allocates the scratch copy; synchronises the
+ * stream. */
+obme_status obme_autotune(obme_handle* h, void* const* tracers, const void* T, const void* S,
+                          int32_t trial_steps, double dt, void* stream);
 
 /* ---- sediment (BiogeochemicalSediment, src/Sediments) -------------------
  * The sediment's state is its storage field (device, Nx*Ny doubles, i

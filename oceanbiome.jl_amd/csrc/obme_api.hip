@@ -1033,6 +1033,109 @@ extern "C" obme_status obme_step_noalloc(obme_handle* h, void* const* tracers, c
     return obme_step_with_sediment(h, tracers, T, S, nullptr, nsteps, dt, t0, s);
 }
 
+// ------------------------------------------------------------------ sweep launch shape
+
+extern "C" obme_status obme_sweep_configure(obme_handle* h, int32_t waves)
+{
+    if (!h) return fail(OBME_ERR_INVALID, "null handle");
+    if (!valid_sweep_waves(waves)) return fail(OBME_ERR_INVALID, "waves per workgroup must be 1, 2, 4 or 8");
+    h->sweep_waves = waves;
+    return OBME_OK;
+}
+
+extern "C" int32_t obme_sweep_waves(const obme_handle* h) { return h ? h->sweep_waves : -1; }
+
+// Times obme_step with every launch shape on a scratch copy of the state and
+// keeps the fastest.  The buffers a step carries forward in the handle (the
+// sediment stepper's G, the [H+] warm starts) are swapped for scratch ones
+// during the trials, so the next real step sees them as they were.
+extern "C" obme_status obme_autotune(obme_handle* h, void* const* tracers, const void* T, const void* S,
+                                     int32_t trial_steps, double dt, void* stream)
+{
+    if (!h || !tracers) return fail(OBME_ERR_INVALID, "null argument");
+    if (trial_steps < 1) return fail(OBME_ERR_INVALID, "trial_steps must be positive");
+    if (!std::isfinite(dt)) return fail(OBME_ERR_INVALID, "non-finite time step");
+    for (int n = 0; n < h->ntracers; n++)
+        if (!tracers[n]) return fail(OBME_ERR_INVALID, "null tracer pointer");
+    hipStream_t s = (hipStream_t)stream;
+    const obme_grid& g = h->cfg.grid;
+    const size_t es = (h->prec == OBME_FP64) ? 8 : 4;
+    const size_t bytes = (size_t)(g.Nx + 2 * g.Hx) * (g.Ny + 2 * g.Hy) * (g.Nz + 2 * g.Hz) * es;
+    const size_t ncol = (size_t)g.Nx * g.Ny;
+
+    struct Keep {
+        obme_handle* h;
+        double *hprev, *sedG;
+        long long sed_iter;
+        double sed_last_dt;
+        bool timing;
+        int waves;
+        std::vector<void*> owned;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Keep()
+        {
+            h->d_Hprev = hprev;
+            h->d_sedG = sedG;
+            h->sed_iter = sed_iter;
+            h->sed_last_dt = sed_last_dt;
+            h->timing = timing;
+            h->sweep_waves = waves;
+            for (void* p : owned) (void)hipFree(p);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } keep{h, h->d_Hprev, h->d_sedG, h->sed_iter, h->sed_last_dt, h->timing, h->sweep_waves, {}};
+    h->timing = false;
+    auto alloc = [&](size_t b, void** p) -> bool {
+        if (hipMalloc(p, b) != hipSuccess) return false;
+        keep.owned.push_back(*p);
+        return true;
+    };
+    std::vector<void*> scratch(h->ntracers, nullptr);
+    for (int n = 0; n < h->ntracers; n++)
+        if (!alloc(bytes, &scratch[n])) return fail(OBME_ERR_NOMEM, "autotune scratch state");
+    if (h->d_Hprev) {
+        void* p = nullptr;
+        if (!alloc(ncol * sizeof(double), &p)) return fail(OBME_ERR_NOMEM, "autotune scratch warm starts");
+        HIP_TRY(hipMemcpyAsync(p, keep.hprev, ncol * sizeof(double), hipMemcpyDeviceToDevice, s));
+        h->d_Hprev = (double*)p;
+    }
+    if (h->d_sedG) {
+        void* p = nullptr;
+        if (!alloc(2 * ncol * sizeof(double), &p)) return fail(OBME_ERR_NOMEM, "autotune scratch sediment");
+        HIP_TRY(hipMemcpyAsync(p, keep.sedG, 2 * ncol * sizeof(double), hipMemcpyDeviceToDevice, s));
+        h->d_sedG = (double*)p;
+    }
+    if (hipEventCreate(&keep.e0) != hipSuccess || hipEventCreate(&keep.e1) != hipSuccess)
+        return fail(OBME_ERR_HIP, "autotune events");
+
+    float best_ms[N_SWEEP_WAVES];
+    for (int q = 0; q < N_SWEEP_WAVES; q++) best_ms[q] = 1e30f;
+    for (int round = 0; round < 2; round++)
+        for (int q = 0; q < N_SWEEP_WAVES; q++) {
+            h->sweep_waves = SWEEP_WAVES[q];
+            for (int n = 0; n < h->ntracers; n++)
+                HIP_TRY(hipMemcpyAsync(scratch[n], tracers[n], bytes, hipMemcpyDeviceToDevice, s));
+            obme_status st = obme_step_with_sediment(h, scratch.data(), T, S, nullptr, 1, dt, 0.0, s);
+            if (st != OBME_OK) return st;
+            HIP_TRY(hipEventRecord(keep.e0, s));
+            st = obme_step_with_sediment(h, scratch.data(), T, S, nullptr, trial_steps, dt, dt, s);
+            if (st != OBME_OK) return st;
+            HIP_TRY(hipEventRecord(keep.e1, s));
+            HIP_TRY(hipEventSynchronize(keep.e1));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, keep.e0, keep.e1));
+            best_ms[q] = std::min(best_ms[q], ms);
+        }
+    int best = 0;
+    for (int q = 1; q < N_SWEEP_WAVES; q++)
+        if (best_ms[q] < best_ms[best]) best = q;
+    keep.waves = SWEEP_WAVES[best];
+    // the scratch buffers are freed once the trials have finished
+    HIP_TRY(hipStreamSynchronize(s));
+    return OBME_OK;
+}
+
 extern "C" obme_status obme_sediment_update_state(obme_handle* h, const void* const* tracers, double* storage,
                                                   double dt, void* stream)
 {

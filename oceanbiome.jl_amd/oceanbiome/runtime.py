@@ -49,6 +49,9 @@ def library():
         "obme_step_with_sediment": (C.c_int32, [vp, C.POINTER(vp), vp, vp, vp, C.c_int32, C.c_double, C.c_double,
                                                 vp]),
         "obme_step_noalloc": (C.c_int32, [vp, C.POINTER(vp), vp, vp, C.c_int32, C.c_double, C.c_double, vp]),
+        "obme_sweep_configure": (C.c_int32, [vp, C.c_int32]),
+        "obme_sweep_waves": (C.c_int32, [vp]),
+        "obme_autotune": (C.c_int32, [vp, C.POINTER(vp), vp, vp, C.c_int32, C.c_double, vp]),
         "obme_sediment_update_state": (C.c_int32, [vp, C.POINTER(vp), vp, C.c_double, vp]),
         "obme_co2_flux": (C.c_int32, [vp, vp, vp, vp, vp, vp, C.c_double, vp, vp, C.c_int64, C.c_int32, vp]),
         "obme_abi_sizeof": (C.c_int64, [C.c_int32]),
@@ -79,6 +82,7 @@ def library():
 EXPORTED_SYMBOLS = (
     "obme_create", "obme_destroy", "obme_last_error", "obme_version", "obme_required_tracers",
     "obme_update_state", "obme_add_tendencies", "obme_step", "obme_step_with_sediment", "obme_step_noalloc",
+    "obme_sweep_configure", "obme_sweep_waves", "obme_autotune",
     "obme_sediment_update_state", "obme_co2_flux", "obme_abi_sizeof",
     "obme_set_kernel_timing", "obme_kernel_timing", "obme_batch_create", "obme_batch_destroy", "obme_batch_step",
     "obme_batch_configure", "obme_batch_autotune", "obme_batch_describe", "obme_pisces_create",
@@ -192,6 +196,22 @@ class HIPBiogeochemistry:
         self._check_fields(tracers)
         _check(library().obme_step_noalloc(self.handle, _ptr_array(tracers), _ptr(T), _ptr(S), int(nsteps),
                                            float(dt), float(t0), _stream()))
+
+    def configure_sweep(self, waves):
+        """Waves per workgroup of the column sweep (1, 2, 4 or 8)."""
+        _check(library().obme_sweep_configure(self.handle, int(waves)))
+
+    @property
+    def sweep_waves(self):
+        return int(library().obme_sweep_waves(self.handle))
+
+    def autotune(self, tracers, trial_steps, dt, T=None, S=None):
+        """Times step() with every sweep launch shape on a scratch copy of
+        `tracers` (which stay unchanged) and keeps the fastest; returns it."""
+        self._check_fields(tracers)
+        _check(library().obme_autotune(self.handle, _ptr_array(tracers), _ptr(T), _ptr(S), int(trial_steps),
+                                       float(dt), _stream()))
+        return self.sweep_waves
 
     def update_sediment_state(self, tracers, dt):
         """update_biogeochemical_state!(model, sediment) (Sediments/update_state.jl:6-17):

@@ -101,21 +101,22 @@ def test_oxygen_gas_exchange(precision, carbonate, co2, scale, Nz):
     assert all(e <= TOL[precision] for e in errs.values()), errs
 
 
-def test_add_tendencies_with_oxygen_gas_exchange():
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_add_tendencies_with_oxygen_gas_exchange(precision):
     import torch
-    print("case oxygen gas exchange tendencies", flush=True)
+    print("case oxygen gas exchange tendencies", precision, flush=True)
     bgc, grid, fields = _model((7, 3, 18), True, o2x=True, seed=17)
     cfg = bgc.config(abi.OBME_FP64)
     G_ref = [np.zeros_like(a) for a in fields]
     O.grid_tendencies(cfg, fields, G_ref)
-    dev = HIPBiogeochemistry(bgc, "fp64")
-    tr = to_device(fields, torch.float64)
+    dev = HIPBiogeochemistry(bgc, precision)
+    tr = to_device(fields, dev.dtype)
     G = [torch.zeros_like(t) for t in tr]
     dev.update_tendencies(tr, G)
     torch.cuda.synchronize()
     out = to_host(G)
     for n, name in enumerate(bgc.required_biogeochemical_tracers()):
-        assert rel_l2(grid.interior(out[n]), grid.interior(G_ref[n])) <= 1e-12, name
+        assert rel_l2(grid.interior(out[n]), grid.interior(G_ref[n])) <= TOL[precision], name
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
@@ -155,18 +156,19 @@ def test_other_tracers_match_the_model_without_oxygen():
         assert rel_l2(grid.interior(x), grid.interior(y)) <= 1e-14, n
 
 
-def test_add_tendencies_with_oxygen():
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_add_tendencies_with_oxygen(precision):
     import torch
-    print("case oxygen tendencies", flush=True)
+    print("case oxygen tendencies", precision, flush=True)
     bgc, grid, fields = _model((11, 3, 20), True)
     cfg = bgc.config(abi.OBME_FP64)
     G_ref = [np.zeros_like(a) for a in fields]
     O.grid_tendencies(cfg, fields, G_ref)
-    dev = HIPBiogeochemistry(bgc, "fp64")
-    tr = to_device(fields, torch.float64)
+    dev = HIPBiogeochemistry(bgc, precision)
+    tr = to_device(fields, dev.dtype)
     G = [torch.zeros_like(t) for t in tr]
     dev.update_tendencies(tr, G)
     torch.cuda.synchronize()
     out = to_host(G)
     for n, name in enumerate(bgc.required_biogeochemical_tracers()):
-        assert rel_l2(grid.interior(out[n]), grid.interior(G_ref[n])) <= 1e-12, name
+        assert rel_l2(grid.interior(out[n]), grid.interior(G_ref[n])) <= TOL[precision], name

@@ -115,24 +115,29 @@ def test_stretched_grid_closed_bottom(precision):
     assert all(e <= tol for e in errs.values()), errs
 
 
-def test_update_state_par_fp64():
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_update_state_par(precision):
     import torch
-    print("case update_state fp64", flush=True)
+    print("case update_state", precision, flush=True)
     wl = workloads.lobster_3d(24, 4, 50, nsteps=1)
     g = wl.grid
     PAR_ref = g.new_field()
     O.grid_update_state(wl.bgc.config(abi.OBME_FP64), wl.tracers, PAR_ref, 0.0)
-    dev = HIPBiogeochemistry(wl.bgc, "fp64")
-    tr = to_device(wl.tracers, torch.float64)
-    PAR = torch.zeros(g.parent_shape, dtype=torch.float64, device="cuda")
+    dev = HIPBiogeochemistry(wl.bgc, precision)
+    tr = to_device(wl.tracers, dev.dtype)
+    PAR = torch.zeros(g.parent_shape, dtype=dev.dtype, device="cuda")
     dev.update_biogeochemical_state(tr, PAR, 0.0)
     torch.cuda.synchronize()
-    assert rel_l2(g.interior(PAR.cpu().numpy()), g.interior(PAR_ref)) <= 1e-13
+    tol = 1e-13 if precision == "fp64" else FP32_TOL
+    assert rel_l2(g.interior(PAR.double().cpu().numpy()), g.interior(PAR_ref)) <= tol
 
 
-def test_add_tendencies_fp64():
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_add_tendencies(precision):
+    """The tendencies accumulate into G (FP32: G holds FP32 values, the
+    comparison subtracts G's rounded start)."""
     import torch
-    print("case add_tendencies fp64", flush=True)
+    print("case add_tendencies", precision, flush=True)
     wl = workloads.lobster_3d(24, 4, 50, nsteps=1)
     g = wl.grid
     rng = np.random.default_rng(3)
@@ -140,15 +145,17 @@ def test_add_tendencies_fp64():
     Gref = [a.copy() for a in G0]
     PAR_ref = g.new_field()
     O.grid_tendencies(wl.bgc.config(abi.OBME_FP64), wl.tracers, Gref, 0.0, PAR=PAR_ref)
-    dev = HIPBiogeochemistry(wl.bgc, "fp64")
-    tr = to_device(wl.tracers, torch.float64)
-    G = to_device(G0, torch.float64)
-    PAR = torch.zeros(g.parent_shape, dtype=torch.float64, device="cuda")
+    dev = HIPBiogeochemistry(wl.bgc, precision)
+    tr = to_device(wl.tracers, dev.dtype)
+    G = to_device(G0, dev.dtype)
+    G0r = to_host(G)
+    PAR = torch.zeros(g.parent_shape, dtype=dev.dtype, device="cuda")
     dev.update_tendencies(tr, G, PAR, t=0.0)
     torch.cuda.synchronize()
-    for name, a, b, a0 in zip(wl.bgc.required_biogeochemical_tracers(), to_host(G), Gref, G0):
-        assert rel_l2(g.interior(a - a0), g.interior(b - a0)) <= 1e-12, name
-    assert rel_l2(g.interior(PAR.cpu().numpy()), g.interior(PAR_ref)) <= 1e-13
+    tol, ptol = (1e-12, 1e-13) if precision == "fp64" else (FP32_TOL, FP32_TOL)
+    for name, a, b, a0, a0r in zip(wl.bgc.required_biogeochemical_tracers(), to_host(G), Gref, G0, G0r):
+        assert rel_l2(g.interior(a - a0r), g.interior(b - a0)) <= tol, name
+    assert rel_l2(g.interior(PAR.double().cpu().numpy()), g.interior(PAR_ref)) <= ptol
 
 
 @pytest.mark.parametrize("mode", [0, 1])

@@ -111,23 +111,28 @@ def test_reference_case_one_cell():
         assert abs(grid.interior(out[1])[0, 0, 0]) < 1e-9
 
 
-def test_update_state_hook_scales_in_place():
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_update_state_hook_scales_in_place(precision):
+    """FP64: the scaled state to 1e-15; FP32: the scaling of the FP32
+    state (the oracle on the rounded inputs) to 1e-6, the PAR to 1e-5."""
     import torch
-    print("case scaling hook", flush=True)
+    print("case scaling hook", precision, flush=True)
     wl = _with_negatives(_lobster((12, 3, 20), True), ("P", "Z", "DOM"))
     cfg = wl.bgc.config(abi.OBME_FP64)
-    ref = [a.copy() for a in wl.tracers]
+    dev = HIPBiogeochemistry(wl.bgc, precision)
+    tr = to_device(wl.tracers, dev.dtype)
+    start = to_host(tr)
+    ref = [a.copy() for a in start]
     O.grid_scale_negatives(cfg, ref)
     g = wl.grid
     PAR_ref = np.zeros_like(wl.tracers[0])
     O.grid_update_state(cfg, ref, PAR_ref)
-    dev = HIPBiogeochemistry(wl.bgc, "fp64")
-    tr = to_device(wl.tracers, torch.float64)
     PAR = torch.zeros_like(tr[0])
     dev.update_biogeochemical_state(tr, PAR, t=0.0)
     torch.cuda.synchronize()
     out = to_host(tr)
+    tol, ptol = (1e-15, 1e-13) if precision == "fp64" else (1e-6, TOL["fp32"])
     for n, (a, b) in enumerate(zip(out, ref)):
-        assert rel_l2(g.interior(a), g.interior(b)) <= 1e-15, n
-        assert np.array_equal(a[0], wl.tracers[n][0])        # halos untouched
-    assert rel_l2(g.interior(PAR.cpu().numpy()), g.interior(PAR_ref)) <= 1e-13
+        assert rel_l2(g.interior(a), g.interior(b)) <= tol, n
+        assert np.array_equal(a[0], start[n][0])        # halos untouched
+    assert rel_l2(g.interior(PAR.double().cpu().numpy()), g.interior(PAR_ref)) <= ptol

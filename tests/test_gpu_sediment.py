@@ -117,15 +117,16 @@ def test_closed_bottom_buries_nothing():
     assert np.all(dev.sediment_storage.cpu().numpy() == 0) and np.all(state.storage == 0)
 
 
-def test_add_tendencies_with_sediment():
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_add_tendencies_with_sediment(precision):
     import torch
-    print("case sediment tendencies", flush=True)
+    print("case sediment tendencies", precision, flush=True)
     bgc, grid, fields = _npzd((11, 3, 20))
     cfg = bgc.config(abi.OBME_FP64)
     G_ref = [np.zeros_like(a) for a in fields]
     O.grid_tendencies(cfg, fields, G_ref)
-    dev = HIPBiogeochemistry(bgc, "fp64")
-    tr = to_device(fields, torch.float64)
+    dev = HIPBiogeochemistry(bgc, precision)
+    tr = to_device(fields, dev.dtype)
     G = [torch.zeros_like(t) for t in tr]
     dev.update_tendencies(tr, G)
     torch.cuda.synchronize()
@@ -133,9 +134,9 @@ def test_add_tendencies_with_sediment():
     for n, name in enumerate(bgc.required_biogeochemical_tracers()):
         if name == "T":
             continue
-        assert rel_l2(grid.interior(out[n]), grid.interior(G_ref[n])) <= 1e-12, name
+        assert rel_l2(grid.interior(out[n]), grid.interior(G_ref[n])) <= TOL[precision], name
     # the receiver's bottom cells carry the coupled tendency
-    assert rel_l2(grid.interior(out[0])[0], grid.interior(G_ref[0])[0]) <= 1e-12
+    assert rel_l2(grid.interior(out[0])[0], grid.interior(G_ref[0])[0]) <= TOL[precision]
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
