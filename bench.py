@@ -120,14 +120,23 @@ def surface_errors(wl, cfg, out, ref, ci, cj, O):
                                    "max_rel": float(np.max(np.abs(F - F_ref) / np.abs(F_ref)))}}
 
 
+# CPU threads one GPU's job may use on the GPU hosts (each GPU's share of
+# the host's cores); the baseline reports the CPUs it could see beside it
+HOST_SHARE = 16
+
+
+def host_cpus():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def cpu_threads(req):
+    """--cpu-threads, else every visible CPU up to the host's share per GPU."""
     if req:
         return req
-    try:
-        n = len(os.sched_getaffinity(0))
-    except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    return max(1, min(HOST_SHARE, host_cpus()))
 
 
 def read_traffic(profile_dir, workload, precision):
@@ -240,7 +249,7 @@ def main():
                 secs += time.perf_counter() - t0
                 steps += chunk
             cpu = {"value": nw * g.Nz * len(ci) * steps / secs, "unit": "tracer-cell updates/s",
-                   "cores": nthreads, "kind": "port",
+                   "cores": nthreads, "host_cpus": host_cpus(), "kind": "port",
                    "sample": f"all {len(ci)} columns x {g.Nz} levels x {steps} steps of the same workload, "
                              f"FP64 oracle (restated reference CPU path), OpenMP over columns, {secs:.1f} s"}
             log(f"cpu baseline: {cpu['value']:.3e} upd/s on {nthreads} threads ({secs:.1f} s)")
@@ -344,7 +353,7 @@ def bench_columns(args):
         "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if "fp32" in res else "f64", "data": "synthetic (seeded)",
         "config": {"workload": wl.name, "columns": len(off) - 1, "cells": ncell, "dt": wl.dt},
-        "cpu_baseline": {"value": 4 * ncell * cpu_steps / cpu_s, "unit": "tracer-cell updates/s", "cores": nthreads,
+        "cpu_baseline": {"value": 4 * ncell * cpu_steps / cpu_s, "unit": "tracer-cell updates/s", "cores": nthreads, "host_cpus": host_cpus(),
                          "kind": "port",
                          "sample": f"the whole ensemble, {cpu_steps} steps, FP64 oracle, OpenMP over columns, "
                                    f"{cpu_s:.1f} s"},
@@ -408,7 +417,7 @@ def bench_carbonate(args):
         "ms_per_step": res["mode1"]["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (seeded)",
         "config": {"workload": f"carbonate_points_{args.points}"},
-        "cpu_baseline": {"value": cpu_pts / cpu_s, "unit": "points/s", "cores": nthreads, "kind": "port",
+        "cpu_baseline": {"value": cpu_pts / cpu_s, "unit": "points/s", "cores": nthreads, "host_cpus": host_cpus(), "kind": "port",
                          "sample": f"{cpu_pts} points ({cpu_pts // args.points} passes over the set), FP64 oracle "
                                    f"(reference solve from pH 8), OpenMP, {cpu_s:.1f} s"},
         "results": res}))
@@ -432,6 +441,34 @@ def _pisces_oracle_inputs(wl, cols=None):
     cc = m.carbon_chemistry
     solver = cc.solver.struct(cc.initial_pH_guess)
     return OP, q, light, F, np.asarray(g.zc), np.asarray(g.zf), np.asarray(g.dz), solver
+
+
+def pisces_surface_errors(wl, tops, F):
+    """pH (free scale) and the CO2 flux (the default
+    CarbonDioxideGasExchangeBoundaryCondition, as a diagnostic of the top
+    cells' DIC and Alk) of the sampled columns' top cells: the device state
+    `tops` (top-cell values per tracer, tracer order) against the oracle's F,
+    both through the oracle's point function (carbon_chemistry.jl:168,
+    gas_exchange.jl:26-38)."""
+    import oracle as O
+    from oceanbiome import CarbonDioxideGasExchangeBoundaryCondition
+    from oceanbiome.pisces import TRACERS
+    gas = CarbonDioxideGasExchangeBoundaryCondition().struct()
+    cc = wl.bgc.carbon_chemistry
+    solver = cc.solver.struct(cc.initial_pH_guess)
+    idx = {n: TRACERS.index(n) for n in ("DIC", "Alk", "T", "S")}
+    ref = {n: np.ascontiguousarray(F[n][-1].ravel()) for n in ("DIC", "Alk")}
+    T = np.ascontiguousarray(tops[idx["T"]], dtype=np.float64)
+    S = np.ascontiguousarray(tops[idx["S"]], dtype=np.float64)
+    u10 = np.full(T.shape, gas.wind_speed)
+    pH_r, F_r = O.co2_flux_batch(gas, solver, ref["DIC"], ref["Alk"], T, S, u10, gas.air_concentration)
+    pH_g, F_g = O.co2_flux_batch(gas, solver, np.ascontiguousarray(tops[idx["DIC"]], dtype=np.float64),
+                                 np.ascontiguousarray(tops[idx["Alk"]], dtype=np.float64), T, S, u10,
+                                 gas.air_concentration)
+    return {"pH (surface)": {"max_abs": float(np.max(np.abs(pH_g - pH_r))),
+                             "rms": float(np.sqrt(np.mean((pH_g - pH_r) ** 2)))},
+            "CO2 flux (surface, diagnostic)": {"rel_l2": float(np.linalg.norm(F_g - F_r) / np.linalg.norm(F_r)),
+                                               "max_rel": float(np.max(np.abs(F_g - F_r) / np.abs(F_r)))}}
 
 
 def calibration_copy(args):
@@ -494,9 +531,19 @@ def bench_pisces(args):
         rng = np.random.default_rng(1)
         cols = np.sort(rng.choice(g.Nx * g.Ny, min(args.sample_columns, g.Nx * g.Ny), replace=False))
         OP, q, light, F, zc, zf, dz, solver = _pisces_oracle_inputs(wl, cols)
+        F0 = {k: v.copy() for k, v in F.items()}
         t0 = time.perf_counter()
         OP.step(q, light, F, zc, zf, dz, nchk, wl.dt, 0.0, solver, nthreads=nthreads)
         log(f"oracle check run: {len(cols)} columns x {nchk} steps in {time.perf_counter() - t0:.1f} s")
+        # the FP32 storage floor: the oracle with its state rounded to FP32
+        # at the start and after every step
+        X = {k: v.astype(np.float32).astype(np.float64) for k, v in F0.items()}
+        with np.errstate(all="ignore"):
+            for st in range(nchk):
+                OP.step(q, light, X, zc, zf, dz, 1, wl.dt, st * wl.dt, solver, nthreads=nthreads)
+                for k in OP.WRITTEN:
+                    X[k] = X[k].astype(np.float32).astype(np.float64)
+        floor = {k: float(np.linalg.norm(X[k] - F[k]) / np.linalg.norm(F[k])) for k in OP.WRITTEN}
         for prec, out in outputs.items():
             e = {}
             for n, name in enumerate(OP.WRITTEN):
@@ -504,9 +551,14 @@ def bench_pisces(args):
                 b = F[name]
                 e[name] = {"rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
                            "max_rel": float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))}
+                if prec == "fp32":
+                    e[name]["fp32_storage_floor"] = floor[name]
+            tops = [g.interior(out[n]).reshape(g.Nz, -1)[-1, cols] for n in range(26)]
+            e.update(pisces_surface_errors(wl, tops, F))
             errors[prec] = e
-            worst = max(v["rel_l2"] for v in e.values())
-            log(f"pisces {prec} vs oracle after {nchk} steps ({len(cols)} columns): worst rel L2 {worst:.3e}")
+            worst = max(v["rel_l2"] for v in e.values() if "rel_l2" in v)
+            log(f"pisces {prec} vs oracle after {nchk} steps ({len(cols)} columns): worst rel L2 {worst:.3e}, "
+                f"surface pH max abs {e['pH (surface)']['max_abs']:.2e}")
     if not args.no_cpu:
         rng = np.random.default_rng(2)
         ncol = 4096
@@ -519,7 +571,7 @@ def bench_pisces(args):
         t0 = time.perf_counter()
         OP.step(q, light, F, zc, zf, dz, n, wl.dt, wl.dt, solver, nthreads=nthreads)
         secs = time.perf_counter() - t0
-        cpu = {"value": nw * g.Nz * ncol * n / secs, "unit": "tracer-cell updates/s", "cores": nthreads,
+        cpu = {"value": nw * g.Nz * ncol * n / secs, "unit": "tracer-cell updates/s", "cores": nthreads, "host_cpus": host_cpus(),
                "kind": "port",
                "sample": f"{ncol} columns x {g.Nz} levels x {n} steps of the same workload, FP64 numpy oracle "
                          f"(calcite saturation in C over {nthreads} threads, the rest single-threaded), {secs:.1f} s"}

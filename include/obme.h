@@ -608,14 +608,19 @@ enum {
  * (MultiBandPhotosyntheticallyActiveRadiation, Light/multi_band.jl: per band
  * kʷ, e, χ and the share of the surface PAR) and the carbonate solver of the
  * calcite saturation.  The mixed-layer depth and the mean mixed-layer
- * diffusivity are prescribed constants (params); the surface PAR is constant. */
+ * diffusivity are prescribed constants (params).  The surface PAR is a
+ * constant, default_surface_PAR (Light.jl:47, the PISCES default,
+ * PISCES.jl:325-331) at the clock of each stage, or a 2-D array. */
 typedef struct obme_pisces_config {
     int32_t precision;             /* OBME_FP64 | OBME_FP32_MIXED (FP64 calcite solve either way) */
     int32_t open_bottom;           /* POC sinking: 1 = -speed on every face, 0 = damped at the bottom */
     obme_grid grid;
     double params[OBME_PP_COUNT];
     double band_kw[3], band_e[3], band_chi[3], band_division[3];
-    double surface_PAR;            /* W/m2 */
+    double surface_PAR;            /* W/m2 (OBME_SURFACE_CONSTANT) */
+    int32_t surface_mode;          /* OBME_SURFACE_CONSTANT | _DIURNAL | _ARRAY */
+    const void* surface_array;     /* OBME_SURFACE_ARRAY: Nx*Ny device values of the element
+                                      type, i fastest */
     obme_carbon_solver solver;
     obme_negative_scaling scaling; /* PISCES(; scale_negatives = true): the five groups of
                                       PISCES/coupling_utils.jl:11-35 */

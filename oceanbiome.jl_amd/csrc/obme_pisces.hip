@@ -55,7 +55,8 @@ struct PiscesArgs {
     double zf_top;          // top face position
     int Nx, Ny, Nz;
     unsigned sx, sxy, base0;
-    FT surface_PAR;
+    FT surface_PAR;         // PAR⁰ at this call's clock (constant and diurnal modes)
+    const FT* surf;         // OBME_SURFACE_ARRAY: PAR⁰[Nx*Ny], else null
     FT kw[3], e[3], chi[3], division[3];
     StageConst<FT> sk;
     FT A, B;                // RK3: Δt γ and Δt ζ of the stage (FP32 path)
@@ -86,7 +87,8 @@ __global__ void __launch_bounds__(256) k_pisces_columns(PiscesArgs<FT> a)
     const FT* zf = a.lev + Nz;
     const FT* sc = a.pp + OBME_PP_SCALARS;
     const FT zmxl = sc[OBME_PS_MIXED_LAYER_DEPTH];
-    const FT thr = a.surface_PAR * FT(1e-3);
+    const FT PAR0 = a.surf ? a.surf[col] : a.surface_PAR;
+    const FT thr = PAR0 * FT(1e-3);
     const FT lthr = m_log(thr);
     FT band[3];
     FT zeu = FT(-INFINITY);
@@ -98,7 +100,7 @@ __global__ void __launch_bounds__(256) k_pisces_columns(PiscesArgs<FT> a)
 #pragma unroll
         for (int n = 0; n < 3; n++) {
             const FT att = a.kw[n] + a.chi[n] * m_pow(Chl, a.e[n]);
-            if (k == Nz - 1) band[n] = a.surface_PAR * a.division[n] * m_exp(zc[k] * att);
+            if (k == Nz - 1) band[n] = PAR0 * a.division[n] * m_exp(zc[k] * att);
             else band[n] = band[n] * m_exp((zc[k] - zc[k + 1]) * att);
         }
         const FT PAR = band[0] + band[1] + band[2];
@@ -397,7 +399,8 @@ struct PiscesFused {
     NegScale<float, 32> ns;         // ScaleNegativeTracers groups (SCALE)
     StageConst<float> sk[3];        // day lengths at the three stage clocks
     float A[3], B[3];               // Δt γ, Δt ζ per stage
-    float PAR0div[3];               // surface PAR × the share of each band
+    float PAR0[3];                  // surface PAR at the three stage clocks (constant, diurnal)
+    float div[3];                   // the share of each band
     int have_H;                     // Hs holds the [H+] of the previous step
     const double* surf;             // [SURF_PACK][Nx*Ny] from k_pisces_surface_prep
 };
@@ -516,8 +519,8 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
         wml = act ? ((zk >= zmxl) ? dzk : dzk1) : FT(0);
     }
     const FT depth = __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, seg_prefix<64>(wml)), 63));
-    const FT thr = a.surface_PAR * FT(1e-3);
-    const FT lthr = m_log(thr);
+    // the column's surface PAR: a 2-D array, or per stage from the host
+    const FT PAR0col = a.surf ? a.surf[col] : FT(0);
     FT zeu_none;
     {
         const FT z0 = zcv[0];
@@ -554,13 +557,16 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
         }
 
         // MultiBand PAR: band n = PAR0 share · exp(Σ from the top of Δz·att)
+        const FT PAR0 = a.surf ? PAR0col : f.PAR0[st];
+        const FT thr = PAR0 * FT(1e-3);
+        const FT lthr = m_log(thr);
         const FT Chl = v[OBME_PI_PCHL] + v[OBME_PI_DCHL];
         FT band[3];
 #pragma unroll
         for (int n = 0; n < 3; n++) {
             const FT att = a.kw[n] + a.chi[n] * m_pow(Chl, a.e[n]);
             const FT S = seg_prefix<64>(act ? dzl * att : FT(0));
-            band[n] = f.PAR0div[n] * m_exp(S);
+            band[n] = (PAR0 * f.div[n]) * m_exp(S);
         }
         const FT PAR = band[0] + band[1] + band[2];
 
@@ -745,6 +751,11 @@ extern "C" obme_status obme_pisces_create(const obme_pisces_config* cfg, obme_pi
     if (g.Nx < 1 || g.Ny < 1 || g.Nz < 2 || g.Hx < 0 || g.Hy < 0 || g.Hz < 0)
         return obme_fail(OBME_ERR_INVALID, "invalid grid size or halo");
     if (!g.zc || !g.zf || !g.dz) return obme_fail(OBME_ERR_INVALID, "grid coordinates missing");
+    if (cfg->surface_mode != OBME_SURFACE_CONSTANT && cfg->surface_mode != OBME_SURFACE_DIURNAL &&
+        cfg->surface_mode != OBME_SURFACE_ARRAY)
+        return obme_fail(OBME_ERR_INVALID, "unknown surface PAR mode");
+    if (cfg->surface_mode == OBME_SURFACE_ARRAY && !cfg->surface_array)
+        return obme_fail(OBME_ERR_INVALID, "surface PAR array missing");
     const size_t es = (cfg->precision == OBME_FP64) ? 8 : 4;
     const size_t parent = (size_t)(g.Nx + 2 * g.Hx) * (g.Ny + 2 * g.Hy) * (g.Nz + 2 * g.Hz);
     if (parent >= ((size_t)1 << 31)) return obme_fail(OBME_ERR_UNSUPPORTED, "field too large for 32-bit offsets");
@@ -804,6 +815,17 @@ extern "C" obme_status obme_pisces_create(const obme_pisces_config* cfg, obme_pi
     return OBME_OK;
 }
 
+// PAR⁰ at time t: the constant, or default_surface_PAR (Light.jl:47),
+// 100 max(0, cos(t π / 12 h)) with Julia's max (NaN propagates, ±0 ordered)
+static double pisces_surface_value(const obme_pisces_config& c, double t)
+{
+    if (c.surface_mode != OBME_SURFACE_DIURNAL) return c.surface_PAR;
+    const double cv = std::cos(t * 3.141592653589793 / (12 * 3600.0));
+    const double d = 0.0 - cv;
+    const double v = std::isnan(cv) ? d : (std::signbit(d) ? cv : 0.0);
+    return 100 * v;
+}
+
 template <typename FT>
 static PiscesArgs<FT> pisces_args(const obme_pisces* h, double t)
 {
@@ -833,7 +855,8 @@ static PiscesArgs<FT> pisces_args(const obme_pisces* h, double t)
     a.sx = (unsigned)(g.Nx + 2 * g.Hx);
     a.sxy = a.sx * (unsigned)(g.Ny + 2 * g.Hy);
     a.base0 = (unsigned)g.Hx + a.sx * (unsigned)g.Hy + a.sxy * (unsigned)g.Hz;
-    a.surface_PAR = (FT)c.surface_PAR;
+    a.surface_PAR = (FT)pisces_surface_value(c, t);
+    a.surf = (c.surface_mode == OBME_SURFACE_ARRAY) ? (const FT*)c.surface_array : nullptr;
     for (int n = 0; n < 3; n++) {
         a.kw[n] = (FT)c.band_kw[n];
         a.e[n] = (FT)c.band_e[n];
@@ -1044,7 +1067,7 @@ static obme_status pisces_fused_steps(obme_pisces* h, void* const* tracers, int 
     for (int q = 0; q < 3; q++) {
         f.A[q] = (float)(dt * gam[q]);
         f.B[q] = (float)(dt * zet[q]);
-        f.PAR0div[q] = (float)h->cfg.surface_PAR * (float)h->cfg.band_division[q];
+        f.div[q] = (float)h->cfg.band_division[q];
     }
     const double lat = h->cfg.params[OBME_PP_SCALARS + OBME_PS_LATITUDE];
     if (h->cfg.co2.enabled || h->cfg.o2.enabled) {
@@ -1063,6 +1086,7 @@ static obme_status pisces_fused_steps(obme_pisces* h, void* const* tracers, int 
             const double tq = t + cst[q] * dt;
             f.sk[q].dl_growth = (float)host_day_length(lat, tq);  // day_length(φ, t), growth_rate.jl:30
             f.sk[q].dl_chl = (float)host_day_length(tq, lat);     // day_length(t, φ), growth_rate.jl:128
+            f.PAR0[q] = (float)pisces_surface_value(h->cfg, tq);
         }
         f.have_H = st > 0 ? 1 : 0;
         if (h->scaling)

@@ -246,6 +246,8 @@ class PiscesConfig(C.Structure):
         ("band_chi", C.c_double * 3),
         ("band_division", C.c_double * 3),
         ("surface_PAR", C.c_double),
+        ("surface_mode", C.c_int32),
+        ("surface_array", C.c_void_p),
         ("solver", CarbonSolver),
         ("scaling", NegativeScaling),
         ("co2", CO2Exchange),

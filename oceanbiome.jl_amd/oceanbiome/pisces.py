@@ -115,9 +115,11 @@ class PISCES:
     """PISCES(; grid) with the reference defaults.  mixed_layer_depth (m,
     negative down) and the mean mixed-layer diffusivity kappa are prescribed
     constants (the reference keeps zₘₓₗ as a field the user sets and κ̄ = 1
-    without a closure); the surface PAR is constant (W/m²)."""
+    without a closure).  surface_PAR: a number (constant, W/m²), "default"
+    (default_surface_PAR, Light.jl:47, the reference's PISCES default,
+    PISCES.jl:325-331) or a 2-D array PAR⁰[j, i] of shape (Ny, Nx)."""
 
-    def __init__(self, grid, surface_PAR=100.0, mixed_layer_depth=-50.0, kappa=1.0, latitude=45.0,
+    def __init__(self, grid, surface_PAR="default", mixed_layer_depth=-50.0, kappa=1.0, latitude=45.0,
                  bands=((400, 500), (500, 600), (600, 700)), surface_PAR_division=None, POC_speed=2 / day,
                  open_bottom=True, scale_negatives=False, invalid_fill_value=float("nan"),
                  co2_exchange=None, o2_exchange=None, **overrides):
@@ -133,7 +135,17 @@ class PISCES:
         self.o2_exchange = choose(o2_exchange, OxygenGasExchangeBoundaryCondition)
         self.scale_negatives = bool(scale_negatives)
         self.invalid_fill_value = float(invalid_fill_value)
-        self.surface_PAR = float(surface_PAR)
+        if isinstance(surface_PAR, str):
+            if surface_PAR != "default":
+                raise ValueError(f"surface_PAR: a number, 'default' or a 2-D array, not {surface_PAR!r}")
+            self.surface_PAR = "default"
+        elif np.isscalar(surface_PAR):
+            self.surface_PAR = float(surface_PAR)
+        else:
+            arr = np.ascontiguousarray(surface_PAR, dtype=np.float64)
+            if arr.shape != (grid.Ny, grid.Nx):
+                raise ValueError(f"surface_PAR array must have shape (Ny, Nx) = {(grid.Ny, grid.Nx)}")
+            self.surface_PAR = arr
         self.nano = _phyto(3 * day, 0.013, 0.13, 0.8, False, 2.1, 0.42, 0.4, 0.0, 0.033, 1.0)
         self.diatoms = _phyto(4 * day, 0.039, 0.39, 2.4, True, 1.6, 0.69, 0.7, 0.03 / day, 0.05, 3.0)
         self.micro = _zoo(3 / day, (1.0, 0.5, 0.1, 0.0), 0.004 / day, 0.03 / day, 0.3, 0.0, 0.5, 0.01)
@@ -226,7 +238,13 @@ class PISCES:
             cfg.band_e[n] = self.e[n]
             cfg.band_chi[n] = self.chi[n]
             cfg.band_division[n] = self.division[n]
-        cfg.surface_PAR = self.surface_PAR
+        if isinstance(self.surface_PAR, str):
+            cfg.surface_mode = abi.OBME_SURFACE_DIURNAL
+        elif isinstance(self.surface_PAR, np.ndarray):
+            cfg.surface_mode = abi.OBME_SURFACE_ARRAY   # the device array comes from the runtime
+        else:
+            cfg.surface_mode = abi.OBME_SURFACE_CONSTANT
+            cfg.surface_PAR = self.surface_PAR
         cfg.open_bottom = 1 if self.open_bottom else 0
         cc = self.carbon_chemistry
         cfg.solver = cc.solver.struct(cc.initial_pH_guess)

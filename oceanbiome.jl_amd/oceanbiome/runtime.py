@@ -339,6 +339,10 @@ class HIPPISCES:
         self.precision = abi.OBME_FP64 if precision == "fp64" else abi.OBME_FP32_MIXED
         self.dtype = torch.float64 if self.precision == abi.OBME_FP64 else torch.float32
         self.cfg = model.config(self.precision)
+        self._surface = None
+        if self.cfg.surface_mode == abi.OBME_SURFACE_ARRAY:
+            self._surface = torch.from_numpy(model.surface_PAR).to(device="cuda", dtype=self.dtype).contiguous()
+            self.cfg.surface_array = self._surface.data_ptr()
         h = C.c_void_p()
         _check(library().obme_pisces_create(C.byref(self.cfg), C.byref(h)))
         self.handle = h

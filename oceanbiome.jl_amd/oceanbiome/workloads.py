@@ -12,8 +12,9 @@ number of steps.  Random draws use numpy's PCG64 with the given seed.
                  per-column parameters and surface PAR.
  - carbonate_points: DIC/Alk/T/S/u10 point sets for the CO2 flux.
  - pisces_3d:    PISCES on Nx×Ny×Nz (default 256×256×64, 500 m spacing, 1 km
-                 deep), tracers lognormal (σ = 0.3) around the reference's
-                 test values, T = (2 + 26 (j-½)/Ny) exp(z/500), S = 35.
+                 deep), tracers lognormal (σ = 0.3) around the initial
+                 values of validation/PISCES/column.jl:80-87, T = (2 + 26
+                 (j-½)/Ny) exp(z/500), S = 35, surface PAR 100 W/m².
 """
 from dataclasses import dataclass, field
 from typing import Optional
@@ -177,17 +178,24 @@ def npzd_columns(n_columns=8192, nz_min=16, nz_max=128, dz=2.0, nsteps=500, dt=6
                     extra={"offsets": offsets, "params": params, "dz": dz, "T": T})
 
 
-# medians of the PISCES tracers (test/test_PISCES.jl:6-16)
-PISCES_MEDIANS = dict(P=0.5, PChl=0.02, PFe=0.005, D=0.1, DChl=0.004, DFe=0.001, DSi=0.01, Z=0.1, M=0.7, DOC=2.1,
-                      POC=7.8, GOC=38.0, SFe=0.206, BFe=1.1, PSi=0.1, CaCO3=0.1, NO3=2.3, NH4=0.9, PO4=0.6,
-                      Fe=0.13, Si=8.5, DIC=2205.0, Alk=2566.0, O2=317.0)
+# the initial values of validation/PISCES/column.jl:80-87 (CaCO3 = 1e-10)
+PISCES_MEDIANS = dict(P=0.1, PChl=0.025, PFe=0.005, D=0.01, DChl=0.003, DFe=0.0006, DSi=0.004, Z=0.06, M=0.5,
+                      DOC=4.0, POC=5.4, GOC=8.2, SFe=0.34, BFe=0.5, PSi=0.04, CaCO3=1e-10, NO3=10.0, NH4=0.1,
+                      PO4=5.0, Fe=0.6, Si=8.6, DIC=2205.0, Alk=2560.0, O2=317.0)
+# the tracer values of the reference's PISCES tests (test/test_PISCES.jl:6-16)
+PISCES_TEST_MEDIANS = dict(P=0.5, PChl=0.02, PFe=0.005, D=0.1, DChl=0.004, DFe=0.001, DSi=0.01, Z=0.1, M=0.7,
+                           DOC=2.1, POC=7.8, GOC=38.0, SFe=0.206, BFe=1.1, PSi=0.1, CaCO3=0.1, NO3=2.3, NH4=0.9,
+                           PO4=0.6, Fe=0.13, Si=8.5, DIC=2205.0, Alk=2566.0, O2=317.0)
 
 
 def pisces_3d(Nx=256, Ny=256, Nz=64, nsteps=100, dt=300.0, seed=42, Lz=1000.0, spacing=500.0,
-              surface_PAR=100.0, mixed_layer_depth=-50.0):
+              surface_PAR=100.0, mixed_layer_depth=-50.0, medians="column"):
     """PISCES on a periodic box: 24 tracers lognormal(σ = 0.3) around the
-    reference's test values, T = (2 + 26 (j-½)/Ny) exp(z/500) °C, S = 35,
-    constant surface PAR, prescribed mixed-layer depth."""
+    initial values of validation/PISCES/column.jl:80-87, T = (2 + 26 (j-½)/Ny)
+    exp(z/500) °C, S = 35, a constant surface PAR of 100 W/m² by default
+    ("default" for default_surface_PAR, or a 2-D array), prescribed
+    mixed-layer depth.  medians="test": around the values of the reference's
+    PISCES tests (test/test_PISCES.jl:6-16) instead."""
     from .pisces import PISCES, TRACERS
     grid = RectilinearGrid(size=(Nx, Ny, Nz), extent=(Nx * spacing, Ny * spacing, Lz), halo=(3, 3, 3),
                            topology=("Periodic", "Periodic", "Bounded"))
@@ -195,10 +203,10 @@ def pisces_3d(Nx=256, Ny=256, Nz=64, nsteps=100, dt=300.0, seed=42, Lz=1000.0, s
     rng = np.random.Generator(np.random.PCG64(seed))
     shape = (Nz, Ny, Nx)
     fields = []
-    keys = list(PISCES_MEDIANS)
-    for name in keys:
+    med = {"column": PISCES_MEDIANS, "test": PISCES_TEST_MEDIANS}[medians]
+    for name in med:
         f = grid.new_field()
-        grid.interior(f)[...] = PISCES_MEDIANS[name] * np.exp(0.3 * rng.standard_normal(shape))
+        grid.interior(f)[...] = med[name] * np.exp(0.3 * rng.standard_normal(shape))
         fields.append(f)
     jj = (np.arange(Ny) + 0.5) / Ny
     T = (2 + 26 * jj)[None, :, None] * np.exp(grid.zc / 500)[:, None, None] * np.ones((1, 1, Nx))

@@ -526,6 +526,21 @@ def morel_band_coefficients(bands=((400, 500), (500, 600), (600, 700))):
     return m(bands)
 
 
+def surface_PAR_at(spec, t):
+    """PAR⁰ of the MultiBand light at time t (the surface boundary value,
+    multi_band.jl:151): a constant, "default" (default_surface_PAR,
+    Light.jl:47, the PISCES default of PISCES.jl:325-331) or a 2-D array
+    PAR⁰[j, i], returned per column."""
+    import oracle as O
+    if isinstance(spec, str):
+        if spec != "default":
+            raise ValueError(spec)
+        return O.default_surface_PAR(t)
+    if np.isscalar(spec):
+        return float(spec)
+    return np.asarray(spec, dtype=np.float64).reshape(-1)
+
+
 def multi_band_PAR(PAR0, zc, Chl, kw, e, chi, division):
     """update_MultiBandPhotosyntheticallyActiveRadiation! (multi_band.jl:147-165)
     for columns: Chl [Nz, ncol] bottom to top; returns [nbands, Nz, ncol]."""
@@ -640,9 +655,11 @@ def update_state(q, light, F, zc, zf, t, solver, nthreads=8):
     Nz = zc.shape[0]
     shp = F["T"].shape
     Chl = (F["PChl"] + F["DChl"]).reshape(Nz, -1)
-    bands = multi_band_PAR(light["surface_PAR"], zc, Chl, light["kw"], light["e"], light["chi"], light["division"])
+    PAR0 = surface_PAR_at(light["surface_PAR"], t)
+    bands = multi_band_PAR(PAR0, zc, Chl, light["kw"], light["e"], light["chi"], light["division"])
     PAR = (bands[0] + bands[1]) + bands[2]
-    zeu = euphotic_depth(PAR, zc, light["surface_PAR"])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        zeu = euphotic_depth(PAR, zc, PAR0)
     mlPAR = mixed_layer_mean(PAR, zf, q["zmxl"])
     z3 = np.broadcast_to(zc[:, None, None], shp)
     P = np.abs(z3) * q["gravity"] * 1026 / 100000

@@ -4,16 +4,21 @@ independent, so a sample of columns stepped by the oracle is an exact check
 of those columns; the halos and the read-only fields stay untouched; every
 written value stays finite.  LOBSTER + DIC/Alk with the CO2 exchange on
 512x512x64 (the headline workload) in both precisions, the same with the
-O2 tracer and its gas exchange in FP32-mixed, and PISCES on
-256x256x64 in FP32-mixed.  Bars as everywhere: FP64 <= 1e-12 relative L2
-per tracer, FP32-mixed <= 1e-5; the surface pH of the sampled columns
-within 1e-6 (FP32-mixed) and 1e-12 (FP64) of the oracle's, and their CO2
-flux within 1e-5 and 1e-10 relative L2."""
+O2 tracer and its gas exchange in FP32-mixed, and PISCES on 256x256x64
+for its 100 steps in both precisions.  Bars as everywhere: FP64 <= 1e-12
+relative L2 per tracer, FP32-mixed <= 1e-5 (PISCES around the values of
+validation/PISCES/column.jl: or three times the FP32 storage floor, the
+deviation the FP64 oracle shows when its state is rounded to FP32 after
+every step, where that is larger: the reference's iron uptake has a pole
+near those iron quotas); the surface pH of the sampled LOBSTER columns within
+1e-6 (FP32-mixed) and 1e-12 (FP64) of the oracle's, and their CO2 flux
+within 1e-5 and 1e-10 relative L2."""
 import numpy as np
 import pytest
 
 import oracle as O
 import pisces as OP
+from helpers import fp32_storage_floor
 from oceanbiome import abi, workloads
 from oceanbiome.runtime import HIPBiogeochemistry, HIPPISCES
 
@@ -76,10 +81,12 @@ def test_lobster_512x512x64_column_sample(precision, oxygen):
     assert dpH <= PH_TOL[precision] and dF <= FLUX_TOL[precision], (dpH, dF)
 
 
-def test_pisces_256x256x64_column_sample_fp32():
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_pisces_256x256x64_column_sample(precision):
+    """The PISCES workload's full length: 100 steps of 300 s."""
     import torch
-    print("case full-size pisces fp32", flush=True)
-    wl = workloads.pisces_3d(256, 256, 64, nsteps=2, dt=300.0)
+    print("case full-size pisces", precision, flush=True)
+    wl = workloads.pisces_3d(256, 256, 64, nsteps=100, dt=300.0)
     g = wl.grid
     m = wl.bgc
     ci, cj = _sample(g, 48, 7)
@@ -87,10 +94,12 @@ def test_pisces_256x256x64_column_sample_fp32():
     light = dict(surface_PAR=m.surface_PAR, kw=m.kw, e=m.e, chi=m.chi, division=m.division)
     F = {n: _columns(g, a, ci, cj)[:, None, :].copy() for n, a in zip(OP.TRACERS, wl.tracers)}
     cc = m.carbon_chemistry
+    F0 = {k: v.copy() for k, v in F.items()}
+    args = (np.asarray(g.zc), np.asarray(g.zf), np.asarray(g.dz))
+    solver = cc.solver.struct(cc.initial_pH_guess)
     with np.errstate(all="ignore"):
-        OP.step(q, light, F, np.asarray(g.zc), np.asarray(g.zf), np.asarray(g.dz), wl.nsteps, wl.dt, 0.0,
-                cc.solver.struct(cc.initial_pH_guess))
-    dev = HIPPISCES(m, "fp32")
+        OP.step(q, light, F, *args, wl.nsteps, wl.dt, 0.0, solver)
+    dev = HIPPISCES(m, precision)
     tr = [torch.from_numpy(a).to(device="cuda", dtype=dev.dtype) for a in wl.tracers]
     dev.step(tr, wl.nsteps, wl.dt)
     torch.cuda.synchronize()
@@ -102,6 +111,16 @@ def test_pisces_256x256x64_column_sample_fp32():
         errs[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
         assert np.isfinite(g.interior(out)).all(), name
     print("errors", errs, flush=True)
-    assert all(e <= TOL["fp32"] for e in errs.values()), errs
+    if precision == "fp64":
+        bars = {k: TOL["fp64"] for k in OP.WRITTEN}
+    else:
+        # 1e-5, or three times the FP32 storage floor where the reference's
+        # iron uptake pole makes that larger
+        floor = fp32_storage_floor(lambda X, s: OP.step(q, light, X, *args, 1, wl.dt, s * wl.dt, solver),
+                                   wl.nsteps, F0, F, OP.WRITTEN)
+        print("storage floor", floor, flush=True)
+        bars = {k: max(TOL["fp32"], 3 * floor[k]) for k in OP.WRITTEN}
+    assert all(errs[k] <= bars[k] for k in OP.WRITTEN), (errs, bars)
     # T and S are read only
-    np.testing.assert_array_equal(tr[24].cpu().numpy(), wl.tracers[24].astype(np.float32))
+    dt_np = np.float32 if precision == "fp32" else np.float64
+    np.testing.assert_array_equal(tr[24].cpu().numpy(), wl.tracers[24].astype(dt_np))

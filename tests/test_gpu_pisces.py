@@ -4,7 +4,10 @@ PAR, euphotic depth, mixed-layer PAR, calcite saturation), the tendencies
 and a few RK3 steps.  FP64 bar: relative L2 <= 1e-12 per tracer, for the
 raw tendencies and after the steps (the raw tendencies measured at most
 3.2e-15, Fe, whose free-iron root moves that much when exp(...) changes by
-one ulp); FP32-mixed: 1e-5."""
+one ulp); FP32-mixed: 1e-5.  The inputs lie around the values of the
+reference's PISCES tests (test/test_PISCES.jl:6-16), whose phytoplankton
+iron quotas sit far from the pole of the iron uptake factor (1 - r)/(1.05 - r)
+(mixed_mondo.jl:161-181)."""
 import numpy as np
 import pytest
 
@@ -39,7 +42,7 @@ def _device(wl, precision):
 def test_update_state_fp64():
     import torch
     print("case pisces update_state fp64", flush=True)
-    wl = workloads.pisces_3d(12, 5, 30, nsteps=1, seed=3)
+    wl = workloads.pisces_3d(12, 5, 30, nsteps=1, seed=3, medians="test")
     q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
     A = OP.update_state(q, light, F, zc, zf, 0.0, solver)
     dev, tr = _device(wl, "fp64")
@@ -57,7 +60,7 @@ def test_update_state_fp64():
 def test_tendencies_fp64():
     import torch
     print("case pisces tendencies fp64", flush=True)
-    wl = workloads.pisces_3d(10, 4, 24, nsteps=1, seed=5)
+    wl = workloads.pisces_3d(10, 4, 24, nsteps=1, seed=5, medians="test")
     q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
     G, _ = OP.full_tendencies(q, light, F, zc, zf, dz, 1.0e5, solver)
     dev, tr = _device(wl, "fp64")
@@ -76,7 +79,7 @@ def test_tendencies_fp64():
 def test_step(precision):
     import torch
     print("case pisces step", precision, flush=True)
-    wl = workloads.pisces_3d(8, 4, 20, nsteps=3, dt=300.0, seed=7)
+    wl = workloads.pisces_3d(8, 4, 20, nsteps=3, dt=300.0, seed=7, medians="test")
     q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
     OP.step(q, light, F, zc, zf, dz, wl.nsteps, wl.dt, 0.0, solver)
     dev, tr = _device(wl, precision)
@@ -94,7 +97,7 @@ def test_step(precision):
 def _scaled_run(precision, flipped, seed=9):
     import torch
     from oceanbiome import pisces as HP
-    wl = workloads.pisces_3d(8, 4, 20, nsteps=3, dt=300.0, seed=seed)
+    wl = workloads.pisces_3d(8, 4, 20, nsteps=3, dt=300.0, seed=seed, medians="test")
     wl.bgc = HP.PISCES(wl.grid, scale_negatives=True)
     rng = np.random.default_rng(3)
     g = wl.grid
@@ -144,7 +147,7 @@ def test_step_with_gas_exchange(precision):
     import torch
     from oceanbiome import pisces as HP
     print("case pisces gas exchange", precision, flush=True)
-    wl = workloads.pisces_3d(8, 4, 20, nsteps=3, dt=300.0, seed=13)
+    wl = workloads.pisces_3d(8, 4, 20, nsteps=3, dt=300.0, seed=13, medians="test")
     wl.bgc = HP.PISCES(wl.grid, co2_exchange=True, o2_exchange=True)
     q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
     cfg = wl.bgc.config(0)
@@ -169,7 +172,7 @@ def test_tendencies_with_gas_exchange_fp64():
     import torch
     from oceanbiome import pisces as HP
     print("case pisces gas exchange tendencies", flush=True)
-    wl = workloads.pisces_3d(10, 4, 24, nsteps=1, seed=5)
+    wl = workloads.pisces_3d(10, 4, 24, nsteps=1, seed=5, medians="test")
     wl.bgc = HP.PISCES(wl.grid, co2_exchange=True, o2_exchange=True)
     q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
     cfg = wl.bgc.config(0)
@@ -180,4 +183,4 @@ def test_tendencies_with_gas_exchange_fp64():
     dev.update_tendencies(tr, Gd, t=1.0e5)
     torch.cuda.synchronize()
     for n, name in enumerate(OP.WRITTEN):
-        assert rel_l2(g.interior(Gd[n].cpu().numpy()), G[name]) <= 1e-10, name
+        assert rel_l2(g.interior(Gd[n].cpu().numpy()), G[name]) <= 1e-12, name

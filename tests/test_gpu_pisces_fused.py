@@ -4,7 +4,8 @@ ragged depth, with the gas exchange and the negative scaling on, over two
 calls (the second warm-starts [H+] from the first); against the per-stage
 kernels (OBME_PISCES_FUSED=0) on a larger grid; and the per-stage path that
 columns deeper than 64 levels take.  FP32-mixed bar: relative L2 <= 1e-5
-per tracer."""
+per tracer.  Inputs around the values of the reference's PISCES tests
+(test/test_PISCES.jl:6-16)."""
 import os
 
 import numpy as np
@@ -52,7 +53,7 @@ def _device_run(wl, calls, nsteps, fused=True):
 @pytest.mark.parametrize("Nz,gas,scale", [(64, False, False), (64, True, True), (37, True, False)])
 def test_fused_step_against_oracle(Nz, gas, scale):
     print("case pisces fused", Nz, "gas", gas, "scale", scale, flush=True)
-    wl = workloads.pisces_3d(6, 3, Nz, nsteps=2, dt=300.0, seed=17)
+    wl = workloads.pisces_3d(6, 3, Nz, nsteps=2, dt=300.0, seed=17, medians="test")
     wl.bgc = HP.PISCES(wl.grid, co2_exchange=gas, o2_exchange=gas, scale_negatives=scale)
     q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
     cfg = wl.bgc.config(0)
@@ -73,7 +74,7 @@ def test_fused_step_against_oracle(Nz, gas, scale):
 
 def test_fused_matches_the_per_stage_kernels():
     print("case pisces fused vs per-stage", flush=True)
-    wl = workloads.pisces_3d(40, 9, 64, nsteps=4, dt=300.0, seed=23)
+    wl = workloads.pisces_3d(40, 9, 64, nsteps=4, dt=300.0, seed=23, medians="test")
     wl.bgc = HP.PISCES(wl.grid, co2_exchange=True, o2_exchange=True)
     a = _device_run(wl, 1, wl.nsteps, fused=True)
     b = _device_run(wl, 1, wl.nsteps, fused=False)
@@ -88,7 +89,7 @@ def test_fused_matches_the_per_stage_kernels():
 
 def test_deep_columns_take_the_per_stage_path():
     print("case pisces deep columns", flush=True)
-    wl = workloads.pisces_3d(4, 2, 70, nsteps=2, dt=300.0, seed=29)
+    wl = workloads.pisces_3d(4, 2, 70, nsteps=2, dt=300.0, seed=29, medians="test")
     q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
     OP.step(q, light, F, zc, zf, dz, wl.nsteps, wl.dt, 0.0, solver)
     out = _device_run(wl, 1, wl.nsteps)
@@ -101,7 +102,7 @@ def test_deep_columns_take_the_per_stage_path():
 def test_fused_step_with_non_default_parameters():
     """A parameter set other than the defaults (a shallower mixed layer)."""
     print("case pisces fused non-default parameters", flush=True)
-    wl = workloads.pisces_3d(5, 3, 64, nsteps=2, dt=300.0, seed=31, mixed_layer_depth=-30.0)
+    wl = workloads.pisces_3d(5, 3, 64, nsteps=2, dt=300.0, seed=31, mixed_layer_depth=-30.0, medians="test")
     q, light, F, zc, zf, dz, solver = _oracle_inputs(wl)
     OP.step(q, light, F, zc, zf, dz, wl.nsteps, wl.dt, 0.0, solver)
     out = _device_run(wl, 1, wl.nsteps)

@@ -232,3 +232,50 @@ def test_oxygen_gas_exchange_known_answers():
     k660 = 0.266 / 3600 / 100 * 2.0 ** 2
     assert f1 / 10 == pytest.approx(k660 / math.sqrt(O.o2_schmidt(o2, 25.0) / 660), rel=1e-12)
     assert co2.transfer_velocity[2] == pytest.approx(0.266 / 3600 / 100, rel=1e-15)
+
+
+def _surface_case(surface, t, Nx=5, Ny=3, Nz=12, seed=4):
+    from oceanbiome import workloads
+    wl = workloads.pisces_3d(Nx, Ny, Nz, nsteps=1, seed=seed, surface_PAR=surface)
+    g = wl.grid
+    m = wl.bgc
+    q = OP.params_from_vector(m.param_vector())
+    light = dict(surface_PAR=m.surface_PAR, kw=m.kw, e=m.e, chi=m.chi, division=m.division)
+    F = {n: g.interior(a).copy() for n, a in zip(OP.TRACERS, wl.tracers)}
+    cc = m.carbon_chemistry
+    return OP.update_state(q, light, F, np.asarray(g.zc), np.asarray(g.zf), 0.0 if t is None else t,
+                           cc.solver.struct(cc.initial_pH_guess), nthreads=2), g
+
+
+def test_surface_PAR_modes_reduce_to_the_constant():
+    """default_surface_PAR (Light.jl:47) at t = 0 is the constant 100 and at
+    t = 3 h the constant 100 cos(π/4); a uniform 2-D PAR⁰ is the constant;
+    a 2-D PAR⁰ gives each column the light of its own constant; at night
+    (t = 12 h, PAR⁰ = 0) the light is zero, the euphotic depth falls back to
+    the level below the bottom (compute_euphotic_depth.jl:33: log 0 - log 0
+    is not finite) and the mixed-layer light is zero."""
+    keys = ("PAR", "PAR1", "PAR2", "PAR3", "zeu", "mlPAR")
+    A0, _ = _surface_case(100.0, 0.0)
+    Ad, _ = _surface_case("default", 0.0)
+    for k in keys:
+        assert np.array_equal(A0[k], Ad[k]), k
+    A3, _ = _surface_case(100.0 * O.default_surface_PAR(3 * 3600.0) / 100.0, 3 * 3600.0)
+    Ad3, _ = _surface_case("default", 3 * 3600.0)
+    for k in keys:
+        assert np.array_equal(A3[k], Ad3[k]), k
+    Au, _ = _surface_case(np.full((3, 5), 70.0), 0.0)
+    Ac, _ = _surface_case(70.0, 0.0)
+    for k in keys:
+        assert np.array_equal(Au[k], Ac[k]), k
+    rng = np.random.default_rng(1)
+    arr = rng.uniform(20.0, 200.0, (3, 5))
+    Aa, g = _surface_case(arr, 0.0)
+    for j in range(3):
+        for i in range(5):
+            Ai, _ = _surface_case(float(arr[j, i]), 0.0)
+            for k in keys:
+                assert np.array_equal(Aa[k][..., j, i], Ai[k][..., j, i]), (k, j, i)
+    An, g = _surface_case("default", 12 * 3600.0)
+    assert not np.any(An["PAR"]) and not np.any(An["mlPAR"])
+    z0 = g.zc[0] - (g.zc[1] - g.zc[0])
+    assert np.all(An["zeu"] == z0)
