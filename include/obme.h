@@ -57,7 +57,10 @@ enum { OBME_FP64 = 0, OBME_FP32_MIXED = 1 };
 
 enum { OBME_LIGHT_MONDO = 0, OBME_LIGHT_ANALYTICAL = 1 };   /* plankton.jl:216-220 */
 enum { OBME_LINEAR = 0, OBME_QUADRATIC = 1 };               /* plankton.jl:83-90   */
-enum { OBME_UPWIND1 = 0, OBME_CENTERED2 = 1 };              /* sinking face value  */
+/* sinking face value: first-order upwind, second-order centred, fifth-order
+ * WENO (the reconstructions of Oceananigans' UpwindBiased(order = 1),
+ * Centered(order = 2) and WENO(order = 5) applied to the drift velocity) */
+enum { OBME_UPWIND1 = 0, OBME_CENTERED2 = 1, OBME_WENO5 = 2 };
 
 enum {
     OBME_SURFACE_CONSTANT = 0,  /* PAR0 = surface_value                          */
@@ -133,7 +136,7 @@ typedef struct obme_two_band_par {
 typedef struct obme_sinking {
     double speed[OBME_MAX_TRACERS];
     int32_t open_bottom;           /* 1: w = -speed on faces 1..Nz; 0: damped to 0 at the bottom */
-    int32_t scheme;                /* OBME_UPWIND1 or OBME_CENTERED2 */
+    int32_t scheme;                /* OBME_UPWIND1, OBME_CENTERED2 or OBME_WENO5 */
     double smoothing_distance;     /* default 2 */
 } obme_sinking;
 

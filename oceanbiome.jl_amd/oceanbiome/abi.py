@@ -11,7 +11,7 @@ OBME_NPZD, OBME_LOBSTER = 1, 2
 OBME_FP64, OBME_FP32_MIXED = 0, 1
 OBME_LIGHT_MONDO, OBME_LIGHT_ANALYTICAL = 0, 1
 OBME_LINEAR, OBME_QUADRATIC = 0, 1
-OBME_UPWIND1, OBME_CENTERED2 = 0, 1
+OBME_UPWIND1, OBME_CENTERED2, OBME_WENO5 = 0, 1, 2
 OBME_SURFACE_CONSTANT, OBME_SURFACE_DIURNAL, OBME_SURFACE_ARRAY = 0, 1, 2
 OBME_FIELD_CONSTANT, OBME_FIELD_PROFILE, OBME_FIELD_3D = 0, 1, 2
 OBME_MAX_TRACERS = 32

@@ -452,7 +452,10 @@ class Biogeochemistry:
         for n, name in enumerate(self.required_biogeochemical_tracers()):
             sk.speed[n] = float(sp.get(name, 0.0))
         sk.open_bottom = 1 if self.open_bottom() else 0
-        sk.scheme = abi.OBME_CENTERED2 if self.sinking_scheme == "centered" else abi.OBME_UPWIND1
+        schemes = {"upwind": abi.OBME_UPWIND1, "centered": abi.OBME_CENTERED2, "weno5": abi.OBME_WENO5}
+        if self.sinking_scheme not in schemes:
+            raise ValueError(f"sinking_scheme: one of {sorted(schemes)}, not {self.sinking_scheme!r}")
+        sk.scheme = schemes[self.sinking_scheme]
         sk.smoothing_distance = 2.0
         cfg.sinking = sk
         cc = self.gas_exchange.carbon_chemistry if self.gas_exchange is not None else CarbonChemistry()

@@ -111,9 +111,12 @@ double orc_default_surface_PAR(double t);
 /* ---- sinking ---- */
 /* w on the Nz+1 faces of a column (bottom face first), Utils/sinking_velocity_fields.jl:10-35 */
 void orc_sinking_profile(const obme_sinking* s, double speed, int Nz, double* w);
-/* (F(k+1) - F(k)) / dz(k) with F = w c(upwind); the tendency is minus this */
+/* (F(k+1) - F(k)) / dz(k) with F = w c(face) and the face value of the
+ * scheme (OBME_UPWIND1, OBME_CENTERED2, OBME_WENO5); the tendency is minus this */
 void orc_sinking_divergence(int scheme, int Nz, const double* w, const double* c, const double* dz,
                             double* div);
+/* The face value at face k (0 .. Nz; k = 0 the bottom) of the scheme, from above. */
+double orc_sinking_face_value(int scheme, int Nz, const double* c, int k);
 
 /* ---- one column, nsteps of RK3 ---- */
 typedef struct orc_column_env {

@@ -87,6 +87,7 @@ def load_library(path):
         "orc_default_surface_PAR": (D, [D]),
         "orc_sinking_profile": (None, [C.POINTER(abi.Sinking), D, I, DP]),
         "orc_sinking_divergence": (None, [I, I, DP, DP, DP, DP]),
+        "orc_sinking_face_value": (C.c_double, [I, I, DP, I]),
         "orc_equilibrium_constants": (None, [D, D, I, D, C.POINTER(Constants)]),
         "orc_grid_run": (I, [C.POINTER(abi.Config), C.POINTER(DP), DP, DP, DP, I, D, D, I, I]),
         "orc_grid_run_columns": (I, [C.POINTER(abi.Config), C.POINTER(DP), DP, DP, DP, I, D, D,
@@ -195,6 +196,11 @@ def sinking_divergence(w, c, dz, scheme=None):
                                  _dp(np.ascontiguousarray(c, dtype=np.float64)),
                                  _dp(np.ascontiguousarray(dz, dtype=np.float64)), _dp(out))
     return out
+
+
+def sinking_face_value(c, k, scheme):
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    return lib().orc_sinking_face_value(scheme, len(c), _dp(c), int(k))
 
 
 def pressure_correction(which, Tk, P):

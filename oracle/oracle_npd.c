@@ -274,27 +274,84 @@ static void sinking_profile(const obme_sinking* s, double speed, int Nz, double*
     w[Nz] = 0.0;
 }
 
+static double sq(double x) { return x * x; }
+
+/* WENO-Z weights (Borges et al. 2008, J. Comput. Phys. 227:3191) on the
+ * Jiang & Shu (1996) smoothness indicators: alpha_r = d_r (1 + (tau /
+ * (beta_r + eps))^2), normalised; eps = 1e-8.  This is the form of
+ * Oceananigans' WENO reconstruction; Oceananigans is not part of the
+ * reference tree, so parity with it is unpinned (the closed forms below are
+ * the textbook scheme). */
+#define WENO_EPS 1e-8
+
+/* The value at face k reconstructed from above (w <= 0: the upwind cell is
+ * k, the cells above it k+1, k+2 are further upwind): fifth order from the
+ * cells k-2 .. k+2, third order from k-1 .. k+1.  The mirror images of the
+ * left-biased textbook stencils. */
+static double weno5_from_above(double cm2, double cm1, double c0, double cp1, double cp2)
+{
+    const double q0 = (2 * cp2 - 7 * cp1 + 11 * c0) / 6;
+    const double q1 = (-cp1 + 5 * c0 + 2 * cm1) / 6;
+    const double q2 = (2 * c0 + 5 * cm1 - cm2) / 6;
+    const double b0 = 13.0 / 12 * sq(cp2 - 2 * cp1 + c0) + 0.25 * sq(cp2 - 4 * cp1 + 3 * c0);
+    const double b1 = 13.0 / 12 * sq(cp1 - 2 * c0 + cm1) + 0.25 * sq(cp1 - cm1);
+    const double b2 = 13.0 / 12 * sq(c0 - 2 * cm1 + cm2) + 0.25 * sq(3 * c0 - 4 * cm1 + cm2);
+    const double tau = fabs(b0 - b2);
+    const double a0 = 0.1 * (1 + sq(tau / (b0 + WENO_EPS)));
+    const double a1 = 0.6 * (1 + sq(tau / (b1 + WENO_EPS)));
+    const double a2 = 0.3 * (1 + sq(tau / (b2 + WENO_EPS)));
+    return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+}
+
+static double weno3_from_above(double cm1, double c0, double cp1)
+{
+    const double q0 = (-cp1 + 3 * c0) / 2;
+    const double q1 = (c0 + cm1) / 2;
+    const double b0 = sq(c0 - cp1);
+    const double b1 = sq(cm1 - c0);
+    const double tau = fabs(b0 - b1);
+    const double a0 = (1.0 / 3) * (1 + sq(tau / (b0 + WENO_EPS)));
+    const double a1 = (2.0 / 3) * (1 + sq(tau / (b1 + WENO_EPS)));
+    return (a0 * q0 + a1 * q1) / (a0 + a1);
+}
+
+/* The face value at an inner face k (1 <= k <= Nz-1).  Near the boundaries
+ * the order drops as Oceananigans' schemes do in a Bounded direction (the
+ * face must lie at least the scheme's halo size from the boundary): WENO5
+ * on faces 3 .. Nz-3, WENO3 on faces 2 and Nz-2, first-order upwind on
+ * faces 1 and Nz-1. */
+static double face_value(int scheme, int Nz, const double* c, int k)
+{
+    if (scheme == OBME_CENTERED2) return (c[k - 1] + c[k]) / 2;
+    if (scheme == OBME_WENO5) {
+        if (k >= 3 && k <= Nz - 3) return weno5_from_above(c[k - 2], c[k - 1], c[k], c[k + 1], c[k + 2]);
+        if (k >= 2 && k <= Nz - 2) return weno3_from_above(c[k - 1], c[k], c[k + 1]);
+    }
+    return c[k];
+}
+
 /* -∂z(w c) in flux form for one tracer column (Oceananigans div_Uc with the
  * drift velocity; the in-tree flux use is Sediments/tracked_fields.jl:59-60).
- * Face k (0-based, between cells k-1 and k): w <= 0, the upwind value is
- * cell k; at the bottom face the zero-gradient halo gives cell 0 for every
+ * Face k (0-based, between cells k-1 and k): w <= 0, so the upwind cell is
+ * k; at the bottom face the zero-gradient halo gives cell 0 for every
  * scheme; the top face carries w = 0. */
 static void sinking_divergence(int scheme, int Nz, const double* w, const double* c, const double* dz,
                                double* div)
 {
     for (int k = 0; k < Nz; k++) {
-        double cf_hi, cf_lo;
-        if (scheme == OBME_CENTERED2) {
-            cf_hi = (k + 1 < Nz) ? (c[k] + c[k + 1]) / 2 : c[k];
-            cf_lo = (k > 0) ? (c[k - 1] + c[k]) / 2 : c[k];
-        } else {
-            cf_hi = (k + 1 < Nz) ? c[k + 1] : c[k];
-            cf_lo = c[k];
-        }
+        const double cf_hi = (k + 1 < Nz) ? face_value(scheme, Nz, c, k + 1) : c[k];
+        const double cf_lo = (k > 0) ? face_value(scheme, Nz, c, k) : c[k];
         double Fhi = w[k + 1] * cf_hi;
         double Flo = w[k] * cf_lo;
         div[k] = (Fhi - Flo) / dz[k];
     }
+}
+
+double orc_sinking_face_value(int scheme, int Nz, const double* c, int k)
+{
+    if (k <= 0) return c[0];
+    if (k >= Nz) return c[Nz - 1];
+    return face_value(scheme, Nz, c, k);
 }
 
 void orc_sinking_profile(const obme_sinking* s, double speed, int Nz, double* w)
