@@ -51,6 +51,8 @@ def parse():
                         "ensemble; carbonate: pH and CO2 flux on point sets")
     p.add_argument("--oxygen", action="store_true",
                    help="lobster workload with the O2 tracer and its gas exchange (10 written tracers)")
+    p.add_argument("--sinking", default="upwind", choices=["upwind", "centered", "weno5"],
+                   help="lobster workload: the sinking face value (centered and weno5 step one stage per launch)")
     p.add_argument("--columns", type=int, default=8192)
     p.add_argument("--points", type=int, default=1 << 24)
     return p.parse_args()
@@ -173,7 +175,8 @@ def main():
     from oceanbiome.runtime import HIPBiogeochemistry
 
     log("building workload", args.nx, args.ny, args.nz)
-    wl = workloads.lobster_3d(args.nx, args.ny, args.nz, nsteps=args.check_steps, oxygen=args.oxygen)
+    wl = workloads.lobster_3d(args.nx, args.ny, args.nz, nsteps=args.check_steps, oxygen=args.oxygen,
+                              sinking_scheme=args.sinking)
     g = wl.grid
     ncell = g.n_interior
     nw = wl.n_written

@@ -50,10 +50,11 @@ extern "C" const char* obme_version(void) { return "obme 0.1.0 gfx950"; }
 // VGPRs (4 waves per SIMD, the occupancy at which the 512x512 grid's 4096
 // waves fill the chip once).
 constexpr int SWEEP_MAX_THREADS = 512;
-template <int MODEL, int CARB, typename FT, int MODE, bool UNIFORM, int FM = -1, int EXTRA = 0, bool MASKX = true>
+template <int MODEL, int CARB, typename FT, int MODE, bool UNIFORM, int FM = -1, int EXTRA = 0, bool MASKX = true,
+          int SCHEME = OBME_UPWIND1>
 __global__ void __launch_bounds__(SWEEP_MAX_THREADS, sizeof(FT) == 4 ? 4 : 1) k_column_lane(StepArgs<FT> a)
 {
-    column_lane<MODEL, CARB, FT, MODE, UNIFORM, FM, EXTRA, MASKX>(a, (int)blockDim.x);
+    column_lane<MODEL, CARB, FT, MODE, UNIFORM, FM, EXTRA, MASKX, true, SCHEME>(a, (int)blockDim.x);
 }
 
 // The O2 gas exchange's per-column coefficients (GasExchange.jl:127-146):
@@ -204,6 +205,11 @@ struct obme_handle {
     bool scaling;              // ScaleNegativeTracers groups configured
     bool co2_inline;           // scaling with the CO2 flux: the flux is computed in the sweep
     bool sediment;             // InstantRemineralisation configured
+    int scheme;                // the sinking face value (OBME_UPWIND1, OBME_CENTERED2, OBME_WENO5)
+    bool staged;               // centred / WENO sinking: steps run one stage per launch
+    int cur_stage = 0;         // the stage of the next TILE_STAGE launch
+    void* d_stageD = nullptr;  // staged FP32: NT * Nx*Ny*Nz, the step's increment so far
+    void* d_stageG = nullptr;  // staged: NT * Nx*Ny*Nz, the previous stage's tendencies
     double* d_sedG = nullptr;  // the sediment time stepper's Gⁿ, G⁻ (2 * Nx*Ny)
     long long sed_iter = 0;    // sediment steps taken (its clock's iteration)
     double sed_last_dt = HUGE_VAL;   // Δt of its last step (Inf before the first)
@@ -401,8 +407,12 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
         sv = validate_sediment(cfg->sediment, ntracers_of(*cfg), cfg->model == OBME_NPZD ? 3 : -1);
         if (sv != OBME_OK) return sv;
     }
-    if (cfg->sinking.scheme != OBME_UPWIND1)
-        return fail(OBME_ERR_UNSUPPORTED, "only the first-order upwind sinking scheme runs on the GPU");
+    if (cfg->sinking.scheme != OBME_UPWIND1 && cfg->sinking.scheme != OBME_CENTERED2 &&
+        cfg->sinking.scheme != OBME_WENO5)
+        return fail(OBME_ERR_INVALID, "unknown sinking scheme");
+    if (cfg->sinking.scheme != OBME_UPWIND1 && (cfg->scaling.n_groups > 0 || cfg->sediment.model != OBME_SEDIMENT_NONE))
+        return fail(OBME_ERR_UNSUPPORTED,
+                    "the centred and WENO sinking schemes run without negative scaling and without a sediment");
     if (cfg->par.surface_mode == OBME_SURFACE_ARRAY && !cfg->par.surface_array)
         return fail(OBME_ERR_INVALID, "surface PAR array missing");
 
@@ -456,6 +466,8 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
         if (valid_sweep_waves(nw)) h->sweep_waves = nw;
     }
     h->scaling = cfg->scaling.n_groups > 0;
+    h->scheme = cfg->sinking.scheme;
+    h->staged = h->scheme != OBME_UPWIND1;
     h->co2_inline = h->scaling && cfg->model == OBME_LOBSTER && cfg->carbonate_system && cfg->co2.enabled;
     if (cfg->model == OBME_LOBSTER && cfg->oxygen.enabled && cfg->o2.enabled) {
         if (hipMalloc((void**)&h->d_o2c, 2 * (size_t)g.Nx * g.Ny * sizeof(double)) != hipSuccess) {
@@ -507,6 +519,8 @@ extern "C" obme_status obme_destroy(obme_handle* h)
     if (h->d_o2c) (void)hipFree(h->d_o2c);
     if (h->d_sedG) (void)hipFree(h->d_sedG);
     if (h->d_surfc) (void)hipFree(h->d_surfc);
+    if (h->d_stageD) (void)hipFree(h->d_stageD);
+    if (h->d_stageG) (void)hipFree(h->d_stageG);
     for (auto& v : h->timed)
         for (auto& p : v) {
             (void)hipEventDestroy(p.first);
@@ -577,7 +591,7 @@ static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers)
     a.ns = convert_scaling<FT, 16>(c.scaling);
     a.o2_on = (c.model == OBME_LOBSTER && c.oxygen.enabled && c.o2.enabled) ? 1 : 0;
     a.o2c = h->d_o2c;
-    if (h->co2_inline || a.o2_on) {
+    if (h->co2_inline || a.o2_on || h->staged) {
         a.Tprof = (c.temperature.mode == OBME_FIELD_PROFILE) ? h->d_Tprof : nullptr;
         a.Sprof = (c.salinity.mode == OBME_FIELD_PROFILE) ? h->d_Sprof : nullptr;
         a.Tconst = c.temperature.value;
@@ -675,7 +689,19 @@ static obme_status launch_lane(obme_handle* h, const StepArgs<FT>& a, hipStream_
     const int cw = 64 * h->sweep_waves;
     // one-dimensional: tile = xcd_tile(blockIdx.x), row-major over (i tile, j)
     const unsigned n_tiles = (unsigned)(((h->cfg.grid.Nx + cw - 1) / cw) * h->cfg.grid.Ny);
-    auto k = h->uniform ? k_column_lane<MODEL, CARB, FT, MODE, true> : k_column_lane<MODEL, CARB, FT, MODE, false>;
+    using Kernel = void (*)(StepArgs<FT>);
+    Kernel k = nullptr;
+    if constexpr (MODE == TILE_STAGE) {
+        // centred / WENO sinking, one stage per launch (the general-grid form)
+        k = (h->scheme == OBME_WENO5) ? k_column_lane<MODEL, CARB, FT, MODE, false, -1, 0, true, OBME_WENO5>
+                                      : k_column_lane<MODEL, CARB, FT, MODE, false, -1, 0, true, OBME_CENTERED2>;
+    } else {
+        k = h->uniform ? k_column_lane<MODEL, CARB, FT, MODE, true> : k_column_lane<MODEL, CARB, FT, MODE, false>;
+    }
+    if constexpr (MODE == TILE_TENDENCY) {
+        if (h->scheme == OBME_WENO5) k = k_column_lane<MODEL, CARB, FT, MODE, false, -1, 0, true, OBME_WENO5>;
+        if (h->scheme == OBME_CENTERED2) k = k_column_lane<MODEL, CARB, FT, MODE, false, -1, 0, true, OBME_CENTERED2>;
+    }
     // the fused step on rows of whole workgroups (Nx a multiple of 64 NW):
     // stores without a mask; FP32 compiled with the model's default formulations
     const bool whole = h->cfg.grid.Nx % cw == 0;
@@ -769,6 +795,16 @@ static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* 
     }
     a.T3 = (const FT*)T;
     a.S3 = (const FT*)S;
+    if constexpr (MODE == TILE_STAGE) {
+        // one stage: the CO2 flux by the top lane, the step's scratch
+        const obme_config& c = h->cfg;
+        a.stage = h->cur_stage;
+        a.co2_stage = (c.model == OBME_LOBSTER && c.carbonate_system && c.co2.enabled) ? 1 : 0;
+        a.co2_on = 0;
+        a.ncell = (unsigned)((size_t)c.grid.Nx * c.grid.Ny * c.grid.Nz);
+        a.Dsc = (FT*)h->d_stageD;
+        a.Gsc = (FT*)h->d_stageG;
+    }
     // the top-cell DIC and Alk residuals (FP32 fused step, CO2 by the surface kernel)
     float* topres = (MODE == TILE_STEP && a.co2_on) ? h->d_topres : nullptr;
     a.topres = topres;
@@ -881,7 +917,8 @@ extern "C" obme_status obme_step_with_sediment(obme_handle* h, void* const* trac
     // alone, which hold for the whole call: set them up once
     const obme_config& c = h->cfg;
     h->surfc_ready = false;
-    if (nsteps > 1 && c.model == OBME_LOBSTER && c.carbonate_system && c.co2.enabled && !h->co2_inline) {
+    if (nsteps > 1 && c.model == OBME_LOBSTER && c.carbonate_system && c.co2.enabled && !h->co2_inline &&
+        !h->staged) {
         const size_t ncol = (size_t)c.grid.Nx * c.grid.Ny;
         if (!h->d_surfc && hipMalloc((void**)&h->d_surfc, SURF_PACK * ncol * sizeof(double)) != hipSuccess)
             return fail(OBME_ERR_NOMEM, "surface constants");
@@ -911,6 +948,24 @@ extern "C" obme_status obme_step_with_sediment(obme_handle* h, void* const* trac
     }
     obme_status r = OBME_OK;
     h->o2c_ready = false;
+    if (h->staged && nsteps > 0) {
+        // centred / WENO sinking: each stage's face values need the stage
+        // inputs of the levels below, so a step is three launches of one stage
+        const size_t ncell = (size_t)c.grid.Nx * c.grid.Ny * c.grid.Nz;
+        const size_t es = (h->prec == OBME_FP64) ? 8 : 4;
+        if (!h->d_stageG && hipMalloc(&h->d_stageG, (size_t)h->ntracers * ncell * es) != hipSuccess)
+            return fail(OBME_ERR_NOMEM, "stage tendencies");
+        if (h->prec != OBME_FP64 && !h->d_stageD &&
+            hipMalloc(&h->d_stageD, (size_t)h->ntracers * ncell * es) != hipSuccess)
+            return fail(OBME_ERR_NOMEM, "stage increments");
+        for (int st = 0; st < nsteps && r == OBME_OK; st++)
+            for (int q = 0; q < 3 && r == OBME_OK; q++) {
+                h->cur_stage = q;
+                r = run_tile<TILE_STAGE>(h, tracers, nullptr, nullptr, T, S, t0 + st * dt, dt, s);
+            }
+        h->o2c_ready = false;
+        return r;
+    }
     for (int st = 0; st < nsteps && r == OBME_OK; st++)
         r = run_tile<TILE_STEP>(h, tracers, nullptr, nullptr, T, S, t0 + st * dt, dt, s, storage);
     h->surfc_ready = false;
@@ -966,6 +1021,9 @@ extern "C" obme_status obme_step_noalloc(obme_handle* h, void* const* tracers, c
     if (nsteps < 0) return fail(OBME_ERR_INVALID, "negative nsteps");
     for (int n = 0; n < h->ntracers; n++)
         if (!tracers[n]) return fail(OBME_ERR_INVALID, "null tracer pointer");
+    if (h->staged)
+        return fail(OBME_ERR_UNSUPPORTED, "obme_step_noalloc: the upwind sinking scheme only (the centred and WENO "
+                                          "steps keep per-cell scratch)");
     const obme_config& c = h->cfg;
     const size_t ncol = (size_t)c.grid.Nx * c.grid.Ny;
     const size_t es = (h->prec == OBME_FP64) ? 8 : 4;

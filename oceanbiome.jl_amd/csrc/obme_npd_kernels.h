@@ -154,6 +154,15 @@ struct StepArgs {
     double sed_cn, sed_cm;         // AB2 weights 1.5 + χ, 0.5 + χ of the first stage's step
     int sed_ab2;                   // the first stage's step is an AB2 (not an Euler) step
     int sed_init;                  // the sediment's first step ever is in this launch
+    // one RK3 stage per launch (TILE_STAGE): the stage, the CO2 flux computed
+    // by the top lane, and the per-cell scratch of the step (compact, tracer
+    // n at n * ncell, cell col + Nx*Ny*k): FP32 the step's increment so far,
+    // both the previous stage's tendencies
+    int stage;
+    int co2_stage;
+    unsigned ncell;
+    FT* Dsc;
+    FT* Gsc;
 };
 
 // Per-level geometry rows in `lev` (each Nz entries).
@@ -161,8 +170,68 @@ enum { LEV_ZC = 0, LEV_UP, LEV_DN, LEV_INVDZ, LEV_COUNT };
 // LEV_UP[k] = zc[k+1] - zf[k+1], LEV_DN[k] = zf[k+1] - zc[k] for k < Nz-1;
 // for k = Nz-1: LEV_UP = 0 and LEV_DN = zf[Nz] - zc[Nz-1] (the top half cell).
 
-// Modes of the column sweep.
-enum { TILE_STEP = 0, TILE_TENDENCY = 1, TILE_PAR = 2 };
+// Modes of the column sweep: the fused RK3 step, the tendencies (hook), the
+// PAR (hook), one RK3 stage with its substep (the step of the sinking schemes
+// whose face values need the levels below).
+enum { TILE_STEP = 0, TILE_TENDENCY = 1, TILE_PAR = 2, TILE_STAGE = 3 };
+
+// The value at face k of a sinking tracer reconstructed from above (w <= 0:
+// the upwind cell is k): fifth-order WENO from the cells k-2 .. k+2, third
+// order from k-1 .. k+1, with WENO-Z weights (Borges et al. 2008) on the
+// Jiang-Shu smoothness indicators, eps = 1e-8: the reconstructions of
+// Oceananigans' WENO(order = 5), whose order drops near a Bounded
+// boundary (WENO5 on faces 3 .. Nz-3, WENO3 on 2 and Nz-2, upwind on 1 and
+// Nz-1).
+template <typename FT>
+__device__ __forceinline__ FT weno5_from_above(FT cm2, FT cm1, FT c0, FT cp1, FT cp2)
+{
+    const FT e = FT(1e-8);
+    const FT q0 = (FT(2) * cp2 - FT(7) * cp1 + FT(11) * c0) / FT(6);
+    const FT q1 = (-cp1 + FT(5) * c0 + FT(2) * cm1) / FT(6);
+    const FT q2 = (FT(2) * c0 + FT(5) * cm1 - cm2) / FT(6);
+    const FT s0 = cp2 - FT(2) * cp1 + c0, t0 = cp2 - FT(4) * cp1 + FT(3) * c0;
+    const FT s1 = cp1 - FT(2) * c0 + cm1, t1 = cp1 - cm1;
+    const FT s2 = c0 - FT(2) * cm1 + cm2, t2 = FT(3) * c0 - FT(4) * cm1 + cm2;
+    const FT b0 = FT(13.0 / 12) * (s0 * s0) + FT(0.25) * (t0 * t0);
+    const FT b1 = FT(13.0 / 12) * (s1 * s1) + FT(0.25) * (t1 * t1);
+    const FT b2 = FT(13.0 / 12) * (s2 * s2) + FT(0.25) * (t2 * t2);
+    const FT tau = fabs(b0 - b2);
+    const FT r0 = tau / (b0 + e), r1 = tau / (b1 + e), r2 = tau / (b2 + e);
+    const FT a0 = FT(0.1) * (FT(1) + r0 * r0);
+    const FT a1 = FT(0.6) * (FT(1) + r1 * r1);
+    const FT a2 = FT(0.3) * (FT(1) + r2 * r2);
+    return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+}
+
+template <typename FT>
+__device__ __forceinline__ FT weno3_from_above(FT cm1, FT c0, FT cp1)
+{
+    const FT e = FT(1e-8);
+    const FT q0 = (-cp1 + FT(3) * c0) / FT(2);
+    const FT q1 = (c0 + cm1) / FT(2);
+    const FT d0 = c0 - cp1, d1 = cm1 - c0;
+    const FT b0 = d0 * d0, b1 = d1 * d1;
+    const FT tau = fabs(b0 - b1);
+    const FT r0 = tau / (b0 + e), r1 = tau / (b1 + e);
+    const FT a0 = FT(1.0 / 3) * (FT(1) + r0 * r0);
+    const FT a1 = FT(2.0 / 3) * (FT(1) + r1 * r1);
+    return (a0 * q0 + a1 * q1) / (a0 + a1);
+}
+
+// The face value at the bottom face k of cell k from the column's values
+// cm2 = c(k-2) .. cp2 = c(k+2) (entries past the column unused).
+template <int SCHEME, typename FT>
+__device__ __forceinline__ FT sink_face(int k, int Nz, FT cm2, FT cm1, FT c0, FT cp1, FT cp2)
+{
+    if (k <= 0) return c0;
+    if constexpr (SCHEME == OBME_CENTERED2) {
+        return (cm1 + c0) / FT(2);
+    } else {
+        if (k == 1 || k >= Nz - 1) return c0;
+        if (k == 2 || k == Nz - 2) return weno3_from_above(cm1, c0, cp1);
+        return weno5_from_above(cm2, cm1, c0, cp1, cp2);
+    }
+}
 
 // The column sweep.  Lane = water column: a workgroup of NW waves owns
 // NW*64 consecutive i-columns of one j row, so every load and store of a
@@ -179,9 +248,14 @@ enum { TILE_STEP = 0, TILE_TENDENCY = 1, TILE_PAR = 2 };
 // 4 the InstantRemineralisation sediment at the bottom level.  MASKX: some
 // lanes lie past the last column (Nx is not a multiple of the workgroup's
 // width); false compiles the stores without a mask.  PARK: the FP32
-// chlorophyll integrals are Kahan-compensated running sums.
+// chlorophyll integrals are Kahan-compensated running sums.  SCHEME: the
+// sinking face value, upwind (the fused step), or centred / WENO5, whose
+// face values need the levels below: those run one stage per launch
+// (TILE_STAGE) or as the tendency hook, reading the levels below a level
+// ahead (the inputs of the stage, not yet overwritten) and keeping the two
+// above in registers.
 template <int MODEL, int CARB, typename FT, int MODE, bool UNIFORM, int FM = -1, int EXTRA = 0, bool MASKX = true,
-          bool PARK = true>
+          bool PARK = true, int SCHEME = OBME_UPWIND1>
 __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
 {
     using TR = ModelTraits<MODEL, CARB>;
@@ -190,8 +264,14 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
     constexpr int S0 = TR::SINK0, S1 = TR::SINK1, PI = TR::P_IDX;
     constexpr bool F32 = sizeof(FT) == 4;
     constexpr int NSTAGE = (MODE == TILE_STEP) ? 3 : 1;
+    constexpr bool TS = MODE == TILE_STAGE;
+    constexpr bool WIN = SCHEME != OBME_UPWIND1;
+    // the CO2 flux computed here at the top level: with the scaling, and in the stage mode
+    constexpr bool INLINE_CO2 = ((EXTRA & 2) != 0) || (TS && (CARB & 1));
     static_assert(EXTRA == 0 || MODE == TILE_STEP, "scaling, inline CO2, sediment: fused step");
     static_assert(!(EXTRA & 2) || (CARB & 1), "inline CO2 needs the carbonate system");
+    static_assert(!WIN || MODE == TILE_STAGE || MODE == TILE_TENDENCY, "centred and WENO sinking: stage or hook mode");
+    static_assert(!TS || WIN, "the stage mode serves the centred and WENO sinking");
 
     const unsigned nbx = ((unsigned)a.Nx + (unsigned)cw - 1) / (unsigned)cw;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -237,11 +317,38 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
         Ir[s] = Ib[s] = Cr[s] = Cb[s] = FT(0);
     }
 
-    // inline CO2 (EXTRA & 2): [H+] of the previous stage (FP32-mixed warm start)
+    // inline CO2: [H+] of the previous stage (FP32-mixed warm start)
     double Hc = 0.0;
-    if constexpr ((EXTRA & 2) && F32) {
+    if constexpr (INLINE_CO2 && F32) {
         if (col_ok && a.Hprev) Hc = a.Hprev[col2d];
     }
+    // the stage mode: this launch's stage, and the scratch of level k at
+    // byte offset coff(k) from each tracer's scratch row
+    const int stg = TS ? a.stage : 0;
+    auto coff = [&](int k) -> unsigned { return (col2d + ncol2d * (unsigned)k) * ES; };
+    auto load_scratch = [&](FT (&dd)[NT], FT (&gg)[NT], int k) {
+        const unsigned off = coff(k);
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            dd[n] = (F32 && stg > 0 && n != RO) ? gload(a.Dsc + (size_t)n * a.ncell, off) : FT(0);
+            gg[n] = (stg > 0 && n != RO) ? gload(a.Gsc + (size_t)n * a.ncell, off) : FT(0);
+        }
+    };
+    // centred / WENO sinking: the sinking tracers' stage inputs of the two
+    // levels above (hs1: k+1, hs2: k+2), the face value of the face above,
+    // and the loads of the level two below
+    FT hs1[2] = {FT(0), FT(0)}, hs2[2] = {FT(0), FT(0)}, fup[2] = {FT(0), FT(0)};
+    auto load_sink_inputs = [&](FT (&dst)[2], int k) {
+        const unsigned off = lev_off(k);
+        dst[0] = gload(a.tr[S0], off);
+        dst[1] = gload(a.tr[S1], off);
+        if constexpr (TS && F32) {
+            if (stg > 0) {
+                dst[0] += gload(a.Dsc + (size_t)S0 * a.ncell, coff(k));
+                dst[1] += gload(a.Dsc + (size_t)S1 * a.ncell, coff(k));
+            }
+        }
+    };
     // sediment (EXTRA & 4): the storage value, the tendency e f of the last
     // stage start and the G⁻ of the sediment's previous step
     double sed_st = 0.0, sed_ge = 0.0, sed_gm = 0.0;
@@ -255,7 +362,9 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
     const double g1 = 8.0 / 15.0, g2 = 5.0 / 12.0, g3 = 3.0 / 4.0, z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
 
     FT nxt[NT];
+    FT nxtD[TS ? NT : 1], nxtG[TS ? NT : 1];
     load_level(nxt, ktop);
+    if constexpr (TS) load_scratch(nxtD, nxtG, ktop);
     // one level, all stages; top: the surface level, bot: the bottom level.
     // The plain fused step passes them as compile-time constants and
     // compiles the surface and the bottom level apart, so the levels between
@@ -275,8 +384,22 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
             U[n] = nxt[n];
             Gm[n] = FT(0);
             if constexpr (F32) D[n] = FT(0);
+            if constexpr (TS) {
+                // the step's increment so far and the previous stage's tendencies
+                if constexpr (F32) D[n] = nxtD[n];
+                Gm[n] = nxtG[n];
+            }
         }
-        if (!bot) load_level(nxt, k - 1);
+        // centred / WENO: the sinking tracers' stage inputs of the level
+        // two below (the level below comes with the prefetch)
+        FT ah2[2] = {FT(0), FT(0)};
+        if constexpr (WIN) {
+            if (k >= 2) load_sink_inputs(ah2, k - 2);
+        }
+        if (!bot) {
+            load_level(nxt, k - 1);
+            if constexpr (TS) load_scratch(nxtD, nxtG, k - 1);
+        }
 
         // geometry of level k (uniform across the wave)
         FT zc, up, dn, invdz, wlo0, whi0, wlo1, whi1;
@@ -312,7 +435,9 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
         FT PARv = FT(0);
         FT Gout[NT];   // TILE_TENDENCY: the transitions
 #pragma unroll
-        for (int st = 0; st < NSTAGE; st++) {
+        for (int si = 0; si < NSTAGE; si++) {
+            // si: the carried per-stage sums; st: the RK3 stage (one per launch in the stage mode)
+            const int st = TS ? stg : si;
             FT PAR0 = PAR0_const;
             if (a.surface_mode == OBME_SURFACE_DIURNAL) PAR0 = a.PAR0_stage[st];
 
@@ -376,33 +501,33 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
             FT ir, ib;
             if constexpr (F32 && UNIFORM) {
                 // dz/2 is folded into the attenuations (xr2h, xb2h)
-                ir = top ? pr : pup[st][0] + pr;
-                ib = top ? pb : pup[st][1] + pb;
+                ir = top ? pr : pup[si][0] + pr;
+                ib = top ? pb : pup[si][1] + pb;
             } else {
-                ir = top ? dn * pr : up * pup[st][0] + dn * pr;
-                ib = top ? dn * pb : up * pup[st][1] + dn * pb;
+                ir = top ? dn * pr : up * pup[si][0] + dn * pr;
+                ib = top ? dn * pb : up * pup[si][1] + dn * pb;
             }
             if constexpr (F32 && PARK) {
                 // Kahan summation: every increment is >= 0, so the running
                 // sum dominates and its lost low part is recovered exactly
-                const FT yr = ir - Cr[st], yb = ib - Cb[st];
-                const FT tr = Ir[st] + yr, tb = Ib[st] + yb;
-                Cr[st] = (tr - Ir[st]) - yr;
-                Cb[st] = (tb - Ib[st]) - yb;
-                Ir[st] = tr;
-                Ib[st] = tb;
+                const FT yr = ir - Cr[si], yb = ib - Cb[si];
+                const FT tr = Ir[si] + yr, tb = Ib[si] + yb;
+                Cr[si] = (tr - Ir[si]) - yr;
+                Cb[si] = (tb - Ib[si]) - yb;
+                Ir[si] = tr;
+                Ib[si] = tb;
             } else {
-                Ir[st] = Ir[st] + ir;
-                Ib[st] = Ib[st] + ib;
+                Ir[si] = Ir[si] + ir;
+                Ib[si] = Ib[si] + ib;
             }
-            pup[st][0] = pr;
-            pup[st][1] = pb;
+            pup[si][0] = pr;
+            pup[si][1] = pb;
             if constexpr (F32 && UNIFORM) {
-                PARv = (PAR0 * 0.5f) * (m_exp2(fmaf(-a.xr2h, Ir[st], ezr)) + m_exp2(fmaf(-a.xb2h, Ib[st], ezb)));
+                PARv = (PAR0 * 0.5f) * (m_exp2(fmaf(-a.xr2h, Ir[si], ezr)) + m_exp2(fmaf(-a.xb2h, Ib[si], ezb)));
             } else if constexpr (F32) {
-                PARv = (PAR0 * 0.5f) * (m_exp2(fmaf(-a.xr2, Ir[st], ezr)) + m_exp2(fmaf(-a.xb2, Ib[st], ezb)));
+                PARv = (PAR0 * 0.5f) * (m_exp2(fmaf(-a.xr2, Ir[si], ezr)) + m_exp2(fmaf(-a.xb2, Ib[si], ezb)));
             } else {
-                PARv = PAR0 * (m_exp(a.kr * zc - a.xr * Ir[st]) + m_exp(a.kb * zc - a.xb * Ib[st])) / FT(2);
+                PARv = PAR0 * (m_exp(a.kr * zc - a.xr * Ir[si]) + m_exp(a.kb * zc - a.xb * Ib[si])) / FT(2);
             }
             if constexpr (MODE == TILE_PAR) continue;
 
@@ -432,22 +557,47 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
             // minus the upwind drift flux divergence, w(face) c(upwind cell):
             // the inflow through the top face is w c of the level above
             // (no flux through the surface: w = 0 there, and Xup = 0)
-            if constexpr (F32 && UNIFORM) {
-                Gv[S0] = fmaf(-a.wi0, (top ? 0.0f : Xup[st][0]) - X[S0], Gv[S0]);
-                Gv[S1] = fmaf(-a.wi1, (top ? 0.0f : Xup[st][1]) - X[S1], Gv[S1]);
+            if constexpr (WIN) {
+                // centred / WENO: the face value of the bottom face from the
+                // stage inputs of the levels k-2 .. k+2; the top face's is
+                // the level above's bottom face
+                FT cm1[2] = {nxt[S0], nxt[S1]};
+                if constexpr (TS && F32) {
+                    if (stg > 0) {
+                        cm1[0] += nxtD[S0];
+                        cm1[1] += nxtD[S1];
+                    }
+                }
+                const FT c0[2] = {X[S0], X[S1]};
+                FT f[2];
+#pragma unroll
+                for (int q = 0; q < 2; q++) f[q] = sink_face<SCHEME>(k, Nz, ah2[q], cm1[q], c0[q], hs1[q], hs2[q]);
+                const FT Fhi0 = top ? FT(0) : whi0 * fup[0];
+                const FT Fhi1 = top ? FT(0) : whi1 * fup[1];
+                Gv[S0] = Gv[S0] - (Fhi0 - wlo0 * f[0]) * invdz;
+                Gv[S1] = Gv[S1] - (Fhi1 - wlo1 * f[1]) * invdz;
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    fup[q] = f[q];
+                    hs2[q] = hs1[q];
+                    hs1[q] = c0[q];
+                }
+            } else if constexpr (F32 && UNIFORM) {
+                Gv[S0] = fmaf(-a.wi0, (top ? 0.0f : Xup[si][0]) - X[S0], Gv[S0]);
+                Gv[S1] = fmaf(-a.wi1, (top ? 0.0f : Xup[si][1]) - X[S1], Gv[S1]);
             } else {
-                const FT Fhi0 = top ? FT(0) : whi0 * Xup[st][0];
-                const FT Fhi1 = top ? FT(0) : whi1 * Xup[st][1];
+                const FT Fhi0 = top ? FT(0) : whi0 * Xup[si][0];
+                const FT Fhi1 = top ? FT(0) : whi1 * Xup[si][1];
                 Gv[S0] = Gv[S0] - (Fhi0 - wlo0 * X[S0]) * invdz;
                 Gv[S1] = Gv[S1] - (Fhi1 - wlo1 * X[S1]) * invdz;
             }
-            Xup[st][0] = X[S0];
-            Xup[st][1] = X[S1];
+            Xup[si][0] = X[S0];
+            Xup[si][1] = X[S1];
 
-            if constexpr ((EXTRA & 2) != 0) {
+            if constexpr (INLINE_CO2) {
                 // the top-cell CO2 flux (GasExchange, gas_exchange.jl:26-38) of
                 // this stage's DIC and Alk, the surface kernel's expressions
-                if (top) {
+                if (top && (!TS || a.co2_stage)) {
                     const unsigned tc = a.base0 + (unsigned)i + a.sx * (unsigned)j + a.sxy * (unsigned)ktop;
                     const double T = a.T3 ? (double)a.T3[tc] : (a.Tprof ? a.Tprof[ktop] : a.Tconst);
                     const double S = a.S3 ? (double)a.S3[tc] : (a.Sprof ? a.Sprof[ktop] : a.Sconst);
@@ -542,6 +692,23 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
         };
         if constexpr (MODE == TILE_PAR) {
             put(a.PAR, PARv);
+        } else if constexpr (TS) {
+            // the stage's substep: FP64 the state in place; FP32 the step's
+            // increment to the scratch, the state U + D after the last stage;
+            // this stage's tendencies for the next
+            const unsigned co = col_ok ? coff(k) : a.ncell * ES;
+            auto putc = [&](FT* p, FT v) { buf_store(make_rsrc(p, a.ncell * ES), co, v); };
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                if (n == RO) continue;
+                if (stg < 2) putc(a.Gsc + (size_t)n * a.ncell, Gm[n]);
+                if constexpr (F32) {
+                    if (stg < 2) putc(a.Dsc + (size_t)n * a.ncell, D[n]);
+                    else put(a.tr[n], U[n] + D[n]);
+                } else {
+                    put(a.tr[n], U[n]);
+                }
+            }
         } else if constexpr (MODE == TILE_TENDENCY) {
             FT g0[NT];
 #pragma unroll
@@ -603,7 +770,7 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
 #pragma unroll 1
         for (int k = ktop; k >= 0; k--) level(k, k == ktop, k == 0);
     }
-    if constexpr ((EXTRA & 2) && F32) {
+    if constexpr (INLINE_CO2 && F32) {
         if (col_ok && a.Hprev) a.Hprev[col2d] = Hc;
     }
 }

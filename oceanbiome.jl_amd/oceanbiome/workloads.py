@@ -82,9 +82,10 @@ def npzd_2d(nsteps=100, dt=50.0, surface_PAR=100.0):
 
 
 def lobster_3d(Nx=512, Ny=512, Nz=64, nsteps=200, dt=60.0, seed=42, surface_PAR=100.0, gas_exchange=True,
-               Lx=1000.0, Ly=1000.0, Lz=200.0, oxygen=False):
+               Lx=1000.0, Ly=1000.0, Lz=200.0, oxygen=False, sinking_scheme="upwind"):
     """LOBSTER + CarbonateSystem on a periodic box with CO2 exchange; with
-    oxygen, also the O2 tracer (last) and its gas exchange."""
+    oxygen, also the O2 tracer (last) and its gas exchange; the sinking face
+    value of `sinking_scheme` ("upwind", "centered", "weno5")."""
     grid = RectilinearGrid(size=(Nx, Ny, Nz), extent=(Lx, Ly, Lz), halo=(3, 3, 3),
                            topology=("Periodic", "Periodic", "Bounded"))
     T_prof = 20 - 0.05 * np.abs(grid.zc)
@@ -93,7 +94,7 @@ def lobster_3d(Nx=512, Ny=512, Nz=64, nsteps=200, dt=60.0, seed=42, surface_PAR=
         ges.append(OxygenGasExchangeBoundaryCondition())
     bgc = LOBSTER(grid, carbonate_system=CarbonateSystem(), oxygen=Oxygen() if oxygen else None,
                   light_attenuation=TwoBandPhotosyntheticallyActiveRadiation(surface_PAR=surface_PAR),
-                  gas_exchange=tuple(ges), temperature=T_prof, salinity=35.0)
+                  gas_exchange=tuple(ges), temperature=T_prof, salinity=35.0, sinking_scheme=sinking_scheme)
     rng = np.random.Generator(np.random.PCG64(seed))
     shape = (Nz, Ny, Nx)
     medians = (10.0, 0.1, 0.1, 0.1, 0.05, 0.05, 0.05)
@@ -106,7 +107,8 @@ def lobster_3d(Nx=512, Ny=512, Nz=64, nsteps=200, dt=60.0, seed=42, surface_PAR=
         f = grid.new_field()
         grid.interior(f)[...] = rng.uniform(lo, hi, shape)
         fields.append(f)
-    name = f"lobster_dic_alk{'_o2' if oxygen else ''}_{Nx}x{Ny}x{Nz}"
+    name = f"lobster_dic_alk{'_o2' if oxygen else ''}{'' if sinking_scheme == 'upwind' else '_' + sinking_scheme}" \
+           f"_{Nx}x{Ny}x{Nz}"
     return Workload(name, bgc, grid, fields, dt, nsteps,
                     extra={"T_profile": T_prof, "S": 35.0})
 

@@ -81,9 +81,14 @@ def test_invalid_configs_are_rejected_before_device_work():
     assert st == abi.OBME_ERR_UNSUPPORTED
 
     cfg = _cfg(b)
-    cfg.sinking.scheme = abi.OBME_CENTERED2
+    cfg.sinking.scheme = 7
     st, _, msg = _create(cfg)
-    assert st == abi.OBME_ERR_UNSUPPORTED and "upwind" in msg
+    assert st == abi.OBME_ERR_INVALID and "sinking scheme" in msg
+
+    cfg = _cfg(NPZD(g, scale_negatives=True))
+    cfg.sinking.scheme = abi.OBME_WENO5
+    st, _, msg = _create(cfg)
+    assert st == abi.OBME_ERR_UNSUPPORTED and "without negative scaling" in msg
 
     cfg = _cfg(b)
     cfg.grid.Nz = 0
