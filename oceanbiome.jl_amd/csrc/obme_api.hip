@@ -210,6 +210,7 @@ struct obme_handle {
     int cur_stage = 0;         // the stage of the next TILE_STAGE launch
     void* d_stageD = nullptr;  // staged FP32: NT * Nx*Ny*Nz, the step's increment so far
     void* d_stageG = nullptr;  // staged: NT * Nx*Ny*Nz, the previous stage's tendencies
+    double* d_sflux = nullptr; // staged with the CO2 exchange: Nx*Ny top-cell fluxes of the stage
     double* d_sedG = nullptr;  // the sediment time stepper's Gⁿ, G⁻ (2 * Nx*Ny)
     long long sed_iter = 0;    // sediment steps taken (its clock's iteration)
     double sed_last_dt = HUGE_VAL;   // Δt of its last step (Inf before the first)
@@ -521,6 +522,7 @@ extern "C" obme_status obme_destroy(obme_handle* h)
     if (h->d_surfc) (void)hipFree(h->d_surfc);
     if (h->d_stageD) (void)hipFree(h->d_stageD);
     if (h->d_stageG) (void)hipFree(h->d_stageG);
+    if (h->d_sflux) (void)hipFree(h->d_sflux);
     for (auto& v : h->timed)
         for (auto& p : v) {
             (void)hipEventDestroy(p.first);
@@ -804,9 +806,10 @@ static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* 
         a.ncell = (unsigned)((size_t)c.grid.Nx * c.grid.Ny * c.grid.Nz);
         a.Dsc = (FT*)h->d_stageD;
         a.Gsc = (FT*)h->d_stageG;
+        a.sflux = h->d_sflux;
     }
     // the top-cell DIC and Alk residuals (FP32 fused step, CO2 by the surface kernel)
-    float* topres = (MODE == TILE_STEP && a.co2_on) ? h->d_topres : nullptr;
+    float* topres = ((MODE == TILE_STEP && a.co2_on) || (MODE == TILE_STAGE && a.co2_stage)) ? h->d_topres : nullptr;
     a.topres = topres;
     if (MODE != TILE_PAR && a.o2_on && !h->o2c_ready) {
         // the O2 exchange coefficients of this call's surface T and S (once
@@ -843,6 +846,23 @@ static obme_status run_typed(obme_handle* h, void* const* tracers, void* const* 
         a.sed_init = h->sed_iter == 0 ? 1 : 0;
         a.sed_ab2 = (!a.sed_init && a.sed_dt[0] == h->sed_last_dt) ? 1 : 0;
         sed_last = a.sed_dt[2];
+    }
+    if constexpr (MODE == TILE_STAGE) {
+        if (a.co2_stage) {
+            // the stage's top-cell CO2 flux per column, from the stage's inputs
+            SurfaceArgs<FT> sa = make_surface_args<FT>(h, tracers, T, S);
+            if (h->surfc_ready) {
+                sa.surfc = h->d_surfc;
+                sa.surfc_cols = h->surfc_cols;
+            }
+            sa.topres = a.topres;
+            sa.stage = a.stage;
+            sa.Dsc = a.Dsc;
+            sa.ncell = a.ncell;
+            sa.flux_out = h->d_sflux;
+            obme_status rs = launch_surface<FT, MODE>(h, sa, s);
+            if (rs != OBME_OK) return rs;
+        }
     }
     obme_status r = dispatch_model<FT, MODE>(h, a, s);
     if (r == OBME_OK && MODE == TILE_STEP && h->sediment) {
@@ -917,8 +937,8 @@ extern "C" obme_status obme_step_with_sediment(obme_handle* h, void* const* trac
     // alone, which hold for the whole call: set them up once
     const obme_config& c = h->cfg;
     h->surfc_ready = false;
-    if (nsteps > 1 && c.model == OBME_LOBSTER && c.carbonate_system && c.co2.enabled && !h->co2_inline &&
-        !h->staged) {
+    if ((nsteps > 1 || (h->staged && nsteps > 0)) && c.model == OBME_LOBSTER && c.carbonate_system &&
+        c.co2.enabled && !h->co2_inline) {
         const size_t ncol = (size_t)c.grid.Nx * c.grid.Ny;
         if (!h->d_surfc && hipMalloc((void**)&h->d_surfc, SURF_PACK * ncol * sizeof(double)) != hipSuccess)
             return fail(OBME_ERR_NOMEM, "surface constants");
@@ -958,6 +978,9 @@ extern "C" obme_status obme_step_with_sediment(obme_handle* h, void* const* trac
         if (h->prec != OBME_FP64 && !h->d_stageD &&
             hipMalloc(&h->d_stageD, (size_t)h->ntracers * ncell * es) != hipSuccess)
             return fail(OBME_ERR_NOMEM, "stage increments");
+        if (c.model == OBME_LOBSTER && c.carbonate_system && c.co2.enabled && !h->d_sflux &&
+            hipMalloc((void**)&h->d_sflux, (size_t)c.grid.Nx * c.grid.Ny * sizeof(double)) != hipSuccess)
+            return fail(OBME_ERR_NOMEM, "stage fluxes");
         for (int st = 0; st < nsteps && r == OBME_OK; st++)
             for (int q = 0; q < 3 && r == OBME_OK; q++) {
                 h->cur_stage = q;

@@ -160,6 +160,7 @@ struct StepArgs {
     // both the previous stage's tendencies
     int stage;
     int co2_stage;
+    const double* sflux;           // the stage's top-cell CO2 flux per column
     unsigned ncell;
     FT* Dsc;
     FT* Gsc;
@@ -266,8 +267,8 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
     constexpr int NSTAGE = (MODE == TILE_STEP) ? 3 : 1;
     constexpr bool TS = MODE == TILE_STAGE;
     constexpr bool WIN = SCHEME != OBME_UPWIND1;
-    // the CO2 flux computed here at the top level: with the scaling, and in the stage mode
-    constexpr bool INLINE_CO2 = ((EXTRA & 2) != 0) || (TS && (CARB & 1));
+    // the CO2 flux computed here at the top level (with the scaling)
+    constexpr bool INLINE_CO2 = (EXTRA & 2) != 0;
     static_assert(EXTRA == 0 || MODE == TILE_STEP, "scaling, inline CO2, sediment: fused step");
     static_assert(!(EXTRA & 2) || (CARB & 1), "inline CO2 needs the carbonate system");
     static_assert(!WIN || MODE == TILE_STAGE || MODE == TILE_TENDENCY, "centred and WENO sinking: stage or hook mode");
@@ -431,6 +432,14 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
                 if (top && a.topres) D[8] = a.topres[ncol2d + col2d];
             }
         }
+        if constexpr ((CARB & 1) && TS && F32) {
+            // the stage mode's top-cell DIC and Alk: the first stage starts
+            // from the residuals the previous step left
+            if (top && a.topres && stg == 0) {
+                D[7] = a.topres[col2d];
+                D[8] = a.topres[ncol2d + col2d];
+            }
+        }
 
         FT PARv = FT(0);
         FT Gout[NT];   // TILE_TENDENCY: the transitions
@@ -481,7 +490,8 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
 #pragma unroll
             for (int n = 0; n < NT; n++) {
                 // (the top-cell Alk includes its residual from the first stage on)
-                const bool inc = F32 && (st > 0 || ((CARB & 1) && MODE == TILE_STEP && top && n == 8));
+                const bool inc = F32 && (st > 0 || ((CARB & 1) && MODE == TILE_STEP && top && n == 8) ||
+                                         ((CARB & 1) && TS && top && (n == 7 || n == 8)));
                 X[n] = inc ? U[n] + D[F32 ? n : 0] : U[n];
             }
 
@@ -597,7 +607,7 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
             if constexpr (INLINE_CO2) {
                 // the top-cell CO2 flux (GasExchange, gas_exchange.jl:26-38) of
                 // this stage's DIC and Alk, the surface kernel's expressions
-                if (top && (!TS || a.co2_stage)) {
+                if (top) {
                     const unsigned tc = a.base0 + (unsigned)i + a.sx * (unsigned)j + a.sxy * (unsigned)ktop;
                     const double T = a.T3 ? (double)a.T3[tc] : (a.Tprof ? a.Tprof[ktop] : a.Tconst);
                     const double S = a.S3 ? (double)a.S3[tc] : (a.Sprof ? a.Sprof[ktop] : a.Sconst);
@@ -610,6 +620,11 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
                                                        F32 ? FP32_NEWTON_TOL : 1e-14);
                     Gv[7] = Gv[7] - (FT)F * a.invdz_top;
                 }
+            }
+            if constexpr (TS && (CARB & 1)) {
+                // the stage mode: the top-cell CO2 flux of this stage's DIC
+                // and Alk, solved per column before the launch (k_surface)
+                if (top && a.co2_stage) Gv[7] = Gv[7] - (FT)a.sflux[col2d] * a.invdz_top;
             }
             if constexpr ((CARB & 2) != 0) {
                 // the top-cell O2 flux (GasExchange for O2, gas_exchange.jl:26-38,
@@ -703,8 +718,16 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
                 if (n == RO) continue;
                 if (stg < 2) putc(a.Gsc + (size_t)n * a.ncell, Gm[n]);
                 if constexpr (F32) {
-                    if (stg < 2) putc(a.Dsc + (size_t)n * a.ncell, D[n]);
-                    else put(a.tr[n], U[n] + D[n]);
+                    if (stg < 2) {
+                        putc(a.Dsc + (size_t)n * a.ncell, D[n]);
+                    } else if ((CARB & 1) && top && a.topres && (n == 7 || n == 8)) {
+                        // U + D = s + e exactly (TwoSum): s stored, e to the next step
+                        const float u = U[n], d = D[n], sum = u + d, dv = sum - u;
+                        if (col_ok) a.topres[(n == 8 ? ncol2d : 0u) + col2d] = (u - (sum - dv)) + (d - dv);
+                        put(a.tr[n], sum);
+                    } else {
+                        put(a.tr[n], U[n] + D[n]);
+                    }
                 } else {
                     put(a.tr[n], U[n]);
                 }
@@ -801,6 +824,10 @@ struct SurfaceArgs {
     int mode;                      // 0: reference damped Newton; 1: mixed
     obme_co2_exchange co2;
     obme_carbon_solver solver;
+    // the stage mode (TILE_STAGE): the stage, and FP32 the step's increments so far
+    int stage;
+    const FT* Dsc;
+    unsigned ncell;
 };
 
 // The surface constants of every column (T and S of the top cell), once per
@@ -854,6 +881,23 @@ __global__ void __launch_bounds__(256) k_surface(SurfaceArgs<FT> a)
         F = surface_flux_warm(sc, a.solver, (double)a.DIC[top], (double)a.Alk[top], a.co2.air_concentration, a.mode,
                               H, pH, ev, sizeof(FT) == 4 ? FP32_NEWTON_TOL : 1e-14);
         a.GDIC[top] = a.GDIC[top] - (FT)F * a.invdz_top;
+    } else if constexpr (MODE == TILE_STAGE) {
+        // the flux of the stage's top-cell DIC and Alk (FP32: U + D exactly;
+        // the first stage's D is the residual the previous step left), into
+        // flux_out for the sweep
+        double dic = (double)a.DIC[top], alk = (double)a.Alk[top];
+        if constexpr (sizeof(FT) == 4) {
+            const unsigned ct = (unsigned)col + (unsigned)ncol * (unsigned)ktop;
+            if (a.stage > 0) {
+                dic += (double)a.Dsc[7 * (size_t)a.ncell + ct];
+                alk += (double)a.Dsc[8 * (size_t)a.ncell + ct];
+            } else if (a.topres) {
+                dic += (double)a.topres[col];
+                alk += (double)a.topres[ncol + col];
+            }
+        }
+        F = surface_flux_warm(sc, a.solver, dic, alk, a.co2.air_concentration, a.mode, H, pH, ev,
+                              sizeof(FT) == 4 ? FP32_NEWTON_TOL : 1e-14);
     } else {
         // the identical update expressions of the sweep, with the flux term
         FT U = a.srec[col];

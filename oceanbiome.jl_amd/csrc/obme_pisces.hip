@@ -39,9 +39,10 @@ struct PiscesArgs {
     FT* Gm;                 // previous-stage transitions [NW][ncell] (compact)
     FT* PAR;                // [4][ncell] compact: total, band 1, 2, 3
     FT* Omega;              // [ncell] compact
-    const double* oc;       // [OMEGA_CONST_FIELDS][ncell]: per-cell carbonate constants
+    const FT* oc;           // [OMEGA_CONST_FIELDS][ncell]: per-cell carbonate constants (the
+                            // element type: FP32-mixed Ω solves in FP64 on FP32-rounded constants)
     FT* tfac;               // [TFAC_FIELDS][ncell]: temperature factors of the FP64 step, or null
-    double* Hs;             // [ncell]: [H+] of the last stage (FP32-mixed warm start)
+    float* Hs;              // [ncell]: [H+] of the last stage (FP32-mixed warm start)
     int have_H;             // Hs holds the previous stage's [H+]
     double* sflux;          // [2][Nx*Ny]: top CO2 and O2 fluxes of the stage (gas exchange on)
     double* Hsurf;          // [Nx*Ny]: [H+] of the surface pCO2 solve (FP32-mixed warm start)
@@ -223,11 +224,11 @@ __global__ void __launch_bounds__(256) k_pisces_prep(PiscesArgs<FT> a)
     const double P = fabs(z) * g * 1026 / 100000;
     OmegaConst o;
     omega_constants((double)a.in[OBME_PI_T][p], (double)a.in[OBME_PI_S][p], P, o);
-    double* out = const_cast<double*>(a.oc);
+    FT* out = const_cast<FT*>(a.oc);
     const double f[OMEGA_CONST_FIELDS] = {o.conv, o.K1, o.K2, o.KB, o.KW, o.KS, o.KF, o.KSi,
                                           o.boron, o.sulfate, o.fluoride, o.ca, o.KSP};
 #pragma unroll
-    for (int n = 0; n < OMEGA_CONST_FIELDS; n++) out[(long)n * ncell + c] = f[n];
+    for (int n = 0; n < OMEGA_CONST_FIELDS; n++) out[(long)n * ncell + c] = (FT)f[n];
     if (a.tfac) {
         // the temperature factors of the transitions, with T fixed
         TFac<FT> t;
@@ -275,7 +276,7 @@ __global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES :
     A.mlPAR = a.mlPAR[col];
     if constexpr (OM == 1) {
         OmegaConst o;
-        const double* oc = a.oc + c;
+        const FT* oc = a.oc + c;
         o.conv = oc[0];
         o.K1 = oc[ncell];
         o.K2 = oc[2 * ncell];
@@ -295,12 +296,12 @@ __global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES :
         if constexpr (sizeof(FT) == 8) {
             H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
         } else {
-            H = a.have_H ? a.Hs[c] : (double)guess_H_fp32(cp);
+            H = a.have_H ? (double)a.Hs[c] : (double)guess_H_fp32(cp);
             // Ω enters the FP32 transitions: after a step below 1e-4 relative,
             // quadratic convergence leaves H within ~1e-8 of the root, below
             // the FP32 rounding of Ω
             if (!refine_H<true>(cp, H, ev, FP32_NEWTON_TOL)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
-            a.Hs[c] = H;
+            a.Hs[c] = (float)H;
         }
         A.Omega = (FT)omega_from_H(o, cp, H);
     } else {
@@ -453,10 +454,11 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
     // the lane's carbonate constants (T, S and pressure fixed) and the
     // previous stage's transitions live in LDS, each lane in its own slots,
     // to leave the registers to the state and the transitions
-    __shared__ double s_oc[OMEGA_CONST_FIELDS][PF_WAVES * 64];
+    __shared__ float s_oc[OMEGA_CONST_FIELDS][PF_WAVES * 64];
     __shared__ FT s_gm[NW][PF_WAVES * 64];
     __shared__ FT s_ts[2][PF_WAVES * 64];
-    __shared__ double s_h[PF_WAVES * 64];
+    __shared__ float s_h[PF_WAVES * 64];
+    __shared__ float s_tf[TFAC_FIELDS][PF_WAVES * 64];   // the lane's temperature factors (T is fixed)
     __shared__ double s_surf[PF_WAVES][SURF_PACK];   // the top lane's surface constants
 
     // stage in: thread t takes level kt = t / PF_WAVES of column ct = t %
@@ -472,10 +474,10 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
         FT stg[NTR];
 #pragma unroll
         for (int n = 0; n < NTR; n++) stg[n] = tok ? a.in[n][pt] : FT(0);
-        double so[OMEGA_CONST_FIELDS];
+        float so[OMEGA_CONST_FIELDS];
 #pragma unroll
-        for (int q = 0; q < OMEGA_CONST_FIELDS; q++) so[q] = tok ? a.oc[(unsigned long)q * ncell + ct_cell] : 1.0;
-        const double sh = (tok && f.have_H) ? a.Hs[ct_cell] : -1.0;
+        for (int q = 0; q < OMEGA_CONST_FIELDS; q++) so[q] = tok ? a.oc[(unsigned long)q * ncell + ct_cell] : 1.0f;
+        const float sh = (tok && f.have_H) ? a.Hs[ct_cell] : -1.0f;
         if (tok) {
 #pragma unroll
             for (int n = 0; n < NW; n++) s_gm[n][slot] = stg[n];
@@ -492,8 +494,16 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
     for (int n = 0; n < NW; n++) v[n] = act ? s_gm[n][tid] : FT(0);
     v[OBME_PI_T] = act ? s_ts[0][tid] : FT(0);
     v[OBME_PI_S] = act ? s_ts[1][tid] : FT(0);
-    double H = act ? s_h[tid] : -1.0;      // < 0: the FP32 first guess
+    double H = act ? (double)s_h[tid] : -1.0;      // < 0: the FP32 first guess
     __syncthreads();
+    {
+        // the temperature factors, once per step
+        TFac<FT> t;
+        temperature_factors(a.pp, v[OBME_PI_T], t);
+        const FT tv[TFAC_FIELDS] = {t.btN, t.btD, t.bZ, t.bM, t.doc, t.pom, t.rain, t.kfe, t.eq, t.t4};
+#pragma unroll
+        for (int q = 0; q < TFAC_FIELDS; q++) s_tf[q][tid] = tv[q];
+    }
 #pragma unroll
     for (int n = 0; n < NW; n++) s_gm[n][tid] = FT(0);
 
@@ -561,10 +571,11 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
         const FT thr = PAR0 * FT(1e-3);
         const FT lthr = m_log(thr);
         const FT Chl = v[OBME_PI_PCHL] + v[OBME_PI_DCHL];
+        const FT lChl = m_log2(Chl);   // Chl^e = 2^(e log2 Chl), one logarithm for the three bands
         FT band[3];
 #pragma unroll
         for (int n = 0; n < 3; n++) {
-            const FT att = a.kw[n] + a.chi[n] * m_pow(Chl, a.e[n]);
+            const FT att = a.kw[n] + a.chi[n] * m_exp2(a.e[n] * lChl);
             const FT S = seg_prefix<64>(act ? dzl * att : FT(0));
             band[n] = (PAR0 * f.div[n]) * m_exp(S);
         }
@@ -625,7 +636,16 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
         }
         FT G[NW];
         TFac<FT> tf;
-        temperature_factors(pps, v[OBME_PI_T], tf);
+        tf.btN = s_tf[0][tid];
+        tf.btD = s_tf[1][tid];
+        tf.bZ = s_tf[2][tid];
+        tf.bM = s_tf[3][tid];
+        tf.doc = s_tf[4][tid];
+        tf.pom = s_tf[5][tid];
+        tf.rain = s_tf[6][tid];
+        tf.kfe = s_tf[7][tid];
+        tf.eq = s_tf[8][tid];
+        tf.t4 = s_tf[9][tid];
         pisces_tendencies(pps, v, A, f.sk[st], tf, G);
         // flux-form upwind sinking: (w(k+1) c(k+1) - w(k) c(k)) / dz, the
         // level above from lane r - 1, w = 0 on the top face
@@ -655,7 +675,7 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
     // then the coalesced stores of the stage-in pattern
 #pragma unroll
     for (int n = 0; n < NW; n++) s_gm[n][tid] = v[n];
-    s_h[tid] = H;
+    s_h[tid] = (float)H;
     __syncthreads();
     if (tok) {
 #pragma unroll
@@ -838,9 +858,9 @@ static PiscesArgs<FT> pisces_args(const obme_pisces* h, double t)
     a.Omega = (FT*)h->d_Omega;
     a.zeu = (FT*)h->d_zeu;
     a.mlPAR = (FT*)h->d_mlPAR;
-    a.oc = (const double*)h->d_oc;
+    a.oc = (const FT*)h->d_oc;
     a.tfac = (FT*)h->d_tfac;
-    a.Hs = (double*)h->d_H;
+    a.Hs = (float*)h->d_H;
     a.sflux = (double*)h->d_sflux;
     a.Hsurf = (double*)h->d_Hsurf;
     a.co2 = c.co2;
@@ -1107,9 +1127,9 @@ extern "C" obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, in
         if (!tracers[n]) return obme_fail(OBME_ERR_INVALID, "null tracer pointer");
     if (nsteps == 0) return OBME_OK;
     hipStream_t s = (hipStream_t)stream;
-    if (!h->d_oc && hipMalloc(&h->d_oc, (size_t)OMEGA_CONST_FIELDS * h->ncell * sizeof(double)) != hipSuccess)
+    if (!h->d_oc && hipMalloc(&h->d_oc, (size_t)OMEGA_CONST_FIELDS * h->ncell * h->es) != hipSuccess)
         return obme_fail(OBME_ERR_NOMEM, "PISCES carbonate constants");
-    if (!h->d_H && hipMalloc(&h->d_H, (size_t)h->ncell * sizeof(double)) != hipSuccess)
+    if (!h->d_H && hipMalloc(&h->d_H, (size_t)h->ncell * sizeof(float)) != hipSuccess)
         return obme_fail(OBME_ERR_NOMEM, "PISCES [H+] store");
     if (h->prec == OBME_FP64 && !h->d_tfac &&
         hipMalloc(&h->d_tfac, (size_t)TFAC_FIELDS * h->ncell * sizeof(double)) != hipSuccess)
