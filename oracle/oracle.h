@@ -9,13 +9,19 @@
  * expressions left to right; the build uses -ffp-contract=off so no FMA is
  * formed where Julia would not form one.
  *
+ * Pins: the reference's own known answers (tests/golden/reference_values.json),
+ * the budgets of its tests, and closed forms at special states of every
+ * NPZD / LOBSTER transition term and of the sinking reconstructions, each
+ * shown to catch a planted mistake (tests/test_oracle_pins_npd.py,
+ * tests/test_oracle_sinking_pins.py).
+ *
  * Parity unpinned (no reference-held value or closed form fixes the bits):
- * orc_teos10_density away from the reference's doctest point; the exact
- * per-tracer tendency values of orc_npd_tendencies away from the budgets
- * the tests check; the RK3 coefficients of orc_column_run beyond the
- * amplification polynomial; the step sequence of orc_sediment_time_step
- * (Oceananigans' QuasiAdamsBashforth2) beyond the hand-worked sequence and
- * the nitrogen budget.
+ * orc_teos10_density away from the reference's doctest point; the RK3
+ * coefficients of orc_column_run beyond the amplification polynomial and
+ * the step sequence of orc_sediment_time_step (Oceananigans' RK3 and
+ * QuasiAdamsBashforth2, outside the reference tree) beyond the hand-worked
+ * sequence and the nitrogen budget; the WENO reconstruction against
+ * Oceananigans' own (the textbook scheme is pinned).
  */
 #ifndef OBME_ORACLE_H
 #define OBME_ORACLE_H

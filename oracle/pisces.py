@@ -12,9 +12,13 @@ products, float^float through pow, left-to-right evaluation of products.
 The calcite saturation per cell uses the C oracle's carbonate chemistry
 (pinned separately by the reference's doctests and known answers).
 
-Parity unpinned: the exact per-tracer transition values away from the
-conservation balances and known answers the tests check, and the Morel
-coefficient table beyond the ranges of its band means.
+Pins: the element balances and known answers of the reference's PISCES
+tests, closed forms of every helper and of single processes of the
+transitions at special states (each shown to catch a planted mistake,
+tests/test_oracle_pisces_pins.py), and the Morel coefficient table against
+a fixture parsed from the reference's morel_coefficients.jl
+(tests/golden/make_morel_fixture.py).  Parity unpinned: the carbonate
+chemistry's TEOS-10 density away from the doctest point (oracle.h).
 """
 import math
 
