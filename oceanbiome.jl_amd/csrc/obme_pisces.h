@@ -517,6 +517,15 @@ struct OmegaConst {
 };
 constexpr int OMEGA_CONST_FIELDS = 13;
 
+// The salinity totals and the calcium concentration (functions of S alone).
+__device__ __forceinline__ void omega_salinity_terms(double S, OmegaConst& c)
+{
+    c.boron = 0.000232 / 10.811 * S / 1.80655;
+    c.sulfate = 0.14 / 96.06 * S / 1.80655;
+    c.fluoride = 0.000067 / 18.9984 * S / 1.80655;
+    c.ca = 0.0103 * S / 35;
+}
+
 // calcite_saturation (CarbonChemistry/calcite_concentration.jl:1-79) with
 // silicate and the pressure P in bar: constants pressure-corrected, density
 // at P.  First half: the constants.
@@ -530,9 +539,7 @@ __device__ __forceinline__ void omega_constants(double T, double S, double P, Om
     static constexpr double PC_KF[5] = {-9.78, -0.0090, -0.000942, -0.00391, 0.000054};
     static constexpr double PC_CAL[5] = {-48.76, 0.5304, -0.0, -0.01176, 0.0003692};
     CarbonParams p;
-    c.boron = 0.000232 / 10.811 * S / 1.80655;
-    c.sulfate = 0.14 / 96.06 * S / 1.80655;
-    c.fluoride = 0.000067 / 18.9984 * S / 1.80655;
+    omega_salinity_terms(S, c);
     const double rho = teos10_density(T, S, P);
     const double Tk = T + 273.15;
     c.conv = 1e-3 / rho;
@@ -544,7 +551,6 @@ __device__ __forceinline__ void omega_constants(double T, double S, double P, Om
     c.KS = p.KS * pressure_correction(PC_KS, Tk, P);
     c.KF = p.KF * pressure_correction(PC_KF, Tk, P);
     c.KSi = p.KSi;
-    c.ca = 0.0103 * S / 35;
     const double lnK_therm = -171.9065 + -0.077993 * Tk + 2839.319 / Tk + 71.595 * log10(Tk);
     const double lnK_sea = ((-0.77712 + 0.0028426 * Tk + 178.34 / Tk) * sqrt(S) + -0.07711 * S + 0.0041249 * pow(S, 1.5));
     c.KSP = pressure_correction(PC_CAL, Tk, P) * pow(10.0, lnK_therm + lnK_sea);

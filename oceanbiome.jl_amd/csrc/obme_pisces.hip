@@ -429,6 +429,13 @@ constexpr int PF_WAVES = OBME_PISCES_PF_WAVES;
 #define OBME_PISCES_PF_OCC 2
 #endif
 
+// the carbonate constants the fused step stages in (the indices of
+// conv, K1, K2, KB, KW, KS, KF, KSi, KSP in k_pisces_prep's order); the
+// salinity terms are formed from S in the kernel (16 B less to read per
+// cell and step)
+constexpr int FUSED_OMEGA_FIELDS = 9;
+__device__ __forceinline__ constexpr int fused_omega_field(int q) { return q < 8 ? q : 12; }
+
 template <bool SCALE>
 __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fused(PiscesArgs<float> a, PiscesFused f)
 {
@@ -454,12 +461,13 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
     // the lane's carbonate constants (T, S and pressure fixed) and the
     // previous stage's transitions live in LDS, each lane in its own slots,
     // to leave the registers to the state and the transitions
-    __shared__ float s_oc[OMEGA_CONST_FIELDS][PF_WAVES * 64];
+    // the constants that depend on T and P (the salinity terms come from S)
+    __shared__ float s_oc[FUSED_OMEGA_FIELDS][PF_WAVES * 64];
     __shared__ FT s_gm[NW][PF_WAVES * 64];
     __shared__ FT s_ts[2][PF_WAVES * 64];
     __shared__ float s_h[PF_WAVES * 64];
-    __shared__ float s_tf[TFAC_FIELDS][PF_WAVES * 64];   // the lane's temperature factors (T is fixed)
     __shared__ double s_surf[PF_WAVES][SURF_PACK];   // the top lane's surface constants
+    __shared__ float s_tf[TFAC_FIELDS][PF_WAVES * 64];   // the lane's temperature factors (T is fixed)
 
     // stage in: thread t takes level kt = t / PF_WAVES of column ct = t %
     // PF_WAVES (PF_WAVES neighbouring values along i per level: coalesced)
@@ -474,9 +482,10 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
         FT stg[NTR];
 #pragma unroll
         for (int n = 0; n < NTR; n++) stg[n] = tok ? a.in[n][pt] : FT(0);
-        float so[OMEGA_CONST_FIELDS];
+        float so[FUSED_OMEGA_FIELDS];
 #pragma unroll
-        for (int q = 0; q < OMEGA_CONST_FIELDS; q++) so[q] = tok ? a.oc[(unsigned long)q * ncell + ct_cell] : 1.0f;
+        for (int q = 0; q < FUSED_OMEGA_FIELDS; q++)
+            so[q] = tok ? a.oc[(unsigned long)fused_omega_field(q) * ncell + ct_cell] : 1.0f;
         const float sh = (tok && f.have_H) ? a.Hs[ct_cell] : -1.0f;
         if (tok) {
 #pragma unroll
@@ -484,7 +493,7 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
             s_ts[0][slot] = stg[OBME_PI_T];
             s_ts[1][slot] = stg[OBME_PI_S];
 #pragma unroll
-            for (int q = 0; q < OMEGA_CONST_FIELDS; q++) s_oc[q][slot] = so[q];
+            for (int q = 0; q < FUSED_OMEGA_FIELDS; q++) s_oc[q][slot] = so[q];
             s_h[slot] = sh;
         }
     }
@@ -622,11 +631,8 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
             o.KS = s_oc[5][tid];
             o.KF = s_oc[6][tid];
             o.KSi = s_oc[7][tid];
-            o.boron = s_oc[8][tid];
-            o.sulfate = s_oc[9][tid];
-            o.fluoride = s_oc[10][tid];
-            o.ca = s_oc[11][tid];
-            o.KSP = s_oc[12][tid];
+            o.KSP = s_oc[8][tid];
+            omega_salinity_terms((double)v[OBME_PI_S], o);
             const CarbonParams cp =
                 omega_params(o, (double)v[OBME_PI_DIC], (double)v[OBME_PI_ALK], (double)v[OBME_PI_SI]);
             int ev = 0;
