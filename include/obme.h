@@ -331,7 +331,7 @@ obme_status obme_step(obme_handle* h, void* const* tracers,
  * allocates; it calls no hipMalloc or hipFree (the handle itself comes from
  * obme_create).  A grid whose scratch exceeds 64 MiB is rejected with
  * OBME_ERR_INVALID (FP32-mixed LOBSTER + DIC/Alk with CO2 exchange takes
- * 44 B per column, about 1.5 million columns).  The warm starts begin at
+ * 40 B per column, about 1.6 million columns).  The warm starts begin at
  * zero on every call, so FP32-mixed results differ from obme_step's within
  * the root solve's tolerance; FP64 results are the same.  Every call on a
  * device shares that device's array: calls that overlap on one device race.

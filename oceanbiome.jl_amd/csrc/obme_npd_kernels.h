@@ -89,9 +89,9 @@ struct ModelTraits<OBME_LOBSTER, 3> {
 };
 
 // Surface record written by the sweep for the surface kernel, per column:
-// [0] DIC at the start of the step, [1+s] DIC transition without the flux
-// at stage s, [4+s] Alk at the start of stage s (s = 0, 1, 2).
-constexpr int SURF_REC = 7;
+// [s] DIC transition without the flux at stage s, [3+s] Alk at the start of
+// stage s (s = 0, 1, 2).
+constexpr int SURF_REC = 6;
 
 // Kernel arguments (passed by value).
 template <typename FT>
@@ -426,8 +426,9 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
             ezb = a.kb2 * (float)zc;
         }
         if constexpr ((CARB & 1) && MODE == TILE_STEP) {
-            if (top && col_ok && a.co2_on) a.srec[col2d] = U[7];
             // the top-cell Alk carries its residual from the previous step
+            // (the top-cell DIC is the surface kernel's: it reads the step's
+            // start value from the tracer, which the sweep does not write)
             if constexpr (F32) {
                 if (top && a.topres) D[8] = a.topres[ncol2d + col2d];
             }
@@ -672,8 +673,8 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
                 if constexpr (CARB & 1) {
                     // surface record: DIC transition without the flux, Alk at the stage start
                     if (top && col_ok && a.co2_on) {
-                        a.srec[(1 + st) * ncol2d + col2d] = Gv[7];
-                        a.srec[(4 + st) * ncol2d + col2d] = X[8];
+                        a.srec[st * ncol2d + col2d] = Gv[7];
+                        a.srec[(3 + st) * ncol2d + col2d] = X[8];
                     }
                 }
                 // RK3 substep (BoxModel/timesteppers.jl:79-92 form):
@@ -899,8 +900,10 @@ __global__ void __launch_bounds__(256) k_surface(SurfaceArgs<FT> a)
         F = surface_flux_warm(sc, a.solver, dic, alk, a.co2.air_concentration, a.mode, H, pH, ev,
                               sizeof(FT) == 4 ? FP32_NEWTON_TOL : 1e-14);
     } else {
-        // the identical update expressions of the sweep, with the flux term
-        FT U = a.srec[col];
+        // the identical update expressions of the sweep, with the flux term;
+        // the step starts from the top-cell DIC the previous step's surface
+        // kernel wrote (the sweep leaves it alone)
+        FT U = a.DIC[top];
         FT Gm = FT(0);
         if constexpr (sizeof(FT) == 4) {
             // the increment starts from the residual the previous step left
@@ -908,8 +911,8 @@ __global__ void __launch_bounds__(256) k_surface(SurfaceArgs<FT> a)
             float D = a.topres ? a.topres[col] : 0.0f;
 #pragma unroll
             for (int st = 0; st < 3; st++) {
-                const float Gb = a.srec[(1 + st) * ncol + col];
-                const float Alk = a.srec[(4 + st) * ncol + col];
+                const float Gb = a.srec[st * ncol + col];
+                const float Alk = a.srec[(3 + st) * ncol + col];
                 F = surface_flux_warm(sc, a.solver, (double)U + (double)D, (double)Alk, a.co2.air_concentration,
                                       a.mode, H, pH, ev, FP32_NEWTON_TOL);
                 const float G = Gb - (float)F * a.invdz_top;
@@ -925,8 +928,8 @@ __global__ void __launch_bounds__(256) k_surface(SurfaceArgs<FT> a)
             const FT z2 = FT(-17.0 / 60.0), z3 = FT(-5.0 / 12.0);
 #pragma unroll
             for (int st = 0; st < 3; st++) {
-                FT Gb = a.srec[(1 + st) * ncol + col];
-                FT Alk = a.srec[(4 + st) * ncol + col];
+                FT Gb = a.srec[st * ncol + col];
+                FT Alk = a.srec[(3 + st) * ncol + col];
                 F = surface_flux_warm(sc, a.solver, (double)U, (double)Alk, a.co2.air_concentration, a.mode, H, pH,
                                       ev);
                 FT G = Gb - (FT)F * a.invdz_top;
