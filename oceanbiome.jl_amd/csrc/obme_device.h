@@ -824,6 +824,9 @@ __device__ __forceinline__ void alk_residual_and_derivative_fast(double H, const
 // a step below 1e-4 relative, H is within ~1e-8 of the root (measured on
 // the CPU oracle from warm starts off by up to 1e-2: at most 9e-9).
 constexpr double FP32_NEWTON_TOL = 1e-4;
+// The stop of the FP64 warm-started refinements: a step below 1e-14
+// relative leaves H at the root to rounding.
+constexpr double FP64_NEWTON_TOL = 1e-14;
 template <bool SI = false>
 __device__ __forceinline__ bool refine_H(const CarbonParams& p, double& x, int& evals, double tol = 1e-14)
 {

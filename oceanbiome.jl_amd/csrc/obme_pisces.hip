@@ -291,18 +291,19 @@ __global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES :
         o.ca = oc[11 * ncell];
         o.KSP = oc[12 * ncell];
         const CarbonParams cp = omega_params(o, (double)v[OBME_PI_DIC], (double)v[OBME_PI_ALK], (double)v[OBME_PI_SI]);
+        // Newton from the previous stage's [H+] (stored in FP32; the first
+        // stage of a call from the FP32 guess), with the reference solve
+        // for a cell whose refinement fails.  FP32-mixed stops after a step
+        // below 1e-4 relative: quadratic convergence leaves H within ~1e-8 of
+        // the root, below the FP32 rounding of Ω.  FP64 stops after a step
+        // below 1e-14 relative: H at the root to rounding, where the
+        // reference solve from pH 8 also ends, after many more iterations
+        // (and a wave waits for its slowest lane)
         int ev = 0;
-        double H;
-        if constexpr (sizeof(FT) == 8) {
+        double H = a.have_H ? (double)a.Hs[c] : (double)guess_H_fp32(cp);
+        if (!refine_H<true>(cp, H, ev, sizeof(FT) == 8 ? FP64_NEWTON_TOL : FP32_NEWTON_TOL))
             H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
-        } else {
-            H = a.have_H ? (double)a.Hs[c] : (double)guess_H_fp32(cp);
-            // Ω enters the FP32 transitions: after a step below 1e-4 relative,
-            // quadratic convergence leaves H within ~1e-8 of the root, below
-            // the FP32 rounding of Ω
-            if (!refine_H<true>(cp, H, ev, FP32_NEWTON_TOL)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
-            a.Hs[c] = (float)H;
-        }
+        a.Hs[c] = (float)H;
         A.Omega = (FT)omega_from_H(o, cp, H);
     } else {
         A.Omega = a.Omega[c];
