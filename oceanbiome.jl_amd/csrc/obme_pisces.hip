@@ -442,7 +442,9 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
 {
     using FT = float;
     const int r = threadIdx.x & 63;
-    const int wv = threadIdx.x >> 6;
+    // the wave's column, as a scalar: the values derived from it (the column
+    // index, its surface PAR and gas constants) stay out of the vector registers
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const unsigned nbx = ((unsigned)a.Nx + PF_WAVES - 1) / PF_WAVES;
     const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
     const int j = (int)(tile / nbx);
@@ -468,6 +470,7 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
     __shared__ FT s_ts[2][PF_WAVES * 64];
     __shared__ float s_h[PF_WAVES * 64];
     __shared__ double s_surf[PF_WAVES][SURF_PACK];   // the top lane's surface constants
+    __shared__ double s_hsurf[PF_WAVES];             // and its [H+] warm start (not held in registers)
     __shared__ float s_tf[TFAC_FIELDS][PF_WAVES * 64];   // the lane's temperature factors (T is fixed)
 
     // stage in: thread t takes level kt = t / PF_WAVES of column ct = t %
@@ -477,8 +480,9 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
     const bool tok = kt < Nz && ib + ct < a.Nx;
     const int slot = ct * 64 + (Nz - 1 - (kt < Nz ? kt : 0));
     const unsigned pt = pidx(a, tok ? ib + ct : 0, j, tok ? kt : 0);
-    const unsigned long ct_cell = (unsigned long)(tok ? ib + ct : 0) + (unsigned long)a.Nx * (unsigned)j +
-                                  (unsigned long)ncol * (unsigned)(tok ? kt : 0);
+    // (32-bit: a field holds fewer than 2^31 cells, checked at create)
+    const unsigned ct_cell = (unsigned)(tok ? ib + ct : 0) + (unsigned)a.Nx * (unsigned)j +
+                             ncol * (unsigned)(tok ? kt : 0);
     {
         FT stg[NTR];
 #pragma unroll
@@ -486,7 +490,7 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
         float so[FUSED_OMEGA_FIELDS];
 #pragma unroll
         for (int q = 0; q < FUSED_OMEGA_FIELDS; q++)
-            so[q] = tok ? a.oc[(unsigned long)fused_omega_field(q) * ncell + ct_cell] : 1.0f;
+            so[q] = tok ? (a.oc + (unsigned long)fused_omega_field(q) * ncell)[ct_cell] : 1.0f;
         const float sh = (tok && f.have_H) ? a.Hs[ct_cell] : -1.0f;
         if (tok) {
 #pragma unroll
@@ -548,11 +552,11 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
     }
 
     const bool top = r == 0 && colok;
-    double Hsurf = (top && a.co2.enabled) ? a.Hsurf[col] : 0.0;
     const bool gas = a.co2.enabled || a.o2.enabled;
     if (top && gas) {
 #pragma unroll
         for (int q = 0; q < SURF_PACK; q++) s_surf[wv][q] = f.surf[(unsigned long)q * ncol + col];
+        s_hsurf[wv] = a.co2.enabled ? a.Hsurf[col] : 0.0;
     }
 
 #pragma unroll 1
@@ -568,10 +572,11 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
             double kO2, airO2;
             surface_unpack(s_surf[wv], scb, kO2, airO2);
             if (a.co2.enabled) {
-                double pH = 0.0;
+                double pH = 0.0, Hsurf = s_hsurf[wv];
                 int ev = 0;
                 Fc = surface_flux_warm(scb, a.solver, (double)v[OBME_PI_DIC], (double)v[OBME_PI_ALK],
                                        a.co2.air_concentration, 1, Hsurf, pH, ev, FP32_NEWTON_TOL);
+                s_hsurf[wv] = Hsurf;
             }
             if (a.o2.enabled) Fo = kO2 * ((double)v[OBME_PI_O2] - airO2);
         }
@@ -684,12 +689,22 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
     for (int n = 0; n < NW; n++) s_gm[n][tid] = v[n];
     s_h[tid] = (float)H;
     __syncthreads();
-    if (tok) {
+    {
+        // the stage-in indices again, from a copy of the thread index the
+        // compiler cannot see through: they are not held in registers through
+        // the stage loop
+        int t2 = tid;
+        asm volatile("" : "+v"(t2));
+        const int kt2 = t2 / PF_WAVES, ct2 = t2 % PF_WAVES;
+        if (kt2 < Nz && ib + ct2 < a.Nx) {
+            const int slot2 = ct2 * 64 + (Nz - 1 - kt2);
+            const unsigned pt2 = pidx(a, ib + ct2, j, kt2);
 #pragma unroll
-        for (int n = 0; n < NW; n++) a.out[n][pt] = s_gm[n][slot];
-        a.Hs[ct_cell] = s_h[slot];
+            for (int n = 0; n < NW; n++) a.out[n][pt2] = s_gm[n][slot2];
+            a.Hs[(unsigned)(ib + ct2) + (unsigned)a.Nx * (unsigned)j + ncol * (unsigned)kt2] = s_h[slot2];
+        }
     }
-    if (top && a.co2.enabled) a.Hsurf[col] = Hsurf;
+    if (top && a.co2.enabled) a.Hsurf[col] = s_hsurf[wv];
 }
 
 }  // namespace
