@@ -33,6 +33,26 @@ __device__ __forceinline__ FT eps0()
     else return 1.17549435e-38f;  // the smallest normal float keeps the FP32 guards finite
 }
 
+// The quotients of the transitions.  FP64: the IEEE quotient (the
+// reference's expressions, unchanged).  FP32-mixed: the dividend times the
+// hardware reciprocal (within 2 ulp; two instructions, where the compiler's
+// quotient scales both operands for denormal and huge divisors, which these
+// divisors are not: tracer sums guarded by the smallest normal, parameters).
+template <typename FT>
+__device__ __forceinline__ FT qdiv(FT a, FT b)
+{
+    if constexpr (sizeof(FT) == 8) return a / b;
+    else return a * __builtin_amdgcn_rcpf(b);
+}
+
+// a / k for a constant k (FP32-mixed: times 1/k, folded at compile time).
+template <typename FT>
+__device__ __forceinline__ FT qdivk(FT a, double k)
+{
+    if constexpr (sizeof(FT) == 8) return a / k;
+    else return a * (float)(1.0 / k);
+}
+
 template <typename FT>
 __device__ __forceinline__ FT jmin3(FT a, FT b, FT c) { return jmin(jmin(a, b), c); }
 template <typename FT>
@@ -99,10 +119,10 @@ __device__ __forceinline__ void temperature_factors(const FT* pp, FT T, TFac<FT>
     f.doc = m_pow(sc[OBME_PS_DOC_B], T);
     f.pom = m_pow(sc[OBME_PS_POM_B], T);
     const FT Tm = T - 10;
-    f.rain = 1 + m_exp(-(Tm * Tm) / 25);
-    f.kfe = m_exp(FT(16.27) - FT(1565.7) / jmax(T + FT(273.15), FT(5)));
-    f.eq = m_pow(FT(10), FT(6.44) - 968 / (T + FT(273.15)));
-    f.t4 = m_pow(1 + T / 400, FT(4));
+    f.rain = 1 + m_exp(qdivk(-(Tm * Tm), 25));
+    f.kfe = m_exp(FT(16.27) - qdiv(FT(1565.7), jmax(T + FT(273.15), FT(5))));
+    f.eq = m_pow(FT(10), FT(6.44) - qdiv(FT(968), T + FT(273.15)));
+    f.t4 = m_pow(1 + qdivk(T, 400), FT(4));
 }
 
 // One phytoplankton class (parameters at block b of pp).
@@ -119,13 +139,13 @@ __device__ __forceinline__ FT size_factor(const FT* ph, FT I)
     const FT Im = ph[OBME_PH_SIZE_THRESHOLD];
     FT I1 = jmin(I, Im);
     FT I2 = jmax(FT(0), I - Im);
-    return (I1 + ph[OBME_PH_SIZE_RATIO] * I2) / (I1 + I2 + eps0<FT>());
+    return qdiv(I1 + ph[OBME_PH_SIZE_RATIO] * I2, I1 + I2 + eps0<FT>());
 }
 
 template <typename FT>
 __device__ __forceinline__ FT nitrogen_lim(FT N1, FT N2, FT K1, FT K2)
 {
-    return (K2 * N1) / (K1 * K2 + K1 * N2 + K2 * N1 + eps0<FT>());
+    return qdiv(K2 * N1, K1 * K2 + K1 * N2 + K2 * N1 + eps0<FT>());
 }
 
 // NitrogenIronPhosphateSilicateLimitation (nutrient_limitation.jl:20-71) and
@@ -138,36 +158,36 @@ __device__ __forceinline__ void phyto_state(const FT* ph, const FT* sc, const FT
     s.I = I;
     s.IChl = IChl;
     s.IFe = IFe;
-    const FT thFe = (I == FT(0)) ? FT(0) : IFe / (I + eps0<FT>());
-    const FT thChl = (I == FT(0)) ? FT(0) : IChl / (12 * I + eps0<FT>());
+    const FT thFe = (I == FT(0)) ? FT(0) : qdiv(IFe, I + eps0<FT>());
+    const FT thChl = (I == FT(0)) ? FT(0) : qdiv(IChl, 12 * I + eps0<FT>());
     const FT Kb = size_factor(ph, I);
     const FT Kno = ph[OBME_PH_KNO3] * Kb, Knh = ph[OBME_PH_KNH4] * Kb, Kp = ph[OBME_PH_KPO4] * Kb,
              Ksi = ph[OBME_PH_KSI_MIN] * Kb;
     s.LNO3 = nitrogen_lim(NO3, NH4, Kno, Knh);
     s.LNH4 = nitrogen_lim(NH4, NO3, Knh, Kno);
     s.LN = s.LNO3 + s.LNH4;
-    s.LPO4 = PO4 / (PO4 + Kp + eps0<FT>());
+    s.LPO4 = qdiv(PO4, PO4 + Kp + eps0<FT>());
     const FT thm = FT(1000) * (FT(0.0016) / FT(55.85) * FT(12) * thChl
                                + FT(1.5) * FT(1.21e-5) * FT(14) / (FT(55.85) * FT(7.625)) * s.LN
                                + FT(1.15e-4) * FT(14) / (FT(55.85) * FT(7.625)) * s.LNO3);
-    s.LFe = jmin(FT(1), jmax(FT(0), (thFe - thm) / ph[OBME_PH_THETA_FE_OPT]));
+    s.LFe = jmin(FT(1), jmax(FT(0), qdiv(thFe - thm, ph[OBME_PH_THETA_FE_OPT])));
     const FT Sc = sc[OBME_PS_SILICATE_CLIMATOLOGY];
     const FT pk = ph[OBME_PH_PK];
-    const FT KSi = Ksi + 7 * (Sc * Sc) / (pk * pk + Sc * Sc);
-    FT LSi = Si / (Si + KSi);
+    const FT KSi = Ksi + qdiv(7 * (Sc * Sc), pk * pk + Sc * Sc);
+    FT LSi = qdiv(Si, Si + KSi);
     if (ph[OBME_PH_SI_LIMITED] == FT(0)) LSi = FT(INFINITY);
     s.L = jmin4(s.LN, s.LPO4, s.LFe, LSi);
     // growth rate μ(L)
     const FT PAR = ph[OBME_PH_BETA1] * A.PAR1 + ph[OBME_PH_BETA2] * A.PAR2 + ph[OBME_PH_BETA3] * A.PAR3;
     const FT dl = sk.dl_growth;
     const FT d = jmax(FT(0), A.zeu - A.zmxl);
-    const FT drt = d * d / A.kappa;
+    const FT drt = qdiv(d * d, A.kappa);
     const FT mui = ph[OBME_PH_MU0] * btT;
-    const FT f1 = FT(1.5) * dl / (dl + FT(0.5 * DAYS));
-    const FT f2 = 1 - drt / (drt + ph[OBME_PH_DARK_TOLERANCE]);
+    const FT f1 = qdiv(FT(1.5) * dl, dl + FT(0.5 * DAYS));
+    const FT f2 = 1 - qdiv(drt, drt + ph[OBME_PH_DARK_TOLERANCE]);
     const FT alpha = ph[OBME_PH_ALPHA0] * (1 + ph[OBME_PH_BETA] * m_exp(-PAR));
-    const FT theta = IChl / (12 * I + eps0<FT>());
-    const FT fl = 1 - m_exp(-alpha * theta * PAR / (dl * (ph[OBME_PH_BASAL_RESPIRATION] + ph[OBME_PH_REFERENCE_GROWTH])));
+    const FT theta = qdiv(IChl, 12 * I + eps0<FT>());
+    const FT fl = 1 - m_exp(qdiv(-alpha * theta * PAR, dl * (ph[OBME_PH_BASAL_RESPIRATION] + ph[OBME_PH_REFERENCE_GROWTH])));
     s.mu = mui * f1 * f2 * fl * s.L;
     s.muI = s.mu * I;
 }
@@ -178,9 +198,9 @@ __device__ __forceinline__ void phyto_mortality(const FT* ph, const FT* sc, cons
                                                 FT& lin, FT& quad)
 {
     const FT I = s.I;
-    lin = ph[OBME_PH_MORT_LIN] * I / (I + ph[OBME_PH_MORT_K]) * I;
+    lin = qdiv(ph[OBME_PH_MORT_LIN] * I, I + ph[OBME_PH_MORT_K]) * I;
     const FT L2 = s.L * s.L;
-    const FT w = ph[OBME_PH_MORT_W0] + ph[OBME_PH_MORT_W1] * FT(0.25) * (1 - L2) / (FT(0.25) + L2);
+    const FT w = ph[OBME_PH_MORT_W0] + qdiv(ph[OBME_PH_MORT_W1] * FT(0.25) * (1 - L2), FT(0.25) + L2);
     const FT shear = (A.z < A.zmxl) ? sc[OBME_PS_BACKGROUND_SHEAR] : sc[OBME_PS_MIXED_LAYER_SHEAR];
     quad = shear * w * (I * I);
 }
@@ -192,10 +212,10 @@ __device__ __forceinline__ FT chlorophyll_growth(const FT* ph, const Phyto<FT>& 
 {
     const FT PAR = ph[OBME_PH_BETA1] * A.PAR1 + ph[OBME_PH_BETA2] * A.PAR2 + ph[OBME_PH_BETA3] * A.PAR3;
     const FT dl = sk.dl_chl;
-    const FT f1 = FT(1.5) * dl / (dl + FT(0.5 * DAYS));
-    const FT mucheck = s.mu / f1 * dl;
+    const FT f1 = qdiv(FT(1.5) * dl, dl + FT(0.5 * DAYS));
+    const FT mucheck = qdiv(s.mu, f1) * dl;
     const FT alpha = ph[OBME_PH_ALPHA0] * (1 + ph[OBME_PH_BETA] * m_exp(-PAR));
-    const FT rho = 12 * mucheck * s.I / (alpha * s.IChl * PAR + eps0<FT>()) * s.L;
+    const FT rho = qdiv(12 * mucheck * s.I, alpha * s.IChl * PAR + eps0<FT>()) * s.L;
     const FT th0 = ph[OBME_PH_CHL_MIN], th1 = ph[OBME_PH_CHL_MAX];
     return (1 - ph[OBME_PH_DELTA]) * 12 * (th0 + (th1 - th0) * rho) * s.mu * s.I;
 }
@@ -205,13 +225,30 @@ template <typename FT>
 __device__ __forceinline__ FT iron_uptake(const FT* ph, const Phyto<FT>& s, FT btT, FT Fe)
 {
     const FT fe_max = ph[OBME_PH_FE_MAX];
-    const FT thFe = s.IFe / (s.I + eps0<FT>());
     const FT mui = ph[OBME_PH_MU0] * btT;
     const FT K = ph[OBME_PH_KFE_UPTAKE] * size_factor(ph, s.I);
-    const FT L1 = Fe / (Fe + K + eps0<FT>());
-    const FT L2 = 4 - FT(4.5) * s.LFe / (s.LFe + 1);
-    const FT r = thFe / fe_max;
-    return (1 - ph[OBME_PH_DELTA]) * fe_max * L1 * L2 * jmax(FT(0), (1 - r) / (FT(1.05) - r)) * mui * s.I;
+    const FT L1 = qdiv(Fe, Fe + K + eps0<FT>());
+    const FT L2 = 4 - qdiv(FT(4.5) * s.LFe, s.LFe + 1);
+    // (1 - r) / (1.05 - r), r = θFe / θFe_max, θFe = IFe / (I + ε)
+    FT pole;
+    if constexpr (sizeof(FT) == 8) {
+        const FT thFe = s.IFe / (s.I + eps0<FT>());
+        const FT r = thFe / fe_max;
+        pole = (1 - r) / (FT(1.05) - r);
+    } else {
+        // the same quotient times F / F, F = θFe_max (I + ε):
+        // (F - IFe) / (1.05 F - IFe), with F's product exact inside the FMAs
+        // and 1.05 split into two floats.  Near the pole (r -> 1.05) the
+        // denominator is a small difference, and rounding r first would lose
+        // its leading digits.
+        const FT Ie = s.I + eps0<FT>();
+        const FT Fh = fe_max * Ie, Fl = fmaf(fe_max, Ie, -Fh);
+        const FT c = 1.05f, cl = (float)(1.05 - (double)1.05f);
+        const FT num = fmaf(fe_max, Ie, -s.IFe);
+        const FT den = fmaf(c, Fh, -s.IFe) + fmaf(c, Fl, cl * Fh);
+        pole = qdiv(num, den);
+    }
+    return (1 - ph[OBME_PH_DELTA]) * fe_max * L1 * L2 * jmax(FT(0), pole) * mui * s.I;
 }
 
 // silicate uptake (mixed_mondo.jl:195-226)
@@ -219,10 +256,10 @@ template <typename FT>
 __device__ __forceinline__ FT silicate_uptake(const FT* ph, const FT* sc, const Phyto<FT>& s, FT btT, FT Si)
 {
     const FT mui = ph[OBME_PH_MU0] * btT;
-    const FT L1 = Si / (Si + ph[OBME_PH_SI_K1] + eps0<FT>());
+    const FT L1 = qdiv(Si, Si + ph[OBME_PH_SI_K1] + eps0<FT>());
     const FT K2 = ph[OBME_PH_SI_K2];
-    const FT L2 = (sc[OBME_PS_LATITUDE] < FT(0)) ? (Si * Si * Si) / (Si * Si * Si + K2 * K2 * K2) : FT(0);
-    const FT F1 = jmin4(s.mu / (mui * s.L + eps0<FT>()), s.LFe, s.LPO4, s.LN);
+    const FT L2 = (sc[OBME_PS_LATITUDE] < FT(0)) ? qdiv(Si * Si * Si, Si * Si * Si + K2 * K2 * K2) : FT(0);
+    const FT F1 = jmin4(qdiv(s.mu, mui * s.L + eps0<FT>()), s.LFe, s.LPO4, s.LN);
     const FT F2 = jmin(FT(1), FT(2.2) * jmax(FT(0), L1 - FT(0.5)));
     const FT th1 = ph[OBME_PH_SI_THETA0] * L1 * jmin(FT(5.4), (FT(4.4) * m_exp(FT(-4.23) * F1) * F2 + 1) * (1 + 2 * L2));
     return (1 - ph[OBME_PH_DELTA]) * th1 * s.mu * s.I;
@@ -243,8 +280,8 @@ __device__ __forceinline__ void zoo_state(const FT* zp, const FT* v, FT bT, FT m
     g.tf = bT;
     const FT base = zp[OBME_PZ_G0] * g.tf;
     const FT food[4] = {v[OBME_PI_P], v[OBME_PI_D], v[OBME_PI_POC], v[OBME_PI_Z]};
-    const FT iron[4] = {v[OBME_PI_PFE] / (v[OBME_PI_P] + eps0<FT>()), v[OBME_PI_DFE] / (v[OBME_PI_D] + eps0<FT>()),
-                        v[OBME_PI_SFE] / (v[OBME_PI_POC] + eps0<FT>()), micro_iron};
+    const FT iron[4] = {qdiv(v[OBME_PI_PFE], v[OBME_PI_P] + eps0<FT>()), qdiv(v[OBME_PI_DFE], v[OBME_PI_D] + eps0<FT>()),
+                        qdiv(v[OBME_PI_SFE], v[OBME_PI_POC] + eps0<FT>()), micro_iron};
     const FT pref[4] = {zp[OBME_PZ_PREF_P], zp[OBME_PZ_PREF_D], zp[OBME_PZ_PREF_POC], zp[OBME_PZ_PREF_Z]};
     const FT J = zp[OBME_PZ_J];
     FT total_food = food[0] * pref[0];
@@ -259,13 +296,13 @@ __device__ __forceinline__ void zoo_state(const FT* zp, const FT* v, FT bT, FT m
         s = s + jmax(FT(0), food[n] - J) * pref[n] * iron[n];
     }
     const FT clg = jmax(FT(0), avail - jmin(avail / 2, zp[OBME_PZ_FOOD_THRESHOLD]));
-    g.tsg = base * clg / (zp[OBME_PZ_K] + total_food);
+    g.tsg = qdiv(base * clg, zp[OBME_PZ_K] + total_food);
     g.avail = avail;
-    const FT igr = total_iron / (zp[OBME_PZ_IRON_RATIO] * g.tsg + eps0<FT>());
+    const FT igr = qdiv(total_iron, zp[OBME_PZ_IRON_RATIO] * g.tsg + eps0<FT>());
     const FT fq = jmin(FT(1), igr);
     g.ge = fq * jmin(zp[OBME_PZ_E0], (1 - zp[OBME_PZ_SIGMA]) * igr);
     g.gI = g.tsg * I;
-    g.gIFe = s * g.tsg / (avail + eps0<FT>()) * I;
+    g.gIFe = qdiv(s * g.tsg, avail + eps0<FT>()) * I;
     // flux feeding on the sinking particles (food_quality_dependant.jl:185-199, iron_grazing.jl:38-53)
     const FT ffb = zp[OBME_PZ_FF] * g.tf;
     g.ff = ffb * (v[OBME_PI_POC] * wPOCc + v[OBME_PI_GOC] * wGOCc) * I;
@@ -276,13 +313,13 @@ __device__ __forceinline__ void zoo_state(const FT* zp, const FT* v, FT bT, FT m
 template <typename FT>
 __device__ __forceinline__ FT zoo_on(const FT* zp, const Zoo<FT>& g, FT pref, FT prey)
 {
-    return pref * jmax(FT(0), prey - zp[OBME_PZ_J]) * g.tsg / (g.avail + eps0<FT>()) * g.I;
+    return qdiv(pref * jmax(FT(0), prey - zp[OBME_PZ_J]) * g.tsg, g.avail + eps0<FT>()) * g.I;
 }
 
 template <typename FT>
 __device__ __forceinline__ FT anoxia(const FT* sc, FT O2)
 {
-    return jmin(FT(1), jmax(FT(0), FT(0.4) * (sc[OBME_PS_ANOXIA1] - O2) / (sc[OBME_PS_ANOXIA2] + O2)));
+    return jmin(FT(1), jmax(FT(0), qdiv(FT(0.4) * (sc[OBME_PS_ANOXIA1] - O2), sc[OBME_PS_ANOXIA2] + O2)));
 }
 
 // The 24 transitions of one cell (v: tracers in OBME_PI order).
@@ -309,7 +346,7 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
     const FT an = anoxia(sc, O2);
     // zooplankton mortality (food_quality_dependant.jl:201-227)
     auto zmort = [&](const FT* zp, const Zoo<FT>& g, FT& mort, FT& lin) {
-        const FT cf = g.I / (g.I + zp[OBME_PZ_KM]);
+        const FT cf = qdiv(g.I, g.I + zp[OBME_PZ_KM]);
         mort = g.tf * g.I * (zp[OBME_PZ_M0] * g.I + zp[OBME_PZ_R] * (cf + 3 * an));
         lin = g.tf * zp[OBME_PZ_R] * (cf + 3 * an) * g.I;
     };
@@ -331,21 +368,21 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
     {
         const FT death = Plin + Pquad;
         G[OBME_PI_P] = (1 - phN[OBME_PH_DELTA]) * sP.muI - death - grazP;
-        const FT thChl = sP.IChl / (12 * sP.I + e0);
+        const FT thChl = qdiv(sP.IChl, 12 * sP.I + e0);
         G[OBME_PI_PCHL] = chlorophyll_growth(phN, sP, A, sk) - (death + grazP) * thChl * 12;
-        const FT thFe = sP.IFe / (sP.I + e0);
+        const FT thFe = qdiv(sP.IFe, sP.I + e0);
         G[OBME_PI_PFE] = iron_uptake(phN, sP, tf.btN, v[OBME_PI_FE]) - (death + grazP) * thFe;
     }
     {
         const FT death = Dlin + Dquad;
         G[OBME_PI_D] = (1 - phD[OBME_PH_DELTA]) * sD.muI - death - grazD;
-        const FT thChl = sD.IChl / (12 * sD.I + e0);
+        const FT thChl = qdiv(sD.IChl, 12 * sD.I + e0);
         G[OBME_PI_DCHL] = chlorophyll_growth(phD, sD, A, sk) - (death + grazD) * thChl * 12;
-        const FT thFe = sD.IFe / (sD.I + e0);
+        const FT thFe = qdiv(sD.IFe, sD.I + e0);
         G[OBME_PI_DFE] = iron_uptake(phD, sD, tf.btD, v[OBME_PI_FE]) - (death + grazD) * thFe;
     }
     const FT si_uptake = silicate_uptake(phD, sc, sD, tf.btD, v[OBME_PI_SI]);
-    const FT thSi = v[OBME_PI_DSI] / (v[OBME_PI_D] + e0);
+    const FT thSi = qdiv(v[OBME_PI_DSI], v[OBME_PI_D] + e0);
     G[OBME_PI_DSI] = si_uptake - (Dlin + Dquad + grazD) * thSi;
 
     // zooplankton (micro_and_meso.jl:36-48)
@@ -358,7 +395,7 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
     const FT inorg_exc = zpZ[OBME_PZ_GAMMA] * excZ + zpM[OBME_PZ_GAMMA] * excM;
     const FT org_exc = (1 - zpZ[OBME_PZ_GAMMA]) * excZ + (1 - zpM[OBME_PZ_GAMMA]) * excM;
     // upper trophic waste of the meso class (mortality_waste.jl)
-    const FT utw = 1 / (1 - zpM[OBME_PZ_E0]) * zpM[OBME_PZ_M0] * gM.tf * (gM.I * gM.I);
+    const FT utw = qdiv(FT(1), 1 - zpM[OBME_PZ_E0]) * zpM[OBME_PZ_M0] * gM.tf * (gM.I * gM.I);
     const FT R_ut = (1 - zpM[OBME_PZ_E0] - zpM[OBME_PZ_SIGMA]) * utw;
     const FT fecal = zpM[OBME_PZ_SIGMA] * utw;
     const FT ut_excretion = (1 - zpM[OBME_PZ_GAMMA]) * R_ut;
@@ -367,16 +404,16 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
     // bacteria (micro_and_meso.jl:85-132)
     const FT zm = jmin(A.zmxl, A.zeu);
     const FT surface_bact = jmin(sc[OBME_PS_BACT_MAX], sc[OBME_PS_BZ] * v[OBME_PI_Z] + sc[OBME_PS_BM] * v[OBME_PI_M]);
-    const FT Bact = ((A.z >= zm) ? FT(1) : m_pow(zm / A.z, sc[OBME_PS_BACT_EXP])) * surface_bact;
+    const FT Bact = ((A.z >= zm) ? FT(1) : m_pow(qdiv(zm, A.z), sc[OBME_PS_BACT_EXP])) * surface_bact;
     const FT DOC = v[OBME_PI_DOC], NO3 = v[OBME_PI_NO3], NH4 = v[OBME_PI_NH4], PO4 = v[OBME_PI_PO4],
              Fe = v[OBME_PI_FE];
     const FT KNO = sc[OBME_PS_K_NO3_B], KNH = sc[OBME_PS_K_NH4_B];
-    const FT LBact = jmin3((KNO * NH4 + KNH * NO3) / (KNO * KNH + KNO * NH4 + KNH * NO3),
-                           PO4 / (PO4 + sc[OBME_PS_K_PO4_B]), Fe / (Fe + sc[OBME_PS_K_FE_B]))
-                     * (DOC / (DOC + sc[OBME_PS_K_DOC]));
+    const FT LBact = jmin3(qdiv(KNO * NH4 + KNH * NO3, KNO * KNH + KNO * NH4 + KNH * NO3),
+                           qdiv(PO4, PO4 + sc[OBME_PS_K_PO4_B]), qdiv(Fe, Fe + sc[OBME_PS_K_FE_B]))
+                     * qdiv(DOC, DOC + sc[OBME_PS_K_DOC]);
 
     // DOC (dissolved_organic_carbon.jl:39-106)
-    const FT doc_deg = sc[OBME_PS_DOC_LAMBDA] * tf.doc * LBact * Bact / sc[OBME_PS_DOC_BACT_REF] * DOC;
+    const FT doc_deg = qdiv(sc[OBME_PS_DOC_LAMBDA] * tf.doc * LBact * Bact, sc[OBME_PS_DOC_BACT_REF]) * DOC;
     const FT POC = v[OBME_PI_POC], GOC = v[OBME_PI_GOC];
     const FT Phi1 = shear_lt * (sc[OBME_PS_DOC_A1] * DOC + sc[OBME_PS_DOC_A2] * POC) * DOC;
     const FT Phi2 = shear_lt * (sc[OBME_PS_DOC_A3] * GOC) * DOC;
@@ -394,14 +431,14 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
     const FT pom_agg = shear_lt * (sc[OBME_PS_POM_A1] * (POC * POC) + sc[OBME_PS_POM_A2] * POC * GOC)
                        + sc[OBME_PS_POM_A3] * POC * GOC + sc[OBME_PS_POM_A4] * (POC * POC);
     const FT PAR = A.PAR;
-    const FT L_CaCO3 = jmin3(sP.LN, Fe / (Fe + FT(0.05)), sP.LPO4);
+    const FT L_CaCO3 = jmin3(sP.LN, qdiv(Fe, Fe + FT(0.05)), sP.LPO4);
     const FT R = sc[OBME_PS_RAIN_RATIO] * L_CaCO3 * jmax(FT(1), v[OBME_PI_P] / 2)
-                 * (jmax(FT(0), PAR - 1) / (4 + PAR)) * (30 / (30 + PAR)) * jmax(FT(0), T / (T + FT(0.1)))
-                 * tf.rain * jmin(FT(1), -50 / A.zmxl);
+                 * qdiv(jmax(FT(0), PAR - 1), 4 + PAR) * qdiv(FT(30), 30 + PAR) * jmax(FT(0), qdiv(T, T + FT(0.1)))
+                 * tf.rain * jmin(FT(1), qdiv(FT(-50), A.zmxl));
     const FT small_mort = (1 - R / 2) * (Plin + Pquad) + Dlin / 2;
     const FT large_mort = R / 2 * (Plin + Pquad) + Dlin / 2 + Dquad;
-    const FT thP = v[OBME_PI_PFE] / (v[OBME_PI_P] + e0);
-    const FT thD = v[OBME_PI_DFE] / (v[OBME_PI_D] + e0);
+    const FT thP = qdiv(v[OBME_PI_PFE], v[OBME_PI_P] + e0);
+    const FT thD = qdiv(v[OBME_PI_DFE], v[OBME_PI_D] + e0);
     const FT small_mort_fe = (1 - R / 2) * (Plin + Pquad) * thP + Dlin * thD / 2;
     const FT large_mort_fe = R / 2 * (Plin + Pquad) * thP + (Dlin / 2 + Dquad) * thD;
     const FT ffbZ = zpZ[OBME_PZ_FF] * gZ.tf, ffbM = zpM[OBME_PZ_FF] * gM.tf;
@@ -418,15 +455,15 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
     const FT ligands = jmax(FT(0.6), FT(0.09) * (DOC + 40) - 3);
     const FT Kfe = tf.kfe;
     const FT Dl = 1 + Kfe * ligands - Kfe * Fe;
-    const FT Fef = (-Dl + m_sqrt(Dl * Dl + 4 * Kfe * Fe)) / (2 * Kfe);
+    const FT Fef = qdiv(-Dl + m_sqrt(Dl * Dl + 4 * Kfe * Fe), 2 * Kfe);
     const FT colloidal = FT(0.5) * (Fe - Fef);
-    const FT CgFe1 = (Phi1 + Phi3) * colloidal / (DOC + e0);
-    const FT CgFe2 = Phi2 * colloidal / (DOC + e0);
+    const FT CgFe1 = qdiv((Phi1 + Phi3) * colloidal, DOC + e0);
+    const FT CgFe2 = qdiv(Phi2 * colloidal, DOC + e0);
     const FT lamFe = sc[OBME_PS_FE_SCAV_MIN] + sc[OBME_PS_FE_SCAV_LOAD] * (POC + GOC + v[OBME_PI_CACO3] + v[OBME_PI_PSI]);
-    const FT BactFe = sc[OBME_PS_BACT_MU0] * tf.pom * LBact * sc[OBME_PS_BACT_FE_MAX] * Fe
-                      / (Fe + sc[OBME_PS_BACT_FE_K]) * Bact * sc[OBME_PS_BACT_FE_EFF];
-    const FT thS = v[OBME_PI_SFE] / (POC + e0);
-    const FT thB = v[OBME_PI_BFE] / (GOC + e0);
+    const FT BactFe = qdiv(sc[OBME_PS_BACT_MU0] * tf.pom * LBact * sc[OBME_PS_BACT_FE_MAX] * Fe,
+                           Fe + sc[OBME_PS_BACT_FE_K]) * Bact * sc[OBME_PS_BACT_FE_EFF];
+    const FT thS = qdiv(v[OBME_PI_SFE], POC + e0);
+    const FT thB = qdiv(v[OBME_PI_BFE], GOC + e0);
     G[OBME_PI_SFE] = (sigZ * (gZ.gIFe + gZ.ffFe) + small_mort_fe + mZ * zpZ[OBME_PZ_IRON_RATIO]
                       + lam_pom * v[OBME_PI_BFE] + lamFe * POC * Fef + sc[OBME_PS_SMALL_BACT_FE] * BactFe + CgFe1
                       - tot_graz_POC * thS - pom_agg * thS - lam_pom * v[OBME_PI_SFE]);
@@ -436,12 +473,12 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
 
     // particulate silicate (silicate.jl:1-48)
     const FT chi = sc[OBME_PS_CHI0] * ((A.z >= zm) ? FT(1)
-                                                   : m_exp((sc[OBME_PS_SI_FAST] - sc[OBME_PS_SI_SLOW]) * (zm - A.z) / A.wGOCc));
+                                                   : m_exp(qdiv((sc[OBME_PS_SI_FAST] - sc[OBME_PS_SI_SLOW]) * (zm - A.z), A.wGOCc)));
     const FT lam0 = chi * sc[OBME_PS_SI_FAST] + (1 - chi) * sc[OBME_PS_SI_SLOW];
     const FT eq = tf.eq;
-    const FT sat = (eq - v[OBME_PI_SI]) / eq;
+    const FT sat = qdiv(eq - v[OBME_PI_SI], eq);
     const FT t4 = tf.t4;
-    const FT lam_si = lam0 * (FT(0.225) * (1 + T / 15) * sat + FT(0.775) * m_pow(t4 * sat, FT(9)));
+    const FT lam_si = lam0 * (FT(0.225) * (1 + qdivk(T, 15)) * sat + FT(0.775) * m_pow(t4 * sat, FT(9)));
     const FT si_diss = lam_si * v[OBME_PI_PSI];
     G[OBME_PI_PSI] = (grazD + Dlin + Dquad) * thSi - si_diss;
 
@@ -455,15 +492,15 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
 
     // nitrogen (nitrogen/nitrate_ammonia.jl:22-88)
     const FT thN = sc[OBME_PS_THETAN];
-    const FT nitrif = sc[OBME_PS_NITRIF] * NH4 / (1 + A.mlPAR) * (1 - an);
+    const FT nitrif = qdiv(sc[OBME_PS_NITRIF] * NH4, 1 + A.mlPAR) * (1 - an);
     const FT fix_limit = (sP.LN >= FT(0.8)) ? FT(0.01) : 1 - sP.LN;
     const FT mu_nano = phN[OBME_PH_MU0] * tf.btN;
     const FT growth_req = jmax(FT(0), mu_nano - FT(2.15));
-    const FT nut_lim = jmin(Fe / (Fe + sc[OBME_PS_FIX_KFE]), PO4 / (PO4 + sc[OBME_PS_FIX_KPO4]));
-    const FT light_lim = 1 - m_exp(-PAR / sc[OBME_PS_FIX_E]);
+    const FT nut_lim = jmin(qdiv(Fe, Fe + sc[OBME_PS_FIX_KFE]), qdiv(PO4, PO4 + sc[OBME_PS_FIX_KPO4]));
+    const FT light_lim = 1 - m_exp(qdiv(-PAR, sc[OBME_PS_FIX_E]));
     const FT fixation = sc[OBME_PS_FIX_MAX] * growth_req * fix_limit * nut_lim * light_lim;
-    const FT up_NO3 = sP.muI * sP.LNO3 / (sP.LN + e0) + sD.muI * sD.LNO3 / (sD.LN + e0);
-    const FT up_NH4 = sP.muI * sP.LNH4 / (sP.LN + e0) + sD.muI * sD.LNH4 / (sD.LN + e0);
+    const FT up_NO3 = qdiv(sP.muI * sP.LNO3, sP.LN + e0) + qdiv(sD.muI * sD.LNO3, sD.LN + e0);
+    const FT up_NH4 = qdiv(sP.muI * sP.LNH4, sP.LN + e0) + qdiv(sD.muI * sD.LNH4, sD.LN + e0);
     G[OBME_PI_NO3] = nitrif + thN * (oxic - up_NO3);
     G[OBME_PI_NH4] = fixation + thN * (anoxic + inorg_exc + ut_respiration - up_NH4) - nitrif;
 
@@ -490,9 +527,9 @@ __device__ __forceinline__ void pisces_tendencies(const FT* pp, const FT* v, con
 
     // oxygen (oxygen.jl:30-50)
     const FT th_r = sc[OBME_PS_O2_RESP], th_n = sc[OBME_PS_O2_NITRIF];
-    G[OBME_PI_O2] = (th_r * up_NH4 + (th_r + th_n) * up_NO3 + th_n * fixation / thN
+    G[OBME_PI_O2] = (th_r * up_NH4 + (th_r + th_n) * up_NO3 + qdiv(th_n * fixation, thN)
                      - ((th_r + th_n) * oxic + th_r * anoxic) - th_r * inorg_exc - th_r * ut_respiration
-                     - th_n * nitrif / thN);
+                     - qdiv(th_n * nitrif, thN));
 }
 
 // ---------------------------------------------------------------- calcite saturation (FP64)

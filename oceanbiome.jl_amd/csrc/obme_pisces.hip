@@ -15,6 +15,7 @@
 // the sinking stencil's k+1 reads never race with the update.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -397,14 +398,20 @@ __global__ void __launch_bounds__(256) k_pisces_surface_prep(PiscesArgs<FT> a, d
 }
 
 // Per-call constants of the fused step.
+// Steps per launch of the fused step at most: columns are independent (T
+// and S frozen, sinking within the column), so a wave carries its column
+// through several steps with the state in registers.
+constexpr int PF_MAX_STEPS = 16;
+
 struct PiscesFused {
     NegScale<float, 32> ns;         // ScaleNegativeTracers groups (SCALE)
-    StageConst<float> sk[3];        // day lengths at the three stage clocks
     float A[3], B[3];               // Δt γ, Δt ζ per stage
-    float PAR0[3];                  // surface PAR at the three stage clocks (constant, diurnal)
     float div[3];                   // the share of each band
     int have_H;                     // Hs holds the [H+] of the previous step
+    int nsteps;                     // steps of this launch (1 .. PF_MAX_STEPS)
     const double* surf;             // [SURF_PACK][Nx*Ny] from k_pisces_surface_prep
+    StageConst<float> sk[PF_MAX_STEPS][3];   // day lengths at the stage clocks of each step
+    float PAR0[PF_MAX_STEPS][3];             // surface PAR at the stage clocks (constant, diurnal)
 };
 
 // The fused FP32-mixed PISCES step: one wave per water column, lane r
@@ -518,9 +525,6 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
 #pragma unroll
         for (int q = 0; q < TFAC_FIELDS; q++) s_tf[q][tid] = tv[q];
     }
-#pragma unroll
-    for (int n = 0; n < NW; n++) s_gm[n][tid] = FT(0);
-
     // the level's geometry
     const FT* zcv = a.lev;
     const FT* zfv = a.lev + Nz;
@@ -560,127 +564,137 @@ __global__ void __launch_bounds__(PF_WAVES * 64, OBME_PISCES_PF_OCC) k_pisces_fu
     }
 
 #pragma unroll 1
-    for (int st = 0; st < 3; st++) {
-        const FT* pps = a.pp;
-        if constexpr (SCALE) scale_negatives_cell<NW>(f.ns, v);
+    for (int sn = 0; sn < f.nsteps; sn++) {
+        // G⁻ of a step's first stage is zero (ζ = 0 there; the zero also keeps a
+        // non-finite tendency of the previous step out of the product)
+#pragma unroll
+        for (int n = 0; n < NW; n++) s_gm[n][tid] = FT(0);
+#pragma unroll 1
+        for (int st = 0; st < 3; st++) {
+            const FT* pps = a.pp;
+            if constexpr (SCALE) scale_negatives_cell<NW>(f.ns, v);
 
-        // the top boundary fluxes of DIC and O2 (the surface kernel's solve),
-        // first, while few values are live
-        double Fc = 0.0, Fo = 0.0;
-        if (top && gas) {
-            SurfaceCarbon scb;
-            double kO2, airO2;
-            surface_unpack(s_surf[wv], scb, kO2, airO2);
-            if (a.co2.enabled) {
-                double pH = 0.0, Hsurf = s_hsurf[wv];
-                int ev = 0;
-                Fc = surface_flux_warm(scb, a.solver, (double)v[OBME_PI_DIC], (double)v[OBME_PI_ALK],
-                                       a.co2.air_concentration, 1, Hsurf, pH, ev, FP32_NEWTON_TOL);
-                s_hsurf[wv] = Hsurf;
+            // the top boundary fluxes of DIC and O2 (the surface kernel's solve),
+            // first, while few values are live
+            double Fc = 0.0, Fo = 0.0;
+            if (top && gas) {
+                SurfaceCarbon scb;
+                double kO2, airO2;
+                surface_unpack(s_surf[wv], scb, kO2, airO2);
+                if (a.co2.enabled) {
+                    double pH = 0.0, Hsurf = s_hsurf[wv];
+                    int ev = 0;
+                    Fc = surface_flux_warm(scb, a.solver, (double)v[OBME_PI_DIC], (double)v[OBME_PI_ALK],
+                                           a.co2.air_concentration, 1, Hsurf, pH, ev, FP32_NEWTON_TOL);
+                    s_hsurf[wv] = Hsurf;
+                }
+                if (a.o2.enabled) Fo = kO2 * ((double)v[OBME_PI_O2] - airO2);
             }
-            if (a.o2.enabled) Fo = kO2 * ((double)v[OBME_PI_O2] - airO2);
-        }
 
-        // MultiBand PAR: band n = PAR0 share · exp(Σ from the top of Δz·att)
-        const FT PAR0 = a.surf ? PAR0col : f.PAR0[st];
-        const FT thr = PAR0 * FT(1e-3);
-        const FT lthr = m_log(thr);
-        const FT Chl = v[OBME_PI_PCHL] + v[OBME_PI_DCHL];
-        const FT lChl = m_log2(Chl);   // Chl^e = 2^(e log2 Chl), one logarithm for the three bands
-        FT band[3];
+            // MultiBand PAR: band n = PAR0 share · exp(Σ from the top of Δz·att)
+            const FT PAR0 = a.surf ? PAR0col : f.PAR0[sn][st];
+            const FT thr = PAR0 * FT(1e-3);
+            const FT lthr = m_log(thr);
+            const FT Chl = v[OBME_PI_PCHL] + v[OBME_PI_DCHL];
+            const FT lChl = m_log2(Chl);   // Chl^e = 2^(e log2 Chl), one logarithm for the three bands
+            FT band[3];
 #pragma unroll
-        for (int n = 0; n < 3; n++) {
-            const FT att = a.kw[n] + a.chi[n] * m_exp2(a.e[n] * lChl);
-            const FT S = seg_prefix<64>(act ? dzl * att : FT(0));
-            band[n] = (PAR0 * f.div[n]) * m_exp(S);
-        }
-        const FT PAR = band[0] + band[1] + band[2];
+            for (int n = 0; n < 3; n++) {
+                const FT att = a.kw[n] + a.chi[n] * m_exp2(a.e[n] * lChl);
+                const FT S = seg_prefix<64>(act ? dzl * att : FT(0));
+                band[n] = (PAR0 * f.div[n]) * m_exp(S);
+            }
+            const FT PAR = band[0] + band[1] + band[2];
 
-        // euphotic depth: the first level below the top cell, from the top,
-        // with PAR <= thr, interpolated in log PAR with the level above
-        const FT lPAR = m_log(PAR);
-        const unsigned long long cand = __ballot(act && r >= 1 && PAR <= thr);
-        FT zeu = zeu_none;
-        if (cand) {
-            const int rs = __ffsll((long long)cand) - 1;
-            const FT zk = __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, zc), rs));
-            const FT zu = __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, zc), rs - 1));
-            const FT lk = __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, lPAR), rs));
-            const FT lu = __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, lPAR), rs - 1));
-            zeu = zk + (lthr - lk) * (zk - zu) / (lk - lu);
-            if (!(zeu > FT(-INFINITY) && zeu < FT(INFINITY))) zeu = zeu_none;
-        }
-        // mixed-layer mean of the total PAR
-        const FT acc =
-            __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, seg_prefix<64>(PAR * wml)), 63));
+            // euphotic depth: the first level below the top cell, from the top,
+            // with PAR <= thr, interpolated in log PAR with the level above
+            const FT lPAR = m_log(PAR);
+            const unsigned long long cand = __ballot(act && r >= 1 && PAR <= thr);
+            FT zeu = zeu_none;
+            if (cand) {
+                const int rs = __ffsll((long long)cand) - 1;
+                const FT zk = __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, zc), rs));
+                const FT zu = __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, zc), rs - 1));
+                const FT lk = __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, lPAR), rs));
+                const FT lu = __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, lPAR), rs - 1));
+                zeu = zk + (lthr - lk) * (zk - zu) / (lk - lu);
+                if (!(zeu > FT(-INFINITY) && zeu < FT(INFINITY))) zeu = zeu_none;
+            }
+            // mixed-layer mean of the total PAR
+            const FT acc =
+                __builtin_bit_cast(FT, __builtin_amdgcn_readlane(__builtin_bit_cast(int, seg_prefix<64>(PAR * wml)), 63));
 
-        Aux<FT> A;
-        A.z = zc;
-        A.zmxl = zmxl;
-        A.zeu = zeu;
-        A.kappa = pps[OBME_PP_SCALARS + OBME_PS_KAPPA];
-        A.mlPAR = acc / depth;
-        A.PAR = PAR;
-        A.PAR1 = band[0];
-        A.PAR2 = band[1];
-        A.PAR3 = band[2];
-        A.wPOCc = (wPlo + wPhi) / 2;
-        A.wGOCc = (wGlo + wGhi) / 2;
-        A.Omega = FT(0);
-        if (act) {
-            OmegaConst o;
-            o.conv = s_oc[0][tid];
-            o.K1 = s_oc[1][tid];
-            o.K2 = s_oc[2][tid];
-            o.KB = s_oc[3][tid];
-            o.KW = s_oc[4][tid];
-            o.KS = s_oc[5][tid];
-            o.KF = s_oc[6][tid];
-            o.KSi = s_oc[7][tid];
-            o.KSP = s_oc[8][tid];
-            omega_salinity_terms((double)v[OBME_PI_S], o);
-            const CarbonParams cp =
-                omega_params(o, (double)v[OBME_PI_DIC], (double)v[OBME_PI_ALK], (double)v[OBME_PI_SI]);
-            int ev = 0;
-            if (H < 0) H = (double)guess_H_fp32(cp);
-            if (!refine_H<true>(cp, H, ev, FP32_NEWTON_TOL)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
-            A.Omega = (FT)omega_from_H(o, cp, H);
-        }
-        FT G[NW];
-        TFac<FT> tf;
-        tf.btN = s_tf[0][tid];
-        tf.btD = s_tf[1][tid];
-        tf.bZ = s_tf[2][tid];
-        tf.bM = s_tf[3][tid];
-        tf.doc = s_tf[4][tid];
-        tf.pom = s_tf[5][tid];
-        tf.rain = s_tf[6][tid];
-        tf.kfe = s_tf[7][tid];
-        tf.eq = s_tf[8][tid];
-        tf.t4 = s_tf[9][tid];
-        pisces_tendencies(pps, v, A, f.sk[st], tf, G);
-        // flux-form upwind sinking: (w(k+1) c(k+1) - w(k) c(k)) / dz, the
-        // level above from lane r - 1, w = 0 on the top face
-        auto sink = [&](int n, FT wlo, FT whi) {
-            const FT cu = lane_shr1(v[n]);
-            G[n] = G[n] - (whi * cu - wlo * v[n]) * idz;
-        };
-        sink(OBME_PI_POC, wPlo, wPhi);
-        sink(OBME_PI_SFE, wPlo, wPhi);
-        sink(OBME_PI_GOC, wGlo, wGhi);
-        sink(OBME_PI_BFE, wGlo, wGhi);
-        sink(OBME_PI_PSI, wGlo, wGhi);
-        sink(OBME_PI_CACO3, wGlo, wGhi);
-        if (top) {
-            G[OBME_PI_DIC] = G[OBME_PI_DIC] - (FT)Fc * idz;
-            G[OBME_PI_O2] = G[OBME_PI_O2] - (FT)Fo * idz;
-        }
-        const FT As = f.A[st], Bs = f.B[st];
+            Aux<FT> A;
+            A.z = zc;
+            A.zmxl = zmxl;
+            A.zeu = zeu;
+            A.kappa = pps[OBME_PP_SCALARS + OBME_PS_KAPPA];
+            A.mlPAR = acc / depth;
+            A.PAR = PAR;
+            A.PAR1 = band[0];
+            A.PAR2 = band[1];
+            A.PAR3 = band[2];
+            A.wPOCc = (wPlo + wPhi) / 2;
+            A.wGOCc = (wGlo + wGhi) / 2;
+            A.Omega = FT(0);
+            if (act) {
+                OmegaConst o;
+                o.conv = s_oc[0][tid];
+                o.K1 = s_oc[1][tid];
+                o.K2 = s_oc[2][tid];
+                o.KB = s_oc[3][tid];
+                o.KW = s_oc[4][tid];
+                o.KS = s_oc[5][tid];
+                o.KF = s_oc[6][tid];
+                o.KSi = s_oc[7][tid];
+                o.KSP = s_oc[8][tid];
+                omega_salinity_terms((double)v[OBME_PI_S], o);
+                const CarbonParams cp =
+                    omega_params(o, (double)v[OBME_PI_DIC], (double)v[OBME_PI_ALK], (double)v[OBME_PI_SI]);
+                int ev = 0;
+                if (H < 0) H = (double)guess_H_fp32(cp);
+                if (!refine_H<true>(cp, H, ev, FP32_NEWTON_TOL)) H = solve_for_H<true>(cp, a.solver, pow(10.0, -a.solver.initial_pH_guess), ev);
+                A.Omega = (FT)omega_from_H(o, cp, H);
+            }
+            FT G[NW];
+            TFac<FT> tf;
+            tf.btN = s_tf[0][tid];
+            tf.btD = s_tf[1][tid];
+            tf.bZ = s_tf[2][tid];
+            tf.bM = s_tf[3][tid];
+            tf.doc = s_tf[4][tid];
+            tf.pom = s_tf[5][tid];
+            tf.rain = s_tf[6][tid];
+            tf.kfe = s_tf[7][tid];
+            tf.eq = s_tf[8][tid];
+            tf.t4 = s_tf[9][tid];
+            pisces_tendencies(pps, v, A, f.sk[sn][st], tf, G);
+            // flux-form upwind sinking: (w(k+1) c(k+1) - w(k) c(k)) / dz, the
+            // level above from lane r - 1, w = 0 on the top face
+            auto sink = [&](int n, FT wlo, FT whi) {
+                const FT cu = lane_shr1(v[n]);
+                G[n] = G[n] - (whi * cu - wlo * v[n]) * idz;
+            };
+            sink(OBME_PI_POC, wPlo, wPhi);
+            sink(OBME_PI_SFE, wPlo, wPhi);
+            sink(OBME_PI_GOC, wGlo, wGhi);
+            sink(OBME_PI_BFE, wGlo, wGhi);
+            sink(OBME_PI_PSI, wGlo, wGhi);
+            sink(OBME_PI_CACO3, wGlo, wGhi);
+            if (top) {
+                G[OBME_PI_DIC] = G[OBME_PI_DIC] - (FT)Fc * idz;
+                G[OBME_PI_O2] = G[OBME_PI_O2] - (FT)Fo * idz;
+            }
+            const FT As = f.A[st], Bs = f.B[st];
 #pragma unroll
-        for (int n = 0; n < NW; n++) {
-            v[n] = v[n] + fmaf(As, G[n], Bs * s_gm[n][tid]);
-            s_gm[n][tid] = G[n];
+            for (int n = 0; n < NW; n++) {
+                v[n] = v[n] + fmaf(As, G[n], Bs * s_gm[n][tid]);
+                s_gm[n][tid] = G[n];
+            }
         }
+        // between steps [H+] is rounded as the FP32 warm-start store rounds it:
+        // a launch of several steps gives the results of one launch per step
+        H = (double)(float)H;
     }
 
     // stage out: each lane into its own slots (G⁻ is no longer needed),
@@ -733,6 +747,7 @@ struct obme_pisces {
     std::vector<double> zc, zf, dz;
     bool scaling = false;
     bool fused = false;        // FP32-mixed with Nz <= 64: k_pisces_fused (OBME_PISCES_FUSED=0 turns it off)
+    int fused_steps = PF_MAX_STEPS;   // steps per fused launch (OBME_PISCES_FUSED_STEPS sets 1 .. 16)
 };
 
 extern "C" obme_status obme_pisces_destroy(obme_pisces* h)
@@ -834,6 +849,10 @@ extern "C" obme_status obme_pisces_create(const obme_pisces_config* cfg, obme_pi
     // the fused step: FP32-mixed, one wave per column of at most 64 levels
     h->fused = h->prec == OBME_FP32_MIXED && g.Nz <= 64;
     if (const char* e = std::getenv("OBME_PISCES_FUSED")) h->fused = h->fused && std::atoi(e) != 0;
+    if (const char* e = std::getenv("OBME_PISCES_FUSED_STEPS")) {
+        const int n = std::atoi(e);
+        if (n >= 1 && n <= PF_MAX_STEPS) h->fused_steps = n;
+    }
     obme_status st = (h->prec == OBME_FP64) ? pisces_upload<double>(h) : pisces_upload<float>(h);
     auto alloc = [&](void** d, size_t bytes) {
         if (st == OBME_OK && hipMalloc(d, bytes) != hipSuccess) st = obme_fail(OBME_ERR_NOMEM, "PISCES buffers");
@@ -1122,15 +1141,18 @@ static obme_status pisces_fused_steps(obme_pisces* h, void* const* tracers, int 
     }
     const unsigned nbx = (unsigned)((h->cfg.grid.Nx + PF_WAVES - 1) / PF_WAVES);
     const unsigned nb = nbx * (unsigned)h->cfg.grid.Ny;
-    for (int st = 0; st < nsteps; st++) {
-        const double t = t0 + st * dt;
-        for (int q = 0; q < 3; q++) {
-            const double tq = t + cst[q] * dt;
-            f.sk[q].dl_growth = (float)host_day_length(lat, tq);  // day_length(φ, t), growth_rate.jl:30
-            f.sk[q].dl_chl = (float)host_day_length(tq, lat);     // day_length(t, φ), growth_rate.jl:128
-            f.PAR0[q] = (float)pisces_surface_value(h->cfg, tq);
+    for (int s0 = 0; s0 < nsteps; s0 += h->fused_steps) {
+        f.nsteps = std::min(h->fused_steps, nsteps - s0);
+        for (int sn = 0; sn < f.nsteps; sn++) {
+            const double t = t0 + (s0 + sn) * dt;
+            for (int q = 0; q < 3; q++) {
+                const double tq = t + cst[q] * dt;
+                f.sk[sn][q].dl_growth = (float)host_day_length(lat, tq);  // day_length(φ, t), growth_rate.jl:30
+                f.sk[sn][q].dl_chl = (float)host_day_length(tq, lat);     // day_length(t, φ), growth_rate.jl:128
+                f.PAR0[sn][q] = (float)pisces_surface_value(h->cfg, tq);
+            }
         }
-        f.have_H = st > 0 ? 1 : 0;
+        f.have_H = s0 > 0 ? 1 : 0;
         if (h->scaling)
             hipLaunchKernelGGL(k_pisces_fused<true>, dim3(nb), dim3(PF_WAVES * 64), 0, s, a, f);
         else

@@ -2,8 +2,9 @@
 registers) against the numpy FP64 oracle at full 64-level columns and at a
 ragged depth, with the gas exchange and the negative scaling on, over two
 calls (the second warm-starts [H+] from the first); against the per-stage
-kernels (OBME_PISCES_FUSED=0) on a larger grid; and the per-stage path that
-columns deeper than 64 levels take.  FP32-mixed bar: relative L2 <= 1e-5
+kernels (OBME_PISCES_FUSED=0) on a larger grid; several steps per launch
+(the state in registers between steps) against one step per launch, bit for
+bit; and the per-stage path that columns deeper than 64 levels take.  FP32-mixed bar: relative L2 <= 1e-5
 per tracer.  Inputs around the values of the reference's PISCES tests
 (test/test_PISCES.jl:6-16)."""
 import os
@@ -32,20 +33,24 @@ def _oracle_inputs(wl):
     return q, light, F, np.asarray(g.zc), np.asarray(g.zf), np.asarray(g.dz), solver
 
 
-def _device_run(wl, calls, nsteps, fused=True):
+def _device_run(wl, calls, nsteps, fused=True, steps_per_launch=None, t_start=0.0):
     import torch
-    old = os.environ.get("OBME_PISCES_FUSED")
-    os.environ["OBME_PISCES_FUSED"] = "1" if fused else "0"
+    env = {"OBME_PISCES_FUSED": "1" if fused else "0"}
+    if steps_per_launch is not None:
+        env["OBME_PISCES_FUSED_STEPS"] = str(steps_per_launch)
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         dev = HIPPISCES(wl.bgc, "fp32")
     finally:
-        if old is None:
-            del os.environ["OBME_PISCES_FUSED"]
-        else:
-            os.environ["OBME_PISCES_FUSED"] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
     tr = [torch.from_numpy(a).to(device="cuda", dtype=dev.dtype) for a in wl.tracers]
     for c in range(calls):
-        dev.step(tr, nsteps, wl.dt, t0=c * nsteps * wl.dt)
+        dev.step(tr, nsteps, wl.dt, t0=t_start + c * nsteps * wl.dt)
     torch.cuda.synchronize()
     return [t.double().cpu().numpy() for t in tr]
 
@@ -110,3 +115,22 @@ def test_fused_step_with_non_default_parameters():
     errs = {name: rel_l2(g.interior(out[n]), F[name]) for n, name in enumerate(OP.WRITTEN)}
     print("errors", {k: float(v) for k, v in errs.items()}, flush=True)
     assert all(e <= TOL for e in errs.values()), errs
+
+
+@pytest.mark.parametrize("gas,scale", [(True, False), (False, True)])
+def test_steps_per_launch_bit_identical(gas, scale):
+    """20 steps: launches of 16 + 4 steps against 20 launches of one step
+    (the [H+] between steps rounded to FP32 as its store between launches
+    rounds it), with the diurnal surface PAR from 10 h, in daylight, so the
+    surface PAR of every stage differs."""
+    print("case pisces fused steps per launch gas", gas, "scale", scale, flush=True)
+    wl = workloads.pisces_3d(24, 5, 64, nsteps=20, dt=300.0, seed=37, medians="test")
+    wl.bgc = HP.PISCES(wl.grid, co2_exchange=gas, o2_exchange=gas, scale_negatives=scale, surface_PAR="default")
+    a = _device_run(wl, 1, wl.nsteps, t_start=36000.0)
+    b = _device_run(wl, 1, wl.nsteps, steps_per_launch=1, t_start=36000.0)
+    # (cells where the growth rate's day length, day_length(φ, t) in the
+    # reference's argument order, is zero and PAR has underflowed to zero
+    # are NaN, as in the reference: they must coincide)
+    print("non-finite", sum(int((~np.isfinite(x)).sum()) for x in a), flush=True)
+    for n, name in enumerate(OP.WRITTEN):
+        assert np.array_equal(a[n], b[n], equal_nan=True), (name, float(np.nanmax(np.abs(a[n] - b[n]))))
