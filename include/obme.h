@@ -342,7 +342,8 @@ obme_status obme_step_noalloc(obme_handle* h, void* const* tracers, const void* 
 /* No counterpart in the reference: the launch shape of the column sweep.
  * waves = waves per workgroup (1, 2, 4 or 8; a workgroup covers 64*waves
  * consecutive columns of one row).  Results do not depend on it, bit for
- * bit.  The default is 4, or the environment's OBME_SWEEP_WAVES. */
+ * bit.  The default is the widest of 8, 4, 2, 1 whose 64*waves columns a
+ * row of Nx fills (8 from Nx = 512), or the environment's OBME_SWEEP_WAVES. */
 obme_status obme_sweep_configure(obme_handle* h, int32_t waves);
 /* The current waves per workgroup, or < 0 for a null handle. */
 int32_t obme_sweep_waves(const obme_handle* h);

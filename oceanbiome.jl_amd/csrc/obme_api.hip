@@ -461,7 +461,9 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
         obme_destroy(h);
         return st;
     }
-    h->sweep_waves = 4;
+    // a workgroup's waves walk in step, so wider workgroups make longer
+    // contiguous runs per level: the widest that a row fills
+    h->sweep_waves = g.Nx >= 512 ? 8 : g.Nx >= 256 ? 4 : g.Nx >= 128 ? 2 : 1;
     if (const char* e = std::getenv("OBME_SWEEP_WAVES")) {
         const int nw = std::atoi(e);
         if (valid_sweep_waves(nw)) h->sweep_waves = nw;

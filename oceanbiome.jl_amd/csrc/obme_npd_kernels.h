@@ -779,6 +779,12 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
     };
     using T_ = std::integral_constant<bool, true>;
     using F_ = std::integral_constant<bool, false>;
+    // The workgroup's waves walk their columns level by level in step: at
+    // each level their loads of a tracer (256 B per wave) then reach the
+    // memory together as one run of 256 NW contiguous bytes.  (A barrier
+    // only: no memory fence, so the prefetched loads stay in flight.  Every
+    // wave of the workgroup runs every level: no wave leaves early.)
+    auto level_barrier = [] { __builtin_amdgcn_s_barrier(); };
     if constexpr (MODE == TILE_STEP && EXTRA == 0) {
         if (Nz == 1) {
             level(0, T_{}, T_{});
@@ -787,12 +793,19 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
             // (unrolled by two: the prefetched and the current level's
             // registers alternate instead of being copied)
 #pragma unroll 2
-            for (int k = ktop - 1; k >= 1; k--) level(k, F_{}, F_{});
+            for (int k = ktop - 1; k >= 1; k--) {
+                level_barrier();
+                level(k, F_{}, F_{});
+            }
+            level_barrier();
             level(0, F_{}, T_{});
         }
     } else {
 #pragma unroll 1
-        for (int k = ktop; k >= 0; k--) level(k, k == ktop, k == 0);
+        for (int k = ktop; k >= 0; k--) {
+            level_barrier();
+            level(k, k == ktop, k == 0);
+        }
     }
     if constexpr (INLINE_CO2 && F32) {
         if (col_ok && a.Hprev) a.Hprev[col2d] = Hc;
