@@ -201,7 +201,7 @@ typedef struct obme_negative_scaling {
 } obme_negative_scaling;
 
 /* Sediment models (src/Models/Sediments). */
-enum { OBME_SEDIMENT_NONE = 0, OBME_SEDIMENT_INSTANT_REMINERALISATION = 1 };
+enum { OBME_SEDIMENT_NONE = 0, OBME_SEDIMENT_INSTANT_REMINERALISATION = 1, OBME_SEDIMENT_SIMPLE_MULTI_G = 2 };
 enum { OBME_SEDIMENT_AB2 = 0 };   /* QuasiAdamsBashforth2, the sediment's default time stepper */
 
 /* InstantRemineralisationSediment (Models/Sediments/instant_remineralisation.jl:60-123)
@@ -223,6 +223,19 @@ typedef struct obme_sediment {
     double burial_efficiency_constant2;           /* b, 0.53                            */
     double burial_efficiency_half_saturation;     /* k, 7 / 6.56                        */
     double ab2_chi;                /* χ of QuasiAdamsBashforth2, 0.1                     */
+    /* SimpleMultiG, nitrogen form (Models/Sediments/simple_multi_G.jl:37-45,
+     * 158-247, 338-426): fields Ns, Nf, Nr; the sinking nitrogen flux of the
+     * `sinking` tracers feeds them, and the bottom cells of NH4, NO3 and O2
+     * receive the coupled remineralisation.  Unused by InstantRemineralisation. */
+    int32_t ammonia, nitrate, oxygen;             /* tracer indices of NH4, NO3, O2      */
+    double fast_decay_rate, slow_decay_rate;      /* λf 2/day, λs 0.2/day (1/s)          */
+    double fast_fraction, slow_fraction, refactory_fraction;   /* 0.74, 0.26, 0.1       */
+    double sedimentation_rate;     /* cm/year: 982 |z of the bottom cell centre|^-1.548  */
+    double anoxia_half_saturation; /* mmol/m³, 1.0                                      */
+    double sinking_redfield;       /* C:N of the sinking nitrogen, 6.56                 */
+    double nitrate_oxidation_params[6];           /* A .. F of the ammonia oxidation     */
+    double denitrification_params[6];
+    double anoxic_params[6];
 } obme_sediment;
 
 /* OxygenGasExchangeBoundaryCondition (GasExchange/GasExchange.jl:127-146):
@@ -358,12 +371,18 @@ obme_status obme_autotune(obme_handle* h, void* const* tracers, const void* T, c
                           int32_t trial_steps, double dt, void* stream);
 
 /* ---- sediment (BiogeochemicalSediment, src/Sediments) -------------------
- * The sediment's state is its storage field (device, Nx*Ny doubles, i
- * fastest; the caller's) and its time stepper's tendencies, which the handle
- * keeps.  With a sediment configured, obme_add_tendencies also adds the
+ * The sediment's state is its fields (device doubles, the caller's; `storage`
+ * below): InstantRemineralisation's storage (Nx*Ny, i fastest), or
+ * SimpleMultiG's Ns, Nf, Nr one after the other (3 Nx*Ny); and its time
+ * stepper's tendencies, which the handle keeps.  With an
+ * InstantRemineralisation sediment, obme_add_tendencies also adds the
  * coupled tendency of the receiver (update_tendencies!(bgc, sediment, model),
  * tracer_coupling.jl:3-38) from the sinking flux of the tracers it is given,
- * and obme_step applies it at every stage (storage not advanced). */
+ * and obme_step applies it at every stage (storage not advanced).
+ * SimpleMultiG's coupled tendencies depend on its fields: they are added by
+ * obme_add_tendencies_with_sediment and by the steppers, from the fields
+ * given (obme_step, without fields: an empty sediment, which couples
+ * nothing). */
 
 /* update_biogeochemical_state!(model, sediment) (Sediments/update_state.jl:6-17):
  * the tracked sinking flux f of the given tracers (tracked_fields.jl:8-70)
@@ -372,9 +391,20 @@ obme_status obme_autotune(obme_handle* h, void* const* tracers, const void* T, c
  * time_step!): a forward Euler step when dt differs from the previous step's
  * (always, between RK3 stages), else χ-weighted with the previous tendency;
  * the tendency e f is evaluated after each step from the tracked flux, and
- * before the first step (iteration 0) from the flux of that call. */
+ * before the first step (iteration 0) from the flux of that call.
+ * SimpleMultiG: the tendencies of Ns, Nf, Nr from the tracked nitrogen flux
+ * and the fields (simple_multi_G.jl:158-189). */
 obme_status obme_sediment_update_state(obme_handle* h, const void* const* tracers, double* storage,
                                        double dt, void* stream);
+
+/* obme_add_tendencies, with the coupled tendencies of a SimpleMultiG
+ * sediment from its fields `storage` (3 Nx*Ny doubles: Ns, Nf, Nr) and the
+ * bottom cells' NO3, NH4, O2 (update_coupled_tendency!, tracer_coupling.jl:
+ * 30-38: G[bottom] += tendency / Δz; simple_multi_G.jl:193-247).  Any other
+ * configuration: the same as obme_add_tendencies. */
+obme_status obme_add_tendencies_with_sediment(obme_handle* h, const void* const* tracers, void* const* G,
+                                              const void* PAR, const void* T, const void* S,
+                                              const double* storage, double t, void* stream);
 
 /* obme_step with the sediment's storage advanced in the same pass: at every
  * RK3 stage the coupled tendency of the stage's flux, and after every stage

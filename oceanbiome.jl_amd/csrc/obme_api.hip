@@ -117,9 +117,25 @@ __global__ void __launch_bounds__(256) k_sediment_couple(StepArgs<FT> a, int s0,
 {
     const unsigned col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= (unsigned)a.Nx * (unsigned)a.Ny) return;
+    const unsigned p = a.base0 + col % (unsigned)a.Nx + a.sx * (col / (unsigned)a.Nx);
+    if (a.sed_model == OBME_SEDIMENT_SIMPLE_MULTI_G) {
+        // NH4, NO3 and O2 from the sediment's fields (none given: an empty
+        // sediment) and the bottom cell (simple_multi_G.jl:195-247)
+        const unsigned ncol = (unsigned)a.Nx * (unsigned)a.Ny;
+        double u[3] = {0.0, 0.0, 0.0};
+        if (a.sed_fields)
+            for (int q = 0; q < 3; q++) u[q] = a.sed_fields[q * ncol + col];
+        double cnh4, cno3, co2;
+        smg_coupled(a.smg, u, (double)a.tr[a.smg.no3][p], (double)a.tr[a.smg.nh4][p], (double)a.tr[a.smg.o2][p],
+                    cnh4, cno3, co2);
+        const double dz0 = (double)a.sed_dz0;
+        a.G[a.smg.nh4][p] += (FT)(cnh4 / dz0);
+        a.G[a.smg.no3][p] += (FT)(cno3 / dz0);
+        a.G[a.smg.o2][p] += (FT)(co2 / dz0);
+        return;
+    }
     FT f, e;
     sediment_flux(a, s0, s1, col, f, e);
-    const unsigned p = a.base0 + col % (unsigned)a.Nx + a.sx * (col / (unsigned)a.Nx);
     a.G[a.sed_recv][p] += ((FT(1) - e) * f) / a.sed_dz0;
 }
 
@@ -135,6 +151,25 @@ __global__ void __launch_bounds__(256) k_sediment_state(StepArgs<FT> a, int s0, 
     if (col >= ncol) return;
     FT f, e;
     sediment_flux(a, s0, s1, col, f, e);
+    if (a.sed_model == OBME_SEDIMENT_SIMPLE_MULTI_G) {
+        // the three fields Ns, Nf, Nr, their Gⁿ from the fields and the flux
+        double u[3], Gn[3], Gnew[3];
+        for (int q = 0; q < 3; q++) u[q] = a.storage ? a.storage[q * ncol + col] : 0.0;
+        if (a.sed_init) smg_tendencies(a.smg, u, (double)f, Gn);
+        else
+            for (int q = 0; q < 3; q++) Gn[q] = a.sedG[q * ncol + col];
+        for (int q = 0; q < 3; q++) {
+            const double Gu = euler ? Gn[q] : a.sed_cn * Gn[q] - a.sed_cm * a.sedG[(3 + q) * ncol + col];
+            u[q] = u[q] + dt * Gu;
+            a.sedG[(3 + q) * ncol + col] = Gn[q];
+        }
+        smg_tendencies(a.smg, u, (double)f, Gnew);
+        for (int q = 0; q < 3; q++) {
+            if (a.storage) a.storage[q * ncol + col] = u[q];
+            a.sedG[q * ncol + col] = Gnew[q];
+        }
+        return;
+    }
     const double ge = (double)(e * f);
     const double Gn = a.sed_init ? ge : a.sedG[col];
     // Euler: (1.5 + χ) Gⁿ with χ = -1/2, and G⁻ multiplied by a strong zero
@@ -204,7 +239,9 @@ struct obme_handle {
     bool uniform;
     bool scaling;              // ScaleNegativeTracers groups configured
     bool co2_inline;           // scaling with the CO2 flux: the flux is computed in the sweep
-    bool sediment;             // InstantRemineralisation configured
+    bool sediment;             // a sediment configured (InstantRemineralisation or SimpleMultiG)
+    int sed_nf = 0;            // its fields: 1 (storage) or 3 (Ns, Nf, Nr)
+    const double* hook_sed_fields = nullptr;   // the SimpleMultiG fields of the current hook call
     int scheme;                // the sinking face value (OBME_UPWIND1, OBME_CENTERED2, OBME_WENO5)
     bool staged;               // centred / WENO sinking: steps run one stage per launch
     int cur_stage = 0;         // the stage of the next TILE_STAGE launch
@@ -255,9 +292,18 @@ struct LaunchTimer {
 static obme_status validate_sediment(const obme_sediment& s, int ntracers, int readonly)
 {
     if (s.model == OBME_SEDIMENT_NONE) return OBME_OK;
-    if (s.model != OBME_SEDIMENT_INSTANT_REMINERALISATION) return fail(OBME_ERR_INVALID, "unknown sediment model");
-    if (s.receiver < 0 || s.receiver >= ntracers || s.receiver == readonly)
+    if (s.model != OBME_SEDIMENT_INSTANT_REMINERALISATION && s.model != OBME_SEDIMENT_SIMPLE_MULTI_G)
+        return fail(OBME_ERR_INVALID, "unknown sediment model");
+    auto written = [&](int t) { return t >= 0 && t < ntracers && t != readonly; };
+    if (s.model == OBME_SEDIMENT_INSTANT_REMINERALISATION && !written(s.receiver))
         return fail(OBME_ERR_INVALID, "sediment receiver is not a written tracer");
+    if (s.model == OBME_SEDIMENT_SIMPLE_MULTI_G) {
+        if (!written(s.ammonia) || !written(s.nitrate) || !written(s.oxygen) || s.ammonia == s.nitrate ||
+            s.ammonia == s.oxygen || s.nitrate == s.oxygen)
+            return fail(OBME_ERR_INVALID, "SimpleMultiG needs three distinct written tracers NH4, NO3 and O2");
+        if (!(s.sedimentation_rate > 0.0))
+            return fail(OBME_ERR_INVALID, "SimpleMultiG sedimentation rate must be positive");
+    }
     if (s.n_sinking < 1 || s.n_sinking > OBME_MAX_SEDIMENT_TRACERS)
         return fail(OBME_ERR_INVALID, "bad number of sediment sinking tracers");
     unsigned seen = 0;
@@ -493,9 +539,10 @@ extern "C" obme_status obme_create(const obme_config* cfg, obme_handle** out)
             return fail(OBME_ERR_HIP, "hipMemset");
         }
     }
-    h->sediment = cfg->sediment.model == OBME_SEDIMENT_INSTANT_REMINERALISATION;
+    h->sediment = cfg->sediment.model != OBME_SEDIMENT_NONE;
+    h->sed_nf = cfg->sediment.model == OBME_SEDIMENT_SIMPLE_MULTI_G ? 3 : 1;
     if (h->sediment) {
-        size_t bytes = 2 * (size_t)g.Nx * g.Ny * sizeof(double);
+        size_t bytes = 2 * (size_t)h->sed_nf * g.Nx * g.Ny * sizeof(double);
         if (hipMalloc((void**)&h->d_sedG, bytes) != hipSuccess) {
             obme_destroy(h);
             return fail(OBME_ERR_NOMEM, "sediment buffers");
@@ -625,6 +672,28 @@ static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers)
         a.sedG = h->d_sedG;
         a.sed_cn = 1.5 + sd.ab2_chi;
         a.sed_cm = 0.5 + sd.ab2_chi;
+        a.sed_model = sd.model;
+        a.sed_nf = h->sed_nf;
+        a.sed_fields = h->hook_sed_fields;
+        if (sd.model == OBME_SEDIMENT_SIMPLE_MULTI_G) {
+            SmgParams& q = a.smg;
+            q.ls = sd.slow_decay_rate;
+            q.lf = sd.fast_decay_rate;
+            q.fs = sd.slow_fraction;
+            q.ff = sd.fast_fraction;
+            q.fr = sd.refactory_fraction;
+            q.R = sd.sinking_redfield;
+            q.kO2 = sd.anoxia_half_saturation;
+            q.ps = 0.223 * std::pow(sd.sedimentation_rate, 0.336);   // solid_deposition_fraction
+            for (int i = 0; i < 6; i++) {
+                q.qn[i] = sd.nitrate_oxidation_params[i];
+                q.qd[i] = sd.denitrification_params[i];
+                q.qa[i] = sd.anoxic_params[i];
+            }
+            q.no3 = sd.nitrate;
+            q.nh4 = sd.ammonia;
+            q.o2 = sd.oxygen;
+        }
     }
     if (sizeof(FT) == 4) {
         a.f = make_fast_params(c.npd);
@@ -917,6 +986,20 @@ extern "C" obme_status obme_add_tendencies(obme_handle* h, const void* const* tr
     return run_tile<TILE_TENDENCY>(h, (void* const*)tracers, G, (void*)PAR, T, S, t, 0.0, (hipStream_t)stream);
 }
 
+extern "C" obme_status obme_add_tendencies_with_sediment(obme_handle* h, const void* const* tracers, void* const* G,
+                                                         const void* PAR, const void* T, const void* S,
+                                                         const double* storage, double t, void* stream)
+{
+    if (!h) return fail(OBME_ERR_INVALID, "null argument");
+    // the fields reach the coupling kernel through the launch's arguments
+    struct Restore {
+        obme_handle* h;
+        ~Restore() { h->hook_sed_fields = nullptr; }
+    } restore{h};
+    h->hook_sed_fields = storage;
+    return obme_add_tendencies(h, tracers, G, PAR, T, S, t, stream);
+}
+
 // p[0 .. n) = 0 (a kernel rather than hipMemsetAsync, which HIP graph
 // capture does not take reliably)
 __global__ void __launch_bounds__(256) k_zero_f64(double* p, long n)
@@ -1185,8 +1268,9 @@ extern "C" obme_status obme_autotune(obme_handle* h, void* const* tracers, const
     }
     if (h->d_sedG) {
         void* p = nullptr;
-        if (!alloc(2 * ncol * sizeof(double), &p)) return fail(OBME_ERR_NOMEM, "autotune scratch sediment");
-        HIP_TRY(hipMemcpyAsync(p, keep.sedG, 2 * ncol * sizeof(double), hipMemcpyDeviceToDevice, s));
+        const size_t sb = 2 * (size_t)h->sed_nf * ncol * sizeof(double);
+        if (!alloc(sb, &p)) return fail(OBME_ERR_NOMEM, "autotune scratch sediment");
+        HIP_TRY(hipMemcpyAsync(p, keep.sedG, sb, hipMemcpyDeviceToDevice, s));
         h->d_sedG = (double*)p;
     }
     if (hipEventCreate(&keep.e0) != hipSuccess || hipEventCreate(&keep.e1) != hipSuccess)
@@ -1226,6 +1310,9 @@ extern "C" obme_status obme_sediment_update_state(obme_handle* h, const void* co
     if (!h->sediment) return fail(OBME_ERR_INVALID, "no sediment configured");
     const int s0 = h->sink_idx[0], s1 = h->sink_idx[1];
     if (!tracers[s0] || !tracers[s1]) return fail(OBME_ERR_INVALID, "null tracer pointer");
+    const obme_sediment& sd = h->cfg.sediment;
+    if (sd.model == OBME_SEDIMENT_SIMPLE_MULTI_G && (!tracers[sd.ammonia] || !tracers[sd.nitrate] || !tracers[sd.oxygen]))
+        return fail(OBME_ERR_INVALID, "null tracer pointer");
     // the tracked fields only: no step before the model's first stage
     if (!std::isfinite(dt)) return OBME_OK;
     const unsigned ncol = (unsigned)(h->cfg.grid.Nx * h->cfg.grid.Ny);

@@ -93,6 +93,55 @@ struct ModelTraits<OBME_LOBSTER, 3> {
 // stage s (s = 0, 1, 2).
 constexpr int SURF_REC = 6;
 
+// SimpleMultiG's nitrogen form (Models/Sediments/simple_multi_G.jl), FP64:
+// decay rates λs, λf, fractions fs, ff, fr, the sinking C:N, the anoxia
+// half saturation, the solid deposition fraction 0.223 r^0.336 of the
+// sedimentation rate r (:427), the three fractions' A..F, and the tracer
+// indices of NO3, NH4, O2.
+struct SmgParams {
+    double ls, lf, fs, ff, fr, R, kO2, ps;
+    double qn[6], qd[6], qa[6];
+    int no3, nh4, o2;
+};
+
+// The tendencies of Ns, Nf, Nr from the fields u and the sinking nitrogen
+// flux f (simple_multi_G.jl:165-193).
+__device__ __forceinline__ void smg_tendencies(const SmgParams& p, const double* u, double f, double* G)
+{
+    G[0] = (1 - p.fr) * p.fs * f - p.ls * u[0];
+    G[1] = (1 - p.fr) * p.ff * f - p.lf * u[1];
+    G[2] = p.fr * f;
+}
+
+__device__ __forceinline__ double smg_finite(double x) { return isfinite(x) ? x : 0.0; }
+
+// The coupled tendencies of NH4, NO3 and O2 per unit area from the fields
+// (Ns, Nf) and the bottom cell's NO3, NH4, O2 (simple_multi_G.jl:195-247;
+// reactivity and the fractions :372-427; day = 86400 s; a non-finite
+// fraction is zero).
+__device__ __forceinline__ void smg_coupled(const SmgParams& p, const double* u, double NO3, double NH4, double O2,
+                                            double& dNH4, double& dNO3, double& dO2)
+{
+    constexpr double DAY = 86400.0;
+    const double Nr = p.ls * u[0] + p.lf * u[1];
+    const double Cr = Nr * p.R;
+    const double Cs = u[0] * p.R, Cf = u[1] * p.R;
+    const double k = (p.ls * Cs + p.lf * Cf) / (Cs + Cf + 4.9406564584124654e-324);
+    const double lC = log(Cr * DAY), lk = log(k), lkd = log(k * DAY), lO = log(O2), lN = log(NO3), lH = log(NH4);
+    const double ln_n = (p.qn[0] + p.qn[1] * lC * lO + p.qn[2] * (lC * lC) + p.qn[3] * lkd * lH + p.qn[4] * lC
+                         + p.qn[5] * lC * lH);
+    const double pn = smg_finite(exp(ln_n) / (Nr * DAY) * O2 / (p.kO2 + O2));
+    const double ln_d = (p.qd[0] + p.qd[1] * lC + p.qd[2] * (lN * lN) + p.qd[3] * (lC * lC) + p.qd[4] * (lkd * lkd)
+                         + p.qd[5] * lO * lk);
+    const double pd = smg_finite(exp(ln_d) / (Cr * DAY) * O2 / (p.kO2 + O2));
+    const double ln_a = (p.qa[0] + p.qa[1] * lC + p.qa[2] * (lC * lC) + p.qa[3] * lkd + p.qa[4] * lO * lk
+                         + p.qa[5] * (lN * lN));
+    const double pa = smg_finite(exp(ln_a) / (Cr * DAY));
+    dNH4 = (1 - pn) * Nr + 0.8 * pd * Cr;
+    dNO3 = pn * Nr - 0.8 * pd * Cr;
+    dO2 = -(1 - pa * p.ps - pd) * O2 / (p.kO2 + O2) * Cr - 2 * pn * Nr;
+}
+
 // Kernel arguments (passed by value).
 template <typename FT>
 struct StepArgs {
@@ -154,6 +203,11 @@ struct StepArgs {
     double sed_cn, sed_cm;         // AB2 weights 1.5 + χ, 0.5 + χ of the first stage's step
     int sed_ab2;                   // the first stage's step is an AB2 (not an Euler) step
     int sed_init;                  // the sediment's first step ever is in this launch
+    // the sediment model (OBME_SEDIMENT_*), its fields (storage, sedG and
+    // sed_fields hold nf × Nx*Ny: InstantRemineralisation 1, SimpleMultiG 3)
+    int sed_model, sed_nf;
+    SmgParams smg;
+    const double* sed_fields;      // the hook's SimpleMultiG fields, or null (an empty sediment)
     // one RK3 stage per launch (TILE_STAGE): the stage, the CO2 flux computed
     // by the top lane, and the per-cell scratch of the step (compact, tracer
     // n at n * ncell, cell col + Nx*Ny*k): FP32 the step's increment so far,
@@ -352,11 +406,15 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
     };
     // sediment (EXTRA & 4): the storage value, the tendency e f of the last
     // stage start and the G⁻ of the sediment's previous step
-    double sed_st = 0.0, sed_ge = 0.0, sed_gm = 0.0;
+    // (fields: InstantRemineralisation's storage, SimpleMultiG's Ns, Nf, Nr)
+    double sed_u[3] = {0.0, 0.0, 0.0}, sed_ge[3] = {0.0, 0.0, 0.0}, sed_gm[3] = {0.0, 0.0, 0.0};
     if constexpr ((EXTRA & 4) != 0) {
         if (col_ok) {
-            if (a.storage) sed_st = a.storage[col2d];
-            if (a.sed_ab2) sed_gm = a.sedG[ncol2d + col2d];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                if (q < a.sed_nf && a.storage) sed_u[q] = a.storage[q * ncol2d + col2d];
+                if (q < a.sed_nf && a.sed_ab2) sed_gm[q] = a.sedG[(a.sed_nf + q) * ncol2d + col2d];
+            }
         }
     }
 
@@ -641,7 +699,48 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
                 // burial e = a + b (f / (k + f))^2 and the receiver's coupled
                 // tendency (1 - e) f / Δz (instant_remineralisation.jl:101-123,
                 // Sediments/tracer_coupling.jl:30-38)
-                if (bot) {
+                if (bot && a.sed_model == OBME_SEDIMENT_SIMPLE_MULTI_G) {
+                    // SimpleMultiG: the sediment step after this stage's
+                    // substep takes the tendencies evaluated after the
+                    // previous step: from the fields now and this stage's
+                    // flux.  The first step ever (after the first substep)
+                    // evaluates them from the second stage's flux, and the
+                    // coupled tendencies of the second stage see its result.
+                    const double fd = (double)(a.sed_w0 * X[S0] + a.sed_w1 * X[S1]);
+                    double G[3];
+                    if (st == 1 && a.sed_init) {
+                        smg_tendencies(a.smg, sed_u, fd, G);
+#pragma unroll
+                        for (int q = 0; q < 3; q++) sed_u[q] += a.sed_dt[0] * G[q];
+                    }
+                    double vno3 = 0.0, vnh4 = 0.0, vo2 = 0.0;
+#pragma unroll
+                    for (int n = 0; n < NT; n++) {
+                        if (n == a.smg.no3) vno3 = (double)X[n];
+                        if (n == a.smg.nh4) vnh4 = (double)X[n];
+                        if (n == a.smg.o2) vo2 = (double)X[n];
+                    }
+                    double cnh4, cno3, co2;
+                    smg_coupled(a.smg, sed_u, vno3, vnh4, vo2, cnh4, cno3, co2);
+                    const double dz0 = (double)a.sed_dz0;
+#pragma unroll
+                    for (int n = 0; n < NT; n++) {
+                        if (n == a.smg.nh4) Gv[n] = Gv[n] + (FT)(cnh4 / dz0);
+                        if (n == a.smg.no3) Gv[n] = Gv[n] + (FT)(cno3 / dz0);
+                        if (n == a.smg.o2) Gv[n] = Gv[n] + (FT)(co2 / dz0);
+                    }
+                    smg_tendencies(a.smg, sed_u, fd, G);
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        if (st == 0) {
+                            if (!a.sed_init)
+                                sed_u[q] += a.sed_dt[0] * (a.sed_ab2 ? a.sed_cn * G[q] - a.sed_cm * sed_gm[q] : G[q]);
+                        } else {
+                            sed_u[q] += a.sed_dt[st] * G[q];
+                        }
+                        sed_ge[q] = G[q];
+                    }
+                } else if (bot) {
                     const FT f = a.sed_w0 * X[S0] + a.sed_w1 * X[S1];
                     const FT qf = f / (a.sed_k + f);
                     const FT e = a.sed_a + a.sed_b * (qf * qf);
@@ -658,12 +757,12 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
                     const double ge = (double)(e * f);
                     if (st == 0) {
                         if (!a.sed_init)
-                            sed_st += a.sed_dt[0] * (a.sed_ab2 ? a.sed_cn * ge - a.sed_cm * sed_gm : ge);
+                            sed_u[0] += a.sed_dt[0] * (a.sed_ab2 ? a.sed_cn * ge - a.sed_cm * sed_gm[0] : ge);
                     } else {
-                        if (st == 1 && a.sed_init) sed_st += a.sed_dt[0] * ge;
-                        sed_st += a.sed_dt[st] * ge;
+                        if (st == 1 && a.sed_init) sed_u[0] += a.sed_dt[0] * ge;
+                        sed_u[0] += a.sed_dt[st] * ge;
                     }
-                    sed_ge = ge;
+                    sed_ge[0] = ge;
                 }
             }
             if constexpr (MODE == TILE_TENDENCY) {
@@ -768,11 +867,22 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
                     const FT x0 = F32 ? U[S0] + D[F32 ? S0 : 0] : U[S0];
                     const FT x1 = F32 ? U[S1] + D[F32 ? S1 : 0] : U[S1];
                     const FT f = a.sed_w0 * x0 + a.sed_w1 * x1;
-                    const FT qf = f / (a.sed_k + f);
-                    const FT e = a.sed_a + a.sed_b * (qf * qf);
-                    if (a.storage) a.storage[col2d] = sed_st;
-                    a.sedG[col2d] = (double)(e * f);
-                    a.sedG[ncol2d + col2d] = sed_ge;
+                    if (a.sed_model == OBME_SEDIMENT_SIMPLE_MULTI_G) {
+                        double G[3];
+                        smg_tendencies(a.smg, sed_u, (double)f, G);
+#pragma unroll
+                        for (int q = 0; q < 3; q++) {
+                            if (a.storage) a.storage[q * ncol2d + col2d] = sed_u[q];
+                            a.sedG[q * ncol2d + col2d] = G[q];
+                            a.sedG[(3 + q) * ncol2d + col2d] = sed_ge[q];
+                        }
+                    } else {
+                        const FT qf = f / (a.sed_k + f);
+                        const FT e = a.sed_a + a.sed_b * (qf * qf);
+                        if (a.storage) a.storage[col2d] = sed_u[0];
+                        a.sedG[col2d] = (double)(e * f);
+                        a.sedG[ncol2d + col2d] = sed_ge[0];
+                    }
                 }
             }
         }

@@ -129,10 +129,11 @@ end
 
 const OBME_MAX_SEDIMENT_TRACERS = 6
 const OBME_SEDIMENT_INSTANT_REMINERALISATION = Int32(1)
+const OBME_SEDIMENT_SIMPLE_MULTI_G = Int32(2)
 
-# InstantRemineralisationSediment (Models/Sediments/instant_remineralisation.jl):
-# receiver and sinking tracers as 0-based indices in required_biogeochemical_tracers
-# order; model = 0: no sediment.
+# InstantRemineralisationSediment (Models/Sediments/instant_remineralisation.jl) or
+# SimpleMultiGSediment's nitrogen form (Models/Sediments/simple_multi_G.jl): tracers as
+# 0-based indices in required_biogeochemical_tracers order; model = 0: no sediment.
 struct ObmeSediment
     model :: Int32
     receiver :: Int32
@@ -143,6 +144,20 @@ struct ObmeSediment
     burial_efficiency_constant2 :: Float64
     burial_efficiency_half_saturation :: Float64
     ab2_chi :: Float64
+    ammonia :: Int32                 # SimpleMultiG: NH₄, NO₃, O₂
+    nitrate :: Int32
+    oxygen :: Int32
+    fast_decay_rate :: Float64
+    slow_decay_rate :: Float64
+    fast_fraction :: Float64
+    slow_fraction :: Float64
+    refactory_fraction :: Float64
+    sedimentation_rate :: Float64
+    anoxia_half_saturation :: Float64
+    sinking_redfield :: Float64
+    nitrate_oxidation_params :: NTuple{6, Float64}
+    denitrification_params :: NTuple{6, Float64}
+    anoxic_params :: NTuple{6, Float64}
 end
 
 # Oxygen() (NutrientsPlanktonDetritus/oxygen.jl): the O₂ tracer of NPZD or LOBSTER, last in
@@ -198,7 +213,7 @@ mutable struct HIPBiogeochemistry{B, F} <: AbstractBiogeochemistry
     handle :: Ptr{Cvoid}
     PAR :: F                      # CenterField owned by the wrapper
     keep :: Vector{Any}           # host arrays referenced by `config`
-    storage :: Any                # the sediment's storage (device, Nx*Ny Float64) or nothing
+    storage :: Any                # the sediment's fields (device Float64) or nothing
 end
 
 """
@@ -206,8 +221,10 @@ end
 
 Creates the library handle for `config` (built from the OceanBioME model
 `underlying` by the caller, see the Python host package for the field-by-field
-mapping) and owns `PAR`.  With a sediment in `config`, `storage` is the
-sediment's storage field: a device array of Nx*Ny Float64, i fastest.
+mapping) and owns `PAR`.  With a sediment in `config`, `storage` holds the
+sediment's fields as a device array of Float64, i fastest:
+InstantRemineralisation's storage (Nx*Ny), or SimpleMultiG's Ns, Nf, Nr one
+after the other (3 Nx*Ny).
 """
 function HIPBiogeochemistry(underlying, grid, config::ObmeConfig, PAR; keep = Any[], storage = nothing)
     check_abi()
@@ -241,7 +258,7 @@ function update_biogeochemical_state!(b::HIPBiogeochemistry, model)
     check(ccall((:obme_update_state, libobme), Int32,
                 (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Float64, Ptr{Cvoid}),
                 b.handle, ptrs, device_pointer(b.PAR), model.clock.time, stream_pointer()))
-    if b.config[].sediment.model == OBME_SEDIMENT_INSTANT_REMINERALISATION
+    if b.config[].sediment.model != 0
         # update_biogeochemical_state!(model, sediment) (Sediments/update_state.jl:6-17)
         check(ccall((:obme_sediment_update_state, libobme), Int32,
                     (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Float64, Ptr{Cvoid}),
@@ -257,6 +274,15 @@ function update_tendencies!(b::HIPBiogeochemistry, model)
     T = haskey(model.tracers, :T) && !(:T in required_biogeochemical_tracers(b)) ?
         device_pointer(model.tracers.T) : C_NULL
     S = haskey(model.tracers, :S) ? device_pointer(model.tracers.S) : C_NULL
+    if b.config[].sediment.model == OBME_SEDIMENT_SIMPLE_MULTI_G
+        # the coupled NH₄, NO₃, O₂ of SimpleMultiG from its fields (tracer_coupling.jl:3-38)
+        check(ccall((:obme_add_tendencies_with_sediment, libobme), Int32,
+                    (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid},
+                     Float64, Ptr{Cvoid}),
+                    b.handle, ptrs, G, device_pointer(b.PAR), T, S, Ptr{Cvoid}(UInt(pointer(b.storage))),
+                    model.clock.time, stream_pointer()))
+        return nothing
+    end
     check(ccall((:obme_add_tendencies, libobme), Int32,
                 (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Float64, Ptr{Cvoid}),
                 b.handle, ptrs, G, device_pointer(b.PAR), T, S, model.clock.time, stream_pointer()))
@@ -271,7 +297,7 @@ tendencies and substep per stage), in place on the tracer fields.
 """
 function step!(b::HIPBiogeochemistry, tracers, nsteps, Δt; t₀ = 0.0)
     ptrs = tracer_pointers(b, tracers)
-    if b.config[].sediment.model == OBME_SEDIMENT_INSTANT_REMINERALISATION
+    if b.config[].sediment.model != 0
         check(ccall((:obme_step_with_sediment, libobme), Int32,
                     (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Int32, Float64, Float64,
                      Ptr{Cvoid}),

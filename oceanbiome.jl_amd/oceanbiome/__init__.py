@@ -3,7 +3,8 @@
 Mirrors the OceanBioME.jl interface for the NutrientsPlanktonDetritus models
 (NPZD, LOBSTER + CarbonateSystem + Oxygen) and PISCES (oceanbiome.pisces), TwoBand and
 MultiBand PAR, sinking, CarbonChemistry, the CO2 and O2 GasExchange boundary
-conditions, ScaleNegativeTracers and the InstantRemineralisation sediment;
+conditions, ScaleNegativeTracers and the InstantRemineralisation and
+SimpleMultiG sediments;
 the compute runs in libobme_hip.so.
 """
 from . import abi, units
@@ -11,7 +12,7 @@ from .grid import ColumnGrid, RectilinearGrid
 from .models import (LOBSTER, NPZD, AnalyticalLightLimitation, Biogeochemistry, CarbonateSystem,
                      CarbonChemistry, CarbonDioxideGasExchangeBoundaryCondition, DampedNewtonRaphsonSolver,
                      Detritus, InstantRemineralisationSediment, Linear, MondoLightLimitation, NitrateAmmonia, Nutrient,
-                     Oxygen, OxygenGasExchangeBoundaryCondition, PhytoZoo, Quadratic,
+                     Oxygen, OxygenGasExchangeBoundaryCondition, PhytoZoo, Quadratic, SimpleMultiGSediment,
                      TwoBandPhotosyntheticallyActiveRadiation, TwoParticleAndDissolved,
                      biogeochemical_drift_velocity, default_surface_PAR, required_biogeochemical_tracers)
 
@@ -21,7 +22,7 @@ __all__ = [
     "CarbonDioxideGasExchangeBoundaryCondition", "OxygenGasExchangeBoundaryCondition", "DampedNewtonRaphsonSolver",
     "TwoBandPhotosyntheticallyActiveRadiation", "Linear", "Quadratic", "MondoLightLimitation",
     "AnalyticalLightLimitation", "default_surface_PAR", "required_biogeochemical_tracers",
-    "biogeochemical_drift_velocity", "InstantRemineralisationSediment", "Oxygen",
+    "biogeochemical_drift_velocity", "InstantRemineralisationSediment", "SimpleMultiGSediment", "Oxygen",
 ]
 
 

@@ -162,13 +162,14 @@ def negative_scaling(groups, names):
     return s
 
 
-OBME_SEDIMENT_NONE, OBME_SEDIMENT_INSTANT_REMINERALISATION = 0, 1
+OBME_SEDIMENT_NONE, OBME_SEDIMENT_INSTANT_REMINERALISATION, OBME_SEDIMENT_SIMPLE_MULTI_G = 0, 1, 2
 OBME_SEDIMENT_AB2 = 0
 OBME_MAX_SEDIMENT_TRACERS = 6
 
 
 class Sediment(C.Structure):
-    """obme_sediment (InstantRemineralisation through BiogeochemicalSediment)."""
+    """obme_sediment (InstantRemineralisation or SimpleMultiG through
+    BiogeochemicalSediment)."""
     _fields_ = [
         ("model", C.c_int32),
         ("receiver", C.c_int32),
@@ -179,6 +180,20 @@ class Sediment(C.Structure):
         ("burial_efficiency_constant2", C.c_double),
         ("burial_efficiency_half_saturation", C.c_double),
         ("ab2_chi", C.c_double),
+        ("ammonia", C.c_int32),
+        ("nitrate", C.c_int32),
+        ("oxygen", C.c_int32),
+        ("fast_decay_rate", C.c_double),
+        ("slow_decay_rate", C.c_double),
+        ("fast_fraction", C.c_double),
+        ("slow_fraction", C.c_double),
+        ("refactory_fraction", C.c_double),
+        ("sedimentation_rate", C.c_double),
+        ("anoxia_half_saturation", C.c_double),
+        ("sinking_redfield", C.c_double),
+        ("nitrate_oxidation_params", C.c_double * 6),
+        ("denitrification_params", C.c_double * 6),
+        ("anoxic_params", C.c_double * 6),
     ]
 
 

@@ -281,6 +281,93 @@ class InstantRemineralisationSediment:
         return s
 
 
+class SimpleMultiGSediment:
+    """SimpleMultiGSediment(grid; ...) (Models/Sediments/simple_multi_G.jl:60-134),
+    the nitrogen form (sinking_carbon = nothing, the default): a single-layer
+    sediment with slow, fast and refractory organic nitrogen (Ns, Nf, Nr)
+    fed by the sinking nitrogen flux of `sinking_nitrogen`, whose
+    remineralisation enters the bottom cells of NH₄, NO₃ and O₂ through the
+    Soetaert et al. (2000) nitrification, denitrification and anoxic
+    fractions.  fast_redfield, slow_redfield and solid_dep_params are
+    accepted and, as in the reference's tendencies, unused.  The carbon form
+    (sinking_carbon given) is not built."""
+
+    def __init__(self, grid, fast_decay_rate=2 / day, slow_decay_rate=0.2 / day, fast_redfield=0.1509,
+                 slow_redfield=0.13, fast_fraction=0.74, slow_fraction=0.26, refactory_fraction=0.1,
+                 sedimentation_rate=None, anoxia_half_saturation=1.0,
+                 nitrate_oxidation_params=(-1.9785, 0.2261, -0.0615, -0.0289, -0.36109, -0.0232),
+                 denitrification_params=(-3.0790, 1.7509, 0.0593, -0.1923, 0.0604, 0.0662),
+                 anoxic_params=(-3.9476, 2.6269, -0.2426, -1.3349, 0.1826, -0.0143),
+                 solid_dep_params=(0.233, 0.336, 982.0, -1.548), sinking_nitrogen=("sPOM", "bPOM"),
+                 sinking_carbon=None, sinking_redfield=6.56, timestepper="QuasiAdamsBashforth2"):
+        if sinking_carbon is not None:
+            raise NotImplementedError("the carbon form of SimpleMultiG (sinking_carbon) is not built")
+        if timestepper != "QuasiAdamsBashforth2":
+            raise NotImplementedError("only the QuasiAdamsBashforth2 sediment time stepper is built")
+        self.grid = grid
+        self.fast_decay_rate = float(fast_decay_rate)
+        self.slow_decay_rate = float(slow_decay_rate)
+        self.fast_redfield = float(fast_redfield)
+        self.slow_redfield = float(slow_redfield)
+        self.fast_fraction = float(fast_fraction)
+        self.slow_fraction = float(slow_fraction)
+        self.refactory_fraction = float(refactory_fraction)
+        if sedimentation_rate is None:
+            # 982 |z of the bottom cell centre|^-1.548 cm/year (simple_multi_G.jl:104)
+            sedimentation_rate = 982 * abs(float(grid.zc[0])) ** (-1.548)
+        self.sedimentation_rate = float(sedimentation_rate)
+        self.anoxia_half_saturation = float(anoxia_half_saturation)
+        self.nitrate_oxidation_params = tuple(float(x) for x in nitrate_oxidation_params)
+        self.denitrification_params = tuple(float(x) for x in denitrification_params)
+        self.anoxic_params = tuple(float(x) for x in anoxic_params)
+        self.solid_dep_params = tuple(float(x) for x in solid_dep_params)
+        self.sinking_nitrogen = tuple(sinking_nitrogen)
+        self.sinking_redfield = float(sinking_redfield)
+        self.chi = 0.1   # QuasiAdamsBashforth2TimeStepper default χ
+
+    def required_sediment_fields(self):
+        return ("Ns", "Nf", "Nr")
+
+    def required_tracers(self):
+        return ("NO₃", "NH₄", "O₂")
+
+    def sinking_fluxes(self):
+        return self.sinking_nitrogen
+
+    def coupled_tracers(self):
+        return ("NO₃", "NH₄", "O₂")
+
+    def struct(self, names):
+        """obme_sediment with tracer names resolved against `names`."""
+        s = abi.Sediment()
+        s.model = abi.OBME_SEDIMENT_SIMPLE_MULTI_G
+        for t in self.required_tracers() + self.sinking_nitrogen:
+            if t not in names:
+                raise ValueError(f"{t} is not a tracer of the model (SimpleMultiG needs NO₃, NH₄ and O₂)")
+        if len(self.sinking_nitrogen) > abi.OBME_MAX_SEDIMENT_TRACERS:
+            raise ValueError("too many sinking tracers")
+        for q, t in enumerate(self.sinking_nitrogen):
+            s.sinking[q] = names.index(t)
+        s.n_sinking = len(self.sinking_nitrogen)
+        s.receiver = names.index("NH₄")
+        s.ammonia, s.nitrate, s.oxygen = names.index("NH₄"), names.index("NO₃"), names.index("O₂")
+        s.timestepper = abi.OBME_SEDIMENT_AB2
+        s.ab2_chi = self.chi
+        s.fast_decay_rate = self.fast_decay_rate
+        s.slow_decay_rate = self.slow_decay_rate
+        s.fast_fraction = self.fast_fraction
+        s.slow_fraction = self.slow_fraction
+        s.refactory_fraction = self.refactory_fraction
+        s.sedimentation_rate = self.sedimentation_rate
+        s.anoxia_half_saturation = self.anoxia_half_saturation
+        s.sinking_redfield = self.sinking_redfield
+        for q in range(6):
+            s.nitrate_oxidation_params[q] = self.nitrate_oxidation_params[q]
+            s.denitrification_params[q] = self.denitrification_params[q]
+            s.anoxic_params[q] = self.anoxic_params[q]
+        return s
+
+
 def default_surface_PAR(t):
     """default_surface_PAR (Light/Light.jl:219)."""
     return 100 * max(0.0, np.cos(t * np.pi / (12 * hour)))

@@ -45,6 +45,8 @@ def library():
         "obme_required_tracers": (C.c_int32, [vp, C.c_char_p, C.c_int32]),
         "obme_update_state": (C.c_int32, [vp, C.POINTER(vp), vp, C.c_double, vp]),
         "obme_add_tendencies": (C.c_int32, [vp, C.POINTER(vp), C.POINTER(vp), vp, vp, vp, C.c_double, vp]),
+        "obme_add_tendencies_with_sediment": (C.c_int32, [vp, C.POINTER(vp), C.POINTER(vp), vp, vp, vp, vp,
+                                                          C.c_double, vp]),
         "obme_step": (C.c_int32, [vp, C.POINTER(vp), vp, vp, C.c_int32, C.c_double, C.c_double, vp]),
         "obme_step_with_sediment": (C.c_int32, [vp, C.POINTER(vp), vp, vp, vp, C.c_int32, C.c_double, C.c_double,
                                                 vp]),
@@ -81,7 +83,8 @@ def library():
 
 EXPORTED_SYMBOLS = (
     "obme_create", "obme_destroy", "obme_last_error", "obme_version", "obme_required_tracers",
-    "obme_update_state", "obme_add_tendencies", "obme_step", "obme_step_with_sediment", "obme_step_noalloc",
+    "obme_update_state", "obme_add_tendencies", "obme_add_tendencies_with_sediment", "obme_step",
+    "obme_step_with_sediment", "obme_step_noalloc",
     "obme_sweep_configure", "obme_sweep_waves", "obme_autotune",
     "obme_sediment_update_state", "obme_co2_flux", "obme_abi_sizeof",
     "obme_set_kernel_timing", "obme_kernel_timing", "obme_batch_create", "obme_batch_destroy", "obme_batch_step",
@@ -133,11 +136,15 @@ class HIPBiogeochemistry:
         h = C.c_void_p()
         _check(library().obme_create(C.byref(self.cfg), C.byref(h)))
         self.handle = h
-        # the sediment's prognostic field (BiogeochemicalSediment.fields.storage)
+        # the sediment's prognostic fields (BiogeochemicalSediment.fields):
+        # InstantRemineralisation's storage (Ny, Nx), SimpleMultiG's Ns, Nf,
+        # Nr (3, Ny, Nx)
         self.sediment_storage = None
         if getattr(bgc, "sediment", None) is not None:
             g = bgc.grid
-            self.sediment_storage = torch.zeros((g.Ny, g.Nx), dtype=torch.float64, device="cuda")
+            nf = len(bgc.sediment.required_sediment_fields())
+            shape = (g.Ny, g.Nx) if nf == 1 else (nf, g.Ny, g.Nx)
+            self.sediment_storage = torch.zeros(shape, dtype=torch.float64, device="cuda")
 
     def close(self):
         if getattr(self, "handle", None):
@@ -172,8 +179,15 @@ class HIPBiogeochemistry:
         _check(library().obme_update_state(self.handle, _ptr_array(tracers), _ptr(PAR), t, _stream()))
 
     def update_tendencies(self, tracers, G, PAR=None, T=None, S=None, t=0.0):
-        """Add BGC transitions, drift divergence and the top CO2 flux into G."""
+        """Add BGC transitions, drift divergence, the top gas fluxes and the
+        sediment's coupled tendencies into G (a SimpleMultiG sediment's from
+        its fields, self.sediment_storage)."""
         self._check_fields(list(tracers) + list(G) + ([PAR] if PAR is not None else []))
+        if self.sediment_storage is not None and self.sediment_storage.dim() == 3:
+            _check(library().obme_add_tendencies_with_sediment(
+                self.handle, _ptr_array(tracers), _ptr_array(G), _ptr(PAR), _ptr(T), _ptr(S),
+                _ptr(self.sediment_storage), t, _stream()))
+            return
         _check(library().obme_add_tendencies(self.handle, _ptr_array(tracers), _ptr_array(G), _ptr(PAR),
                                              _ptr(T), _ptr(S), t, _stream()))
 

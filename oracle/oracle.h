@@ -141,19 +141,31 @@ typedef struct orc_column_env {
     struct orc_sediment_column* sed;
 } orc_column_env;
 
-/* One column's share of a BiogeochemicalSediment's state. */
+/* One column's share of a BiogeochemicalSediment's state: its fields
+ * (InstantRemineralisation: storage; SimpleMultiG: Ns, Nf, Nr) and their
+ * Gⁿ, G⁻, and the sediment clock. */
+#define ORC_SEDIMENT_MAX_FIELDS 3
 typedef struct orc_sediment_column {
-    double storage, Gn, Gm;
+    double u[ORC_SEDIMENT_MAX_FIELDS], Gn[ORC_SEDIMENT_MAX_FIELDS], Gm[ORC_SEDIMENT_MAX_FIELDS];
     double last_dt;
     long iteration;
 } orc_sediment_column;
 
-/* InstantRemineralisation: the sinking flux through the bottom face of a
- * column (U[n*Nz + k] layout), its burial efficiency, and one sediment
- * time step (QuasiAdamsBashforth2) from the tracked flux f. */
+/* The number of sediment fields of the configured model (0 without one). */
+int orc_sediment_fields(const obme_sediment* s);
+/* The sinking flux through the bottom face of a column (U[n*Nz + k]
+ * layout), InstantRemineralisation's burial efficiency, the tendencies of
+ * the sediment's fields from their values u and the tracked flux f, and one
+ * sediment time step (QuasiAdamsBashforth2) from the tracked flux f. */
 double orc_sediment_flux(const obme_config* cfg, const double* U, int Nz);
 double orc_burial_efficiency(const obme_sediment* s, double f);
+void orc_sediment_tendencies(const obme_sediment* s, const double* u, double f, double* G);
 void orc_sediment_time_step(const obme_sediment* s, orc_sediment_column* c, double f, double dt);
+/* SimpleMultiG's coupled tendencies (per unit area, before the division by
+ * Δz) of NH4, NO3 and O2 from the fields u = (Ns, Nf, Nr) and the bottom
+ * cell's NO3, NH4, O2 (simple_multi_G.jl:193-247, 384-426). */
+void orc_smg_coupled(const obme_sediment* s, const double* u, double NO3, double NH4, double O2,
+                     double* dNH4, double* dNO3, double* dO2);
 
 /* U[n*Nz + k]: tracer n at level k (0-based, bottom to top).  Advances nsteps
  * in place; if PAR_out != NULL writes the PAR of the last stage evaluated. */
@@ -174,8 +186,9 @@ void orc_column_tendencies(const orc_column_env* env, const double* U, double t,
 int orc_grid_run(const obme_config* cfg, double* const* tracers, const double* T3,
                  const double* S3, const double* PAR0, int nsteps, double dt, double t0,
                  int nthreads, int fast_exit);
-/* orc_grid_run with the sediment: storage, Gn, Gm host Nx*Ny (i fastest),
- * the sediment clock in *last_dt, *iteration (Inf and 0 before its first step). */
+/* orc_grid_run with the sediment: storage, Gn, Gm host fields x Nx*Ny (i
+ * fastest, field after field), the sediment clock in *last_dt, *iteration
+ * (Inf and 0 before its first step). */
 int orc_grid_run_sediment(const obme_config* cfg, double* const* tracers, const double* T3,
                           const double* S3, const double* PAR0, int nsteps, double dt, double t0,
                           int nthreads, int fast_exit, double* storage, double* Gn, double* Gm,
@@ -194,6 +207,11 @@ int orc_grid_update_state(const obme_config* cfg, const double* const* tracers,
 int orc_grid_tendencies(const obme_config* cfg, const double* const* tracers,
                         double* const* G, const double* T3, const double* S3,
                         const double* PAR0, double t, double* PAR);
+/* orc_grid_tendencies with the sediment's fields (host, fields x Nx*Ny, i
+ * fastest; NULL: an empty sediment) for the coupled tendencies. */
+int orc_grid_tendencies_sediment(const obme_config* cfg, const double* const* tracers,
+                                 double* const* G, const double* T3, const double* S3,
+                                 const double* PAR0, double t, double* PAR, const double* fields);
 
 /* ---- batched ensemble of independent NPZD columns ---- */
 /* offsets[c]..offsets[c+1]: levels of column c, k fastest, bottom to top.

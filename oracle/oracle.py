@@ -102,6 +102,11 @@ def load_library(path):
                                                C.POINTER(C.c_long), D]),
         "orc_sediment_flux": (D, [C.POINTER(abi.Config), DP, I]),
         "orc_burial_efficiency": (D, [C.POINTER(abi.Sediment), D]),
+        "orc_sediment_fields": (I, [C.POINTER(abi.Sediment)]),
+        "orc_sediment_tendencies": (None, [C.POINTER(abi.Sediment), DP, D, DP]),
+        "orc_smg_coupled": (None, [C.POINTER(abi.Sediment), DP, D, D, D, DP, DP, DP]),
+        "orc_grid_tendencies_sediment": (I, [C.POINTER(abi.Config), C.POINTER(DP), C.POINTER(DP), DP, DP, DP, D, DP,
+                                             DP]),
         "orc_batch_run": (I, [C.POINTER(abi.Config), I, C.POINTER(I), C.POINTER(DP), D, D, C.POINTER(DP), I, D,
                               D, I]),
     }
@@ -314,13 +319,16 @@ def grid_run(cfg, tracers, nsteps, dt, t0=0.0, T3=None, S3=None, PAR0=None, nthr
 
 
 class SedimentState:
-    """A BiogeochemicalSediment's state on the oracle side: the storage field
-    and its time stepper's Gⁿ, G⁻ (Ny, Nx), and its clock (last Δt, iteration)."""
+    """A BiogeochemicalSediment's state on the oracle side: its fields and
+    their time stepper's Gⁿ, G⁻ ((Ny, Nx) for one field, as
+    InstantRemineralisation's storage; (fields, Ny, Nx) for more, as
+    SimpleMultiG's Ns, Nf, Nr), and its clock (last Δt, iteration)."""
 
-    def __init__(self, Nx, Ny):
-        self.storage = np.zeros((Ny, Nx))
-        self.Gn = np.zeros((Ny, Nx))
-        self.Gm = np.zeros((Ny, Nx))
+    def __init__(self, Nx, Ny, fields=1):
+        shape = (Ny, Nx) if fields == 1 else (fields, Ny, Nx)
+        self.storage = np.zeros(shape)
+        self.Gn = np.zeros(shape)
+        self.Gm = np.zeros(shape)
         self.last_dt = math.inf
         self.iteration = 0
 
@@ -358,6 +366,26 @@ def burial_efficiency(sediment, f):
     return lib().orc_burial_efficiency(C.byref(sediment), float(f))
 
 
+def sediment_fields(sediment):
+    return lib().orc_sediment_fields(C.byref(sediment))
+
+
+def sediment_tendencies(sediment, u, f):
+    """The tendencies of the sediment's fields u from the tracked flux f."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    G = np.zeros(3)
+    lib().orc_sediment_tendencies(C.byref(sediment), _dp(u), float(f), _dp(G))
+    return G[:sediment_fields(sediment)]
+
+
+def smg_coupled(sediment, u, NO3, NH4, O2):
+    """SimpleMultiG's coupled tendencies (NH4, NO3, O2), per unit area."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = [C.c_double(), C.c_double(), C.c_double()]
+    lib().orc_smg_coupled(C.byref(sediment), _dp(u), float(NO3), float(NH4), float(O2), *[C.byref(o) for o in out])
+    return tuple(o.value for o in out)
+
+
 def grid_run_columns(cfg, tracers, cols_i, cols_j, nsteps, dt, t0=0.0, T3=None, S3=None, PAR0=None,
                      nthreads=8, fast_exit=1):
     ci = np.ascontiguousarray(cols_i, dtype=np.int32)
@@ -383,9 +411,12 @@ def grid_update_state(cfg, tracers, PAR, t=0.0, PAR0=None):
     lib().orc_grid_update_state(C.byref(cfg), _ptrs(tracers), _dp(PAR0), _dp(PAR), t)
 
 
-def grid_tendencies(cfg, tracers, G, t=0.0, T3=None, S3=None, PAR0=None, PAR=None):
-    lib().orc_grid_tendencies(C.byref(cfg), _ptrs(tracers), _ptrs(G), _dp(T3), _dp(S3), _dp(PAR0), t,
-                              _dp(PAR))
+def grid_tendencies(cfg, tracers, G, t=0.0, T3=None, S3=None, PAR0=None, PAR=None, sediment_fields=None):
+    """Tendencies added into G; sediment_fields: the sediment's fields (a
+    SimpleMultiG's (3, Ny, Nx)) for its coupled tendencies, or None."""
+    fields = None if sediment_fields is None else np.ascontiguousarray(sediment_fields, dtype=np.float64)
+    lib().orc_grid_tendencies_sediment(C.byref(cfg), _ptrs(tracers), _ptrs(G), _dp(T3), _dp(S3), _dp(PAR0), t,
+                                       _dp(PAR), _dp(fields))
 
 
 def batch_run(cfg, offsets, params, dz, T_const, U, nsteps, dt, t0=0.0, nthreads=8):

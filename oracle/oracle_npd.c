@@ -435,24 +435,104 @@ double orc_burial_efficiency(const obme_sediment* s, double f)
     return s->burial_efficiency_constant1 + s->burial_efficiency_constant2 * (x * x);
 }
 
+int orc_sediment_fields(const obme_sediment* s)
+{
+    if (s->model == OBME_SEDIMENT_INSTANT_REMINERALISATION) return 1;
+    if (s->model == OBME_SEDIMENT_SIMPLE_MULTI_G) return 3;
+    return 0;
+}
+
+/* The tendencies of the sediment's fields (compute_tendencies.jl:7-31):
+ * InstantRemineralisation e f (instant_remineralisation.jl:101-110);
+ * SimpleMultiG's nitrogen form (simple_multi_G.jl:165-193):
+ * Ns: (1 - fr) fs f - λs Ns, Nf: (1 - fr) ff f - λf Nf, Nr: fr f. */
+void orc_sediment_tendencies(const obme_sediment* s, const double* u, double f, double* G)
+{
+    if (s->model == OBME_SEDIMENT_INSTANT_REMINERALISATION) {
+        G[0] = orc_burial_efficiency(s, f) * f;
+    } else if (s->model == OBME_SEDIMENT_SIMPLE_MULTI_G) {
+        const double fr = s->refactory_fraction;
+        G[0] = (1 - fr) * s->slow_fraction * f - s->slow_decay_rate * u[0];
+        G[1] = (1 - fr) * s->fast_fraction * f - s->fast_decay_rate * u[1];
+        G[2] = fr * f;
+    }
+}
+
+/* SimpleMultiG's fractions (simple_multi_G.jl:372-427), day = 86400 s;
+ * a non-finite fraction is zero, as the reference's ifelse(isfinite(p), p, 0). */
+static const double SMG_DAY = 86400.0;
+
+static double smg_finite(double p) { return isfinite(p) ? p : 0.0; }
+
+static double smg_ammonia_oxidation(const obme_sediment* s, double Nr, double Cr, double k, double NH4, double O2)
+{
+    const double* q = s->nitrate_oxidation_params;
+    double lC = log(Cr * SMG_DAY);
+    double ln = (q[0] + q[1] * lC * log(O2) + q[2] * (lC * lC) + q[3] * log(k * SMG_DAY) * log(NH4) + q[4] * lC
+                 + q[5] * lC * log(NH4));
+    double p = exp(ln) / (Nr * SMG_DAY) * O2 / (s->anoxia_half_saturation + O2);
+    return smg_finite(p);
+}
+
+static double smg_denitrification(const obme_sediment* s, double Cr, double k, double NO3, double O2)
+{
+    const double* q = s->denitrification_params;
+    double lC = log(Cr * SMG_DAY), lN = log(NO3), lk = log(k * SMG_DAY);
+    double ln = (q[0] + q[1] * lC + q[2] * (lN * lN) + q[3] * (lC * lC) + q[4] * (lk * lk) + q[5] * log(O2) * log(k));
+    double p = exp(ln) / (Cr * SMG_DAY) * O2 / (s->anoxia_half_saturation + O2);
+    return smg_finite(p);
+}
+
+static double smg_anoxic(const obme_sediment* s, double Cr, double k, double NO3, double O2)
+{
+    const double* q = s->anoxic_params;
+    double lC = log(Cr * SMG_DAY), lN = log(NO3);
+    double ln = (q[0] + q[1] * lC + q[2] * (lC * lC) + q[3] * log(k * SMG_DAY) + q[4] * log(O2) * log(k)
+                 + q[5] * (lN * lN));
+    double p = exp(ln) / (Cr * SMG_DAY);
+    return smg_finite(p);
+}
+
+void orc_smg_coupled(const obme_sediment* s, const double* u, double NO3, double NH4, double O2,
+                     double* dNH4, double* dNO3, double* dO2)
+{
+    const double R = s->sinking_redfield, ls = s->slow_decay_rate, lf = s->fast_decay_rate;
+    const double Ns = u[0], Nf = u[1];
+    const double Nr = ls * Ns + lf * Nf;
+    const double Cr = Nr * R;
+    /* reactivity(s, Ns R, Nf R) (simple_multi_G.jl:372-379), eps(0.0) = 5e-324 */
+    const double Cs = Ns * R, Cf = Nf * R;
+    const double k = (ls * Cs + lf * Cf) / (Cs + Cf + 4.9406564584124654e-324);
+    const double pn = smg_ammonia_oxidation(s, Nr, Cr, k, NH4, O2);
+    const double pd = smg_denitrification(s, Cr, k, NO3, O2);
+    const double pa = smg_anoxic(s, Cr, k, NO3, O2);
+    const double ps = 0.223 * pow(s->sedimentation_rate, 0.336);
+    *dNH4 = (1 - pn) * Nr + 0.8 * pd * Cr;                       /* :195-209 */
+    *dNO3 = pn * Nr - 0.8 * pd * Cr;                             /* :211-226 */
+    *dO2 = -(1 - pa * ps - pd) * O2 / (s->anoxia_half_saturation + O2) * Cr - 2 * pn * Nr;   /* :228-247 */
+}
+
 /* time_step!(sediment, Δt) of update_biogeochemical_state!
  * (Sediments/update_state.jl:6-17) with its QuasiAdamsBashforth2 stepper:
  * at iteration 0 the tendencies are computed first (update_state!); the
  * step is forward Euler (χ = -1/2, G⁻ multiplied by a strong zero) when Δt
  * differs from the clock's last Δt, else (3/2 + χ) Gⁿ - (1/2 + χ) G⁻
  * (timesteppers.jl:16-43); then G⁻ = Gⁿ (cache_previous_tendencies!,
- * :81-91) and Gⁿ = e f from the tracked flux (compute_tendencies.jl:7-31). */
+ * :81-91) and Gⁿ from the new fields and the tracked flux
+ * (compute_tendencies.jl:7-31). */
 void orc_sediment_time_step(const obme_sediment* s, orc_sediment_column* c, double f, double dt)
 {
     if (!isfinite(dt)) return;
-    double ge = orc_burial_efficiency(s, f) * f;
-    if (c->iteration == 0) c->Gn = ge;
+    int nf = orc_sediment_fields(s);
+    if (c->iteration == 0) orc_sediment_tendencies(s, c->u, f, c->Gn);
     int euler = !(dt == c->last_dt);
     double chi = euler ? -0.5 : s->ab2_chi;
-    double Gu = (1.5 + chi) * c->Gn - (euler ? 0.0 : (0.5 + chi) * c->Gm);
-    c->storage = c->storage + dt * Gu;
-    c->Gm = c->Gn;
-    c->Gn = ge;
+    for (int q = 0; q < nf; q++) {
+        double Gu = (1.5 + chi) * c->Gn[q] - (euler ? 0.0 : (0.5 + chi) * c->Gm[q]);
+        c->u[q] = c->u[q] + dt * Gu;
+        c->Gm[q] = c->Gn[q];
+    }
+    orc_sediment_tendencies(s, c->u, f, c->Gn);
     c->last_dt = dt;
     c->iteration += 1;
 }
@@ -534,6 +614,18 @@ void orc_column_tendencies(const orc_column_env* env, const double* U, double t,
         double e = orc_burial_efficiency(&cfg->sediment, f);
         int r = cfg->sediment.receiver;
         G[r * Nz + 0] = G[r * Nz + 0] + ((1 - e) * f) / env->dz[0];
+    } else if (cfg->sediment.model == OBME_SEDIMENT_SIMPLE_MULTI_G) {
+        /* from the sediment's fields now (an empty sediment without them)
+         * and the bottom cell's tracked NO3, NH4, O2 */
+        const obme_sediment* sd = &cfg->sediment;
+        const double zero[3] = {0.0, 0.0, 0.0};
+        const double* u = env->sed ? env->sed->u : zero;
+        double dNH4, dNO3, dO2;
+        orc_smg_coupled(sd, u, U[sd->nitrate * Nz + 0], U[sd->ammonia * Nz + 0], U[sd->oxygen * Nz + 0], &dNH4,
+                        &dNO3, &dO2);
+        G[sd->ammonia * Nz + 0] = G[sd->ammonia * Nz + 0] + dNH4 / env->dz[0];
+        G[sd->nitrate * Nz + 0] = G[sd->nitrate * Nz + 0] + dNO3 / env->dz[0];
+        G[sd->oxygen * Nz + 0] = G[sd->oxygen * Nz + 0] + dO2 / env->dz[0];
     }
 }
 
@@ -553,7 +645,7 @@ void orc_column_run(const orc_column_env* env, double* U, int nsteps, double dt,
     const double z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
     int readonly = (env->cfg->model == OBME_NPZD) ? 3 : -1;
     const obme_sediment* sd = &env->cfg->sediment;
-    int sed = env->sed && sd->model == OBME_SEDIMENT_INSTANT_REMINERALISATION;
+    int sed = env->sed && orc_sediment_fields(sd) > 0;
     /* the modifiers run first in update_biogeochemical_state!
      * (src/OceanBioME.jl:161-167): scaling before every tendency evaluation;
      * after every substep the update-state hooks end with the sediment's
@@ -671,26 +763,30 @@ int orc_grid_run_sediment(const obme_config* cfg, double* const* tracers, const 
 {
     const obme_grid* g = &cfg->grid;
     long ncols = (long)g->Nx * g->Ny;
-    int sed = cfg->sediment.model == OBME_SEDIMENT_INSTANT_REMINERALISATION && storage && Gn && Gm &&
-              last_dt && iteration;
+    const int nf = orc_sediment_fields(&cfg->sediment);
+    int sed = nf > 0 && storage && Gn && Gm && last_dt && iteration;
     double last_dt_after = sed ? *last_dt : 0.0;
     long iteration_after = sed ? *iteration : 0;
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 4)
     for (long c = 0; c < ncols; c++) {
         orc_sediment_column sc;
         if (sed) {
-            sc.storage = storage[c];
-            sc.Gn = Gn[c];
-            sc.Gm = Gm[c];
+            for (int q = 0; q < nf; q++) {
+                sc.u[q] = storage[q * ncols + c];
+                sc.Gn[q] = Gn[q * ncols + c];
+                sc.Gm[q] = Gm[q * ncols + c];
+            }
             sc.last_dt = *last_dt;
             sc.iteration = *iteration;
         }
         run_one_column(cfg, tracers, T3, S3, PAR0, (int)(c % g->Nx), (int)(c / g->Nx), nsteps, dt, t0,
                        fast_exit, sed ? &sc : NULL);
         if (sed) {
-            storage[c] = sc.storage;
-            Gn[c] = sc.Gn;
-            Gm[c] = sc.Gm;
+            for (int q = 0; q < nf; q++) {
+                storage[q * ncols + c] = sc.u[q];
+                Gn[q * ncols + c] = sc.Gn[q];
+                Gm[q * ncols + c] = sc.Gm[q];
+            }
             if (c == 0) {
                 last_dt_after = sc.last_dt;
                 iteration_after = sc.iteration;
@@ -719,19 +815,25 @@ int orc_grid_sediment_update_state(const obme_config* cfg, const double* const* 
     int nt = cfg_ntracers(cfg);
     double* B = (double*)malloc(sizeof(double) * nt);
     long ncols = (long)g->Nx * g->Ny;
-    orc_sediment_column sc = {0};
+    orc_sediment_column sc;
+    memset(&sc, 0, sizeof(sc));
+    const int nf = orc_sediment_fields(&cfg->sediment);
     for (long c = 0; c < ncols; c++) {
         int i = (int)(c % g->Nx), j = (int)(c / g->Nx);
         for (int n = 0; n < nt; n++) B[n] = tracers[n][lidx(g, i, j, 0)];
-        sc.storage = storage[c];
-        sc.Gn = Gn[c];
-        sc.Gm = Gm[c];
+        for (int q = 0; q < nf; q++) {
+            sc.u[q] = storage[q * ncols + c];
+            sc.Gn[q] = Gn[q * ncols + c];
+            sc.Gm[q] = Gm[q * ncols + c];
+        }
         sc.last_dt = *last_dt;
         sc.iteration = *iteration;
         orc_sediment_time_step(&cfg->sediment, &sc, orc_sediment_flux(cfg, B, 1), dt);
-        storage[c] = sc.storage;
-        Gn[c] = sc.Gn;
-        Gm[c] = sc.Gm;
+        for (int q = 0; q < nf; q++) {
+            storage[q * ncols + c] = sc.u[q];
+            Gn[q * ncols + c] = sc.Gn[q];
+            Gm[q * ncols + c] = sc.Gm[q];
+        }
     }
     if (ncols > 0) {
         *last_dt = sc.last_dt;
@@ -787,6 +889,15 @@ int orc_grid_update_state(const obme_config* cfg, const double* const* tracers, 
 int orc_grid_tendencies(const obme_config* cfg, const double* const* tracers, double* const* G,
                         const double* T3, const double* S3, const double* PAR0, double t, double* PAR)
 {
+    return orc_grid_tendencies_sediment(cfg, tracers, G, T3, S3, PAR0, t, PAR, NULL);
+}
+
+int orc_grid_tendencies_sediment(const obme_config* cfg, const double* const* tracers, double* const* G,
+                                 const double* T3, const double* S3, const double* PAR0, double t, double* PAR,
+                                 const double* fields)
+{
+    const int nf = orc_sediment_fields(&cfg->sediment);
+    const long ncols = (long)cfg->grid.Nx * cfg->grid.Ny;
     const obme_grid* g = &cfg->grid;
     int Nz = g->Nz;
     int nt = cfg_ntracers(cfg);
@@ -799,6 +910,12 @@ int orc_grid_tendencies(const obme_config* cfg, const double* const* tracers, do
             orc_column_env env;
             env_for_column(cfg, T3, S3, i, j, Tcol, &env, 1);
             if (PAR0) env.PAR0_const = PAR0[i + g->Nx * j];
+            orc_sediment_column sc;
+    memset(&sc, 0, sizeof(sc));
+            if (fields && nf > 0) {
+                for (int q = 0; q < nf; q++) sc.u[q] = fields[q * ncols + i + (long)g->Nx * j];
+                env.sed = &sc;
+            }
             for (int n = 0; n < nt; n++)
                 for (int k = 0; k < Nz; k++) U[n * Nz + k] = tracers[n][lidx(g, i, j, k)];
             if (cfg->model == OBME_NPZD) {
