@@ -13,7 +13,9 @@
  * the budgets of its tests, and closed forms at special states of every
  * NPZD / LOBSTER transition term and of the sinking reconstructions, each
  * shown to catch a planted mistake (tests/test_oracle_pins_npd.py,
- * tests/test_oracle_sinking_pins.py).
+ * tests/test_oracle_sinking_pins.py); SimpleMultiG's nitrogen balances, its
+ * fractions where their logarithms vanish and the reference's budget
+ * (tests/test_oracle_sediment_smg.py).
  *
  * Parity unpinned (no reference-held value or closed form fixes the bits):
  * orc_teos10_density away from the reference's doctest point; the RK3
