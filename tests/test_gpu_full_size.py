@@ -5,7 +5,8 @@ of those columns; the halos and the read-only fields stay untouched; every
 written value stays finite.  LOBSTER + DIC/Alk with the CO2 exchange on
 512x512x64 (the headline workload) in both precisions, the same with the
 O2 tracer and its gas exchange in FP32-mixed, and PISCES on 256x256x64
-for its 100 steps in both precisions.  Bars as everywhere: FP64 <= 1e-12
+for its 100 steps in both precisions, and the carbonate chemistry on all 2^24
+surface points of its configuration in both modes.  Bars as everywhere: FP64 <= 1e-12
 relative L2 per tracer, FP32-mixed <= 1e-5 (PISCES around the values of
 validation/PISCES/column.jl: or three times the FP32 storage floor, the
 deviation the FP64 oracle shows when its state is rounded to FP32 after
@@ -124,3 +125,32 @@ def test_pisces_256x256x64_column_sample(precision):
     # T and S are read only
     dt_np = np.float32 if precision == "fp32" else np.float64
     np.testing.assert_array_equal(tr[24].cpu().numpy(), wl.tracers[24].astype(dt_np))
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_carbonate_points_2pow24(mode):
+    """pH and CO2 flux on the 2^24 points of the carbonate configuration:
+    mode 0 (the reference damped Newton) and mode 1 (FP32 guess refined by
+    FP64 Newton) against the oracle's reference solve on every point: pH
+    within 1e-12 absolute, the flux within 1e-10 relative L2."""
+    import os
+    import torch
+    print("case carbonate 2^24 mode", mode, flush=True)
+    pts = workloads.carbonate_points()
+    wl = workloads.lobster_3d(4, 4, 4, nsteps=1)
+    dev = HIPBiogeochemistry(wl.bgc, "fp64")
+    cfg = wl.bgc.config(abi.OBME_FP64)
+    nthreads = max(1, min(16, os.cpu_count() or 1))
+    pH_ref, F_ref = O.co2_flux_batch(cfg.co2, cfg.solver, pts["DIC"], pts["Alk"], pts["T"], pts["S"],
+                                     pts["u10"], pts["pCO2_air"], nthreads=nthreads)
+    args = [torch.from_numpy(pts[k]).cuda() for k in ("DIC", "Alk", "T", "S", "u10")]
+    pH, F = dev.co2_flux(*args, pCO2_air=pts["pCO2_air"], mode=mode)
+    torch.cuda.synchronize()
+    pH = pH.cpu().numpy()
+    F = F.cpu().numpy()
+    assert np.all(np.isfinite(pH)) and np.all(np.isfinite(F))
+    dpH = float(np.max(np.abs(pH - pH_ref)))
+    dF = float(np.linalg.norm(F - F_ref) / np.linalg.norm(F_ref))
+    print("pH max abs", dpH, "flux rel L2", dF, flush=True)
+    assert dpH <= 1e-12
+    assert dF <= 1e-10
