@@ -46,9 +46,10 @@ def parse():
     p.add_argument("--no-check", action="store_true")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--precisions", default="fp32,fp64")
-    p.add_argument("--workload", default="lobster", choices=["lobster", "columns", "carbonate", "pisces"],
+    p.add_argument("--workload", default="lobster", choices=["lobster", "columns", "carbonate", "pisces", "hooks"],
                    help="lobster: the headline LOBSTER+DIC/Alk step; columns: the ragged NPZD column "
-                        "ensemble; carbonate: pH and CO2 flux on point sets")
+                        "ensemble; carbonate: pH and CO2 flux on point sets; hooks: the Oceananigans "
+                        "hook pair (update-state, then tendencies added into G) on the headline grid")
     p.add_argument("--oxygen", action="store_true",
                    help="lobster workload with the O2 tracer and its gas exchange (10 written tracers)")
     p.add_argument("--sinking", default="upwind", choices=["upwind", "centered", "weno5"],
@@ -170,6 +171,8 @@ def main():
         return bench_carbonate(args)
     if args.workload == "pisces":
         return bench_pisces(args)
+    if args.workload == "hooks":
+        return bench_hooks(args)
     import torch
     from oceanbiome import abi, workloads
     from oceanbiome.runtime import HIPBiogeochemistry
@@ -289,6 +292,89 @@ def main():
         "errors_vs_oracle": errors,
     }
     print(json.dumps(line))
+
+
+def bench_hooks(args):
+    """The hooks a coupled Oceananigans model calls once per RK3 stage:
+    obme_update_state (the PAR field from P) and obme_add_tendencies (the
+    transitions, the sinking divergence and the top CO2 flux added into the
+    model's Gⁿ), on the headline workload.  One "stage" is the pair; the
+    substep itself is the model's.  Each hook is timed on its own (HIP events
+    around every sweep launch) and the pair together; the tendency sweep's
+    algorithmic bytes per cell: 9 tracers read, 9 tendencies read and
+    written, PAR written (FP32: 112 B; FP64: 224 B)."""
+    import torch
+    from oceanbiome import workloads
+    from oceanbiome.runtime import HIPBiogeochemistry
+
+    wl = workloads.lobster_3d(args.nx, args.ny, args.nz, nsteps=1)
+    g = wl.grid
+    ncell = g.n_interior
+    nw = wl.n_written
+    res = {}
+    for prec in args.precisions.split(","):
+        dev = HIPBiogeochemistry(wl.bgc, prec)
+        tr = [torch.from_numpy(a).to(device="cuda", dtype=dev.dtype) for a in wl.tracers]
+        G = [torch.zeros_like(t) for t in tr]
+        PAR = torch.zeros_like(tr[0])
+        es = 4 if prec == "fp32" else 8
+        stages = max(1, args.steps)
+
+        def pair(t):
+            dev.update_biogeochemical_state(tr, PAR, t)
+            dev.update_tendencies(tr, G, PAR, t=t)
+
+        for q in range(args.warmup):
+            pair(q * wl.dt)
+        torch.cuda.synchronize()
+        s0, s1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s0.record()
+        for q in range(stages):
+            pair(q * wl.dt)
+        s1.record()
+        torch.cuda.synchronize()
+        pair_ms = s0.elapsed_time(s1) / stages
+        dev.set_kernel_timing(True)
+        dev.kernel_timing()
+        for q in range(stages):
+            dev.update_biogeochemical_state(tr, PAR, q * wl.dt)
+        torch.cuda.synchronize()
+        par_ms, n1 = dev.kernel_timing()
+        for q in range(stages):
+            dev.update_tendencies(tr, G, PAR, t=q * wl.dt)
+        torch.cuda.synchronize()
+        tend_ms, n2 = dev.kernel_timing()
+        surf_ms, n3 = dev.kernel_timing(which=1)
+        dev.set_kernel_timing(False)
+        par_ms, tend_ms = par_ms / max(n1, 1), tend_ms / max(n2, 1)
+        tbytes = ncell * (3 * nw + 1) * es
+        pbytes = ncell * 2 * es          # P read, PAR written
+        res[prec] = {"ms_per_stage": pair_ms, "tendency_evaluations_per_s": nw * ncell / (pair_ms * 1e-3),
+                     "update_state_sweep_ms": par_ms, "update_state_sweep_GBs": pbytes / (par_ms * 1e-3) / 1e9,
+                     "tendency_sweep_ms": tend_ms,
+                     "tendency_sweep_bytes": tbytes, "tendency_sweep_GBs": tbytes / (tend_ms * 1e-3) / 1e9,
+                     "surface_flux_ms": surf_ms / max(n3, 1)}
+        log(f"{prec}: {pair_ms:.4f} ms per stage (update-state {par_ms:.4f} ms = "
+            f"{res[prec]['update_state_sweep_GBs']:.1f} GB/s, tendencies {tend_ms:.4f} ms "
+            f"= {res[prec]['tendency_sweep_GBs']:.1f} GB/s, surface flux {res[prec]['surface_flux_ms']:.4f} ms)")
+        del tr, G, PAR, dev
+        torch.cuda.empty_cache()
+    head_prec = "fp32" if "fp32" in res else "fp64"
+    head = res[head_prec]
+    print(json.dumps({
+        "metric": "tracer-cell tendency evaluations/s of the coupled-model hooks (update-state + tendencies)",
+        "value": head["tendency_evaluations_per_s"], "unit": "tracer-cell tendency evaluations/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms_per_stage"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if head_prec == "fp32" else "f64", "data": "synthetic (seeded lognormal/uniform tracers)",
+        "config": {"workload": wl.name + "_hooks", "grid": [g.Nx, g.Ny, g.Nz], "tracers_written": nw,
+                   "one_step_is": "one stage: update-state then tendencies"},
+        "roofline": {"bound": "hbm", "achieved": head["tendency_sweep_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": head["tendency_sweep_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_column_lane (tendency mode)",
+                     "algorithmic_bytes_per_launch": head["tendency_sweep_bytes"],
+                     "launch_ms": head["tendency_sweep_ms"]},
+        "cpu_baseline": None, "results": res}))
 
 
 def bench_columns(args):

@@ -52,7 +52,8 @@ extern "C" const char* obme_version(void) { return "obme 0.1.0 gfx950"; }
 constexpr int SWEEP_MAX_THREADS = 512;
 template <int MODEL, int CARB, typename FT, int MODE, bool UNIFORM, int FM = -1, int EXTRA = 0, bool MASKX = true,
           int SCHEME = OBME_UPWIND1>
-__global__ void __launch_bounds__(SWEEP_MAX_THREADS, sizeof(FT) == 4 ? 4 : 1) k_column_lane(StepArgs<FT> a)
+__global__ void __launch_bounds__(SWEEP_MAX_THREADS, (sizeof(FT) == 4 && (EXTRA & 2) == 0) ? 4 : 1)
+k_column_lane(StepArgs<FT> a)
 {
     column_lane<MODEL, CARB, FT, MODE, UNIFORM, FM, EXTRA, MASKX, true, SCHEME>(a, (int)blockDim.x);
 }
@@ -771,13 +772,27 @@ static obme_status launch_lane(obme_handle* h, const StepArgs<FT>& a, hipStream_
     } else {
         k = h->uniform ? k_column_lane<MODEL, CARB, FT, MODE, true> : k_column_lane<MODEL, CARB, FT, MODE, false>;
     }
-    if constexpr (MODE == TILE_TENDENCY) {
-        if (h->scheme == OBME_WENO5) k = k_column_lane<MODEL, CARB, FT, MODE, false, -1, 0, true, OBME_WENO5>;
-        if (h->scheme == OBME_CENTERED2) k = k_column_lane<MODEL, CARB, FT, MODE, false, -1, 0, true, OBME_CENTERED2>;
-    }
-    // the fused step on rows of whole workgroups (Nx a multiple of 64 NW):
-    // stores without a mask; FP32 compiled with the model's default formulations
+    // rows of whole workgroups (Nx a multiple of 64 NW): stores without a
+    // mask; the FP32 step and tendencies compiled with the model's default
+    // formulations
     const bool whole = h->cfg.grid.Nx % cw == 0;
+    if constexpr (MODE == TILE_TENDENCY) {
+        if (h->scheme == OBME_WENO5) {
+            k = k_column_lane<MODEL, CARB, FT, MODE, false, -1, 0, true, OBME_WENO5>;
+        } else if (h->scheme == OBME_CENTERED2) {
+            k = k_column_lane<MODEL, CARB, FT, MODE, false, -1, 0, true, OBME_CENTERED2>;
+        } else if (whole && h->uniform) {
+            if constexpr (sizeof(FT) == 4) {
+                if (fm_code(a.f) == default_fm<MODEL>())
+                    k = k_column_lane<MODEL, CARB, FT, MODE, true, default_fm<MODEL>(), 0, false>;
+            } else {
+                k = k_column_lane<MODEL, CARB, FT, MODE, true, -1, 0, false>;
+            }
+        }
+    }
+    if constexpr (MODE == TILE_PAR) {
+        if (whole && h->uniform) k = k_column_lane<MODEL, CARB, FT, MODE, true, -1, 0, false>;
+    }
     if constexpr (MODE == TILE_STEP) {
         if constexpr (sizeof(FT) == 4) {
             if (fm_code(a.f) == default_fm<MODEL>()) {

@@ -11,9 +11,11 @@
 //    own top-down order; Kahan-compensated in FP32);
 //  - upwind sinking needs c(k+1) of the same stage, one level of history.
 // Every load and store of a level is one contiguous run of 64 values per
-// wave, straight from and to HBM (no LDS staging, no cross-lane traffic,
-// no barriers); the next level's loads are issued before the current
-// level's arithmetic.  Each tracer is read and written once per step.
+// wave, straight from and to HBM (no LDS staging, no cross-lane traffic);
+// the workgroup's waves pass a barrier (no fence) before each level, so
+// their runs reach the memory together as one run of 64 NW values.  The next
+// level's loads are issued before the current level's arithmetic.  Each
+// tracer is read and written once per step.
 //  - FP32-mixed: the step's increment D is accumulated apart from the state
 //    at the start of the step U0 (stage inputs U0 + D, one rounding at the
 //    store), so no stage rounds an increment against the magnitude of the
@@ -406,9 +408,10 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
     };
     // sediment (EXTRA & 4): the storage value, the tendency e f of the last
     // stage start and the G⁻ of the sediment's previous step
-    // (fields: InstantRemineralisation's storage, SimpleMultiG's Ns, Nf, Nr)
+    // (fields: InstantRemineralisation's storage, SimpleMultiG's Ns, Nf, Nr),
+    // loaded just before the bottom level, the only one that uses them
     double sed_u[3] = {0.0, 0.0, 0.0}, sed_ge[3] = {0.0, 0.0, 0.0}, sed_gm[3] = {0.0, 0.0, 0.0};
-    if constexpr ((EXTRA & 4) != 0) {
+    auto load_sediment = [&] {
         if (col_ok) {
 #pragma unroll
             for (int q = 0; q < 3; q++) {
@@ -416,7 +419,7 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
                 if (q < a.sed_nf && a.sed_ab2) sed_gm[q] = a.sedG[(a.sed_nf + q) * ncol2d + col2d];
             }
         }
-    }
+    };
 
     const double g1 = 8.0 / 15.0, g2 = 5.0 / 12.0, g3 = 3.0 / 4.0, z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
 
@@ -563,9 +566,8 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
                 pr = m_exp2(fmaf(a.er, lg, a.cr));
                 pb = m_exp2(fmaf(a.eb, lg, a.cb));
             } else {
-                const FT x = X[PI] * a.Rr;
-                pr = m_pow(x, a.er);
-                pb = m_pow(x, a.eb);
+                // both bands' powers from one logarithm
+                pow_pair(X[PI] * a.Rr, a.er, a.eb, pr, pb);
             }
             FT ir, ib;
             if constexpr (F32 && UNIFORM) {
@@ -910,7 +912,21 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
             level_barrier();
             level(0, F_{}, T_{});
         }
+    } else if constexpr ((EXTRA & 4) != 0 && !((EXTRA & 1) != 0 && (CARB & 1) != 0)) {
+        // the levels above the bottom compiled apart from it: the sediment's
+        // state and code are not carried through the column walk (with the
+        // scaling and the carbonate system together this form spills more
+        // than the single loop below, which those variants keep)
+#pragma unroll 1
+        for (int k = ktop; k >= 1; k--) {
+            level_barrier();
+            level(k, k == ktop, F_{});
+        }
+        load_sediment();
+        level_barrier();
+        level(0, ktop == 0, T_{});
     } else {
+        if constexpr ((EXTRA & 4) != 0) load_sediment();
 #pragma unroll 1
         for (int k = ktop; k >= 0; k--) {
             level_barrier();
