@@ -345,8 +345,10 @@ obme_status obme_step(obme_handle* h, void* const* tracers,
  * obme_create).  A grid whose scratch exceeds 64 MiB is rejected with
  * OBME_ERR_INVALID (FP32-mixed LOBSTER + DIC/Alk with CO2 exchange takes
  * 40 B per column, about 1.6 million columns).  The warm starts begin at
- * zero on every call, so FP32-mixed results differ from obme_step's within
- * the root solve's tolerance; FP64 results are the same.  Every call on a
+ * zero on every call: where obme_step's handle carries warm starts from
+ * earlier calls, results differ from it within the root solve's tolerance
+ * (a relative Newton step of 1e-4 FP32-mixed, 1e-14 FP64); on a fresh
+ * handle FP64 results are the same.  Every call on a
  * device shares that device's array: calls that overlap on one device race.
  * Must be called on the device the handle was created on. */
 obme_status obme_step_noalloc(obme_handle* h, void* const* tracers, const void* T, const void* S,

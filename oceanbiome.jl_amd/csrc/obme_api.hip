@@ -230,7 +230,7 @@ struct obme_handle {
     void* d_srec = nullptr;    // SURF_REC * Nx*Ny of FT (CO2 exchange on)
     double* d_o2c = nullptr;   // 2*Nx*Ny: k / Δz_top and C_air α of the O2 exchange
     bool o2c_ready = false;    // d_o2c holds the current obme_step call's coefficients
-    double* d_Hprev = nullptr; // Nx*Ny warm starts of [H+] (mixed path)
+    double* d_Hprev = nullptr; // Nx*Ny warm starts of [H+]
     float* d_topres = nullptr; // 2*Nx*Ny residuals of the top-cell DIC and Alk (mixed path, CO2 exchange)
     double* d_surfc = nullptr; // [SURF_PACK][surfc_cols] surface constants of the current stepper call
     int surfc_cols = 0;        // Nx*Ny, or 1 when the call's T and S are the same in every column
@@ -648,7 +648,7 @@ static StepArgs<FT> make_args(const obme_handle* h, void* const* tracers)
         a.Sprof = (c.salinity.mode == OBME_FIELD_PROFILE) ? h->d_Sprof : nullptr;
         a.Tconst = c.temperature.value;
         a.Sconst = c.salinity.value;
-        a.Hprev = (h->prec == OBME_FP64) ? nullptr : h->d_Hprev;
+        a.Hprev = h->d_Hprev;
         a.invdz_top = (FT)(1.0 / h->dz[g.Nz - 1]);
         a.co2 = c.co2;
         a.solver = c.solver;
@@ -739,7 +739,7 @@ static SurfaceArgs<FT> make_surface_args(const obme_handle* h, void* const* trac
     s.DIC = (FT*)tracers[7];
     s.Alk = (const FT*)tracers[8];
     s.srec = (const FT*)h->d_srec;
-    s.Hprev = (h->prec == OBME_FP64) ? nullptr : h->d_Hprev;
+    s.Hprev = h->d_Hprev;
     s.T3 = (const FT*)T;
     s.S3 = (const FT*)S;
     s.Tprof = (c.temperature.mode == OBME_FIELD_PROFILE) ? h->d_Tprof : nullptr;
@@ -751,7 +751,10 @@ static SurfaceArgs<FT> make_surface_args(const obme_handle* h, void* const* trac
     s.sxy = s.sx * (unsigned)(g.Ny + 2 * g.Hy);
     s.base0 = (unsigned)g.Hx + s.sx * (unsigned)g.Hy + s.sxy * (unsigned)g.Hz;
     s.invdz_top = (FT)(1.0 / h->dz[g.Nz - 1]);
-    s.mode = (h->prec == OBME_FP64) ? 0 : 1;
+    // both precisions: Newton warm-started from the column's previous [H+]
+    // (FP32 to FP32_NEWTON_TOL, FP64 to a 1e-14 relative step), the
+    // reference's damped Newton as the fallback
+    s.mode = 1;
     s.co2 = c.co2;
     s.solver = c.solver;
     return s;

@@ -374,9 +374,9 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
         Ir[s] = Ib[s] = Cr[s] = Cb[s] = FT(0);
     }
 
-    // inline CO2: [H+] of the previous stage (FP32-mixed warm start)
+    // inline CO2: [H+] of the previous stage (the Newton's warm start)
     double Hc = 0.0;
-    if constexpr (INLINE_CO2 && F32) {
+    if constexpr (INLINE_CO2) {
         if (col_ok && a.Hprev) Hc = a.Hprev[col2d];
     }
     // the stage mode: this launch's stage, and the scratch of level k at
@@ -677,7 +677,7 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
                     double pH = 0.0;
                     int ev = 0;
                     const double F = surface_flux_warm(sc, a.solver, (double)X[7], (double)X[8],
-                                                       a.co2.air_concentration, F32 ? 1 : 0, Hc, pH, ev,
+                                                       a.co2.air_concentration, 1, Hc, pH, ev,
                                                        F32 ? FP32_NEWTON_TOL : 1e-14);
                     Gv[7] = Gv[7] - (FT)F * a.invdz_top;
                 }
@@ -933,7 +933,7 @@ __device__ __forceinline__ void column_lane(const StepArgs<FT>& a, const int cw)
             level(k, k == ktop, k == 0);
         }
     }
-    if constexpr (INLINE_CO2 && F32) {
+    if constexpr (INLINE_CO2) {
         if (col_ok && a.Hprev) a.Hprev[col2d] = Hc;
     }
 }
@@ -947,7 +947,7 @@ struct SurfaceArgs {
     const FT* Alk;                 // Alk parent array (tendency mode)
     const FT* srec;                // surface records (step mode)
     float* topres;                 // FP32 step mode: the top-cell DIC residuals [Nx*Ny], or null
-    double* Hprev;                 // [Nx*Ny] warm start of [H+] (mixed path) or null
+    double* Hprev;                 // [Nx*Ny] warm start of [H+] or null
     const double* surfc;           // [SURF_PACK][surfc_cols] surface constants of the call (step mode) or null
     int surfc_cols;                // Nx*Ny, or 1 where T and S are the same in every column (profile, constant)
     const FT* T3;                  // 3-D temperature / salinity or null
@@ -961,7 +961,7 @@ struct SurfaceArgs {
     unsigned sx, sxy, base0;
     FT dtf, invdz_top;
     float rk_a[3], rk_b[3];        // FP32 path: Δt γ and Δt ζ per stage
-    int mode;                      // 0: reference damped Newton; 1: mixed
+    int mode;                      // 0: reference damped Newton; 1: warm-started Newton (fallback: 0)
     obme_co2_exchange co2;
     obme_carbon_solver solver;
     // the stage mode (TILE_STAGE): the stage, and FP32 the step's increments so far
