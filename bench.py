@@ -300,9 +300,11 @@ def bench_hooks(args):
     transitions, the sinking divergence and the top CO2 flux added into the
     model's Gⁿ), on the headline workload.  One "stage" is the pair; the
     substep itself is the model's.  Each hook is timed on its own (HIP events
-    around every sweep launch) and the pair together; the tendency sweep's
-    algorithmic bytes per cell: 9 tracers read, 9 tendencies read and
-    written, PAR written (FP32: 112 B; FP64: 224 B)."""
+    around every sweep launch) and the pair together.  The tendency sweep's
+    algorithmic bytes per cell: the 7 tracers the tendencies depend on read
+    (DIC and Alk feed none: the surface kernel reads their top cells), the 9
+    tendencies read and written, PAR written (FP32: 104 B; FP64: 208 B); the
+    update-state sweep's: P read, PAR written."""
     import torch
     from oceanbiome import workloads
     from oceanbiome.runtime import HIPBiogeochemistry
@@ -347,7 +349,8 @@ def bench_hooks(args):
         surf_ms, n3 = dev.kernel_timing(which=1)
         dev.set_kernel_timing(False)
         par_ms, tend_ms = par_ms / max(n1, 1), tend_ms / max(n2, 1)
-        tbytes = ncell * (3 * nw + 1) * es
+        assert wl.bgc.carbonate_system is not None
+        tbytes = ncell * ((nw - 2) + 2 * nw + 1) * es
         pbytes = ncell * 2 * es          # P read, PAR written
         res[prec] = {"ms_per_stage": pair_ms, "tendency_evaluations_per_s": nw * ncell / (pair_ms * 1e-3),
                      "update_state_sweep_ms": par_ms, "update_state_sweep_GBs": pbytes / (par_ms * 1e-3) / 1e9,
@@ -359,8 +362,10 @@ def bench_hooks(args):
             f"= {res[prec]['tendency_sweep_GBs']:.1f} GB/s, surface flux {res[prec]['surface_flux_ms']:.4f} ms)")
         del tr, G, PAR, dev
         torch.cuda.empty_cache()
+    calibration_copy(args)
     head_prec = "fp32" if "fp32" in res else "fp64"
     head = res[head_prec]
+    traffic = read_traffic(os.path.join(ROOT, "profiles"), wl.name + "_hooks", head_prec)
     print(json.dumps({
         "metric": "tracer-cell tendency evaluations/s of the coupled-model hooks (update-state + tendencies)",
         "value": head["tendency_evaluations_per_s"], "unit": "tracer-cell tendency evaluations/s",
@@ -370,7 +375,7 @@ def bench_hooks(args):
         "config": {"workload": wl.name + "_hooks", "grid": [g.Nx, g.Ny, g.Nz], "tracers_written": nw,
                    "one_step_is": "one stage: update-state then tendencies"},
         "roofline": {"bound": "hbm", "achieved": head["tendency_sweep_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": head["tendency_sweep_GBs"] / HBM_PEAK_GBS, "traffic": None,
+                     "frac": head["tendency_sweep_GBs"] / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_column_lane (tendency mode)",
                      "algorithmic_bytes_per_launch": head["tendency_sweep_bytes"],
                      "launch_ms": head["tendency_sweep_ms"]},
