@@ -469,7 +469,8 @@ typedef struct obme_batch obme_batch;
  * column c holds cells [offsets[c], offsets[c+1]) of every tracer array,
  * k = 0 at the bottom, uniform thickness dz, surface at z = 0.  `cfg`
  * supplies the model (OBME_NPZD), precision, TwoBand coefficients, sinking
- * (open bottom, upwind) and every parameter that `params` leaves NULL;
+ * (open bottom, upwind), an optional InstantRemineralisation sediment, and
+ * every parameter that `params` leaves NULL;
  * params->fields[OBME_BP_*] are device arrays of n_columns doubles (read once
  * here).  Temperature is uniform.  `offsets` is a host array of n_columns+1. */
 obme_status obme_batch_create(const obme_config* cfg, int32_t n_columns, const int32_t* offsets, double dz,
@@ -482,6 +483,20 @@ obme_status obme_batch_destroy(obme_batch* b);
  * recorded once into a hipGraph and replayed. */
 obme_status obme_batch_step(obme_batch* b, void* const* tracers, int32_t nsteps, double dt, double t0,
                             void* stream);
+/* obme_batch_step with the ensemble's InstantRemineralisation sediment
+ * (cfg->sediment at create, tracer indices in NPZD's order N, P, Z, T, D;
+ * BiogeochemicalSediment, src/Sediments, with instant_remineralisation.jl):
+ * storage is a device array of n_columns doubles, one sediment per column.
+ * At every stage the bottom cell's sinking flux f (P and D leave through the
+ * open bottom face) returns (1 - e) f / dz to the receiver's bottom cell,
+ * e = a + b (f / (k + f))^2, and after each stage's substep the storage
+ * takes a sediment step of the stage Δt with the tendency e f, as
+ * obme_step_with_sediment does for a grid (update_state.jl:6-17); the handle
+ * keeps the sediment clock and each column's previous tendency between
+ * calls.  obme_batch_step on such an ensemble applies the coupling and
+ * leaves the storage alone, as obme_step does. */
+obme_status obme_batch_step_with_sediment(obme_batch* b, void* const* tracers, double* storage, int32_t nsteps,
+                                          double dt, double t0, void* stream);
 /* Column packing: plan = index of a set of tile shapes (columns go to the
  * smallest tile that holds them), steps_per_launch as above. */
 obme_status obme_batch_configure(obme_batch* b, int32_t plan, int32_t steps_per_launch);

@@ -1404,10 +1404,10 @@ extern "C" obme_status obme_kernel_timing(obme_handle* h, int32_t which, double*
 
 // ------------------------------------------------------------------ batched ensemble
 
-template <typename FT, int L, int CPW, int NW>
+template <typename FT, int L, int CPW, int NW, bool SED = false>
 __global__ void __launch_bounds__(NW * 64) k_batch(BatchArgs<FT> a)
 {
-    batch_columns<FT, L, CPW, NW>(a);
+    batch_columns<FT, L, CPW, NW, SED>(a);
 }
 
 // Tile shapes of the ensemble kernel: capacity L*64/CPW levels per column.
@@ -1444,6 +1444,15 @@ struct obme_batch {
     int g_nsteps = -1, g_spl = -1, g_plan = -1;
     double g_dt = 0.0;
     void* g_ptrs[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* g_storage = nullptr;
+    int g_sed_init = -1, g_sed_ab2 = -1;
+    // an InstantRemineralisation sediment under every column (cfg.sediment)
+    bool sediment = false;
+    int sed_recv = 0;                     // receiver in the ensemble's order N, P, Z, D
+    bool sed_P = false, sed_D = false;    // P / D among the sinking tracers
+    double* d_sedGm = nullptr;            // [n_columns] G⁻ of the sediment's last step
+    long long sed_iter = 0;               // sediment steps taken (its clock's iteration)
+    double sed_last_dt = HUGE_VAL;        // Δt of its last step (Inf before the first)
 };
 
 static void batch_free_classes(obme_batch* b)
@@ -1541,37 +1550,81 @@ static obme_status batch_launch_class(const obme_batch* b, int q, BatchArgs<FT> 
     const BatchTile& t = BATCH_TILES[b->class_tile[q]];
     const int per_wg = t.NW * t.CPW;
     dim3 grid((a.ncols + per_wg - 1) / per_wg), block(t.NW * 64);
+    using Kernel = void (*)(BatchArgs<FT>);
+    Kernel k = nullptr;
+#define OBME_BATCH_KERNEL(L_, CPW_) (b->sediment ? k_batch<FT, L_, CPW_, 4, true> : k_batch<FT, L_, CPW_, 4, false>)
     switch (b->class_tile[q]) {
-    case 0: hipLaunchKernelGGL((k_batch<FT, 1, 4, 4>), grid, block, 0, s, a); break;
-    case 1: hipLaunchKernelGGL((k_batch<FT, 2, 4, 4>), grid, block, 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_batch<FT, 1, 2, 4>), grid, block, 0, s, a); break;
-    case 3: hipLaunchKernelGGL((k_batch<FT, 2, 2, 4>), grid, block, 0, s, a); break;
-    case 4: hipLaunchKernelGGL((k_batch<FT, 1, 1, 4>), grid, block, 0, s, a); break;
-    case 5: hipLaunchKernelGGL((k_batch<FT, 2, 1, 4>), grid, block, 0, s, a); break;
-    case 6: hipLaunchKernelGGL((k_batch<FT, 4, 2, 4>), grid, block, 0, s, a); break;
-    case 7: hipLaunchKernelGGL((k_batch<FT, 4, 4, 4>), grid, block, 0, s, a); break;
+    case 0: k = OBME_BATCH_KERNEL(1, 4); break;
+    case 1: k = OBME_BATCH_KERNEL(2, 4); break;
+    case 2: k = OBME_BATCH_KERNEL(1, 2); break;
+    case 3: k = OBME_BATCH_KERNEL(2, 2); break;
+    case 4: k = OBME_BATCH_KERNEL(1, 1); break;
+    case 5: k = OBME_BATCH_KERNEL(2, 1); break;
+    case 6: k = OBME_BATCH_KERNEL(4, 2); break;
+    case 7: k = OBME_BATCH_KERNEL(4, 4); break;
     default: return fail(OBME_ERR_INVALID, "bad batch tile");
     }
+#undef OBME_BATCH_KERNEL
+    hipLaunchKernelGGL(k, grid, block, 0, s, a);
     HIP_TRY(hipGetLastError());
     return OBME_OK;
+}
+
+// The sediment's stepping of a call: its first step ever (the lazy first
+// tendency) and whether the call's first sediment step is an AB2 step (its
+// Δt repeats the previous step's; between RK3 stages it never does).
+struct BatchSedimentCall {
+    double* storage = nullptr;
+    double* Gm = nullptr;
+    int init = 0, ab2 = 0;
+};
+
+static BatchSedimentCall batch_sediment_call(const obme_batch* b, double* storage, double* Gm, double dt)
+{
+    BatchSedimentCall c;
+    c.storage = storage;
+    c.Gm = Gm;
+    c.init = b->sed_iter == 0 ? 1 : 0;
+    c.ab2 = (!c.init && (8.0 / 15.0) * dt == b->sed_last_dt) ? 1 : 0;
+    return c;
 }
 
 // All launches of nsteps steps: per chunk of steps_per_launch steps, one
 // launch per class.
 template <typename FT>
-static obme_status batch_enqueue(const obme_batch* b, void* const* tracers, int nsteps, double dt, hipStream_t s)
+static obme_status batch_enqueue(const obme_batch* b, void* const* tracers, int nsteps, double dt, hipStream_t s,
+                                 const BatchSedimentCall& sc)
 {
     BatchArgs<FT> a = batch_args<FT>(b, tracers, dt);
     const int chunk = (b->steps_per_launch > 0) ? b->steps_per_launch : nsteps;
     const size_t ncell = (size_t)b->offsets[b->n_columns];
     if (b->d_comp)
         for (int n = 0; n < 4; n++) a.C[n] = (FT*)(b->d_comp + n * ncell);
+    if (b->sediment) {
+        const obme_sediment& sd = b->cfg.sediment;
+        a.sed_on = 1;
+        a.storage = sc.storage;
+        a.sedGm = sc.Gm;
+        a.sed_recv = b->sed_recv;
+        a.sed_dt[0] = (8.0 / 15.0) * dt;
+        a.sed_dt[1] = (5.0 / 12.0 + -17.0 / 60.0) * dt;
+        a.sed_dt[2] = (3.0 / 4.0 + -5.0 / 12.0) * dt;
+        a.sed_cn = 1.5 + sd.ab2_chi;
+        a.sed_cm = 0.5 + sd.ab2_chi;
+        a.sed_a = (FT)sd.burial_efficiency_constant1;
+        a.sed_b = (FT)sd.burial_efficiency_constant2;
+        a.sed_k = (FT)sd.burial_efficiency_half_saturation;
+        a.sedP = b->sed_P ? FT(1) : FT(0);
+        a.sedD = b->sed_D ? FT(1) : FT(0);
+    }
     for (int done = 0; done < nsteps; done += chunk) {
         const int n = std::min(chunk, nsteps - done);
         // FP32: the residuals start at zero with each call and pass from
         // launch to launch within it
         a.c_in = (done > 0 && b->d_comp) ? 1 : 0;
         a.c_out = (done + n < nsteps && b->d_comp) ? 1 : 0;
+        a.sed_init = (done == 0) ? sc.init : 0;
+        a.sed_ab2 = (done == 0) ? sc.ab2 : 0;
         for (size_t q = 0; q < b->class_tile.size(); q++) {
             obme_status r = batch_launch_class<FT>(b, (int)q, a, n, s);
             if (r != OBME_OK) return r;
@@ -1580,10 +1633,11 @@ static obme_status batch_enqueue(const obme_batch* b, void* const* tracers, int 
     return OBME_OK;
 }
 
-static obme_status batch_enqueue_any(const obme_batch* b, void* const* tracers, int nsteps, double dt, hipStream_t s)
+static obme_status batch_enqueue_any(const obme_batch* b, void* const* tracers, int nsteps, double dt, hipStream_t s,
+                                     const BatchSedimentCall& sc)
 {
-    return (b->prec == OBME_FP64) ? batch_enqueue<double>(b, tracers, nsteps, dt, s)
-                                  : batch_enqueue<float>(b, tracers, nsteps, dt, s);
+    return (b->prec == OBME_FP64) ? batch_enqueue<double>(b, tracers, nsteps, dt, s, sc)
+                                  : batch_enqueue<float>(b, tracers, nsteps, dt, s, sc);
 }
 
 extern "C" obme_status obme_batch_destroy(obme_batch* b)
@@ -1597,6 +1651,7 @@ extern "C" obme_status obme_batch_destroy(obme_batch* b)
     if (b->d_dpar) (void)hipFree(b->d_dpar);
     if (b->d_bcol) (void)hipFree(b->d_bcol);
     if (b->d_comp) (void)hipFree(b->d_comp);
+    if (b->d_sedGm) (void)hipFree(b->d_sedGm);
     delete b;
     return OBME_OK;
 }
@@ -1609,8 +1664,26 @@ extern "C" obme_status obme_batch_create(const obme_config* cfg, int32_t n_colum
     if (cfg->model != OBME_NPZD) return fail(OBME_ERR_UNSUPPORTED, "the batched ensemble runs NPZD columns");
     if (cfg->scaling.n_groups != 0)
         return fail(OBME_ERR_UNSUPPORTED, "negative scaling is not built for the batched ensemble");
-    if (cfg->sediment.model != OBME_SEDIMENT_NONE)
-        return fail(OBME_ERR_UNSUPPORTED, "a sediment is not built for the batched ensemble");
+    if (cfg->sediment.model != OBME_SEDIMENT_NONE && cfg->sediment.model != OBME_SEDIMENT_INSTANT_REMINERALISATION)
+        return fail(OBME_ERR_UNSUPPORTED, "the batched ensemble's sediment is InstantRemineralisation (SimpleMultiG "
+                                          "needs NH4, NO3 and O2, which NPZD does not have)");
+    if (cfg->sediment.model == OBME_SEDIMENT_INSTANT_REMINERALISATION) {
+        // NPZD's tracers N, P, Z, T, D: T is read-only
+        const obme_sediment& sd = cfg->sediment;
+        if (sd.receiver < 0 || sd.receiver > 4 || sd.receiver == 3)
+            return fail(OBME_ERR_INVALID, "the sediment's receiver must be N, P, Z or D");
+        if (sd.n_sinking < 1 || sd.n_sinking > OBME_MAX_SEDIMENT_TRACERS)
+            return fail(OBME_ERR_INVALID, "the sediment needs sinking tracers");
+        for (int q = 0; q < sd.n_sinking; q++) {
+            if (sd.sinking[q] < 0 || sd.sinking[q] > 4)
+                return fail(OBME_ERR_INVALID, "a sinking tracer of the sediment is not an NPZD tracer");
+            for (int q2 = 0; q2 < q; q2++)
+                if (sd.sinking[q2] == sd.sinking[q])
+                    return fail(OBME_ERR_INVALID, "a sinking tracer of the sediment is listed twice");
+        }
+        if (sd.timestepper != OBME_SEDIMENT_AB2)
+            return fail(OBME_ERR_UNSUPPORTED, "only the QuasiAdamsBashforth2 sediment time stepper is built");
+    }
     if (cfg->oxygen.enabled || cfg->o2.enabled)
         return fail(OBME_ERR_UNSUPPORTED, "the oxygen component is not built for the batched ensemble");
     if (cfg->precision != OBME_FP64 && cfg->precision != OBME_FP32_MIXED)
@@ -1692,6 +1765,18 @@ extern "C" obme_status obme_batch_create(const obme_config* cfg, int32_t n_colum
     if (ok && b->prec != OBME_FP64) ok = up((void**)&b->d_fpar, fpar.data(), fpar.size() * sizeof(FastParams));
     if (ok && b->prec != OBME_FP64)
         ok = hipMalloc((void**)&b->d_comp, 4 * (size_t)b->offsets[n_columns] * sizeof(float)) == hipSuccess;
+    if (ok && cfg->sediment.model == OBME_SEDIMENT_INSTANT_REMINERALISATION) {
+        const obme_sediment& sd = cfg->sediment;
+        b->sediment = true;
+        b->sed_recv = sd.receiver == 4 ? 3 : sd.receiver;
+        for (int q = 0; q < sd.n_sinking; q++) {
+            // only P and D sink in the ensemble: the other tracers' flux is 0
+            if (sd.sinking[q] == 1) b->sed_P = true;
+            if (sd.sinking[q] == 4) b->sed_D = true;
+        }
+        ok = hipMalloc((void**)&b->d_sedGm, (size_t)n_columns * sizeof(double)) == hipSuccess &&
+             hipMemset(b->d_sedGm, 0, (size_t)n_columns * sizeof(double)) == hipSuccess;
+    }
     if (!ok) {
         obme_batch_destroy(b);
         return fail(OBME_ERR_NOMEM, "batch buffers");
@@ -1719,48 +1804,81 @@ extern "C" obme_status obme_batch_configure(obme_batch* b, int32_t plan, int32_t
     return OBME_OK;
 }
 
-extern "C" obme_status obme_batch_step(obme_batch* b, void* const* tracers, int32_t nsteps, double dt, double t0,
-                                       void* stream)
+// nsteps steps (with the sediment when the ensemble has one: storage null
+// applies its coupling alone)
+static obme_status batch_step(obme_batch* b, void* const* tracers, double* storage, int32_t nsteps, double dt,
+                              void* stream)
 {
-    (void)t0;  // surface PAR is constant per column
     if (!b || !tracers) return fail(OBME_ERR_INVALID, "null argument");
     for (int n = 0; n < 4; n++)
         if (!tracers[n]) return fail(OBME_ERR_INVALID, "null tracer pointer");
     if (nsteps < 0) return fail(OBME_ERR_INVALID, "negative nsteps");
     if (nsteps == 0) return OBME_OK;
     hipStream_t s = (hipStream_t)stream;
+    const BatchSedimentCall sc = batch_sediment_call(b, storage, b->d_sedGm, dt);
     const bool multi = b->steps_per_launch > 0 && b->steps_per_launch < nsteps;
-    if (!multi) return batch_enqueue_any(b, tracers, nsteps, dt, s);
-    // several launches per call: recorded once into a hipGraph per
-    // (nsteps, steps_per_launch, dt, tracers, plan) and replayed
-    const bool same = b->graph && b->g_nsteps == nsteps && b->g_spl == b->steps_per_launch && b->g_dt == dt &&
-                      b->g_plan == b->plan && std::memcmp(b->g_ptrs, tracers, sizeof(b->g_ptrs)) == 0;
-    if (!same) {
-        batch_drop_graph(b);
-        if (!b->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
-        HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
-        obme_status r = batch_enqueue_any(b, tracers, nsteps, dt, b->cap_stream);
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
-        if (r != OBME_OK) {
-            if (g) (void)hipGraphDestroy(g);
-            return r;
+    obme_status r = OBME_OK;
+    if (!multi) {
+        r = batch_enqueue_any(b, tracers, nsteps, dt, s, sc);
+    } else {
+        // several launches per call: recorded once into a hipGraph per
+        // (nsteps, steps_per_launch, dt, tracers, plan, sediment call) and replayed
+        const bool same = b->graph && b->g_nsteps == nsteps && b->g_spl == b->steps_per_launch && b->g_dt == dt &&
+                          b->g_plan == b->plan && std::memcmp(b->g_ptrs, tracers, sizeof(b->g_ptrs)) == 0 &&
+                          b->g_storage == storage && b->g_sed_init == sc.init && b->g_sed_ab2 == sc.ab2;
+        if (!same) {
+            batch_drop_graph(b);
+            if (!b->cap_stream) HIP_TRY(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking));
+            HIP_TRY(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal));
+            obme_status rc = batch_enqueue_any(b, tracers, nsteps, dt, b->cap_stream, sc);
+            hipGraph_t g = nullptr;
+            hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+            if (rc != OBME_OK) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess)
+                return fail(OBME_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                b->graph = nullptr;
+                return fail(OBME_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            }
+            b->g_nsteps = nsteps;
+            b->g_spl = b->steps_per_launch;
+            b->g_dt = dt;
+            b->g_plan = b->plan;
+            std::memcpy(b->g_ptrs, tracers, sizeof(b->g_ptrs));
+            b->g_storage = storage;
+            b->g_sed_init = sc.init;
+            b->g_sed_ab2 = sc.ab2;
         }
-        if (e != hipSuccess) return fail(OBME_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-        e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) {
-            b->graph = nullptr;
-            return fail(OBME_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-        }
-        b->g_nsteps = nsteps;
-        b->g_spl = b->steps_per_launch;
-        b->g_dt = dt;
-        b->g_plan = b->plan;
-        std::memcpy(b->g_ptrs, tracers, sizeof(b->g_ptrs));
+        if (hipGraphLaunch(b->graph, s) != hipSuccess) r = fail(OBME_ERR_HIP, "hipGraphLaunch");
     }
-    HIP_TRY(hipGraphLaunch(b->graph, s));
-    return OBME_OK;
+    if (r == OBME_OK && b->sediment && storage) {
+        // three sediment steps per step, the last of Δt (γ3 + ζ3)
+        b->sed_iter += 3 * (long long)nsteps;
+        b->sed_last_dt = (3.0 / 4.0 + -5.0 / 12.0) * dt;
+    }
+    return r;
+}
+
+extern "C" obme_status obme_batch_step(obme_batch* b, void* const* tracers, int32_t nsteps, double dt, double t0,
+                                       void* stream)
+{
+    (void)t0;  // surface PAR is constant per column
+    return batch_step(b, tracers, nullptr, nsteps, dt, stream);
+}
+
+extern "C" obme_status obme_batch_step_with_sediment(obme_batch* b, void* const* tracers, double* storage,
+                                                     int32_t nsteps, double dt, double t0, void* stream)
+{
+    (void)t0;
+    if (!b) return fail(OBME_ERR_INVALID, "null handle");
+    if (!b->sediment) return fail(OBME_ERR_INVALID, "the ensemble has no sediment");
+    if (!storage) return fail(OBME_ERR_INVALID, "null sediment storage");
+    return batch_step(b, tracers, storage, nsteps, dt, stream);
 }
 
 // Times each packing plan and steps-per-launch choice on a scratch copy of
@@ -1774,16 +1892,27 @@ extern "C" obme_status obme_batch_autotune(obme_batch* b, void* const* tracers, 
     const size_t es = (b->prec == OBME_FP64) ? 8 : 4;
     const size_t bytes = (size_t)b->offsets[b->n_columns] * es;
     void* scratch[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* sed_scratch = nullptr;   // the sediment's storage and G⁻ (the caller's are not touched)
     hipEvent_t e0 = nullptr, e1 = nullptr;
     obme_status st = OBME_OK;
     auto cleanup = [&]() {
         for (void* p : scratch)
             if (p) (void)hipFree(p);
+        if (sed_scratch) (void)hipFree(sed_scratch);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
     };
     for (int n = 0; n < 4 && st == OBME_OK; n++)
         if (hipMalloc(&scratch[n], bytes) != hipSuccess) st = fail(OBME_ERR_NOMEM, "autotune scratch");
+    BatchSedimentCall sc;
+    if (st == OBME_OK && b->sediment) {
+        const size_t sb = 2 * (size_t)b->n_columns * sizeof(double);
+        if (hipMalloc((void**)&sed_scratch, sb) != hipSuccess || hipMemsetAsync(sed_scratch, 0, sb, s) != hipSuccess)
+            st = fail(OBME_ERR_NOMEM, "autotune scratch sediment");
+        sc.storage = sed_scratch;
+        sc.Gm = sed_scratch + b->n_columns;
+        sc.init = 1;
+    }
     if (st == OBME_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
         st = fail(OBME_ERR_HIP, "autotune events");
     if (st != OBME_OK) {
@@ -1802,9 +1931,9 @@ extern "C" obme_status obme_batch_autotune(obme_batch* b, void* const* tracers, 
                 if (hipMemcpyAsync(scratch[n], tracers[n], bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
                     st = fail(OBME_ERR_HIP, "autotune copy");
             if (st != OBME_OK) break;
-            st = batch_enqueue_any(b, scratch, 1, dt, s);  // warm-up
+            st = batch_enqueue_any(b, scratch, 1, dt, s, sc);  // warm-up
             if (st == OBME_OK && hipEventRecord(e0, s) != hipSuccess) st = fail(OBME_ERR_HIP, "event");
-            if (st == OBME_OK) st = batch_enqueue_any(b, scratch, trial_steps, dt, s);
+            if (st == OBME_OK) st = batch_enqueue_any(b, scratch, trial_steps, dt, s, sc);
             if (st == OBME_OK && hipEventRecord(e1, s) != hipSuccess) st = fail(OBME_ERR_HIP, "event");
             float ms = 0.0f;
             if (st == OBME_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))

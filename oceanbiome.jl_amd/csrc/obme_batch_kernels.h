@@ -54,10 +54,28 @@ struct BatchArgs {
     // TwoBand coefficients: FP64 reference form and FP32 log2-scaled form
     double kr, kb, xr, xb, er, eb, Rr;
     float kr2, kb2, er_f, eb_f, cr, cb, xr2h, xb2h;
+    // InstantRemineralisation sediment under every column (sed_on; storage
+    // null: the coupling alone, the storage not advanced).  The flux through
+    // the bottom face f = sedP (-wP) P + sedD (-wD) D
+    // of the bottom cell, burial e = a + b (f / (k + f))^2, the receiver's
+    // coupled tendency (1 - e) f / dz; after each stage's substep the storage
+    // steps by the stage Δt with the tendency e f of the stage's start
+    // (update_state.jl:6-17; instant_remineralisation.jl:101-123).
+    int sed_on;
+    double* storage;                // [n_columns] or null
+    double* sedGm;                  // [n_columns] G⁻: the tendency of the sediment's last step
+    int sed_init;                   // the sediment's first step ever is this launch's first step
+    int sed_ab2;                    // this launch's first sediment step is a QuasiAdamsBashforth2 step
+    int sed_recv;                   // receiver: 0 N, 1 P, 2 Z, 3 D
+    double sed_dt[3];               // Δt γ1, Δt (γ2 + ζ2), Δt (γ3 + ζ3)
+    double sed_cn, sed_cm;          // 3/2 + χ, 1/2 + χ
+    FT sed_a, sed_b, sed_k;         // burial efficiency constants
+    FT sedP, sedD;                  // 1 where P / D is one of the sinking tracers, else 0
 };
 
-// One persistent workgroup loops over tiles of NW*CPW columns.
-template <typename FT, int L, int CPW, int NW>
+// One persistent workgroup loops over tiles of NW*CPW columns.  SED: the
+// sediment's code compiled in (the ensemble without one keeps its registers).
+template <typename FT, int L, int CPW, int NW, bool SED>
 __device__ __forceinline__ void batch_columns(const BatchArgs<FT>& a)
 {
     constexpr int LPC = 64 / CPW;
@@ -104,6 +122,17 @@ __device__ __forceinline__ void batch_columns(const BatchArgs<FT>& a)
         if constexpr (F32) fp = a.fpar[cc];
         else dp = a.dpar[cc];
         const float wiP = (float)(bc.wP / (double)a.dz), wiD = (float)(bc.wD / (double)a.dz);
+
+        // the sediment of this column, held by the lane of its bottom cell
+        // (j = Nzc - 1: lane rb, level mb)
+        const int jb = Nzc - 1;
+        const bool sed = SED && a.sed_on && col >= 0 && r == jb / L;
+        const int mb = jb % L;
+        double su = 0.0, sgm = 0.0;
+        if (sed && a.storage) {
+            su = a.storage[col];
+            if (a.sed_ab2) sgm = a.sedGm[col];
+        }
 
         // water terms of the own levels
         float ezr[L], ezb[L];
@@ -177,6 +206,34 @@ __device__ __forceinline__ void batch_columns(const BatchArgs<FT>& a)
                     else
                         PARv[m] = PAR0 * (m_exp(a.kr * zc[m] - a.xr * runr) + m_exp(a.kb * zc[m] - a.xb * runb)) / 2;
                 }
+                // the sediment: the bottom cell's flux and burial, the
+                // receiver's coupled tendency, the sediment step
+                FT cpl = FT(0);
+                if (sed) {
+                    FT Pb = val(1, 0), Db = val(3, 0);
+#pragma unroll
+                    for (int m = 1; m < L; m++)
+                        if (m == mb) {
+                            Pb = val(1, m);
+                            Db = val(3, m);
+                        }
+                    const FT f = a.sedP * (-wP) * Pb + a.sedD * (-wD) * Db;
+                    const FT qf = f / (a.sed_k + f);
+                    const FT e = a.sed_a + a.sed_b * (qf * qf);
+                    cpl = ((FT(1) - e) * f) / a.dz;
+                    const double ge = (double)(e * f);
+                    const bool init = a.sed_init && s == 0;
+                    if (st == 0) {
+                        if (!init) {
+                            const bool ab2 = a.sed_ab2 && s == 0;
+                            su += a.sed_dt[0] * (ab2 ? a.sed_cn * ge - a.sed_cm * sgm : ge);
+                        }
+                    } else {
+                        if (st == 1 && init) su += a.sed_dt[0] * ge;
+                        su += a.sed_dt[st] * ge;
+                    }
+                    sgm = ge;
+                }
                 // tendencies bottom-up within the lane, sinking, RK3 substep
 #pragma unroll
                 for (int m = L - 1; m >= 0; m--) {
@@ -194,6 +251,11 @@ __device__ __forceinline__ void batch_columns(const BatchArgs<FT>& a)
                         const double FhiP = top ? 0.0 : wP * cPn, FhiD = top ? 0.0 : wD * cDn;
                         G[1] = G[1] - (FhiP - wP * P) * a.invdz;
                         G[3] = G[3] - (FhiD - wD * Dd) * a.invdz;
+                    }
+                    if (sed && m == mb) {
+#pragma unroll
+                        for (int n = 0; n < 4; n++)
+                            if (n == a.sed_recv) G[n] = G[n] + cpl;
                     }
 #pragma unroll
                     for (int n = 0; n < 4; n++) {
@@ -223,6 +285,10 @@ __device__ __forceinline__ void batch_columns(const BatchArgs<FT>& a)
                         U[n][m] = sum;
                     }
             }
+        }
+        if (sed && a.storage) {
+            a.storage[col] = su;
+            a.sedGm[col] = sgm;
         }
 #pragma unroll
         for (int n = 0; n < 4; n++)

@@ -62,6 +62,8 @@ def library():
                                           C.POINTER(abi.BatchParams), C.c_double, C.POINTER(vp)]),
         "obme_batch_destroy": (C.c_int32, [vp]),
         "obme_batch_step": (C.c_int32, [vp, C.POINTER(vp), C.c_int32, C.c_double, C.c_double, vp]),
+        "obme_batch_step_with_sediment": (C.c_int32, [vp, C.POINTER(vp), vp, C.c_int32, C.c_double, C.c_double,
+                                                      vp]),
         "obme_batch_configure": (C.c_int32, [vp, C.c_int32, C.c_int32]),
         "obme_batch_autotune": (C.c_int32, [vp, C.POINTER(vp), C.c_int32, C.c_double, vp]),
         "obme_batch_describe": (C.c_int32, [vp, C.c_char_p, C.c_int32]),
@@ -88,6 +90,7 @@ EXPORTED_SYMBOLS = (
     "obme_sweep_configure", "obme_sweep_waves", "obme_autotune",
     "obme_sediment_update_state", "obme_co2_flux", "obme_abi_sizeof",
     "obme_set_kernel_timing", "obme_kernel_timing", "obme_batch_create", "obme_batch_destroy", "obme_batch_step",
+    "obme_batch_step_with_sediment",
     "obme_batch_configure", "obme_batch_autotune", "obme_batch_describe", "obme_pisces_create",
     "obme_pisces_destroy", "obme_pisces_update_state", "obme_pisces_add_tendencies", "obme_pisces_step",
 )
@@ -273,7 +276,9 @@ class HIPBatch:
     parameter not given per column; offsets: host int array (n_columns + 1);
     params: host array [OBME_BP_COUNT, n_columns] (NaN rows = model default);
     tracers passed to step() are 4 CUDA tensors (N, P, Z, D) of offsets[-1]
-    elements."""
+    elements.  With bgc's InstantRemineralisation sediment (sediment_model),
+    one sediment per column: its storage is self.sediment_storage
+    (n_columns doubles), advanced by step()."""
 
     def __init__(self, bgc, offsets, params, dz, temperature, precision="fp32"):
         import torch
@@ -300,6 +305,9 @@ class HIPBatch:
                                            off.ctypes.data_as(C.POINTER(C.c_int32)), float(dz), C.byref(bp),
                                            float(temperature), C.byref(h)))
         self.handle = h
+        self.sediment_storage = None
+        if getattr(bgc, "sediment", None) is not None:
+            self.sediment_storage = torch.zeros(self.n_columns, dtype=torch.float64, device="cuda")
 
     def close(self):
         if getattr(self, "handle", None):
@@ -320,7 +328,14 @@ class HIPBatch:
                 raise OBMEError(f"tracers must be contiguous {self.dtype} CUDA tensors of {self.n_cells} elements")
 
     def step(self, tracers, nsteps, dt, t0=0.0):
+        """nsteps RK3 steps of every column in place (with the sediment, its
+        storage advanced in the same launches)."""
         self._check_fields(tracers)
+        if self.sediment_storage is not None:
+            _check(library().obme_batch_step_with_sediment(self.handle, _ptr_array(tracers),
+                                                           _ptr(self.sediment_storage), int(nsteps), float(dt),
+                                                           float(t0), _stream()))
+            return
         _check(library().obme_batch_step(self.handle, _ptr_array(tracers), int(nsteps), float(dt), float(t0),
                                          _stream()))
 

@@ -222,6 +222,13 @@ int orc_grid_tendencies_sediment(const obme_config* cfg, const double* const* tr
 int orc_batch_run(const obme_config* cfg, int n_columns, const int* offsets,
                   const double* const* params, double dz, double T_const,
                   double* const* U, int nsteps, double dt, double t0, int nthreads);
+/* orc_batch_run with an InstantRemineralisation sediment under every column
+ * (cfg->sediment; the tracer indices in NPZD's order N, P, Z, T, D):
+ * storage, Gn, Gm of n_columns each and the shared sediment clock. */
+int orc_batch_run_sediment(const obme_config* cfg, int n_columns, const int* offsets,
+                           const double* const* params, double dz, double T_const,
+                           double* const* U, int nsteps, double dt, double t0, int nthreads,
+                           double* storage, double* Gn, double* Gm, double* last_dt, long* iteration);
 
 #ifdef __cplusplus
 }

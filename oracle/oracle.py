@@ -109,6 +109,8 @@ def load_library(path):
                                              DP]),
         "orc_batch_run": (I, [C.POINTER(abi.Config), I, C.POINTER(I), C.POINTER(DP), D, D, C.POINTER(DP), I, D,
                               D, I]),
+        "orc_batch_run_sediment": (I, [C.POINTER(abi.Config), I, C.POINTER(I), C.POINTER(DP), D, D, C.POINTER(DP),
+                                       I, D, D, I, DP, DP, DP, C.POINTER(D), C.POINTER(C.c_long)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -419,7 +421,19 @@ def grid_tendencies(cfg, tracers, G, t=0.0, T3=None, S3=None, PAR0=None, PAR=Non
                                        _dp(PAR), _dp(fields))
 
 
-def batch_run(cfg, offsets, params, dz, T_const, U, nsteps, dt, t0=0.0, nthreads=8):
+def batch_run(cfg, offsets, params, dz, T_const, U, nsteps, dt, t0=0.0, nthreads=8, sediment=None):
+    """The ensemble's columns, each run as one column; with `sediment` (a
+    SedimentState of one field over the columns: storage of shape (1,
+    n_columns)) the InstantRemineralisation sediment of cfg under each."""
     off = np.ascontiguousarray(offsets, dtype=np.int32)
-    lib().orc_batch_run(C.byref(cfg), len(off) - 1, off.ctypes.data_as(C.POINTER(I)), _ptrs(params), dz,
-                        T_const, _ptrs(U), nsteps, dt, t0, nthreads)
+    if sediment is None:
+        lib().orc_batch_run(C.byref(cfg), len(off) - 1, off.ctypes.data_as(C.POINTER(I)), _ptrs(params), dz,
+                            T_const, _ptrs(U), nsteps, dt, t0, nthreads)
+        return
+    last = C.c_double(sediment.last_dt)
+    it = C.c_long(sediment.iteration)
+    lib().orc_batch_run_sediment(C.byref(cfg), len(off) - 1, off.ctypes.data_as(C.POINTER(I)), _ptrs(params), dz,
+                                 T_const, _ptrs(U), nsteps, dt, t0, nthreads, _dp(sediment.storage),
+                                 _dp(sediment.Gn), _dp(sediment.Gm), C.byref(last), C.byref(it))
+    sediment.last_dt = last.value
+    sediment.iteration = it.value

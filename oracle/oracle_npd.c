@@ -940,10 +940,14 @@ int orc_grid_tendencies_sediment(const obme_config* cfg, const double* const* tr
 /* Independent NPZD columns with per-column parameters: each default
  * multiplied by a per-column factor, per-column surface PAR, uniform Δz and
  * T.  The semantics are N independent column runs of the model above. */
-int orc_batch_run(const obme_config* cfg, int n_columns, const int* offsets, const double* const* params,
-                  double dz, double T_const, double* const* U, int nsteps, double dt, double t0,
-                  int nthreads)
+int orc_batch_run_sediment(const obme_config* cfg, int n_columns, const int* offsets, const double* const* params,
+                           double dz, double T_const, double* const* U, int nsteps, double dt, double t0,
+                           int nthreads, double* storage, double* Gn, double* Gm, double* last_dt, long* iteration)
 {
+    const int nf = orc_sediment_fields(&cfg->sediment);
+    const int sed = nf == 1 && storage && Gn && Gm && last_dt && iteration;
+    double last_dt_after = sed ? *last_dt : 0.0;
+    long iteration_after = sed ? *iteration : 0;
 #pragma omp parallel for num_threads(nthreads) schedule(dynamic, 1)
     for (int c = 0; c < n_columns; c++) {
         int Nz = offsets[c + 1] - offsets[c];
@@ -994,6 +998,17 @@ int orc_batch_run(const obme_config* cfg, int n_columns, const int* offsets, con
         env.S_surface = 35.0;
         env.Tcol = Tcol;
         env.fast_exit = 1;
+        /* the column's sediment (storage, Gⁿ, G⁻ and the shared clock) */
+        orc_sediment_column sc;
+        memset(&sc, 0, sizeof(sc));
+        if (sed) {
+            sc.u[0] = storage[c];
+            sc.Gn[0] = Gn[c];
+            sc.Gm[0] = Gm[c];
+            sc.last_dt = *last_dt;
+            sc.iteration = *iteration;
+        }
+        env.sed = sed ? &sc : NULL;
         /* tracer order N, P, Z, T, D */
         const int map[5] = {0, 1, 2, -1, 3};
         for (int n = 0; n < 5; n++)
@@ -1003,11 +1018,32 @@ int orc_batch_run(const obme_config* cfg, int n_columns, const int* offsets, con
         for (int n = 0; n < 5; n++)
             if (map[n] >= 0)
                 for (int k = 0; k < Nz; k++) U[map[n]][offsets[c] + k] = col[n * Nz + k];
+        if (sed) {
+            storage[c] = sc.u[0];
+            Gn[c] = sc.Gn[0];
+            Gm[c] = sc.Gm[0];
+            if (c == 0) {
+                last_dt_after = sc.last_dt;
+                iteration_after = sc.iteration;
+            }
+        }
         free(zc);
         free(zf);
         free(dzs);
         free(Tcol);
         free(col);
     }
+    if (sed) {
+        *last_dt = last_dt_after;
+        *iteration = iteration_after;
+    }
     return 0;
+}
+
+int orc_batch_run(const obme_config* cfg, int n_columns, const int* offsets, const double* const* params,
+                  double dz, double T_const, double* const* U, int nsteps, double dt, double t0,
+                  int nthreads)
+{
+    return orc_batch_run_sediment(cfg, n_columns, offsets, params, dz, T_const, U, nsteps, dt, t0, nthreads,
+                                  NULL, NULL, NULL, NULL, NULL);
 }
