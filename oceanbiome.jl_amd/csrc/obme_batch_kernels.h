@@ -218,9 +218,17 @@ __device__ __forceinline__ void batch_columns(const BatchArgs<FT>& a)
                             Db = val(3, m);
                         }
                     const FT f = a.sedP * (-wP) * Pb + a.sedD * (-wD) * Db;
-                    const FT qf = f / (a.sed_k + f);
-                    const FT e = a.sed_a + a.sed_b * (qf * qf);
-                    cpl = ((FT(1) - e) * f) / a.dz;
+                    FT qf, e;
+                    if constexpr (F32) {
+                        // FP32-mixed: the hardware reciprocal (1 ulp) and 1/Δz
+                        qf = f * __builtin_amdgcn_rcpf(a.sed_k + f);
+                        e = a.sed_a + a.sed_b * (qf * qf);
+                        cpl = ((FT(1) - e) * f) * a.invdz;
+                    } else {
+                        qf = f / (a.sed_k + f);
+                        e = a.sed_a + a.sed_b * (qf * qf);
+                        cpl = ((FT(1) - e) * f) / a.dz;
+                    }
                     const double ge = (double)(e * f);
                     const bool init = a.sed_init && s == 0;
                     if (st == 0) {

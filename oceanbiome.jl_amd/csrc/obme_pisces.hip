@@ -95,13 +95,19 @@ __global__ void __launch_bounds__(256) k_pisces_columns(PiscesArgs<FT> a)
     FT band[3];
     FT zeu = FT(-INFINITY);
     FT acc = 0, depth = 0;
-    FT PARup = 0, lPARup = 0;
+    FT PARup = 0;
+    // Chl^e of the three bands from one logarithm (FP64: exp(e ln Chl),
+    // within a few ulp of pow; pow for Chl <= 0 or a zero exponent)
+    const bool pw = !(a.e[0] != FT(0) && a.e[1] != FT(0) && a.e[2] != FT(0));
     for (int k = Nz - 1; k >= 0; k--) {
         const unsigned p = pidx(a, i, j, k);
         const FT Chl = a.in[OBME_PI_PCHL][p] + a.in[OBME_PI_DCHL][p];
+        const bool via_log = Chl > FT(0) && !pw;
+        const FT lChl = via_log ? m_log(Chl) : FT(0);
 #pragma unroll
         for (int n = 0; n < 3; n++) {
-            const FT att = a.kw[n] + a.chi[n] * m_pow(Chl, a.e[n]);
+            const FT ce = via_log ? m_exp(a.e[n] * lChl) : m_pow(Chl, a.e[n]);
+            const FT att = a.kw[n] + a.chi[n] * ce;
             if (k == Nz - 1) band[n] = PAR0 * a.division[n] * m_exp(zc[k] * att);
             else band[n] = band[n] * m_exp((zc[k] - zc[k + 1]) * att);
         }
@@ -111,11 +117,13 @@ __global__ void __launch_bounds__(256) k_pisces_columns(PiscesArgs<FT> a)
         a.PAR[ncell + c] = band[0];
         a.PAR[2 * ncell + c] = band[1];
         a.PAR[3 * ncell + c] = band[2];
-        const FT lPAR = m_log(PAR);
-        if (k < Nz - 1 && PAR <= thr && zeu == FT(-INFINITY))
+        // the euphotic depth: the logarithms of PAR at the first crossing
+        // and the level above it only
+        if (k < Nz - 1 && PAR <= thr && zeu == FT(-INFINITY)) {
+            const FT lPAR = m_log(PAR), lPARup = m_log(PARup);
             zeu = zc[k] + (lthr - lPAR) * (zc[k] - zc[k + 1]) / (lPAR - lPARup);
+        }
         PARup = PAR;
-        lPARup = lPAR;
         // mixed-layer integral, top cell first
         const FT zk = zf[k];
         const FT zk1 = (k + 1 < Nz) ? zf[k + 1] : FT(a.zf_top);
