@@ -15,7 +15,9 @@
  * shown to catch a planted mistake (tests/test_oracle_pins_npd.py,
  * tests/test_oracle_sinking_pins.py); SimpleMultiG's nitrogen balances, its
  * fractions where their logarithms vanish and the reference's budget
- * (tests/test_oracle_sediment_smg.py).
+ * (tests/test_oracle_sediment_smg.py); the ensemble with its sediment
+ * against the 3-D oracle's single column and the reference's budget per
+ * column (tests/test_oracle_batch_sediment.py).
  *
  * Parity unpinned (no reference-held value or closed form fixes the bits):
  * orc_teos10_density away from the reference's doctest point; the RK3
