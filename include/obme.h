@@ -677,6 +677,9 @@ typedef struct obme_pisces_config {
                                       PISCES/coupling_utils.jl:11-35 */
     obme_co2_exchange co2;         /* top DIC flux (CarbonDioxideGasExchangeBoundaryCondition, no Si/PO4) */
     obme_o2_exchange o2;           /* top O2 flux */
+    obme_sediment sediment;        /* PISCES(; sediment): InstantRemineralisation or none; tracer
+                                      indices in the OBME_PI_* order, sinking ones among POC, SFe,
+                                      GOC, BFe, PSi, CaCO3 */
 } obme_pisces_config;
 
 typedef struct obme_pisces obme_pisces;
@@ -704,6 +707,20 @@ obme_status obme_pisces_add_tendencies(obme_pisces* h, const void* const* tracer
  * on tracers[0..23]; T and S (tracers[24], [25]) are read. */
 obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, int32_t nsteps, double dt, double t0,
                              void* stream);
+/* obme_pisces_step with PISCES's InstantRemineralisation sediment
+ * (cfg->sediment; PISCES.jl:233,344-353 takes a `sediment`,
+ * instant_remineralisation.jl:60-123): storage is a device array of Nx*Ny
+ * doubles (i fastest).  At every stage the bottom cell's sinking flux f of
+ * the listed tracers (their bottom-face velocities; open_bottom for POC)
+ * returns (1 - e) f / dz to the receiver's bottom cell, and after each
+ * stage's substep the storage takes the sediment's step of the stage Δt
+ * with the tendency e f (update_state.jl:6-17); the handle keeps the
+ * sediment clock and each column's previous tendency.  The per-stage
+ * kernels step a handle with a sediment.  obme_pisces_step (and
+ * obme_pisces_add_tendencies) on such a handle apply the coupling and leave
+ * the storage alone. */
+obme_status obme_pisces_step_with_sediment(obme_pisces* h, void* const* tracers, double* storage, int32_t nsteps,
+                                           double dt, double t0, void* stream);
 
 /* ---- kernel timing ------------------------------------------------------ */
 

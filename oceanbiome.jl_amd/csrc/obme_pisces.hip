@@ -65,6 +65,18 @@ struct PiscesArgs {
     double dt;
     int stage;
     obme_carbon_solver solver;
+    // InstantRemineralisation sediment (sed_on): the flux through the bottom
+    // face f = Σ sedw[n] c[n] of the bottom cell over the sinking tracers
+    // (sedw = -w of the tracer's bottom face), burial e = a + b (f/(k + f))^2,
+    // the receiver's coupled tendency (1 - e) f / Δz; in the step mode the
+    // column's storage (when given) takes the sediment step that follows
+    // this stage's substep, with the tendency e f of the stage's input
+    int sed_on, sed_recv, sed_init, sed_ab2;
+    FT sedw[NW];
+    FT sed_a, sed_b, sed_k;
+    double sed_dt[3], sed_cn, sed_cm;
+    double* storage;        // [Nx*Ny] or null
+    double* sedGm;          // [Nx*Ny] G⁻: the tendency of the sediment's last step
 };
 
 template <typename FT>
@@ -259,7 +271,7 @@ __global__ void __launch_bounds__(256) k_pisces_prep(PiscesArgs<FT> a)
 #ifndef OBME_PISCES_F32_WAVES
 #define OBME_PISCES_F32_WAVES 4
 #endif
-template <typename FT, int MODE, int OM>
+template <typename FT, int MODE, int OM, bool SED = false>
 __global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES : 2) k_pisces_cell(PiscesArgs<FT> a)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -274,6 +286,22 @@ __global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES :
     FT v[NTR];
 #pragma unroll
     for (int n = 0; n < NTR; n++) v[n] = a.in[n][p];
+    // the sediment under the column (SED; Sediments/tracked_fields.jl:53-70,
+    // instant_remineralisation.jl:101-123): the bottom cell's flux f of the
+    // listed tracers only and the burial e, formed before the tendencies
+    FT sed_f = FT(0), sed_e = FT(0);
+    const bool sed_cell = SED && k == 0;
+    if (sed_cell) {
+        bool first = true;
+#pragma unroll
+        for (int n = 0; n < NW; n++)
+            if (a.sedw[n] != FT(0)) {
+                sed_f = first ? a.sedw[n] * v[n] : sed_f + a.sedw[n] * v[n];
+                first = false;
+            }
+        const FT qf = sed_f / (a.sed_k + sed_f);
+        sed_e = a.sed_a + a.sed_b * (qf * qf);
+    }
     const FT* sc = a.pp + OBME_PP_SCALARS;
     const FT* wP = a.w;
     const FT* wG = a.w + (Nz + 1);
@@ -360,6 +388,27 @@ __global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES :
         // the top boundary fluxes of DIC and O2
         G[OBME_PI_DIC] = G[OBME_PI_DIC] - (FT)a.sflux[col] * idz;
         G[OBME_PI_O2] = G[OBME_PI_O2] - (FT)a.sflux[ncol + col] * idz;
+    }
+    if (sed_cell) {
+        // the receiver's coupled tendency (tracer_coupling.jl:30-38) and, in
+        // the step mode, the storage's step that follows this stage's substep
+        const FT f = sed_f, e = sed_e;
+        const FT cpl = ((FT(1) - e) * f) / a.lev[2 * Nz];
+#pragma unroll
+        for (int n = 0; n < NW; n++)
+            if (n == a.sed_recv) G[n] = G[n] + cpl;
+        if (MODE == 0 && a.storage) {
+            const double ge = (double)(e * f);
+            double su = a.storage[col];
+            if (a.stage == 0) {
+                if (!a.sed_init) su += a.sed_dt[0] * (a.sed_ab2 ? a.sed_cn * ge - a.sed_cm * a.sedGm[col] : ge);
+            } else {
+                if (a.stage == 1 && a.sed_init) su += a.sed_dt[0] * ge;
+                su += a.sed_dt[a.stage] * ge;
+            }
+            a.storage[col] = su;
+            a.sedGm[col] = ge;
+        }
     }
     if constexpr (MODE == 1) {
 #pragma unroll
@@ -753,9 +802,19 @@ struct obme_pisces {
     void* d_copy[NW] = {};
     size_t field_bytes = 0;
     std::vector<double> zc, zf, dz;
+    std::vector<double> w_host;   // [2][Nz+1]: the POC and GOC face velocities (FP64)
     bool scaling = false;
     bool fused = false;        // FP32-mixed with Nz <= 64: k_pisces_fused (OBME_PISCES_FUSED=0 turns it off)
     int fused_steps = PF_MAX_STEPS;   // steps per fused launch (OBME_PISCES_FUSED_STEPS sets 1 .. 16)
+    // an InstantRemineralisation sediment under every column (cfg.sediment)
+    bool sediment = false;
+    double* d_sedGm = nullptr;         // [Nx*Ny] G⁻ of the sediment's last step
+    long long sed_iter = 0;            // sediment steps taken (its clock's iteration)
+    double sed_last_dt = HUGE_VAL;     // Δt of its last step (Inf before the first)
+    // the current step call's sediment: storage (or null), and this step's
+    // lazy first tendency and AB2 first step
+    double* call_storage = nullptr;
+    int call_init = 0, call_ab2 = 0;
 };
 
 extern "C" obme_status obme_pisces_destroy(obme_pisces* h)
@@ -766,6 +825,7 @@ extern "C" obme_status obme_pisces_destroy(obme_pisces* h)
         if (p) (void)hipFree(p);
     for (void* p : h->d_copy)
         if (p) (void)hipFree(p);
+    if (h->d_sedGm) (void)hipFree(h->d_sedGm);
     delete h;
     return OBME_OK;
 }
@@ -789,12 +849,14 @@ static obme_status pisces_upload(obme_pisces* h)
     // zmxl = zeu = 0 (PISCES/common.jl:39-55): face k takes the centre depth of cell k
     const double* sc = c.params + OBME_PP_SCALARS;
     const double sPOC = sc[OBME_PS_POC_SPEED], wmin = sc[OBME_PS_GOC_MIN_SPEED], wmax = sc[OBME_PS_GOC_MAX_SPEED];
-    std::vector<FT> w(2 * ((size_t)Nz + 1), (FT)0);
+    h->w_host.assign(2 * ((size_t)Nz + 1), 0.0);
     for (int k = 1; k <= Nz; k++) {
-        w[k - 1] = (FT)(-sPOC * (c.open_bottom ? 1.0 : (1 - std::exp((1 - k) / 2.0))));
+        h->w_host[k - 1] = -sPOC * (c.open_bottom ? 1.0 : (1 - std::exp((1 - k) / 2.0)));
         const double z = h->zc[k - 1];
-        w[(Nz + 1) + (k - 1)] = (FT)(-wmin + (wmax - wmin) * std::min(0.0, z - std::min(0.0, 0.0)) / 5000);
+        h->w_host[(Nz + 1) + (k - 1)] = -wmin + (wmax - wmin) * std::min(0.0, z - std::min(0.0, 0.0)) / 5000;
     }
+    std::vector<FT> w(h->w_host.size());
+    for (size_t q = 0; q < w.size(); q++) w[q] = (FT)h->w_host[q];
     auto up = [&](void** d, const void* src, size_t bytes) -> obme_status {
         OBME_HIP_TRY(hipMalloc(d, bytes));
         OBME_HIP_TRY(hipMemcpy(*d, src, bytes, hipMemcpyHostToDevice));
@@ -842,6 +904,29 @@ extern "C" obme_status obme_pisces_create(const obme_pisces_config* cfg, obme_pi
             }
         }
     }
+    {
+        const obme_sediment& sd = cfg->sediment;
+        if (sd.model != OBME_SEDIMENT_NONE && sd.model != OBME_SEDIMENT_INSTANT_REMINERALISATION)
+            return obme_fail(OBME_ERR_UNSUPPORTED, "PISCES takes an InstantRemineralisation sediment");
+        if (sd.model == OBME_SEDIMENT_INSTANT_REMINERALISATION) {
+            if (sd.receiver < 0 || sd.receiver >= NW)
+                return obme_fail(OBME_ERR_INVALID, "the sediment's receiver is not a written tracer");
+            if (sd.n_sinking < 1 || sd.n_sinking > OBME_MAX_SEDIMENT_TRACERS)
+                return obme_fail(OBME_ERR_INVALID, "the sediment needs sinking tracers");
+            for (int q = 0; q < sd.n_sinking; q++) {
+                const int t = sd.sinking[q];
+                if (t != OBME_PI_POC && t != OBME_PI_SFE && t != OBME_PI_GOC && t != OBME_PI_BFE &&
+                    t != OBME_PI_PSI && t != OBME_PI_CACO3)
+                    return obme_fail(OBME_ERR_INVALID, "a sediment sinking tracer is not one of PISCES's "
+                                                       "sinking tracers (POC, SFe, GOC, BFe, PSi, CaCO3)");
+                for (int q2 = 0; q2 < q; q2++)
+                    if (sd.sinking[q2] == t)
+                        return obme_fail(OBME_ERR_INVALID, "a sinking tracer of the sediment is listed twice");
+            }
+            if (sd.timestepper != OBME_SEDIMENT_AB2)
+                return obme_fail(OBME_ERR_UNSUPPORTED, "only the QuasiAdamsBashforth2 sediment time stepper is built");
+        }
+    }
     obme_pisces* h = new obme_pisces();
     h->cfg = *cfg;
     h->prec = cfg->precision;
@@ -854,8 +939,10 @@ extern "C" obme_status obme_pisces_create(const obme_pisces_config* cfg, obme_pi
     h->dz.assign(g.dz, g.dz + g.Nz);
     h->cfg.grid.zc = h->cfg.grid.zf = h->cfg.grid.dz = nullptr;
     h->scaling = cfg->scaling.n_groups > 0;
+    h->sediment = cfg->sediment.model == OBME_SEDIMENT_INSTANT_REMINERALISATION;
     // the fused step: FP32-mixed, one wave per column of at most 64 levels
-    h->fused = h->prec == OBME_FP32_MIXED && g.Nz <= 64;
+    // (with a sediment, the per-stage kernels)
+    h->fused = h->prec == OBME_FP32_MIXED && g.Nz <= 64 && !h->sediment;
     if (const char* e = std::getenv("OBME_PISCES_FUSED")) h->fused = h->fused && std::atoi(e) != 0;
     if (const char* e = std::getenv("OBME_PISCES_FUSED_STEPS")) {
         const int n = std::atoi(e);
@@ -870,6 +957,11 @@ extern "C" obme_status obme_pisces_create(const obme_pisces_config* cfg, obme_pi
     alloc(&h->d_zeu, h->ncol * es);
     alloc(&h->d_mlPAR, h->ncol * es);
     alloc(&h->d_Gm, (size_t)NW * h->ncell * es);
+    if (h->sediment) {
+        alloc((void**)&h->d_sedGm, (size_t)h->ncol * sizeof(double));
+        if (st == OBME_OK && hipMemset(h->d_sedGm, 0, (size_t)h->ncol * sizeof(double)) != hipSuccess)
+            st = obme_fail(OBME_ERR_HIP, "hipMemset");
+    }
     if (cfg->co2.enabled || cfg->o2.enabled) {
         alloc(&h->d_sflux, 2 * (size_t)h->ncol * sizeof(double));
         alloc(&h->d_Hsurf, (size_t)h->ncol * sizeof(double));
@@ -936,6 +1028,25 @@ static PiscesArgs<FT> pisces_args(const obme_pisces* h, double t)
     a.solver = c.solver;
     (void)t;
     (void)lat;
+    if (h->sediment) {
+        // -w of each listed tracer's bottom face (the face velocities the
+        // sinking uses: w[0] of the POC or GOC profile)
+        const obme_sediment& sd = c.sediment;
+        a.sed_on = 1;
+        a.sed_recv = sd.receiver;
+        for (int q = 0; q < sd.n_sinking; q++) {
+            const int t = sd.sinking[q];
+            const bool poc = (t == OBME_PI_POC || t == OBME_PI_SFE);
+            a.sedw[t] = -(poc ? h->w_host[0] : h->w_host[h->cfg.grid.Nz + 1]);
+        }
+        a.sed_a = (FT)sd.burial_efficiency_constant1;
+        a.sed_b = (FT)sd.burial_efficiency_constant2;
+        a.sed_k = (FT)sd.burial_efficiency_half_saturation;
+        a.sed_cn = 1.5 + sd.ab2_chi;
+        a.sed_cm = 0.5 + sd.ab2_chi;
+        a.storage = h->call_storage;
+        a.sedGm = h->d_sedGm;
+    }
     return a;
 }
 
@@ -974,6 +1085,14 @@ static obme_status pisces_stage(obme_pisces* h, const void* const* in, void* con
     a.sk.dl_chl = (FT)host_day_length(t, lat);     // day_length(t, φ), growth_rate.jl:128
     a.dt = dt;
     a.stage = stage;
+    if (h->sediment) {
+        a.sed_dt[0] = (8.0 / 15.0) * dt;
+        a.sed_dt[1] = (5.0 / 12.0 + -17.0 / 60.0) * dt;
+        a.sed_dt[2] = (3.0 / 4.0 + -5.0 / 12.0) * dt;
+        a.sed_init = h->call_init;
+        a.sed_ab2 = h->call_ab2;
+        if (mode != 0) a.storage = nullptr;
+    }
     const double gam[3] = {8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0}, zet[3] = {0.0, -17.0 / 60.0, -5.0 / 12.0};
     a.A = (FT)(dt * gam[stage]);
     a.B = (FT)(dt * zet[stage]);
@@ -992,11 +1111,13 @@ static obme_status pisces_stage(obme_pisces* h, const void* const* in, void* con
     }
     const unsigned nb = (unsigned)((h->ncell + bs - 1) / bs);
     if (mode == 0) {
-        hipLaunchKernelGGL((k_pisces_cell<FT, 0, 1>), dim3(nb), dim3(bs), 0, s, a);
+        if (h->sediment) hipLaunchKernelGGL((k_pisces_cell<FT, 0, 1, true>), dim3(nb), dim3(bs), 0, s, a);
+        else hipLaunchKernelGGL((k_pisces_cell<FT, 0, 1>), dim3(nb), dim3(bs), 0, s, a);
     } else {
         hipLaunchKernelGGL(k_pisces_omega<FT>, dim3(nb), dim3(bs), 0, s, a);
         OBME_HIP_TRY(hipGetLastError());
-        hipLaunchKernelGGL((k_pisces_cell<FT, 1, 0>), dim3(nb), dim3(bs), 0, s, a);
+        if (h->sediment) hipLaunchKernelGGL((k_pisces_cell<FT, 1, 0, true>), dim3(nb), dim3(bs), 0, s, a);
+        else hipLaunchKernelGGL((k_pisces_cell<FT, 1, 0>), dim3(nb), dim3(bs), 0, s, a);
     }
     OBME_HIP_TRY(hipGetLastError());
     return OBME_OK;
@@ -1170,8 +1291,28 @@ static obme_status pisces_fused_steps(obme_pisces* h, void* const* tracers, int 
     return OBME_OK;
 }
 
+static obme_status pisces_step(obme_pisces* h, void* const* tracers, double* storage, int32_t nsteps, double dt,
+                               double t0, void* stream);
+
 extern "C" obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, int32_t nsteps, double dt, double t0,
                                         void* stream)
+{
+    return pisces_step(h, tracers, nullptr, nsteps, dt, t0, stream);
+}
+
+extern "C" obme_status obme_pisces_step_with_sediment(obme_pisces* h, void* const* tracers, double* storage,
+                                                      int32_t nsteps, double dt, double t0, void* stream)
+{
+    if (!h) return obme_fail(OBME_ERR_INVALID, "null handle");
+    if (!h->sediment) return obme_fail(OBME_ERR_INVALID, "the PISCES handle has no sediment");
+    if (!storage) return obme_fail(OBME_ERR_INVALID, "null sediment storage");
+    return pisces_step(h, tracers, storage, nsteps, dt, t0, stream);
+}
+
+// nsteps steps; with the sediment's storage, its steps in the same passes
+// (the handle's clock and G⁻ advanced), without, its coupling alone
+static obme_status pisces_step(obme_pisces* h, void* const* tracers, double* storage, int32_t nsteps, double dt,
+                               double t0, void* stream)
 {
     if (!h || !tracers) return obme_fail(OBME_ERR_INVALID, "null argument");
     if (nsteps < 0) return obme_fail(OBME_ERR_INVALID, "negative nsteps");
@@ -1213,9 +1354,20 @@ extern "C" obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, in
         outC[n] = h->d_copy[n];
     }
     const double cst[3] = {0.0, 8.0 / 15.0, 2.0 / 3.0};
+    // the sediment of this call: its first step ever (the lazy first
+    // tendency) and whether the first sediment step repeats the previous Δt
+    const bool sed_init = h->sed_iter == 0;
+    const bool sed_ab2 = !sed_init && (8.0 / 15.0) * dt == h->sed_last_dt;
+    struct CallSediment {
+        obme_pisces* h;
+        ~CallSediment() { h->call_storage = nullptr; h->call_init = h->call_ab2 = 0; }
+    } keep{h};
+    h->call_storage = h->sediment ? storage : nullptr;
     bool inCopy = false;  // where the current state lives
     for (int st = 0; st < nsteps; st++) {
         const double t = t0 + st * dt;
+        h->call_init = (sed_init && st == 0) ? 1 : 0;
+        h->call_ab2 = (sed_ab2 && st == 0) ? 1 : 0;
         for (int q = 0; q < 3; q++) {
             const void* const* in = inCopy ? inC : inU;
             void* const* out = inCopy ? outU : outC;
@@ -1227,5 +1379,10 @@ extern "C" obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, in
     if (inCopy)
         for (int n = 0; n < NW; n++)
             OBME_HIP_TRY(hipMemcpyAsync(tracers[n], h->d_copy[n], h->field_bytes, hipMemcpyDeviceToDevice, s));
+    if (h->sediment && storage) {
+        // three sediment steps per step, the last of Δt (γ3 + ζ3)
+        h->sed_iter += 3 * (long long)nsteps;
+        h->sed_last_dt = (3.0 / 4.0 + -5.0 / 12.0) * dt;
+    }
     return OBME_OK;
 }

@@ -336,10 +336,13 @@ struct ObmePiscesConfig
     band_chi :: NTuple{3, Float64}
     band_division :: NTuple{3, Float64}
     surface_PAR :: Float64
+    surface_mode :: Int32            # OBME_SURFACE_CONSTANT | _DIURNAL | _ARRAY
+    surface_array :: Ptr{Cvoid}      # OBME_SURFACE_ARRAY: Nx×Ny device values
     solver :: ObmeCarbonSolver
     scaling :: ObmeNegativeScaling
     co2 :: ObmeCO2Exchange           # top DIC flux (validation/PISCES/column.jl:63)
     o2 :: ObmeO2Exchange             # top O₂ flux (:64)
+    sediment :: ObmeSediment         # PISCES(; sediment): InstantRemineralisation or none
 end
 
 """

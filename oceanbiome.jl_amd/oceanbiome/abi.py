@@ -267,6 +267,7 @@ class PiscesConfig(C.Structure):
         ("scaling", NegativeScaling),
         ("co2", CO2Exchange),
         ("o2", O2Exchange),
+        ("sediment", Sediment),
     ]
 
 

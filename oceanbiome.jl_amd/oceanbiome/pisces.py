@@ -122,10 +122,12 @@ class PISCES:
     def __init__(self, grid, surface_PAR="default", mixed_layer_depth=-50.0, kappa=1.0, latitude=45.0,
                  bands=((400, 500), (500, 600), (600, 700)), surface_PAR_division=None, POC_speed=2 / day,
                  open_bottom=True, scale_negatives=False, invalid_fill_value=float("nan"),
-                 co2_exchange=None, o2_exchange=None, **overrides):
+                 co2_exchange=None, o2_exchange=None, sediment=None, **overrides):
         """co2_exchange / o2_exchange: the top boundary conditions of DIC and O2
         (a CarbonDioxideGasExchangeBoundaryCondition / OxygenGasExchangeBoundaryCondition,
-        or True for the defaults; validation/PISCES/column.jl:63-64)."""
+        or True for the defaults; validation/PISCES/column.jl:63-64).  sediment:
+        an InstantRemineralisationSediment over PISCES's tracers (PISCES.jl:233),
+        e.g. sinking_tracers=("POC", "GOC", "CaCO₃"), remineralisation_reciever="DIC"."""
         from .models import CarbonDioxideGasExchangeBoundaryCondition, OxygenGasExchangeBoundaryCondition
         self.grid = grid
         def choose(x, make):
@@ -177,6 +179,7 @@ class PISCES:
             raise NotImplementedError("the device path has three PAR bands")
         self.division = [1 / nb] * nb if surface_PAR_division is None else list(surface_PAR_division)
         self.open_bottom = bool(open_bottom)
+        self.sediment = sediment
         from .models import CarbonChemistry
         self.carbon_chemistry = CarbonChemistry()
         self._keep = []
@@ -253,6 +256,8 @@ class PISCES:
             cfg.co2 = self.co2_exchange.struct()
         if self.o2_exchange is not None:
             cfg.o2 = self.o2_exchange.struct()
+        if self.sediment is not None:
+            cfg.sediment = self.sediment.struct(list(self.required_biogeochemical_tracers()))
         return cfg
 
     def __repr__(self):

@@ -73,6 +73,8 @@ def library():
                                                  vp]),
         "obme_pisces_add_tendencies": (C.c_int32, [vp, C.POINTER(vp), C.POINTER(vp), C.c_double, vp]),
         "obme_pisces_step": (C.c_int32, [vp, C.POINTER(vp), C.c_int32, C.c_double, C.c_double, vp]),
+        "obme_pisces_step_with_sediment": (C.c_int32, [vp, C.POINTER(vp), vp, C.c_int32, C.c_double, C.c_double,
+                                                       vp]),
         "obme_kernel_timing": (C.c_int32, [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
@@ -93,6 +95,7 @@ EXPORTED_SYMBOLS = (
     "obme_batch_step_with_sediment",
     "obme_batch_configure", "obme_batch_autotune", "obme_batch_describe", "obme_pisces_create",
     "obme_pisces_destroy", "obme_pisces_update_state", "obme_pisces_add_tendencies", "obme_pisces_step",
+    "obme_pisces_step_with_sediment",
 )
 
 KERNEL_SWEEP = 0      # OBME_KERNEL_SWEEP: the column sweep
@@ -375,6 +378,11 @@ class HIPPISCES:
         h = C.c_void_p()
         _check(library().obme_pisces_create(C.byref(self.cfg), C.byref(h)))
         self.handle = h
+        # PISCES(; sediment): the InstantRemineralisation storage (Ny, Nx)
+        self.sediment_storage = None
+        if getattr(model, "sediment", None) is not None:
+            g = model.grid
+            self.sediment_storage = torch.zeros((g.Ny, g.Nx), dtype=torch.float64, device="cuda")
 
     def close(self):
         if getattr(self, "handle", None):
@@ -416,7 +424,14 @@ class HIPPISCES:
                                                     _stream()))
 
     def step(self, tracers, nsteps, dt, t0=0.0):
+        """nsteps RK3 steps in place (with the sediment, its storage advanced
+        in the same passes)."""
         self._check_fields(tracers, 26)
+        if self.sediment_storage is not None:
+            _check(library().obme_pisces_step_with_sediment(self.handle, _ptr_array(tracers),
+                                                            _ptr(self.sediment_storage), int(nsteps), float(dt),
+                                                            float(t0), _stream()))
+            return
         _check(library().obme_pisces_step(self.handle, _ptr_array(tracers), int(nsteps), float(dt), float(t0),
                                           _stream()))
 

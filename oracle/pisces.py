@@ -765,13 +765,64 @@ def scale_negatives(F, groups):
     return F
 
 
-def step(q, light, F, zc, zf, dz, nsteps, dt, t0, solver, open_bottom=True, nthreads=8, scaling=None, gas=None):
+def sediment_flux(q, zc, F, sed, open_bottom=True, scaling=None):
+    """The sinking flux through the bottom face of every column (Sediments/
+    tracked_fields.jl:53-70): -w c of the bottom cell over the sediment's
+    sinking tracers, in their listed order, with their bottom-face velocity;
+    the bottom cell after the modifiers when `scaling` is given."""
+    wP, wG = face_velocities(q, zc, open_bottom)
+    bottom = {name: F[name][0].copy() for name in TRACERS}
+    if scaling:
+        scale_negatives(bottom, scaling)
+    which = dict(SINKING)
+    f = None
+    for qi in range(sed.n_sinking):
+        name = TRACERS[sed.sinking[qi]]
+        w0 = (wP, wG)[which[name]][0] if name in which else 0.0
+        fq = -(w0 * bottom[name])
+        f = fq if f is None else f + fq
+    return f
+
+
+def burial_efficiency(sed, f):
+    """e = a + b (f / (k + f))^2 (instant_remineralisation.jl:101-123)."""
+    qf = f / (sed.burial_efficiency_half_saturation + f)
+    return sed.burial_efficiency_constant1 + sed.burial_efficiency_constant2 * (qf * qf)
+
+
+def sediment_time_step(sed, state, f, dt):
+    """One QuasiAdamsBashforth2 step of the storage (timesteppers.jl:16-43 in
+    Oceananigans' AB2 time_step!; the oracle's orc_sediment_time_step over
+    the columns): the tendency e f evaluated before the first step from this
+    flux, Euler where dt differs from the previous step's."""
+    if not math.isfinite(dt):
+        return
+    if state.iteration == 0:
+        state.Gn = burial_efficiency(sed, f) * f
+    euler = not (dt == state.last_dt)
+    chi = -0.5 if euler else sed.ab2_chi
+    Gu = (1.5 + chi) * state.Gn - (0.0 if euler else (0.5 + chi) * state.Gm)
+    state.storage = state.storage + dt * Gu
+    state.Gm = state.Gn
+    state.Gn = burial_efficiency(sed, f) * f
+    state.last_dt = dt
+    state.iteration += 1
+
+
+def step(q, light, F, zc, zf, dz, nsteps, dt, t0, solver, open_bottom=True, nthreads=8, scaling=None, gas=None,
+         sediment=None):
     """nsteps RK3 steps (Oceananigans γ, ζ; the substep form of
     BoxModel/timesteppers.jl:79-92) of the 24 written tracers, in place;
-    `scaling` (conserved_groups) runs at the start of every stage."""
+    `scaling` (conserved_groups) runs at the start of every stage.
+    `sediment` = (obme_sediment, SedimentState (Ny, Nx)): an
+    InstantRemineralisation sediment under every column: the receiver's
+    bottom cell gains (1 - e) f / Δz at every stage (tracer_coupling.jl:30-38)
+    and after every substep the storage takes its step of the stage Δt
+    (Sediments/update_state.jl:6-17) from the flux of the new state."""
     g = (8 / 15, 5 / 12, 3 / 4)
     z = (None, -17 / 60, -5 / 12)
     cst = (0.0, 8 / 15, 2 / 3)
+    sed, state = sediment if sediment is not None else (None, None)
     for s in range(nsteps):
         Gm = None
         for st in range(3):
@@ -779,10 +830,17 @@ def step(q, light, F, zc, zf, dz, nsteps, dt, t0, solver, open_bottom=True, nthr
             if scaling:
                 scale_negatives(F, scaling)
             G, _ = full_tendencies(q, light, F, zc, zf, dz, t, solver, open_bottom, nthreads, gas)
+            if sed is not None:
+                f = sediment_flux(q, zc, F, sed, open_bottom)
+                recv = TRACERS[sed.receiver]
+                G[recv][0] = G[recv][0] + (1 - burial_efficiency(sed, f)) * f / dz[0]
             for name in WRITTEN:
                 if st == 0:
                     F[name] = F[name] + (dt * g[0]) * G[name]
                 else:
                     F[name] = F[name] + dt * (g[st] * G[name] + z[st] * Gm[name])
             Gm = G
+            if sed is not None:
+                sdt = (g[0], g[1] + z[1], g[2] + z[2])[st] * dt
+                sediment_time_step(sed, state, sediment_flux(q, zc, F, sed, open_bottom, scaling), sdt)
     return F
