@@ -104,48 +104,76 @@ __global__ void __launch_bounds__(256) k_pisces_columns(PiscesArgs<FT> a)
     const FT PAR0 = a.surf ? a.surf[col] : a.surface_PAR;
     const FT thr = PAR0 * FT(1e-3);
     const FT lthr = m_log(thr);
-    FT band[3];
-    FT zeu = FT(-INFINITY);
+    FT band[3] = {FT(0), FT(0), FT(0)};
     FT acc = 0, depth = 0;
     FT PARup = 0;
+    // the euphotic crossing: the first level below the top with PAR <= thr,
+    // its PAR and the PAR of the level above (the logarithms after the walk)
+    int kc = -1;
+    FT Pc = FT(0), Pu = FT(0);
     // Chl^e of the three bands from one logarithm (FP64: exp(e ln Chl),
     // within a few ulp of pow; pow for Chl <= 0 or a zero exponent)
     const bool pw = !(a.e[0] != FT(0) && a.e[1] != FT(0) && a.e[2] != FT(0));
-    for (int k = Nz - 1; k >= 0; k--) {
-        const unsigned p = pidx(a, i, j, k);
-        const FT Chl = a.in[OBME_PI_PCHL][p] + a.in[OBME_PI_DCHL][p];
-        const bool via_log = Chl > FT(0) && !pw;
-        const FT lChl = via_log ? m_log(Chl) : FT(0);
+    const long plane = (long)a.Nx * a.Ny;
+    // One thread per column, one wave per SIMD: the walk goes in chunks of
+    // four levels, whose loads and logarithms and exponentials are
+    // independent of each other; only the bands' running products, the
+    // stores and the sums follow level by level.
+    constexpr int CH = 4;
+    for (int k0 = Nz - 1; k0 >= 0; k0 -= CH) {
+        FT E[CH][3];
 #pragma unroll
-        for (int n = 0; n < 3; n++) {
-            const FT ce = via_log ? m_exp(a.e[n] * lChl) : m_pow(Chl, a.e[n]);
-            const FT att = a.kw[n] + a.chi[n] * ce;
-            if (k == Nz - 1) band[n] = PAR0 * a.division[n] * m_exp(zc[k] * att);
-            else band[n] = band[n] * m_exp((zc[k] - zc[k + 1]) * att);
+        for (int u = 0; u < CH; u++) {
+            const int k = k0 - u;
+            const int kk = k >= 0 ? k : 0;
+            const unsigned p = pidx(a, i, j, kk);
+            const FT Chl = a.in[OBME_PI_PCHL][p] + a.in[OBME_PI_DCHL][p];
+            const bool via_log = Chl > FT(0) && !pw;
+            const FT lChl = via_log ? m_log(Chl) : FT(0);
+            // the exponent increment from the level above (the top level:
+            // from the surface)
+            const FT dzl = (kk == Nz - 1) ? zc[kk] : zc[kk] - zc[kk + 1 < Nz ? kk + 1 : kk];
+#pragma unroll
+            for (int n = 0; n < 3; n++) {
+                const FT ce = via_log ? m_exp(a.e[n] * lChl) : m_pow(Chl, a.e[n]);
+                const FT att = a.kw[n] + a.chi[n] * ce;
+                E[u][n] = m_exp(dzl * att);
+            }
         }
-        const FT PAR = band[0] + band[1] + band[2];
-        const long c = (long)col + (long)a.Nx * a.Ny * k;
-        a.PAR[c] = PAR;
-        a.PAR[ncell + c] = band[0];
-        a.PAR[2 * ncell + c] = band[1];
-        a.PAR[3 * ncell + c] = band[2];
-        // the euphotic depth: the logarithms of PAR at the first crossing
-        // and the level above it only
-        if (k < Nz - 1 && PAR <= thr && zeu == FT(-INFINITY)) {
-            const FT lPAR = m_log(PAR), lPARup = m_log(PARup);
-            zeu = zc[k] + (lthr - lPAR) * (zc[k] - zc[k + 1]) / (lPAR - lPARup);
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int k = k0 - u;
+            if (k < 0) break;
+#pragma unroll
+            for (int n = 0; n < 3; n++)
+                band[n] = (k == Nz - 1) ? PAR0 * a.division[n] * E[u][n] : band[n] * E[u][n];
+            const FT PAR = band[0] + band[1] + band[2];
+            const long c = (long)col + plane * k;
+            a.PAR[c] = PAR;
+            a.PAR[ncell + c] = band[0];
+            a.PAR[2 * ncell + c] = band[1];
+            a.PAR[3 * ncell + c] = band[2];
+            const bool cross = kc < 0 && k < Nz - 1 && PAR <= thr;
+            kc = cross ? k : kc;
+            Pc = cross ? PAR : Pc;
+            Pu = cross ? PARup : Pu;
+            PARup = PAR;
+            // mixed-layer integral, top cell first
+            const FT zk = zf[k];
+            const FT zk1 = (k + 1 < Nz) ? zf[k + 1] : FT(a.zf_top);
+            const FT dzk = zk1 - zk;
+            const FT dzk1 = (zk1 > zmxl) ? zk1 - zmxl : FT(0);
+            const FT dz = (zk >= zmxl) ? dzk : dzk1;
+            acc = acc + PAR * dz;
+            depth = depth + dz;
         }
-        PARup = PAR;
-        // mixed-layer integral, top cell first
-        const FT zk = zf[k];
-        const FT zk1 = (k + 1 < Nz) ? zf[k + 1] : FT(a.zf_top);
-        const FT dzk = zk1 - zk;
-        const FT dzk1 = (zk1 > zmxl) ? zk1 - zmxl : FT(0);
-        const FT dz = (zk >= zmxl) ? dzk : dzk1;
-        acc = acc + PAR * dz;
-        depth = depth + dz;
     }
-    (void)PARup;
+    // the euphotic depth, interpolated in log PAR with the level above
+    FT zeu = FT(-INFINITY);
+    if (kc >= 0) {
+        const FT lPAR = m_log(Pc), lPARup = m_log(Pu);
+        zeu = zc[kc] + (lthr - lPAR) * (zc[kc] - zc[kc + 1]) / (lPAR - lPARup);
+    }
     if (!(zeu > FT(-INFINITY) && zeu < FT(INFINITY))) zeu = zc[0] - (Nz > 1 ? zc[1] - zc[0] : FT(1));
     a.zeu[col] = zeu;
     a.mlPAR[col] = acc / depth;
