@@ -721,6 +721,12 @@ obme_status obme_pisces_step(obme_pisces* h, void* const* tracers, int32_t nstep
  * the storage alone. */
 obme_status obme_pisces_step_with_sediment(obme_pisces* h, void* const* tracers, double* storage, int32_t nsteps,
                                            double dt, double t0, void* stream);
+/* update_biogeochemical_state!(model, sediment) for PISCES's sediment
+ * (Sediments/update_state.jl:6-17), as obme_sediment_update_state: the
+ * bottom-face flux of the listed tracers and, for a finite dt (the model's
+ * last stage Δt), one QuasiAdamsBashforth2 step of the storage. */
+obme_status obme_pisces_sediment_update_state(obme_pisces* h, const void* const* tracers, double* storage,
+                                              double dt, void* stream);
 
 /* ---- kernel timing ------------------------------------------------------ */
 

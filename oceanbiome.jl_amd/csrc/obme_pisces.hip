@@ -77,6 +77,7 @@ struct PiscesArgs {
     double sed_dt[3], sed_cn, sed_cm;
     double* storage;        // [Nx*Ny] or null
     double* sedGm;          // [Nx*Ny] G⁻: the tendency of the sediment's last step
+    double* sedGn;          // [Nx*Ny] Gⁿ: e f after the sediment's last step
 };
 
 template <typename FT>
@@ -443,6 +444,8 @@ __global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES :
         for (int n = 0; n < NW; n++) a.out[n][p] = a.out[n][p] + G[n];
     } else {
         const double g1 = 8.0 / 15.0, g2 = 5.0 / 12.0, g3 = 3.0 / 4.0, z2 = -17.0 / 60.0, z3 = -5.0 / 12.0;
+        FT fnew = FT(0);
+        bool fnew_first = true;
 #pragma unroll
         for (int n = 0; n < NW; n++) {
             FT u = v[n];
@@ -457,8 +460,46 @@ __global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES :
             }
             a.out[n][p] = u;
             a.Gm[(long)n * ncell + c] = G[n];
+            if (SED && sed_cell && a.stage == 2 && a.sedw[n] != FT(0)) {
+                fnew = fnew_first ? a.sedw[n] * u : fnew + a.sedw[n] * u;
+                fnew_first = false;
+            }
+        }
+        if (SED && sed_cell && a.stage == 2 && a.storage) {
+            // Gⁿ = e f of the step's final state, for a sediment step by the hook
+            const FT qf = fnew / (a.sed_k + fnew);
+            a.sedGn[col] = (double)((a.sed_a + a.sed_b * (qf * qf)) * fnew);
         }
     }
+}
+
+// One time step of the sediment's storage under PISCES (Sediments/
+// update_state.jl:6-17; QuasiAdamsBashforth2, timesteppers.jl:16-43): the
+// tendency Gⁿ (at the first step, e f of the tracked flux), a forward Euler
+// step or the χ-weighted step with G⁻, then G⁻ = Gⁿ and Gⁿ = e f.  One
+// thread per column.
+template <typename FT>
+__global__ void __launch_bounds__(256) k_pisces_sediment_state(PiscesArgs<FT> a, double* storage, double dt,
+                                                               int euler, int init)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= a.Nx * a.Ny) return;
+    const unsigned p = pidx(a, col % a.Nx, col / a.Nx, 0);
+    FT f = FT(0);
+    bool first = true;
+#pragma unroll
+    for (int n = 0; n < NW; n++)
+        if (a.sedw[n] != FT(0)) {
+            f = first ? a.sedw[n] * a.in[n][p] : f + a.sedw[n] * a.in[n][p];
+            first = false;
+        }
+    const FT qf = f / (a.sed_k + f);
+    const double ge = (double)((a.sed_a + a.sed_b * (qf * qf)) * f);
+    const double Gn = init ? ge : a.sedGn[col];
+    const double Gu = euler ? Gn : a.sed_cn * Gn - a.sed_cm * a.sedGm[col];
+    if (storage) storage[col] = storage[col] + dt * Gu;
+    a.sedGm[col] = Gn;
+    a.sedGn[col] = ge;
 }
 
 // Thread per column: the surface constants of the CO2 and O2 exchange
@@ -837,6 +878,7 @@ struct obme_pisces {
     // an InstantRemineralisation sediment under every column (cfg.sediment)
     bool sediment = false;
     double* d_sedGm = nullptr;         // [Nx*Ny] G⁻ of the sediment's last step
+    double* d_sedGn = nullptr;         // [Nx*Ny] Gⁿ: e f after the sediment's last step
     long long sed_iter = 0;            // sediment steps taken (its clock's iteration)
     double sed_last_dt = HUGE_VAL;     // Δt of its last step (Inf before the first)
     // the current step call's sediment: storage (or null), and this step's
@@ -854,6 +896,7 @@ extern "C" obme_status obme_pisces_destroy(obme_pisces* h)
     for (void* p : h->d_copy)
         if (p) (void)hipFree(p);
     if (h->d_sedGm) (void)hipFree(h->d_sedGm);
+    if (h->d_sedGn) (void)hipFree(h->d_sedGn);
     delete h;
     return OBME_OK;
 }
@@ -987,7 +1030,9 @@ extern "C" obme_status obme_pisces_create(const obme_pisces_config* cfg, obme_pi
     alloc(&h->d_Gm, (size_t)NW * h->ncell * es);
     if (h->sediment) {
         alloc((void**)&h->d_sedGm, (size_t)h->ncol * sizeof(double));
-        if (st == OBME_OK && hipMemset(h->d_sedGm, 0, (size_t)h->ncol * sizeof(double)) != hipSuccess)
+        alloc((void**)&h->d_sedGn, (size_t)h->ncol * sizeof(double));
+        if (st == OBME_OK && (hipMemset(h->d_sedGm, 0, (size_t)h->ncol * sizeof(double)) != hipSuccess ||
+                              hipMemset(h->d_sedGn, 0, (size_t)h->ncol * sizeof(double)) != hipSuccess))
             st = obme_fail(OBME_ERR_HIP, "hipMemset");
     }
     if (cfg->co2.enabled || cfg->o2.enabled) {
@@ -1074,6 +1119,7 @@ static PiscesArgs<FT> pisces_args(const obme_pisces* h, double t)
         a.sed_cm = 0.5 + sd.ab2_chi;
         a.storage = h->call_storage;
         a.sedGm = h->d_sedGm;
+        a.sedGn = h->d_sedGn;
     }
     return a;
 }
@@ -1335,6 +1381,36 @@ extern "C" obme_status obme_pisces_step_with_sediment(obme_pisces* h, void* cons
     if (!h->sediment) return obme_fail(OBME_ERR_INVALID, "the PISCES handle has no sediment");
     if (!storage) return obme_fail(OBME_ERR_INVALID, "null sediment storage");
     return pisces_step(h, tracers, storage, nsteps, dt, t0, stream);
+}
+
+extern "C" obme_status obme_pisces_sediment_update_state(obme_pisces* h, const void* const* tracers, double* storage,
+                                                         double dt, void* stream)
+{
+    if (!h || !tracers) return obme_fail(OBME_ERR_INVALID, "null argument");
+    if (!h->sediment) return obme_fail(OBME_ERR_INVALID, "the PISCES handle has no sediment");
+    const obme_sediment& sd = h->cfg.sediment;
+    for (int q = 0; q < sd.n_sinking; q++)
+        if (!tracers[sd.sinking[q]]) return obme_fail(OBME_ERR_INVALID, "null tracer pointer");
+    // the tracked fields only: no step before the model's first stage
+    if (!std::isfinite(dt)) return OBME_OK;
+    const bool euler = !(dt == h->sed_last_dt);
+    const int init = h->sed_iter == 0 ? 1 : 0;
+    const unsigned nb = (unsigned)((h->ncol + 255) / 256);
+    if (h->prec == OBME_FP64) {
+        PiscesArgs<double> a = pisces_args<double>(h, 0.0);
+        for (int n = 0; n < NTR; n++) a.in[n] = (const double*)tracers[n];
+        hipLaunchKernelGGL(k_pisces_sediment_state<double>, dim3(nb), dim3(256), 0, (hipStream_t)stream, a, storage,
+                           dt, euler ? 1 : 0, init);
+    } else {
+        PiscesArgs<float> a = pisces_args<float>(h, 0.0);
+        for (int n = 0; n < NTR; n++) a.in[n] = (const float*)tracers[n];
+        hipLaunchKernelGGL(k_pisces_sediment_state<float>, dim3(nb), dim3(256), 0, (hipStream_t)stream, a, storage,
+                           dt, euler ? 1 : 0, init);
+    }
+    OBME_HIP_TRY(hipGetLastError());
+    h->sed_iter += 1;
+    h->sed_last_dt = dt;
+    return OBME_OK;
 }
 
 // nsteps steps; with the sediment's storage, its steps in the same passes

@@ -75,6 +75,7 @@ def library():
         "obme_pisces_step": (C.c_int32, [vp, C.POINTER(vp), C.c_int32, C.c_double, C.c_double, vp]),
         "obme_pisces_step_with_sediment": (C.c_int32, [vp, C.POINTER(vp), vp, C.c_int32, C.c_double, C.c_double,
                                                        vp]),
+        "obme_pisces_sediment_update_state": (C.c_int32, [vp, C.POINTER(vp), vp, C.c_double, vp]),
         "obme_kernel_timing": (C.c_int32, [vp, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     }
     for name, (res, args) in sig.items():
@@ -95,7 +96,7 @@ EXPORTED_SYMBOLS = (
     "obme_batch_step_with_sediment",
     "obme_batch_configure", "obme_batch_autotune", "obme_batch_describe", "obme_pisces_create",
     "obme_pisces_destroy", "obme_pisces_update_state", "obme_pisces_add_tendencies", "obme_pisces_step",
-    "obme_pisces_step_with_sediment",
+    "obme_pisces_step_with_sediment", "obme_pisces_sediment_update_state",
 )
 
 KERNEL_SWEEP = 0      # OBME_KERNEL_SWEEP: the column sweep
@@ -422,6 +423,16 @@ class HIPPISCES:
         self._check_fields(G, 24)
         _check(library().obme_pisces_add_tendencies(self.handle, _ptr_array(tracers), _ptr_array(G), float(t),
                                                     _stream()))
+
+    def update_sediment_state(self, tracers, dt):
+        """update_biogeochemical_state!(model, sediment) (Sediments/update_state.jl:6-17):
+        one sediment step of dt (none when dt is not finite) from the sinking
+        flux of the tracers, on self.sediment_storage."""
+        if self.sediment_storage is None:
+            raise OBMEError("no sediment configured")
+        self._check_fields(tracers, 26)
+        _check(library().obme_pisces_sediment_update_state(self.handle, _ptr_array(tracers),
+                                                           _ptr(self.sediment_storage), float(dt), _stream()))
 
     def step(self, tracers, nsteps, dt, t0=0.0):
         """nsteps RK3 steps in place (with the sediment, its storage advanced

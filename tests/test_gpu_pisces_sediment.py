@@ -6,7 +6,8 @@ sinking into the sediment, remineralised into DIC.  FP64: relative L2 <=
 1e-12 per tracer and for the storage; FP32-mixed (the per-stage kernels):
 <= 1e-5.  Also: two calls in a row (the sediment clock), obme_pisces_step
 on such a handle (the coupling alone, storage untouched), the tendency
-hook's coupled term, and the configurations PISCES rejects."""
+hook's coupled term, the update-state hook's sediment steps, and the
+configurations PISCES rejects."""
 import copy
 
 import numpy as np
@@ -127,6 +128,32 @@ def test_coupling_alone_and_the_tendency_hook():
             assert np.all(d[1:] == 0)
         else:
             assert np.all(d == 0), n
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_update_state_hook_sequence(precision):
+    """obme_pisces_sediment_update_state call by call on fixed tracers: no
+    step for a non-finite dt, the lazy first tendency, Euler on a changed dt,
+    the χ-weighted step on a repeated one, against the oracle's
+    sediment_time_step on the same flux."""
+    import torch
+    print("case pisces sediment hook", precision, flush=True)
+    wl = _case(seed=4)
+    g = wl.grid
+    q = OP.params_from_vector(wl.bgc.param_vector())
+    Fi = {n: g.interior(a).copy() for n, a in zip(OP.TRACERS, wl.tracers)}
+    sed = wl.bgc.config().sediment
+    f = OP.sediment_flux(q, np.asarray(g.zc), Fi, sed)
+    state = O.SedimentState(g.Nx, g.Ny)
+    dev, tr = _device(wl, precision)
+    for dt in (float("inf"), 120.0, 120.0, 40.0, 40.0):
+        dev.update_sediment_state(tr, dt)
+        OP.sediment_time_step(sed, state, f, dt)
+    torch.cuda.synchronize()
+    es = rel_l2(dev.sediment_storage.cpu().numpy(), state.storage)
+    print("hook storage", es, flush=True)
+    assert es <= TOL[precision]
+    assert state.iteration == 4
 
 
 def test_rejected_configurations():
