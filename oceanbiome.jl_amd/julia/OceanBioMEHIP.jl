@@ -351,16 +351,18 @@ end
 `PISCES(; grid)` (PISCES.jl:268-330) on the device.  `aux` holds the
 auxiliary fields the library writes in `update_biogeochemical_state!`:
 `PAR`, `PAR₁`, `PAR₂`, `PAR₃`, `Ω` (CenterFields), `zₑᵤ` and
-`mixed_layer_PAR` (2-D, Nx×Ny).
+`mixed_layer_PAR` (2-D, Nx×Ny).  With a sediment in `config` (PISCES's
+InstantRemineralisation), `storage` is its Nx×Ny device array of Float64.
 """
 mutable struct HIPPISCES{B, A} <: AbstractBiogeochemistry
     underlying :: B
     config :: Base.RefValue{ObmePiscesConfig}
     handle :: Ptr{Cvoid}
     aux :: A
+    storage :: Any                   # the sediment's storage (device Float64) or nothing
 end
 
-function HIPPISCES(underlying, config::ObmePiscesConfig, aux)
+function HIPPISCES(underlying, config::ObmePiscesConfig, aux; storage = nothing)
     n = ccall((:obme_abi_sizeof, libobme), Int64, (Int32,), 9)
     n == sizeof(ObmePiscesConfig) || error("ABI mismatch for ObmePiscesConfig")
     n = ccall((:obme_abi_sizeof, libobme), Int64, (Int32,), 11)
@@ -368,7 +370,7 @@ function HIPPISCES(underlying, config::ObmePiscesConfig, aux)
     h = Ref{Ptr{Cvoid}}(C_NULL)
     cfg = Ref(config)
     check(ccall((:obme_pisces_create, libobme), Int32, (Ref{ObmePiscesConfig}, Ref{Ptr{Cvoid}}), cfg, h))
-    bgc = HIPPISCES(underlying, cfg, h[], aux)
+    bgc = HIPPISCES(underlying, cfg, h[], aux, storage)
     finalizer(b -> ccall((:obme_pisces_destroy, libobme), Int32, (Ptr{Cvoid},), b.handle), bgc)
     return bgc
 end
@@ -388,6 +390,13 @@ function update_biogeochemical_state!(b::HIPPISCES, model)
                 b.handle, ptrs, device_pointer(b.aux.PAR), bands, device_pointer(b.aux.Ω),
                 Ptr{Cvoid}(UInt(pointer(b.aux.zₑᵤ))), Ptr{Cvoid}(UInt(pointer(b.aux.mixed_layer_PAR))),
                 model.clock.time, stream_pointer()))
+    if b.config[].sediment.model != 0
+        # update_biogeochemical_state!(model, sediment) (Sediments/update_state.jl:6-17)
+        check(ccall((:obme_pisces_sediment_update_state, libobme), Int32,
+                    (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Float64, Ptr{Cvoid}),
+                    b.handle, ptrs, Ptr{Cvoid}(UInt(pointer(b.storage))), model.clock.last_stage_Δt,
+                    stream_pointer()))
+    end
     return nothing
 end
 
@@ -403,6 +412,13 @@ end
 
 function step!(b::HIPPISCES, tracers, nsteps, Δt; t₀ = 0.0)
     ptrs = Ptr{Cvoid}[device_pointer(tracers[name]) for name in required_biogeochemical_tracers(b)]
+    if b.config[].sediment.model != 0
+        check(ccall((:obme_pisces_step_with_sediment, libobme), Int32,
+                    (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Ptr{Cvoid}, Int32, Float64, Float64, Ptr{Cvoid}),
+                    b.handle, ptrs, Ptr{Cvoid}(UInt(pointer(b.storage))), Int32(nsteps), Float64(Δt),
+                    Float64(t₀), stream_pointer()))
+        return nothing
+    end
     check(ccall((:obme_pisces_step, libobme), Int32,
                 (Ptr{Cvoid}, Ptr{Ptr{Cvoid}}, Int32, Float64, Float64, Ptr{Cvoid}),
                 b.handle, ptrs, Int32(nsteps), Float64(Δt), Float64(t₀), stream_pointer()))
