@@ -297,11 +297,15 @@ __global__ void __launch_bounds__(256) k_pisces_prep(PiscesArgs<FT> a)
 // FP32-mixed: FP64 Newton warm-started from the previous stage's [H+], or
 // from an FP32 first guess, falling back to the reference solve);
 // OM = 0: it is read from the Omega array.
+// Waves per SIMD the registers are held to: FP32 tendencies (the hook) 4;
+// the FP32 step 2 (at 4 its Ω Newton and substep spilled 416 B per lane and
+// the step took 4.46 ms at 256x256x64, at 2 none and 2.45 ms); FP64 2.
 #ifndef OBME_PISCES_F32_WAVES
 #define OBME_PISCES_F32_WAVES 4
 #endif
 template <typename FT, int MODE, int OM, bool SED = false>
-__global__ void __launch_bounds__(256, sizeof(FT) == 4 ? OBME_PISCES_F32_WAVES : 2) k_pisces_cell(PiscesArgs<FT> a)
+__global__ void __launch_bounds__(256, (sizeof(FT) == 4 && MODE == 1) ? OBME_PISCES_F32_WAVES : 2)
+k_pisces_cell(PiscesArgs<FT> a)
 {
     const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const long ncol = (long)a.Nx * a.Ny;
