@@ -83,7 +83,7 @@ def _step_and_compare(bgc, grid, fields, precision, nsteps, dt, calls=2):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-@pytest.mark.parametrize("shape", [(20, 3, 50), (17, 2, 64), (9, 2, 7)])
+@pytest.mark.parametrize("shape", [(20, 3, 50), (17, 2, 64), (9, 2, 7), (5, 2, 1)])
 def test_npzd_step_with_sediment(precision, shape):
     print("case sediment npzd", precision, shape, flush=True)
     bgc, grid, fields = _npzd(shape)

@@ -56,7 +56,7 @@ def _errors(bgc, grid, out, ref, dev, state):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-@pytest.mark.parametrize("shape,co2", [((20, 3, 40), False), ((9, 2, 7), True)])
+@pytest.mark.parametrize("shape,co2", [((20, 3, 40), False), ((9, 2, 7), True), ((6, 2, 1), False)])
 def test_step_with_simple_multi_g(precision, shape, co2):
     import torch
     print("case smg step", precision, shape, "co2", co2, flush=True)
