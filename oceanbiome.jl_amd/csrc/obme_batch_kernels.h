@@ -128,10 +128,12 @@ __device__ __forceinline__ void batch_columns(const BatchArgs<FT>& a)
         const int jb = Nzc - 1;
         const bool sed = SED && a.sed_on && col >= 0 && r == jb / L;
         const int mb = jb % L;
-        double su = 0.0, sgm = 0.0;
-        if (sed && a.storage) {
-            su = a.storage[col];
-            if (a.sed_ab2) sgm = a.sedGm[col];
+        // the storage and the last tendency in LDS, one slot per segment
+        // (not held in registers through the launch)
+        __shared__ double s_sed[2][NW * CPW];
+        if (sed) {
+            s_sed[0][seg] = a.storage ? a.storage[col] : 0.0;
+            s_sed[1][seg] = (a.storage && a.sed_ab2) ? a.sedGm[col] : 0.0;
         }
 
         // water terms of the own levels
@@ -234,13 +236,13 @@ __device__ __forceinline__ void batch_columns(const BatchArgs<FT>& a)
                     if (st == 0) {
                         if (!init) {
                             const bool ab2 = a.sed_ab2 && s == 0;
-                            su += a.sed_dt[0] * (ab2 ? a.sed_cn * ge - a.sed_cm * sgm : ge);
+                            s_sed[0][seg] += a.sed_dt[0] * (ab2 ? a.sed_cn * ge - a.sed_cm * s_sed[1][seg] : ge);
                         }
                     } else {
-                        if (st == 1 && init) su += a.sed_dt[0] * ge;
-                        su += a.sed_dt[st] * ge;
+                        if (st == 1 && init) s_sed[0][seg] += a.sed_dt[0] * ge;
+                        s_sed[0][seg] += a.sed_dt[st] * ge;
                     }
-                    sgm = ge;
+                    s_sed[1][seg] = ge;
                 }
                 // tendencies bottom-up within the lane, sinking, RK3 substep
 #pragma unroll
@@ -295,8 +297,8 @@ __device__ __forceinline__ void batch_columns(const BatchArgs<FT>& a)
             }
         }
         if (sed && a.storage) {
-            a.storage[col] = su;
-            a.sedGm[col] = sgm;
+            a.storage[col] = s_sed[0][seg];
+            a.sedGm[col] = s_sed[1][seg];
         }
 #pragma unroll
         for (int n = 0; n < 4; n++)
