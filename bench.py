@@ -12,7 +12,7 @@ FP32-mixed path; the plain FP64 path is timed beside it, and both are
 compared against the FP64 oracle on a sample of columns after the full
 200-step run (columns are independent, so a column sample is exact).
 
-Prints ONE JSON line.  --gpus N with N > 1 exits: this build runs on one GPU.
+Prints ONE JSON line.  --gpus N with N > 1 exits: the benchmark runs on one GPU.
 """
 import argparse
 import json
@@ -164,7 +164,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the biogeochemistry step "
-              f"shards by column tiles with no exchange, and this build measures one GPU's share.",
+              f"shards by column tiles with no exchange, and the benchmark measures one GPU's share.",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "columns":

@@ -42,7 +42,7 @@ enum {
     OBME_OK = 0,
     OBME_ERR_INVALID = 1,     /* bad argument or configuration            */
     OBME_ERR_HIP = 2,         /* HIP runtime error                        */
-    OBME_ERR_UNSUPPORTED = 3, /* valid request this build does not handle */
+    OBME_ERR_UNSUPPORTED = 3, /* valid request the library does not handle */
     OBME_ERR_NOMEM = 4
 };
 
