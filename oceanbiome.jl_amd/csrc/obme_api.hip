@@ -1405,7 +1405,7 @@ extern "C" obme_status obme_kernel_timing(obme_handle* h, int32_t which, double*
 // ------------------------------------------------------------------ batched ensemble
 
 template <typename FT, int L, int CPW, int NW, bool SED = false>
-__global__ void __launch_bounds__(NW * 64) k_batch(BatchArgs<FT> a)
+__global__ void __launch_bounds__(NW * 64, (SED && sizeof(FT) == 4 && L == 2) ? 5 : 1) k_batch(BatchArgs<FT> a)
 {
     batch_columns<FT, L, CPW, NW, SED>(a);
 }
